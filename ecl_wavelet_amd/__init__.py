@@ -1,0 +1,10 @@
+"""MI355X (gfx950) wavelet-tree / rank-select library.
+
+The product is ``libecl_wavelet_hip.so`` (hand-written HIP kernels behind the
+C ABI in ``include/ecl_wavelet_c.h``).  This package builds it
+(``build.build_library``), binds it (``_lib``) and mirrors the reference's
+class names in Python (``api``).
+"""
+from .api import (BitArray, DeviceBuffer, Profiler, RankSelect, Timer,  # noqa: F401
+                  WaveletTree, device_name, generate_text, mem_info,
+                  synchronize, MIXED_DTYPE, QUERY_DTYPE)

@@ -1,0 +1,445 @@
+"""Python view of the library, mirroring the reference's class and method
+names (ecl::BitArray, ecl::RankSelect, ecl::WaveletTree<T>; see
+include/ecl_wavelet/ for the C++ drop-in).  Every call goes through the
+C ABI of libecl_wavelet_hip.so.
+"""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib
+from ._lib import call
+
+_DTYPES = {1: np.uint8, 2: np.uint16, 4: np.uint32, 8: np.uint64}
+
+# RankSelectQuery<T> as 16 bytes: index, then the symbol in the low bytes
+QUERY_DTYPE = np.dtype([("index", "<u8"), ("symbol", "<u8")])
+MIXED_DTYPE = np.dtype([("arg", "<u8"), ("symbol", "<u4"), ("kind", "<u4")])
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+class DeviceBuffer:
+    """Raw device allocation owned by Python (inputs and outputs of the
+    device-resident entry points)."""
+
+    def __init__(self, nbytes, device=0):
+        self.device = device
+        self.nbytes = int(nbytes)
+        ptr = C.c_void_p()
+        call("ecl_device_alloc", device, max(self.nbytes, 1), C.byref(ptr))
+        self.ptr = ptr
+
+    @classmethod
+    def from_array(cls, arr, device=0):
+        arr = np.ascontiguousarray(arr)
+        b = cls(arr.nbytes, device)
+        b.upload(arr)
+        return b
+
+    def upload(self, arr):
+        arr = np.ascontiguousarray(arr)
+        assert arr.nbytes <= self.nbytes
+        call("ecl_memcpy_h2d", self.device, self.ptr, _p(arr), arr.nbytes)
+
+    def download(self, dtype, count=None):
+        dtype = np.dtype(dtype)
+        if count is None:
+            count = self.nbytes // dtype.itemsize
+        out = np.empty(count, dtype=dtype)
+        assert out.nbytes <= self.nbytes
+        call("ecl_memcpy_d2h", self.device, _p(out), self.ptr, out.nbytes)
+        return out
+
+    def free(self):
+        if self.ptr:
+            call("ecl_device_free", self.device, self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass
+
+
+def synchronize(device=0):
+    call("ecl_device_synchronize", device)
+
+
+def generate_text(n, sigma, dtype=np.uint8, seed=1, thresholds=None, device=0):
+    """Synthetic text on the device: hash-uniform or inverse-CDF draws."""
+    esize = np.dtype(dtype).itemsize
+    buf = DeviceBuffer(n * esize, device)
+    thr = None if thresholds is None else np.ascontiguousarray(thresholds, np.uint64)
+    call("ecl_generate_text", device, esize, seed, n, sigma,
+         None if thr is None else _p(thr), buf.ptr)
+    return buf
+
+
+class BitArray:
+    """ecl::BitArray (bitarray/bit_array.cuh)."""
+
+    def __init__(self, sizes, init_value=None, device=0):
+        sizes = np.ascontiguousarray(sizes, dtype=np.uint64)
+        h = C.c_void_p()
+        init = -1 if init_value is None else int(bool(init_value))
+        call("ecl_bit_array_create", _p(sizes), len(sizes), init, device, C.byref(h))
+        self._h = h
+        self._owned = True
+        self.device = device
+
+    @classmethod
+    def _borrow(cls, handle, device):
+        self = cls.__new__(cls)
+        self._h = handle
+        self._owned = False
+        self.device = device
+        return self
+
+    def __del__(self):
+        if getattr(self, "_owned", False) and self._h:
+            try:
+                call("ecl_bit_array_destroy", self._h)
+            except Exception:  # noqa: BLE001
+                pass
+            self._h = None
+
+    def numArrays(self):
+        n = C.c_size_t()
+        call("ecl_bit_array_num_arrays", self._h, C.byref(n))
+        return n.value
+
+    def size(self, i):
+        n = C.c_size_t()
+        call("ecl_bit_array_size", self._h, i, C.byref(n))
+        return n.value
+
+    def getOffset(self, i):
+        n = C.c_size_t()
+        call("ecl_bit_array_offset", self._h, i, C.byref(n))
+        return n.value
+
+    def sizeInWords(self, i):
+        return (self.size(i) + 31) // 32
+
+    def writeWords(self, i, first_word, words):
+        words = np.ascontiguousarray(words, dtype=np.uint32)
+        call("ecl_bit_array_write_words", self._h, i, first_word, _p(words), len(words))
+
+    def words(self, i, first_word=0, count=None):
+        if count is None:
+            count = self.sizeInWords(i) - first_word
+        out = np.empty(count, dtype=np.uint32)
+        call("ecl_bit_array_read_words", self._h, i, first_word, _p(out), count)
+        return out
+
+    def bits(self, i):
+        w = self.words(i)
+        return np.unpackbits(w.view(np.uint8), bitorder="little")[: self.size(i)]
+
+    def fillRandom(self, i, seed, threshold):
+        call("ecl_bit_array_fill_random", self._h, i, seed, threshold)
+
+
+class RankSelect:
+    """ecl::RankSelect (bitarray/rank_select.cuh).  Consumes the BitArray."""
+
+    def __init__(self, bit_array, device=0):
+        h = C.c_void_p()
+        ba = C.c_void_p(bit_array._h.value if isinstance(bit_array._h, C.c_void_p)
+                        else bit_array._h)
+        call("ecl_rank_select_create", C.byref(ba), device, C.byref(h))
+        bit_array._owned = False  # moved into the RankSelect
+        bit_array._h = None
+        self._h = h
+        self._owned = True
+        self.device = device
+
+    @classmethod
+    def _borrow(cls, handle, device):
+        self = cls.__new__(cls)
+        self._h = handle
+        self._owned = False
+        self.device = device
+        return self
+
+    def __del__(self):
+        if getattr(self, "_owned", False) and self._h:
+            try:
+                call("ecl_rank_select_destroy", self._h)
+            except Exception:  # noqa: BLE001
+                pass
+            self._h = None
+
+    @property
+    def bit_array_(self):
+        h = C.c_void_p()
+        call("ecl_rank_select_bit_array", self._h, C.byref(h))
+        return BitArray._borrow(h, self.device)
+
+    def info(self, i):
+        inf = _lib.RankSelectInfo()
+        call("ecl_rank_select_info_get", self._h, i, C.byref(inf))
+        return inf
+
+    def getNumL1Blocks(self, i):
+        return self.info(i).num_l1_blocks
+
+    def getNumL2Blocks(self, i):
+        return self.info(i).num_l2_blocks
+
+    def getNumLastL2Blocks(self, i):
+        return self.info(i).num_last_l2_blocks
+
+    def getTotalNumVals(self, i, value):
+        inf = self.info(i)
+        return inf.total_ones if value else inf.total_zeros
+
+    def l1Entries(self, i):
+        n = self.getNumL1Blocks(i)
+        out = np.empty(n, dtype=np.uint64)
+        call("ecl_rank_select_get_l1", self._h, i, 0, n, _p(out))
+        return out
+
+    def l2Entries(self, i):
+        n = self.getNumL2Blocks(i)
+        out = np.empty(n, dtype=np.uint16)
+        call("ecl_rank_select_get_l2", self._h, i, 0, n, _p(out))
+        return out
+
+    def selectSamples(self, i, value):
+        inf = self.info(i)
+        n = inf.num_samples_1 if value else inf.num_samples_0
+        out = np.empty(n, dtype=np.uint64)
+        call("ecl_rank_select_get_samples", self._h, i, int(value), 0, n, _p(out))
+        return out
+
+    def rank(self, i, index, value=1):
+        index = np.ascontiguousarray(index, dtype=np.uint64)
+        out = np.empty(len(index), dtype=np.uint64)
+        call("ecl_rank_select_rank", self._h, i, int(value), _p(index), len(index), _p(out))
+        return out
+
+    def select(self, i, k, value=1):
+        k = np.ascontiguousarray(k, dtype=np.uint64)
+        out = np.empty(len(k), dtype=np.uint64)
+        call("ecl_rank_select_select", self._h, i, int(value), _p(k), len(k), _p(out))
+        return out
+
+    def rankDevice(self, i, d_index, m, d_out, value=1, stream=None):
+        call("ecl_rank_select_rank_device", self._h, i, int(value), d_index.ptr, m,
+             d_out.ptr, stream)
+
+    def selectDevice(self, i, d_k, m, d_out, value=1, stream=None):
+        call("ecl_rank_select_select_device", self._h, i, int(value), d_k.ptr, m,
+             d_out.ptr, stream)
+
+    def rankGenerated(self, i, seed, m, d_out, value=1, stream=None):
+        call("ecl_rank_select_rank_generated", self._h, i, int(value), seed, m,
+             d_out.ptr, stream)
+
+    def selectGenerated(self, i, seed, m, d_out, value=1, stream=None):
+        call("ecl_rank_select_select_generated", self._h, i, int(value), seed, m,
+             d_out.ptr, stream)
+
+
+class WaveletTree:
+    """ecl::WaveletTree<T> (tree/wavelet_tree.cuh).
+
+    ``data`` is a numpy array of uint8/16/32/64 (host text) or a
+    DeviceBuffer together with ``dtype`` and ``n`` (device-resident text).
+    ``alphabet`` must be sorted; None or empty discovers it.
+    """
+
+    def __init__(self, data, alphabet=None, GPU_index=0, dtype=None, n=None):
+        self.device = GPU_index
+        h = C.c_void_p()
+        if isinstance(data, DeviceBuffer):
+            self.dtype = np.dtype(dtype)
+            esize = self.dtype.itemsize
+            a = (np.zeros(0, self.dtype) if alphabet is None
+                 else np.ascontiguousarray(alphabet, dtype=self.dtype))
+            call("ecl_wavelet_tree_create_device", esize, data.ptr, n, _p(a), len(a),
+                 GPU_index, C.byref(h))
+        else:
+            data = np.ascontiguousarray(data)
+            self.dtype = data.dtype
+            esize = data.dtype.itemsize
+            a = (np.zeros(0, data.dtype) if alphabet is None
+                 else np.ascontiguousarray(alphabet, dtype=data.dtype))
+            call("ecl_wavelet_tree_create", esize, _p(data), len(data), _p(a), len(a),
+                 GPU_index, C.byref(h))
+        self._h = h
+
+    def __del__(self):
+        if getattr(self, "_h", None):
+            try:
+                call("ecl_wavelet_tree_destroy", self._h)
+            except Exception:  # noqa: BLE001
+                pass
+            self._h = None
+
+    # ---- accessors (wavelet_tree.cuh:2165-2282)
+    def info(self):
+        inf = _lib.TreeInfo()
+        call("ecl_wavelet_tree_info_get", self._h, C.byref(inf))
+        return inf
+
+    def getAlphabetSize(self):
+        return self.info().alphabet_size
+
+    def getNumLevels(self):
+        return self.info().num_levels
+
+    def isMinAlphabet(self):
+        return bool(self.info().is_min_alphabet)
+
+    def getCodesStart(self):
+        return self.info().codes_start
+
+    def getAlphabet(self):
+        out = np.empty(self.getAlphabetSize(), dtype=self.dtype)
+        call("ecl_wavelet_tree_alphabet", self._h, _p(out))
+        return out
+
+    def getCounts(self):
+        out = np.empty(self.getAlphabetSize(), dtype=np.uint64)
+        call("ecl_wavelet_tree_counts", self._h, _p(out))
+        return out
+
+    def getTotalAppearances(self, i):
+        c = self.getCounts()
+        n = self.info().data_size
+        return int((n if i == len(c) - 1 else c[i + 1]) - c[i])
+
+    def codes(self):
+        inf = self.info()
+        k = inf.alphabet_size - inf.codes_start if not inf.is_pow_two else 0
+        codes = np.empty(k, dtype=np.uint64)
+        lens = np.empty(k, dtype=np.uint8)
+        call("ecl_wavelet_tree_codes", self._h, _p(codes), _p(lens))
+        return codes, lens
+
+    def nodeRanks(self):
+        k = self.info().num_ranks
+        r = np.empty(k, dtype=np.uint64)
+        s = np.empty(k, dtype=np.uint64)
+        lv = np.empty(k, dtype=np.uint8)
+        call("ecl_wavelet_tree_node_ranks", self._h, _p(r), _p(s), _p(lv))
+        return r, s, lv
+
+    def numNodesAtLevel(self):
+        L = self.getNumLevels()
+        out = np.empty(max(L - 1, 0), dtype=np.uint64)
+        call("ecl_wavelet_tree_nodes_at_level", self._h, _p(out))
+        return out
+
+    @property
+    def rank_select_(self):
+        h = C.c_void_p()
+        call("ecl_wavelet_tree_rank_select", self._h, C.byref(h))
+        return RankSelect._borrow(h, self.device)
+
+    # ---- host batch queries
+    def access(self, indices, flags=0):
+        idx = np.ascontiguousarray(indices, dtype=np.uint64)
+        out = np.empty(len(idx), dtype=self.dtype)
+        call("ecl_wavelet_tree_access", self._h, _p(idx), len(idx), _p(out), flags)
+        return out
+
+    @staticmethod
+    def make_queries(indices, symbols):
+        q = np.zeros(len(indices), dtype=QUERY_DTYPE)
+        q["index"] = np.asarray(indices, dtype=np.uint64)
+        q["symbol"] = np.asarray(symbols, dtype=np.uint64)
+        return q
+
+    def rank(self, queries, flags=0):
+        q = np.ascontiguousarray(queries, dtype=QUERY_DTYPE)
+        out = np.empty(len(q), dtype=np.uint64)
+        call("ecl_wavelet_tree_rank", self._h, _p(q), len(q), _p(out), flags)
+        return out
+
+    def select(self, queries, flags=0):
+        q = np.ascontiguousarray(queries, dtype=QUERY_DTYPE)
+        out = np.empty(len(q), dtype=np.uint64)
+        call("ecl_wavelet_tree_select", self._h, _p(q), len(q), _p(out), flags)
+        return out
+
+    # ---- device-resident batch queries
+    def accessDevice(self, d_idx, m, d_out, flags=0, stream=None):
+        call("ecl_wavelet_tree_access_device", self._h, d_idx.ptr, m, d_out.ptr, flags, stream)
+
+    def rankDevice(self, d_q, m, d_out, flags=0, stream=None):
+        call("ecl_wavelet_tree_rank_device", self._h, d_q.ptr, m, d_out.ptr, flags, stream)
+
+    def selectDevice(self, d_q, m, d_out, flags=0, stream=None):
+        call("ecl_wavelet_tree_select_device", self._h, d_q.ptr, m, d_out.ptr, flags, stream)
+
+    def queryBatchDevice(self, d_q, m, d_out, stream=None):
+        call("ecl_wavelet_tree_query_batch_device", self._h, d_q.ptr, m, d_out.ptr, stream)
+
+    def generateQueries(self, kind, seed, m, d_out, stream=None):
+        call("ecl_wavelet_tree_generate_queries", self._h, kind, seed, m, d_out.ptr, stream)
+
+
+class Profiler:
+    """Per-kernel HIP-event timing inside the library."""
+
+    @staticmethod
+    def enable(on=True):
+        call("ecl_profile_enable", 1 if on else 0)
+
+    @staticmethod
+    def reset():
+        call("ecl_profile_reset")
+
+    @staticmethod
+    def get(kernel):
+        n = C.c_uint64()
+        ms = C.c_double()
+        call("ecl_profile_get", kernel.encode(), C.byref(n), C.byref(ms))
+        return n.value, ms.value
+
+
+class Timer:
+    """HIP events on a stream (default stream when None)."""
+
+    def __init__(self, stream=None):
+        self.stream = stream
+        self.a = C.c_void_p()
+        self.b = C.c_void_p()
+        call("ecl_event_create", C.byref(self.a))
+        call("ecl_event_create", C.byref(self.b))
+
+    def start(self):
+        call("ecl_event_record", self.a, self.stream)
+
+    def stop(self):
+        call("ecl_event_record", self.b, self.stream)
+        ms = C.c_float()
+        call("ecl_event_elapsed", self.a, self.b, C.byref(ms))
+        return ms.value
+
+    def __del__(self):
+        try:
+            call("ecl_event_destroy", self.a)
+            call("ecl_event_destroy", self.b)
+        except Exception:  # noqa: BLE001
+            pass
+
+
+def device_name(device=0):
+    buf = C.create_string_buffer(256)
+    call("ecl_device_name", device, buf, 256)
+    return buf.value.decode()
+
+
+def mem_info(device=0):
+    fr = C.c_size_t()
+    tot = C.c_size_t()
+    call("ecl_device_mem_info", device, C.byref(fr), C.byref(tot))
+    return fr.value, tot.value

@@ -1,0 +1,1246 @@
+// capi.hip -- the C ABI of libecl_wavelet_hip.so: object lifetimes, memory
+// admission, build orchestration and the batch query entry points.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <new>
+#include <numeric>
+#include <string>
+#include <vector>
+
+#include "internal.hpp"
+
+using ecl::detail::LevelView;
+using ecl::detail::TreeView;
+using namespace ecl::lib;
+namespace detail = ecl::detail;
+
+// ===========================================================================
+// errors, buffers, profiling
+// ===========================================================================
+namespace ecl {
+namespace lib {
+
+static thread_local std::string g_last_error;
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess)
+    throw Error(ECL_ERROR_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+DevBuf::DevBuf(size_t b) : bytes(b) {
+  if (b == 0) return;
+  hipError_t e = hipMalloc(&ptr, b);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    ptr = nullptr;
+    throw Error(ECL_ERROR_HIP, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+  }
+}
+DevBuf::~DevBuf() { reset(); }
+DevBuf& DevBuf::operator=(DevBuf&& o) noexcept {
+  if (this != &o) {
+    reset();
+    ptr = o.ptr; bytes = o.bytes;
+    o.ptr = nullptr; o.bytes = 0;
+  }
+  return *this;
+}
+void DevBuf::reset() {
+  if (ptr) (void)hipFree(ptr);
+  ptr = nullptr;
+  bytes = 0;
+}
+
+static const char* kKernelNames[K_COUNT] = {
+    "histogram", "wt_level", "rs_build", "wt_access", "wt_rank", "wt_select",
+    "wt_mixed", "rs_rank", "rs_select", "generate", "fill"};
+const char* kernel_name(int id) { return id >= 0 && id < K_COUNT ? kKernelNames[id] : "?"; }
+
+struct ProfRecord { int id; hipEvent_t e0, e1; };
+static std::mutex g_prof_mu;
+static bool g_prof_on = false;
+static std::vector<ProfRecord> g_prof_pending;
+static uint64_t g_prof_count[K_COUNT];
+static double g_prof_ms[K_COUNT];
+
+ProfScope::ProfScope(int i, hipStream_t s) : id(i), stream(s) {
+  if (!g_prof_on) return;
+  if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+    e0 = e1 = nullptr;
+    return;
+  }
+  (void)hipEventRecord(e0, s);
+}
+ProfScope::~ProfScope() {
+  if (!e0) return;
+  (void)hipEventRecord(e1, stream);
+  std::lock_guard<std::mutex> lk(g_prof_mu);
+  g_prof_pending.push_back({id, e0, e1});
+}
+
+static void prof_drain() {
+  std::vector<ProfRecord> pend;
+  {
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    pend.swap(g_prof_pending);
+  }
+  for (auto& r : pend) {
+    float ms = 0.f;
+    if (hipEventSynchronize(r.e1) == hipSuccess &&
+        hipEventElapsedTime(&ms, r.e0, r.e1) == hipSuccess) {
+      std::lock_guard<std::mutex> lk(g_prof_mu);
+      g_prof_count[r.id]++;
+      g_prof_ms[r.id] += ms;
+    }
+    (void)hipEventDestroy(r.e0);
+    (void)hipEventDestroy(r.e1);
+  }
+}
+
+}  // namespace lib
+}  // namespace ecl
+
+template <typename F>
+static ecl_status guard(F&& f) {
+  try {
+    f();
+    return ECL_OK;
+  } catch (const Error& e) {
+    g_last_error = e.what();
+    return e.status;
+  } catch (const std::bad_alloc&) {
+    g_last_error = "host allocation failed";
+    return ECL_ERROR_INTERNAL;
+  } catch (const std::exception& e) {
+    g_last_error = e.what();
+    return ECL_ERROR_INTERNAL;
+  }
+}
+
+static void require(bool ok, const char* msg) {
+  if (!ok) throw Error(ECL_ERROR_INVALID_ARGUMENT, msg);
+}
+
+static void set_device(int device) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
+    (void)hipGetLastError();
+    throw Error(ECL_ERROR_NO_DEVICE, "no HIP device available");
+  }
+  require(device >= 0 && device < n, "invalid device index");
+  ECL_HIP(hipSetDevice(device));
+}
+
+static hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
+
+template <typename T>
+static void h2d(void* dst, const T* src, size_t count) {
+  if (count) ECL_HIP(hipMemcpy(dst, src, count * sizeof(T), hipMemcpyHostToDevice));
+}
+template <typename T>
+static void d2h(T* dst, const void* src, size_t count) {
+  if (count) ECL_HIP(hipMemcpy(dst, src, count * sizeof(T), hipMemcpyDeviceToHost));
+}
+
+static size_t free_device_memory() {
+  size_t fr = 0, tot = 0;
+  ECL_HIP(hipMemGetInfo(&fr, &tot));
+  return fr;
+}
+
+// ===========================================================================
+// objects
+// ===========================================================================
+struct ecl_bit_array_t {
+  int device = 0;
+  std::vector<uint64_t> sizes;    // bits per array
+  std::vector<uint64_t> offsets;  // word offset per array (multiple of 32)
+  uint64_t total_words = 0;
+  DevBuf data, d_sizes, d_offsets;
+  uint32_t* words() const { return data.as<uint32_t>(); }
+};
+
+struct ecl_rank_select_t {
+  int device = 0;
+  ecl_bit_array_t* ba = nullptr;  // owned
+  std::vector<uint64_t> num_l1, num_l2, num_last_l2, l1_off, l2_off, s_off, cap, ones;
+  DevBuf l1, l2, samples, total_ones;
+  DevBuf d_l1_off, d_l2_off, d_num_l1, d_num_last_l2, d_s_off, d_cap, d_levels;
+  std::vector<LevelView> levels;
+  uint64_t total_l2 = 0;
+  ~ecl_rank_select_t() { delete ba; }
+};
+
+struct ecl_wavelet_tree_t {
+  int device = 0;
+  int esize = 1;
+  uint64_t n = 0, sigma = 0;
+  uint32_t L = 0;
+  bool min_alphabet = true, pow2 = true;
+  uint64_t codes_start = 0;
+  std::vector<uint64_t> alphabet;       // original symbols (as u64)
+  std::vector<uint64_t> counts;         // exclusive prefix C
+  std::vector<uint64_t> codes;          // codes of symbols codes_start..
+  std::vector<uint8_t> code_lens;
+  std::vector<uint64_t> ranks, node_start, nodes_at_level;
+  std::vector<uint8_t> node_level;
+  uint64_t nodes_until_last_level = 0;
+  ecl_rank_select_t* rs = nullptr;      // owned
+  DevBuf d_counts, d_ranks, d_nal, d_codes, d_code_lens, d_alphabet, d_lut;
+  int map_kind = 0;
+  TreeView view{};
+  ~ecl_wavelet_tree_t() { delete rs; }
+};
+
+// ===========================================================================
+// BitArray
+// ===========================================================================
+static ecl_bit_array_t* make_bit_array(const std::vector<uint64_t>& sizes, int init,
+                                       int device) {
+  require(sizes.size() < 255, "a BitArray holds at most 254 arrays");
+  auto* ba = new ecl_bit_array_t();
+  try {
+    ba->device = device;
+    ba->sizes = sizes;
+    ba->offsets.resize(sizes.size());
+    uint64_t total = 0;
+    for (size_t i = 0; i < sizes.size(); ++i) {
+      ba->offsets[i] = total;
+      const uint64_t w = (sizes[i] + 31) / 32;
+      total += (w + 31) & ~31ull;  // 128 B per array start
+    }
+    total += total % 2;  // even word count: u64 loads never leave the buffer
+    if (total == 0) total = 2;
+    ba->total_words = total;
+    ba->data = DevBuf(total * sizeof(uint32_t));
+    ba->d_sizes = DevBuf(std::max<size_t>(1, sizes.size()) * sizeof(uint64_t));
+    ba->d_offsets = DevBuf(std::max<size_t>(1, sizes.size()) * sizeof(uint64_t));
+    h2d(ba->d_sizes.ptr, ba->sizes.data(), sizes.size());
+    h2d(ba->d_offsets.ptr, ba->offsets.data(), sizes.size());
+    if (init >= 0)
+      launch_fill_u32(ba->words(), total, init ? 0xFFFFFFFFu : 0u, nullptr);
+  } catch (...) {
+    delete ba;
+    throw;
+  }
+  return ba;
+}
+
+// ===========================================================================
+// RankSelect
+// ===========================================================================
+static ecl_rank_select_t* make_rank_select(ecl_bit_array_t* ba, hipStream_t s) {
+  auto* rs = new ecl_rank_select_t();
+  try {
+    rs->device = ba->device;
+    const size_t na = ba->sizes.size();
+    rs->num_l1.resize(na); rs->num_l2.resize(na); rs->num_last_l2.resize(na);
+    rs->l1_off.resize(na); rs->l2_off.resize(na); rs->s_off.resize(na);
+    rs->cap.resize(na); rs->ones.assign(na, 0);
+    uint64_t t1 = 0, t2 = 0, ts = 0, max_l1 = 0;
+    for (size_t i = 0; i < na; ++i) {
+      const uint64_t m = ba->sizes[i];
+      // rank_select.cu:57-117
+      rs->num_l1[i] = (m + detail::kL1Bits - 1) / detail::kL1Bits;
+      uint64_t last = ((m % detail::kL1Bits) + detail::kL2Bits - 1) / detail::kL2Bits;
+      if (last == 0 && m) last = detail::kL2PerL1;
+      rs->num_last_l2[i] = m ? last : 0;
+      rs->num_l2[i] = m ? (rs->num_l1[i] - 1) * detail::kL2PerL1 + last : 0;
+      rs->cap[i] = m / detail::kSampleRate;
+      rs->l1_off[i] = t1; rs->l2_off[i] = t2; rs->s_off[i] = ts;
+      t1 += rs->num_l1[i]; t2 += rs->num_l2[i]; ts += rs->cap[i];
+      max_l1 = std::max(max_l1, rs->num_l1[i]);
+    }
+    rs->total_l2 = t2;
+    rs->l1 = DevBuf(std::max<uint64_t>(t1, 1) * sizeof(uint64_t));
+    rs->l2 = DevBuf(std::max<uint64_t>(t2, 1) * sizeof(uint16_t));
+    rs->samples = DevBuf(std::max<uint64_t>(ts, 1) * sizeof(uint64_t));
+    rs->total_ones = DevBuf(std::max<size_t>(na, 1) * sizeof(uint64_t));
+    launch_fill_u64(rs->total_ones.as<uint64_t>(), std::max<size_t>(na, 1), 0, s);
+    DevBuf status((max_l1 + 2) * sizeof(uint64_t));
+    for (size_t i = 0; i < na; ++i) {
+      if (ba->sizes[i] == 0) continue;
+      launch_fill_u64(status.as<uint64_t>(), rs->num_l1[i] + 2, 0, s);
+      RsBuildParams p{};
+      p.words64 = reinterpret_cast<const uint64_t*>(ba->words() + ba->offsets[i]);
+      p.m = ba->sizes[i];
+      p.l1 = rs->l1.as<uint64_t>() + rs->l1_off[i];
+      p.l2 = rs->l2.as<uint16_t>() + rs->l2_off[i];
+      p.num_l1 = rs->num_l1[i];
+      p.num_l2 = rs->num_l2[i];
+      p.samples = rs->samples.as<uint64_t>() + rs->s_off[i];
+      p.cap = rs->cap[i];
+      p.total_ones = rs->total_ones.as<uint64_t>() + i;
+      p.status = status.as<uint64_t>();
+      p.counter = status.as<uint64_t>() + rs->num_l1[i] + 1;
+      launch_rs_build(p, s);
+    }
+    ECL_HIP(hipStreamSynchronize(s));
+    d2h(rs->ones.data(), rs->total_ones.ptr, na);
+    // reference-shaped device directory + LevelView table
+    auto up = [&](DevBuf& b, const std::vector<uint64_t>& v) {
+      b = DevBuf(std::max<size_t>(v.size(), 1) * sizeof(uint64_t));
+      h2d(b.ptr, v.data(), v.size());
+    };
+    up(rs->d_l1_off, rs->l1_off);
+    up(rs->d_l2_off, rs->l2_off);
+    up(rs->d_num_l1, rs->num_l1);
+    up(rs->d_s_off, rs->s_off);
+    up(rs->d_cap, rs->cap);
+    std::vector<uint16_t> last16(na);
+    for (size_t i = 0; i < na; ++i) last16[i] = static_cast<uint16_t>(rs->num_last_l2[i]);
+    rs->d_num_last_l2 = DevBuf(std::max<size_t>(na, 1) * sizeof(uint16_t));
+    h2d(rs->d_num_last_l2.ptr, last16.data(), na);
+    rs->levels.resize(na);
+    for (size_t i = 0; i < na; ++i) {
+      LevelView& v = rs->levels[i];
+      v.words = ba->words() + ba->offsets[i];
+      v.l1 = rs->l1.as<uint64_t>() + rs->l1_off[i];
+      v.l2 = rs->l2.as<uint16_t>() + rs->l2_off[i];
+      v.samples = rs->samples.as<uint64_t>() + rs->s_off[i];
+      v.size = ba->sizes[i];
+      v.ones = rs->ones[i];
+      v.num_l1 = rs->num_l1[i];
+      v.sample_cap = rs->cap[i];
+    }
+    rs->d_levels = DevBuf(std::max<size_t>(na, 1) * sizeof(LevelView));
+    h2d(rs->d_levels.ptr, rs->levels.data(), na);
+    rs->ba = ba;  // ownership transferred last
+  } catch (...) {
+    rs->ba = nullptr;
+    delete rs;
+    throw;
+  }
+  return rs;
+}
+
+// ===========================================================================
+// WaveletTree: host-side shape (codes, nodes, node tables)
+// ===========================================================================
+namespace {
+
+struct Node { uint64_t lo, hi; };
+
+// Minimal codes as the leaves of the left-complete tree: a node of s >= 2
+// symbols splits after prev_pow2(s) symbols (the shape
+// createMinimalCodes, wavelet_tree.cuh:2100-2159, produces).
+void assign_codes(uint64_t lo, uint64_t hi, uint32_t depth, uint64_t prefix, uint32_t L,
+                  std::vector<uint64_t>& code, std::vector<uint8_t>& len,
+                  std::vector<std::vector<Node>>& levels) {
+  // iterative over the left spine to bound recursion depth by L
+  while (true) {
+    if (lo == hi) {
+      code[lo] = L > depth ? (prefix << (L - depth)) : prefix;
+      len[lo] = static_cast<uint8_t>(depth);
+      return;
+    }
+    levels[depth].push_back({lo, hi});
+    const uint64_t mid = lo + detail::prev_pow2(hi - lo + 1);
+    assign_codes(lo, mid - 1, depth + 1, prefix << 1, L, code, len, levels);
+    lo = mid;
+    depth += 1;
+    prefix = (prefix << 1) | 1;
+  }
+}
+
+// first-round count of coded symbols (wavelet_tree.cuh:2122-2133)
+uint64_t codes_start_of(uint64_t sigma, uint32_t L) {
+  if (detail::is_pow2(sigma)) return sigma;
+  uint64_t num = sigma;
+  for (uint32_t i = L - 1; i > 0; --i) {
+    const uint64_t p = 1ull << i;
+    if (num <= p) break;
+    num -= p;
+  }
+  return sigma - num;
+}
+
+template <typename T>
+std::vector<uint64_t> to_u64(const void* a, size_t n) {
+  std::vector<uint64_t> v(n);
+  const T* p = static_cast<const T*>(a);
+  for (size_t i = 0; i < n; ++i) v[i] = p[i];
+  return v;
+}
+std::vector<uint64_t> elems_to_u64(const void* a, size_t n, int esize) {
+  switch (esize) {
+    case 1: return to_u64<uint8_t>(a, n);
+    case 2: return to_u64<uint16_t>(a, n);
+    case 4: return to_u64<uint32_t>(a, n);
+    default: return to_u64<uint64_t>(a, n);
+  }
+}
+void u64_to_elems(const std::vector<uint64_t>& v, void* out, int esize) {
+  for (size_t i = 0; i < v.size(); ++i) {
+    switch (esize) {
+      case 1: static_cast<uint8_t*>(out)[i] = static_cast<uint8_t>(v[i]); break;
+      case 2: static_cast<uint16_t*>(out)[i] = static_cast<uint16_t>(v[i]); break;
+      case 4: static_cast<uint32_t*>(out)[i] = static_cast<uint32_t>(v[i]); break;
+      default: static_cast<uint64_t*>(out)[i] = v[i]; break;
+    }
+  }
+}
+
+// Memory a tree of n symbols needs beyond its text (upper bound, bytes):
+// bits + L2/L1/samples + level buffers + metadata + the reference's
+// 10M-query headroom (wavelet_tree.cuh:644-648).
+uint64_t tree_needed_bytes(uint64_t n, uint64_t sigma, uint32_t L, int esize,
+                           int level_buffers) {
+  const uint64_t per_level = ((n + 1023) / 1024) * 128 + 256 /* bits, padded */ +
+                             ((n + 511) / 512) * 2 + ((n + 65535) / 65536) * 8 +
+                             (n / 4096) * 8 + 64;
+  return per_level * L + static_cast<uint64_t>(level_buffers) * n * esize +
+         sigma * 64 + (n / 4096 + 4096) * 8 + 10'000'000ull * 16;
+}
+
+}  // namespace
+
+static const char* kOomTree = "Not enough memory available for the wavelet tree.";
+
+static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_device,
+                                      uint64_t n, const void* alphabet, size_t asize,
+                                      int device) {
+  require(n > 0, "data_size must be positive");
+  require(esize == 1 || esize == 2 || esize == 4 || esize == 8, "invalid element type");
+  require(text != nullptr, "text is NULL");
+  set_device(device);
+  hipStream_t s = nullptr;
+  auto* wt = new ecl_wavelet_tree_t();
+  try {
+    wt->device = device;
+    wt->esize = esize;
+    wt->n = n;
+    // ---- alphabet (wavelet_tree.cuh:590-621)
+    if (asize > 0) {
+      require(asize >= 2, "the alphabet needs at least 2 symbols");
+      wt->alphabet = elems_to_u64(alphabet, asize, esize);
+      require(std::is_sorted(wt->alphabet.begin(), wt->alphabet.end()),
+              "the alphabet must be sorted");
+      require(std::adjacent_find(wt->alphabet.begin(), wt->alphabet.end()) ==
+                  wt->alphabet.end(), "the alphabet must not repeat symbols");
+    } else if (esize <= 2) {
+      const uint64_t range = esize == 1 ? 256 : 65536;
+      std::vector<uint32_t> flags(range);
+      if (text_on_device) {
+        DevBuf d_flags(range * 4);
+        launch_fill_u32(d_flags.as<uint32_t>(), range, 0, s);
+        launch_presence(text, esize, n, d_flags.as<uint32_t>(), s);
+        ECL_HIP(hipStreamSynchronize(s));
+        d2h(flags.data(), d_flags.ptr, range);
+      } else {
+        for (uint64_t i = 0; i < n; ++i)
+          flags[esize == 1 ? static_cast<const uint8_t*>(text)[i]
+                           : static_cast<const uint16_t*>(text)[i]] = 1;
+      }
+      for (uint64_t v = 0; v < range; ++v)
+        if (flags[v]) wt->alphabet.push_back(v);
+    } else {
+      std::vector<uint64_t> all;
+      if (text_on_device) {
+        std::vector<uint8_t> raw(n * esize);
+        d2h(raw.data(), text, n * esize);
+        all = elems_to_u64(raw.data(), n, esize);
+      } else {
+        all = elems_to_u64(text, n, esize);
+      }
+      std::sort(all.begin(), all.end());
+      all.erase(std::unique(all.begin(), all.end()), all.end());
+      wt->alphabet = std::move(all);
+    }
+    require(wt->alphabet.size() >= 2, "the alphabet needs at least 2 symbols");
+    const uint64_t sigma = wt->sigma = wt->alphabet.size();
+    const uint32_t L = wt->L = detail::ceil_log2(sigma);
+    require(L <= 64, "alphabet too large");
+    wt->pow2 = detail::is_pow2(sigma);
+    wt->min_alphabet = true;
+    for (uint64_t i = 0; i < sigma; ++i)
+      if (wt->alphabet[i] != i) { wt->min_alphabet = false; break; }
+    if (!wt->min_alphabet)
+      wt->map_kind = esize <= 2 ? 1 : 2;
+
+    // ---- codes and nodes
+    std::vector<uint64_t> code(sigma);
+    std::vector<uint8_t> len(sigma);
+    std::vector<std::vector<Node>> nodes(L + 1);
+    assign_codes(0, sigma - 1, 0, 0, L, code, len, nodes);
+    wt->codes_start = codes_start_of(sigma, L);
+    for (uint64_t c = wt->codes_start; c < sigma; ++c) {
+      wt->codes.push_back(code[c]);
+      wt->code_lens.push_back(len[c]);
+    }
+    const bool remap = !wt->min_alphabet || !wt->pow2;
+
+    // ---- memory admission (wavelet_tree.cuh:644-689)
+    const int buffers = 2;  // host text: own copy + 1; device text: 2 scratch
+    const uint64_t needed = tree_needed_bytes(n, sigma, L, esize, buffers);
+    if (needed > free_device_memory()) throw Error(ECL_ERROR_OUT_OF_MEMORY_TREE, kOomTree);
+
+    // ---- text on the device and the two level buffers
+    DevBuf own_text, buf_y;
+    const void* d_text = text;
+    if (!text_on_device) {
+      own_text = DevBuf(n * esize);
+      h2d(own_text.ptr, static_cast<const uint8_t*>(text), n * esize);
+      d_text = own_text.ptr;
+    }
+    DevBuf buf_x;
+    void* X;  // buffer that receives the codes / odd levels
+    if (text_on_device) {
+      buf_x = DevBuf(n * esize);
+      X = buf_x.ptr;
+    } else {
+      X = own_text.ptr;  // our own copy: reused in place
+    }
+    buf_y = DevBuf(n * esize);
+    void* Y = buf_y.ptr;
+
+    // ---- symbol maps
+    if (wt->map_kind == 1) {
+      const uint64_t range = esize == 1 ? 256 : 65536;
+      std::vector<uint32_t> lut(range);
+      uint64_t j = 0;
+      for (uint64_t v = 0; v < range; ++v) {  // lower_bound
+        while (j < sigma && wt->alphabet[j] < v) ++j;
+        lut[v] = static_cast<uint32_t>(std::min<uint64_t>(j, sigma));
+      }
+      wt->d_lut = DevBuf(range * 4);
+      h2d(wt->d_lut.ptr, lut.data(), range);
+    }
+    wt->d_alphabet = DevBuf(sigma * esize);
+    {
+      std::vector<uint8_t> raw(sigma * esize);
+      u64_to_elems(wt->alphabet, raw.data(), esize);
+      h2d(wt->d_alphabet.ptr, raw.data(), sigma * esize);
+    }
+    DevBuf d_codes_full;
+    if (!wt->pow2) {
+      wt->d_codes = DevBuf(std::max<size_t>(wt->codes.size(), 1) * 8);
+      h2d(wt->d_codes.ptr, wt->codes.data(), wt->codes.size());
+      wt->d_code_lens = DevBuf(std::max<size_t>(wt->codes.size(), 1));
+      h2d(wt->d_code_lens.ptr, wt->code_lens.data(), wt->code_lens.size());
+    }
+
+    // ---- histogram (+ remap to codes into X)
+    DevBuf d_hist(sigma * 8);
+    launch_fill_u64(d_hist.as<uint64_t>(), sigma, 0, s);
+    HistParams hp{};
+    hp.text = d_text; hp.n = n; hp.esize = esize; hp.sigma = sigma;
+    hp.map_kind = wt->map_kind;
+    hp.lut = wt->d_lut.as<uint32_t>();
+    hp.alphabet = wt->d_alphabet.ptr;
+    hp.codes_start = wt->codes_start;
+    hp.pow2 = wt->pow2 ? 1 : 0;
+    hp.codes = wt->d_codes.as<uint64_t>();
+    hp.codes_out = remap ? X : nullptr;
+    hp.hist = d_hist.as<uint64_t>();
+    launch_histogram(hp, s);
+    std::vector<uint64_t> hist(sigma);
+    ECL_HIP(hipStreamSynchronize(s));
+    d2h(hist.data(), d_hist.ptr, sigma);
+    wt->counts.resize(sigma);
+    {
+      uint64_t acc = 0;
+      for (uint64_t c = 0; c < sigma; ++c) { wt->counts[c] = acc; acc += hist[c]; }
+      if (acc != n) throw Error(ECL_ERROR_INTERNAL, "histogram does not sum to n");
+    }
+    auto C = [&](uint64_t c) { return c >= sigma ? n : wt->counts[c]; };
+
+    // ---- level sizes, node tables, node ranks
+    std::vector<uint64_t> level_size(L, 0);
+    std::vector<std::vector<uint64_t>> tables(L);
+    wt->nodes_at_level.assign(L > 1 ? L - 1 : 0, 0);
+    for (uint32_t l = 0; l < L; ++l) {
+      uint64_t zb = 0;  // zeros of level l before the node
+      const bool need_table = l + 1 < L;
+      if (need_table) tables[l].assign(2ull << l, detail::kSkip);
+      for (const Node& nd : nodes[l]) {
+        const uint64_t mid = nd.lo + detail::prev_pow2(nd.hi - nd.lo + 1);
+        level_size[l] += C(nd.hi + 1) - C(nd.lo);
+        if (l >= 1 && nd.lo >= 1) {
+          wt->ranks.push_back(zb);
+          wt->node_start.push_back(nd.lo);
+          wt->node_level.push_back(static_cast<uint8_t>(l));
+          if (l + 1 < L) wt->nodes_at_level[l]++;
+        }
+        if (need_table) {
+          const uint64_t pfx = detail::shr64(code[nd.lo], L - l);
+          const bool left_node = mid - nd.lo >= 2, right_node = nd.hi - mid + 1 >= 2;
+          tables[l][2 * pfx] = left_node ? C(nd.lo) - zb : detail::kSkip;
+          tables[l][2 * pfx + 1] = right_node ? C(mid) - C(nd.lo) + zb : detail::kSkip;
+        }
+        zb += C(mid) - C(nd.lo);
+      }
+    }
+    for (uint64_t x : wt->nodes_at_level) wt->nodes_until_last_level += x;
+
+    // ---- bit arrays and the level passes
+    ecl_bit_array_t* ba = make_bit_array(level_size, -1, device);
+    std::unique_ptr<ecl_bit_array_t> ba_guard(ba);
+    uint64_t max_tiles = 0;
+    for (uint32_t l = 0; l < L; ++l) max_tiles = std::max(max_tiles, level_tiles(level_size[l]));
+    DevBuf status((max_tiles + 2) * 8);
+    // all node tables in one upload; level l uses 2^(l+1) entries
+    std::vector<uint64_t> table_off(L, 0);
+    std::vector<uint64_t> all_tables;
+    for (uint32_t l = 0; l < L; ++l) {
+      table_off[l] = all_tables.size();
+      all_tables.insert(all_tables.end(), tables[l].begin(), tables[l].end());
+    }
+    DevBuf d_table(std::max<size_t>(all_tables.size(), 1) * 8);
+    h2d(d_table.ptr, all_tables.data(), all_tables.size());
+    const void* src = remap ? X : d_text;
+    void* dst = Y;
+    for (uint32_t l = 0; l < L; ++l) {
+      const uint64_t nl = level_size[l];
+      uint32_t* words = ba->words() + ba->offsets[l];
+      const uint64_t nw64 = (nl + 63) / 64;
+      // zero the padding words this pass does not write
+      const uint64_t padded = (((nl + 31) / 32 + 31) & ~31ull);
+      if (padded > 2 * nw64) launch_fill_u32(words + 2 * nw64, padded - 2 * nw64, 0, s);
+      if (nl == 0) continue;
+      const bool last = l + 1 == L;
+      if (!last) launch_fill_u64(status.as<uint64_t>(), level_tiles(nl) + 2, 0, s);
+      LevelParams lp{};
+      lp.src = src;
+      lp.dst = last ? nullptr : dst;
+      lp.esize = esize;
+      lp.n = nl;
+      lp.n_next = last ? 0 : level_size[l + 1];
+      lp.bit = L - 1 - l;
+      lp.pshift = L - l;
+      lp.table = d_table.as<uint64_t>() + table_off[l];
+      lp.num_prefixes = last ? 0 : (1ull << l);
+      lp.words64 = reinterpret_cast<uint64_t*>(words);
+      lp.nwords64 = nw64;
+      lp.status = status.as<uint64_t>();
+      lp.counter = status.as<uint64_t>() + level_tiles(nl) + 1;
+      launch_level(lp, s);
+      if (!last) {
+        src = dst;
+        dst = (dst == Y) ? X : Y;
+      }
+    }
+    if (ba->total_words > ba->offsets.back() + (((level_size.back() + 31) / 32 + 31) & ~31ull))
+      launch_fill_u32(ba->words() + ba->total_words - 1, 1, 0, s);  // even-count pad word
+    ECL_HIP(hipStreamSynchronize(s));
+    own_text.reset();
+    buf_x.reset();
+    buf_y.reset();
+    status.reset();
+
+    // ---- rank/select directory over every level (RankSelect ctor)
+    ba_guard.release();
+    wt->rs = make_rank_select(ba, s);
+
+    // ---- device metadata and view
+    wt->d_counts = DevBuf(sigma * 8);
+    h2d(wt->d_counts.ptr, wt->counts.data(), sigma);
+    wt->d_ranks = DevBuf(std::max<size_t>(wt->ranks.size(), 1) * 8);
+    h2d(wt->d_ranks.ptr, wt->ranks.data(), wt->ranks.size());
+    wt->d_nal = DevBuf(std::max<size_t>(wt->nodes_at_level.size(), 1) * 8);
+    h2d(wt->d_nal.ptr, wt->nodes_at_level.data(), wt->nodes_at_level.size());
+    TreeView& v = wt->view;
+    v.levels = wt->rs->d_levels.as<LevelView>();
+    v.counts = wt->d_counts.as<uint64_t>();
+    v.ranks = wt->d_ranks.as<uint64_t>();
+    v.nodes_at_level = wt->d_nal.as<uint64_t>();
+    v.codes = wt->d_codes.as<uint64_t>();
+    v.code_lens = wt->d_code_lens.as<uint8_t>();
+    v.n = n;
+    v.sigma = sigma;
+    v.codes_start = wt->codes_start;
+    v.nodes_until_last_level = wt->nodes_until_last_level;
+    v.num_ranks = wt->ranks.size();
+    v.num_levels = L;
+    v.pow2 = wt->pow2 ? 1 : 0;
+  } catch (...) {
+    delete wt;
+    throw;
+  }
+  return wt;
+}
+
+// ===========================================================================
+// query helpers
+// ===========================================================================
+static TreeQueryParams query_params(const ecl_wavelet_tree_t* wt, uint32_t flags) {
+  TreeQueryParams p{};
+  p.view = wt->view;
+  p.esize = wt->esize;
+  p.symbols_minimal = (flags & ECL_SYMBOLS_MINIMAL) ? 1 : 0;
+  p.map_kind = wt->map_kind;
+  p.lut = wt->d_lut.as<uint32_t>();
+  p.alphabet = wt->d_alphabet.ptr;
+  p.alphabet_out = (!wt->min_alphabet && !(flags & ECL_RESULTS_MINIMAL)) ? wt->d_alphabet.ptr
+                                                                         : nullptr;
+  return p;
+}
+
+static const uint64_t kHostChunk = 1ull << 24;
+
+static void check_query_memory(uint64_t bytes, const char* kind) {
+  if (bytes > free_device_memory())
+    throw Error(ECL_ERROR_OUT_OF_MEMORY_QUERIES,
+                std::string("Not enough GPU memory available for the ") + kind + " queries.");
+}
+
+// ===========================================================================
+// C ABI
+// ===========================================================================
+extern "C" {
+
+const char* ecl_last_error(void) { return g_last_error.c_str(); }
+int ecl_abi_version(void) { return ECL_WAVELET_ABI_VERSION; }
+
+ecl_status ecl_device_count(int* count) {
+  return guard([&] {
+    require(count != nullptr, "count is NULL");
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+      (void)hipGetLastError();
+      n = 0;
+    }
+    *count = n;
+  });
+}
+ecl_status ecl_device_alloc(int device, size_t bytes, void** ptr) {
+  return guard([&] {
+    require(ptr != nullptr, "ptr is NULL");
+    set_device(device);
+    *ptr = nullptr;
+    if (bytes) ECL_HIP(hipMalloc(ptr, bytes));
+  });
+}
+ecl_status ecl_device_free(int device, void* ptr) {
+  return guard([&] {
+    set_device(device);
+    if (ptr) ECL_HIP(hipFree(ptr));
+  });
+}
+ecl_status ecl_memcpy_h2d(int device, void* dst, const void* src, size_t bytes) {
+  return guard([&] {
+    set_device(device);
+    if (bytes) ECL_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  });
+}
+ecl_status ecl_memcpy_d2h(int device, void* dst, const void* src, size_t bytes) {
+  return guard([&] {
+    set_device(device);
+    if (bytes) ECL_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  });
+}
+ecl_status ecl_device_synchronize(int device) {
+  return guard([&] {
+    set_device(device);
+    ECL_HIP(hipDeviceSynchronize());
+  });
+}
+ecl_status ecl_device_mem_info(int device, size_t* fr, size_t* tot) {
+  return guard([&] {
+    set_device(device);
+    ECL_HIP(hipMemGetInfo(fr, tot));
+  });
+}
+ecl_status ecl_device_name(int device, char* buf, size_t len) {
+  return guard([&] {
+    set_device(device);
+    hipDeviceProp_t prop;
+    ECL_HIP(hipGetDeviceProperties(&prop, device));
+    std::string nm = std::string(prop.name) + " (" + prop.gcnArchName + ")";
+    if (len) {
+      std::strncpy(buf, nm.c_str(), len - 1);
+      buf[len - 1] = 0;
+    }
+  });
+}
+ecl_status ecl_event_create(void** ev) {
+  return guard([&] {
+    hipEvent_t e;
+    ECL_HIP(hipEventCreate(&e));
+    *ev = e;
+  });
+}
+ecl_status ecl_event_destroy(void* ev) {
+  return guard([&] { ECL_HIP(hipEventDestroy(static_cast<hipEvent_t>(ev))); });
+}
+ecl_status ecl_event_record(void* ev, void* stream) {
+  return guard([&] { ECL_HIP(hipEventRecord(static_cast<hipEvent_t>(ev), as_stream(stream))); });
+}
+ecl_status ecl_event_elapsed(void* a, void* b, float* ms) {
+  return guard([&] {
+    ECL_HIP(hipEventSynchronize(static_cast<hipEvent_t>(b)));
+    ECL_HIP(hipEventElapsedTime(ms, static_cast<hipEvent_t>(a), static_cast<hipEvent_t>(b)));
+  });
+}
+ecl_status ecl_profile_enable(int on) {
+  return guard([&] {
+    if (!on) prof_drain();
+    g_prof_on = on != 0;
+  });
+}
+ecl_status ecl_profile_reset(void) {
+  return guard([&] {
+    prof_drain();
+    std::lock_guard<std::mutex> lk(g_prof_mu);
+    for (int i = 0; i < K_COUNT; ++i) { g_prof_count[i] = 0; g_prof_ms[i] = 0; }
+  });
+}
+ecl_status ecl_profile_get(const char* kernel, uint64_t* launches, double* total_ms) {
+  return guard([&] {
+    prof_drain();
+    require(kernel != nullptr, "kernel is NULL");
+    for (int i = 0; i < K_COUNT; ++i) {
+      if (std::strcmp(kernel, kKernelNames[i]) == 0) {
+        std::lock_guard<std::mutex> lk(g_prof_mu);
+        *launches = g_prof_count[i];
+        *total_ms = g_prof_ms[i];
+        return;
+      }
+    }
+    throw Error(ECL_ERROR_INVALID_ARGUMENT, std::string("unknown kernel ") + kernel);
+  });
+}
+ecl_status ecl_generate_text(int device, ecl_type type, uint64_t seed, size_t n, uint64_t sigma,
+                             const uint64_t* thresholds, void* d_out) {
+  return guard([&] {
+    require(sigma >= 1, "sigma must be positive");
+    set_device(device);
+    DevBuf thr;
+    if (thresholds) {
+      thr = DevBuf(sigma * 8);
+      h2d(thr.ptr, thresholds, sigma);
+    }
+    launch_gen_text(d_out, static_cast<int>(type), n, seed, sigma, thr.as<uint64_t>(), nullptr);
+    ECL_HIP(hipDeviceSynchronize());
+  });
+}
+
+// ---- BitArray ---------------------------------------------------------------
+ecl_status ecl_bit_array_create(const size_t* sizes, size_t num_arrays, int init_value,
+                                int device, ecl_bit_array* out) {
+  return guard([&] {
+    require(out != nullptr, "out is NULL");
+    require(num_arrays == 0 || sizes != nullptr, "sizes is NULL");
+    set_device(device);
+    std::vector<uint64_t> sz(sizes, sizes + num_arrays);
+    *out = make_bit_array(sz, init_value, device);
+    ECL_HIP(hipDeviceSynchronize());
+  });
+}
+ecl_status ecl_bit_array_destroy(ecl_bit_array ba) {
+  return guard([&] {
+    if (ba) {
+      set_device(ba->device);
+      delete ba;
+    }
+  });
+}
+ecl_status ecl_bit_array_num_arrays(ecl_bit_array ba, size_t* out) {
+  return guard([&] {
+    require(ba && out, "NULL argument");
+    *out = ba->sizes.size();
+  });
+}
+ecl_status ecl_bit_array_size(ecl_bit_array ba, size_t i, size_t* bits) {
+  return guard([&] {
+    require(ba && bits && i < ba->sizes.size(), "invalid argument");
+    *bits = ba->sizes[i];
+  });
+}
+ecl_status ecl_bit_array_offset(ecl_bit_array ba, size_t i, size_t* off) {
+  return guard([&] {
+    require(ba && off && i < ba->sizes.size(), "invalid argument");
+    *off = ba->offsets[i];
+  });
+}
+ecl_status ecl_bit_array_device_pointers(ecl_bit_array ba, uint32_t** data, uint64_t** offsets,
+                                         uint64_t** sizes) {
+  return guard([&] {
+    require(ba != nullptr, "ba is NULL");
+    if (data) *data = ba->words();
+    if (offsets) *offsets = ba->d_offsets.as<uint64_t>();
+    if (sizes) *sizes = ba->d_sizes.as<uint64_t>();
+  });
+}
+ecl_status ecl_bit_array_write_words(ecl_bit_array ba, size_t i, size_t first,
+                                     const uint32_t* words, size_t num) {
+  return guard([&] {
+    require(ba && i < ba->sizes.size(), "invalid argument");
+    const uint64_t nw = (ba->sizes[i] + 31) / 32;
+    require(first + num <= nw, "word range outside the array");
+    set_device(ba->device);
+    h2d(ba->words() + ba->offsets[i] + first, words, num);
+  });
+}
+ecl_status ecl_bit_array_read_words(ecl_bit_array ba, size_t i, size_t first, uint32_t* words,
+                                    size_t num) {
+  return guard([&] {
+    require(ba && i < ba->sizes.size(), "invalid argument");
+    const uint64_t nw = (ba->sizes[i] + 31) / 32;
+    require(first + num <= nw, "word range outside the array");
+    set_device(ba->device);
+    d2h(words, ba->words() + ba->offsets[i] + first, num);
+  });
+}
+ecl_status ecl_bit_array_fill_random(ecl_bit_array ba, size_t i, uint64_t seed,
+                                     uint64_t threshold) {
+  return guard([&] {
+    require(ba && i < ba->sizes.size(), "invalid argument");
+    set_device(ba->device);
+    const uint64_t padded = (((ba->sizes[i] + 31) / 32 + 31) & ~31ull);
+    launch_gen_bits(ba->words() + ba->offsets[i], ba->sizes[i], padded, seed, threshold,
+                    nullptr);
+    ECL_HIP(hipDeviceSynchronize());
+  });
+}
+size_t ecl_bit_array_needed_gpu_memory(size_t size, uint8_t num_arrays) {
+  // bit_array.cu:140-151: sizes, offsets and data padded per array
+  return static_cast<size_t>(num_arrays) * (2 * sizeof(uint64_t) + (size + 7) / 8 + 128);
+}
+
+// ---- RankSelect ---------------------------------------------------------------
+ecl_status ecl_rank_select_create(ecl_bit_array* ba, int device, ecl_rank_select* out) {
+  return guard([&] {
+    require(ba && *ba && out, "NULL argument");
+    require((*ba)->device == device, "the bit array lives on another device");
+    set_device(device);
+    *out = make_rank_select(*ba, nullptr);
+    *ba = nullptr;
+  });
+}
+ecl_status ecl_rank_select_destroy(ecl_rank_select rs) {
+  return guard([&] {
+    if (rs) {
+      set_device(rs->device);
+      delete rs;
+    }
+  });
+}
+ecl_status ecl_rank_select_bit_array(ecl_rank_select rs, ecl_bit_array* out) {
+  return guard([&] {
+    require(rs && out, "NULL argument");
+    *out = rs->ba;
+  });
+}
+ecl_status ecl_rank_select_info_get(ecl_rank_select rs, size_t i, ecl_rank_select_info* o) {
+  return guard([&] {
+    require(rs && o && i < rs->ba->sizes.size(), "invalid argument");
+    o->size_bits = rs->ba->sizes[i];
+    o->offset_words = rs->ba->offsets[i];
+    o->num_l1_blocks = rs->num_l1[i];
+    o->num_l2_blocks = rs->num_l2[i];
+    o->num_last_l2_blocks = rs->num_last_l2[i];
+    o->total_ones = rs->ones[i];
+    o->total_zeros = rs->ba->sizes[i] - rs->ones[i];
+    o->num_samples_1 = rs->ones[i] / detail::kSampleRate;
+    o->num_samples_0 = o->total_zeros / detail::kSampleRate;
+  });
+}
+ecl_status ecl_rank_select_get_l1(ecl_rank_select rs, size_t i, size_t first, size_t count,
+                                  uint64_t* out) {
+  return guard([&] {
+    require(rs && i < rs->ba->sizes.size(), "invalid argument");
+    require(first + count <= rs->num_l1[i], "L1 range out of bounds");
+    set_device(rs->device);
+    d2h(out, rs->l1.as<uint64_t>() + rs->l1_off[i] + first, count);
+  });
+}
+ecl_status ecl_rank_select_get_l2(ecl_rank_select rs, size_t i, size_t first, size_t count,
+                                  uint16_t* out) {
+  return guard([&] {
+    require(rs && i < rs->ba->sizes.size(), "invalid argument");
+    require(first + count <= rs->num_l2[i], "L2 range out of bounds");
+    set_device(rs->device);
+    d2h(out, rs->l2.as<uint16_t>() + rs->l2_off[i] + first, count);
+  });
+}
+ecl_status ecl_rank_select_get_samples(ecl_rank_select rs, size_t i, int value, size_t first,
+                                       size_t count, uint64_t* out) {
+  return guard([&] {
+    require(rs && i < rs->ba->sizes.size(), "invalid argument");
+    const uint64_t ones = rs->ones[i], zeros = rs->ba->sizes[i] - ones;
+    const uint64_t ns = (value ? ones : zeros) / detail::kSampleRate;
+    require(first + count <= ns, "sample range out of bounds");
+    set_device(rs->device);
+    const uint64_t* base = rs->samples.as<uint64_t>() + rs->s_off[i];
+    if (value) {
+      d2h(out, base + first, count);
+    } else {
+      // zeros samples are stored from the back: S0[j] at cap - 1 - j
+      std::vector<uint64_t> tmp(count);
+      if (count) d2h(tmp.data(), base + rs->cap[i] - first - count, count);
+      for (size_t j = 0; j < count; ++j) out[j] = tmp[count - 1 - j];
+    }
+  });
+}
+ecl_status ecl_rank_select_device_pointers(ecl_rank_select rs, ecl_rank_select_device* o) {
+  return guard([&] {
+    require(rs && o, "NULL argument");
+    o->l1 = rs->l1.as<uint64_t>();
+    o->l2 = rs->l2.as<uint16_t>();
+    o->l1_offsets = rs->d_l1_off.as<uint64_t>();
+    o->l2_offsets = rs->d_l2_off.as<uint64_t>();
+    o->num_l1_blocks = rs->d_num_l1.as<uint64_t>();
+    o->num_last_l2_blocks = rs->d_num_last_l2.as<uint16_t>();
+    o->samples = rs->samples.as<uint64_t>();
+    o->sample_offsets = rs->d_s_off.as<uint64_t>();
+    o->sample_caps = rs->d_cap.as<uint64_t>();
+    o->total_ones = rs->total_ones.as<uint64_t>();
+    o->levels = rs->d_levels.ptr;
+    o->total_num_l2_blocks = rs->total_l2;
+  });
+}
+ecl_status ecl_rank_select_rank_device(ecl_rank_select rs, size_t i, int value,
+                                       const uint64_t* d_index, size_t m, uint64_t* d_out,
+                                       void* stream) {
+  return guard([&] {
+    require(rs && i < rs->ba->sizes.size(), "invalid argument");
+    set_device(rs->device);
+    launch_rs_rank(rs->d_levels.as<LevelView>() + i, value, d_index, m, d_out, 0, 0,
+                   as_stream(stream));
+  });
+}
+ecl_status ecl_rank_select_select_device(ecl_rank_select rs, size_t i, int value,
+                                         const uint64_t* d_k, size_t m, uint64_t* d_out,
+                                         void* stream) {
+  return guard([&] {
+    require(rs && i < rs->ba->sizes.size(), "invalid argument");
+    set_device(rs->device);
+    launch_rs_select(rs->d_levels.as<LevelView>() + i, value, d_k, m, d_out, 0, 0,
+                     as_stream(stream));
+  });
+}
+static void rs_host(ecl_rank_select rs, size_t i, int value, const uint64_t* in, size_t m,
+                    uint64_t* out, bool is_rank) {
+  require(rs && i < rs->ba->sizes.size(), "invalid argument");
+  set_device(rs->device);
+  for (size_t off = 0; off < m; off += kHostChunk) {
+    const size_t c = std::min<size_t>(kHostChunk, m - off);
+    check_query_memory(c * 16, is_rank ? "rank" : "select");
+    DevBuf din(c * 8), dout(c * 8);
+    h2d(din.ptr, in + off, c);
+    if (is_rank)
+      launch_rs_rank(rs->d_levels.as<LevelView>() + i, value, din.as<uint64_t>(), c,
+                     dout.as<uint64_t>(), 0, 0, nullptr);
+    else
+      launch_rs_select(rs->d_levels.as<LevelView>() + i, value, din.as<uint64_t>(), c,
+                       dout.as<uint64_t>(), 0, 0, nullptr);
+    d2h(out + off, dout.ptr, c);
+  }
+}
+ecl_status ecl_rank_select_rank(ecl_rank_select rs, size_t i, int value, const uint64_t* index,
+                                size_t m, uint64_t* out) {
+  return guard([&] { rs_host(rs, i, value, index, m, out, true); });
+}
+ecl_status ecl_rank_select_select(ecl_rank_select rs, size_t i, int value, const uint64_t* k,
+                                  size_t m, uint64_t* out) {
+  return guard([&] { rs_host(rs, i, value, k, m, out, false); });
+}
+ecl_status ecl_rank_select_rank_generated(ecl_rank_select rs, size_t i, int value,
+                                          uint64_t seed, size_t m, uint64_t* d_out,
+                                          void* stream) {
+  return guard([&] {
+    require(rs && i < rs->ba->sizes.size(), "invalid argument");
+    set_device(rs->device);
+    launch_rs_rank(rs->d_levels.as<LevelView>() + i, value, nullptr, m, d_out, 1, seed,
+                   as_stream(stream));
+  });
+}
+ecl_status ecl_rank_select_select_generated(ecl_rank_select rs, size_t i, int value,
+                                            uint64_t seed, size_t m, uint64_t* d_out,
+                                            void* stream) {
+  return guard([&] {
+    require(rs && i < rs->ba->sizes.size(), "invalid argument");
+    set_device(rs->device);
+    launch_rs_select(rs->d_levels.as<LevelView>() + i, value, nullptr, m, d_out, 1, seed,
+                     as_stream(stream));
+  });
+}
+size_t ecl_rank_select_needed_gpu_memory(size_t size, uint8_t num_arrays) {
+  // rank_select.cu:382-409, with the shared sample buffer
+  const size_t l1 = (size + 65535) / 65536, l2 = (size + 511) / 512;
+  return static_cast<size_t>(num_arrays) *
+         (l1 * 8 + l2 * 2 + (size / 4096) * 8 + 7 * sizeof(uint64_t) + sizeof(LevelView));
+}
+
+// ---- WaveletTree --------------------------------------------------------------
+ecl_status ecl_wavelet_tree_create(ecl_type type, const void* text, size_t n,
+                                   const void* alphabet, size_t asize, int device,
+                                   ecl_wavelet_tree* out) {
+  return guard([&] {
+    require(out != nullptr, "out is NULL");
+    *out = build_tree(static_cast<int>(type), text, false, n, alphabet, asize, device);
+  });
+}
+ecl_status ecl_wavelet_tree_create_device(ecl_type type, const void* d_text, size_t n,
+                                          const void* alphabet, size_t asize, int device,
+                                          ecl_wavelet_tree* out) {
+  return guard([&] {
+    require(out != nullptr, "out is NULL");
+    *out = build_tree(static_cast<int>(type), d_text, true, n, alphabet, asize, device);
+  });
+}
+ecl_status ecl_wavelet_tree_destroy(ecl_wavelet_tree wt) {
+  return guard([&] {
+    if (wt) {
+      set_device(wt->device);
+      delete wt;
+    }
+  });
+}
+ecl_status ecl_wavelet_tree_info_get(ecl_wavelet_tree wt, ecl_wavelet_tree_info* o) {
+  return guard([&] {
+    require(wt && o, "NULL argument");
+    o->data_size = wt->n;
+    o->alphabet_size = wt->sigma;
+    o->num_levels = wt->L;
+    o->is_min_alphabet = wt->min_alphabet ? 1 : 0;
+    o->codes_start = wt->codes_start;
+    o->num_ranks = wt->ranks.size();
+    o->nodes_until_last_level = wt->nodes_until_last_level;
+    o->element_size = static_cast<uint32_t>(wt->esize);
+    o->is_pow_two = wt->pow2 ? 1 : 0;
+  });
+}
+ecl_status ecl_wavelet_tree_alphabet(ecl_wavelet_tree wt, void* out) {
+  return guard([&] {
+    require(wt && out, "NULL argument");
+    u64_to_elems(wt->alphabet, out, wt->esize);
+  });
+}
+ecl_status ecl_wavelet_tree_counts(ecl_wavelet_tree wt, uint64_t* out) {
+  return guard([&] {
+    require(wt && out, "NULL argument");
+    std::copy(wt->counts.begin(), wt->counts.end(), out);
+  });
+}
+ecl_status ecl_wavelet_tree_codes(ecl_wavelet_tree wt, uint64_t* codes, uint8_t* lens) {
+  return guard([&] {
+    require(wt != nullptr, "NULL argument");
+    if (codes) std::copy(wt->codes.begin(), wt->codes.end(), codes);
+    if (lens) std::copy(wt->code_lens.begin(), wt->code_lens.end(), lens);
+  });
+}
+ecl_status ecl_wavelet_tree_node_ranks(ecl_wavelet_tree wt, uint64_t* ranks,
+                                       uint64_t* node_start, uint8_t* node_level) {
+  return guard([&] {
+    require(wt != nullptr, "NULL argument");
+    if (ranks) std::copy(wt->ranks.begin(), wt->ranks.end(), ranks);
+    if (node_start) std::copy(wt->node_start.begin(), wt->node_start.end(), node_start);
+    if (node_level) std::copy(wt->node_level.begin(), wt->node_level.end(), node_level);
+  });
+}
+ecl_status ecl_wavelet_tree_nodes_at_level(ecl_wavelet_tree wt, uint64_t* out) {
+  return guard([&] {
+    require(wt && out, "NULL argument");
+    for (uint32_t l = 0; l + 1 < wt->L; ++l)
+      out[l] = wt->pow2 ? (1ull << l) - 1 : wt->nodes_at_level[l];
+  });
+}
+ecl_status ecl_wavelet_tree_rank_select(ecl_wavelet_tree wt, ecl_rank_select* out) {
+  return guard([&] {
+    require(wt && out, "NULL argument");
+    *out = wt->rs;
+  });
+}
+ecl_status ecl_wavelet_tree_device_view(ecl_wavelet_tree wt, void* view, size_t bytes) {
+  return guard([&] {
+    require(wt && view && bytes == sizeof(TreeView), "view size mismatch");
+    std::memcpy(view, &wt->view, sizeof(TreeView));
+  });
+}
+size_t ecl_wavelet_tree_needed_gpu_memory(size_t n, size_t sigma, uint8_t L, size_t) {
+  return tree_needed_bytes(n, sigma, L, 8, 0);
+}
+
+ecl_status ecl_wavelet_tree_access(ecl_wavelet_tree wt, const uint64_t* idx, size_t m,
+                                   void* out, uint32_t flags) {
+  return guard([&] {
+    require(wt && (m == 0 || (idx && out)), "NULL argument");
+    set_device(wt->device);
+    const TreeQueryParams p = query_params(wt, flags);
+    for (size_t off = 0; off < m; off += kHostChunk) {
+      const size_t c = std::min<size_t>(kHostChunk, m - off);
+      check_query_memory(c * (8 + wt->esize), "access");
+      DevBuf din(c * 8), dout(c * wt->esize);
+      h2d(din.ptr, idx + off, c);
+      launch_tree_access(p, din.as<uint64_t>(), c, dout.ptr, nullptr);
+      d2h(static_cast<uint8_t*>(out) + off * wt->esize, dout.ptr, c * wt->esize);
+    }
+  });
+}
+static void tree_rs_host(ecl_wavelet_tree wt, const ecl_rank_select_query* q, size_t m,
+                         uint64_t* out, uint32_t flags, bool is_rank) {
+  require(wt && (m == 0 || (q && out)), "NULL argument");
+  set_device(wt->device);
+  const TreeQueryParams p = query_params(wt, flags);
+  for (size_t off = 0; off < m; off += kHostChunk) {
+    const size_t c = std::min<size_t>(kHostChunk, m - off);
+    check_query_memory(c * 24, is_rank ? "rank" : "select");
+    DevBuf din(c * 16), dout(c * 8);
+    h2d(din.ptr, q + off, c);
+    if (is_rank)
+      launch_tree_rank(p, din.ptr, c, dout.as<uint64_t>(), nullptr);
+    else
+      launch_tree_select(p, din.ptr, c, dout.as<uint64_t>(), nullptr);
+    d2h(out + off, dout.ptr, c);
+  }
+}
+ecl_status ecl_wavelet_tree_rank(ecl_wavelet_tree wt, const ecl_rank_select_query* q,
+                                 size_t m, uint64_t* out, uint32_t flags) {
+  return guard([&] { tree_rs_host(wt, q, m, out, flags, true); });
+}
+ecl_status ecl_wavelet_tree_select(ecl_wavelet_tree wt, const ecl_rank_select_query* q,
+                                   size_t m, uint64_t* out, uint32_t flags) {
+  return guard([&] { tree_rs_host(wt, q, m, out, flags, false); });
+}
+ecl_status ecl_wavelet_tree_access_device(ecl_wavelet_tree wt, const uint64_t* d_idx, size_t m,
+                                          void* d_out, uint32_t flags, void* stream) {
+  return guard([&] {
+    require(wt != nullptr, "NULL argument");
+    set_device(wt->device);
+    launch_tree_access(query_params(wt, flags), d_idx, m, d_out, as_stream(stream));
+  });
+}
+ecl_status ecl_wavelet_tree_rank_device(ecl_wavelet_tree wt, const ecl_rank_select_query* d_q,
+                                        size_t m, uint64_t* d_out, uint32_t flags,
+                                        void* stream) {
+  return guard([&] {
+    require(wt != nullptr, "NULL argument");
+    set_device(wt->device);
+    launch_tree_rank(query_params(wt, flags), d_q, m, d_out, as_stream(stream));
+  });
+}
+ecl_status ecl_wavelet_tree_select_device(ecl_wavelet_tree wt,
+                                          const ecl_rank_select_query* d_q, size_t m,
+                                          uint64_t* d_out, uint32_t flags, void* stream) {
+  return guard([&] {
+    require(wt != nullptr, "NULL argument");
+    set_device(wt->device);
+    launch_tree_select(query_params(wt, flags), d_q, m, d_out, as_stream(stream));
+  });
+}
+ecl_status ecl_wavelet_tree_query_batch_device(ecl_wavelet_tree wt, const ecl_query* d_q,
+                                               size_t m, uint64_t* d_out, void* stream) {
+  return guard([&] {
+    require(wt != nullptr, "NULL argument");
+    set_device(wt->device);
+    launch_tree_mixed(query_params(wt, ECL_SYMBOLS_MINIMAL | ECL_RESULTS_MINIMAL), d_q, m,
+                      d_out, as_stream(stream));
+  });
+}
+ecl_status ecl_wavelet_tree_generate_queries(ecl_wavelet_tree wt, int kind, uint64_t seed,
+                                             size_t m, void* d_out, void* stream) {
+  return guard([&] {
+    require(wt != nullptr && kind >= 0 && kind <= 3, "invalid argument");
+    set_device(wt->device);
+    launch_tree_gen_queries(wt->view, nullptr, kind, seed, m, d_out, wt->esize,
+                            as_stream(stream));
+  });
+}
+
+}  // extern "C"

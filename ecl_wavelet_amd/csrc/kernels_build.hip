@@ -1,0 +1,511 @@
+// kernels_build.hip -- construction kernels for gfx950 (wave64).
+//
+//  * histogram_kernel: LDS-privatised u32 bins, remaps to codes in the same
+//    pass (replaces computeGlobalHistogramKernel, tree/wavelet_tree.cuh:
+//    170-227).
+//  * level_kernel: one pass per level.  Writes the level's bits with one
+//    ballot per 64 symbols and scatters every symbol to its position in the
+//    next level's order (stable one-bit partition inside each node).  The
+//    global ones-prefix needed by the partition comes from a decoupled
+//    look-back over tiles.  Replaces the per-level cub::DeviceRadixSort +
+//    fillLevelKernel pair (wavelet_tree.cuh:241-323, 806-818).
+//  * rs_build_kernel: one pass per bit array.  L2 counts, L1 prefix (look-
+//    back), total ones and both select-sample streams in a single read of
+//    the bits.  Replaces calculateL2EntriesKernel + cub InclusiveSum +
+//    addNumOnesKernel + calculateSelectSamplesKernel (src/bitarray/
+//    rank_select.cu:412-591).
+#include <hip/hip_runtime.h>
+
+#include "internal.hpp"
+
+namespace ecl {
+namespace lib {
+namespace {
+
+using detail::ballot64;
+using detail::lane_id;
+using detail::mbcnt64;
+
+constexpr uint64_t kFlagAgg = 1ull << 62;
+constexpr uint64_t kFlagPre = 2ull << 62;
+constexpr uint64_t kValMask = (1ull << 62) - 1;
+
+__device__ __forceinline__ void status_store(uint64_t* p, uint64_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ uint64_t status_load(const uint64_t* p) {
+  return __hip_atomic_load(const_cast<uint64_t*>(p), __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Decoupled look-back, run by all 64 lanes of one wave.  Publishes this
+// tile's aggregate, walks back 64 predecessors per step until it meets an
+// inclusive prefix, publishes its own inclusive prefix, returns the
+// exclusive one.  Tiles get their ids in launch order from an atomic
+// counter, so every predecessor is resident or finished: the wait ends.
+__device__ uint64_t lookback(uint64_t* status, uint64_t tile, uint64_t agg) {
+  const uint32_t lane = lane_id();
+  if (tile == 0) {
+    if (lane == 0) status_store(&status[0], kFlagPre | agg);
+    return 0;
+  }
+  if (lane == 0) status_store(&status[tile], kFlagAgg | agg);
+  uint64_t excl = 0;
+  int64_t top = static_cast<int64_t>(tile) - 1;
+  while (true) {
+    const int64_t idx = top - static_cast<int64_t>(lane);
+    const uint64_t v = idx >= 0 ? status_load(&status[idx]) : kFlagPre;
+    const uint64_t pre = ballot64((v >> 62) == 2);
+    const uint64_t empty = ballot64((v >> 62) == 0);
+    const uint32_t stop = pre ? static_cast<uint32_t>(__builtin_ctzll(pre)) : 64u;
+    const uint64_t window = stop >= 63 ? ~0ull : ((2ull << stop) - 1);
+    if (empty & window) {
+      __builtin_amdgcn_s_sleep(1);
+      continue;
+    }
+    excl += detail::wave_sum_u64(lane <= stop ? (v & kValMask) : 0ull);
+    if (pre) break;
+    top -= 64;
+  }
+  if (lane == 0) status_store(&status[tile], kFlagPre | (excl + agg));
+  return excl;
+}
+
+template <typename T>
+__device__ __forceinline__ uint64_t load_elem(const T* p, uint64_t i) {
+  return static_cast<uint64_t>(p[i]);
+}
+
+// ---------------------------------------------------------------------------
+__global__ void fill_u64_kernel(uint64_t* p, uint64_t n, uint64_t v) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+       i < n; i += stride)
+    p[i] = v;
+}
+__global__ void fill_u32_kernel(uint32_t* p, uint64_t n, uint32_t v) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+       i < n; i += stride)
+    p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Histogram.  Each wave owns `copies` replicas of the bins in LDS (lane l
+// uses replica l % copies) so that small alphabets do not serialise on a
+// few addresses; large alphabets fall back to global u64 atomics.
+// map_kind: 0 identity, 1 LUT, 2 binary search over the sorted alphabet.
+template <typename T, int MapKind, bool UseLds>
+__global__ __launch_bounds__(256) void histogram_kernel(HistParams p,
+                                                        uint32_t copies) {
+  extern __shared__ uint32_t s_bins[];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t wave = tid >> 6, lane = tid & 63;
+  const uint64_t sigma = p.sigma;
+  uint32_t* my_bins = nullptr;
+  if constexpr (UseLds) {
+    const uint32_t total = 4u * copies * static_cast<uint32_t>(sigma);
+    for (uint32_t i = tid; i < total; i += 256) s_bins[i] = 0;
+    __syncthreads();
+    my_bins = s_bins + (wave * copies + (lane % copies)) * sigma;
+  }
+  const T* text = static_cast<const T*>(p.text);
+  T* out = static_cast<T*>(p.codes_out);
+  const T* alpha = static_cast<const T*>(p.alphabet);
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * 256u;
+  for (uint64_t i = blockIdx.x * 256ull + tid; i < p.n; i += stride) {
+    const uint64_t x = static_cast<uint64_t>(text[i]);
+    uint64_t c;
+    if constexpr (MapKind == 0) {
+      c = x;
+    } else if constexpr (MapKind == 1) {
+      c = p.lut[x];
+    } else {
+      uint64_t lo = 0, hi = sigma;  // lower_bound
+      while (lo < hi) {
+        uint64_t mid = (lo + hi) >> 1;
+        if (static_cast<uint64_t>(alpha[mid]) < x) lo = mid + 1; else hi = mid;
+      }
+      c = lo;
+    }
+    if (c >= sigma) c = sigma - 1;  // symbol outside the alphabet
+    if constexpr (UseLds) {
+      atomicAdd(&my_bins[c], 1u);
+    } else {
+      atomicAdd(reinterpret_cast<unsigned long long*>(&p.hist[c]), 1ull);
+    }
+    if (out != nullptr) {
+      const uint64_t code =
+          (p.pow2 || c < p.codes_start) ? c : p.codes[c - p.codes_start];
+      out[i] = static_cast<T>(code);
+    }
+  }
+  if constexpr (UseLds) {
+    __syncthreads();
+    const uint32_t reps = 4u * copies;
+    for (uint64_t c = tid; c < sigma; c += 256) {
+      uint64_t sum = 0;
+      for (uint32_t r = 0; r < reps; ++r) sum += s_bins[r * sigma + c];
+      if (sum) atomicAdd(reinterpret_cast<unsigned long long*>(&p.hist[c]),
+                         static_cast<unsigned long long>(sum));
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Level pass.  Block = 4 waves; wave w owns K consecutive 64-symbol slots
+// of the tile; lane t of slot k holds symbol tile_base + (w*K + k)*64 + t,
+// so one ballot per slot is exactly one u64 word of the level's bits and
+// the scatter of a slot writes two contiguous runs per node.
+constexpr int kLevelSlots = 16;
+constexpr int kLevelTile = 4 * kLevelSlots * 64;
+
+template <typename T>
+__global__ __launch_bounds__(256) void level_kernel(LevelParams p) {
+  __shared__ uint64_t s_tile;
+  __shared__ uint64_t s_excl;
+  __shared__ uint32_t s_wave[4];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t lane = tid & 63, wave = tid >> 6;
+  if (tid == 0)
+    s_tile = atomicAdd(reinterpret_cast<unsigned long long*>(p.counter), 1ull);
+  __syncthreads();
+  const uint64_t tile = s_tile;
+  const uint64_t wbase =
+      tile * kLevelTile + static_cast<uint64_t>(wave) * (kLevelSlots * 64);
+  const T* src = static_cast<const T*>(p.src);
+
+  uint64_t v[kLevelSlots];
+  uint64_t m[kLevelSlots];
+  uint32_t ones = 0;
+#pragma unroll
+  for (int k = 0; k < kLevelSlots; ++k) {
+    const uint64_t idx = wbase + k * 64 + lane;
+    const bool valid = idx < p.n;
+    const uint64_t x = valid ? load_elem(src, idx) : 0ull;
+    v[k] = x;
+    m[k] = ballot64(valid && ((x >> p.bit) & 1ull));
+    ones += static_cast<uint32_t>(__popcll(m[k]));
+  }
+  // bits: lane k of the wave stores the word of slot k
+  uint64_t my = 0;
+#pragma unroll
+  for (int k = 0; k < kLevelSlots; ++k)
+    if (lane == static_cast<uint32_t>(k)) my = m[k];
+  const uint64_t widx = wbase / 64 + lane;
+  if (lane < static_cast<uint32_t>(kLevelSlots) && widx < p.nwords64)
+    p.words64[widx] = my;
+
+  if (p.dst == nullptr) return;  // last level: nothing to partition
+
+  if (lane == 0) s_wave[wave] = ones;
+  __syncthreads();
+  uint32_t wexcl = 0, ttot = 0;
+#pragma unroll
+  for (uint32_t w = 0; w < 4; ++w) {
+    const uint32_t c = s_wave[w];
+    wexcl += (w < wave) ? c : 0u;
+    ttot += c;
+  }
+  if (wave == 0) {
+    const uint64_t g = lookback(p.status, tile, ttot);
+    if (lane == 0) s_excl = g;
+  }
+  __syncthreads();
+  uint64_t run = s_excl + wexcl;  // ones before this wave's first symbol
+  T* dst = static_cast<T*>(p.dst);
+#pragma unroll
+  for (int k = 0; k < kLevelSlots; ++k) {
+    const uint64_t idx = wbase + k * 64 + lane;
+    const bool b = (m[k] >> lane) & 1ull;
+    const uint64_t r1 = run + mbcnt64(m[k]);
+    const uint64_t r0 = idx - r1;
+    const uint64_t pfx = detail::shr64(v[k], p.pshift);
+    if (idx < p.n && pfx < p.num_prefixes) {
+      const uint64_t base = p.table[2 * pfx + (b ? 1 : 0)];
+      const uint64_t dest = base + (b ? r1 : r0);
+      if (base != detail::kSkip && dest < p.n_next)
+        dst[dest] = static_cast<T>(v[k]);
+    }
+    run += static_cast<uint64_t>(__popcll(m[k]));
+  }
+}
+
+// ---------------------------------------------------------------------------
+// RankSelect directory, one L1 block (1024 u64 words) per workgroup of 256
+// threads, 4 words per thread.
+__global__ __launch_bounds__(256) void rs_build_kernel(RsBuildParams p) {
+  __shared__ uint64_t s_tile;
+  __shared__ uint64_t s_excl;
+  __shared__ uint32_t s_wave[4];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t lane = tid & 63, wave = tid >> 6;
+  if (tid == 0)
+    s_tile = atomicAdd(reinterpret_cast<unsigned long long*>(p.counter), 1ull);
+  __syncthreads();
+  const uint64_t tile = s_tile;
+  const uint64_t nwords = (p.m + 63) / 64;
+  const uint64_t w0 = tile * 1024 + static_cast<uint64_t>(tid) * 4;
+  uint64_t w[4];
+  uint32_t c = 0;
+  if (w0 + 3 < nwords) {
+    const ulonglong2* q = reinterpret_cast<const ulonglong2*>(p.words64 + w0);
+    const ulonglong2 a = q[0], b2 = q[1];
+    w[0] = a.x; w[1] = a.y; w[2] = b2.x; w[3] = b2.y;
+  } else {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) w[k] = (w0 + k < nwords) ? p.words64[w0 + k] : 0ull;
+  }
+  const uint32_t tail = static_cast<uint32_t>(p.m & 63);
+#pragma unroll
+  for (int k = 0; k < 4; ++k) {
+    if (tail && w0 + k == nwords - 1) w[k] &= (1ull << tail) - 1;
+    c += static_cast<uint32_t>(__popcll(w[k]));
+  }
+  // block exclusive scan of c
+  const uint32_t incl = detail::wave_incl_scan_u32(c);
+  if (lane == 63) s_wave[wave] = incl;
+  __syncthreads();
+  uint32_t wexcl = 0, ttot = 0;
+#pragma unroll
+  for (uint32_t ww = 0; ww < 4; ++ww) {
+    const uint32_t x = s_wave[ww];
+    wexcl += (ww < wave) ? x : 0u;
+    ttot += x;
+  }
+  const uint32_t excl = wexcl + incl - c;  // ones in the L1 block before me
+  // L2 entry of 512-bit block j = threads 2j, 2j+1
+  if ((tid & 1) == 0) {
+    const uint64_t k = tile * 128 + (tid >> 1);
+    if (k < p.num_l2) p.l2[k] = static_cast<uint16_t>(excl);
+  }
+  if (wave == 0) {
+    const uint64_t g = lookback(p.status, tile, ttot);
+    if (lane == 0) s_excl = g;
+  }
+  __syncthreads();
+  const uint64_t g = s_excl;
+  if (tid == 0 && tile < p.num_l1) p.l1[tile] = g;
+  if (tid == 0 && tile == p.num_l1 - 1) *p.total_ones = g + ttot;
+
+  // select samples: at most one multiple of 4096 ones (and of zeros) falls
+  // inside a thread's 256 bits
+  const uint64_t b0 = w0 * 64;
+  if (p.cap == 0 || b0 >= p.m) return;
+  const uint64_t b1 = (b0 + 256 < p.m) ? b0 + 256 : p.m;
+  const uint64_t ones_before = g + excl;
+  const uint64_t k1 = (ones_before + c) / detail::kSampleRate;
+  if (k1 > ones_before / detail::kSampleRate) {
+    uint32_t r = static_cast<uint32_t>(k1 * detail::kSampleRate - ones_before);
+    uint64_t pos = b0;
+#pragma unroll 1
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t pc = static_cast<uint32_t>(__popcll(w[k]));
+      if (r <= pc) {
+        pos += detail::select_in_word(w[k], r);
+        break;
+      }
+      r -= pc;
+      pos += 64;
+    }
+    p.samples[k1 - 1] = pos;
+  }
+  const uint64_t zeros_before = b0 - ones_before;
+  const uint64_t zc = (b1 - b0) - c;
+  const uint64_t k0 = (zeros_before + zc) / detail::kSampleRate;
+  if (k0 > zeros_before / detail::kSampleRate) {
+    uint32_t r = static_cast<uint32_t>(k0 * detail::kSampleRate - zeros_before);
+    uint64_t pos = b0;
+#pragma unroll 1
+    for (int k = 0; k < 4; ++k) {
+      uint64_t z = ~w[k];
+      const uint64_t wb = b0 + 64ull * k;
+      if (wb + 64 > p.m) z &= (p.m > wb) ? ((1ull << (p.m - wb)) - 1) : 0ull;
+      const uint32_t pc = static_cast<uint32_t>(__popcll(z));
+      if (r <= pc) {
+        pos += detail::select_in_word(z, r);
+        break;
+      }
+      r -= pc;
+      pos += 64;
+    }
+    p.samples[p.cap - k0] = pos;
+  }
+}
+
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void gen_text_kernel(T* out, uint64_t n, uint64_t seed,
+                                uint64_t sigma, const uint64_t* thr) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+       i < n; i += stride) {
+    const uint64_t h = detail::hash64(seed, i);
+    uint64_t s;
+    if (thr == nullptr) {
+      s = detail::bounded64(h, sigma);
+    } else {
+      uint64_t lo = 0, hi = sigma - 1;
+      while (lo < hi) {
+        const uint64_t mid = (lo + hi) >> 1;
+        if (h < thr[mid]) hi = mid; else lo = mid + 1;
+      }
+      s = lo;
+    }
+    out[i] = static_cast<T>(s);
+  }
+}
+
+// one u32 word per thread; words beyond nbits are cleared
+__global__ void gen_bits_kernel(uint32_t* words, uint64_t nbits,
+                                uint64_t nwords_total, uint64_t seed,
+                                uint64_t threshold) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t w = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+       w < nwords_total; w += stride) {
+    uint32_t x = 0;
+    const uint64_t base = w * 32;
+    if (base < nbits) {
+#pragma unroll 4
+      for (uint32_t b = 0; b < 32; ++b)
+        if (base + b < nbits && detail::hash64(seed, base + b) < threshold)
+          x |= 1u << b;
+    }
+    words[w] = x;
+  }
+}
+
+template <typename T>
+__global__ void presence_kernel(const T* text, uint64_t n, uint32_t* flags) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t i = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+       i < n; i += stride)
+    flags[static_cast<uint64_t>(text[i])] = 1u;
+}
+
+uint32_t grid_for(uint64_t work, uint32_t block, uint32_t max_blocks) {
+  uint64_t g = (work + block - 1) / block;
+  if (g < 1) g = 1;
+  if (g > max_blocks) g = max_blocks;
+  return static_cast<uint32_t>(g);
+}
+
+}  // namespace
+
+void launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t s) {
+  if (n == 0) return;
+  ProfScope ps(K_FILL, s);
+  hipLaunchKernelGGL(fill_u64_kernel, dim3(grid_for(n, 256, 4096)), dim3(256),
+                     0, s, p, n, v);
+  ECL_HIP(hipGetLastError());
+}
+void launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s) {
+  if (n == 0) return;
+  ProfScope ps(K_FILL, s);
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(n, 256, 4096)), dim3(256),
+                     0, s, p, n, v);
+  ECL_HIP(hipGetLastError());
+}
+
+void launch_histogram(const HistParams& p, hipStream_t s) {
+  if (p.n == 0) return;
+  ProfScope ps(K_HIST, s);
+  // replicas per wave: keep 4 * copies * sigma u32 bins within 32 KiB
+  uint32_t copies = 0;
+  if (p.sigma <= 2048) {
+    copies = 16;
+    while (copies > 1 && 4ull * copies * p.sigma * 4 > 32768) copies >>= 1;
+  }
+  const bool lds = copies > 0;
+  // each block handles at most 2^31 symbols so u32 bins cannot overflow
+  uint64_t min_blocks = (p.n >> 31) + 1;
+  uint32_t grid = grid_for(p.n, 256 * 16, 8192);
+  if (grid < min_blocks) grid = static_cast<uint32_t>(min_blocks);
+  const size_t shmem = lds ? 4ull * copies * p.sigma * sizeof(uint32_t) : 0;
+#define ECL_HIST_LAUNCH(TT, MK)                                                   \
+  do {                                                                            \
+    if (lds)                                                                      \
+      hipLaunchKernelGGL((histogram_kernel<TT, MK, true>), dim3(grid), dim3(256), \
+                         shmem, s, p, copies);                                    \
+    else                                                                          \
+      hipLaunchKernelGGL((histogram_kernel<TT, MK, false>), dim3(grid),           \
+                         dim3(256), 0, s, p, 0u);                                 \
+  } while (0)
+#define ECL_HIST_TYPE(TT)                     \
+  do {                                        \
+    if (p.map_kind == 0) ECL_HIST_LAUNCH(TT, 0); \
+    else if (p.map_kind == 1) ECL_HIST_LAUNCH(TT, 1); \
+    else ECL_HIST_LAUNCH(TT, 2);              \
+  } while (0)
+  switch (p.esize) {
+    case 1: ECL_HIST_TYPE(uint8_t); break;
+    case 2: ECL_HIST_TYPE(uint16_t); break;
+    case 4: ECL_HIST_TYPE(uint32_t); break;
+    default: ECL_HIST_TYPE(uint64_t); break;
+  }
+#undef ECL_HIST_TYPE
+#undef ECL_HIST_LAUNCH
+  ECL_HIP(hipGetLastError());
+}
+
+uint64_t level_tiles(uint64_t n) { return (n + kLevelTile - 1) / kLevelTile; }
+
+void launch_level(const LevelParams& p, hipStream_t s) {
+  if (p.n == 0) return;
+  const uint64_t tiles = level_tiles(p.n);
+  if (tiles > 0x7FFFFFFFull) throw Error(ECL_ERROR_INVALID_ARGUMENT, "level too large");
+  ProfScope ps(K_LEVEL, s);
+  switch (p.esize) {
+    case 1: hipLaunchKernelGGL(level_kernel<uint8_t>, dim3(tiles), dim3(256), 0, s, p); break;
+    case 2: hipLaunchKernelGGL(level_kernel<uint16_t>, dim3(tiles), dim3(256), 0, s, p); break;
+    case 4: hipLaunchKernelGGL(level_kernel<uint32_t>, dim3(tiles), dim3(256), 0, s, p); break;
+    default: hipLaunchKernelGGL(level_kernel<uint64_t>, dim3(tiles), dim3(256), 0, s, p); break;
+  }
+  ECL_HIP(hipGetLastError());
+}
+
+void launch_rs_build(const RsBuildParams& p, hipStream_t s) {
+  if (p.m == 0) return;
+  if (p.num_l1 > 0x7FFFFFFFull) throw Error(ECL_ERROR_INVALID_ARGUMENT, "bit array too large");
+  ProfScope ps(K_RS_BUILD, s);
+  hipLaunchKernelGGL(rs_build_kernel, dim3(static_cast<uint32_t>(p.num_l1)), dim3(256), 0, s, p);
+  ECL_HIP(hipGetLastError());
+}
+
+void launch_gen_text(void* out, int esize, uint64_t n, uint64_t seed,
+                     uint64_t sigma, const uint64_t* thr, hipStream_t s) {
+  if (n == 0) return;
+  ProfScope ps(K_GEN, s);
+  const dim3 g(grid_for(n, 256, 65536)), b(256);
+  switch (esize) {
+    case 1: hipLaunchKernelGGL(gen_text_kernel<uint8_t>, g, b, 0, s, static_cast<uint8_t*>(out), n, seed, sigma, thr); break;
+    case 2: hipLaunchKernelGGL(gen_text_kernel<uint16_t>, g, b, 0, s, static_cast<uint16_t*>(out), n, seed, sigma, thr); break;
+    case 4: hipLaunchKernelGGL(gen_text_kernel<uint32_t>, g, b, 0, s, static_cast<uint32_t*>(out), n, seed, sigma, thr); break;
+    default: hipLaunchKernelGGL(gen_text_kernel<uint64_t>, g, b, 0, s, static_cast<uint64_t*>(out), n, seed, sigma, thr); break;
+  }
+  ECL_HIP(hipGetLastError());
+}
+
+void launch_gen_bits(uint32_t* words, uint64_t nbits, uint64_t nwords_total,
+                     uint64_t seed, uint64_t threshold, hipStream_t s) {
+  if (nwords_total == 0) return;
+  ProfScope ps(K_GEN, s);
+  hipLaunchKernelGGL(gen_bits_kernel, dim3(grid_for(nwords_total, 256, 65536)),
+                     dim3(256), 0, s, words, nbits, nwords_total, seed, threshold);
+  ECL_HIP(hipGetLastError());
+}
+
+void launch_presence(const void* text, int esize, uint64_t n, uint32_t* flags,
+                     hipStream_t s) {
+  if (n == 0) return;
+  const dim3 g(grid_for(n, 256, 16384)), b(256);
+  switch (esize) {
+    case 1: hipLaunchKernelGGL(presence_kernel<uint8_t>, g, b, 0, s, static_cast<const uint8_t*>(text), n, flags); break;
+    case 2: hipLaunchKernelGGL(presence_kernel<uint16_t>, g, b, 0, s, static_cast<const uint16_t*>(text), n, flags); break;
+    default: throw Error(ECL_ERROR_INTERNAL, "presence kernel only for 8/16-bit texts");
+  }
+  ECL_HIP(hipGetLastError());
+}
+
+}  // namespace lib
+}  // namespace ecl

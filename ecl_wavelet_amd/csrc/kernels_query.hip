@@ -1,0 +1,360 @@
+// kernels_query.hip -- batched query kernels for gfx950.
+//
+// One lane per query, grid-stride.  Per-block LDS caches the small, hot
+// metadata every descent touches (level descriptors, node counts per level,
+// and, when they fit, the C array and the node ranks), like the reference's
+// shmem variants of accessKernel / rankKernel / selectKernel
+// (tree/wavelet_tree.cuh:373-544).  The random bit-array lines are the only
+// HBM traffic left per level.
+#include <hip/hip_runtime.h>
+
+#include "internal.hpp"
+
+namespace ecl {
+namespace lib {
+namespace {
+
+using detail::LevelView;
+using detail::TreeView;
+
+constexpr uint64_t kMaxLdsTable = 2048;  // u64 entries for C and for ranks
+constexpr uint32_t kMaxLevels = 64;
+
+struct Meta {
+  const LevelView* lv;
+  const uint64_t* counts;
+  const uint64_t* ranks;
+  const uint64_t* nal;
+};
+
+__host__ __device__ __forceinline__ bool lds_counts(const TreeView& t) {
+  return t.sigma <= kMaxLdsTable;
+}
+__host__ __device__ __forceinline__ bool lds_ranks(const TreeView& t) {
+  return t.num_ranks <= kMaxLdsTable;
+}
+// LDS layout: levels[L] | nal[L] | counts[sigma]? | ranks[R]?
+__host__ __device__ __forceinline__ size_t meta_bytes_of(const TreeView& t) {
+  size_t b = t.num_levels * (sizeof(LevelView) + sizeof(uint64_t));
+  if (lds_counts(t)) b += t.sigma * sizeof(uint64_t);
+  if (lds_ranks(t)) b += t.num_ranks * sizeof(uint64_t);
+  return b;
+}
+
+__device__ Meta stage_meta(const TreeView& t, uint8_t* lds) {
+  const uint32_t L = t.num_levels;
+  LevelView* s_lv = reinterpret_cast<LevelView*>(lds);
+  uint64_t* s_nal = reinterpret_cast<uint64_t*>(lds + L * sizeof(LevelView));
+  uint64_t* s_next = s_nal + L;
+  const uint32_t tid = threadIdx.x;
+  for (uint32_t i = tid; i < L; i += blockDim.x) s_lv[i] = t.levels[i];
+  for (uint32_t i = tid; i < L; i += blockDim.x)
+    s_nal[i] = (t.pow2 || i + 1 >= L) ? 0 : t.nodes_at_level[i];
+  Meta m{s_lv, t.counts, t.ranks, s_nal};
+  if (lds_counts(t)) {
+    for (uint64_t i = tid; i < t.sigma; i += blockDim.x) s_next[i] = t.counts[i];
+    m.counts = s_next;
+    s_next += t.sigma;
+  }
+  if (lds_ranks(t)) {
+    for (uint64_t i = tid; i < t.num_ranks; i += blockDim.x) s_next[i] = t.ranks[i];
+    m.ranks = s_next;
+  }
+  __syncthreads();
+  return m;
+}
+
+__device__ __forceinline__ uint64_t load_sym(const void* q16, int esize) {
+  // symbol of a 16-byte RankSelectQuery<T>: first sizeof(T) bytes at +8
+  const uint8_t* p = static_cast<const uint8_t*>(q16) + 8;
+  switch (esize) {
+    case 1: return *p;
+    case 2: return *reinterpret_cast<const uint16_t*>(p);
+    case 4: return *reinterpret_cast<const uint32_t*>(p);
+    default: return *reinterpret_cast<const uint64_t*>(p);
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ uint64_t map_symbol(const TreeQueryParams& p,
+                                               uint64_t x) {
+  if (p.symbols_minimal || p.map_kind == 0) return x;
+  if (p.map_kind == 1) {
+    const uint64_t lim = sizeof(T) == 1 ? 256 : 65536;
+    return x < lim ? p.lut[x] : p.view.sigma;
+  }
+  const T* a = static_cast<const T*>(p.alphabet);
+  uint64_t lo = 0, hi = p.view.sigma;
+  while (lo < hi) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (static_cast<uint64_t>(a[mid]) < x) lo = mid + 1; else hi = mid;
+  }
+  return lo;  // == sigma when x is larger than every symbol
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void tree_access_kernel(
+    TreeQueryParams p, const uint64_t* __restrict__ idx, uint64_t m,
+    T* __restrict__ out) {
+  extern __shared__ uint8_t s_meta[];
+  const Meta mt = stage_meta(p.view, s_meta);
+  const T* amap = static_cast<const T*>(p.alphabet_out);
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t q = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+       q < m; q += stride) {
+    const uint64_t i = idx[q];
+    if (i >= p.view.n) continue;  // accessKernel skips such indices
+    const uint64_t r =
+        detail::tree_access(p.view, mt.lv, mt.counts, mt.ranks, mt.nal, i);
+    out[q] = amap ? amap[r] : static_cast<T>(r);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void tree_rank_kernel(
+    TreeQueryParams p, const uint8_t* __restrict__ qs, uint64_t m,
+    uint64_t* __restrict__ out) {
+  extern __shared__ uint8_t s_meta[];
+  const Meta mt = stage_meta(p.view, s_meta);
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t q = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+       q < m; q += stride) {
+    const uint8_t* e = qs + q * 16;
+    const uint64_t i = *reinterpret_cast<const uint64_t*>(e);
+    const uint64_t c = map_symbol<T>(p, load_sym(e, sizeof(T)));
+    out[q] = i > p.view.n ? 0
+                          : detail::tree_rank(p.view, mt.lv, mt.counts,
+                                              mt.ranks, mt.nal, c, i);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void tree_select_kernel(
+    TreeQueryParams p, const uint8_t* __restrict__ qs, uint64_t m,
+    uint64_t* __restrict__ out) {
+  extern __shared__ uint8_t s_meta[];
+  const Meta mt = stage_meta(p.view, s_meta);
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t q = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+       q < m; q += stride) {
+    const uint8_t* e = qs + q * 16;
+    const uint64_t k = *reinterpret_cast<const uint64_t*>(e);
+    const uint64_t c = map_symbol<T>(p, load_sym(e, sizeof(T)));
+    out[q] = detail::tree_select(p.view, mt.lv, mt.counts, mt.ranks, mt.nal,
+                                 c, k);
+  }
+}
+
+__global__ __launch_bounds__(256) void tree_mixed_kernel(
+    TreeQueryParams p, const ecl_query* __restrict__ qs, uint64_t m,
+    uint64_t* __restrict__ out) {
+  extern __shared__ uint8_t s_meta[];
+  const Meta mt = stage_meta(p.view, s_meta);
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t q = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+       q < m; q += stride) {
+    const ecl_query e = qs[q];
+    uint64_t r;
+    if (e.kind == 0) {
+      r = e.arg < p.view.n ? detail::tree_access(p.view, mt.lv, mt.counts,
+                                                 mt.ranks, mt.nal, e.arg)
+                           : 0;
+    } else if (e.kind == 1) {
+      r = e.arg > p.view.n ? 0
+                           : detail::tree_rank(p.view, mt.lv, mt.counts,
+                                               mt.ranks, mt.nal, e.symbol, e.arg);
+    } else {
+      r = detail::tree_select(p.view, mt.lv, mt.counts, mt.ranks, mt.nal,
+                              e.symbol, e.arg);
+    }
+    out[q] = r;
+  }
+}
+
+// Query generators: field f of query q uses hash(seed + f * golden2, q).
+constexpr uint64_t kField = 0xD1B54A32D192ED03ull;
+
+__device__ __forceinline__ uint64_t symbol_count(const TreeView& t, uint64_t c) {
+  return c == t.sigma - 1 ? t.n - t.counts[c] : t.counts[c + 1] - t.counts[c];
+}
+
+__global__ void tree_gen_kernel(TreeView t, int kind, uint64_t seed,
+                                uint64_t m, void* out, int esize) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t q = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+       q < m; q += stride) {
+    const uint64_t h0 = detail::hash64(seed, q);
+    const uint64_t h1 = detail::hash64(seed + kField, q);
+    const uint64_t h2 = detail::hash64(seed + 2 * kField, q);
+    int k = kind == 3 ? static_cast<int>(q % 3) : kind;
+    const uint64_t c = detail::bounded64(h1, t.sigma);
+    uint64_t arg;
+    if (k == 0) {
+      arg = detail::bounded64(h0, t.n);
+    } else if (k == 1) {
+      arg = detail::bounded64(h2, t.n);
+    } else {
+      const uint64_t cnt = symbol_count(t, c);
+      arg = cnt ? 1 + detail::bounded64(h2, cnt) : 1;
+    }
+    if (kind == 0) {
+      static_cast<uint64_t*>(out)[q] = arg;
+    } else if (kind == 3) {
+      ecl_query e;
+      e.arg = arg;
+      e.symbol = static_cast<uint32_t>(k == 0 ? 0 : c);
+      e.kind = static_cast<uint32_t>(k);
+      static_cast<ecl_query*>(out)[q] = e;
+    } else {
+      uint8_t* e = static_cast<uint8_t*>(out) + q * 16;
+      *reinterpret_cast<uint64_t*>(e) = arg;
+      *reinterpret_cast<uint64_t*>(e + 8) = 0;
+      switch (esize) {
+        case 1: e[8] = static_cast<uint8_t>(c); break;
+        case 2: *reinterpret_cast<uint16_t*>(e + 8) = static_cast<uint16_t>(c); break;
+        case 4: *reinterpret_cast<uint32_t*>(e + 8) = static_cast<uint32_t>(c); break;
+        default: *reinterpret_cast<uint64_t*>(e + 8) = c; break;
+      }
+    }
+  }
+}
+
+template <int Value>
+__global__ __launch_bounds__(256) void rs_rank_kernel(
+    const LevelView* lvp, const uint64_t* __restrict__ idx, uint64_t m,
+    uint64_t* __restrict__ out, int generated, uint64_t seed) {
+  const LevelView v = *lvp;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t q = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+       q < m; q += stride) {
+    const uint64_t i =
+        generated ? detail::bounded64(detail::hash64(seed, q), v.size) : idx[q];
+    out[q] = detail::level_rank<Value>(v, i);
+  }
+}
+
+template <int Value>
+__global__ __launch_bounds__(256) void rs_select_kernel(
+    const LevelView* lvp, const uint64_t* __restrict__ ks, uint64_t m,
+    uint64_t* __restrict__ out, int generated, uint64_t seed) {
+  const LevelView v = *lvp;
+  const uint64_t total = Value ? v.ones : v.size - v.ones;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t q = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+       q < m; q += stride) {
+    const uint64_t k =
+        generated ? (total ? 1 + detail::bounded64(detail::hash64(seed, q), total) : 1)
+                  : ks[q];
+    out[q] = detail::level_select<Value>(v, k);
+  }
+}
+
+uint32_t query_grid(uint64_t m) {
+  uint64_t g = (m + 255) / 256;
+  if (g < 1) g = 1;
+  if (g > 8192) g = 8192;
+  return static_cast<uint32_t>(g);
+}
+
+size_t meta_bytes(const TreeView& t) {
+  if (t.num_levels > kMaxLevels)
+    throw Error(ECL_ERROR_INVALID_ARGUMENT, "too many levels");
+  return meta_bytes_of(t);
+}
+
+}  // namespace
+
+void launch_tree_access(const TreeQueryParams& p, const uint64_t* idx,
+                        uint64_t m, void* out, hipStream_t s) {
+  if (m == 0) return;
+  ProfScope ps(K_WT_ACCESS, s);
+  const dim3 g(query_grid(m)), b(256);
+  const size_t sh = meta_bytes(p.view);
+  switch (p.esize) {
+    case 1: hipLaunchKernelGGL(tree_access_kernel<uint8_t>, g, b, sh, s, p, idx, m, static_cast<uint8_t*>(out)); break;
+    case 2: hipLaunchKernelGGL(tree_access_kernel<uint16_t>, g, b, sh, s, p, idx, m, static_cast<uint16_t*>(out)); break;
+    case 4: hipLaunchKernelGGL(tree_access_kernel<uint32_t>, g, b, sh, s, p, idx, m, static_cast<uint32_t*>(out)); break;
+    default: hipLaunchKernelGGL(tree_access_kernel<uint64_t>, g, b, sh, s, p, idx, m, static_cast<uint64_t*>(out)); break;
+  }
+  ECL_HIP(hipGetLastError());
+}
+
+void launch_tree_rank(const TreeQueryParams& p, const void* q, uint64_t m,
+                      uint64_t* out, hipStream_t s) {
+  if (m == 0) return;
+  ProfScope ps(K_WT_RANK, s);
+  const dim3 g(query_grid(m)), b(256);
+  const size_t sh = meta_bytes(p.view);
+  const uint8_t* qq = static_cast<const uint8_t*>(q);
+  switch (p.esize) {
+    case 1: hipLaunchKernelGGL(tree_rank_kernel<uint8_t>, g, b, sh, s, p, qq, m, out); break;
+    case 2: hipLaunchKernelGGL(tree_rank_kernel<uint16_t>, g, b, sh, s, p, qq, m, out); break;
+    case 4: hipLaunchKernelGGL(tree_rank_kernel<uint32_t>, g, b, sh, s, p, qq, m, out); break;
+    default: hipLaunchKernelGGL(tree_rank_kernel<uint64_t>, g, b, sh, s, p, qq, m, out); break;
+  }
+  ECL_HIP(hipGetLastError());
+}
+
+void launch_tree_select(const TreeQueryParams& p, const void* q, uint64_t m,
+                        uint64_t* out, hipStream_t s) {
+  if (m == 0) return;
+  ProfScope ps(K_WT_SELECT, s);
+  const dim3 g(query_grid(m)), b(256);
+  const size_t sh = meta_bytes(p.view);
+  const uint8_t* qq = static_cast<const uint8_t*>(q);
+  switch (p.esize) {
+    case 1: hipLaunchKernelGGL(tree_select_kernel<uint8_t>, g, b, sh, s, p, qq, m, out); break;
+    case 2: hipLaunchKernelGGL(tree_select_kernel<uint16_t>, g, b, sh, s, p, qq, m, out); break;
+    case 4: hipLaunchKernelGGL(tree_select_kernel<uint32_t>, g, b, sh, s, p, qq, m, out); break;
+    default: hipLaunchKernelGGL(tree_select_kernel<uint64_t>, g, b, sh, s, p, qq, m, out); break;
+  }
+  ECL_HIP(hipGetLastError());
+}
+
+void launch_tree_mixed(const TreeQueryParams& p, const ecl_query* q,
+                       uint64_t m, uint64_t* out, hipStream_t s) {
+  if (m == 0) return;
+  ProfScope ps(K_WT_MIXED, s);
+  hipLaunchKernelGGL(tree_mixed_kernel, dim3(query_grid(m)), dim3(256),
+                     meta_bytes(p.view), s, p, q, m, out);
+  ECL_HIP(hipGetLastError());
+}
+
+void launch_tree_gen_queries(const TreeView& v, const uint64_t*, int kind,
+                             uint64_t seed, uint64_t m, void* out, int esize,
+                             hipStream_t s) {
+  if (m == 0) return;
+  ProfScope ps(K_GEN, s);
+  hipLaunchKernelGGL(tree_gen_kernel, dim3(query_grid(m)), dim3(256), 0, s, v,
+                     kind, seed, m, out, esize);
+  ECL_HIP(hipGetLastError());
+}
+
+void launch_rs_rank(const LevelView* lv, int value, const uint64_t* idx,
+                    uint64_t m, uint64_t* out, int generated, uint64_t seed,
+                    hipStream_t s) {
+  if (m == 0) return;
+  ProfScope ps(K_RS_RANK, s);
+  const dim3 g(query_grid(m)), b(256);
+  if (value)
+    hipLaunchKernelGGL(rs_rank_kernel<1>, g, b, 0, s, lv, idx, m, out, generated, seed);
+  else
+    hipLaunchKernelGGL(rs_rank_kernel<0>, g, b, 0, s, lv, idx, m, out, generated, seed);
+  ECL_HIP(hipGetLastError());
+}
+
+void launch_rs_select(const LevelView* lv, int value, const uint64_t* k,
+                      uint64_t m, uint64_t* out, int generated, uint64_t seed,
+                      hipStream_t s) {
+  if (m == 0) return;
+  ProfScope ps(K_RS_SELECT, s);
+  const dim3 g(query_grid(m)), b(256);
+  if (value)
+    hipLaunchKernelGGL(rs_select_kernel<1>, g, b, 0, s, lv, k, m, out, generated, seed);
+  else
+    hipLaunchKernelGGL(rs_select_kernel<0>, g, b, 0, s, lv, k, m, out, generated, seed);
+  ECL_HIP(hipGetLastError());
+}
+
+}  // namespace lib
+}  // namespace ecl

@@ -1,0 +1,431 @@
+// device_core.hpp -- wave64 (gfx950) device primitives shared by the HIP
+// library's kernels and the header-only device API of the drop-in classes.
+//
+// Layout conventions kept from the reference so that every exposed value is
+// identical (mfranzreb/ECL_wavelet):
+//   * bit j of an array is (word[j/32] >> (j%32)) & 1, u32 words
+//     (bitarray/bit_array.cuh:39-46); each array is padded to a multiple of
+//     32 words = 128 B (src/bitarray/bit_array.cu:57-72);
+//   * L2 = u16 ones count per 512-bit block relative to its 65536-bit L1
+//     block, L1 = u64 ones before the L1 block (bitarray/rank_select.cuh:
+//     44-64);
+//   * select samples every 4096 ones / zeros (rank_select.cuh:59).
+// Select samples of one array share one buffer of floor(m/4096) slots: the
+// ones samples grow from the front, the zeros samples from the back.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace ecl {
+namespace detail {
+
+constexpr uint64_t kL2Bits = 512;
+constexpr uint64_t kL2PerL1 = 128;
+constexpr uint64_t kL1Bits = kL2Bits * kL2PerL1;
+constexpr uint64_t kSampleRate = 4096;
+constexpr uint64_t kSkip = ~0ull;  // node-table entry of a leaf child
+
+// ---------------------------------------------------------------------------
+// wave64 helpers
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t lane_id() { return __lane_id(); }
+
+__device__ __forceinline__ uint64_t ballot64(bool p) {
+  return static_cast<uint64_t>(__ballot(p));
+}
+
+// number of set bits of `mask` in lanes below the calling lane
+__device__ __forceinline__ uint32_t mbcnt64(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi(
+      static_cast<uint32_t>(mask >> 32),
+      __builtin_amdgcn_mbcnt_lo(static_cast<uint32_t>(mask), 0u));
+}
+
+__device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+
+__device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v) {
+  const uint32_t lane = lane_id();
+#pragma unroll
+  for (int off = 1; off < 64; off <<= 1) {
+    uint32_t t = __shfl_up(v, off, 64);
+    if (lane >= static_cast<uint32_t>(off)) v += t;
+  }
+  return v;
+}
+
+// position (0-based) of the r-th (1-based) set bit of x; r <= popcount(x)
+__device__ __forceinline__ uint32_t select_in_word(uint64_t x, uint32_t r) {
+  uint32_t pos = 0;
+  uint32_t c = __popc(static_cast<uint32_t>(x));
+  if (r > c) {
+    r -= c;
+    x >>= 32;
+    pos = 32;
+  }
+  uint32_t y = static_cast<uint32_t>(x);
+  c = __popc(y & 0xFFFFu);
+  if (r > c) { r -= c; y >>= 16; pos += 16; }
+  c = __popc(y & 0xFFu);
+  if (r > c) { r -= c; y >>= 8; pos += 8; }
+  c = __popc(y & 0xFu);
+  if (r > c) { r -= c; y >>= 4; pos += 4; }
+  c = __popc(y & 0x3u);
+  if (r > c) { r -= c; y >>= 2; pos += 2; }
+  if (r > (y & 1u)) pos += 1;
+  return pos;
+}
+
+__host__ __device__ __forceinline__ bool is_pow2(uint64_t n) {
+  return (n & (n - 1)) == 0;
+}
+// largest power of two strictly below n, 0 for n == 0
+// (utils/utils.cuh:138-158)
+__host__ __device__ __forceinline__ uint64_t prev_pow2(uint64_t n) {
+  if (n == 0) return 0;
+  if (is_pow2(n)) return n >> 1;
+#if defined(__HIP_DEVICE_COMPILE__)
+  return 1ull << (63 - __clzll(static_cast<long long>(n)));
+#else
+  return 1ull << (63 - __builtin_clzll(n));
+#endif
+}
+__host__ __device__ __forceinline__ uint32_t ceil_log2(uint64_t n) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint32_t hb = 63 - __clzll(static_cast<long long>(n));
+#else
+  uint32_t hb = 63 - __builtin_clzll(n);
+#endif
+  return is_pow2(n) ? hb : hb + 1;
+}
+__host__ __device__ __forceinline__ uint64_t shr64(uint64_t v, uint32_t s) {
+  return s >= 64 ? 0 : (v >> s);
+}
+
+// counter-based generator shared with the oracle: splitmix64 finaliser over
+// seed + (i+1)*golden, Lemire's multiply-high reduction to [0, m)
+__host__ __device__ __forceinline__ uint64_t hash64(uint64_t seed,
+                                                    uint64_t i) {
+  uint64_t z = seed + (i + 1) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__host__ __device__ __forceinline__ uint64_t bounded64(uint64_t h,
+                                                       uint64_t m) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __umul64hi(h, m);
+#else
+  return static_cast<uint64_t>((static_cast<unsigned __int128>(h) * m) >> 64);
+#endif
+}
+
+// ---------------------------------------------------------------------------
+// One bit array with its rank/select directory.  Trivially copyable; the
+// library keeps one per array in device memory.
+// ---------------------------------------------------------------------------
+struct LevelView {
+  const uint32_t* words;    // word 0 of this array (128 B aligned)
+  const uint64_t* l1;       // L1 entries of this array
+  const uint16_t* l2;       // L2 entries of this array
+  const uint64_t* samples;  // ones samples from the front, zeros from back
+  uint64_t size;            // bits
+  uint64_t ones;            // total ones
+  uint64_t num_l1;
+  uint64_t sample_cap;      // floor(size / 4096)
+};
+
+__device__ __forceinline__ bool level_bit(const LevelView& v, uint64_t i) {
+  return (v.words[i >> 5] >> (i & 31)) & 1u;
+}
+
+// Ones in [0, index) for index in (0, size): L1 + L2 + popcount of the
+// 512-bit block prefix.  The 64 B block lies inside the array's 128 B
+// padding, so the full-block load never leaves the allocation.
+__device__ __forceinline__ uint64_t rank1_inner(const LevelView& v,
+                                                uint64_t index, bool* bit) {
+  const uint64_t blk = index >> 9;
+  const uint4* p = reinterpret_cast<const uint4*>(v.words + (blk << 4));
+  uint4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3];
+  uint64_t r = v.l1[index >> 16] + v.l2[blk];
+  uint64_t w[8] = {
+      (uint64_t)q0.x | ((uint64_t)q0.y << 32), (uint64_t)q0.z | ((uint64_t)q0.w << 32),
+      (uint64_t)q1.x | ((uint64_t)q1.y << 32), (uint64_t)q1.z | ((uint64_t)q1.w << 32),
+      (uint64_t)q2.x | ((uint64_t)q2.y << 32), (uint64_t)q2.z | ((uint64_t)q2.w << 32),
+      (uint64_t)q3.x | ((uint64_t)q3.y << 32), (uint64_t)q3.z | ((uint64_t)q3.w << 32)};
+  const uint32_t wi = static_cast<uint32_t>((index >> 6) & 7);
+  const uint32_t bi = static_cast<uint32_t>(index & 63);
+  uint32_t acc = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < 8; ++k) {
+    uint64_t x = w[k];
+    if (k == wi) {
+      *bit = (x >> bi) & 1u;
+      x &= (bi == 0) ? 0ull : (~0ull >> (64 - bi));
+    }
+    acc += (k <= wi) ? static_cast<uint32_t>(__popcll(x)) : 0u;
+  }
+  return r + acc;
+}
+
+// rank_Value(index) over [0, index), index in [0, size]; rank_select.cuh:
+// 125-208 semantics (index == size returns the total, bit = false).
+template <int Value>
+__device__ __forceinline__ uint64_t level_rank(const LevelView& v,
+                                               uint64_t index,
+                                               bool* bit = nullptr) {
+  bool b = false;
+  uint64_t ones;
+  if (index == 0) {
+    ones = 0;
+    if (bit) b = level_bit(v, 0);
+  } else if (index >= v.size) {
+    ones = v.ones;
+    index = v.size;
+  } else {
+    ones = rank1_inner(v, index, &b);
+  }
+  if (bit) *bit = b;
+  return Value ? ones : index - ones;
+}
+
+// Position of the i-th (1-based) Value; size when i exceeds the total
+// (rank_select.cuh:224-523 semantics).  Search: sample pair brackets the
+// answer, binary search over L1 then L2 inside the bracket, then one 64 B
+// block scan.
+template <int Value>
+__device__ __forceinline__ uint64_t level_select(const LevelView& v,
+                                                 uint64_t i) {
+  const uint64_t total = Value ? v.ones : v.size - v.ones;
+  if (i == 0 || i > total) return v.size;
+  const uint64_t j = i / kSampleRate;
+  const uint64_t nsamp = total / kSampleRate;
+  uint64_t lo_pos = 0, hi_pos = v.size - 1;
+  if (j > 0) {
+    lo_pos = Value ? v.samples[j - 1] : v.samples[v.sample_cap - j];
+    if (i == j * kSampleRate) return lo_pos;
+  }
+  if (j < nsamp) hi_pos = Value ? v.samples[j] : v.samples[v.sample_cap - 1 - j];
+  // L1: largest block b in [lo_pos>>16, hi_pos>>16] with count_before(b) < i
+  uint64_t b_lo = lo_pos >> 16, b_hi = hi_pos >> 16;
+  while (b_lo < b_hi) {
+    uint64_t mid = (b_lo + b_hi + 1) >> 1;
+    uint64_t c = Value ? v.l1[mid] : (mid << 16) - v.l1[mid];
+    if (c < i) b_lo = mid; else b_hi = mid - 1;
+  }
+  const uint64_t blk1 = b_lo;
+  const uint64_t c1 = Value ? v.l1[blk1] : (blk1 << 16) - v.l1[blk1];
+  uint64_t r = i - c1;  // rank inside the L1 block (1-based)
+  // L2: largest k in [max(lo_pos>>9, blk1*128), min(hi_pos>>9, end)] with
+  // count_before(k) < r
+  uint64_t k_lo = lo_pos >> 9, k_hi = hi_pos >> 9;
+  const uint64_t first = blk1 << 7, last = first + kL2PerL1 - 1;
+  if (k_lo < first) k_lo = first;
+  if (k_hi > last) k_hi = last;
+  while (k_lo < k_hi) {
+    uint64_t mid = (k_lo + k_hi + 1) >> 1;
+    uint64_t c = Value ? v.l2[mid] : ((mid - first) << 9) - v.l2[mid];
+    if (c < r) k_lo = mid; else k_hi = mid - 1;
+  }
+  const uint64_t blk2 = k_lo;
+  r -= Value ? v.l2[blk2] : ((blk2 - first) << 9) - v.l2[blk2];
+  const uint64_t* w = reinterpret_cast<const uint64_t*>(v.words) + (blk2 << 3);
+  uint64_t pos = blk2 << 9;
+#pragma unroll 1
+  for (uint32_t k = 0; k < 8; ++k) {
+    uint64_t x = Value ? w[k] : ~w[k];
+    uint32_t c = static_cast<uint32_t>(__popcll(x));
+    if (r <= c) return pos + select_in_word(x, static_cast<uint32_t>(r));
+    r -= c;
+    pos += 64;
+  }
+  return v.size;  // unreachable for a consistent directory
+}
+
+// ---------------------------------------------------------------------------
+// Wavelet tree view (minimal-alphabet symbols), trivially copyable.
+// ---------------------------------------------------------------------------
+struct TreeView {
+  const LevelView* levels;      // [num_levels]
+  const uint64_t* counts;       // C[c]: occurrences of symbols < c
+  const uint64_t* ranks;        // node ranks ordered by (level, symbol)
+  const uint64_t* nodes_at_level;  // [num_levels - 1] node starts per level
+  const uint64_t* codes;        // code of symbol codes_start + k
+  const uint8_t* code_lens;     // length of that code
+  uint64_t n;
+  uint64_t sigma;
+  uint64_t codes_start;
+  uint64_t nodes_until_last_level;
+  uint64_t num_ranks;
+  uint32_t num_levels;
+  uint32_t pow2;
+};
+
+__device__ __forceinline__ uint64_t tree_total(const TreeView& t,
+                                               const uint64_t* counts,
+                                               uint64_t c) {
+  return c == t.sigma - 1 ? t.n - counts[c] : counts[c + 1] - counts[c];
+}
+
+__device__ __forceinline__ uint64_t tree_nodes_at(const TreeView& t,
+                                                  const uint64_t* nal,
+                                                  uint32_t l) {
+  return t.pow2 ? (1ull << l) - 1 : nal[l];
+}
+
+// getNodePosAtLevel (tree/wavelet_tree.cuh:2241-2268)
+__device__ __forceinline__ uint64_t tree_node_pos(const TreeView& t,
+                                                  uint64_t symbol,
+                                                  uint32_t level) {
+  if (level == 1) return 0;
+  uint64_t prev = 0, node_len = 1;
+  if (t.pow2) {
+    node_len = 1ull << (t.num_levels - level);
+  } else {
+    uint64_t remaining = t.sigma;
+    for (int32_t i = static_cast<int32_t>(level) - 1; i >= 0; --i) {
+      uint64_t sub = prev_pow2(remaining);
+      node_len = sub >> i;
+      if (symbol <= sub) break;
+      remaining -= sub;
+      prev += sub / node_len;
+      symbol -= sub;
+    }
+  }
+  return symbol / node_len + prev - 1;
+}
+
+// access: top-down descent (tree/wavelet_tree.cuh:1815-1877)
+__device__ __forceinline__ uint64_t tree_access(const TreeView& t,
+                                                const LevelView* lv,
+                                                const uint64_t* counts,
+                                                const uint64_t* ranks,
+                                                const uint64_t* nal,
+                                                uint64_t index) {
+  uint64_t lo = 0, hi = t.sigma - 1, rs = 0;
+  for (uint32_t l = 0; l < t.num_levels; ++l) {
+    const LevelView v = lv[l];
+    const uint64_t cc = counts[lo];
+    if (hi - lo < 2) {
+      if (hi - lo > 0 && level_bit(v, cc + index)) lo++;
+      break;
+    }
+    const uint64_t start = lo == 0 ? 0 : ranks[rs + tree_node_pos(t, lo, l)];
+    bool bit;
+    const uint64_t pos = level_rank<0>(v, cc + index, &bit);
+    const uint64_t diff = prev_pow2(hi - lo + 1);
+    if (!bit) {
+      index = pos - start;
+      hi = lo + diff - 1;
+    } else {
+      index -= pos - start;
+      lo += diff;
+    }
+    rs += tree_nodes_at(t, nal, l);
+  }
+  return lo;
+}
+
+// rank: rankKernel early-out + top-down descent (wavelet_tree.cuh:481-486,
+// 1896-1956)
+__device__ __forceinline__ uint64_t tree_rank(const TreeView& t,
+                                              const LevelView* lv,
+                                              const uint64_t* counts,
+                                              const uint64_t* ranks,
+                                              const uint64_t* nal,
+                                              uint64_t symbol,
+                                              uint64_t index) {
+  if (symbol >= t.sigma || tree_total(t, counts, symbol) == 0) return 0;
+  uint64_t lo = 0, hi = t.sigma - 1, rs = 0, result = index;
+  for (uint32_t l = 0; l < t.num_levels; ++l) {
+    if (hi == lo) break;
+    const uint64_t cc = counts[lo];
+    const uint64_t start = lo == 0 ? 0 : ranks[rs + tree_node_pos(t, lo, l)];
+    const uint64_t pos = level_rank<0>(lv[l], cc + result);
+    const uint64_t split = lo + prev_pow2(hi - lo + 1);
+    if (symbol < split) {
+      result = pos - start;
+      hi = split - 1;
+    } else {
+      result -= pos - start;
+      lo = split;
+    }
+    if (l + 1 < t.num_levels) rs += tree_nodes_at(t, nal, l);
+  }
+  return result;
+}
+
+// getPrevCharStart (wavelet_tree.cuh:1970-1992)
+__device__ __forceinline__ uint64_t tree_prev_char_start(
+    const TreeView& t, uint64_t cs, bool rightmost, uint32_t level,
+    uint32_t code_len) {
+  if (t.pow2) {
+    const uint64_t node_len = 1ull << (t.num_levels - level);
+    return cs & ~(node_len - 1);
+  }
+  if (rightmost) {
+    uint64_t s = 0;
+    for (uint32_t l = 0; l < level; ++l) s += prev_pow2(t.sigma - s);
+    return s;
+  }
+  return cs - (1ull << (code_len - 1 - level));
+}
+
+// select: selectKernel early-out + bottom-up ascent (wavelet_tree.cuh:
+// 532-535, 2007-2079)
+__device__ __forceinline__ uint64_t tree_select(const TreeView& t,
+                                                const LevelView* lv,
+                                                const uint64_t* counts,
+                                                const uint64_t* ranks,
+                                                const uint64_t* nal,
+                                                uint64_t symbol, uint64_t k) {
+  if (symbol >= t.sigma || k == 0 || k > tree_total(t, counts, symbol))
+    return t.n;
+  const uint32_t L = t.num_levels;
+  uint64_t code = symbol;
+  uint32_t clen = L;
+  if (!t.pow2 && symbol >= t.codes_start) {
+    code = t.codes[symbol - t.codes_start];
+    clen = t.code_lens[symbol - t.codes_start];
+  }
+  uint64_t cs = symbol, result = k;
+  uint64_t rs = t.nodes_until_last_level;
+  int32_t l = static_cast<int32_t>(clen) - 1;
+  for (int32_t lv2 = static_cast<int32_t>(L) - 2; lv2 >= l; --lv2)
+    rs -= tree_nodes_at(t, nal, static_cast<uint32_t>(lv2));
+  for (; l >= 0; --l) {
+    const LevelView v = lv[l];
+    uint64_t start;
+    if ((code >> (L - 1 - l)) & 1) {
+      const bool rightmost =
+          static_cast<uint32_t>(__popcll(shr64(code, L - l))) ==
+          static_cast<uint32_t>(l);
+      cs = tree_prev_char_start(t, cs, rightmost, static_cast<uint32_t>(l),
+                                clen);
+      start = cs == 0 ? 0
+                      : counts[cs] - ranks[rs + tree_node_pos(
+                                                    t, cs, static_cast<uint32_t>(l))];
+      result = level_select<1>(v, start + result) + 1;
+    } else {
+      start = cs == 0
+                  ? 0
+                  : ranks[rs + tree_node_pos(t, cs, static_cast<uint32_t>(l))];
+      result = level_select<0>(v, start + result) + 1;
+    }
+    if (static_cast<uint32_t>(l) == clen - 1 && result > v.size) {
+      result = t.n + 1;
+      break;
+    }
+    result -= counts[cs];
+    if (l > 1) rs -= tree_nodes_at(t, nal, static_cast<uint32_t>(l - 1));
+  }
+  return result - 1;
+}
+
+}  // namespace detail
+}  // namespace ecl

@@ -1,0 +1,269 @@
+/*
+ * ecl_wavelet_c.h -- C ABI of libecl_wavelet_hip.so, the MI355X (gfx950)
+ * wavelet-tree / rank-select library.
+ *
+ * Plain pointers and sizes only.  Every entry point returns an ecl_status;
+ * ecl_last_error() gives the message of the calling thread's last failure.
+ * The out-of-memory messages are the reference's exact strings, so C++
+ * wrappers can rethrow them unchanged.
+ *
+ * Each declaration names the reference (mfranzreb/ECL_wavelet) interface it
+ * replaces.  Entry points marked "new" have no reference counterpart.
+ */
+#ifndef ECL_WAVELET_C_H
+#define ECL_WAVELET_C_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ECL_WAVELET_ABI_VERSION 1
+
+typedef enum ecl_status {
+  ECL_OK = 0,
+  ECL_ERROR_INVALID_ARGUMENT = 1,
+  /* "Not enough memory available for the wavelet tree."
+   * (tree/wavelet_tree.cuh:652-653, 686-687) */
+  ECL_ERROR_OUT_OF_MEMORY_TREE = 2,
+  /* "Not enough GPU memory available for the {access,rank,select} queries."
+   * (tree/wavelet_tree.cuh:972-973, 1268-1269, 1587-1588) */
+  ECL_ERROR_OUT_OF_MEMORY_QUERIES = 3,
+  ECL_ERROR_HIP = 4,
+  ECL_ERROR_NO_DEVICE = 5,
+  ECL_ERROR_INTERNAL = 6
+} ecl_status;
+
+/* element type of a text: sizeof(T) of WaveletTree<T> */
+typedef enum ecl_type { ECL_U8 = 1, ECL_U16 = 2, ECL_U32 = 4, ECL_U64 = 8 } ecl_type;
+
+/* query flags */
+#define ECL_SYMBOLS_MINIMAL 1u /* query symbols are already minimal-alphabet
+                                  indices (as after the reference's in-place
+                                  remap, wavelet_tree.cuh:1426-1436) */
+#define ECL_RESULTS_MINIMAL 2u /* access returns minimal-alphabet indices
+                                  (deviceAccess semantics) */
+
+typedef struct ecl_bit_array_t* ecl_bit_array;
+typedef struct ecl_rank_select_t* ecl_rank_select;
+typedef struct ecl_wavelet_tree_t* ecl_wavelet_tree;
+
+/* RankSelectQuery<T> (utils/utils.cuh:70-74): 16 bytes for every T, the
+ * symbol occupies the first sizeof(T) bytes of `symbol`. */
+typedef struct ecl_rank_select_query {
+  uint64_t index;
+  uint64_t symbol;
+} ecl_rank_select_query;
+
+/* new: one query of a mixed batch.  kind 0 = access(arg), 1 = rank(symbol,
+ * arg), 2 = select(symbol, arg); symbol is a minimal-alphabet index. */
+typedef struct ecl_query {
+  uint64_t arg;
+  uint32_t symbol;
+  uint32_t kind;
+} ecl_query;
+
+typedef struct ecl_rank_select_info {
+  uint64_t size_bits;     /* BitArray::size (bit_array.cuh:274-283) */
+  uint64_t offset_words;  /* BitArray::getOffset (bit_array.cuh:160-164) */
+  uint64_t num_l1_blocks; /* getNumL1Blocks (rank_select.cuh:530-535) */
+  uint64_t num_l2_blocks; /* getNumL2Blocks (rank_select.cuh:542-550) */
+  uint64_t num_last_l2_blocks; /* getNumLastL2Blocks (:557-561) */
+  uint64_t total_ones;    /* getTotalNumVals<1> (:569-578) */
+  uint64_t total_zeros;   /* getTotalNumVals<0> */
+  uint64_t num_samples_1; /* floor(ones / 4096) */
+  uint64_t num_samples_0; /* floor(zeros / 4096) */
+} ecl_rank_select_info;
+
+typedef struct ecl_wavelet_tree_info {
+  uint64_t data_size;       /* rank_select_.bit_array_.size(0) */
+  uint64_t alphabet_size;   /* getAlphabetSize (wavelet_tree.cuh:2165) */
+  uint32_t num_levels;      /* getNumLevels (:2181) */
+  uint32_t is_min_alphabet; /* isMinAlphabet (:2220) */
+  uint64_t codes_start;     /* getCodesStart (:2228) */
+  uint64_t num_ranks;       /* number of precomputed node ranks */
+  uint64_t nodes_until_last_level;
+  uint32_t element_size;    /* sizeof(T) */
+  uint32_t is_pow_two;
+} ecl_wavelet_tree_info;
+
+/* ---- library / device helpers (new) ------------------------------------ */
+const char* ecl_last_error(void);
+int ecl_abi_version(void);
+ecl_status ecl_device_count(int* count);
+ecl_status ecl_device_alloc(int device, size_t bytes, void** ptr);
+ecl_status ecl_device_free(int device, void* ptr);
+ecl_status ecl_memcpy_h2d(int device, void* dst, const void* src, size_t bytes);
+ecl_status ecl_memcpy_d2h(int device, void* dst, const void* src, size_t bytes);
+ecl_status ecl_device_synchronize(int device);
+ecl_status ecl_device_mem_info(int device, size_t* free_bytes, size_t* total_bytes);
+ecl_status ecl_device_name(int device, char* buf, size_t len);
+/* events on `stream` (NULL = default stream); elapsed milliseconds */
+ecl_status ecl_event_create(void** event);
+ecl_status ecl_event_destroy(void* event);
+ecl_status ecl_event_record(void* event, void* stream);
+ecl_status ecl_event_elapsed(void* start, void* stop, float* ms);
+/* per-kernel timing: events around every launch of the library's kernels
+ * while enabled; query by kernel name ("wt_level", "rs_build", ...) */
+ecl_status ecl_profile_enable(int on);
+ecl_status ecl_profile_reset(void);
+ecl_status ecl_profile_get(const char* kernel, uint64_t* launches, double* total_ms);
+/* deterministic synthetic inputs on the device: text[i] = bounded(hash(seed,
+ * i), sigma), or the inverse CDF of `thresholds` (host array of sigma u64)
+ * when non-NULL; bits i.i.d. with P(1) = threshold / 2^64 */
+ecl_status ecl_generate_text(int device, ecl_type type, uint64_t seed, size_t n,
+                             uint64_t sigma, const uint64_t* thresholds, void* d_out);
+
+/* ---- BitArray (bitarray/bit_array.cuh, src/bitarray/bit_array.cu) ------- */
+/* BitArray(sizes) / BitArray(sizes, init_value) (bit_array.cu:40-92);
+ * init_value: -1 = uninitialised payload, 0 / 1 = every word set. */
+ecl_status ecl_bit_array_create(const size_t* sizes, size_t num_arrays, int init_value,
+                                int device, ecl_bit_array* out);
+/* ~BitArray (bit_array.cu:127-133) */
+ecl_status ecl_bit_array_destroy(ecl_bit_array ba);
+/* numArrays / size / getOffset / sizeInWords (bit_array.cuh:160-320) */
+ecl_status ecl_bit_array_num_arrays(ecl_bit_array ba, size_t* out);
+ecl_status ecl_bit_array_size(ecl_bit_array ba, size_t array_index, size_t* bits);
+ecl_status ecl_bit_array_offset(ecl_bit_array ba, size_t array_index, size_t* word_offset);
+/* device pointers of d_data_, d_offsets_, d_bit_sizes_ (bit_array.cuh:336-345) */
+ecl_status ecl_bit_array_device_pointers(ecl_bit_array ba, uint32_t** data,
+                                         uint64_t** offsets, uint64_t** sizes);
+/* writeWord / word over a host range (bit_array.cuh:117-220) */
+ecl_status ecl_bit_array_write_words(ecl_bit_array ba, size_t array_index, size_t first_word,
+                                     const uint32_t* words, size_t num_words);
+ecl_status ecl_bit_array_read_words(ecl_bit_array ba, size_t array_index, size_t first_word,
+                                    uint32_t* words, size_t num_words);
+/* new: fill an array on the device with i.i.d. bits (hash(seed, i) < threshold) */
+ecl_status ecl_bit_array_fill_random(ecl_bit_array ba, size_t array_index, uint64_t seed,
+                                     uint64_t threshold);
+/* BitArray::getNeededGPUMemory (bit_array.cu:140-151) */
+size_t ecl_bit_array_needed_gpu_memory(size_t size, uint8_t num_arrays);
+
+/* ---- RankSelect (bitarray/rank_select.cuh, src/bitarray/rank_select.cu) - */
+/* RankSelect(BitArray&&, GPU_index) (rank_select.cu:50-304): takes
+ * ownership of *ba and sets it to NULL. */
+ecl_status ecl_rank_select_create(ecl_bit_array* ba, int device, ecl_rank_select* out);
+ecl_status ecl_rank_select_destroy(ecl_rank_select rs);
+/* the public member bit_array_ (rank_select.cuh:81), borrowed */
+ecl_status ecl_rank_select_bit_array(ecl_rank_select rs, ecl_bit_array* out);
+ecl_status ecl_rank_select_info_get(ecl_rank_select rs, size_t array_index,
+                                    ecl_rank_select_info* out);
+/* getL1Entry / getL2Entry / getSelectSample over a range
+ * (rank_select.cuh:569-701) */
+ecl_status ecl_rank_select_get_l1(ecl_rank_select rs, size_t array_index, size_t first,
+                                  size_t count, uint64_t* out);
+ecl_status ecl_rank_select_get_l2(ecl_rank_select rs, size_t array_index, size_t first,
+                                  size_t count, uint16_t* out);
+ecl_status ecl_rank_select_get_samples(ecl_rank_select rs, size_t array_index, int value,
+                                       size_t first, size_t count, uint64_t* out);
+/* Device pointers of the reference-shaped directory, for the header-only
+ * device API (rank_select.cuh:827-853): l1, l2, l1_offsets, l2_offsets,
+ * num_l1_blocks, num_last_l2_blocks (u16), samples, sample_offsets,
+ * sample_caps, total_ones, and the per-array LevelView table. */
+typedef struct ecl_rank_select_device {
+  uint64_t* l1; uint16_t* l2;
+  uint64_t* l1_offsets; uint64_t* l2_offsets;
+  uint64_t* num_l1_blocks; uint16_t* num_last_l2_blocks;
+  uint64_t* samples; uint64_t* sample_offsets; uint64_t* sample_caps;
+  uint64_t* total_ones;
+  void* levels;
+  uint64_t total_num_l2_blocks;
+} ecl_rank_select_device;
+ecl_status ecl_rank_select_device_pointers(ecl_rank_select rs, ecl_rank_select_device* out);
+/* batched rank<1,false,value>(array, i) / select<value,1>(array, k) over
+ * device arrays (rank_select.cuh:125-523 semantics) -- new batch form of the
+ * reference's per-thread device calls */
+ecl_status ecl_rank_select_rank_device(ecl_rank_select rs, size_t array_index, int value,
+                                       const uint64_t* d_index, size_t m, uint64_t* d_out,
+                                       void* stream);
+ecl_status ecl_rank_select_select_device(ecl_rank_select rs, size_t array_index, int value,
+                                         const uint64_t* d_k, size_t m, uint64_t* d_out,
+                                         void* stream);
+/* host arrays in and out */
+ecl_status ecl_rank_select_rank(ecl_rank_select rs, size_t array_index, int value,
+                                const uint64_t* index, size_t m, uint64_t* out);
+ecl_status ecl_rank_select_select(ecl_rank_select rs, size_t array_index, int value,
+                                  const uint64_t* k, size_t m, uint64_t* out);
+/* new: queries drawn on the device: rank i = bounded(hash(seed, q), size),
+ * select k = 1 + bounded(hash(seed, q), total of `value`) */
+ecl_status ecl_rank_select_rank_generated(ecl_rank_select rs, size_t array_index, int value,
+                                          uint64_t seed, size_t m, uint64_t* d_out,
+                                          void* stream);
+ecl_status ecl_rank_select_select_generated(ecl_rank_select rs, size_t array_index,
+                                            int value, uint64_t seed, size_t m,
+                                            uint64_t* d_out, void* stream);
+/* RankSelect::getNeededGPUMemory (rank_select.cu:382-409) */
+size_t ecl_rank_select_needed_gpu_memory(size_t size, uint8_t num_arrays);
+
+/* ---- WaveletTree<T> (tree/wavelet_tree.cuh) ----------------------------- */
+/* WaveletTree(data, data_size, alphabet&&, GPU_index) (wavelet_tree.cuh:
+ * 576-867).  alphabet_size == 0 discovers the alphabet; otherwise the
+ * alphabet must be sorted and hold at least 2 symbols. */
+ecl_status ecl_wavelet_tree_create(ecl_type type, const void* text, size_t n,
+                                   const void* alphabet, size_t alphabet_size, int device,
+                                   ecl_wavelet_tree* out);
+/* new: the text is already in device memory (read only, not modified) */
+ecl_status ecl_wavelet_tree_create_device(ecl_type type, const void* d_text, size_t n,
+                                          const void* alphabet, size_t alphabet_size,
+                                          int device, ecl_wavelet_tree* out);
+/* ~WaveletTree (wavelet_tree.cuh:931-943) */
+ecl_status ecl_wavelet_tree_destroy(ecl_wavelet_tree wt);
+ecl_status ecl_wavelet_tree_info_get(ecl_wavelet_tree wt, ecl_wavelet_tree_info* out);
+/* getAlphabet (:2173): alphabet_size elements of type T */
+ecl_status ecl_wavelet_tree_alphabet(ecl_wavelet_tree wt, void* out);
+/* getCounts for every symbol (:2193-2201): exclusive prefix counts */
+ecl_status ecl_wavelet_tree_counts(ecl_wavelet_tree wt, uint64_t* out);
+/* createMinimalCodes result (:2100-2159): codes of symbols codes_start.. */
+ecl_status ecl_wavelet_tree_codes(ecl_wavelet_tree wt, uint64_t* codes, uint8_t* lens);
+/* getPrecomputedRank over all node ranks (:2270-2272) and the node starts
+ * (getNodeInfos :2501-2532) */
+ecl_status ecl_wavelet_tree_node_ranks(ecl_wavelet_tree wt, uint64_t* ranks,
+                                       uint64_t* node_start, uint8_t* node_level);
+/* getNumNodesAtLevel for levels 0..L-2 (:2278-2282) */
+ecl_status ecl_wavelet_tree_nodes_at_level(ecl_wavelet_tree wt, uint64_t* out);
+/* the public member rank_select_ (:561), borrowed */
+ecl_status ecl_wavelet_tree_rank_select(ecl_wavelet_tree wt, ecl_rank_select* out);
+/* device view of the whole tree (detail::TreeView in
+ * include/ecl_wavelet/detail/device_core.hpp) */
+ecl_status ecl_wavelet_tree_device_view(ecl_wavelet_tree wt, void* view, size_t view_bytes);
+/* WaveletTree::getNeededGPUMemory (:2544-2559) */
+size_t ecl_wavelet_tree_needed_gpu_memory(size_t data_size, size_t alphabet_size,
+                                          uint8_t num_levels, size_t num_codes);
+
+/* Host batch queries (access :956-1231, rank :1247-1551, select :1567-1797).
+ * Inputs and outputs are host arrays owned by the caller.  access writes
+ * sizeof(T)-byte symbols (original alphabet unless ECL_RESULTS_MINIMAL);
+ * rank/select read ecl_rank_select_query[] (original-alphabet symbols unless
+ * ECL_SYMBOLS_MINIMAL) and write u64 results. */
+ecl_status ecl_wavelet_tree_access(ecl_wavelet_tree wt, const uint64_t* indices, size_t m,
+                                   void* out, uint32_t flags);
+ecl_status ecl_wavelet_tree_rank(ecl_wavelet_tree wt, const ecl_rank_select_query* queries,
+                                 size_t m, uint64_t* out, uint32_t flags);
+ecl_status ecl_wavelet_tree_select(ecl_wavelet_tree wt, const ecl_rank_select_query* queries,
+                                   size_t m, uint64_t* out, uint32_t flags);
+/* The same over device-resident arrays (accessKernel :373-422, rankKernel
+ * :443-494, selectKernel :510-544), launched on `stream` (NULL = default) */
+ecl_status ecl_wavelet_tree_access_device(ecl_wavelet_tree wt, const uint64_t* d_indices,
+                                          size_t m, void* d_out, uint32_t flags, void* stream);
+ecl_status ecl_wavelet_tree_rank_device(ecl_wavelet_tree wt,
+                                        const ecl_rank_select_query* d_queries, size_t m,
+                                        uint64_t* d_out, uint32_t flags, void* stream);
+ecl_status ecl_wavelet_tree_select_device(ecl_wavelet_tree wt,
+                                          const ecl_rank_select_query* d_queries, size_t m,
+                                          uint64_t* d_out, uint32_t flags, void* stream);
+/* new: one fused launch over a batch mixing the three kinds */
+ecl_status ecl_wavelet_tree_query_batch_device(ecl_wavelet_tree wt, const ecl_query* d_queries,
+                                               size_t m, uint64_t* d_out, void* stream);
+/* new: uniform queries drawn on the device (kind 0: u64 indices in [0,n);
+ * kind 1: ecl_rank_select_query with c uniform over the alphabet and i in
+ * [0,n); kind 2: c uniform, k uniform in [1, count(c)] (1 when count is 0);
+ * kind 3: ecl_query mixing the three kinds 1:1:1).  Symbols are minimal. */
+ecl_status ecl_wavelet_tree_generate_queries(ecl_wavelet_tree wt, int kind, uint64_t seed,
+                                             size_t m, void* d_out, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ECL_WAVELET_C_H */

@@ -1,0 +1,57 @@
+"""The C-ABI library loads and exports every symbol the header declares.
+No compute call: runs without a GPU."""
+import ctypes
+import os
+
+import pytest
+
+from ecl_wavelet_amd import _lib
+
+
+def test_library_built():
+    assert os.path.exists(_lib.LIB_PATH), "run __graft_entry__.build() first"
+
+
+def test_exports_every_declared_symbol():
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    declared = _lib.header_symbols()
+    assert len(declared) > 50
+    missing = [s for s in declared if not hasattr(lib, s)]
+    assert missing == []
+
+
+def test_binding_covers_header():
+    assert sorted(_lib._SIGS) == _lib.header_symbols()
+
+
+def test_abi_version_and_no_device_error():
+    lib = _lib.load()
+    assert lib.ecl_abi_version() == 1
+    n = ctypes.c_int(-1)
+    assert lib.ecl_device_count(ctypes.byref(n)) == 0
+    if n.value == 0:
+        h = ctypes.c_void_p()
+        import numpy as np
+        sizes = np.array([100], dtype=np.uint64)
+        rc = lib.ecl_bit_array_create(sizes.ctypes.data_as(ctypes.c_void_p), 1, 0, 0,
+                                      ctypes.byref(h))
+        assert rc == _lib.ECL_ERROR_NO_DEVICE
+        assert b"no HIP device" in lib.ecl_last_error()
+
+
+def test_needed_memory_formulas_monotone():
+    lib = _lib.load()
+    a = lib.ecl_wavelet_tree_needed_gpu_memory(10**6, 256, 8, 0)
+    b = lib.ecl_wavelet_tree_needed_gpu_memory(10**7, 256, 8, 0)
+    assert b > a > 0
+    assert lib.ecl_rank_select_needed_gpu_memory(2**20, 4) > 0
+    assert lib.ecl_bit_array_needed_gpu_memory(2**20, 4) >= 4 * 2**17
+
+
+@pytest.mark.parametrize("name", ["ecl_wavelet_tree_access", "ecl_rank_select_select"])
+def test_null_handle_is_invalid_argument(name):
+    lib = _lib.load()
+    f = getattr(lib, name)
+    rc = f(None, None, 0, None, 0) if name == "ecl_wavelet_tree_access" else f(
+        None, 0, 1, None, 0, None)
+    assert rc == _lib.ECL_ERROR_INVALID_ARGUMENT

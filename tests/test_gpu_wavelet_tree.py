@@ -1,0 +1,298 @@
+"""WaveletTree<T> on the GPU vs the oracle.  Mirrors the reference's
+test_wavelet_tree.cu: structure KATs, codes, node ranks, histogram, and
+randomized access/rank/select against direct scans of the text."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+DT = [np.uint8, np.uint16, np.uint32, np.uint64]
+
+
+def _bits(rs, level):
+    return rs.bit_array_.bits(level)
+
+
+def _check_structure_vs_oracle(wt, sym, sigma):
+    """Every level bit array, level size, count and node rank equals the
+    oracle's tree over the same minimal-alphabet text."""
+    ot = O.OracleTree(sym, sigma)
+    rs = wt.rank_select_
+    sizes = ot.level_sizes()
+    assert wt.getNumLevels() == ot.num_levels
+    for l in range(ot.num_levels):
+        assert rs.bit_array_.size(l) == sizes[l]
+        if sizes[l]:
+            assert np.array_equal(rs.bit_array_.words(l), ot.level_words(l)), l
+    assert np.array_equal(wt.getCounts(), ot.counts())
+    r, s, lv = wt.nodeRanks()
+    os_, olv = ot.node_starts()
+    assert np.array_equal(s, os_) and np.array_equal(lv, olv)
+    assert np.array_equal(r, ot.ranks())
+    assert wt.getCodesStart() == ot.codes_start
+    if ot.num_levels > 1:
+        assert np.array_equal(wt.numNodesAtLevel(), ot.nodes_at_level()) or \
+            O.is_pow2(sigma)
+    return ot
+
+
+def _check_queries(wt, text, sym_of, nq=3000, seed=0):
+    """access/rank/select through the host API vs direct scans (exact)."""
+    rng = np.random.default_rng(seed)
+    n = len(text)
+    alphabet = wt.getAlphabet()
+    idx = rng.integers(0, n, nq).astype(np.uint64)
+    got = wt.access(idx)
+    assert np.array_equal(got, text[idx])
+    # rank: positions in [0, n], symbols drawn from the alphabet
+    ri = rng.integers(0, n + 1, nq).astype(np.uint64)
+    rc = alphabet[rng.integers(0, len(alphabet), nq)]
+    exp = O.scan_rank_batch(text, np.minimum(ri, n), rc.astype(np.uint64))
+    got = wt.rank(wt.make_queries(ri, rc))
+    assert np.array_equal(got, exp)
+    # select: k in [1, count(c) + 1] (one past the last occurrence gives n)
+    sc = alphabet[rng.integers(0, len(alphabet), nq)]
+    hist = {int(a): int(c) for a, c in zip(*np.unique(text, return_counts=True))}
+    k = np.array([int(rng.integers(1, hist.get(int(c), 0) + 2)) for c in sc],
+                 dtype=np.uint64)
+    exp = O.scan_select_batch(text, k, sc.astype(np.uint64))
+    got = wt.select(wt.make_queries(k, sc))
+    assert np.array_equal(got, exp)
+
+
+# --- construction ------------------------------------------------------------
+@pytest.mark.parametrize("dt", DT)
+def test_constructor_small(gpu, dt):
+    from ecl_wavelet_amd import WaveletTree
+    WaveletTree(np.array([1, 2, 3, 4, 5, 6, 7, 8, 9], dtype=dt),
+                np.arange(1, 10, dtype=dt))
+    WaveletTree(np.array([0, 1, 2, 3], dtype=dt), np.arange(4, dtype=dt))
+    WaveletTree(np.array([0, 1, 2], dtype=dt), np.arange(3, dtype=dt))
+
+
+@pytest.mark.parametrize("dt", DT)
+def test_structure_sigma10(gpu, dt):
+    from ecl_wavelet_amd import WaveletTree
+    data = (np.arange(100) % 10).astype(dt)
+    wt = WaveletTree(data, np.arange(10, dtype=dt))
+    rs = wt.rank_select_
+    b = _bits(rs, 0)
+    assert list(b) == [1 if i % 10 >= 8 else 0 for i in range(100)]
+    b = _bits(rs, 1)
+    assert list(b[:80]) == [1 if i % 8 >= 4 else 0 for i in range(80)]
+    assert list(b[80:100]) == [i % 2 for i in range(80, 100)]
+    assert list(_bits(rs, 2)[:80]) == [1 if i % 4 >= 2 else 0 for i in range(80)]
+    assert list(_bits(rs, 3)[:80]) == [i % 2 for i in range(80)]
+
+
+@pytest.mark.parametrize("dt", DT)
+def test_structure_sigma5(gpu, dt):
+    from ecl_wavelet_amd import WaveletTree
+    data = np.array([4, 1, 1, 2, 1, 1, 3, 4, 4, 0, 0, 2, 3, 2, 4, 2, 4, 1, 3, 2], dtype=dt)
+    wt = WaveletTree(data, np.arange(5, dtype=dt))
+    rs = wt.rank_select_
+    assert "".join(map(str, _bits(rs, 0))) == "10000001100000101000"
+    assert "".join(map(str, _bits(rs, 1))) == "001001001111011"
+    assert "".join(map(str, _bits(rs, 2))) == "111100101010010"
+
+
+@pytest.mark.parametrize("sigma", [72, 63, 75, 20, 5, 10, 256, 1000])
+def test_codes_vs_oracle(gpu, sigma):
+    from ecl_wavelet_amd import WaveletTree
+    dt = np.uint8 if sigma <= 256 else np.uint16
+    wt = WaveletTree((np.arange(sigma * 2) % sigma).astype(dt), np.arange(sigma, dtype=dt))
+    code, ln, total = O.minimal_codes(sigma)
+    codes, lens = wt.codes()
+    assert wt.getCodesStart() == sigma - total
+    assert np.array_equal(codes, code[sigma - total:])
+    assert np.array_equal(lens, ln[sigma - total:])
+
+
+@pytest.mark.parametrize("sigma", [3, 4, 5, 10, 17, 64, 75, 100, 255, 256])
+def test_structure_random_vs_oracle(gpu, sigma):
+    from ecl_wavelet_amd import WaveletTree
+    rng = np.random.default_rng(sigma)
+    n = int(rng.integers(1000, 300000))
+    text = rng.integers(0, sigma, n).astype(np.uint8)
+    wt = WaveletTree(text, np.arange(sigma, dtype=np.uint8))
+    _check_structure_vs_oracle(wt, text, sigma)
+
+
+def test_histogram_kats(gpu):
+    """TestGlobalHistogram (test_wavelet_tree.cu:261-379): counts of four
+    fixed texts over the alphabet 0..9."""
+    from ecl_wavelet_amd import WaveletTree
+    alphabet = np.arange(10, dtype=np.uint8)
+    texts = [np.arange(1000) % 10]
+    t = np.zeros(1000, dtype=np.int64)
+    t[:10] = np.arange(10)
+    t[10:] = 9
+    texts.append(t)
+    t = np.arange(1000) % 8
+    t[:5] = 9
+    t[5:15] = 8
+    texts.append(t)
+    for text in texts:
+        text = text.astype(np.uint8)
+        wt = WaveletTree(text, alphabet)
+        hist = np.bincount(text, minlength=10)
+        exp = np.concatenate([[0], np.cumsum(hist)[:-1]]).astype(np.uint64)
+        assert np.array_equal(wt.getCounts(), exp)
+        for c in range(10):
+            assert wt.getTotalAppearances(c) == hist[c]
+
+
+# --- queries -----------------------------------------------------------------
+@pytest.mark.parametrize("dt", DT)
+def test_queries_kat_mod10(gpu, dt):
+    from ecl_wavelet_amd import WaveletTree
+    data = (np.arange(100) % 10).astype(dt)
+    wt = WaveletTree(data, np.arange(10, dtype=dt))
+    for i in range(1, 100):  # every prefix length as a batch (reference loop)
+        idx = np.arange(i, dtype=np.uint64)
+        assert np.array_equal(wt.access(idx), data[:i])
+    j = np.arange(100)
+    q = wt.make_queries(j, data)
+    assert list(wt.rank(q)) == [x // 10 for x in j]
+    q = wt.make_queries(j // 10 + 1, data)
+    assert list(wt.select(q)) == list(j)
+    assert list(wt.select(wt.make_queries([11], [0]))) == [100]
+
+
+def test_queries_kat_mod256(gpu):
+    from ecl_wavelet_amd import WaveletTree
+    data = (np.arange(1280) % 256).astype(np.uint8)
+    wt = WaveletTree(data, np.arange(256, dtype=np.uint8))
+    j = np.arange(1280)
+    assert np.array_equal(wt.access(j), data)
+    assert list(wt.rank(wt.make_queries(j, data))) == list(j // 256)
+    assert list(wt.select(wt.make_queries(j // 256 + 1, data))) == list(j)
+    assert list(wt.select(wt.make_queries([11], [0]))) == [1280]
+
+
+@pytest.mark.parametrize("sigma", [2, 3, 4, 5, 7, 16, 20, 32, 64, 100, 128, 200, 256])
+def test_queries_random_u8(gpu, sigma):
+    from ecl_wavelet_amd import WaveletTree
+    rng = np.random.default_rng(1000 + sigma)
+    n = int(rng.integers(10**5, 3 * 10**6))
+    text = rng.integers(0, sigma, n).astype(np.uint8)
+    wt = WaveletTree(text, np.arange(sigma, dtype=np.uint8))
+    _check_queries(wt, text, None, seed=sigma)
+
+
+@pytest.mark.parametrize("sigma", [300, 4097, 65536])
+def test_queries_random_u16(gpu, sigma):
+    from ecl_wavelet_amd import WaveletTree
+    rng = np.random.default_rng(2000 + sigma)
+    n = 2 * 10**6
+    text = rng.integers(0, sigma, n).astype(np.uint16)
+    wt = WaveletTree(text, np.arange(sigma, dtype=np.uint16))
+    _check_queries(wt, text, None, nq=2000, seed=sigma)
+
+
+def test_queries_zipf_u16_full_alphabet(gpu):
+    """sigma = 2^16 with a Zipf(1.1) text: many tiny and empty nodes."""
+    from ecl_wavelet_amd import WaveletTree
+    thr = O.zipf_thresholds(65536, 1.1)
+    text = O.gen_cdf(8, 3 * 10**6, thr, np.uint16)
+    wt = WaveletTree(text, np.arange(65536, dtype=np.uint16))
+    _check_queries(wt, text, None, nq=2000, seed=3)
+
+
+@pytest.mark.parametrize("dt", DT)
+def test_queries_random_alphabet(gpu, dt):
+    """A random sorted, non-minimal alphabet (generateRandomAlphabet): the
+    tree remaps symbols, access returns original symbols."""
+    from ecl_wavelet_amd import WaveletTree
+    rng = np.random.default_rng(np.dtype(dt).itemsize)
+    info = np.iinfo(dt)
+    for sigma in (3, 50, 200):
+        alphabet = np.unique(rng.integers(0, int(info.max), 3 * sigma, dtype=np.uint64)
+                             .astype(dt))[:sigma]
+        text = alphabet[rng.integers(0, len(alphabet), 200000)]
+        wt = WaveletTree(text, alphabet)
+        assert not wt.isMinAlphabet()
+        assert np.array_equal(wt.getAlphabet(), alphabet)
+        _check_queries(wt, text, None, nq=1500, seed=sigma)
+
+
+@pytest.mark.parametrize("dt", DT)
+def test_alphabet_discovery(gpu, dt):
+    """genAlphabetRandom: an empty alphabet is discovered from the text."""
+    from ecl_wavelet_amd import WaveletTree
+    rng = np.random.default_rng(5)
+    sigma = 1000 if np.dtype(dt).itemsize > 1 else 200
+    alphabet = np.arange(sigma, dtype=dt)
+    text = np.concatenate([alphabet[rng.integers(0, sigma, 100000)], alphabet])
+    wt = WaveletTree(text, None)
+    assert np.array_equal(wt.getAlphabet(), alphabet)
+    _check_queries(wt, text, None, nq=500, seed=1)
+
+
+def test_device_resident_build_and_queries(gpu):
+    """Text generated on the device (the bench's path) vs the oracle's copy of
+    the same text; device-resident query kernels and the mixed batch."""
+    from ecl_wavelet_amd import DeviceBuffer, WaveletTree, generate_text
+    n, sigma = 5 * 10**6 + 17, 256
+    d_text = generate_text(n, sigma, np.uint8, seed=4)
+    text = O.gen_uniform(4, n, sigma, np.uint8)
+    assert np.array_equal(d_text.download(np.uint8, n), text)
+    wt = WaveletTree(d_text, np.arange(sigma, dtype=np.uint8), dtype=np.uint8, n=n)
+    _check_structure_vs_oracle(wt, text, sigma) if n < 10**6 else None
+    m = 30000
+    d_q = DeviceBuffer(m * 16)
+    d_out = DeviceBuffer(m * 8)
+    # access
+    wt.generateQueries(0, 9, m, d_q)
+    idx = d_q.download(np.uint64, m)
+    wt.accessDevice(d_q, m, d_out)
+    assert np.array_equal(d_out.download(np.uint8, m), text[idx])
+    # rank
+    wt.generateQueries(1, 10, m, d_q)
+    q = d_q.download(np.dtype([("index", "<u8"), ("symbol", "<u8")]), m)
+    wt.rankDevice(d_q, m, d_out)
+    exp = O.scan_rank_batch(text, q["index"], q["symbol"] & 0xFF)
+    assert np.array_equal(d_out.download(np.uint64, m), exp)
+    # select
+    wt.generateQueries(2, 11, m, d_q)
+    q = d_q.download(np.dtype([("index", "<u8"), ("symbol", "<u8")]), m)
+    wt.selectDevice(d_q, m, d_out)
+    exp = O.scan_select_batch(text, q["index"], q["symbol"] & 0xFF)
+    assert np.array_equal(d_out.download(np.uint64, m), exp)
+    # mixed 1:1:1 in one launch == the three single-kind answers
+    from ecl_wavelet_amd import MIXED_DTYPE
+    d_mq = DeviceBuffer(m * 16)
+    wt.generateQueries(3, 12, m, d_mq)
+    mq = d_mq.download(MIXED_DTYPE, m)
+    wt.queryBatchDevice(d_mq, m, d_out)
+    got = d_out.download(np.uint64, m)
+    kinds = mq["kind"]
+    a = mq["arg"]
+    s = mq["symbol"].astype(np.uint64)
+    assert np.array_equal(got[kinds == 0], text[a[kinds == 0]].astype(np.uint64))
+    assert np.array_equal(got[kinds == 1],
+                          O.scan_rank_batch(text, a[kinds == 1], s[kinds == 1]))
+    assert np.array_equal(got[kinds == 2],
+                          O.scan_select_batch(text, a[kinds == 2], s[kinds == 2]))
+
+
+def test_symbol_with_no_occurrence(gpu):
+    from ecl_wavelet_amd import WaveletTree
+    text = np.array([0, 1, 3, 3, 1, 0, 4, 4] * 100, dtype=np.uint8)
+    wt = WaveletTree(text, np.arange(5, dtype=np.uint8))  # symbol 2 absent
+    assert list(wt.rank(wt.make_queries([0, 400, 800], [2, 2, 2]))) == [0, 0, 0]
+    assert list(wt.select(wt.make_queries([1], [2]))) == [800]
+
+
+def test_oom_message(gpu):
+    """Asking for more memory than the device has raises the reference's
+    exact message (wavelet_tree.cuh:652-653)."""
+    from ecl_wavelet_amd import DeviceBuffer, WaveletTree
+    from ecl_wavelet_amd._lib import EclError, ECL_ERROR_OUT_OF_MEMORY_TREE
+    d = DeviceBuffer(1024)
+    with pytest.raises(EclError) as e:
+        WaveletTree(d, np.arange(256, dtype=np.uint8), dtype=np.uint8, n=1 << 42)
+    assert e.value.status == ECL_ERROR_OUT_OF_MEMORY_TREE
+    assert str(e.value) == "Not enough memory available for the wavelet tree."
