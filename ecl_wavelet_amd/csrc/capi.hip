@@ -603,7 +603,8 @@ static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_
       if (padded > 2 * nw64) launch_fill_u32(words + 2 * nw64, padded - 2 * nw64, 0, s);
       if (nl == 0) continue;
       const bool last = l + 1 == L;
-      if (!last) launch_fill_u64(status.as<uint64_t>(), level_tiles(nl) + 2, 0, s);
+      // the tile counter (and look-back state) must start from zero every pass
+      launch_fill_u64(status.as<uint64_t>(), level_tiles(nl) + 2, 0, s);
       LevelParams lp{};
       lp.src = src;
       lp.dst = last ? nullptr : dst;
