@@ -64,5 +64,25 @@ def build_library(verbose=False):
     return LIB
 
 
+DROPIN_TEST_SRC = os.path.join(ROOT, "tests", "cpp", "test_dropin.cpp")
+DROPIN_TEST_BIN = os.path.join(ROOT, "tests", "cpp", "test_dropin")
+
+
+def build_dropin_test():
+    """The C++ drop-in test program (tests/cpp/test_dropin.cpp), linked
+    against the in-tree library with an rpath to it."""
+    cmd = [hipcc(), "-O2", "-std=c++20", f"--offload-arch={ARCH}",
+           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+           "-I", os.path.join(ROOT, "include"), DROPIN_TEST_SRC,
+           "-o", DROPIN_TEST_BIN + ".tmp", "-L", PKG, "-lecl_wavelet_hip",
+           "-Wl,-rpath,$ORIGIN/../../ecl_wavelet_amd"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for test_dropin.cpp:\n{r.stdout}\n{r.stderr}")
+    os.replace(DROPIN_TEST_BIN + ".tmp", DROPIN_TEST_BIN)
+    return DROPIN_TEST_BIN
+
+
 if __name__ == "__main__":
     print(build_library(verbose=True))
+    print(build_dropin_test())

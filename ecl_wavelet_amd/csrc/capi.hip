@@ -233,7 +233,10 @@ static ecl_bit_array_t* make_bit_array(const std::vector<uint64_t>& sizes, int i
 // ===========================================================================
 // RankSelect
 // ===========================================================================
-static ecl_rank_select_t* make_rank_select(ecl_bit_array_t* ba, hipStream_t s) {
+// Directory allocation for every array of `ba` (rank_select.cu:57-143
+// sizes); the contents are written by rs_build_kernel or, for a wavelet
+// tree, by the level passes.
+static ecl_rank_select_t* alloc_rank_select(ecl_bit_array_t* ba, hipStream_t s) {
   auto* rs = new ecl_rank_select_t();
   try {
     rs->device = ba->device;
@@ -241,10 +244,9 @@ static ecl_rank_select_t* make_rank_select(ecl_bit_array_t* ba, hipStream_t s) {
     rs->num_l1.resize(na); rs->num_l2.resize(na); rs->num_last_l2.resize(na);
     rs->l1_off.resize(na); rs->l2_off.resize(na); rs->s_off.resize(na);
     rs->cap.resize(na); rs->ones.assign(na, 0);
-    uint64_t t1 = 0, t2 = 0, ts = 0, max_l1 = 0;
+    uint64_t t1 = 0, t2 = 0, ts = 0;
     for (size_t i = 0; i < na; ++i) {
       const uint64_t m = ba->sizes[i];
-      // rank_select.cu:57-117
       rs->num_l1[i] = (m + detail::kL1Bits - 1) / detail::kL1Bits;
       uint64_t last = ((m % detail::kL1Bits) + detail::kL2Bits - 1) / detail::kL2Bits;
       if (last == 0 && m) last = detail::kL2PerL1;
@@ -253,7 +255,6 @@ static ecl_rank_select_t* make_rank_select(ecl_bit_array_t* ba, hipStream_t s) {
       rs->cap[i] = m / detail::kSampleRate;
       rs->l1_off[i] = t1; rs->l2_off[i] = t2; rs->s_off[i] = ts;
       t1 += rs->num_l1[i]; t2 += rs->num_l2[i]; ts += rs->cap[i];
-      max_l1 = std::max(max_l1, rs->num_l1[i]);
     }
     rs->total_l2 = t2;
     rs->l1 = DevBuf(std::max<uint64_t>(t1, 1) * sizeof(uint64_t));
@@ -261,6 +262,56 @@ static ecl_rank_select_t* make_rank_select(ecl_bit_array_t* ba, hipStream_t s) {
     rs->samples = DevBuf(std::max<uint64_t>(ts, 1) * sizeof(uint64_t));
     rs->total_ones = DevBuf(std::max<size_t>(na, 1) * sizeof(uint64_t));
     launch_fill_u64(rs->total_ones.as<uint64_t>(), std::max<size_t>(na, 1), 0, s);
+  } catch (...) {
+    delete rs;
+    throw;
+  }
+  return rs;
+}
+
+// Totals back to the host, reference-shaped device directory and the
+// LevelView table; takes ownership of `ba`.
+static void finalize_rank_select(ecl_rank_select_t* rs, ecl_bit_array_t* ba, hipStream_t s) {
+  const size_t na = ba->sizes.size();
+  ECL_HIP(hipStreamSynchronize(s));
+  d2h(rs->ones.data(), rs->total_ones.ptr, na);
+  auto up = [&](DevBuf& b, const std::vector<uint64_t>& v) {
+    b = DevBuf(std::max<size_t>(v.size(), 1) * sizeof(uint64_t));
+    h2d(b.ptr, v.data(), v.size());
+  };
+  up(rs->d_l1_off, rs->l1_off);
+  up(rs->d_l2_off, rs->l2_off);
+  up(rs->d_num_l1, rs->num_l1);
+  up(rs->d_s_off, rs->s_off);
+  up(rs->d_cap, rs->cap);
+  std::vector<uint16_t> last16(na);
+  for (size_t i = 0; i < na; ++i) last16[i] = static_cast<uint16_t>(rs->num_last_l2[i]);
+  rs->d_num_last_l2 = DevBuf(std::max<size_t>(na, 1) * sizeof(uint16_t));
+  h2d(rs->d_num_last_l2.ptr, last16.data(), na);
+  rs->levels.resize(na);
+  for (size_t i = 0; i < na; ++i) {
+    LevelView& v = rs->levels[i];
+    v.words = ba->words() + ba->offsets[i];
+    v.l1 = rs->l1.as<uint64_t>() + rs->l1_off[i];
+    v.l2 = rs->l2.as<uint16_t>() + rs->l2_off[i];
+    v.samples = rs->samples.as<uint64_t>() + rs->s_off[i];
+    v.size = ba->sizes[i];
+    v.ones = rs->ones[i];
+    v.num_l1 = rs->num_l1[i];
+    v.sample_cap = rs->cap[i];
+  }
+  rs->d_levels = DevBuf(std::max<size_t>(na, 1) * sizeof(LevelView));
+  h2d(rs->d_levels.ptr, rs->levels.data(), na);
+  rs->ba = ba;
+}
+
+// RankSelect(BitArray&&) (rank_select.cu:50-304): one rs_build pass per array.
+static ecl_rank_select_t* make_rank_select(ecl_bit_array_t* ba, hipStream_t s) {
+  ecl_rank_select_t* rs = alloc_rank_select(ba, s);
+  try {
+    const size_t na = ba->sizes.size();
+    uint64_t max_l1 = 0;
+    for (size_t i = 0; i < na; ++i) max_l1 = std::max(max_l1, rs->num_l1[i]);
     DevBuf status((max_l1 + 2) * sizeof(uint64_t));
     for (size_t i = 0; i < na; ++i) {
       if (ba->sizes[i] == 0) continue;
@@ -279,37 +330,7 @@ static ecl_rank_select_t* make_rank_select(ecl_bit_array_t* ba, hipStream_t s) {
       p.counter = status.as<uint64_t>() + rs->num_l1[i] + 1;
       launch_rs_build(p, s);
     }
-    ECL_HIP(hipStreamSynchronize(s));
-    d2h(rs->ones.data(), rs->total_ones.ptr, na);
-    // reference-shaped device directory + LevelView table
-    auto up = [&](DevBuf& b, const std::vector<uint64_t>& v) {
-      b = DevBuf(std::max<size_t>(v.size(), 1) * sizeof(uint64_t));
-      h2d(b.ptr, v.data(), v.size());
-    };
-    up(rs->d_l1_off, rs->l1_off);
-    up(rs->d_l2_off, rs->l2_off);
-    up(rs->d_num_l1, rs->num_l1);
-    up(rs->d_s_off, rs->s_off);
-    up(rs->d_cap, rs->cap);
-    std::vector<uint16_t> last16(na);
-    for (size_t i = 0; i < na; ++i) last16[i] = static_cast<uint16_t>(rs->num_last_l2[i]);
-    rs->d_num_last_l2 = DevBuf(std::max<size_t>(na, 1) * sizeof(uint16_t));
-    h2d(rs->d_num_last_l2.ptr, last16.data(), na);
-    rs->levels.resize(na);
-    for (size_t i = 0; i < na; ++i) {
-      LevelView& v = rs->levels[i];
-      v.words = ba->words() + ba->offsets[i];
-      v.l1 = rs->l1.as<uint64_t>() + rs->l1_off[i];
-      v.l2 = rs->l2.as<uint16_t>() + rs->l2_off[i];
-      v.samples = rs->samples.as<uint64_t>() + rs->s_off[i];
-      v.size = ba->sizes[i];
-      v.ones = rs->ones[i];
-      v.num_l1 = rs->num_l1[i];
-      v.sample_cap = rs->cap[i];
-    }
-    rs->d_levels = DevBuf(std::max<size_t>(na, 1) * sizeof(LevelView));
-    h2d(rs->d_levels.ptr, rs->levels.data(), na);
-    rs->ba = ba;  // ownership transferred last
+    finalize_rank_select(rs, ba, s);
   } catch (...) {
     rs->ba = nullptr;
     delete rs;
@@ -479,9 +500,15 @@ static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_
     const uint64_t needed = tree_needed_bytes(n, sigma, L, esize, buffers);
     if (needed > free_device_memory()) throw Error(ECL_ERROR_OUT_OF_MEMORY_TREE, kOomTree);
 
-    // ---- text on the device and the two level buffers
+    // ---- text on the device and the two level buffers; a device text that
+    // is not 16-byte aligned is copied once (the passes use 16 B vectors)
     DevBuf own_text, buf_y;
     const void* d_text = text;
+    if (text_on_device && (reinterpret_cast<uintptr_t>(text) & 15) != 0) {
+      own_text = DevBuf(n * esize);
+      ECL_HIP(hipMemcpy(own_text.ptr, text, n * esize, hipMemcpyDeviceToDevice));
+      d_text = own_text.ptr;
+    }
     if (!text_on_device) {
       own_text = DevBuf(n * esize);
       h2d(own_text.ptr, static_cast<const uint8_t*>(text), n * esize);
@@ -580,6 +607,8 @@ static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_
     // ---- bit arrays and the level passes
     ecl_bit_array_t* ba = make_bit_array(level_size, -1, device);
     std::unique_ptr<ecl_bit_array_t> ba_guard(ba);
+    ecl_rank_select_t* rs = alloc_rank_select(ba, s);
+    std::unique_ptr<ecl_rank_select_t> rs_guard(rs);
     uint64_t max_tiles = 0;
     for (uint32_t l = 0; l < L; ++l) max_tiles = std::max(max_tiles, level_tiles(level_size[l]));
     DevBuf status((max_tiles + 2) * 8);
@@ -617,6 +646,13 @@ static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_
       lp.num_prefixes = last ? 0 : (1ull << l);
       lp.words64 = reinterpret_cast<uint64_t*>(words);
       lp.nwords64 = nw64;
+      lp.l1 = rs->l1.as<uint64_t>() + rs->l1_off[l];
+      lp.l2 = rs->l2.as<uint16_t>() + rs->l2_off[l];
+      lp.num_l1 = rs->num_l1[l];
+      lp.num_l2 = rs->num_l2[l];
+      lp.samples = rs->samples.as<uint64_t>() + rs->s_off[l];
+      lp.cap = rs->cap[l];
+      lp.total_ones = rs->total_ones.as<uint64_t>() + l;
       lp.status = status.as<uint64_t>();
       lp.counter = status.as<uint64_t>() + level_tiles(nl) + 1;
       launch_level(lp, s);
@@ -633,9 +669,10 @@ static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_
     buf_y.reset();
     status.reset();
 
-    // ---- rank/select directory over every level (RankSelect ctor)
+    // ---- the level passes wrote the rank/select directory of every level
+    finalize_rank_select(rs, ba, s);
     ba_guard.release();
-    wt->rs = make_rank_select(ba, s);
+    wt->rs = rs_guard.release();
 
     // ---- device metadata and view
     wt->d_counts = DevBuf(sigma * 8);

@@ -72,8 +72,11 @@ struct HistParams {
 };
 void launch_histogram(const HistParams& p, hipStream_t s);
 
+// One level pass: bits of the level, its rank/select directory (L1, L2,
+// select samples, total ones) and, unless it is the last level, the stable
+// partition of the symbols into the next level's order.
 struct LevelParams {
-  const void* src; void* dst; int esize;
+  const void* src; void* dst; int esize;   // dst == nullptr: last level
   uint64_t n, n_next;
   uint32_t bit;            // code bit of this level (L-1-l)
   uint32_t pshift;         // code >> pshift = node prefix (L-l)
@@ -81,6 +84,9 @@ struct LevelParams {
   uint64_t num_prefixes;   // 2^l
   uint64_t* words64;       // level bit words (u64 view)
   uint64_t nwords64;       // u64 words this level writes
+  uint64_t* l1; uint16_t* l2; uint64_t num_l1, num_l2;
+  uint64_t* samples; uint64_t cap;
+  uint64_t* total_ones;
   uint64_t* status; uint64_t* counter;  // lookback state (zeroed)
 };
 uint64_t level_tiles(uint64_t n);

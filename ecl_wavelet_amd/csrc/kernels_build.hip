@@ -39,10 +39,14 @@ __device__ __forceinline__ uint64_t status_load(const uint64_t* p) {
 }
 
 // Decoupled look-back, run by all 64 lanes of one wave.  Publishes this
-// tile's aggregate, walks back 64 predecessors per step until it meets an
-// inclusive prefix, publishes its own inclusive prefix, returns the
-// exclusive one.  Tiles get their ids in launch order from an atomic
-// counter, so every predecessor is resident or finished: the wait ends.
+// tile's aggregate, then reads the status of up to 64*J predecessors per
+// step (lane i covers predecessors 8i+1 .. 8i+J nearest-first) until it
+// meets an inclusive prefix, publishes its own inclusive prefix and returns
+// the exclusive one.  Publication can only advance by one window per
+// status-load latency, so the window is wide and the tiles are large.
+// Tiles take their ids in launch order from an atomic counter: every
+// predecessor is resident or finished, so the wait ends.
+template <int J>
 __device__ uint64_t lookback(uint64_t* status, uint64_t tile, uint64_t agg) {
   const uint32_t lane = lane_id();
   if (tile == 0) {
@@ -53,23 +57,38 @@ __device__ uint64_t lookback(uint64_t* status, uint64_t tile, uint64_t agg) {
   uint64_t excl = 0;
   int64_t top = static_cast<int64_t>(tile) - 1;
   while (true) {
-    const int64_t idx = top - static_cast<int64_t>(lane);
-    const uint64_t v = idx >= 0 ? status_load(&status[idx]) : kFlagPre;
-    const uint64_t pre = ballot64((v >> 62) == 2);
-    const uint64_t empty = ballot64((v >> 62) == 0);
-    const uint32_t stop = pre ? static_cast<uint32_t>(__builtin_ctzll(pre)) : 64u;
-    const uint64_t window = stop >= 63 ? ~0ull : ((2ull << stop) - 1);
-    if (empty & window) {
+    uint64_t v[J];
+#pragma unroll
+    for (int j = 0; j < J; ++j) {
+      const int64_t idx = top - static_cast<int64_t>(lane) * J - j;
+      v[j] = idx >= 0 ? status_load(&status[idx]) : kFlagPre;
+    }
+    int first_pre = J;
+    bool empty = false;
+#pragma unroll
+    for (int j = J - 1; j >= 0; --j)
+      if ((v[j] >> 62) == 2) first_pre = j;
+#pragma unroll
+    for (int j = 0; j < J; ++j)
+      if (j <= first_pre && (v[j] >> 62) == 0) empty = true;
+    const uint64_t pre_mask = ballot64(first_pre < J);
+    const uint32_t stop = pre_mask ? static_cast<uint32_t>(__builtin_ctzll(pre_mask)) : 64u;
+    if (ballot64(lane <= stop && empty)) {
       __builtin_amdgcn_s_sleep(1);
       continue;
     }
-    excl += detail::wave_sum_u64(lane <= stop ? (v & kValMask) : 0ull);
-    if (pre) break;
-    top -= 64;
+    uint64_t sum = 0;
+#pragma unroll
+    for (int j = 0; j < J; ++j)
+      if (lane < stop || (lane == stop && j <= first_pre)) sum += v[j] & kValMask;
+    excl += detail::wave_sum_u64(sum);
+    if (pre_mask) break;
+    top -= 64 * J;
   }
   if (lane == 0) status_store(&status[tile], kFlagPre | (excl + agg));
   return excl;
 }
+constexpr int kLookbackWords = 8;  // window of 512 tiles
 
 template <typename T>
 __device__ __forceinline__ uint64_t load_elem(const T* p, uint64_t i) {
@@ -112,9 +131,7 @@ __global__ __launch_bounds__(256) void histogram_kernel(HistParams p,
   const T* text = static_cast<const T*>(p.text);
   T* out = static_cast<T*>(p.codes_out);
   const T* alpha = static_cast<const T*>(p.alphabet);
-  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * 256u;
-  for (uint64_t i = blockIdx.x * 256ull + tid; i < p.n; i += stride) {
-    const uint64_t x = static_cast<uint64_t>(text[i]);
+  auto count = [&](uint64_t x) -> T {
     uint64_t c;
     if constexpr (MapKind == 0) {
       c = x;
@@ -134,11 +151,24 @@ __global__ __launch_bounds__(256) void histogram_kernel(HistParams p,
     } else {
       atomicAdd(reinterpret_cast<unsigned long long*>(&p.hist[c]), 1ull);
     }
-    if (out != nullptr) {
-      const uint64_t code =
-          (p.pow2 || c < p.codes_start) ? c : p.codes[c - p.codes_start];
-      out[i] = static_cast<T>(code);
-    }
+    return static_cast<T>((p.pow2 || c < p.codes_start) ? c : p.codes[c - p.codes_start]);
+  };
+  // 16-byte vectors: E symbols per lane per step (text is 16 B aligned)
+  constexpr uint32_t E = 16 / sizeof(T);
+  const uint64_t nvec = p.n / E;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * 256u;
+  const uint4* tv = reinterpret_cast<const uint4*>(text);
+  uint4* ov = reinterpret_cast<uint4*>(out);
+  for (uint64_t i = blockIdx.x * 256ull + tid; i < nvec; i += stride) {
+    union { uint4 v; T e[E]; } in, res;
+    in.v = tv[i];
+#pragma unroll
+    for (uint32_t k = 0; k < E; ++k) res.e[k] = count(static_cast<uint64_t>(in.e[k]));
+    if (out != nullptr) ov[i] = res.v;
+  }
+  for (uint64_t i = nvec * E + blockIdx.x * 256ull + tid; i < p.n; i += stride) {
+    const T code = count(static_cast<uint64_t>(text[i]));
+    if (out != nullptr) out[i] = code;
   }
   if constexpr (UseLds) {
     __syncthreads();
@@ -153,51 +183,57 @@ __global__ __launch_bounds__(256) void histogram_kernel(HistParams p,
 }
 
 // ---------------------------------------------------------------------------
-// Level pass.  Block = 4 waves; wave w owns K consecutive 64-symbol slots
-// of the tile; lane t of slot k holds symbol tile_base + (w*K + k)*64 + t,
-// so one ballot per slot is exactly one u64 word of the level's bits and
-// the scatter of a slot writes two contiguous runs per node.
-constexpr int kLevelSlots = 16;
-constexpr int kLevelTile = 4 * kLevelSlots * 64;
+// Level pass.  One workgroup (4 waves) per tile of 65536 symbols = one L1
+// block of the level's bit array; wave w owns 256 consecutive 64-symbol
+// slots and lane t of slot k holds symbol tile + w*16384 + k*64 + t, so one
+// ballot per slot is exactly one u64 word of the level's bits.
+//  phase 1: load, write the bit words, count ones (L2 entries are in-tile
+//           prefixes, every 8 slots);
+//  look-back on the tile's ones count -> L1 entry of the block;
+//  phase 2: reload the tile (it is still in the L2 / Infinity Cache), emit
+//           select samples where a multiple of 4096 ones / zeros falls, and
+//           scatter every symbol to dest = base(node, bit) + rank_bit(pos).
+constexpr uint32_t kTileBits = 65536;
+constexpr uint32_t kWaveSlots = 256;
+constexpr int kUnroll = 16;
 
 template <typename T>
 __global__ __launch_bounds__(256) void level_kernel(LevelParams p) {
   __shared__ uint64_t s_tile;
   __shared__ uint64_t s_excl;
   __shared__ uint32_t s_wave[4];
+  __shared__ uint32_t s_l2[4][32];
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & 63, wave = tid >> 6;
   if (tid == 0)
     s_tile = atomicAdd(reinterpret_cast<unsigned long long*>(p.counter), 1ull);
   __syncthreads();
   const uint64_t tile = s_tile;
-  const uint64_t wbase =
-      tile * kLevelTile + static_cast<uint64_t>(wave) * (kLevelSlots * 64);
+  const uint64_t wbase = tile * kTileBits + static_cast<uint64_t>(wave) * (kWaveSlots * 64);
   const T* src = static_cast<const T*>(p.src);
 
-  uint64_t v[kLevelSlots];
-  uint64_t m[kLevelSlots];
+  // ---- phase 1
   uint32_t ones = 0;
+  for (uint32_t k0 = 0; k0 < kWaveSlots; k0 += kUnroll) {
+    if (wbase + k0 * 64 >= p.n) break;  // uniform per wave
+    uint64_t m[kUnroll];
 #pragma unroll
-  for (int k = 0; k < kLevelSlots; ++k) {
-    const uint64_t idx = wbase + k * 64 + lane;
-    const bool valid = idx < p.n;
-    const uint64_t x = valid ? load_elem(src, idx) : 0ull;
-    v[k] = x;
-    m[k] = ballot64(valid && ((x >> p.bit) & 1ull));
-    ones += static_cast<uint32_t>(__popcll(m[k]));
+    for (int u = 0; u < kUnroll; ++u) {
+      const uint64_t idx = wbase + (k0 + u) * 64 + lane;
+      const bool valid = idx < p.n;
+      const uint64_t x = valid ? load_elem(src, idx) : 0ull;
+      m[u] = ballot64(valid && ((x >> p.bit) & 1ull));
+    }
+    uint64_t my = 0;
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      if ((u & 7) == 0 && lane == 0) s_l2[wave][(k0 + u) >> 3] = ones;
+      ones += static_cast<uint32_t>(__popcll(m[u]));
+      if (lane == static_cast<uint32_t>(u)) my = m[u];
+    }
+    const uint64_t widx = wbase / 64 + k0 + lane;
+    if (lane < static_cast<uint32_t>(kUnroll) && widx < p.nwords64) p.words64[widx] = my;
   }
-  // bits: lane k of the wave stores the word of slot k
-  uint64_t my = 0;
-#pragma unroll
-  for (int k = 0; k < kLevelSlots; ++k)
-    if (lane == static_cast<uint32_t>(k)) my = m[k];
-  const uint64_t widx = wbase / 64 + lane;
-  if (lane < static_cast<uint32_t>(kLevelSlots) && widx < p.nwords64)
-    p.words64[widx] = my;
-
-  if (p.dst == nullptr) return;  // last level: nothing to partition
-
   if (lane == 0) s_wave[wave] = ones;
   __syncthreads();
   uint32_t wexcl = 0, ttot = 0;
@@ -207,27 +243,72 @@ __global__ __launch_bounds__(256) void level_kernel(LevelParams p) {
     wexcl += (w < wave) ? c : 0u;
     ttot += c;
   }
+  if (tid < 128) {  // L2 entries of this L1 block
+    const uint64_t k = tile * 128 + tid;
+    if (k < p.num_l2) {
+      uint32_t before = 0;
+      for (uint32_t w = 0; w < (tid >> 5); ++w) before += s_wave[w];
+      p.l2[k] = static_cast<uint16_t>(before + s_l2[tid >> 5][tid & 31]);
+    }
+  }
   if (wave == 0) {
-    const uint64_t g = lookback(p.status, tile, ttot);
+    const uint64_t g = lookback<kLookbackWords>(p.status, tile, ttot);
     if (lane == 0) s_excl = g;
   }
   __syncthreads();
-  uint64_t run = s_excl + wexcl;  // ones before this wave's first symbol
+  const uint64_t G = s_excl;
+  if (tid == 0) {
+    if (tile < p.num_l1) p.l1[tile] = G;
+    if (tile == p.num_l1 - 1) *p.total_ones = G + ttot;
+  }
+
+  // ---- phase 2
+  uint64_t run = G + wexcl;  // ones before this wave's first symbol
   T* dst = static_cast<T*>(p.dst);
+  for (uint32_t k0 = 0; k0 < kWaveSlots; k0 += kUnroll) {
+    if (wbase + k0 * 64 >= p.n) break;
+    T v[kUnroll];
+    uint64_t m[kUnroll];
 #pragma unroll
-  for (int k = 0; k < kLevelSlots; ++k) {
-    const uint64_t idx = wbase + k * 64 + lane;
-    const bool b = (m[k] >> lane) & 1ull;
-    const uint64_t r1 = run + mbcnt64(m[k]);
-    const uint64_t r0 = idx - r1;
-    const uint64_t pfx = detail::shr64(v[k], p.pshift);
-    if (idx < p.n && pfx < p.num_prefixes) {
-      const uint64_t base = p.table[2 * pfx + (b ? 1 : 0)];
-      const uint64_t dest = base + (b ? r1 : r0);
-      if (base != detail::kSkip && dest < p.n_next)
-        dst[dest] = static_cast<T>(v[k]);
+    for (int u = 0; u < kUnroll; ++u) {
+      const uint64_t idx = wbase + (k0 + u) * 64 + lane;
+      const bool valid = idx < p.n;
+      v[u] = valid ? src[idx] : T(0);
+      m[u] = ballot64(valid && ((static_cast<uint64_t>(v[u]) >> p.bit) & 1ull));
     }
-    run += static_cast<uint64_t>(__popcll(m[k]));
+#pragma unroll
+    for (int u = 0; u < kUnroll; ++u) {
+      const uint64_t s0 = wbase + (k0 + u) * 64;  // first position of the slot
+      if (s0 >= p.n) break;  // uniform: the rest of the group is past the end
+      const uint32_t c = static_cast<uint32_t>(__popcll(m[u]));
+      if (p.cap) {
+        const uint64_t k1 = (run + c) / detail::kSampleRate;
+        if (k1 > run / detail::kSampleRate && lane == 0)
+          p.samples[k1 - 1] =
+              s0 + detail::select_in_word(m[u], static_cast<uint32_t>(k1 * detail::kSampleRate - run));
+        const uint64_t valid_n = (p.n - s0 < 64) ? p.n - s0 : 64;
+        const uint64_t zb = s0 - run;  // zeros before the slot
+        const uint64_t zc = valid_n - c;
+        const uint64_t k0z = (zb + zc) / detail::kSampleRate;
+        if (k0z > zb / detail::kSampleRate && lane == 0) {
+          const uint64_t zm = ~m[u] & (valid_n == 64 ? ~0ull : ((1ull << valid_n) - 1));
+          p.samples[p.cap - k0z] =
+              s0 + detail::select_in_word(zm, static_cast<uint32_t>(k0z * detail::kSampleRate - zb));
+        }
+      }
+      if (dst != nullptr) {
+        const uint64_t idx = s0 + lane;
+        const bool b = (m[u] >> lane) & 1ull;
+        const uint64_t r1 = run + mbcnt64(m[u]);
+        const uint64_t pfx = detail::shr64(static_cast<uint64_t>(v[u]), p.pshift);
+        if (idx < p.n && pfx < p.num_prefixes) {
+          const uint64_t base = p.table[2 * pfx + (b ? 1 : 0)];
+          const uint64_t dest = base + (b ? r1 : idx - r1);
+          if (base != detail::kSkip && dest < p.n_next) dst[dest] = v[u];
+        }
+      }
+      run += c;
+    }
   }
 }
 
@@ -280,7 +361,7 @@ __global__ __launch_bounds__(256) void rs_build_kernel(RsBuildParams p) {
     if (k < p.num_l2) p.l2[k] = static_cast<uint16_t>(excl);
   }
   if (wave == 0) {
-    const uint64_t g = lookback(p.status, tile, ttot);
+    const uint64_t g = lookback<kLookbackWords>(p.status, tile, ttot);
     if (lane == 0) s_excl = g;
   }
   __syncthreads();
@@ -419,7 +500,7 @@ void launch_histogram(const HistParams& p, hipStream_t s) {
   const bool lds = copies > 0;
   // each block handles at most 2^31 symbols so u32 bins cannot overflow
   uint64_t min_blocks = (p.n >> 31) + 1;
-  uint32_t grid = grid_for(p.n, 256 * 16, 8192);
+  uint32_t grid = grid_for(p.n / (16 / p.esize), 256 * 4, 8192);
   if (grid < min_blocks) grid = static_cast<uint32_t>(min_blocks);
   const size_t shmem = lds ? 4ull * copies * p.sigma * sizeof(uint32_t) : 0;
 #define ECL_HIST_LAUNCH(TT, MK)                                                   \
@@ -448,12 +529,13 @@ void launch_histogram(const HistParams& p, hipStream_t s) {
   ECL_HIP(hipGetLastError());
 }
 
-uint64_t level_tiles(uint64_t n) { return (n + kLevelTile - 1) / kLevelTile; }
+uint64_t level_tiles(uint64_t n) { return (n + kTileBits - 1) / kTileBits; }
 
 void launch_level(const LevelParams& p, hipStream_t s) {
   if (p.n == 0) return;
   const uint64_t tiles = level_tiles(p.n);
   if (tiles > 0x7FFFFFFFull) throw Error(ECL_ERROR_INVALID_ARGUMENT, "level too large");
+  if (tiles != p.num_l1) throw Error(ECL_ERROR_INTERNAL, "level tiles != L1 blocks");
   ProfScope ps(K_LEVEL, s);
   switch (p.esize) {
     case 1: hipLaunchKernelGGL(level_kernel<uint8_t>, dim3(tiles), dim3(256), 0, s, p); break;
