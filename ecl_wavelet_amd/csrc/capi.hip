@@ -175,6 +175,46 @@ struct ecl_rank_select_t {
   ~ecl_rank_select_t() { delete ba; }
 };
 
+// Host-issued batches: two default-priority streams, pinned staging and
+// device buffers per stream, grown on demand and kept with the handle.
+struct HostPipe {
+  hipStream_t stream[2] = {nullptr, nullptr};
+  void* pin_in[2] = {nullptr, nullptr};
+  void* pin_out[2] = {nullptr, nullptr};
+  DevBuf d_in[2], d_out[2];
+  size_t cap_in = 0, cap_out = 0;
+  std::mutex mu;
+  ~HostPipe() {
+    for (int i = 0; i < 2; ++i) {
+      if (stream[i]) (void)hipStreamDestroy(stream[i]);
+      if (pin_in[i]) (void)hipHostFree(pin_in[i]);
+      if (pin_out[i]) (void)hipHostFree(pin_out[i]);
+    }
+  }
+  void reserve(size_t in_bytes, size_t out_bytes) {
+    for (int i = 0; i < 2; ++i)
+      if (!stream[i]) ECL_HIP(hipStreamCreate(&stream[i]));
+    if (in_bytes > cap_in) {
+      for (int i = 0; i < 2; ++i) {
+        if (pin_in[i]) ECL_HIP(hipHostFree(pin_in[i]));
+        pin_in[i] = nullptr;
+        ECL_HIP(hipHostMalloc(&pin_in[i], in_bytes, hipHostMallocDefault));
+        d_in[i] = DevBuf(in_bytes);
+      }
+      cap_in = in_bytes;
+    }
+    if (out_bytes > cap_out) {
+      for (int i = 0; i < 2; ++i) {
+        if (pin_out[i]) ECL_HIP(hipHostFree(pin_out[i]));
+        pin_out[i] = nullptr;
+        ECL_HIP(hipHostMalloc(&pin_out[i], out_bytes, hipHostMallocDefault));
+        d_out[i] = DevBuf(out_bytes);
+      }
+      cap_out = out_bytes;
+    }
+  }
+};
+
 struct ecl_wavelet_tree_t {
   int device = 0;
   int esize = 1;
@@ -193,6 +233,7 @@ struct ecl_wavelet_tree_t {
   DevBuf d_counts, d_ranks, d_nal, d_codes, d_code_lens, d_alphabet, d_lut;
   int map_kind = 0;
   TreeView view{};
+  std::unique_ptr<HostPipe> pipe{new HostPipe()};
   ~ecl_wavelet_tree_t() { delete rs; }
 };
 
@@ -1195,38 +1236,74 @@ size_t ecl_wavelet_tree_needed_gpu_memory(size_t n, size_t sigma, uint8_t L, siz
   return tree_needed_bytes(n, sigma, L, 8, 0);
 }
 
+// Chunked host batch (the reference's graph of H2D -> kernel -> D2H per
+// chunk, wavelet_tree.cuh:94-151, 1121-1211): chunk i uses stream i % 2; the
+// host copies chunk i+1 into pinned memory while chunk i runs, and copies
+// the results of chunk i-2 out before reusing its buffers.
+}  // extern "C"
+
+template <typename Launch>
+static void pipelined(ecl_wavelet_tree wt, const void* in, size_t in_elem, void* out,
+                      size_t out_elem, size_t m, const char* kind, Launch&& launch) {
+  HostPipe& P = *wt->pipe;
+  std::lock_guard<std::mutex> lk(P.mu);
+  const size_t chunk = std::min<size_t>(m, kHostChunk / 4);
+  const size_t nchunks = (m + chunk - 1) / chunk;
+  check_query_memory(2 * chunk * (in_elem + out_elem), kind);
+  P.reserve(chunk * in_elem, chunk * out_elem);
+  size_t pending_off[2] = {0, 0}, pending_len[2] = {0, 0};
+  auto drain = [&](int b) {
+    if (!pending_len[b]) return;
+    ECL_HIP(hipStreamSynchronize(P.stream[b]));
+    std::memcpy(static_cast<uint8_t*>(out) + pending_off[b] * out_elem, P.pin_out[b],
+                pending_len[b] * out_elem);
+    pending_len[b] = 0;
+  };
+  for (size_t c = 0; c < nchunks; ++c) {
+    const int b = static_cast<int>(c & 1);
+    const size_t off = c * chunk, len = std::min(chunk, m - off);
+    drain(b);
+    std::memcpy(P.pin_in[b], static_cast<const uint8_t*>(in) + off * in_elem, len * in_elem);
+    ECL_HIP(hipMemcpyAsync(P.d_in[b].ptr, P.pin_in[b], len * in_elem, hipMemcpyHostToDevice,
+                           P.stream[b]));
+    launch(P.d_in[b].ptr, len, P.d_out[b].ptr, P.stream[b]);
+    ECL_HIP(hipMemcpyAsync(P.pin_out[b], P.d_out[b].ptr, len * out_elem,
+                           hipMemcpyDeviceToHost, P.stream[b]));
+    pending_off[b] = off;
+    pending_len[b] = len;
+  }
+  drain(0);
+  drain(1);
+}
+
+extern "C" {
+
 ecl_status ecl_wavelet_tree_access(ecl_wavelet_tree wt, const uint64_t* idx, size_t m,
                                    void* out, uint32_t flags) {
   return guard([&] {
     require(wt && (m == 0 || (idx && out)), "NULL argument");
+    if (m == 0) return;
     set_device(wt->device);
     const TreeQueryParams p = query_params(wt, flags);
-    for (size_t off = 0; off < m; off += kHostChunk) {
-      const size_t c = std::min<size_t>(kHostChunk, m - off);
-      check_query_memory(c * (8 + wt->esize), "access");
-      DevBuf din(c * 8), dout(c * wt->esize);
-      h2d(din.ptr, idx + off, c);
-      launch_tree_access(p, din.as<uint64_t>(), c, dout.ptr, nullptr);
-      d2h(static_cast<uint8_t*>(out) + off * wt->esize, dout.ptr, c * wt->esize);
-    }
+    pipelined(wt, idx, 8, out, wt->esize, m, "access",
+              [&](void* din, size_t len, void* dout, hipStream_t st) {
+                launch_tree_access(p, static_cast<uint64_t*>(din), len, dout, st);
+              });
   });
 }
 static void tree_rs_host(ecl_wavelet_tree wt, const ecl_rank_select_query* q, size_t m,
                          uint64_t* out, uint32_t flags, bool is_rank) {
   require(wt && (m == 0 || (q && out)), "NULL argument");
+  if (m == 0) return;
   set_device(wt->device);
   const TreeQueryParams p = query_params(wt, flags);
-  for (size_t off = 0; off < m; off += kHostChunk) {
-    const size_t c = std::min<size_t>(kHostChunk, m - off);
-    check_query_memory(c * 24, is_rank ? "rank" : "select");
-    DevBuf din(c * 16), dout(c * 8);
-    h2d(din.ptr, q + off, c);
-    if (is_rank)
-      launch_tree_rank(p, din.ptr, c, dout.as<uint64_t>(), nullptr);
-    else
-      launch_tree_select(p, din.ptr, c, dout.as<uint64_t>(), nullptr);
-    d2h(out + off, dout.ptr, c);
-  }
+  pipelined(wt, q, 16, out, 8, m, is_rank ? "rank" : "select",
+            [&](void* din, size_t len, void* dout, hipStream_t st) {
+              if (is_rank)
+                launch_tree_rank(p, din, len, static_cast<uint64_t*>(dout), st);
+              else
+                launch_tree_select(p, din, len, static_cast<uint64_t*>(dout), st);
+            });
 }
 ecl_status ecl_wavelet_tree_rank(ecl_wavelet_tree wt, const ecl_rank_select_query* q,
                                  size_t m, uint64_t* out, uint32_t flags) {
