@@ -651,8 +651,9 @@ static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_
     ecl_rank_select_t* rs = alloc_rank_select(ba, s);
     std::unique_ptr<ecl_rank_select_t> rs_guard(rs);
     uint64_t max_tiles = 0;
-    for (uint32_t l = 0; l < L; ++l) max_tiles = std::max(max_tiles, level_tiles(level_size[l]));
-    DevBuf status((max_tiles + 2) * 8);
+    for (uint32_t l = 0; l < L; ++l)
+      max_tiles = std::max(max_tiles, level_tiles(level_size[l], esize));
+    DevBuf status((2 * max_tiles + 2) * 8);  // status, aggregates, counter
     // all node tables in one upload; level l uses 2^(l+1) entries
     std::vector<uint64_t> table_off(L, 0);
     std::vector<uint64_t> all_tables;
@@ -674,7 +675,8 @@ static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_
       if (nl == 0) continue;
       const bool last = l + 1 == L;
       // the tile counter (and look-back state) must start from zero every pass
-      launch_fill_u64(status.as<uint64_t>(), level_tiles(nl) + 2, 0, s);
+      const uint64_t tiles = level_tiles(nl, esize);
+      launch_fill_u64(status.as<uint64_t>(), 2 * tiles + 2, 0, s);
       LevelParams lp{};
       lp.src = src;
       lp.dst = last ? nullptr : dst;
@@ -695,7 +697,7 @@ static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_
       lp.cap = rs->cap[l];
       lp.total_ones = rs->total_ones.as<uint64_t>() + l;
       lp.status = status.as<uint64_t>();
-      lp.counter = status.as<uint64_t>() + level_tiles(nl) + 1;
+      lp.counter = status.as<uint64_t>() + 2 * tiles + 1;
       launch_level(lp, s);
       if (!last) {
         src = dst;

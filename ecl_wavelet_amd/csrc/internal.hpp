@@ -89,7 +89,8 @@ struct LevelParams {
   uint64_t* total_ones;
   uint64_t* status; uint64_t* counter;  // lookback state (zeroed)
 };
-uint64_t level_tiles(uint64_t n);
+// tiles of a level pass; the look-back state needs 2 * tiles + 2 u64
+uint64_t level_tiles(uint64_t n, int esize);
 void launch_level(const LevelParams& p, hipStream_t s);
 
 struct RsBuildParams {

@@ -85,6 +85,9 @@ __device__ uint64_t lookback(uint64_t* status, uint64_t tile, uint64_t agg) {
     if (pre_mask) break;
     top -= 64 * J;
   }
+  // what this tile read happens-before its inclusive prefix becomes visible,
+  // so a successor that sees the prefix also sees every earlier aggregate
+  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
   if (lane == 0) status_store(&status[tile], kFlagPre | (excl + agg));
   return excl;
 }
@@ -183,131 +186,240 @@ __global__ __launch_bounds__(256) void histogram_kernel(HistParams p,
 }
 
 // ---------------------------------------------------------------------------
-// Level pass.  One workgroup (4 waves) per tile of 65536 symbols = one L1
-// block of the level's bit array; wave w owns 256 consecutive 64-symbol
-// slots and lane t of slot k holds symbol tile + w*16384 + k*64 + t, so one
-// ballot per slot is exactly one u64 word of the level's bits.
-//  phase 1: load, write the bit words, count ones (L2 entries are in-tile
-//           prefixes, every 8 slots);
-//  look-back on the tile's ones count -> L1 entry of the block;
-//  phase 2: reload the tile (it is still in the L2 / Infinity Cache), emit
-//           select samples where a multiple of 4096 ones / zeros falls, and
-//           scatter every symbol to dest = base(node, bit) + rank_bit(pos).
-constexpr uint32_t kTileBits = 65536;
-constexpr uint32_t kWaveSlots = 256;
-constexpr int kUnroll = 16;
+// Level pass.  One workgroup (256 threads) per tile of 32 KiB of symbols
+// (2 tiles per 65536-bit L1 block for u8, 4 for u16, 8 for u32, 16 for u64),
+// staged in LDS so the level is read from HBM once.
+//  phase 1: 16-byte vector loads (chunk c = E symbols), bit mask of every
+//           chunk packed in registers, u64 bit words assembled over the
+//           lanes of one word with xor-shuffles and stored; chunk ones
+//           counts go to LDS and are scanned (in-tile prefixes).
+//  look-back on the tile's ones count -> global ones before the tile; the
+//           first tile of an L1 block writes the L1 entry, every tile writes
+//           its L2 entries relative to the block start.
+//  phase 2: each thread walks its chunks from LDS: select samples where a
+//           multiple of 4096 ones / zeros falls, and the stable one-bit
+//           partition dest = base(node, bit) + rank_bit(position).
+constexpr uint32_t kTileBytes = 32768;
+constexpr int kLevelLookbackWords = 16;  // window of 1024 tiles
+
+template <typename T>
+struct LevelCfg {
+  static constexpr uint32_t E = 16 / sizeof(T);               // symbols per chunk
+  static constexpr uint32_t TILE = kTileBytes / sizeof(T);     // symbols per tile
+  static constexpr uint32_t CHUNKS = TILE / E;                 // 2048
+  static constexpr uint32_t ITERS = CHUNKS / 256;              // 8
+  static constexpr uint32_t TILES_PER_L1 = 65536 / TILE;       // 2..16
+  static constexpr uint32_t LPW = 64 / E;                      // lanes per u64 word
+};
+
+// bit `bit` of each of the E symbols of a 16-byte chunk, symbol k at bit k
+template <typename T>
+__device__ __forceinline__ uint32_t chunk_mask(const uint4& v, uint32_t bit) {
+  const uint32_t d[4] = {v.x, v.y, v.z, v.w};
+  uint32_t m = 0;
+  if constexpr (sizeof(T) == 1) {
+    // four bytes of a dword: isolate the bit of each byte, gather the four
+    // bits into a nibble with one multiply (no carries: the partial
+    // products land on distinct bit positions)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t x = (d[k] >> bit) & 0x01010101u;
+      m |= ((x * 0x01020408u) >> 24 & 0xFu) << (4 * k);
+    }
+  } else if constexpr (sizeof(T) == 2) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      m |= ((d[k] >> bit) & 1u) << (2 * k);
+      m |= ((d[k] >> (16 + bit)) & 1u) << (2 * k + 1);
+    }
+  } else if constexpr (sizeof(T) == 4) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) m |= ((d[k] >> bit) & 1u) << k;
+  } else {
+    const uint64_t a = (static_cast<uint64_t>(v.y) << 32) | v.x;
+    const uint64_t b = (static_cast<uint64_t>(v.w) << 32) | v.z;
+    m = static_cast<uint32_t>((a >> bit) & 1u) | (static_cast<uint32_t>((b >> bit) & 1u) << 1);
+  }
+  return m;
+}
+
+// symbol k of a 16-byte chunk, by shifts (k is a constant after unrolling)
+template <typename T>
+__device__ __forceinline__ T chunk_elem(const uint4& q, uint32_t k) {
+  const uint32_t wi = (k * static_cast<uint32_t>(sizeof(T))) >> 2;
+  const uint32_t d = wi == 0 ? q.x : wi == 1 ? q.y : wi == 2 ? q.z : q.w;
+  if constexpr (sizeof(T) == 1) return static_cast<T>(d >> (8 * (k & 3)));
+  else if constexpr (sizeof(T) == 2) return static_cast<T>(d >> (16 * (k & 1)));
+  else if constexpr (sizeof(T) == 4) return static_cast<T>(d);
+  else return static_cast<T>(k == 0 ? ((static_cast<uint64_t>(q.y) << 32) | q.x)
+                                    : ((static_cast<uint64_t>(q.w) << 32) | q.z));
+}
+// the same chunk assembled from scalar loads of the symbols below n
+template <typename T>
+__device__ __forceinline__ uint4 load_partial_chunk(const T* src, uint64_t s0, uint64_t n) {
+  uint32_t d[4] = {0, 0, 0, 0};
+  constexpr uint32_t E = 16 / sizeof(T);
+#pragma unroll
+  for (uint32_t k = 0; k < E; ++k) {
+    if (s0 + k < n) {
+      const uint64_t x = static_cast<uint64_t>(src[s0 + k]);
+      const uint32_t bitpos = k * 8 * static_cast<uint32_t>(sizeof(T));
+      if constexpr (sizeof(T) == 8) {
+        d[2 * k] = static_cast<uint32_t>(x);
+        d[2 * k + 1] = static_cast<uint32_t>(x >> 32);
+      } else {
+        d[bitpos >> 5] |= static_cast<uint32_t>(x) << (bitpos & 31);
+      }
+    }
+  }
+  return make_uint4(d[0], d[1], d[2], d[3]);
+}
 
 template <typename T>
 __global__ __launch_bounds__(256) void level_kernel(LevelParams p) {
-  __shared__ uint64_t s_tile;
-  __shared__ uint64_t s_excl;
+  using C = LevelCfg<T>;
+  __shared__ uint4 s_tile[C::CHUNKS];        // 32 KiB of symbols
+  __shared__ uint16_t s_pre[C::CHUNKS];      // in-tile exclusive ones prefix per chunk
   __shared__ uint32_t s_wave[4];
-  __shared__ uint32_t s_l2[4][32];
+  __shared__ uint64_t s_tile_id, s_G, s_L1base;
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & 63, wave = tid >> 6;
   if (tid == 0)
-    s_tile = atomicAdd(reinterpret_cast<unsigned long long*>(p.counter), 1ull);
+    s_tile_id = atomicAdd(reinterpret_cast<unsigned long long*>(p.counter), 1ull);
   __syncthreads();
-  const uint64_t tile = s_tile;
-  const uint64_t wbase = tile * kTileBits + static_cast<uint64_t>(wave) * (kWaveSlots * 64);
+  const uint64_t tile = s_tile_id;
+  const uint64_t tbase = tile * C::TILE;  // first symbol of the tile
   const T* src = static_cast<const T*>(p.src);
+  const uint64_t ntiles = (p.n + C::TILE - 1) / C::TILE;
+  uint64_t* agg = p.status + ntiles;      // plain per-tile aggregates
 
-  // ---- phase 1
-  uint32_t ones = 0;
-  for (uint32_t k0 = 0; k0 < kWaveSlots; k0 += kUnroll) {
-    if (wbase + k0 * 64 >= p.n) break;  // uniform per wave
-    uint64_t m[kUnroll];
+  // ---- phase 1: load, stage, bit words, chunk counts
+  uint4 v[C::ITERS];
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-      const uint64_t idx = wbase + (k0 + u) * 64 + lane;
-      const bool valid = idx < p.n;
-      const uint64_t x = valid ? load_elem(src, idx) : 0ull;
-      m[u] = ballot64(valid && ((x >> p.bit) & 1ull));
+  for (uint32_t r = 0; r < C::ITERS; ++r) {
+    const uint64_t s0 = tbase + static_cast<uint64_t>(r * 256 + tid) * C::E;
+    if (s0 + C::E <= p.n) {
+      v[r] = reinterpret_cast<const uint4*>(src + s0)[0];
+    } else {
+      v[r] = load_partial_chunk<T>(src, s0, p.n);
     }
-    uint64_t my = 0;
-#pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-      if ((u & 7) == 0 && lane == 0) s_l2[wave][(k0 + u) >> 3] = ones;
-      ones += static_cast<uint32_t>(__popcll(m[u]));
-      if (lane == static_cast<uint32_t>(u)) my = m[u];
-    }
-    const uint64_t widx = wbase / 64 + k0 + lane;
-    if (lane < static_cast<uint32_t>(kUnroll) && widx < p.nwords64) p.words64[widx] = my;
   }
-  if (lane == 0) s_wave[wave] = ones;
+#pragma unroll
+  for (uint32_t r = 0; r < C::ITERS; ++r) {
+    const uint32_t c = r * 256 + tid;
+    const uint64_t s0 = tbase + static_cast<uint64_t>(c) * C::E;
+    s_tile[c] = v[r];
+    uint32_t m = chunk_mask<T>(v[r], p.bit);
+    if (s0 + C::E > p.n) m &= (s0 >= p.n) ? 0u : ((1u << (p.n - s0)) - 1u);
+    s_pre[c] = static_cast<uint16_t>(__popc(m));
+    uint64_t w = static_cast<uint64_t>(m) << (C::E * (lane % C::LPW));
+#pragma unroll
+    for (uint32_t off = 1; off < C::LPW; off <<= 1) w |= __shfl_xor(w, off, 64);
+    const uint64_t widx = (tbase + static_cast<uint64_t>(c) * C::E) / 64;
+    if (lane % C::LPW == 0 && widx < p.nwords64) p.words64[widx] = w;
+  }
   __syncthreads();
-  uint32_t wexcl = 0, ttot = 0;
+  // in-tile exclusive scan of the chunk counts: thread t owns chunks 8t..8t+7
+  uint32_t cnt[8], sum = 0;
+#pragma unroll
+  for (int k = 0; k < 8; ++k) { cnt[k] = s_pre[tid * 8 + k]; sum += cnt[k]; }
+  const uint32_t incl = detail::wave_incl_scan_u32(sum);
+  if (lane == 63) s_wave[wave] = incl;
+  __syncthreads();
+  uint32_t before = incl - sum, ttot = 0;
 #pragma unroll
   for (uint32_t w = 0; w < 4; ++w) {
-    const uint32_t c = s_wave[w];
-    wexcl += (w < wave) ? c : 0u;
-    ttot += c;
+    before += (w < wave) ? s_wave[w] : 0u;
+    ttot += s_wave[w];
   }
-  if (tid < 128) {  // L2 entries of this L1 block
-    const uint64_t k = tile * 128 + tid;
-    if (k < p.num_l2) {
-      uint32_t before = 0;
-      for (uint32_t w = 0; w < (tid >> 5); ++w) before += s_wave[w];
-      p.l2[k] = static_cast<uint16_t>(before + s_l2[tid >> 5][tid & 31]);
-    }
-  }
+#pragma unroll
+  for (int k = 0; k < 8; ++k) { s_pre[tid * 8 + k] = static_cast<uint16_t>(before); before += cnt[k]; }
+
+  // ---- look-back, L1 and L2 entries
   if (wave == 0) {
-    const uint64_t g = lookback<kLookbackWords>(p.status, tile, ttot);
-    if (lane == 0) s_excl = g;
+    if (lane == 0) __hip_atomic_store(&agg[tile], static_cast<uint64_t>(ttot), __ATOMIC_RELAXED,
+                                      __HIP_MEMORY_SCOPE_AGENT);
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // aggregate before the status
+    const uint64_t g = lookback<kLevelLookbackWords>(p.status, tile, ttot);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    // ones before the L1 block = g minus the tiles of this block before me
+    const uint64_t first = tile - tile % C::TILES_PER_L1;
+    uint64_t mine = 0;
+    if (first + lane < tile)
+      mine = __hip_atomic_load(&agg[first + lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const uint64_t inblock = detail::wave_sum_u64(mine);
+    if (lane == 0) {
+      s_G = g;
+      s_L1base = g - inblock;
+    }
   }
   __syncthreads();
-  const uint64_t G = s_excl;
+  const uint64_t G = s_G, L1base = s_L1base;
   if (tid == 0) {
-    if (tile < p.num_l1) p.l1[tile] = G;
-    if (tile == p.num_l1 - 1) *p.total_ones = G + ttot;
+    if (tile % C::TILES_PER_L1 == 0) {
+      const uint64_t k = tile / C::TILES_PER_L1;
+      if (k < p.num_l1) p.l1[k] = G;
+    }
+    if (tile == ntiles - 1) *p.total_ones = G + ttot;
+  }
+  {  // L2 entries of this tile: blocks of 512 symbols = 512/E chunks
+    constexpr uint32_t kBlocks = C::TILE / 512;
+    if (tid < kBlocks) {
+      const uint64_t k = tbase / 512 + tid;
+      if (k < p.num_l2)
+        p.l2[k] = static_cast<uint16_t>((G - L1base) + s_pre[tid * (512 / C::E)]);
+    }
   }
 
-  // ---- phase 2
-  uint64_t run = G + wexcl;  // ones before this wave's first symbol
+  // ---- phase 2: samples and scatter, one chunk at a time per thread
   T* dst = static_cast<T*>(p.dst);
-  for (uint32_t k0 = 0; k0 < kWaveSlots; k0 += kUnroll) {
-    if (wbase + k0 * 64 >= p.n) break;
-    T v[kUnroll];
-    uint64_t m[kUnroll];
-#pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-      const uint64_t idx = wbase + (k0 + u) * 64 + lane;
-      const bool valid = idx < p.n;
-      v[u] = valid ? src[idx] : T(0);
-      m[u] = ballot64(valid && ((static_cast<uint64_t>(v[u]) >> p.bit) & 1ull));
+  uint64_t cached_pfx = ~0ull, base0 = 0, base1 = 0;
+  for (uint32_t r = 0; r < C::ITERS; ++r) {
+    const uint32_t c = r * 256 + tid;
+    const uint64_t s0 = tbase + static_cast<uint64_t>(c) * C::E;
+    if (s0 >= p.n) break;
+    const uint4 q = s_tile[c];
+    const uint32_t valid = (p.n - s0 < C::E) ? static_cast<uint32_t>(p.n - s0) : C::E;
+    uint32_t m = chunk_mask<T>(q, p.bit);
+    if (valid < C::E) m &= (1u << valid) - 1u;
+    const uint32_t pc = __popc(m);
+    uint64_t r1 = G + s_pre[c];  // ones before the chunk
+    if (p.cap) {
+      const uint64_t k1 = (r1 + pc) / detail::kSampleRate;
+      if (k1 > r1 / detail::kSampleRate)
+        p.samples[k1 - 1] =
+            s0 + detail::select_in_word(m, static_cast<uint32_t>(k1 * detail::kSampleRate - r1));
+      const uint64_t zb = s0 - r1;
+      const uint64_t k0 = (zb + valid - pc) / detail::kSampleRate;
+      if (k0 > zb / detail::kSampleRate) {
+        const uint32_t zm = ~m & ((valid >= 32) ? ~0u : ((1u << valid) - 1u));
+        p.samples[p.cap - k0] =
+            s0 + detail::select_in_word(zm, static_cast<uint32_t>(k0 * detail::kSampleRate - zb));
+      }
     }
+    if (dst != nullptr) {
+      uint64_t r0 = s0 - r1;
 #pragma unroll
-    for (int u = 0; u < kUnroll; ++u) {
-      const uint64_t s0 = wbase + (k0 + u) * 64;  // first position of the slot
-      if (s0 >= p.n) break;  // uniform: the rest of the group is past the end
-      const uint32_t c = static_cast<uint32_t>(__popcll(m[u]));
-      if (p.cap) {
-        const uint64_t k1 = (run + c) / detail::kSampleRate;
-        if (k1 > run / detail::kSampleRate && lane == 0)
-          p.samples[k1 - 1] =
-              s0 + detail::select_in_word(m[u], static_cast<uint32_t>(k1 * detail::kSampleRate - run));
-        const uint64_t valid_n = (p.n - s0 < 64) ? p.n - s0 : 64;
-        const uint64_t zb = s0 - run;  // zeros before the slot
-        const uint64_t zc = valid_n - c;
-        const uint64_t k0z = (zb + zc) / detail::kSampleRate;
-        if (k0z > zb / detail::kSampleRate && lane == 0) {
-          const uint64_t zm = ~m[u] & (valid_n == 64 ? ~0ull : ((1ull << valid_n) - 1));
-          p.samples[p.cap - k0z] =
-              s0 + detail::select_in_word(zm, static_cast<uint32_t>(k0z * detail::kSampleRate - zb));
+      for (uint32_t k = 0; k < C::E; ++k) {
+        if (k >= valid) break;
+        const T x = chunk_elem<T>(q, k);
+        const uint64_t pfx = detail::shr64(static_cast<uint64_t>(x), p.pshift);
+        if (pfx != cached_pfx) {
+          cached_pfx = pfx;
+          if (pfx < p.num_prefixes) {
+            base0 = p.table[2 * pfx];
+            base1 = p.table[2 * pfx + 1];
+          } else {
+            base0 = base1 = detail::kSkip;
+          }
         }
+        const bool b = (m >> k) & 1u;
+        const uint64_t base = b ? base1 : base0;
+        const uint64_t dest = base + (b ? r1 : r0);
+        if (base != detail::kSkip && dest < p.n_next) dst[dest] = x;
+        r1 += b;
+        r0 += !b;
       }
-      if (dst != nullptr) {
-        const uint64_t idx = s0 + lane;
-        const bool b = (m[u] >> lane) & 1ull;
-        const uint64_t r1 = run + mbcnt64(m[u]);
-        const uint64_t pfx = detail::shr64(static_cast<uint64_t>(v[u]), p.pshift);
-        if (idx < p.n && pfx < p.num_prefixes) {
-          const uint64_t base = p.table[2 * pfx + (b ? 1 : 0)];
-          const uint64_t dest = base + (b ? r1 : idx - r1);
-          if (base != detail::kSkip && dest < p.n_next) dst[dest] = v[u];
-        }
-      }
-      run += c;
     }
   }
 }
@@ -529,13 +641,15 @@ void launch_histogram(const HistParams& p, hipStream_t s) {
   ECL_HIP(hipGetLastError());
 }
 
-uint64_t level_tiles(uint64_t n) { return (n + kTileBits - 1) / kTileBits; }
+uint64_t level_tiles(uint64_t n, int esize) {
+  const uint64_t tile = kTileBytes / esize;
+  return (n + tile - 1) / tile;
+}
 
 void launch_level(const LevelParams& p, hipStream_t s) {
   if (p.n == 0) return;
-  const uint64_t tiles = level_tiles(p.n);
+  const uint64_t tiles = level_tiles(p.n, p.esize);
   if (tiles > 0x7FFFFFFFull) throw Error(ECL_ERROR_INVALID_ARGUMENT, "level too large");
-  if (tiles != p.num_l1) throw Error(ECL_ERROR_INTERNAL, "level tiles != L1 blocks");
   ProfScope ps(K_LEVEL, s);
   switch (p.esize) {
     case 1: hipLaunchKernelGGL(level_kernel<uint8_t>, dim3(tiles), dim3(256), 0, s, p); break;
