@@ -653,7 +653,10 @@ static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_
     uint64_t max_tiles = 0;
     for (uint32_t l = 0; l < L; ++l)
       max_tiles = std::max(max_tiles, level_tiles(level_size[l], esize));
-    DevBuf status((2 * max_tiles + 2) * 8);  // status, aggregates, counter
+    DevBuf status((max_tiles + 2) * 8);  // look-back status + tile counter
+    uint64_t max_l2 = 1;
+    for (uint32_t l = 0; l < L; ++l) max_l2 = std::max(max_l2, rs->num_l2[l]);
+    DevBuf l2_global(max_l2 * 8);
     // all node tables in one upload; level l uses 2^(l+1) entries
     std::vector<uint64_t> table_off(L, 0);
     std::vector<uint64_t> all_tables;
@@ -676,7 +679,7 @@ static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_
       const bool last = l + 1 == L;
       // the tile counter (and look-back state) must start from zero every pass
       const uint64_t tiles = level_tiles(nl, esize);
-      launch_fill_u64(status.as<uint64_t>(), 2 * tiles + 2, 0, s);
+      launch_fill_u64(status.as<uint64_t>(), tiles + 2, 0, s);
       LevelParams lp{};
       lp.src = src;
       lp.dst = last ? nullptr : dst;
@@ -697,7 +700,8 @@ static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_
       lp.cap = rs->cap[l];
       lp.total_ones = rs->total_ones.as<uint64_t>() + l;
       lp.status = status.as<uint64_t>();
-      lp.counter = status.as<uint64_t>() + 2 * tiles + 1;
+      lp.counter = status.as<uint64_t>() + tiles + 1;
+      lp.l2_global = l2_global.as<uint64_t>();
       launch_level(lp, s);
       if (!last) {
         src = dst;

@@ -85,6 +85,7 @@ struct LevelParams {
   uint64_t* words64;       // level bit words (u64 view)
   uint64_t nwords64;       // u64 words this level writes
   uint64_t* l1; uint16_t* l2; uint64_t num_l1, num_l2;
+  uint64_t* l2_global;     // scratch: ones before each 512-bit block
   uint64_t* samples; uint64_t cap;
   uint64_t* total_ones;
   uint64_t* status; uint64_t* counter;  // lookback state (zeroed)

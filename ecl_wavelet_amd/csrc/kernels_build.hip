@@ -85,9 +85,6 @@ __device__ uint64_t lookback(uint64_t* status, uint64_t tile, uint64_t agg) {
     if (pre_mask) break;
     top -= 64 * J;
   }
-  // what this tile read happens-before its inclusive prefix becomes visible,
-  // so a successor that sees the prefix also sees every earlier aggregate
-  __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "agent");
   if (lane == 0) status_store(&status[tile], kFlagPre | (excl + agg));
   return excl;
 }
@@ -281,7 +278,7 @@ __global__ __launch_bounds__(256) void level_kernel(LevelParams p) {
   __shared__ uint4 s_tile[C::CHUNKS];        // 32 KiB of symbols
   __shared__ uint16_t s_pre[C::CHUNKS];      // in-tile exclusive ones prefix per chunk
   __shared__ uint32_t s_wave[4];
-  __shared__ uint64_t s_tile_id, s_G, s_L1base;
+  __shared__ uint64_t s_tile_id, s_G;
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & 63, wave = tid >> 6;
   if (tid == 0)
@@ -291,7 +288,6 @@ __global__ __launch_bounds__(256) void level_kernel(LevelParams p) {
   const uint64_t tbase = tile * C::TILE;  // first symbol of the tile
   const T* src = static_cast<const T*>(p.src);
   const uint64_t ntiles = (p.n + C::TILE - 1) / C::TILE;
-  uint64_t* agg = p.status + ntiles;      // plain per-tile aggregates
 
   // ---- phase 1: load, stage, bit words, chunk counts
   uint4 v[C::ITERS];
@@ -335,26 +331,16 @@ __global__ __launch_bounds__(256) void level_kernel(LevelParams p) {
 #pragma unroll
   for (int k = 0; k < 8; ++k) { s_pre[tid * 8 + k] = static_cast<uint16_t>(before); before += cnt[k]; }
 
-  // ---- look-back, L1 and L2 entries
+  // ---- look-back, L1 entry, global prefixes of the L2 blocks.  The look-back
+  // communicates only through its status words (value and flag in one atomic
+  // word), so no fence is needed; L2 entries are made block-relative by a
+  // separate pass (l2_fixup_kernel) instead of reading other tiles' counts.
   if (wave == 0) {
-    if (lane == 0) __hip_atomic_store(&agg[tile], static_cast<uint64_t>(ttot), __ATOMIC_RELAXED,
-                                      __HIP_MEMORY_SCOPE_AGENT);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // aggregate before the status
     const uint64_t g = lookback<kLevelLookbackWords>(p.status, tile, ttot);
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    // ones before the L1 block = g minus the tiles of this block before me
-    const uint64_t first = tile - tile % C::TILES_PER_L1;
-    uint64_t mine = 0;
-    if (first + lane < tile)
-      mine = __hip_atomic_load(&agg[first + lane], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const uint64_t inblock = detail::wave_sum_u64(mine);
-    if (lane == 0) {
-      s_G = g;
-      s_L1base = g - inblock;
-    }
+    if (lane == 0) s_G = g;
   }
   __syncthreads();
-  const uint64_t G = s_G, L1base = s_L1base;
+  const uint64_t G = s_G;
   if (tid == 0) {
     if (tile % C::TILES_PER_L1 == 0) {
       const uint64_t k = tile / C::TILES_PER_L1;
@@ -362,12 +348,11 @@ __global__ __launch_bounds__(256) void level_kernel(LevelParams p) {
     }
     if (tile == ntiles - 1) *p.total_ones = G + ttot;
   }
-  {  // L2 entries of this tile: blocks of 512 symbols = 512/E chunks
+  {  // ones before each 512-symbol block of this tile (512/E chunks apart)
     constexpr uint32_t kBlocks = C::TILE / 512;
     if (tid < kBlocks) {
       const uint64_t k = tbase / 512 + tid;
-      if (k < p.num_l2)
-        p.l2[k] = static_cast<uint16_t>((G - L1base) + s_pre[tid * (512 / C::E)]);
+      if (k < p.num_l2) p.l2_global[k] = G + s_pre[tid * (512 / C::E)];
     }
   }
 
@@ -422,6 +407,16 @@ __global__ __launch_bounds__(256) void level_kernel(LevelParams p) {
       }
     }
   }
+}
+
+// L2 entry = ones before the 512-bit block minus ones before its L1 block
+__global__ void l2_fixup_kernel(const uint64_t* __restrict__ l2_global,
+                                const uint64_t* __restrict__ l1, uint16_t* __restrict__ l2,
+                                uint64_t num_l2) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t k = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x; k < num_l2;
+       k += stride)
+    l2[k] = static_cast<uint16_t>(l2_global[k] - l1[k >> 7]);
 }
 
 // ---------------------------------------------------------------------------
@@ -657,6 +652,9 @@ void launch_level(const LevelParams& p, hipStream_t s) {
     case 4: hipLaunchKernelGGL(level_kernel<uint32_t>, dim3(tiles), dim3(256), 0, s, p); break;
     default: hipLaunchKernelGGL(level_kernel<uint64_t>, dim3(tiles), dim3(256), 0, s, p); break;
   }
+  ECL_HIP(hipGetLastError());
+  hipLaunchKernelGGL(l2_fixup_kernel, dim3(grid_for(p.num_l2, 256, 16384)), dim3(256), 0, s,
+                     p.l2_global, p.l1, p.l2, p.num_l2);
   ECL_HIP(hipGetLastError());
 }
 
