@@ -56,8 +56,8 @@ void DevBuf::reset() {
 }
 
 static const char* kKernelNames[K_COUNT] = {
-    "histogram", "wt_level", "rs_build", "wt_access", "wt_rank", "wt_select",
-    "wt_mixed", "rs_rank", "rs_select", "generate", "fill"};
+    "histogram", "wt_pass", "rs_build", "wt_access", "wt_rank", "wt_select",
+    "wt_mixed", "rs_rank", "rs_select", "generate", "fill", "tile_hist", "tile_scan"};
 const char* kernel_name(int id) { return id >= 0 && id < K_COUNT ? kKernelNames[id] : "?"; }
 
 struct ProfRecord { int id; hipEvent_t e0, e1; };
@@ -346,31 +346,39 @@ static void finalize_rank_select(ecl_rank_select_t* rs, ecl_bit_array_t* ba, hip
   rs->ba = ba;
 }
 
+// rs_build pass of every non-empty array of `ba` into the directory of `rs`.
+static void build_directories(ecl_rank_select_t* rs, ecl_bit_array_t* ba, hipStream_t s) {
+  const size_t na = ba->sizes.size();
+  uint64_t max_tiles = 0;
+  for (size_t i = 0; i < na; ++i) max_tiles = std::max(max_tiles, rs_build_tiles(rs->num_l1[i]));
+  DevBuf status((max_tiles + 2) * sizeof(uint64_t));
+  for (size_t i = 0; i < na; ++i) {
+    if (ba->sizes[i] == 0) continue;
+    const uint64_t tiles = rs_build_tiles(rs->num_l1[i]);
+    launch_fill_u64(status.as<uint64_t>(), tiles + 2, 0, s);
+    RsBuildParams p{};
+    p.words64 = reinterpret_cast<const uint64_t*>(ba->words() + ba->offsets[i]);
+    p.m = ba->sizes[i];
+    p.l1 = rs->l1.as<uint64_t>() + rs->l1_off[i];
+    p.l2 = rs->l2.as<uint16_t>() + rs->l2_off[i];
+    p.num_l1 = rs->num_l1[i];
+    p.num_l2 = rs->num_l2[i];
+    p.samples = rs->samples.as<uint64_t>() + rs->s_off[i];
+    p.cap = rs->cap[i];
+    p.total_ones = rs->total_ones.as<uint64_t>() + i;
+    p.status = status.as<uint64_t>();
+    p.counter = status.as<uint64_t>() + tiles + 1;
+    launch_rs_build(p, s);
+  }
+  // the status buffer is freed on return: finish with it first
+  ECL_HIP(hipStreamSynchronize(s));
+}
+
 // RankSelect(BitArray&&) (rank_select.cu:50-304): one rs_build pass per array.
 static ecl_rank_select_t* make_rank_select(ecl_bit_array_t* ba, hipStream_t s) {
   ecl_rank_select_t* rs = alloc_rank_select(ba, s);
   try {
-    const size_t na = ba->sizes.size();
-    uint64_t max_l1 = 0;
-    for (size_t i = 0; i < na; ++i) max_l1 = std::max(max_l1, rs->num_l1[i]);
-    DevBuf status((max_l1 + 2) * sizeof(uint64_t));
-    for (size_t i = 0; i < na; ++i) {
-      if (ba->sizes[i] == 0) continue;
-      launch_fill_u64(status.as<uint64_t>(), rs->num_l1[i] + 2, 0, s);
-      RsBuildParams p{};
-      p.words64 = reinterpret_cast<const uint64_t*>(ba->words() + ba->offsets[i]);
-      p.m = ba->sizes[i];
-      p.l1 = rs->l1.as<uint64_t>() + rs->l1_off[i];
-      p.l2 = rs->l2.as<uint16_t>() + rs->l2_off[i];
-      p.num_l1 = rs->num_l1[i];
-      p.num_l2 = rs->num_l2[i];
-      p.samples = rs->samples.as<uint64_t>() + rs->s_off[i];
-      p.cap = rs->cap[i];
-      p.total_ones = rs->total_ones.as<uint64_t>() + i;
-      p.status = status.as<uint64_t>();
-      p.counter = status.as<uint64_t>() + rs->num_l1[i] + 1;
-      launch_rs_build(p, s);
-    }
+    build_directories(rs, ba, s);
     finalize_rank_select(rs, ba, s);
   } catch (...) {
     rs->ba = nullptr;
@@ -447,16 +455,44 @@ void u64_to_elems(const std::vector<uint64_t>& v, void* out, int esize) {
   }
 }
 
+// Construction plan: codes are stored cw bytes wide and the levels are
+// built in passes of at most 8 levels (kernels_tree.hip).
+struct BuildPlan {
+  int cw = 1;
+  uint32_t npasses = 1;
+  std::vector<uint32_t> k, l0;  // levels per pass and first level
+};
+BuildPlan plan_of(uint32_t L) {
+  BuildPlan b;
+  b.cw = L <= 8 ? 1 : L <= 16 ? 2 : L <= 32 ? 4 : 8;
+  b.npasses = (L + 7) / 8;
+  b.k.resize(b.npasses);
+  b.l0.resize(b.npasses);
+  uint32_t acc = 0;
+  for (uint32_t q = 0; q < b.npasses; ++q) {
+    b.k[q] = L / b.npasses + (q < L % b.npasses ? 1u : 0u);
+    b.l0[q] = acc;
+    acc += b.k[q];
+  }
+  return b;
+}
+
 // Memory a tree of n symbols needs beyond its text (upper bound, bytes):
-// bits + L2/L1/samples + level buffers + metadata + the reference's
-// 10M-query headroom (wavelet_tree.cuh:644-648).
-uint64_t tree_needed_bytes(uint64_t n, uint64_t sigma, uint32_t L, int esize,
-                           int level_buffers) {
+// bits + L2/L1/samples per level, the mapped codes and the level-order
+// buffers of multi-pass trees, the tile histograms and their scan, a host
+// text's device copy, metadata and the reference's 10M-query headroom
+// (wavelet_tree.cuh:644-648).
+uint64_t tree_needed_bytes(uint64_t n, uint64_t sigma, uint32_t L, int esize, bool host_text) {
+  const BuildPlan b = plan_of(L);
   const uint64_t per_level = ((n + 1023) / 1024) * 128 + 256 /* bits, padded */ +
                              ((n + 511) / 512) * 2 + ((n + 65535) / 65536) * 8 +
                              (n / 4096) * 8 + 64;
-  return per_level * L + static_cast<uint64_t>(level_buffers) * n * esize +
-         sigma * 64 + (n / 4096 + 4096) * 8 + 10'000'000ull * 16;
+  const uint64_t codes = n * b.cw + 4 * sigma * 16;
+  const uint64_t buffers = (b.npasses > 1 ? 2 : 1) * codes;
+  const uint64_t tiles = n / pass_tile_codes(b.cw) + 2 * sigma + 2;
+  const uint64_t scan = tiles * 256 * 12 + (tiles / 256 + 1) * 256 * 8;
+  return per_level * L + buffers + scan + (host_text ? n * esize : 0) + sigma * 64 +
+         (n / 4096 + 4096) * 8 + 10'000'000ull * 16;
 }
 
 }  // namespace
@@ -534,37 +570,27 @@ static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_
       wt->codes.push_back(code[c]);
       wt->code_lens.push_back(len[c]);
     }
-    const bool remap = !wt->min_alphabet || !wt->pow2;
+    const BuildPlan plan = plan_of(L);
+    const int cw = plan.cw;
+    const uint64_t T = pass_tile_codes(cw);   // codes per tile
+    const uint64_t E = 16 / cw;               // codes per 16 bytes
 
     // ---- memory admission (wavelet_tree.cuh:644-689)
-    const int buffers = 2;  // host text: own copy + 1; device text: 2 scratch
-    const uint64_t needed = tree_needed_bytes(n, sigma, L, esize, buffers);
+    const uint64_t needed = tree_needed_bytes(n, sigma, L, esize, !text_on_device);
     if (needed > free_device_memory()) throw Error(ECL_ERROR_OUT_OF_MEMORY_TREE, kOomTree);
 
-    // ---- text on the device and the two level buffers; a device text that
-    // is not 16-byte aligned is copied once (the passes use 16 B vectors)
-    DevBuf own_text, buf_y;
+    // ---- the text on the device
+    DevBuf own_text;
     const void* d_text = text;
-    if (text_on_device && (reinterpret_cast<uintptr_t>(text) & 15) != 0) {
-      own_text = DevBuf(n * esize);
-      ECL_HIP(hipMemcpy(own_text.ptr, text, n * esize, hipMemcpyDeviceToDevice));
-      d_text = own_text.ptr;
-    }
     if (!text_on_device) {
       own_text = DevBuf(n * esize);
       h2d(own_text.ptr, static_cast<const uint8_t*>(text), n * esize);
       d_text = own_text.ptr;
     }
-    DevBuf buf_x;
-    void* X;  // buffer that receives the codes / odd levels
-    if (text_on_device) {
-      buf_x = DevBuf(n * esize);
-      X = buf_x.ptr;
-    } else {
-      X = own_text.ptr;  // our own copy: reused in place
-    }
-    buf_y = DevBuf(n * esize);
-    void* Y = buf_y.ptr;
+    // the passes read 16-byte vectors: a text that is its own code and is
+    // aligned is read in place, anything else is mapped to codes first
+    const bool aligned = (reinterpret_cast<uintptr_t>(d_text) & 15) == 0;
+    const bool identity = wt->min_alphabet && wt->pow2 && esize == cw && aligned;
 
     // ---- symbol maps
     if (wt->map_kind == 1) {
@@ -584,139 +610,282 @@ static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_
       u64_to_elems(wt->alphabet, raw.data(), esize);
       h2d(wt->d_alphabet.ptr, raw.data(), sigma * esize);
     }
-    DevBuf d_codes_full;
     if (!wt->pow2) {
       wt->d_codes = DevBuf(std::max<size_t>(wt->codes.size(), 1) * 8);
       h2d(wt->d_codes.ptr, wt->codes.data(), wt->codes.size());
-      wt->d_code_lens = DevBuf(std::max<size_t>(wt->codes.size(), 1));
+      wt->d_code_lens = DevBuf(std::max<size_t>(wt->code_lens.size(), 1));
       h2d(wt->d_code_lens.ptr, wt->code_lens.data(), wt->code_lens.size());
     }
 
-    // ---- histogram (+ remap to codes into X)
-    DevBuf d_hist(sigma * 8);
-    launch_fill_u64(d_hist.as<uint64_t>(), sigma, 0, s);
-    HistParams hp{};
-    hp.text = d_text; hp.n = n; hp.esize = esize; hp.sigma = sigma;
-    hp.map_kind = wt->map_kind;
-    hp.lut = wt->d_lut.as<uint32_t>();
-    hp.alphabet = wt->d_alphabet.ptr;
-    hp.codes_start = wt->codes_start;
-    hp.pow2 = wt->pow2 ? 1 : 0;
-    hp.codes = wt->d_codes.as<uint64_t>();
-    hp.codes_out = remap ? X : nullptr;
-    hp.hist = d_hist.as<uint64_t>();
-    launch_histogram(hp, s);
-    std::vector<uint64_t> hist(sigma);
+    // ---- symbol counts: a multi-pass tree over a non-power-of-2 alphabet
+    // needs its level sizes (hence the counts) before the first pass;
+    // otherwise they follow from the digit counts of the passes
+    std::vector<uint64_t> sym_count;
+    if (plan.npasses > 1 && !wt->pow2) {
+      DevBuf d_hist(sigma * 8);
+      launch_fill_u64(d_hist.as<uint64_t>(), sigma, 0, s);
+      HistParams hp{};
+      hp.text = d_text; hp.n = n; hp.esize = esize; hp.sigma = sigma;
+      hp.map_kind = wt->map_kind;
+      hp.lut = wt->d_lut.as<uint32_t>();
+      hp.alphabet = wt->d_alphabet.ptr;
+      hp.codes_start = wt->codes_start;
+      hp.pow2 = 0;
+      hp.codes = wt->d_codes.as<uint64_t>();
+      hp.codes_out = nullptr;
+      hp.hist = d_hist.as<uint64_t>();
+      launch_histogram(hp, s);
+      sym_count.resize(sigma);
+      ECL_HIP(hipStreamSynchronize(s));
+      d2h(sym_count.data(), d_hist.ptr, sigma);
+    }
+
+    // ---- code and level-order buffers; scan scratch for the largest pass
+    DevBuf buf_x, buf_u, buf_v;
+    uint64_t max_tiles = (n + T - 1) / T;
+    for (uint32_t q = 1; q < plan.npasses; ++q)
+      max_tiles = std::max<uint64_t>(max_tiles, n / T + (1ull << plan.l0[q]) + 1);
+    const uint32_t kmax = plan.k[0];
+    const uint64_t pad_cap = n + (2 * sigma + 2) * E;  // level order with padded nodes
+    if (!identity) buf_x = DevBuf((plan.npasses > 2 ? pad_cap : n) * cw);
+    if (plan.npasses > 1) buf_u = DevBuf(pad_cap * cw);
+    if (plan.npasses > 2 && identity) buf_v = DevBuf(pad_cap * cw);
+    DevBuf d_hist_t(max_tiles * (1ull << kmax) * 4);
+    DevBuf d_excl((max_tiles + 1) * (1ull << kmax) * 8);
+    DevBuf d_partial(scan_partial_entries(max_tiles, kmax) * 8);
+
+    PassParams pp{};
+    pp.map_kind = wt->map_kind;
+    pp.lut = wt->d_lut.as<uint32_t>();
+    pp.alphabet = wt->d_alphabet.ptr;
+    pp.sigma = sigma;
+    pp.codes_start = wt->codes_start;
+    pp.pow2 = wt->pow2 ? 1 : 0;
+    pp.code_table = wt->d_codes.as<uint64_t>();
+    pp.cw = cw;
+    DevBuf d_err(4);
+    launch_fill_u32(d_err.as<uint32_t>(), 1, 0, s);
+    pp.err = d_err.as<uint32_t>();
+    pp.hist = d_hist_t.as<uint32_t>();
+    pp.excl = d_excl.as<uint64_t>();
+    pp.partial = d_partial.as<uint64_t>();
+
+    // ---- pass 0 digit counts (the whole text is one node)
+    const uint32_t k0 = plan.k[0], nb0 = 1u << k0;
+    const uint64_t tiles0 = (n + T - 1) / T;
+    pp.src = d_text;
+    pp.src_esize = esize;
+    pp.codes_out = identity ? nullptr : buf_x.ptr;
+    pp.codes = identity ? d_text : buf_x.ptr;
+    pp.k = k0;
+    pp.dshift = L - k0;
+    pp.n = n;
+    pp.num_tiles = tiles0;
+    launch_tile_hist(pp, s);
+    launch_tile_scan(pp, s);
+    // bin counts of the current pass: [node][digit] codes of the pass input
+    std::vector<uint64_t> binc(nb0);
     ECL_HIP(hipStreamSynchronize(s));
-    d2h(hist.data(), d_hist.ptr, sigma);
+    d2h(binc.data(), d_excl.as<uint64_t>() + tiles0 * nb0, nb0);
+    if (plan.npasses == 1) {  // digits are whole codes
+      sym_count.resize(sigma);
+      for (uint64_t c = 0; c < sigma; ++c) sym_count[c] = binc[code[c]];
+    }
+
+    // ---- level sizes and the bit arrays (zeroed: run edges are OR-ed in)
+    std::vector<uint64_t> level_size(L, n);
+    if (!sym_count.empty()) {
+      uint64_t tot = 0;
+      for (uint64_t c = 0; c < sigma; ++c) tot += sym_count[c];
+      if (tot != n) throw Error(ECL_ERROR_INTERNAL, "histogram does not sum to n");
+      for (uint32_t l = 0; l < L; ++l) {
+        uint64_t x = 0;
+        for (uint64_t c = 0; c < sigma; ++c)
+          if (len[c] > l) x += sym_count[c];
+        level_size[l] = x;
+      }
+    }
+    ecl_bit_array_t* ba = make_bit_array(level_size, 0, device);
+    std::unique_ptr<ecl_bit_array_t> ba_guard(ba);
+    ecl_rank_select_t* rs = alloc_rank_select(ba, s);
+    std::unique_ptr<ecl_rank_select_t> rs_guard(rs);
+
+    // ---- the passes
+    std::vector<uint64_t> node_size{n};   // level-l0 node sizes of the current pass
+    std::vector<uint64_t> node_pstart{0}; // their padded starts in the pass input
+    DevBuf d_tstart, d_tlen, d_tnode, d_tfirst, d_rows, d_pc, d_thr, d_obase;
+    const void* in_codes = pp.codes;
+    for (uint32_t q = 0; q < plan.npasses; ++q) {
+      const uint32_t k = plan.k[q], l0 = plan.l0[q], nb = 1u << k;
+      const uint64_t nodes = 1ull << l0;
+      const bool last = q + 1 == plan.npasses;
+      pp.k = k;
+      pp.dshift = L - l0 - k;
+      if (q > 0) {
+        // node-aligned tiles over the padded level-l0 order
+        std::vector<uint64_t> tstart;
+        std::vector<uint32_t> tlen, tnode, tfirst(nodes + 1);
+        for (uint64_t P = 0; P < nodes; ++P) {
+          tfirst[P] = static_cast<uint32_t>(tstart.size());
+          for (uint64_t o = 0; o < node_size[P]; o += T) {
+            tstart.push_back(node_pstart[P] + o);
+            tlen.push_back(static_cast<uint32_t>(std::min<uint64_t>(T, node_size[P] - o)));
+            tnode.push_back(static_cast<uint32_t>(P));
+          }
+        }
+        tfirst[nodes] = static_cast<uint32_t>(tstart.size());
+        const uint64_t ntiles = tstart.size();
+        if (ntiles > max_tiles) throw Error(ECL_ERROR_INTERNAL, "tile estimate exceeded");
+        d_tstart = DevBuf(std::max<uint64_t>(ntiles, 1) * 8);
+        d_tlen = DevBuf(std::max<uint64_t>(ntiles, 1) * 4);
+        d_tnode = DevBuf(std::max<uint64_t>(ntiles, 1) * 4);
+        d_tfirst = DevBuf((nodes + 1) * 4);
+        h2d(d_tstart.ptr, tstart.data(), ntiles);
+        h2d(d_tlen.ptr, tlen.data(), ntiles);
+        h2d(d_tnode.ptr, tnode.data(), ntiles);
+        h2d(d_tfirst.ptr, tfirst.data(), nodes + 1);
+        pp.src = in_codes;
+        pp.src_esize = cw;
+        pp.codes_out = nullptr;
+        pp.codes = in_codes;
+        pp.map_kind = 0;
+        pp.num_tiles = ntiles;
+        pp.tile_start = d_tstart.as<uint64_t>();
+        pp.tile_len = d_tlen.as<uint32_t>();
+        pp.tile_node = d_tnode.as<uint32_t>();
+        pp.node_first_tile = d_tfirst.as<uint32_t>();
+        launch_tile_hist(pp, s);
+        launch_tile_scan(pp, s);
+        // digit counts per node: differences of the scan rows at node starts
+        d_rows = DevBuf((nodes + 1) * nb * 8);
+        launch_gather_rows(d_excl.as<uint64_t>(), d_tfirst.as<uint32_t>(), nodes + 1, nb,
+                           d_rows.as<uint64_t>(), s);
+        std::vector<uint64_t> rows((nodes + 1) * nb);
+        ECL_HIP(hipStreamSynchronize(s));
+        d2h(rows.data(), d_rows.ptr, rows.size());
+        binc.assign(nodes * nb, 0);
+        for (uint64_t P = 0; P < nodes; ++P)
+          for (uint32_t d = 0; d < nb; ++d)
+            binc[P * nb + d] = rows[(P + 1) * nb + d] - rows[P * nb + d];
+      }
+      // first absent digit of every node at levels l0 .. l0+k (group aligned)
+      std::vector<uint16_t> thr(nodes * (k + 1), static_cast<uint16_t>(nb));
+      if (!wt->pow2) {
+        std::vector<int32_t> maxd(nodes * (k + 1), -1);
+        for (uint64_t c = 0; c < sigma; ++c) {
+          if (len[c] <= l0) continue;
+          const uint64_t P = detail::shr64(code[c], L - l0);
+          const int32_t d = static_cast<int32_t>((code[c] >> (L - l0 - k)) & (nb - 1));
+          for (uint32_t i = 0; i <= k; ++i)
+            if (len[c] > l0 + i) maxd[P * (k + 1) + i] = std::max(maxd[P * (k + 1) + i], d);
+        }
+        for (uint64_t x = 0; x < nodes * (k + 1); ++x) {
+          const uint32_t i = static_cast<uint32_t>(x % (k + 1));
+          thr[x] = maxd[x] < 0 ? 0
+                               : static_cast<uint16_t>(((static_cast<uint32_t>(maxd[x]) >> (k - i)) + 1)
+                                                       << (k - i));
+        }
+      }
+      // global start of (node, digit): codes of earlier nodes + earlier digits
+      std::vector<uint64_t> pc(nodes * (nb + 1));
+      {
+        uint64_t ns = 0;
+        for (uint64_t P = 0; P < nodes; ++P) {
+          uint64_t run = ns;
+          for (uint32_t d = 0; d < nb; ++d) {
+            pc[P * (nb + 1) + d] = run;
+            run += binc[P * nb + d];
+          }
+          pc[P * (nb + 1) + nb] = run;
+          ns += node_size[P];
+        }
+      }
+      d_pc = DevBuf(pc.size() * 8);
+      h2d(d_pc.ptr, pc.data(), pc.size());
+      d_thr = DevBuf(thr.size() * 2);
+      h2d(d_thr.ptr, thr.data(), thr.size());
+      pp.pc = d_pc.as<uint64_t>();
+      pp.thr = d_thr.as<uint16_t>();
+      // next pass: node sizes and padded starts of level l0+k
+      std::vector<uint64_t> next_size, next_pstart;
+      void* out_buf = nullptr;
+      if (!last) {
+        next_size.assign(nodes * nb, 0);
+        next_pstart.assign(nodes * nb, 0);
+        uint64_t run = 0;
+        for (uint64_t P = 0; P < nodes; ++P)
+          for (uint32_t d = 0; d < nb; ++d) {
+            const uint64_t Q = P * nb + d;
+            next_size[Q] = d < thr[P * (k + 1) + k] ? binc[Q] : 0;
+            next_pstart[Q] = run;
+            run += (next_size[Q] + E - 1) / E * E;
+          }
+        if (run > pad_cap) throw Error(ECL_ERROR_INTERNAL, "level buffer too small");
+        d_obase = DevBuf(next_pstart.size() * 8);
+        h2d(d_obase.ptr, next_pstart.data(), next_pstart.size());
+        // outputs alternate between two buffers that differ from the input
+        out_buf = (q % 2 == 0) ? buf_u.ptr : (buf_x.ptr ? buf_x.ptr : buf_v.ptr);
+        pp.obase = d_obase.as<uint64_t>();
+      }
+      pp.dst = out_buf;
+      pp.dst_cap = out_buf ? pad_cap : 0;
+      pp.words = ba->words();
+      for (uint32_t i = 0; i < k; ++i) {
+        pp.level_word[i] = ba->offsets[l0 + i];
+        pp.level_nwords[i] = (level_size[l0 + i] + 31) / 32;
+      }
+      launch_pass(pp, s);
+      if (!last) {
+        in_codes = out_buf;
+        node_size.swap(next_size);
+        node_pstart.swap(next_pstart);
+      }
+    }
+    if (sym_count.empty()) {  // power-of-2 alphabet: the last pass's digits end the codes
+      const uint32_t kl = plan.k.back(), nbl = 1u << kl;
+      sym_count.resize(sigma);
+      for (uint64_t c = 0; c < sigma; ++c)
+        sym_count[c] = binc[(code[c] >> kl) * nbl + (code[c] & (nbl - 1))];
+    }
+
+    {
+      uint32_t err = 0;
+      ECL_HIP(hipStreamSynchronize(s));
+      d2h(&err, d_err.ptr, 1);
+      if (err) throw Error(ECL_ERROR_INTERNAL, "construction pass wrote outside its arrays");
+    }
+
+    // ---- rank/select directory of every level
+    build_directories(rs, ba, s);
+    own_text.reset();
+    buf_x.reset();
+    buf_u.reset();
+    buf_v.reset();
+
+    // ---- counts, node ranks, nodes per level (host metadata)
     wt->counts.resize(sigma);
     {
       uint64_t acc = 0;
-      for (uint64_t c = 0; c < sigma; ++c) { wt->counts[c] = acc; acc += hist[c]; }
+      for (uint64_t c = 0; c < sigma; ++c) { wt->counts[c] = acc; acc += sym_count[c]; }
       if (acc != n) throw Error(ECL_ERROR_INTERNAL, "histogram does not sum to n");
     }
     auto C = [&](uint64_t c) { return c >= sigma ? n : wt->counts[c]; };
-
-    // ---- level sizes, node tables, node ranks
-    std::vector<uint64_t> level_size(L, 0);
-    std::vector<std::vector<uint64_t>> tables(L);
     wt->nodes_at_level.assign(L > 1 ? L - 1 : 0, 0);
     for (uint32_t l = 0; l < L; ++l) {
       uint64_t zb = 0;  // zeros of level l before the node
-      const bool need_table = l + 1 < L;
-      if (need_table) tables[l].assign(2ull << l, detail::kSkip);
       for (const Node& nd : nodes[l]) {
         const uint64_t mid = nd.lo + detail::prev_pow2(nd.hi - nd.lo + 1);
-        level_size[l] += C(nd.hi + 1) - C(nd.lo);
         if (l >= 1 && nd.lo >= 1) {
           wt->ranks.push_back(zb);
           wt->node_start.push_back(nd.lo);
           wt->node_level.push_back(static_cast<uint8_t>(l));
           if (l + 1 < L) wt->nodes_at_level[l]++;
         }
-        if (need_table) {
-          const uint64_t pfx = detail::shr64(code[nd.lo], L - l);
-          const bool left_node = mid - nd.lo >= 2, right_node = nd.hi - mid + 1 >= 2;
-          tables[l][2 * pfx] = left_node ? C(nd.lo) - zb : detail::kSkip;
-          tables[l][2 * pfx + 1] = right_node ? C(mid) - C(nd.lo) + zb : detail::kSkip;
-        }
         zb += C(mid) - C(nd.lo);
       }
     }
     for (uint64_t x : wt->nodes_at_level) wt->nodes_until_last_level += x;
 
-    // ---- bit arrays and the level passes
-    ecl_bit_array_t* ba = make_bit_array(level_size, -1, device);
-    std::unique_ptr<ecl_bit_array_t> ba_guard(ba);
-    ecl_rank_select_t* rs = alloc_rank_select(ba, s);
-    std::unique_ptr<ecl_rank_select_t> rs_guard(rs);
-    uint64_t max_tiles = 0;
-    for (uint32_t l = 0; l < L; ++l)
-      max_tiles = std::max(max_tiles, level_tiles(level_size[l], esize));
-    DevBuf status((max_tiles + 2) * 8);  // look-back status + tile counter
-    uint64_t max_l2 = 1;
-    for (uint32_t l = 0; l < L; ++l) max_l2 = std::max(max_l2, rs->num_l2[l]);
-    DevBuf l2_global(max_l2 * 8);
-    // all node tables in one upload; level l uses 2^(l+1) entries
-    std::vector<uint64_t> table_off(L, 0);
-    std::vector<uint64_t> all_tables;
-    for (uint32_t l = 0; l < L; ++l) {
-      table_off[l] = all_tables.size();
-      all_tables.insert(all_tables.end(), tables[l].begin(), tables[l].end());
-    }
-    DevBuf d_table(std::max<size_t>(all_tables.size(), 1) * 8);
-    h2d(d_table.ptr, all_tables.data(), all_tables.size());
-    const void* src = remap ? X : d_text;
-    void* dst = Y;
-    for (uint32_t l = 0; l < L; ++l) {
-      const uint64_t nl = level_size[l];
-      uint32_t* words = ba->words() + ba->offsets[l];
-      const uint64_t nw64 = (nl + 63) / 64;
-      // zero the padding words this pass does not write
-      const uint64_t padded = (((nl + 31) / 32 + 31) & ~31ull);
-      if (padded > 2 * nw64) launch_fill_u32(words + 2 * nw64, padded - 2 * nw64, 0, s);
-      if (nl == 0) continue;
-      const bool last = l + 1 == L;
-      // the tile counter (and look-back state) must start from zero every pass
-      const uint64_t tiles = level_tiles(nl, esize);
-      launch_fill_u64(status.as<uint64_t>(), tiles + 2, 0, s);
-      LevelParams lp{};
-      lp.src = src;
-      lp.dst = last ? nullptr : dst;
-      lp.esize = esize;
-      lp.n = nl;
-      lp.n_next = last ? 0 : level_size[l + 1];
-      lp.bit = L - 1 - l;
-      lp.pshift = L - l;
-      lp.table = d_table.as<uint64_t>() + table_off[l];
-      lp.num_prefixes = last ? 0 : (1ull << l);
-      lp.words64 = reinterpret_cast<uint64_t*>(words);
-      lp.nwords64 = nw64;
-      lp.l1 = rs->l1.as<uint64_t>() + rs->l1_off[l];
-      lp.l2 = rs->l2.as<uint16_t>() + rs->l2_off[l];
-      lp.num_l1 = rs->num_l1[l];
-      lp.num_l2 = rs->num_l2[l];
-      lp.samples = rs->samples.as<uint64_t>() + rs->s_off[l];
-      lp.cap = rs->cap[l];
-      lp.total_ones = rs->total_ones.as<uint64_t>() + l;
-      lp.status = status.as<uint64_t>();
-      lp.counter = status.as<uint64_t>() + tiles + 1;
-      lp.l2_global = l2_global.as<uint64_t>();
-      launch_level(lp, s);
-      if (!last) {
-        src = dst;
-        dst = (dst == Y) ? X : Y;
-      }
-    }
-    if (ba->total_words > ba->offsets.back() + (((level_size.back() + 31) / 32 + 31) & ~31ull))
-      launch_fill_u32(ba->words() + ba->total_words - 1, 1, 0, s);  // even-count pad word
-    ECL_HIP(hipStreamSynchronize(s));
-    own_text.reset();
-    buf_x.reset();
-    buf_y.reset();
-    status.reset();
-
-    // ---- the level passes wrote the rank/select directory of every level
     finalize_rank_select(rs, ba, s);
     ba_guard.release();
     wt->rs = rs_guard.release();
@@ -1239,7 +1408,7 @@ ecl_status ecl_wavelet_tree_device_view(ecl_wavelet_tree wt, void* view, size_t 
   });
 }
 size_t ecl_wavelet_tree_needed_gpu_memory(size_t n, size_t sigma, uint8_t L, size_t) {
-  return tree_needed_bytes(n, sigma, L, 8, 0);
+  return tree_needed_bytes(n, sigma, L, 8, false);
 }
 
 // Chunked host batch (the reference's graph of H2D -> kernel -> D2H per

@@ -45,8 +45,9 @@ struct DevBuf {
 
 // ---- kernel profiling (events around selected launches) -------------------
 enum KernelId : int {
-  K_HIST = 0, K_LEVEL, K_RS_BUILD, K_WT_ACCESS, K_WT_RANK, K_WT_SELECT,
-  K_WT_MIXED, K_RS_RANK, K_RS_SELECT, K_GEN, K_FILL, K_COUNT
+  K_HIST = 0, K_PASS, K_RS_BUILD, K_WT_ACCESS, K_WT_RANK, K_WT_SELECT,
+  K_WT_MIXED, K_RS_RANK, K_RS_SELECT, K_GEN, K_FILL, K_TILE_HIST, K_TILE_SCAN,
+  K_COUNT
 };
 const char* kernel_name(int id);
 struct ProfScope {
@@ -72,27 +73,41 @@ struct HistParams {
 };
 void launch_histogram(const HistParams& p, hipStream_t s);
 
-// One level pass: bits of the level, its rank/select directory (L1, L2,
-// select samples, total ones) and, unless it is the last level, the stable
-// partition of the symbols into the next level's order.
-struct LevelParams {
-  const void* src; void* dst; int esize;   // dst == nullptr: last level
-  uint64_t n, n_next;
-  uint32_t bit;            // code bit of this level (L-1-l)
-  uint32_t pshift;         // code >> pshift = node prefix (L-l)
-  const uint64_t* table;   // [2 * prefixes] child bases (kSkip = leaf)
-  uint64_t num_prefixes;   // 2^l
-  uint64_t* words64;       // level bit words (u64 view)
-  uint64_t nwords64;       // u64 words this level writes
-  uint64_t* l1; uint16_t* l2; uint64_t num_l1, num_l2;
-  uint64_t* l2_global;     // scratch: ones before each 512-bit block
-  uint64_t* samples; uint64_t cap;
-  uint64_t* total_ones;
-  uint64_t* status; uint64_t* counter;  // lookback state (zeroed)
+// One construction pass over levels l0 .. l0+k-1 (kernels_tree.hip).
+struct PassParams {
+  // tile_hist input: the text (first pass) or the level-l0 codes
+  const void* src; int src_esize;
+  void* codes_out;            // first pass: mapped codes (width cw), or nullptr
+  int map_kind;               // 0 identity, 1 LUT, 2 binary search
+  const uint32_t* lut; const void* alphabet; uint64_t sigma;
+  uint64_t codes_start; int pow2; const uint64_t* code_table;
+  // pass input: level-l0 codes of width cw (src, or codes_out after mapping)
+  const void* codes; int cw;
+  uint32_t k, dshift;         // digit = (code >> dshift) & (2^k - 1)
+  uint64_t n;                 // uniform tiles over [0, n) when tile_start == nullptr
+  uint64_t num_tiles;
+  const uint64_t* tile_start; const uint32_t* tile_len; const uint32_t* tile_node;
+  const uint32_t* node_first_tile;   // [nodes + 1], nullptr for uniform tiles
+  uint32_t* hist;             // [tiles][2^k] digit counts per tile
+  uint64_t* excl;             // [tiles + 1][2^k] exclusive column scan
+  uint64_t* partial;          // scan scratch, scan_partial_entries() u64
+  const uint64_t* pc;         // [nodes][2^k + 1] global start of (node, digit)
+  const uint64_t* obase;      // [nodes][2^k] padded start of (node, digit) in dst
+  const uint16_t* thr;        // [nodes][k + 1] first absent digit per level
+  void* dst;                  // level-(l0+k) codes, or nullptr on the last pass
+  uint64_t dst_cap;           // codes dst can hold
+  uint32_t* words;            // bit array words
+  uint64_t level_word[8];     // word offset of levels l0 .. l0+k-1
+  uint64_t level_nwords[8];   // words of each of those levels
+  uint32_t* err;              // set to 1 if a write fell outside its array
 };
-// tiles of a level pass; the look-back state needs 2 * tiles + 2 u64
-uint64_t level_tiles(uint64_t n, int esize);
-void launch_level(const LevelParams& p, hipStream_t s);
+uint64_t pass_tile_codes(int cw);                   // codes per tile
+uint64_t scan_partial_entries(uint64_t tiles, uint32_t k);
+void launch_tile_hist(const PassParams& p, hipStream_t s);
+void launch_tile_scan(const PassParams& p, hipStream_t s);
+void launch_pass(const PassParams& p, hipStream_t s);
+void launch_gather_rows(const uint64_t* excl, const uint32_t* rows, uint64_t nrows,
+                        uint32_t nb, uint64_t* out, hipStream_t s);
 
 struct RsBuildParams {
   const uint64_t* words64; uint64_t m;
@@ -101,6 +116,7 @@ struct RsBuildParams {
   uint64_t* total_ones;
   uint64_t* status; uint64_t* counter;  // lookback state (zeroed)
 };
+uint64_t rs_build_tiles(uint64_t num_l1);  // look-back tiles (status: tiles + 2 u64)
 void launch_rs_build(const RsBuildParams& p, hipStream_t s);
 
 void launch_gen_text(void* out, int esize, uint64_t n, uint64_t seed,

@@ -1,0 +1,687 @@
+// kernels_tree.hip -- wavelet-tree construction for gfx950: several levels
+// per pass over the text.
+//
+// Level l of a wavelet tree lists the symbols whose code is longer than l,
+// stably ordered by their top l code bits, and stores bit L-1-l of each code
+// (the per-level radix sort and fill of tree/wavelet_tree.cuh:806-818 and
+// fillLevelKernel, :241-323).  The reference sorts the whole text once per
+// level.  Here a pass covers k <= 8 consecutive levels l0 .. l0+k-1 and
+// reads the level-l0 order once:
+//
+//  * tile_hist_kernel: per tile of the level-l0 order, the histogram of the
+//    k-bit digits (the code bits of the pass's levels).  The first pass also
+//    maps the text to codes here when the text is not already its own code.
+//  * scan_*_kernel: for every digit, the number of codes with that digit in
+//    all earlier tiles (exclusive column scan of the tile histograms).
+//  * pass_kernel: the tile is staged in LDS and partitioned level by level
+//    inside LDS (stable one-bit split within each node).  At level l0+i the
+//    tile's codes of one node form a single run of that level's bit array,
+//    starting at the node's global start plus the node's codes in earlier
+//    tiles, both known from the digit counts.  Runs are written as 32-bit
+//    words; the two edge words of a run are shared with neighbouring runs and
+//    use atomicOr on the zeroed bit array.  A pass that is not the last also
+//    writes its codes in the order of level l0+k for the next pass.
+//
+// Tiles of later passes never straddle a node of level l0: every node of the
+// pass input starts on a 16-byte boundary (the previous pass pads between
+// nodes) and is cut into tiles of its own.
+#include <hip/hip_runtime.h>
+
+#include "internal.hpp"
+
+namespace ecl {
+namespace lib {
+namespace {
+
+constexpr uint32_t kPassThreads = 512;
+constexpr uint32_t kChunkRows = 4;                            // 16-byte chunks per thread
+constexpr uint32_t kTileChunks = kPassThreads * kChunkRows;   // 2048 chunks = 32 KiB
+constexpr uint32_t kMaxBins = 256;
+
+template <int CW> struct CodeOf;
+template <> struct CodeOf<1> { using type = uint8_t; };
+template <> struct CodeOf<2> { using type = uint16_t; };
+template <> struct CodeOf<4> { using type = uint32_t; };
+template <> struct CodeOf<8> { using type = uint64_t; };
+
+// bit `bit` of each of the 16/sizeof(T) codes of a 16-byte chunk, code k at bit k
+template <typename T>
+__device__ __forceinline__ uint32_t chunk_mask(const uint4& v, uint32_t bit) {
+  const uint32_t d[4] = {v.x, v.y, v.z, v.w};
+  uint32_t m = 0;
+  if constexpr (sizeof(T) == 1) {
+    // isolate the bit of each byte of a dword and gather the four bits into
+    // a nibble with one multiply (the partial products do not overlap)
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint32_t x = (d[k] >> bit) & 0x01010101u;
+      m |= ((x * 0x01020408u) >> 24 & 0xFu) << (4 * k);
+    }
+  } else if constexpr (sizeof(T) == 2) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      m |= ((d[k] >> bit) & 1u) << (2 * k);
+      m |= ((d[k] >> (16 + bit)) & 1u) << (2 * k + 1);
+    }
+  } else if constexpr (sizeof(T) == 4) {
+#pragma unroll
+    for (int k = 0; k < 4; ++k) m |= ((d[k] >> bit) & 1u) << k;
+  } else {
+    const uint64_t a = (static_cast<uint64_t>(v.y) << 32) | v.x;
+    const uint64_t b = (static_cast<uint64_t>(v.w) << 32) | v.z;
+    m = static_cast<uint32_t>((a >> bit) & 1u) | (static_cast<uint32_t>((b >> bit) & 1u) << 1);
+  }
+  return m;
+}
+
+// code k of a 16-byte chunk, by shifts (k is a constant after unrolling)
+template <typename T>
+__device__ __forceinline__ T chunk_elem(const uint4& q, uint32_t k) {
+  const uint32_t wi = (k * static_cast<uint32_t>(sizeof(T))) >> 2;
+  const uint32_t d = wi == 0 ? q.x : wi == 1 ? q.y : wi == 2 ? q.z : q.w;
+  if constexpr (sizeof(T) == 1) return static_cast<T>(d >> (8 * (k & 3)));
+  else if constexpr (sizeof(T) == 2) return static_cast<T>(d >> (16 * (k & 1)));
+  else if constexpr (sizeof(T) == 4) return static_cast<T>(d);
+  else return static_cast<T>(k == 0 ? ((static_cast<uint64_t>(q.y) << 32) | q.x)
+                                    : ((static_cast<uint64_t>(q.w) << 32) | q.z));
+}
+
+// the chunk at src[s0 ..] with only the first `valid` codes loaded (rest 0)
+template <typename T>
+__device__ __forceinline__ uint4 load_partial_chunk(const T* src, uint64_t s0, uint32_t valid) {
+  uint32_t d[4] = {0, 0, 0, 0};
+  constexpr uint32_t E = 16 / sizeof(T);
+#pragma unroll
+  for (uint32_t k = 0; k < E; ++k) {
+    if (k < valid) {
+      const uint64_t x = static_cast<uint64_t>(src[s0 + k]);
+      if constexpr (sizeof(T) == 8) {
+        d[2 * k] = static_cast<uint32_t>(x);
+        d[2 * k + 1] = static_cast<uint32_t>(x >> 32);
+      } else {
+        const uint32_t bitpos = k * 8 * static_cast<uint32_t>(sizeof(T));
+        d[bitpos >> 5] |= static_cast<uint32_t>(x) << (bitpos & 31);
+      }
+    }
+  }
+  return make_uint4(d[0], d[1], d[2], d[3]);
+}
+
+// Tile geometry: tile t covers codes [start, start + len) of the pass input,
+// inside node `node` of level l0.
+struct TileGeo {
+  uint64_t start;
+  uint32_t len, node;
+};
+__device__ __forceinline__ TileGeo tile_geo(const PassParams& p, uint64_t t, uint32_t T) {
+  TileGeo g;
+  if (p.tile_start != nullptr) {
+    g.start = p.tile_start[t];
+    g.len = p.tile_len[t];
+    g.node = p.tile_node[t];
+  } else {
+    g.start = t * T;
+    const uint64_t rest = p.n - g.start;
+    g.len = rest < T ? static_cast<uint32_t>(rest) : T;
+    g.node = 0;
+  }
+  return g;
+}
+
+// Symbol -> code on the first pass (tree/wavelet_tree.cuh:2295-2413 remap):
+// map_kind 0 identity, 1 LUT (8/16-bit texts), 2 lower_bound over the sorted
+// alphabet; values outside the alphabet clamp to the last symbol.
+template <typename TIn, int MapKind>
+__device__ __forceinline__ uint64_t map_code(const PassParams& p, uint64_t x) {
+  uint64_t c;
+  if constexpr (MapKind == 0) {
+    c = x;
+  } else if constexpr (MapKind == 1) {
+    c = p.lut[x];
+  } else {
+    const TIn* alpha = static_cast<const TIn*>(p.alphabet);
+    uint64_t lo = 0, hi = p.sigma;
+    while (lo < hi) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if (static_cast<uint64_t>(alpha[mid]) < x) lo = mid + 1; else hi = mid;
+    }
+    c = lo;
+  }
+  if (c >= p.sigma) c = p.sigma - 1;
+  return (p.pow2 || c < p.codes_start) ? c : p.code_table[c - p.codes_start];
+}
+
+// ---------------------------------------------------------------------------
+// Tile digit histograms.  Each wave owns 256 LDS bins split into
+// 256 / 2^k replicas of the 2^k digits (lane l uses replica l % replicas),
+// so small digit ranges do not serialise on a few addresses.
+template <typename TIn, int CW, int MapKind, bool Remap>
+__global__ __launch_bounds__(kPassThreads) void tile_hist_kernel(PassParams p) {
+  using Code = typename CodeOf<CW>::type;
+  constexpr uint32_t T = kTileChunks * (16 / CW);   // codes per tile
+  constexpr uint32_t EI = 16 / sizeof(TIn);          // input symbols per chunk
+  __shared__ uint32_t s_bins[(kPassThreads / 64) * kMaxBins];
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint32_t nb = 1u << p.k, reps = kMaxBins >> p.k;
+  const uint32_t dmask = nb - 1;
+  for (uint32_t i = tid; i < (kPassThreads / 64) * kMaxBins; i += kPassThreads) s_bins[i] = 0;
+  __syncthreads();
+  uint32_t* bins = s_bins + wave * kMaxBins + (lane % reps) * nb;
+  const uint64_t t = blockIdx.x;
+  const TileGeo g = tile_geo(p, t, T);
+  const TIn* in = static_cast<const TIn*>(p.src);
+  Code* out = static_cast<Code*>(p.codes_out);
+  const uint32_t nchunks = (g.len + EI - 1) / EI;
+  for (uint32_t c = tid; c < nchunks; c += kPassThreads) {
+    const uint64_t e0 = g.start + static_cast<uint64_t>(c) * EI;
+    const uint32_t valid = (g.len - c * EI < EI) ? g.len - c * EI : EI;
+    uint4 q;
+    if (valid == EI) q = *reinterpret_cast<const uint4*>(in + e0);
+    else q = load_partial_chunk<TIn>(in, e0, valid);
+    if constexpr (Remap) {
+      // codes of this chunk, assembled for one store when the chunk is full
+      constexpr uint32_t OB = EI * CW;  // output bytes per chunk (<= 16)
+      uint32_t o[4] = {0, 0, 0, 0};
+#pragma unroll
+      for (uint32_t k = 0; k < EI; ++k) {
+        if (k < valid) {
+          const uint64_t code = map_code<TIn, MapKind>(p, static_cast<uint64_t>(chunk_elem<TIn>(q, k)));
+          atomicAdd(&bins[(code >> p.dshift) & dmask], 1u);
+          if constexpr (CW == 8) {
+            o[2 * k] = static_cast<uint32_t>(code);
+            o[2 * k + 1] = static_cast<uint32_t>(code >> 32);
+          } else {
+            const uint32_t bp = k * 8 * CW;
+            o[bp >> 5] |= static_cast<uint32_t>(code) << (bp & 31);
+          }
+        }
+      }
+      if (valid == EI) {
+        if constexpr (OB == 16) *reinterpret_cast<uint4*>(out + e0) = make_uint4(o[0], o[1], o[2], o[3]);
+        else if constexpr (OB == 8) *reinterpret_cast<uint2*>(out + e0) = make_uint2(o[0], o[1]);
+        else if constexpr (OB == 4) *reinterpret_cast<uint32_t*>(out + e0) = o[0];
+        else if constexpr (OB == 2) *reinterpret_cast<uint16_t*>(out + e0) = static_cast<uint16_t>(o[0]);
+        else *reinterpret_cast<uint8_t*>(out + e0) = static_cast<uint8_t>(o[0]);
+      } else {
+#pragma unroll
+        for (uint32_t k = 0; k < EI; ++k) {
+          if (k < valid) {
+            if constexpr (CW == 8) {
+              out[e0 + k] = (static_cast<uint64_t>(o[2 * k + 1]) << 32) | o[2 * k];
+            } else {
+              const uint32_t bp = k * 8 * CW;
+              out[e0 + k] = static_cast<Code>(o[bp >> 5] >> (bp & 31));
+            }
+          }
+        }
+      }
+    } else {
+#pragma unroll
+      for (uint32_t k = 0; k < EI; ++k)
+        if (k < valid)
+          atomicAdd(&bins[(static_cast<uint64_t>(chunk_elem<TIn>(q, k)) >> p.dshift) & dmask], 1u);
+    }
+  }
+  __syncthreads();
+  for (uint32_t d = tid; d < nb; d += kPassThreads) {
+    uint32_t sum = 0;
+    for (uint32_t w = 0; w < kPassThreads / 64; ++w)
+      for (uint32_t r = 0; r < reps; ++r) sum += s_bins[w * kMaxBins + r * nb + d];
+    p.hist[t * nb + d] = sum;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Exclusive column scan of the tile histograms, in blocks of 256 tiles:
+//  partial[b][d] = sum of block b  ->  exclusive over blocks  ->  excl rows.
+constexpr uint32_t kScanTiles = 256;
+
+__global__ __launch_bounds__(256) void scan_partial_kernel(const uint32_t* __restrict__ hist,
+                                                           uint64_t tiles, uint32_t nb,
+                                                           uint64_t* __restrict__ partial) {
+  __shared__ uint64_t s[256];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t rows = 256 / nb, col = tid % nb, r0 = tid / nb;
+  const uint64_t t0 = blockIdx.x * static_cast<uint64_t>(kScanTiles);
+  const uint64_t t1 = (t0 + kScanTiles < tiles) ? t0 + kScanTiles : tiles;
+  uint64_t sum = 0;
+  for (uint64_t t = t0 + r0; t < t1; t += rows) sum += hist[t * nb + col];
+  s[tid] = sum;
+  __syncthreads();
+  if (tid < nb) {
+    uint64_t tot = 0;
+    for (uint32_t r = 0; r < rows; ++r) tot += s[r * nb + tid];
+    partial[blockIdx.x * static_cast<uint64_t>(nb) + tid] = tot;
+  }
+}
+
+// one workgroup per digit column: exclusive scan of the block sums in place
+__global__ __launch_bounds__(256) void scan_blocks_kernel(uint64_t* __restrict__ partial,
+                                                          uint64_t nblk, uint32_t nb) {
+  __shared__ uint64_t s_wave[4];
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint32_t col = blockIdx.x;
+  const uint64_t per = (nblk + 255) / 256;
+  const uint64_t b0 = tid * per;
+  const uint64_t b1 = (b0 + per < nblk) ? b0 + per : nblk;
+  uint64_t sum = 0;
+  for (uint64_t b = b0; b < b1; ++b) sum += partial[b * nb + col];
+  uint64_t incl = sum;
+#pragma unroll
+  for (uint32_t off = 1; off < 64; off <<= 1) {
+    const uint64_t y = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += y;
+  }
+  if (lane == 63) s_wave[wave] = incl;
+  __syncthreads();
+  uint64_t run = incl - sum;
+  for (uint32_t w = 0; w < wave; ++w) run += s_wave[w];
+  for (uint64_t b = b0; b < b1; ++b) {
+    const uint64_t x = partial[b * nb + col];
+    partial[b * nb + col] = run;
+    run += x;
+  }
+}
+
+// excl[t][d] = codes with digit d in tiles < t; row `tiles` holds the totals
+__global__ __launch_bounds__(256) void scan_final_kernel(const uint32_t* __restrict__ hist,
+                                                         const uint64_t* __restrict__ partial,
+                                                         uint64_t tiles, uint32_t nb,
+                                                         uint64_t* __restrict__ excl) {
+  __shared__ uint64_t s[256];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t rows = 256 / nb, col = tid % nb, r0 = tid / nb;
+  const uint32_t seg = kScanTiles / rows;  // consecutive tiles per thread
+  const uint64_t blk = blockIdx.x;
+  const uint64_t t0 = blk * kScanTiles + static_cast<uint64_t>(r0) * seg;
+  uint64_t sum = 0;
+  for (uint32_t j = 0; j < seg; ++j)
+    if (t0 + j < tiles) sum += hist[(t0 + j) * nb + col];
+  s[tid] = sum;
+  __syncthreads();
+  uint64_t run = partial[blk * nb + col];
+  for (uint32_t r = 0; r < r0; ++r) run += s[r * nb + col];
+  for (uint32_t j = 0; j < seg; ++j) {
+    const uint64_t t = t0 + j;
+    if (t >= tiles) break;
+    excl[t * nb + col] = run;
+    run += hist[t * nb + col];
+  }
+  const uint64_t last_blk = (tiles - 1) / kScanTiles;
+  if (blk == last_blk && r0 == rows - 1) excl[tiles * nb + col] = run;
+}
+
+// rows[j] of excl -> out[j] (digit counts at node boundaries)
+__global__ void gather_rows_kernel(const uint64_t* __restrict__ excl,
+                                   const uint32_t* __restrict__ rows, uint32_t nb,
+                                   uint64_t* __restrict__ out) {
+  const uint64_t j = blockIdx.x;
+  const uint64_t r = rows[j];
+  for (uint32_t d = threadIdx.x; d < nb; d += blockDim.x) out[j * nb + d] = excl[r * nb + d];
+}
+
+// ---------------------------------------------------------------------------
+// Block-wide exclusive scan of one u64 per thread (kPassThreads threads).
+__device__ __forceinline__ uint64_t block_excl_scan_u64(uint64_t x, uint64_t* s_wave,
+                                                        uint64_t& total) {
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  uint64_t incl = x;
+#pragma unroll
+  for (uint32_t off = 1; off < 64; off <<= 1) {
+    const uint64_t y = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += y;
+  }
+  if (lane == 63) s_wave[wave] = incl;
+  __syncthreads();
+  uint64_t before = 0, tot = 0;
+#pragma unroll
+  for (uint32_t w = 0; w < kPassThreads / 64; ++w) {
+    const uint64_t v = s_wave[w];
+    before += (w < wave) ? v : 0ull;
+    tot += v;
+  }
+  total = tot;
+  return before + incl - x;
+}
+
+// Exclusive scan of 256 values held 4 per lane by one wave (lane l holds
+// entries 4l .. 4l+3); returns the exclusive prefixes in place.
+template <typename V>
+__device__ __forceinline__ void wave_excl_scan4(V (&v)[4], V& total) {
+  const uint32_t lane = threadIdx.x & 63;
+  V s = v[0] + v[1] + v[2] + v[3];
+  V incl = s;
+#pragma unroll
+  for (uint32_t off = 1; off < 64; off <<= 1) {
+    const V y = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += y;
+  }
+  total = __shfl(incl, 63, 64);
+  V run = incl - s;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const V x = v[j];
+    v[j] = run;
+    run += x;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// One pass over k levels of one tile (see the file comment).  Thread tid
+// owns chunks c = r * 512 + tid (r < 4) of 16 bytes; chunk c holds the codes
+// at tile positions [c*E, c*E + E) of the current level order.
+template <int CW>
+__global__ __launch_bounds__(kPassThreads) void pass_kernel(PassParams p) {
+  using Code = typename CodeOf<CW>::type;
+  constexpr uint32_t E = 16 / CW;                 // codes per chunk
+  constexpr uint32_t T = kTileChunks * E;          // codes per tile
+  constexpr uint32_t LPW = 32 / E;                 // lanes per 32-bit bit word
+  __shared__ uint4 s_buf[2][kTileChunks];          // 64 KiB: current / next order
+  __shared__ uint32_t s_bits[kTileChunks * E / 32]; // level bits of the tile
+  __shared__ uint16_t s_opre[kTileChunks];         // ones before each chunk
+  __shared__ uint32_t s_hc[kMaxBins + 1];          // codes of the tile with digit < x
+  __shared__ uint64_t s_ec[kMaxBins + 1];          // node's codes with digit < x in earlier tiles
+  __shared__ uint64_t s_pc[kMaxBins + 1];          // global start of (node, digit x)
+  __shared__ uint32_t s_base[kMaxBins];            // partition base per child
+  __shared__ uint32_t s_gw[kMaxBins / 2 + 1];      // word offsets of the level's runs
+  __shared__ uint64_t s_wave[kPassThreads / 64];
+  __shared__ uint32_t s_thr[9];
+  __shared__ uint32_t s_tot_ones;
+
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint64_t t = blockIdx.x;
+  const TileGeo g = tile_geo(p, t, T);
+  const uint32_t k = p.k, nb = 1u << k;
+
+  // ---- per-tile tables (one wave each)
+  if (wave == 0) {
+    uint32_t v[4], tot;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t d = 4 * lane + j;
+      v[j] = d < nb ? p.hist[t * nb + d] : 0u;
+    }
+    wave_excl_scan4<uint32_t>(v, tot);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) s_hc[4 * lane + j] = v[j];
+    if (lane == 0) s_hc[kMaxBins] = tot;
+  } else if (wave == 1) {
+    const uint64_t f = p.node_first_tile != nullptr ? p.node_first_tile[g.node] : 0;
+    uint64_t v[4], tot;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t d = 4 * lane + j;
+      v[j] = d < nb ? p.excl[t * nb + d] - p.excl[f * nb + d] : 0ull;
+    }
+    wave_excl_scan4<uint64_t>(v, tot);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) s_ec[4 * lane + j] = v[j];
+    if (lane == 0) s_ec[kMaxBins] = tot;
+  } else if (wave == 2) {
+    const uint64_t* pc = p.pc + static_cast<uint64_t>(g.node) * (nb + 1);
+    for (uint32_t x = lane; x <= nb; x += 64) s_pc[x] = pc[x];
+  } else if (wave == 3) {
+    if (lane <= k) s_thr[lane] = p.thr[static_cast<uint64_t>(g.node) * (k + 1) + lane];
+  }
+  // ---- stage the tile
+  const Code* src = static_cast<const Code*>(p.codes);
+#pragma unroll
+  for (uint32_t r = 0; r < kChunkRows; ++r) {
+    const uint32_t c = r * kPassThreads + tid;
+    const uint32_t p0 = c * E;
+    uint4 q = make_uint4(0, 0, 0, 0);
+    if (p0 + E <= g.len) q = *reinterpret_cast<const uint4*>(src + g.start + p0);
+    else if (p0 < g.len) q = load_partial_chunk<Code>(src, g.start + p0, g.len - p0);
+    s_buf[0][c] = q;
+  }
+  // (digits >= nb never occur, so s_hc[x] and s_ec[x] for x >= nb hold the totals)
+  __syncthreads();
+
+  uint32_t cur = 0;
+  for (uint32_t i = 0; i < k; ++i) {
+    const uint32_t npres = s_hc[s_thr[i]];            // tile codes present at level l0+i
+    const uint32_t cbit = p.dshift + k - 1 - i;        // code bit of the level
+    const uint32_t gshift = k - i;                     // digit >> gshift = group
+    uint4 v[kChunkRows];
+    uint64_t packed = 0;
+#pragma unroll
+    for (uint32_t r = 0; r < kChunkRows; ++r) {
+      const uint32_t c = r * kPassThreads + tid;
+      v[r] = s_buf[cur][c];
+      uint32_t mm = chunk_mask<Code>(v[r], cbit & 63);
+      const uint32_t p0 = c * E;
+      if (p0 + E > npres) mm &= (p0 >= npres) ? 0u : ((1u << (npres - p0)) - 1u);
+      // 32-bit words of the tile's bit string: the chunks of LPW neighbouring lanes
+      uint32_t w32 = mm << (E * (lane % LPW));
+#pragma unroll
+      for (uint32_t off = 1; off < LPW; off <<= 1) w32 |= __shfl_xor(w32, off, 64);
+      if (lane % LPW == 0) s_bits[p0 >> 5] = w32;
+      packed |= static_cast<uint64_t>(__popc(mm)) << (16 * r);
+    }
+    uint64_t tot_packed;
+    const uint64_t ex = block_excl_scan_u64(packed, s_wave, tot_packed);
+    uint32_t O[kChunkRows];
+    {
+      uint32_t rows_before = 0;
+#pragma unroll
+      for (uint32_t r = 0; r < kChunkRows; ++r) {
+        O[r] = rows_before + static_cast<uint32_t>((ex >> (16 * r)) & 0xFFFF);
+        s_opre[r * kPassThreads + tid] = static_cast<uint16_t>(O[r]);
+        rows_before += static_cast<uint32_t>((tot_packed >> (16 * r)) & 0xFFFF);
+      }
+      if (tid == 0) s_tot_ones = rows_before;
+    }
+    // runs of this level: present groups q < thr >> gshift
+    const uint32_t ngroups = s_thr[i] >> gshift;
+    if (wave == 0) {
+      uint32_t w2[2];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const uint32_t q = 2 * lane + j;
+        w2[j] = 0;
+        if (q < ngroups) {
+          const uint32_t a = q << gshift, b = (q + 1) << gshift;
+          const uint32_t len = s_hc[b] - s_hc[a];
+          const uint64_t R = s_pc[a] + s_ec[b] - s_ec[a];
+          w2[j] = len ? static_cast<uint32_t>(((R & 31) + len + 31) >> 5) : 0u;
+        }
+      }
+      const uint32_t s2 = w2[0] + w2[1];
+      const uint32_t incl = detail::wave_incl_scan_u32(s2);
+      s_gw[2 * lane] = incl - s2;
+      s_gw[2 * lane + 1] = incl - s2 + w2[0];
+      if (lane == 63) s_gw[kMaxBins / 2] = incl;
+    }
+    __syncthreads();
+    // ---- write the level's runs
+    {
+      const uint32_t total_words = s_gw[kMaxBins / 2];
+      const uint32_t* bits32 = s_bits;
+      uint32_t* words = p.words + p.level_word[i];
+      for (uint32_t w = tid; w < total_words; w += kPassThreads) {
+        uint32_t lo = 0, hi = ngroups;  // last group with s_gw[q] <= w
+        while (hi - lo > 1) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (s_gw[mid] <= w) lo = mid; else hi = mid;
+        }
+        const uint32_t q = lo;
+        const uint32_t a = q << gshift, b = (q + 1) << gshift;
+        const uint32_t src0 = s_hc[a];
+        const uint32_t len = s_hc[b] - src0;
+        const uint64_t R = s_pc[a] + s_ec[b] - s_ec[a];
+        const uint64_t D = (R >> 5) + (w - s_gw[q]);    // destination word
+        const uint64_t db = D * 32;
+        const uint32_t blo = R > db ? static_cast<uint32_t>(R - db) : 0u;
+        const uint64_t rend = R + len;
+        const uint32_t bhi = rend < db + 32 ? static_cast<uint32_t>(rend - db) : 32u;
+        // source bit of destination bit blo
+        const uint32_t sp = src0 + static_cast<uint32_t>(db + blo - R);
+        const uint32_t sw = sp >> 5;
+        const uint64_t win = static_cast<uint64_t>(bits32[sw]) |
+                             (sw + 1 < T / 32 ? static_cast<uint64_t>(bits32[sw + 1]) << 32 : 0ull);
+        uint32_t val = static_cast<uint32_t>(win >> (sp & 31));
+        const uint32_t nbits = bhi - blo;
+        if (nbits < 32) val &= (1u << nbits) - 1u;
+        val <<= blo;
+        if (D >= p.level_nwords[i]) {
+          atomicOr(p.err, 1u);
+        } else if (nbits == 32) {
+          words[D] = val;
+        } else if (val) {
+          atomicOr(&words[D], val);
+        }
+      }
+    }
+    const bool partition = (i + 1 < k) || (p.dst != nullptr);
+    if (!partition) break;
+    // ---- partition bases: child c of group c >> 1 takes its zeros (c even)
+    // or ones (c odd) in order, starting at the child's first position
+    {
+      const uint32_t nchild = 2u << i;
+      for (uint32_t c = tid; c < nchild; c += kPassThreads) {
+        const uint32_t q = c >> 1, bit = c & 1;
+        const uint32_t ha = s_hc[q << gshift];
+        const uint32_t ch = ha / E, off = ha % E;
+        const uint32_t cw = s_bits[(ch * E) >> 5] >> ((ch * E) & 31);  // chunk's bits
+        const uint32_t Oa = ch < kTileChunks ? s_opre[ch] + __popc(cw & ((1u << off) - 1u))
+                                             : s_tot_ones;
+        const uint32_t child_lo = c << (gshift - 1);
+        s_base[c] = s_hc[child_lo] - (bit ? Oa : ha - Oa);
+      }
+    }
+    __syncthreads();
+    Code* nxt = reinterpret_cast<Code*>(s_buf[cur ^ 1]);
+    const uint32_t cmask = (2u << i) - 1u;
+#pragma unroll
+    for (uint32_t r = 0; r < kChunkRows; ++r) {
+      const uint32_t c = r * kPassThreads + tid;
+      const uint32_t p0 = c * E;
+      uint32_t o = O[r], z = p0 - O[r];
+#pragma unroll
+      for (uint32_t j = 0; j < E; ++j) {
+        if (p0 + j < npres) {
+          const Code x = chunk_elem<Code>(v[r], j);
+          const uint32_t child = static_cast<uint32_t>(static_cast<uint64_t>(x) >> (cbit & 63)) & cmask;
+          const uint32_t b = child & 1u;
+          const uint32_t dest = s_base[child] + (b ? o : z);
+          if (dest < T) nxt[dest] = x;
+          else atomicOr(p.err, 1u);
+          o += b;
+          z += b ^ 1u;
+        }
+      }
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+
+  // ---- codes of level l0+k for the next pass, grouped by digit
+  if (p.dst != nullptr) {
+    // s_pc becomes the output offset per digit: the (node, digit) start in
+    // the padded output + its codes in earlier tiles - its first tile position
+    const uint64_t* ob = p.obase + static_cast<uint64_t>(g.node) * nb;
+    for (uint32_t d = tid; d < nb; d += kPassThreads)
+      s_pc[d] = ob[d] + (s_ec[d + 1] - s_ec[d]) - s_hc[d];
+    __syncthreads();
+    const uint32_t nout = s_hc[s_thr[k]];
+    const Code* cb = reinterpret_cast<const Code*>(s_buf[cur]);
+    Code* dst = static_cast<Code*>(p.dst);
+    const uint32_t dmask = nb - 1;
+    for (uint32_t x = tid; x < nout; x += kPassThreads) {
+      const Code c = cb[x];
+      const uint32_t d = static_cast<uint32_t>(static_cast<uint64_t>(c) >> p.dshift) & dmask;
+      const uint64_t o = s_pc[d] + x;
+      if (o < p.dst_cap) dst[o] = c;
+      else atomicOr(p.err, 1u);
+    }
+  }
+}
+
+}  // namespace
+
+uint64_t pass_tile_codes(int cw) { return kTileChunks * (16u / static_cast<uint32_t>(cw)); }
+
+void launch_tile_hist(const PassParams& p, hipStream_t s) {
+  if (p.num_tiles == 0) return;
+  if (p.num_tiles > 0x7FFFFFFFull) throw Error(ECL_ERROR_INVALID_ARGUMENT, "text too large");
+  ProfScope ps(K_TILE_HIST, s);
+  const dim3 grid(static_cast<uint32_t>(p.num_tiles)), block(kPassThreads);
+  const bool remap = p.codes_out != nullptr;
+#define ECL_TH(TIN, CWV, MK)                                                                 \
+  do {                                                                                       \
+    if (remap) hipLaunchKernelGGL((tile_hist_kernel<TIN, CWV, MK, true>), grid, block, 0, s, p); \
+    else hipLaunchKernelGGL((tile_hist_kernel<TIN, CWV, MK, false>), grid, block, 0, s, p);      \
+  } while (0)
+#define ECL_TH_MK(TIN, CWV)                          \
+  do {                                               \
+    if (p.map_kind == 0) ECL_TH(TIN, CWV, 0);        \
+    else if (p.map_kind == 1) ECL_TH(TIN, CWV, 1);   \
+    else ECL_TH(TIN, CWV, 2);                        \
+  } while (0)
+#define ECL_TH_MK02(TIN, CWV)                        \
+  do {                                               \
+    if (p.map_kind == 0) ECL_TH(TIN, CWV, 0);        \
+    else ECL_TH(TIN, CWV, 2);                        \
+  } while (0)
+  if (p.src_esize == 1) {
+    ECL_TH_MK(uint8_t, 1);
+  } else if (p.src_esize == 2) {
+    if (p.cw == 1) ECL_TH_MK(uint16_t, 1); else ECL_TH_MK(uint16_t, 2);
+  } else if (p.src_esize == 4) {
+    if (p.cw == 1) ECL_TH_MK02(uint32_t, 1);
+    else if (p.cw == 2) ECL_TH_MK02(uint32_t, 2);
+    else ECL_TH_MK02(uint32_t, 4);
+  } else {
+    if (p.cw == 1) ECL_TH_MK02(uint64_t, 1);
+    else if (p.cw == 2) ECL_TH_MK02(uint64_t, 2);
+    else if (p.cw == 4) ECL_TH_MK02(uint64_t, 4);
+    else ECL_TH_MK02(uint64_t, 8);
+  }
+#undef ECL_TH_MK02
+#undef ECL_TH_MK
+#undef ECL_TH
+  ECL_HIP(hipGetLastError());
+}
+
+uint64_t scan_partial_entries(uint64_t tiles, uint32_t k) {
+  return ((tiles + kScanTiles - 1) / kScanTiles) << k;
+}
+
+void launch_tile_scan(const PassParams& p, hipStream_t s) {
+  if (p.num_tiles == 0) return;
+  ProfScope ps(K_TILE_SCAN, s);
+  const uint32_t nb = 1u << p.k;
+  const uint64_t nblk = (p.num_tiles + kScanTiles - 1) / kScanTiles;
+  hipLaunchKernelGGL(scan_partial_kernel, dim3(static_cast<uint32_t>(nblk)), dim3(256), 0, s,
+                     p.hist, p.num_tiles, nb, p.partial);
+  ECL_HIP(hipGetLastError());
+  hipLaunchKernelGGL(scan_blocks_kernel, dim3(nb), dim3(256), 0, s, p.partial, nblk, nb);
+  ECL_HIP(hipGetLastError());
+  hipLaunchKernelGGL(scan_final_kernel, dim3(static_cast<uint32_t>(nblk)), dim3(256), 0, s,
+                     p.hist, p.partial, p.num_tiles, nb, p.excl);
+  ECL_HIP(hipGetLastError());
+}
+
+void launch_gather_rows(const uint64_t* excl, const uint32_t* rows, uint64_t nrows, uint32_t nb,
+                        uint64_t* out, hipStream_t s) {
+  if (nrows == 0) return;
+  hipLaunchKernelGGL(gather_rows_kernel, dim3(static_cast<uint32_t>(nrows)), dim3(256), 0, s,
+                     excl, rows, nb, out);
+  ECL_HIP(hipGetLastError());
+}
+
+void launch_pass(const PassParams& p, hipStream_t s) {
+  if (p.num_tiles == 0) return;
+  ProfScope ps(K_PASS, s);
+  const dim3 grid(static_cast<uint32_t>(p.num_tiles)), block(kPassThreads);
+  switch (p.cw) {
+    case 1: hipLaunchKernelGGL(pass_kernel<1>, grid, block, 0, s, p); break;
+    case 2: hipLaunchKernelGGL(pass_kernel<2>, grid, block, 0, s, p); break;
+    case 4: hipLaunchKernelGGL(pass_kernel<4>, grid, block, 0, s, p); break;
+    default: hipLaunchKernelGGL(pass_kernel<8>, grid, block, 0, s, p); break;
+  }
+  ECL_HIP(hipGetLastError());
+}
+
+}  // namespace lib
+}  // namespace ecl
