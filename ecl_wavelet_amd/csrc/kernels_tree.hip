@@ -86,6 +86,21 @@ __device__ __forceinline__ T chunk_elem(const uint4& q, uint32_t k) {
                                     : ((static_cast<uint64_t>(q.w) << 32) | q.z));
 }
 
+// `width` bits of code k of the chunk from bit `shift` on: one bit-field
+// extract from the chunk's dword below 64-bit codes
+template <typename T>
+__device__ __forceinline__ uint32_t child_of(const uint4& q, uint32_t k, uint32_t shift,
+                                             uint32_t width) {
+  if constexpr (sizeof(T) == 8) {
+    return static_cast<uint32_t>(chunk_elem<T>(q, k) >> (shift & 63)) & ((1u << width) - 1u);
+  } else {
+    const uint32_t wi = (k * static_cast<uint32_t>(sizeof(T))) >> 2;
+    const uint32_t d = wi == 0 ? q.x : wi == 1 ? q.y : wi == 2 ? q.z : q.w;
+    const uint32_t bitpos = (k * 8 * static_cast<uint32_t>(sizeof(T))) & 31;
+    return __builtin_amdgcn_ubfe(d, bitpos + shift, width);
+  }
+}
+
 // the chunk at src[s0 ..] with only the first `valid` codes loaded (rest 0)
 template <typename T>
 __device__ __forceinline__ uint4 load_partial_chunk(const T* src, uint64_t s0, uint32_t valid) {
@@ -535,9 +550,11 @@ __global__ __launch_bounds__(kPassThreads) void pass_kernel(PassParams p) {
     const bool partition = (i + 1 < k) || (p.dst != nullptr);
     if (!partition) break;
     // ---- partition bases: child c of group c >> 1 takes its zeros (c even)
-    // or ones (c odd) in order, starting at the child's first position
+    // or ones (c odd) in order, starting at the child's first position; the
+    // entries are indices into both buffers seen as one array of 2T codes
     {
       const uint32_t nchild = 2u << i;
+      const uint32_t boff = (cur ^ 1u) * T;
       for (uint32_t c = tid; c < nchild; c += kPassThreads) {
         const uint32_t q = c >> 1, bit = c & 1;
         const uint32_t ha = s_hc[q << gshift];
@@ -546,28 +563,46 @@ __global__ __launch_bounds__(kPassThreads) void pass_kernel(PassParams p) {
         const uint32_t Oa = ch < kTileChunks ? s_opre[ch] + __popc(cw & ((1u << off) - 1u))
                                              : s_tot_ones;
         const uint32_t child_lo = c << (gshift - 1);
-        s_base[c] = s_hc[child_lo] - (bit ? Oa : ha - Oa);
+        s_base[c] = boff + s_hc[child_lo] - (bit ? Oa : ha - Oa);
       }
     }
     __syncthreads();
-    Code* nxt = reinterpret_cast<Code*>(s_buf[cur ^ 1]);
-    const uint32_t cmask = (2u << i) - 1u;
+    Code* both = reinterpret_cast<Code*>(&s_buf[0][0]);
+    const uint32_t cwidth = i + 1;  // child index = top i+1 digit bits
+    if (npres == T) {
+      // every code of the tile is present at this level: no per-code tests
 #pragma unroll
-    for (uint32_t r = 0; r < kChunkRows; ++r) {
-      const uint32_t c = r * kPassThreads + tid;
-      const uint32_t p0 = c * E;
-      uint32_t o = O[r], z = p0 - O[r];
+      for (uint32_t r = 0; r < kChunkRows; ++r) {
+        const uint32_t p0 = (r * kPassThreads + tid) * E;
+        uint32_t child[E], base[E];
 #pragma unroll
-      for (uint32_t j = 0; j < E; ++j) {
-        if (p0 + j < npres) {
-          const Code x = chunk_elem<Code>(v[r], j);
-          const uint32_t child = static_cast<uint32_t>(static_cast<uint64_t>(x) >> (cbit & 63)) & cmask;
-          const uint32_t b = child & 1u;
-          const uint32_t dest = s_base[child] + (b ? o : z);
-          if (dest < T) nxt[dest] = x;
-          else atomicOr(p.err, 1u);
+        for (uint32_t j = 0; j < E; ++j) child[j] = child_of<Code>(v[r], j, cbit, cwidth);
+#pragma unroll
+        for (uint32_t j = 0; j < E; ++j) base[j] = s_base[child[j]];
+        uint32_t o = O[r], z = p0 - O[r];
+#pragma unroll
+        for (uint32_t j = 0; j < E; ++j) {
+          const uint32_t b = child[j] & 1u;
+          both[base[j] + (b ? o : z)] = chunk_elem<Code>(v[r], j);
           o += b;
           z += b ^ 1u;
+        }
+      }
+    } else {
+#pragma unroll
+      for (uint32_t r = 0; r < kChunkRows; ++r) {
+        const uint32_t p0 = (r * kPassThreads + tid) * E;
+        if (p0 >= npres) continue;
+        uint32_t o = O[r], z = p0 - O[r];
+#pragma unroll
+        for (uint32_t j = 0; j < E; ++j) {
+          if (p0 + j < npres) {
+            const uint32_t child = child_of<Code>(v[r], j, cbit, cwidth);
+            const uint32_t b = child & 1u;
+            both[s_base[child] + (b ? o : z)] = chunk_elem<Code>(v[r], j);
+            o += b;
+            z += b ^ 1u;
+          }
         }
       }
     }
