@@ -45,6 +45,17 @@ def parse():
                     help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--latency-calls", type=int, default=200)
+    ap.add_argument("--bt-texts", type=int, default=10_000,
+                    help="texts of the batched configuration (0 skips it)")
+    ap.add_argument("--bt-queries", type=int, default=100_000_000)
+    ap.add_argument("--sv-trees", type=int, default=64,
+                    help="resident trees of the serving configuration (0 skips it)")
+    ap.add_argument("--sv-max-log2", type=float, default=32.0,
+                    help="log2 of the largest serving text")
+    ap.add_argument("--pmc-workload", action="store_true",
+                    help="short run for hardware-counter passes: one 1 GiB "
+                         "device-to-device copy (the calibration kernel) and "
+                         "two builds of the benchmark tree; prints no JSON")
     return ap.parse_args()
 
 
@@ -119,8 +130,193 @@ def cpu_baseline(n, sigma, seed, args):
     }
 
 
+def bench_batched(args):
+    """Batched configuration: args.bt_texts uint8 texts, lengths log-uniform in
+    [2^10, 2^24], sigma uniform over {4, 20, 64, 256}, uniform or Zipf(1.1)
+    symbols.  Batched form: one forest build, then all queries (per-text
+    count proportional to length, mixed 1:1:1) as one launch replayed from a
+    HIP graph.  Plain form: one build per text through the reference-shaped
+    API and three device calls (access, rank, select) per tree."""
+    import ecl_wavelet_amd as E
+    rng = np.random.default_rng(9)
+    N = args.bt_texts
+    lens = np.floor(2.0 ** rng.uniform(10, 24, N)).astype(np.uint64)
+    sig = rng.choice([4, 20, 64, 256], N).astype(np.uint64)
+    zipf = rng.random(N) < 0.5
+    starts = np.zeros(N, np.uint64)
+    pos = 0
+    for i in range(N):
+        starts[i] = pos
+        pos += int((int(lens[i]) + 15) // 16 * 16)
+    buf = E.DeviceBuffer(pos)
+    thr = {int(x): E.zipf_thresholds(int(x), 1.1) for x in (4, 20, 64, 256)}
+    for i in range(N):
+        E.generate_text_into(buf, int(starts[i]), int(lens[i]), int(sig[i]), np.uint8,
+                             seed=100_000 + i, thresholds=thr[int(sig[i])] if zipf[i] else None)
+    E.synchronize()
+    total = int(lens.sum())
+    log(f"batched: {N} texts, {total} symbols")
+    timer = E.Timer()
+    # ---- batched build (one forest), warm once
+    forest = E.Forest(buf, starts, lens, sig)
+    del forest
+    timer.start()
+    forest = E.Forest(buf, starts, lens, sig)
+    forest_build_ms = timer.stop()
+    # ---- batched queries
+    per = np.maximum(1, np.round(args.bt_queries * lens.astype(np.float64) / total)).astype(np.uint64)
+    qoff = np.concatenate([[0], np.cumsum(per)]).astype(np.uint64)
+    m = int(qoff[-1])
+    d_q = E.DeviceBuffer(m * E.FOREST_QUERY_DTYPE.itemsize)
+    d_out = E.DeviceBuffer(m * 8)
+    forest.generateQueries(qoff, 77, d_q)
+    E.synchronize()
+    block, grid = forest.autotune(d_q, m, d_out)
+    g = forest.graph(d_q, m, d_out)
+    g.launch()
+    E.synchronize()
+    runs = []
+    for _ in range(3):
+        timer.start()
+        g.launch()
+        runs.append(timer.stop())
+    batched_ms = float(np.median(runs))
+    log(f"batched: build {forest_build_ms:.1f} ms, {m} queries in {batched_ms:.2f} ms "
+        f"(block {block}, grid {grid})")
+    del g, forest
+    # ---- plain: one tree per text, per-tree per-kind calls
+    alph = {int(x): np.arange(int(x), dtype=np.uint8) for x in (4, 20, 64, 256)}
+    timer.start()
+    trees = [E.WaveletTree(buf.view(int(starts[i]), int(lens[i])), alph[int(sig[i])],
+                           dtype=np.uint8, n=int(lens[i])) for i in range(N)]
+    plain_build_ms = timer.stop()
+    kinds = [(per + 2 - k) // 3 for k in range(3)]  # access, rank, select counts per tree
+    offs = [np.concatenate([[0], np.cumsum(kc)]).astype(np.uint64) for kc in kinds]
+    d_idx = E.DeviceBuffer(int(offs[0][-1]) * 8)
+    d_rq = E.DeviceBuffer(int(offs[1][-1]) * 16)
+    d_sq = E.DeviceBuffer(int(offs[2][-1]) * 16)
+    d_po = E.DeviceBuffer(max(int(o[-1]) for o in offs) * 8)
+    for i, wt in enumerate(trees):
+        for k, (d, w) in enumerate(((d_idx, 8), (d_rq, 16), (d_sq, 16))):
+            c = int(kinds[k][i])
+            if c:
+                wt.generateQueries(k, 500 + 100 * k + i, c, d.view(int(offs[k][i]) * w, c * w))
+    E.synchronize()
+
+    def plain_calls(i, wt):
+        ka, kr, ks = int(kinds[0][i]), int(kinds[1][i]), int(kinds[2][i])
+        if ka:
+            wt.accessDevice(d_idx.view(int(offs[0][i]) * 8, ka * 8), ka, d_po.view(0, ka * 8),
+                            flags=2)
+        if kr:
+            wt.rankDevice(d_rq.view(int(offs[1][i]) * 16, kr * 16), kr, d_po.view(0, kr * 8),
+                          flags=1)
+        if ks:
+            wt.selectDevice(d_sq.view(int(offs[2][i]) * 16, ks * 16), ks, d_po.view(0, ks * 8),
+                            flags=1)
+
+    for i, wt in enumerate(trees[:64]):  # warm
+        plain_calls(i, wt)
+    E.synchronize()
+    timer.start()
+    for i, wt in enumerate(trees):
+        plain_calls(i, wt)
+    plain_ms = timer.stop()
+    # per-tree latency of the plain calls (each tree's three calls, synchronised)
+    lat = []
+    for i in rng.choice(N, min(N, 300), replace=False):
+        timer.start()
+        plain_calls(int(i), trees[int(i)])
+        lat.append(timer.stop())
+    del trees
+    log(f"plain: build {plain_build_ms:.1f} ms, queries {plain_ms:.2f} ms")
+    return {
+        "config": (f"{N} uint8 texts, lengths log-uniform in [2^10, 2^24] (total {total}), "
+                   f"sigma in {{4, 20, 64, 256}}, uniform or Zipf(1.1); {m} mixed queries "
+                   f"(per-text count proportional to length)"),
+        "batched_build_ms": forest_build_ms,
+        "plain_build_ms": plain_build_ms,
+        "build_speedup": plain_build_ms / forest_build_ms,
+        "batched_qps": m / (batched_ms / 1e3),
+        "plain_qps": m / (plain_ms / 1e3),
+        "query_speedup": plain_ms / batched_ms,
+        "batched_launch_ms": batched_ms,
+        "plain_per_tree_p50_ms": float(np.percentile(lat, 50)),
+        "plain_per_tree_p99_ms": float(np.percentile(lat, 99)),
+        "autotuned": {"block": block, "grid": grid},
+    }
+
+
+def bench_serving(args):
+    """Serving configuration: args.sv_trees resident uint8 trees, lengths
+    log-uniform in [2^20, 2^sv_max_log2], sigma in {4, 256}; Poisson request
+    streams of mixed batches of log-uniform size in [1, 10^6] against a
+    random tree, served with coalescing and in the plain per-call form."""
+    import ecl_wavelet_amd as E
+    rng = np.random.default_rng(10)
+    T = args.sv_trees
+    lens = np.floor(2.0 ** rng.uniform(20, args.sv_max_log2, T)).astype(np.uint64)
+    sig = rng.choice([4, 256], T)
+    srv = E.Server(max_batch=1 << 22, max_inflight=3)
+    timer = E.Timer()
+    timer.start()
+    for i in range(T):
+        d = E.generate_text(int(lens[i]), int(sig[i]), np.uint8, seed=200_000 + i)
+        srv.addTree(d, int(sig[i]), n=int(lens[i]))
+        d.free()
+    admit_ms = timer.stop()
+    st0 = srv.stats()
+    log(f"serving: {T} trees, {int(lens.sum())} symbols, admitted in {admit_ms:.0f} ms, "
+        f"{st0['hbm_used'] / 1e9:.1f} GB resident")
+    out = {"config": (f"{T} resident uint8 trees, lengths log-uniform in [2^20, "
+                      f"2^{args.sv_max_log2:g}] (total {int(lens.sum())}), sigma in {{4, 256}}; "
+                      f"Poisson requests of log-uniform size in [1, 10^6] mixed queries"),
+           "admit_ms": admit_ms, "hbm_used_gb": st0["hbm_used"] / 1e9, "rates": []}
+    for rate, secs in ((1e3, 3.0), (1e4, 1.0), (1e5, 0.1)):
+        row = {"rate_req_s": rate, "seconds": secs}
+        for mode, coalesce in (("served", True), ("plain", False)):
+            srv.setMode(coalesce)
+            st = srv.load(rate, secs, seed=int(rate), min_size=1, max_size=1_000_000,
+                          pool=1 << 20)
+            row[mode] = {k: st[k] for k in ("requests", "queries", "batches", "p50_ms",
+                                            "p99_ms", "queries_per_s")}
+            log(f"serving {mode} @ {rate:g} req/s: {st['queries_per_s']:.3g} q/s, "
+                f"p50 {st['p50_ms']:.2f} ms, p99 {st['p99_ms']:.2f} ms, {st['batches']} batches")
+        out["rates"].append(row)
+    srv.close()
+    return out
+
+
+def pmc_workload(args):
+    """Kernels for the FETCH_SIZE / WRITE_SIZE passes: the HIP runtime's
+    copy kernel over 1 GiB (known traffic: 1 GiB read + 1 GiB written) and
+    two builds of the benchmark tree."""
+    import ctypes as C
+    import ecl_wavelet_amd as E
+    hip = C.CDLL("libamdhip64.so")
+    nbytes = 1 << 30
+    a, b = E.DeviceBuffer(nbytes), E.DeviceBuffer(nbytes)
+    rc = hip.hipMemcpy(b.ptr, a.ptr, C.c_size_t(nbytes), 3)
+    if rc != 0:
+        raise SystemExit(f"hipMemcpy failed: {rc}")
+    E.synchronize()
+    a.free()
+    b.free()
+    log(f"copy calibration: {nbytes} bytes device to device")
+    d_text = E.generate_text(args.n, args.sigma, np.uint8, seed=4)
+    alphabet = np.arange(args.sigma, dtype=np.uint8)
+    for _ in range(2):
+        wt = E.WaveletTree(d_text, alphabet, dtype=np.uint8, n=args.n)
+        E.synchronize()
+        del wt
+    log(f"built the tree twice (n={args.n}, sigma={args.sigma})")
+
+
 def main():
     args = parse()
+    if args.pmc_workload:
+        pmc_workload(args)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU per process; "
                           "multi-GPU runs are not part of it", "n_gpus": args.gpus}))
@@ -268,8 +464,13 @@ def main():
             "launch_ms": pass_ms,
         },
     }
+    d_text.free()
+    del d_idx, d_rq, d_sq, d_mq, d_out
+    if args.bt_texts > 0:
+        res["batched"] = bench_batched(args)
+    if args.sv_trees > 0:
+        res["serving"] = bench_serving(args)
     if not args.no_cpu:
-        d_text.free()
         res["cpu_baseline"] = cpu_baseline(n, sigma, seed, args)
     print(json.dumps(res), flush=True)
 

@@ -5,6 +5,7 @@ C ABI in ``include/ecl_wavelet_c.h``).  This package builds it
 (``build.build_library``), binds it (``_lib``) and mirrors the reference's
 class names in Python (``api``).
 """
-from .api import (BitArray, DeviceBuffer, Profiler, RankSelect, Timer,  # noqa: F401
-                  WaveletTree, device_name, generate_text, mem_info,
-                  synchronize, MIXED_DTYPE, QUERY_DTYPE)
+from .api import (BitArray, DeviceBuffer, Forest, Graph, Profiler, RankSelect,  # noqa: F401
+                  Server, Timer, WaveletTree, device_name, generate_text,
+                  generate_text_into, mem_info, zipf_thresholds,
+                  synchronize, FOREST_QUERY_DTYPE, MIXED_DTYPE, QUERY_DTYPE)

@@ -125,6 +125,27 @@ _SIGS = {
     "ecl_wavelet_tree_select_device": (st, [vp, vp, sz, vp, u32, vp]),
     "ecl_wavelet_tree_query_batch_device": (st, [vp, vp, sz, vp, vp]),
     "ecl_wavelet_tree_generate_queries": (st, [vp, i32, u64, sz, vp, vp]),
+    "ecl_forest_create": (st, [vp, vp, vp, vp, sz, i32, u32, C.POINTER(vp)]),
+    "ecl_forest_destroy": (st, [vp]),
+    "ecl_forest_num_trees": (st, [vp, C.POINTER(sz)]),
+    "ecl_forest_tree_info": (st, [vp, sz, C.POINTER(u64), C.POINTER(u64), C.POINTER(u32)]),
+    "ecl_forest_device_view": (st, [vp, sz, vp, sz]),
+    "ecl_forest_query_device": (st, [vp, vp, sz, vp, vp]),
+    "ecl_forest_generate_queries": (st, [vp, vp, u64, vp, vp]),
+    "ecl_forest_autotune": (st, [vp, vp, sz, vp, C.POINTER(u32), C.POINTER(u32)]),
+    "ecl_forest_graph_create": (st, [vp, vp, sz, vp, C.POINTER(vp)]),
+    "ecl_graph_launch": (st, [vp, vp]),
+    "ecl_graph_destroy": (st, [vp]),
+    "ecl_server_create": (st, [i32, vp, C.POINTER(vp)]),
+    "ecl_server_destroy": (st, [vp]),
+    "ecl_server_add_tree": (st, [vp, vp, i32, sz, u64, C.POINTER(u32)]),
+    "ecl_server_remove_tree": (st, [vp, u32]),
+    "ecl_server_submit": (st, [vp, u32, vp, sz, vp, C.POINTER(u64)]),
+    "ecl_server_wait": (st, [vp, u64, C.POINTER(C.c_double)]),
+    "ecl_server_set_mode": (st, [vp, u32]),
+    "ecl_server_stats_get": (st, [vp, vp]),
+    "ecl_server_stats_reset": (st, [vp]),
+    "ecl_server_load": (st, [vp, vp, vp]),
 }
 
 _lib = None

@@ -15,6 +15,8 @@ _DTYPES = {1: np.uint8, 2: np.uint16, 4: np.uint32, 8: np.uint64}
 # RankSelectQuery<T> as 16 bytes: index, then the symbol in the low bytes
 QUERY_DTYPE = np.dtype([("index", "<u8"), ("symbol", "<u8")])
 MIXED_DTYPE = np.dtype([("arg", "<u8"), ("symbol", "<u4"), ("kind", "<u4")])
+FOREST_QUERY_DTYPE = np.dtype([("arg", "<u8"), ("symbol", "<u4"), ("kind", "<u2"),
+                               ("reserved", "<u2"), ("tree", "<u4"), ("reserved2", "<u4")])
 
 
 def _p(a):
@@ -53,7 +55,20 @@ class DeviceBuffer:
         call("ecl_memcpy_d2h", self.device, _p(out), self.ptr, out.nbytes)
         return out
 
+    def view(self, offset, nbytes):
+        """A non-owning buffer over bytes [offset, offset + nbytes)."""
+        assert 0 <= offset and offset + nbytes <= self.nbytes
+        v = DeviceBuffer.__new__(DeviceBuffer)
+        v.device = self.device
+        v.nbytes = int(nbytes)
+        v.ptr = C.c_void_p(self.ptr.value + int(offset))
+        v._owner = self
+        return v
+
     def free(self):
+        if getattr(self, "_owner", None) is not None:
+            self.ptr = C.c_void_p()
+            return
         if self.ptr:
             call("ecl_device_free", self.device, self.ptr)
             self.ptr = C.c_void_p()
@@ -67,6 +82,27 @@ class DeviceBuffer:
 
 def synchronize(device=0):
     call("ecl_device_synchronize", device)
+
+
+def zipf_thresholds(sigma, s=1.1):
+    """u64 inverse-CDF thresholds of Zipf(s) over sigma symbols for
+    generate_text (the last threshold is 2^64 - 1)."""
+    w = 1.0 / np.power(np.arange(1, sigma + 1, dtype=np.float64), s)
+    cdf = np.cumsum(w) / w.sum()
+    thr = np.minimum(np.floor(cdf * 2.0 ** 64), 2.0 ** 64 - 1)
+    out = np.array([int(x) if x < 2.0 ** 64 else 2 ** 64 - 1 for x in thr], dtype=np.uint64)
+    out[-1] = np.uint64(2 ** 64 - 1)
+    return out
+
+
+def generate_text_into(buf, offset, n, sigma, dtype=np.uint8, seed=1, thresholds=None,
+                       device=0):
+    """The same draws written into an existing device buffer at a byte offset."""
+    esize = np.dtype(dtype).itemsize
+    assert offset + n * esize <= buf.nbytes
+    thr = None if thresholds is None else np.ascontiguousarray(thresholds, np.uint64)
+    call("ecl_generate_text", device, esize, seed, n, sigma,
+         None if thr is None else _p(thr), C.c_void_p(buf.ptr.value + offset))
 
 
 def generate_text(n, sigma, dtype=np.uint8, seed=1, thresholds=None, device=0):
@@ -384,6 +420,167 @@ class WaveletTree:
 
     def generateQueries(self, kind, seed, m, d_out, stream=None):
         call("ecl_wavelet_tree_generate_queries", self._h, kind, seed, m, d_out.ptr, stream)
+
+
+class Forest:
+    """Many trees over uint8 texts built together (ecl_forest_create): text
+    i is d_texts[starts[i] : starts[i] + lengths[i]] (starts multiples of 16)
+    over the alphabet 0 .. sigmas[i]-1."""
+
+    def __init__(self, d_texts, starts, lengths, sigmas, device=0, tile_rows=0):
+        starts = np.ascontiguousarray(starts, dtype=np.uint64)
+        lengths = np.ascontiguousarray(lengths, dtype=np.uint64)
+        sigmas = np.ascontiguousarray(sigmas, dtype=np.uint64)
+        assert starts.size == lengths.size == sigmas.size
+        h = C.c_void_p()
+        call("ecl_forest_create", d_texts.ptr, _p(starts), _p(lengths), _p(sigmas), sigmas.size,
+             device, tile_rows, C.byref(h))
+        self._h = h
+        self.device = device
+        self.num_trees = int(sigmas.size)
+
+    def __del__(self):
+        try:
+            if self._h:
+                call("ecl_forest_destroy", self._h)
+                self._h = C.c_void_p()
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass
+
+    def treeInfo(self, i):
+        n, s = C.c_uint64(), C.c_uint64()
+        L = C.c_uint32()
+        call("ecl_forest_tree_info", self._h, i, C.byref(n), C.byref(s), C.byref(L))
+        return n.value, s.value, L.value
+
+    def queryDevice(self, d_q, m, d_out, stream=None):
+        call("ecl_forest_query_device", self._h, d_q.ptr, m, d_out.ptr, stream)
+
+    def generateQueries(self, qoff, seed, d_out, stream=None):
+        qoff = np.ascontiguousarray(qoff, dtype=np.uint64)
+        call("ecl_forest_generate_queries", self._h, _p(qoff), seed, d_out.ptr, stream)
+
+    def autotune(self, d_q, m, d_out):
+        b, g = C.c_uint32(), C.c_uint32()
+        call("ecl_forest_autotune", self._h, d_q.ptr, m, d_out.ptr, C.byref(b), C.byref(g))
+        return b.value, g.value
+
+    def graph(self, d_q, m, d_out):
+        return Graph(self, d_q, m, d_out)
+
+
+class Graph:
+    """A forest batch launch captured into a HIP graph (ecl_forest_graph_create)."""
+
+    def __init__(self, forest, d_q, m, d_out):
+        h = C.c_void_p()
+        call("ecl_forest_graph_create", forest._h, d_q.ptr, m, d_out.ptr, C.byref(h))
+        self._h = h
+        self._keep = (forest, d_q, d_out)
+
+    def launch(self, stream=None):
+        call("ecl_graph_launch", self._h, stream)
+
+    def __del__(self):
+        try:
+            if self._h:
+                call("ecl_graph_destroy", self._h)
+                self._h = C.c_void_p()
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass
+
+
+class _ServerConfig(C.Structure):
+    _fields_ = [("hbm_budget", C.c_uint64), ("max_batch", C.c_uint64),
+                ("max_inflight", C.c_uint32), ("max_trees", C.c_uint32),
+                ("coalesce", C.c_uint32), ("reserved", C.c_uint32)]
+
+
+class _ServerStats(C.Structure):
+    _fields_ = [("requests", C.c_uint64), ("queries", C.c_uint64), ("batches", C.c_uint64),
+                ("p50_ms", C.c_double), ("p99_ms", C.c_double), ("mean_ms", C.c_double),
+                ("max_ms", C.c_double), ("elapsed_s", C.c_double),
+                ("queries_per_s", C.c_double), ("hbm_used", C.c_uint64),
+                ("trees", C.c_uint64)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+class _LoadSpec(C.Structure):
+    _fields_ = [("rate", C.c_double), ("seconds", C.c_double), ("seed", C.c_uint64),
+                ("min_size", C.c_uint64), ("max_size", C.c_uint64), ("pool", C.c_uint64)]
+
+
+class Server:
+    """Query server with continuous batching (ecl_server_*): resident trees
+    admitted by an HBM budget, requests coalesced into device batches."""
+
+    def __init__(self, device=0, hbm_budget=0, max_batch=0, max_inflight=0, max_trees=0,
+                 coalesce=True):
+        cfg = _ServerConfig(hbm_budget, max_batch, max_inflight, max_trees,
+                            1 if coalesce else 0, 0)
+        h = C.c_void_p()
+        call("ecl_server_create", device, C.byref(cfg), C.byref(h))
+        self._h = h
+        self._keep = {}
+
+    def close(self):
+        if self._h:
+            call("ecl_server_destroy", self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:  # noqa: BLE001 - interpreter shutdown
+            pass
+
+    def addTree(self, text, sigma, n=None):
+        """text: numpy uint8 array (host) or DeviceBuffer (then pass n)."""
+        t = C.c_uint32()
+        if isinstance(text, DeviceBuffer):
+            call("ecl_server_add_tree", self._h, text.ptr, 1, n, sigma, C.byref(t))
+        else:
+            text = np.ascontiguousarray(text, dtype=np.uint8)
+            call("ecl_server_add_tree", self._h, _p(text), 0, text.size, sigma, C.byref(t))
+        return t.value
+
+    def removeTree(self, tree):
+        call("ecl_server_remove_tree", self._h, tree)
+
+    def submit(self, tree, queries):
+        """queries: MIXED_DTYPE array; returns a request id (the arrays are
+        kept alive until wait())."""
+        q = np.ascontiguousarray(queries, dtype=MIXED_DTYPE)
+        out = np.zeros(q.size, dtype=np.uint64)
+        rid = C.c_uint64()
+        call("ecl_server_submit", self._h, tree, _p(q), q.size, _p(out), C.byref(rid))
+        self._keep[rid.value] = (q, out)
+        return rid.value
+
+    def wait(self, rid):
+        lat = C.c_double()
+        call("ecl_server_wait", self._h, rid, C.byref(lat))
+        _, out = self._keep.pop(rid)
+        return out, lat.value
+
+    def setMode(self, coalesce):
+        call("ecl_server_set_mode", self._h, 1 if coalesce else 0)
+
+    def stats(self):
+        st = _ServerStats()
+        call("ecl_server_stats_get", self._h, C.byref(st))
+        return st.as_dict()
+
+    def resetStats(self):
+        call("ecl_server_stats_reset", self._h)
+
+    def load(self, rate, seconds, seed=1, min_size=1, max_size=1_000_000, pool=0):
+        spec = _LoadSpec(rate, seconds, seed, min_size, max_size, pool)
+        st = _ServerStats()
+        call("ecl_server_load", self._h, C.byref(spec), C.byref(st))
+        return st.as_dict()
 
 
 class Profiler:

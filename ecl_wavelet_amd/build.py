@@ -15,7 +15,8 @@ ROOT = os.path.dirname(PKG)
 CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(PKG, "libecl_wavelet_hip.so")
-SOURCES = ["capi.hip", "kernels_build.hip", "kernels_tree.hip", "kernels_query.hip"]
+SOURCES = ["capi.hip", "forest.hip", "server.hip", "kernels_build.hip", "kernels_tree.hip",
+           "kernels_query.hip"]
 ARCH = os.environ.get("ECL_OFFLOAD_ARCH", "gfx950")
 
 

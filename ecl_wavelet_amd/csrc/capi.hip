@@ -11,7 +11,7 @@
 #include <string>
 #include <vector>
 
-#include "internal.hpp"
+#include "host.hpp"
 
 using ecl::detail::LevelView;
 using ecl::detail::TreeView;
@@ -25,6 +25,7 @@ namespace ecl {
 namespace lib {
 
 static thread_local std::string g_last_error;
+std::string& last_error() { return g_last_error; }
 
 void hip_check(hipError_t e, const char* what) {
   if (e != hipSuccess)
@@ -57,7 +58,8 @@ void DevBuf::reset() {
 
 static const char* kKernelNames[K_COUNT] = {
     "histogram", "wt_pass", "rs_build", "wt_access", "wt_rank", "wt_select",
-    "wt_mixed", "rs_rank", "rs_select", "generate", "fill", "tile_hist", "tile_scan"};
+    "wt_mixed", "rs_rank", "rs_select", "generate", "fill", "tile_hist", "tile_scan",
+    "forest_query"};
 const char* kernel_name(int id) { return id >= 0 && id < K_COUNT ? kKernelNames[id] : "?"; }
 
 struct ProfRecord { int id; hipEvent_t e0, e1; };
@@ -69,6 +71,8 @@ static double g_prof_ms[K_COUNT];
 
 ProfScope::ProfScope(int i, hipStream_t s) : id(i), stream(s) {
   if (!g_prof_on) return;
+  hipStreamCaptureStatus cap = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s, &cap) != hipSuccess || cap != hipStreamCaptureStatusNone) return;
   if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
     e0 = e1 = nullptr;
     return;
@@ -104,28 +108,10 @@ static void prof_drain() {
 }  // namespace lib
 }  // namespace ecl
 
-template <typename F>
-static ecl_status guard(F&& f) {
-  try {
-    f();
-    return ECL_OK;
-  } catch (const Error& e) {
-    g_last_error = e.what();
-    return e.status;
-  } catch (const std::bad_alloc&) {
-    g_last_error = "host allocation failed";
-    return ECL_ERROR_INTERNAL;
-  } catch (const std::exception& e) {
-    g_last_error = e.what();
-    return ECL_ERROR_INTERNAL;
-  }
-}
+namespace ecl {
+namespace lib {
 
-static void require(bool ok, const char* msg) {
-  if (!ok) throw Error(ECL_ERROR_INVALID_ARGUMENT, msg);
-}
-
-static void set_device(int device) {
+void set_device(int device) {
   int n = 0;
   if (hipGetDeviceCount(&n) != hipSuccess || n == 0) {
     (void)hipGetLastError();
@@ -135,22 +121,16 @@ static void set_device(int device) {
   ECL_HIP(hipSetDevice(device));
 }
 
-static hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
-
-template <typename T>
-static void h2d(void* dst, const T* src, size_t count) {
-  if (count) ECL_HIP(hipMemcpy(dst, src, count * sizeof(T), hipMemcpyHostToDevice));
-}
-template <typename T>
-static void d2h(T* dst, const void* src, size_t count) {
-  if (count) ECL_HIP(hipMemcpy(dst, src, count * sizeof(T), hipMemcpyDeviceToHost));
-}
-
-static size_t free_device_memory() {
+size_t free_device_memory() {
   size_t fr = 0, tot = 0;
   ECL_HIP(hipMemGetInfo(&fr, &tot));
   return fr;
 }
+
+}  // namespace lib
+}  // namespace ecl
+
+static hipStream_t as_stream(void* s) { return static_cast<hipStream_t>(s); }
 
 // ===========================================================================
 // objects
@@ -346,31 +326,39 @@ static void finalize_rank_select(ecl_rank_select_t* rs, ecl_bit_array_t* ba, hip
   rs->ba = ba;
 }
 
-// rs_build pass of every non-empty array of `ba` into the directory of `rs`.
+// Directory of every non-empty array of `ba` into `rs`, one launch.
 static void build_directories(ecl_rank_select_t* rs, ecl_bit_array_t* ba, hipStream_t s) {
   const size_t na = ba->sizes.size();
-  uint64_t max_tiles = 0;
-  for (size_t i = 0; i < na; ++i) max_tiles = std::max(max_tiles, rs_build_tiles(rs->num_l1[i]));
-  DevBuf status((max_tiles + 2) * sizeof(uint64_t));
+  std::vector<RsArrayDesc> desc;
+  std::vector<uint32_t> tile_array;
   for (size_t i = 0; i < na; ++i) {
     if (ba->sizes[i] == 0) continue;
-    const uint64_t tiles = rs_build_tiles(rs->num_l1[i]);
-    launch_fill_u64(status.as<uint64_t>(), tiles + 2, 0, s);
-    RsBuildParams p{};
-    p.words64 = reinterpret_cast<const uint64_t*>(ba->words() + ba->offsets[i]);
-    p.m = ba->sizes[i];
-    p.l1 = rs->l1.as<uint64_t>() + rs->l1_off[i];
-    p.l2 = rs->l2.as<uint16_t>() + rs->l2_off[i];
-    p.num_l1 = rs->num_l1[i];
-    p.num_l2 = rs->num_l2[i];
-    p.samples = rs->samples.as<uint64_t>() + rs->s_off[i];
-    p.cap = rs->cap[i];
-    p.total_ones = rs->total_ones.as<uint64_t>() + i;
-    p.status = status.as<uint64_t>();
-    p.counter = status.as<uint64_t>() + tiles + 1;
-    launch_rs_build(p, s);
+    RsArrayDesc d{};
+    d.words64 = reinterpret_cast<const uint64_t*>(ba->words() + ba->offsets[i]);
+    d.m = ba->sizes[i];
+    d.l1 = rs->l1.as<uint64_t>() + rs->l1_off[i];
+    d.l2 = rs->l2.as<uint16_t>() + rs->l2_off[i];
+    d.num_l1 = rs->num_l1[i];
+    d.num_l2 = rs->num_l2[i];
+    d.samples = rs->samples.as<uint64_t>() + rs->s_off[i];
+    d.cap = rs->cap[i];
+    d.total_ones = rs->total_ones.as<uint64_t>() + i;
+    d.first_tile = tile_array.size();
+    tile_array.insert(tile_array.end(), rs_build_tiles(d.num_l1),
+                      static_cast<uint32_t>(desc.size()));
+    desc.push_back(d);
   }
-  // the status buffer is freed on return: finish with it first
+  if (desc.empty()) return;
+  const uint64_t tiles = tile_array.size();
+  DevBuf d_desc(desc.size() * sizeof(RsArrayDesc));
+  DevBuf d_tiles(tiles * sizeof(uint32_t));
+  DevBuf status((tiles + 1) * sizeof(uint64_t));
+  h2d(d_desc.ptr, desc.data(), desc.size());
+  h2d(d_tiles.ptr, tile_array.data(), tiles);
+  launch_fill_u64(status.as<uint64_t>(), tiles + 1, 0, s);
+  launch_rs_build(d_desc.as<RsArrayDesc>(), d_tiles.as<uint32_t>(), tiles, status.as<uint64_t>(),
+                  status.as<uint64_t>() + tiles, s);
+  // the buffers are freed on return: finish with them first
   ECL_HIP(hipStreamSynchronize(s));
 }
 
@@ -391,9 +379,9 @@ static ecl_rank_select_t* make_rank_select(ecl_bit_array_t* ba, hipStream_t s) {
 // ===========================================================================
 // WaveletTree: host-side shape (codes, nodes, node tables)
 // ===========================================================================
+namespace ecl {
+namespace lib {
 namespace {
-
-struct Node { uint64_t lo, hi; };
 
 // Minimal codes as the leaves of the left-complete tree: a node of s >= 2
 // symbols splits after prev_pow2(s) symbols (the shape
@@ -428,6 +416,63 @@ uint64_t codes_start_of(uint64_t sigma, uint32_t L) {
   }
   return sigma - num;
 }
+
+}  // namespace
+
+TreeShape tree_shape(uint64_t sigma) {
+  TreeShape sh;
+  sh.sigma = sigma;
+  sh.L = detail::ceil_log2(sigma);
+  sh.pow2 = detail::is_pow2(sigma);
+  sh.code.resize(sigma);
+  sh.len.resize(sigma);
+  sh.nodes.resize(sh.L + 1);
+  assign_codes(0, sigma - 1, 0, 0, sh.L, sh.code, sh.len, sh.nodes);
+  sh.codes_start = codes_start_of(sigma, sh.L);
+  return sh;
+}
+
+// node ranks: zeros of level l before each node start (precomputeRanksKernel,
+// wavelet_tree.cuh:332-352, over the node starts of :2501-2532)
+TreeMeta tree_meta(const TreeShape& sh, const std::vector<uint64_t>& sym_count, uint64_t n) {
+  TreeMeta m;
+  const uint64_t sigma = sh.sigma;
+  const uint32_t L = sh.L;
+  m.counts.resize(sigma);
+  uint64_t acc = 0;
+  for (uint64_t c = 0; c < sigma; ++c) { m.counts[c] = acc; acc += sym_count[c]; }
+  if (acc != n) throw Error(ECL_ERROR_INTERNAL, "histogram does not sum to n");
+  auto C = [&](uint64_t c) { return c >= sigma ? n : m.counts[c]; };
+  m.nodes_at_level.assign(L > 1 ? L - 1 : 0, 0);
+  for (uint32_t l = 0; l < L; ++l) {
+    uint64_t zb = 0;  // zeros of level l before the node
+    for (const Node& nd : sh.nodes[l]) {
+      const uint64_t mid = nd.lo + detail::prev_pow2(nd.hi - nd.lo + 1);
+      if (l >= 1 && nd.lo >= 1) {
+        m.ranks.push_back(zb);
+        m.node_start.push_back(nd.lo);
+        m.node_level.push_back(static_cast<uint8_t>(l));
+        if (l + 1 < L) m.nodes_at_level[l]++;
+      }
+      zb += C(mid) - C(nd.lo);
+    }
+  }
+  for (uint64_t x : m.nodes_at_level) m.nodes_until_last_level += x;
+  return m;
+}
+
+std::vector<uint64_t> level_sizes(const TreeShape& sh, const std::vector<uint64_t>& sym_count) {
+  std::vector<uint64_t> out(sh.L, 0);
+  for (uint32_t l = 0; l < sh.L; ++l)
+    for (uint64_t c = 0; c < sh.sigma; ++c)
+      if (sh.len[c] > l) out[l] += sym_count[c];
+  return out;
+}
+
+}  // namespace lib
+}  // namespace ecl
+
+namespace {
 
 template <typename T>
 std::vector<uint64_t> to_u64(const void* a, size_t n) {
@@ -489,7 +534,7 @@ uint64_t tree_needed_bytes(uint64_t n, uint64_t sigma, uint32_t L, int esize, bo
                              (n / 4096) * 8 + 64;
   const uint64_t codes = n * b.cw + 4 * sigma * 16;
   const uint64_t buffers = (b.npasses > 1 ? 2 : 1) * codes;
-  const uint64_t tiles = n / pass_tile_codes(b.cw) + 2 * sigma + 2;
+  const uint64_t tiles = n / pass_tile_codes(b.cw, 4) + 2 * sigma + 2;
   const uint64_t scan = tiles * 256 * 12 + (tiles / 256 + 1) * 256 * 8;
   return per_level * L + buffers + scan + (host_text ? n * esize : 0) + sigma * 64 +
          (n / 4096 + 4096) * 8 + 10'000'000ull * 16;
@@ -561,18 +606,18 @@ static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_
       wt->map_kind = esize <= 2 ? 1 : 2;
 
     // ---- codes and nodes
-    std::vector<uint64_t> code(sigma);
-    std::vector<uint8_t> len(sigma);
-    std::vector<std::vector<Node>> nodes(L + 1);
-    assign_codes(0, sigma - 1, 0, 0, L, code, len, nodes);
-    wt->codes_start = codes_start_of(sigma, L);
+    const TreeShape shape = tree_shape(sigma);
+    const std::vector<uint64_t>& code = shape.code;
+    const std::vector<uint8_t>& len = shape.len;
+    wt->codes_start = shape.codes_start;
     for (uint64_t c = wt->codes_start; c < sigma; ++c) {
       wt->codes.push_back(code[c]);
       wt->code_lens.push_back(len[c]);
     }
     const BuildPlan plan = plan_of(L);
     const int cw = plan.cw;
-    const uint64_t T = pass_tile_codes(cw);   // codes per tile
+    const uint32_t rows = 4;
+    const uint64_t T = pass_tile_codes(cw, rows);   // codes per tile
     const uint64_t E = 16 / cw;               // codes per 16 bytes
 
     // ---- memory admission (wavelet_tree.cuh:644-689)
@@ -663,6 +708,8 @@ static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_
     pp.pow2 = wt->pow2 ? 1 : 0;
     pp.code_table = wt->d_codes.as<uint64_t>();
     pp.cw = cw;
+    pp.tile_rows = rows;
+    pp.tile_codes = static_cast<uint32_t>(T);
     DevBuf d_err(4);
     launch_fill_u32(d_err.as<uint32_t>(), 1, 0, s);
     pp.err = d_err.as<uint32_t>();
@@ -698,12 +745,7 @@ static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_
       uint64_t tot = 0;
       for (uint64_t c = 0; c < sigma; ++c) tot += sym_count[c];
       if (tot != n) throw Error(ECL_ERROR_INTERNAL, "histogram does not sum to n");
-      for (uint32_t l = 0; l < L; ++l) {
-        uint64_t x = 0;
-        for (uint64_t c = 0; c < sigma; ++c)
-          if (len[c] > l) x += sym_count[c];
-        level_size[l] = x;
-      }
+      level_size = level_sizes(shape, sym_count);
     }
     ecl_bit_array_t* ba = make_bit_array(level_size, 0, device);
     std::unique_ptr<ecl_bit_array_t> ba_guard(ba);
@@ -863,28 +905,15 @@ static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_
     buf_v.reset();
 
     // ---- counts, node ranks, nodes per level (host metadata)
-    wt->counts.resize(sigma);
     {
-      uint64_t acc = 0;
-      for (uint64_t c = 0; c < sigma; ++c) { wt->counts[c] = acc; acc += sym_count[c]; }
-      if (acc != n) throw Error(ECL_ERROR_INTERNAL, "histogram does not sum to n");
+      TreeMeta meta = tree_meta(shape, sym_count, n);
+      wt->counts = std::move(meta.counts);
+      wt->ranks = std::move(meta.ranks);
+      wt->node_start = std::move(meta.node_start);
+      wt->node_level = std::move(meta.node_level);
+      wt->nodes_at_level = std::move(meta.nodes_at_level);
+      wt->nodes_until_last_level = meta.nodes_until_last_level;
     }
-    auto C = [&](uint64_t c) { return c >= sigma ? n : wt->counts[c]; };
-    wt->nodes_at_level.assign(L > 1 ? L - 1 : 0, 0);
-    for (uint32_t l = 0; l < L; ++l) {
-      uint64_t zb = 0;  // zeros of level l before the node
-      for (const Node& nd : nodes[l]) {
-        const uint64_t mid = nd.lo + detail::prev_pow2(nd.hi - nd.lo + 1);
-        if (l >= 1 && nd.lo >= 1) {
-          wt->ranks.push_back(zb);
-          wt->node_start.push_back(nd.lo);
-          wt->node_level.push_back(static_cast<uint8_t>(l));
-          if (l + 1 < L) wt->nodes_at_level[l]++;
-        }
-        zb += C(mid) - C(nd.lo);
-      }
-    }
-    for (uint64_t x : wt->nodes_at_level) wt->nodes_until_last_level += x;
 
     finalize_rank_select(rs, ba, s);
     ba_guard.release();

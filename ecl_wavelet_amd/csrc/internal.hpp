@@ -47,7 +47,7 @@ struct DevBuf {
 enum KernelId : int {
   K_HIST = 0, K_PASS, K_RS_BUILD, K_WT_ACCESS, K_WT_RANK, K_WT_SELECT,
   K_WT_MIXED, K_RS_RANK, K_RS_SELECT, K_GEN, K_FILL, K_TILE_HIST, K_TILE_SCAN,
-  K_COUNT
+  K_FOREST_QUERY, K_COUNT
 };
 const char* kernel_name(int id);
 struct ProfScope {
@@ -85,6 +85,8 @@ struct PassParams {
   const void* codes; int cw;
   uint32_t k, dshift;         // digit = (code >> dshift) & (2^k - 1)
   uint64_t n;                 // uniform tiles over [0, n) when tile_start == nullptr
+  uint32_t tile_rows;         // 16-byte chunks per thread: 1, 2 or 4
+  uint32_t tile_codes;        // codes per tile = 512 * tile_rows * 16 / cw
   uint64_t num_tiles;
   const uint64_t* tile_start; const uint32_t* tile_len; const uint32_t* tile_node;
   const uint32_t* node_first_tile;   // [nodes + 1], nullptr for uniform tiles
@@ -100,8 +102,18 @@ struct PassParams {
   uint64_t level_word[8];     // word offset of levels l0 .. l0+k-1
   uint64_t level_nwords[8];   // words of each of those levels
   uint32_t* err;              // set to 1 if a write fell outside its array
+  // Batched construction of independent trees (one node per text, single
+  // pass): per-node levels, level words and symbol->code tables.  nullptr
+  // entries mean "the same for every node" (the fields above).
+  const uint8_t* node_k;      // [nodes] levels of the node's tree
+  const uint64_t* node_lw;    // [nodes][8] word offsets of its levels
+  const uint64_t* node_lnw;   // [nodes][8] words of its levels
+  const uint8_t* node_lut;    // [nodes][256] symbol -> code, for remapped nodes
+  const uint8_t* node_remap;  // [nodes] 1: codes come from codes_out
+  uint32_t row_bins;          // entries per hist/excl row (0: 2^k)
+  uint32_t thr_stride;        // entries per node in thr (0: k + 1)
 };
-uint64_t pass_tile_codes(int cw);                   // codes per tile
+uint64_t pass_tile_codes(int cw, uint32_t rows);    // codes per tile
 uint64_t scan_partial_entries(uint64_t tiles, uint32_t k);
 void launch_tile_hist(const PassParams& p, hipStream_t s);
 void launch_tile_scan(const PassParams& p, hipStream_t s);
@@ -109,15 +121,19 @@ void launch_pass(const PassParams& p, hipStream_t s);
 void launch_gather_rows(const uint64_t* excl, const uint32_t* rows, uint64_t nrows,
                         uint32_t nb, uint64_t* out, hipStream_t s);
 
-struct RsBuildParams {
+// One bit array of a directory build: its words, its directory arrays and
+// the first global tile of its look-back chain.
+struct RsArrayDesc {
   const uint64_t* words64; uint64_t m;
   uint64_t* l1; uint16_t* l2; uint64_t num_l1, num_l2;
   uint64_t* samples; uint64_t cap;
   uint64_t* total_ones;
-  uint64_t* status; uint64_t* counter;  // lookback state (zeroed)
+  uint64_t first_tile;
 };
-uint64_t rs_build_tiles(uint64_t num_l1);  // look-back tiles (status: tiles + 2 u64)
-void launch_rs_build(const RsBuildParams& p, hipStream_t s);
+uint64_t rs_build_tiles(uint64_t num_l1);  // tiles of one array's chain
+// status: tiles + 1 zeroed u64 (the last one is the tile counter)
+void launch_rs_build(const RsArrayDesc* arrays, const uint32_t* tile_array, uint64_t tiles,
+                     uint64_t* status, uint64_t* counter, hipStream_t s);
 
 void launch_gen_text(void* out, int esize, uint64_t n, uint64_t seed,
                      uint64_t sigma, const uint64_t* thresholds,
@@ -147,6 +163,15 @@ void launch_tree_mixed(const TreeQueryParams& p, const ecl_query* q,
 void launch_tree_gen_queries(const TreeView& v, const uint64_t* hcounts_dev,
                              int kind, uint64_t seed, uint64_t m, void* out,
                              int esize, hipStream_t s);
+void launch_forest_query(const TreeView* views, uint64_t num_trees, const ecl_forest_query* q,
+                         uint64_t m, uint64_t* out, uint32_t block, uint32_t grid,
+                         hipStream_t s);
+void launch_segment_query(const TreeView* views, uint64_t num_views, const ecl_query* q,
+                          uint64_t m, const uint64_t* seg_start, const uint32_t* seg_tree,
+                          uint32_t nseg, uint64_t* out, uint32_t block, uint32_t grid,
+                          hipStream_t s);
+void launch_forest_gen(const TreeView* views, uint64_t num_trees, const uint64_t* qoff,
+                       uint64_t seed, uint64_t m, ecl_forest_query* out, hipStream_t s);
 void launch_rs_rank(const LevelView* lv, int value, const uint64_t* idx,
                     uint64_t m, uint64_t* out, int generated, uint64_t seed,
                     hipStream_t s);

@@ -40,11 +40,15 @@ __device__ __forceinline__ uint64_t status_load(const uint64_t* p) {
 // status-load latency, so the window is wide and the tiles are large.
 // Tiles take their ids in launch order from an atomic counter: every
 // predecessor is resident or finished, so the wait ends.
+// Tiles of several independent chains may share one status array: the
+// first tile of each chain publishes its inclusive prefix at once, so every
+// later tile of the chain meets an inclusive prefix before leaving it.
 template <int J>
-__device__ uint64_t lookback(uint64_t* status, uint64_t tile, uint64_t agg) {
+__device__ uint64_t lookback(uint64_t* status, uint64_t tile, uint64_t agg,
+                             uint64_t first = 0) {
   const uint32_t lane = lane_id();
-  if (tile == 0) {
-    if (lane == 0) status_store(&status[0], kFlagPre | agg);
+  if (tile == first) {
+    if (lane == 0) status_store(&status[tile], kFlagPre | agg);
     return 0;
   }
   if (lane == 0) status_store(&status[tile], kFlagAgg | agg);
@@ -175,19 +179,25 @@ __global__ __launch_bounds__(256) void histogram_kernel(HistParams p,
 // ---------------------------------------------------------------------------
 // RankSelect directory: 8 L1 blocks (8 x 1024 u64 words) per workgroup of
 // 512 threads; wave w owns L1 block 8 * tile + w, 16 words per lane.  One
-// look-back per workgroup gives the ones before its first L1 block.
+// look-back per workgroup gives the ones before its first L1 block.  One
+// launch covers any number of bit arrays: global tile t belongs to array
+// tile_array[t] and is tile t - first_tile of that array's chain.
 constexpr uint32_t kRsBlocksPerTile = 8;
 
-__global__ __launch_bounds__(512) void rs_build_kernel(RsBuildParams p) {
+__global__ __launch_bounds__(512) void rs_build_kernel(const RsArrayDesc* __restrict__ arrays,
+                                                       const uint32_t* __restrict__ tile_array,
+                                                       uint64_t* status, uint64_t* counter) {
   __shared__ uint64_t s_tile;
   __shared__ uint64_t s_excl;
   __shared__ uint32_t s_wave[kRsBlocksPerTile];
   const uint32_t tid = threadIdx.x;
   const uint32_t lane = tid & 63, wave = tid >> 6;
   if (tid == 0)
-    s_tile = atomicAdd(reinterpret_cast<unsigned long long*>(p.counter), 1ull);
+    s_tile = atomicAdd(reinterpret_cast<unsigned long long*>(counter), 1ull);
   __syncthreads();
-  const uint64_t tile = s_tile;
+  const uint64_t gtile = s_tile;
+  const RsArrayDesc p = arrays[tile_array[gtile]];
+  const uint64_t tile = gtile - p.first_tile;
   const uint64_t kb = tile * kRsBlocksPerTile + wave;  // this wave's L1 block
   const uint64_t nwords = (p.m + 63) / 64;
   const uint64_t w0 = kb * 1024 + static_cast<uint64_t>(lane) * 16;
@@ -231,7 +241,7 @@ __global__ __launch_bounds__(512) void rs_build_kernel(RsBuildParams p) {
     ttot += x;
   }
   if (wave == 0) {
-    const uint64_t g = lookback<kLookbackWords>(p.status, tile, ttot);
+    const uint64_t g = lookback<kLookbackWords>(status, gtile, ttot, p.first_tile);
     if (lane == 0) s_excl = g;
   }
   __syncthreads();
@@ -403,12 +413,13 @@ uint64_t rs_build_tiles(uint64_t num_l1) {
   return (num_l1 + kRsBlocksPerTile - 1) / kRsBlocksPerTile;
 }
 
-void launch_rs_build(const RsBuildParams& p, hipStream_t s) {
-  if (p.m == 0) return;
-  if (p.num_l1 > 0x7FFFFFFFull) throw Error(ECL_ERROR_INVALID_ARGUMENT, "bit array too large");
+void launch_rs_build(const RsArrayDesc* arrays, const uint32_t* tile_array, uint64_t tiles,
+                     uint64_t* status, uint64_t* counter, hipStream_t s) {
+  if (tiles == 0) return;
+  if (tiles > 0x7FFFFFFFull) throw Error(ECL_ERROR_INVALID_ARGUMENT, "bit arrays too large");
   ProfScope ps(K_RS_BUILD, s);
-  hipLaunchKernelGGL(rs_build_kernel, dim3(static_cast<uint32_t>(rs_build_tiles(p.num_l1))),
-                     dim3(512), 0, s, p);
+  hipLaunchKernelGGL(rs_build_kernel, dim3(static_cast<uint32_t>(tiles)), dim3(512), 0, s,
+                     arrays, tile_array, status, counter);
   ECL_HIP(hipGetLastError());
 }
 

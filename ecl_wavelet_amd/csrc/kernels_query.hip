@@ -219,6 +219,114 @@ __global__ void tree_gen_kernel(TreeView t, int kind, uint64_t seed,
   }
 }
 
+// ---------------------------------------------------------------------------
+// Batched queries over many trees: query q names its tree; the tree's view
+// (levels, counts, node ranks) is read from global memory, where the
+// metadata of all trees stays in the L2 / MALL.  A grid of a few
+// workgroups per CU strides over the batch.
+__global__ void forest_query_kernel(const TreeView* __restrict__ views, uint64_t num_trees,
+                                    const ecl_forest_query* __restrict__ qs, uint64_t m,
+                                    uint64_t* __restrict__ out) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t q = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+       q < m; q += stride) {
+    const ecl_forest_query e = qs[q];
+    uint64_t r = 0;
+    if (e.tree < num_trees) {
+      const TreeView t = views[e.tree];
+      if (e.kind == 0) {
+        r = e.arg < t.n ? detail::tree_access(t, t.levels, t.counts, t.ranks,
+                                              t.nodes_at_level, e.arg)
+                        : 0;
+      } else if (e.kind == 1) {
+        r = e.arg > t.n ? 0
+                        : detail::tree_rank(t, t.levels, t.counts, t.ranks,
+                                            t.nodes_at_level, e.symbol, e.arg);
+      } else {
+        r = detail::tree_select(t, t.levels, t.counts, t.ranks, t.nodes_at_level,
+                                e.symbol, e.arg);
+      }
+    }
+    out[q] = r;
+  }
+}
+
+// A served batch: requests' queries back to back, segment j (queries from
+// seg_start[j] on) belonging to the tree in view slot seg_tree[j].
+__global__ void segment_query_kernel(const TreeView* __restrict__ views, uint64_t num_views,
+                                     const ecl_query* __restrict__ qs, uint64_t m,
+                                     const uint64_t* __restrict__ seg_start,
+                                     const uint32_t* __restrict__ seg_tree, uint32_t nseg,
+                                     uint64_t* __restrict__ out) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t q = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+       q < m; q += stride) {
+    uint32_t lo = 0, hi = nseg;  // last segment with seg_start <= q
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      if (seg_start[mid] <= q) lo = mid; else hi = mid;
+    }
+    const uint32_t tree = seg_tree[lo];
+    const ecl_query e = qs[q];
+    uint64_t r = 0;
+    if (tree < num_views) {
+      const TreeView t = views[tree];
+      if (e.kind == 0) {
+        r = e.arg < t.n ? detail::tree_access(t, t.levels, t.counts, t.ranks,
+                                              t.nodes_at_level, e.arg)
+                        : 0;
+      } else if (e.kind == 1) {
+        r = e.arg > t.n ? 0
+                        : detail::tree_rank(t, t.levels, t.counts, t.ranks,
+                                            t.nodes_at_level, e.symbol, e.arg);
+      } else {
+        r = detail::tree_select(t, t.levels, t.counts, t.ranks, t.nodes_at_level,
+                                e.symbol, e.arg);
+      }
+    }
+    out[q] = r;
+  }
+}
+
+// Mixed queries for a forest: queries [qoff[i], qoff[i+1]) go to tree i;
+// kind q % 3, fields drawn like tree_gen_kernel's.
+__global__ void forest_gen_kernel(const TreeView* __restrict__ views, uint64_t num_trees,
+                                  const uint64_t* __restrict__ qoff, uint64_t seed, uint64_t m,
+                                  ecl_forest_query* __restrict__ out) {
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+  for (uint64_t q = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+       q < m; q += stride) {
+    uint64_t lo = 0, hi = num_trees;  // last tree with qoff[tree] <= q
+    while (hi - lo > 1) {
+      const uint64_t mid = (lo + hi) >> 1;
+      if (qoff[mid] <= q) lo = mid; else hi = mid;
+    }
+    const TreeView t = views[lo];
+    const uint64_t h0 = detail::hash64(seed, q);
+    const uint64_t h1 = detail::hash64(seed + kField, q);
+    const uint64_t h2 = detail::hash64(seed + 2 * kField, q);
+    const uint32_t k = static_cast<uint32_t>(q % 3);
+    const uint64_t c = detail::bounded64(h1, t.sigma);
+    uint64_t arg;
+    if (k == 0) {
+      arg = detail::bounded64(h0, t.n);
+    } else if (k == 1) {
+      arg = detail::bounded64(h2, t.n);
+    } else {
+      const uint64_t cnt = symbol_count(t, c);
+      arg = cnt ? 1 + detail::bounded64(h2, cnt) : 1;
+    }
+    ecl_forest_query e;
+    e.arg = arg;
+    e.symbol = static_cast<uint32_t>(k == 0 ? 0 : c);
+    e.kind = static_cast<uint16_t>(k);
+    e.reserved = 0;
+    e.tree = static_cast<uint32_t>(lo);
+    e.reserved2 = 0;
+    out[q] = e;
+  }
+}
+
 template <int Value>
 __global__ __launch_bounds__(256) void rs_rank_kernel(
     const LevelView* lvp, const uint64_t* __restrict__ idx, uint64_t m,
@@ -317,6 +425,36 @@ void launch_tree_mixed(const TreeQueryParams& p, const ecl_query* q,
   ProfScope ps(K_WT_MIXED, s);
   hipLaunchKernelGGL(tree_mixed_kernel, dim3(query_grid(m)), dim3(256),
                      meta_bytes(p.view), s, p, q, m, out);
+  ECL_HIP(hipGetLastError());
+}
+
+void launch_forest_query(const TreeView* views, uint64_t num_trees, const ecl_forest_query* q,
+                         uint64_t m, uint64_t* out, uint32_t block, uint32_t grid,
+                         hipStream_t s) {
+  if (m == 0) return;
+  ProfScope ps(K_FOREST_QUERY, s);
+  hipLaunchKernelGGL(forest_query_kernel, dim3(grid), dim3(block), 0, s, views, num_trees, q,
+                     m, out);
+  ECL_HIP(hipGetLastError());
+}
+
+void launch_segment_query(const TreeView* views, uint64_t num_views, const ecl_query* q,
+                          uint64_t m, const uint64_t* seg_start, const uint32_t* seg_tree,
+                          uint32_t nseg, uint64_t* out, uint32_t block, uint32_t grid,
+                          hipStream_t s) {
+  if (m == 0) return;
+  ProfScope ps(K_FOREST_QUERY, s);
+  hipLaunchKernelGGL(segment_query_kernel, dim3(grid), dim3(block), 0, s, views, num_views, q, m,
+                     seg_start, seg_tree, nseg, out);
+  ECL_HIP(hipGetLastError());
+}
+
+void launch_forest_gen(const TreeView* views, uint64_t num_trees, const uint64_t* qoff,
+                       uint64_t seed, uint64_t m, ecl_forest_query* out, hipStream_t s) {
+  if (m == 0) return;
+  ProfScope ps(K_GEN, s);
+  hipLaunchKernelGGL(forest_gen_kernel, dim3(query_grid(m)), dim3(256), 0, s, views, num_trees,
+                     qoff, seed, m, out);
   ECL_HIP(hipGetLastError());
 }
 

@@ -34,8 +34,6 @@ namespace lib {
 namespace {
 
 constexpr uint32_t kPassThreads = 512;
-constexpr uint32_t kChunkRows = 4;                            // 16-byte chunks per thread
-constexpr uint32_t kTileChunks = kPassThreads * kChunkRows;   // 2048 chunks = 32 KiB
 constexpr uint32_t kMaxBins = 256;
 
 template <int CW> struct CodeOf;
@@ -99,6 +97,16 @@ __device__ __forceinline__ uint32_t child_of(const uint4& q, uint32_t k, uint32_
     const uint32_t bitpos = (k * 8 * static_cast<uint32_t>(sizeof(T))) & 31;
     return __builtin_amdgcn_ubfe(d, bitpos + shift, width);
   }
+}
+
+// `nbits` (1..32) bits of an LDS bit string of T bits from bit position x
+__device__ __forceinline__ uint32_t bit_window(const uint32_t* bits, uint32_t x, uint32_t nbits,
+                                               uint32_t T) {
+  const uint32_t w = x >> 5;
+  const uint64_t win = static_cast<uint64_t>(bits[w]) |
+                       (w + 1 < T / 32 ? static_cast<uint64_t>(bits[w + 1]) << 32 : 0ull);
+  const uint32_t val = static_cast<uint32_t>(win >> (x & 31));
+  return nbits < 32 ? val & ((1u << nbits) - 1u) : val;
 }
 
 // the chunk at src[s0 ..] with only the first `valid` codes loaded (rest 0)
@@ -173,12 +181,13 @@ __device__ __forceinline__ uint64_t map_code(const PassParams& p, uint64_t x) {
 template <typename TIn, int CW, int MapKind, bool Remap>
 __global__ __launch_bounds__(kPassThreads) void tile_hist_kernel(PassParams p) {
   using Code = typename CodeOf<CW>::type;
-  constexpr uint32_t T = kTileChunks * (16 / CW);   // codes per tile
+  const uint32_t T = p.tile_codes;                   // codes per tile
   constexpr uint32_t EI = 16 / sizeof(TIn);          // input symbols per chunk
   __shared__ uint32_t s_bins[(kPassThreads / 64) * kMaxBins];
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint32_t nb = 1u << p.k, reps = kMaxBins >> p.k;
   const uint32_t dmask = nb - 1;
+  const uint32_t row = p.row_bins ? p.row_bins : nb;
   for (uint32_t i = tid; i < (kPassThreads / 64) * kMaxBins; i += kPassThreads) s_bins[i] = 0;
   __syncthreads();
   uint32_t* bins = s_bins + wave * kMaxBins + (lane % reps) * nb;
@@ -186,6 +195,14 @@ __global__ __launch_bounds__(kPassThreads) void tile_hist_kernel(PassParams p) {
   const TileGeo g = tile_geo(p, t, T);
   const TIn* in = static_cast<const TIn*>(p.src);
   Code* out = static_cast<Code*>(p.codes_out);
+  // batched trees: only remapped nodes are written, through their own table
+  const uint8_t* lut = nullptr;
+  if constexpr (Remap) {
+    if (p.node_remap != nullptr) {
+      if (p.node_remap[g.node]) lut = p.node_lut + static_cast<uint64_t>(g.node) * 256;
+      else out = nullptr;  // read in place: its symbols are its codes
+    }
+  }
   const uint32_t nchunks = (g.len + EI - 1) / EI;
   for (uint32_t c = tid; c < nchunks; c += kPassThreads) {
     const uint64_t e0 = g.start + static_cast<uint64_t>(c) * EI;
@@ -200,7 +217,8 @@ __global__ __launch_bounds__(kPassThreads) void tile_hist_kernel(PassParams p) {
 #pragma unroll
       for (uint32_t k = 0; k < EI; ++k) {
         if (k < valid) {
-          const uint64_t code = map_code<TIn, MapKind>(p, static_cast<uint64_t>(chunk_elem<TIn>(q, k)));
+          const uint64_t x = static_cast<uint64_t>(chunk_elem<TIn>(q, k));
+          const uint64_t code = lut != nullptr ? lut[x & 255] : map_code<TIn, MapKind>(p, x);
           atomicAdd(&bins[(code >> p.dshift) & dmask], 1u);
           if constexpr (CW == 8) {
             o[2 * k] = static_cast<uint32_t>(code);
@@ -211,7 +229,9 @@ __global__ __launch_bounds__(kPassThreads) void tile_hist_kernel(PassParams p) {
           }
         }
       }
-      if (valid == EI) {
+      if (out == nullptr) {
+        // counted only (a batched node that is read in place)
+      } else if (valid == EI) {
         if constexpr (OB == 16) *reinterpret_cast<uint4*>(out + e0) = make_uint4(o[0], o[1], o[2], o[3]);
         else if constexpr (OB == 8) *reinterpret_cast<uint2*>(out + e0) = make_uint2(o[0], o[1]);
         else if constexpr (OB == 4) *reinterpret_cast<uint32_t*>(out + e0) = o[0];
@@ -238,11 +258,15 @@ __global__ __launch_bounds__(kPassThreads) void tile_hist_kernel(PassParams p) {
     }
   }
   __syncthreads();
-  for (uint32_t d = tid; d < nb; d += kPassThreads) {
+  for (uint32_t d = tid; d < row; d += kPassThreads) {
+    if (d >= nb) {
+      p.hist[t * row + d] = 0;
+      continue;
+    }
     uint32_t sum = 0;
     for (uint32_t w = 0; w < kPassThreads / 64; ++w)
       for (uint32_t r = 0; r < reps; ++r) sum += s_bins[w * kMaxBins + r * nb + d];
-    p.hist[t * nb + d] = sum;
+    p.hist[t * row + d] = sum;
   }
 }
 
@@ -336,29 +360,6 @@ __global__ void gather_rows_kernel(const uint64_t* __restrict__ excl,
 }
 
 // ---------------------------------------------------------------------------
-// Block-wide exclusive scan of one u64 per thread (kPassThreads threads).
-__device__ __forceinline__ uint64_t block_excl_scan_u64(uint64_t x, uint64_t* s_wave,
-                                                        uint64_t& total) {
-  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  uint64_t incl = x;
-#pragma unroll
-  for (uint32_t off = 1; off < 64; off <<= 1) {
-    const uint64_t y = __shfl_up(incl, off, 64);
-    if (lane >= off) incl += y;
-  }
-  if (lane == 63) s_wave[wave] = incl;
-  __syncthreads();
-  uint64_t before = 0, tot = 0;
-#pragma unroll
-  for (uint32_t w = 0; w < kPassThreads / 64; ++w) {
-    const uint64_t v = s_wave[w];
-    before += (w < wave) ? v : 0ull;
-    tot += v;
-  }
-  total = tot;
-  return before + incl - x;
-}
-
 // Exclusive scan of 256 values held 4 per lane by one wave (lane l holds
 // entries 4l .. 4l+3); returns the exclusive prefixes in place.
 template <typename V>
@@ -385,28 +386,40 @@ __device__ __forceinline__ void wave_excl_scan4(V (&v)[4], V& total) {
 // One pass over k levels of one tile (see the file comment).  Thread tid
 // owns chunks c = r * 512 + tid (r < 4) of 16 bytes; chunk c holds the codes
 // at tile positions [c*E, c*E + E) of the current level order.
-template <int CW>
-__global__ __launch_bounds__(kPassThreads) void pass_kernel(PassParams p) {
+// Waves per SIMD that the tile's LDS allows (two, three, four workgroups of
+// eight waves per CU for 4, 2, 1 chunk rows): the register budget follows.
+template <int ROWS> struct PassOcc;
+template <> struct PassOcc<4> { static constexpr int waves = 4; };
+template <> struct PassOcc<2> { static constexpr int waves = 6; };
+template <> struct PassOcc<1> { static constexpr int waves = 8; };
+
+template <int CW, int ROWS>
+__global__ __launch_bounds__(kPassThreads)
+__attribute__((amdgpu_waves_per_eu(PassOcc<ROWS>::waves)))
+void pass_kernel(PassParams p) {
   using Code = typename CodeOf<CW>::type;
+  constexpr uint32_t kChunkRows = ROWS;
+  constexpr uint32_t kTileChunks = kPassThreads * ROWS;
   constexpr uint32_t E = 16 / CW;                 // codes per chunk
   constexpr uint32_t T = kTileChunks * E;          // codes per tile
   constexpr uint32_t LPW = 32 / E;                 // lanes per 32-bit bit word
   __shared__ uint4 s_buf[2][kTileChunks];          // 64 KiB: current / next order
   __shared__ uint32_t s_bits[kTileChunks * E / 32]; // level bits of the tile
-  __shared__ uint16_t s_opre[kTileChunks];         // ones before each chunk
+  __shared__ uint16_t s_opre[kTileChunks];         // ones before each chunk in its wave row
   __shared__ uint32_t s_hc[kMaxBins + 1];          // codes of the tile with digit < x
   __shared__ uint64_t s_ec[kMaxBins + 1];          // node's codes with digit < x in earlier tiles
   __shared__ uint64_t s_pc[kMaxBins + 1];          // global start of (node, digit x)
   __shared__ uint32_t s_base[kMaxBins];            // partition base per child
-  __shared__ uint32_t s_gw[kMaxBins / 2 + 1];      // word offsets of the level's runs
-  __shared__ uint64_t s_wave[kPassThreads / 64];
+  __shared__ uint64_t s_wave[kPassThreads / 64];   // ones per wave, 16-bit field per row
   __shared__ uint32_t s_thr[9];
-  __shared__ uint32_t s_tot_ones;
 
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint64_t t = blockIdx.x;
   const TileGeo g = tile_geo(p, t, T);
-  const uint32_t k = p.k, nb = 1u << k;
+  const uint32_t k = p.node_k != nullptr ? p.node_k[g.node] : p.k;
+  const uint32_t nb = 1u << k;
+  const uint32_t row = p.row_bins ? p.row_bins : nb;        // hist / excl row stride
+  const uint32_t ts = p.thr_stride ? p.thr_stride : k + 1;  // thr entries per node
 
   // ---- per-tile tables (one wave each)
   if (wave == 0) {
@@ -414,7 +427,7 @@ __global__ __launch_bounds__(kPassThreads) void pass_kernel(PassParams p) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const uint32_t d = 4 * lane + j;
-      v[j] = d < nb ? p.hist[t * nb + d] : 0u;
+      v[j] = d < nb ? p.hist[t * row + d] : 0u;
     }
     wave_excl_scan4<uint32_t>(v, tot);
 #pragma unroll
@@ -426,20 +439,21 @@ __global__ __launch_bounds__(kPassThreads) void pass_kernel(PassParams p) {
 #pragma unroll
     for (int j = 0; j < 4; ++j) {
       const uint32_t d = 4 * lane + j;
-      v[j] = d < nb ? p.excl[t * nb + d] - p.excl[f * nb + d] : 0ull;
+      v[j] = d < nb ? p.excl[t * row + d] - p.excl[f * row + d] : 0ull;
     }
     wave_excl_scan4<uint64_t>(v, tot);
 #pragma unroll
     for (int j = 0; j < 4; ++j) s_ec[4 * lane + j] = v[j];
     if (lane == 0) s_ec[kMaxBins] = tot;
   } else if (wave == 2) {
-    const uint64_t* pc = p.pc + static_cast<uint64_t>(g.node) * (nb + 1);
+    const uint64_t* pc = p.pc + static_cast<uint64_t>(g.node) * (row + 1);
     for (uint32_t x = lane; x <= nb; x += 64) s_pc[x] = pc[x];
   } else if (wave == 3) {
-    if (lane <= k) s_thr[lane] = p.thr[static_cast<uint64_t>(g.node) * (k + 1) + lane];
+    if (lane <= k) s_thr[lane] = p.thr[static_cast<uint64_t>(g.node) * ts + lane];
   }
   // ---- stage the tile
-  const Code* src = static_cast<const Code*>(p.codes);
+  const Code* src = static_cast<const Code*>(
+      (p.node_remap != nullptr && p.node_remap[g.node]) ? p.codes_out : p.codes);
 #pragma unroll
   for (uint32_t r = 0; r < kChunkRows; ++r) {
     const uint32_t c = r * kPassThreads + tid;
@@ -457,6 +471,7 @@ __global__ __launch_bounds__(kPassThreads) void pass_kernel(PassParams p) {
     const uint32_t npres = s_hc[s_thr[i]];            // tile codes present at level l0+i
     const uint32_t cbit = p.dshift + k - 1 - i;        // code bit of the level
     const uint32_t gshift = k - i;                     // digit >> gshift = group
+    // ---- level bits of the four chunks; ones per chunk (16-bit field per row)
     uint4 v[kChunkRows];
     uint64_t packed = 0;
 #pragma unroll
@@ -473,77 +488,80 @@ __global__ __launch_bounds__(kPassThreads) void pass_kernel(PassParams p) {
       if (lane % LPW == 0) s_bits[p0 >> 5] = w32;
       packed |= static_cast<uint64_t>(__popc(mm)) << (16 * r);
     }
-    uint64_t tot_packed;
-    const uint64_t ex = block_excl_scan_u64(packed, s_wave, tot_packed);
-    uint32_t O[kChunkRows];
-    {
-      uint32_t rows_before = 0;
+    // wave-local exclusive prefixes (fields stay below 2^16: no carries)
+    const uint64_t incl = detail::wave_incl_scan_2x32(packed);
+    const uint64_t lex = incl - packed;
+#pragma unroll
+    for (uint32_t r = 0; r < kChunkRows; ++r)
+      s_opre[r * kPassThreads + tid] = static_cast<uint16_t>(lex >> (16 * r));
+    if (lane == 63) s_wave[wave] = incl;
+    __syncthreads();
+    // ones before each of my chunks: earlier rows + earlier waves of my row
+    uint32_t rowtot[kChunkRows], wbef[kChunkRows];
+#pragma unroll
+    for (uint32_t r = 0; r < kChunkRows; ++r) rowtot[r] = wbef[r] = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kPassThreads / 64; ++w) {
+      const uint64_t x = s_wave[w];
 #pragma unroll
       for (uint32_t r = 0; r < kChunkRows; ++r) {
-        O[r] = rows_before + static_cast<uint32_t>((ex >> (16 * r)) & 0xFFFF);
-        s_opre[r * kPassThreads + tid] = static_cast<uint16_t>(O[r]);
-        rows_before += static_cast<uint32_t>((tot_packed >> (16 * r)) & 0xFFFF);
+        const uint32_t f = static_cast<uint32_t>(x >> (16 * r)) & 0xFFFFu;
+        rowtot[r] += f;
+        wbef[r] += (w < wave) ? f : 0u;
       }
-      if (tid == 0) s_tot_ones = rows_before;
     }
-    // runs of this level: present groups q < thr >> gshift
-    const uint32_t ngroups = s_thr[i] >> gshift;
-    if (wave == 0) {
-      uint32_t w2[2];
+    uint32_t O[kChunkRows];
+    uint32_t tot_ones = 0;
 #pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        const uint32_t q = 2 * lane + j;
-        w2[j] = 0;
-        if (q < ngroups) {
-          const uint32_t a = q << gshift, b = (q + 1) << gshift;
-          const uint32_t len = s_hc[b] - s_hc[a];
-          const uint64_t R = s_pc[a] + s_ec[b] - s_ec[a];
-          w2[j] = len ? static_cast<uint32_t>(((R & 31) + len + 31) >> 5) : 0u;
-        }
-      }
-      const uint32_t s2 = w2[0] + w2[1];
-      const uint32_t incl = detail::wave_incl_scan_u32(s2);
-      s_gw[2 * lane] = incl - s2;
-      s_gw[2 * lane + 1] = incl - s2 + w2[0];
-      if (lane == 63) s_gw[kMaxBins / 2] = incl;
+    for (uint32_t r = 0; r < kChunkRows; ++r) {
+      O[r] = tot_ones + wbef[r] + (static_cast<uint32_t>(lex >> (16 * r)) & 0xFFFFu);
+      tot_ones += rowtot[r];
     }
-    __syncthreads();
-    // ---- write the level's runs
+    // ---- write the level's runs.  Source word sw of the tile's bit string
+    // starts at most one destination word per run piece inside it; the run
+    // of group q (tile positions [g0, g1)) goes to level bit R + (x - g0).
     {
-      const uint32_t total_words = s_gw[kMaxBins / 2];
-      const uint32_t* bits32 = s_bits;
-      uint32_t* words = p.words + p.level_word[i];
-      for (uint32_t w = tid; w < total_words; w += kPassThreads) {
-        uint32_t lo = 0, hi = ngroups;  // last group with s_gw[q] <= w
-        while (hi - lo > 1) {
-          const uint32_t mid = (lo + hi) >> 1;
-          if (s_gw[mid] <= w) lo = mid; else hi = mid;
-        }
-        const uint32_t q = lo;
-        const uint32_t a = q << gshift, b = (q + 1) << gshift;
-        const uint32_t src0 = s_hc[a];
-        const uint32_t len = s_hc[b] - src0;
-        const uint64_t R = s_pc[a] + s_ec[b] - s_ec[a];
-        const uint64_t D = (R >> 5) + (w - s_gw[q]);    // destination word
-        const uint64_t db = D * 32;
-        const uint32_t blo = R > db ? static_cast<uint32_t>(R - db) : 0u;
-        const uint64_t rend = R + len;
-        const uint32_t bhi = rend < db + 32 ? static_cast<uint32_t>(rend - db) : 32u;
-        // source bit of destination bit blo
-        const uint32_t sp = src0 + static_cast<uint32_t>(db + blo - R);
-        const uint32_t sw = sp >> 5;
-        const uint64_t win = static_cast<uint64_t>(bits32[sw]) |
-                             (sw + 1 < T / 32 ? static_cast<uint64_t>(bits32[sw + 1]) << 32 : 0ull);
-        uint32_t val = static_cast<uint32_t>(win >> (sp & 31));
-        const uint32_t nbits = bhi - blo;
-        if (nbits < 32) val &= (1u << nbits) - 1u;
-        val <<= blo;
-        if (D >= p.level_nwords[i]) {
-          atomicOr(p.err, 1u);
-        } else if (nbits == 32) {
-          words[D] = val;
-        } else if (val) {
-          atomicOr(&words[D], val);
+      const uint32_t nsw = (npres + 31) >> 5;
+      const Code* curc = reinterpret_cast<const Code*>(s_buf[cur]);
+      const uint64_t lw = p.node_lw != nullptr ? p.node_lw[g.node * 8ull + i] : p.level_word[i];
+      const uint64_t lnw = p.node_lnw != nullptr ? p.node_lnw[g.node * 8ull + i] : p.level_nwords[i];
+      uint32_t* words = p.words + lw;
+      const uint32_t ng = 1u << i;
+      for (uint32_t sw = tid; sw < nsw; sw += kPassThreads) {
+        const uint32_t pa0 = sw << 5;
+        const uint32_t pend = (pa0 + 32 < npres) ? pa0 + 32 : npres;
+        uint32_t q = i ? static_cast<uint32_t>(static_cast<uint64_t>(curc[pa0]) >> (cbit + 1)) &
+                             (ng - 1u)
+                       : 0u;
+        uint32_t pa = pa0;
+        while (pa < pend && q < ng) {
+          const uint32_t a = q << gshift, b = (q + 1) << gshift;
+          const uint32_t g0 = s_hc[a], g1 = s_hc[b];
+          if (g1 > pa) {
+            const uint32_t pb = g1 < pend ? g1 : pend;
+            const uint64_t R = s_pc[a] + (s_ec[b] - s_ec[a]);
+            const uint32_t ra = static_cast<uint32_t>(R) & 31u;
+            if (pa == g0 && ra != 0) {  // the run's first word, partly another run's
+              const uint32_t nbits = (32 - ra < g1 - g0) ? 32 - ra : g1 - g0;
+              const uint32_t val = bit_window(s_bits, g0, nbits, T) << ra;
+              const uint64_t D = R >> 5;
+              if (D >= lnw) atomicOr(p.err, 1u);
+              else if (val) atomicOr(&words[D], val);
+            }
+            // destination words start where (R - g0 + x) % 32 == 0
+            const uint32_t delta = (static_cast<uint32_t>(R) - g0) & 31u;
+            const uint32_t x0 = pa + ((32u - ((delta + pa) & 31u)) & 31u);
+            if (x0 < pb) {
+              const uint32_t nbits = (g1 - x0 < 32) ? g1 - x0 : 32u;
+              const uint32_t val = bit_window(s_bits, x0, nbits, T);
+              const uint64_t D = (R + (x0 - g0)) >> 5;
+              if (D >= lnw) atomicOr(p.err, 1u);
+              else if (nbits == 32) words[D] = val;
+              else if (val) atomicOr(&words[D], val);
+            }
+            pa = pb;
+          }
+          ++q;
         }
       }
     }
@@ -559,9 +577,16 @@ __global__ __launch_bounds__(kPassThreads) void pass_kernel(PassParams p) {
         const uint32_t q = c >> 1, bit = c & 1;
         const uint32_t ha = s_hc[q << gshift];
         const uint32_t ch = ha / E, off = ha % E;
-        const uint32_t cw = s_bits[(ch * E) >> 5] >> ((ch * E) & 31);  // chunk's bits
-        const uint32_t Oa = ch < kTileChunks ? s_opre[ch] + __popc(cw & ((1u << off) - 1u))
-                                             : s_tot_ones;
+        uint32_t Oa = tot_ones;
+        if (ch < kTileChunks) {
+          const uint32_t r = ch / kPassThreads, cw = (ch % kPassThreads) >> 6;
+          Oa = s_opre[ch];
+          for (uint32_t rr = 0; rr < r; ++rr) Oa += rowtot[rr];
+          for (uint32_t w = 0; w < cw; ++w)
+            Oa += static_cast<uint32_t>(s_wave[w] >> (16 * r)) & 0xFFFFu;
+          const uint32_t bits = s_bits[(ch * E) >> 5] >> ((ch * E) & 31);  // chunk's bits
+          Oa += __popc(bits & ((1u << off) - 1u));
+        }
         const uint32_t child_lo = c << (gshift - 1);
         s_base[c] = boff + s_hc[child_lo] - (bit ? Oa : ha - Oa);
       }
@@ -634,7 +659,9 @@ __global__ __launch_bounds__(kPassThreads) void pass_kernel(PassParams p) {
 
 }  // namespace
 
-uint64_t pass_tile_codes(int cw) { return kTileChunks * (16u / static_cast<uint32_t>(cw)); }
+uint64_t pass_tile_codes(int cw, uint32_t rows) {
+  return kPassThreads * rows * (16u / static_cast<uint32_t>(cw));
+}
 
 void launch_tile_hist(const PassParams& p, hipStream_t s) {
   if (p.num_tiles == 0) return;
@@ -685,7 +712,7 @@ uint64_t scan_partial_entries(uint64_t tiles, uint32_t k) {
 void launch_tile_scan(const PassParams& p, hipStream_t s) {
   if (p.num_tiles == 0) return;
   ProfScope ps(K_TILE_SCAN, s);
-  const uint32_t nb = 1u << p.k;
+  const uint32_t nb = p.row_bins ? p.row_bins : (1u << p.k);  // row stride
   const uint64_t nblk = (p.num_tiles + kScanTiles - 1) / kScanTiles;
   hipLaunchKernelGGL(scan_partial_kernel, dim3(static_cast<uint32_t>(nblk)), dim3(256), 0, s,
                      p.hist, p.num_tiles, nb, p.partial);
@@ -709,12 +736,21 @@ void launch_pass(const PassParams& p, hipStream_t s) {
   if (p.num_tiles == 0) return;
   ProfScope ps(K_PASS, s);
   const dim3 grid(static_cast<uint32_t>(p.num_tiles)), block(kPassThreads);
+  if (p.tile_codes != pass_tile_codes(p.cw, p.tile_rows))
+    throw Error(ECL_ERROR_INTERNAL, "tile size does not match the tile rows");
+#define ECL_PASS_ROWS(CWV)                                                                  \
+  do {                                                                                      \
+    if (p.tile_rows == 4) hipLaunchKernelGGL((pass_kernel<CWV, 4>), grid, block, 0, s, p);  \
+    else if (p.tile_rows == 2) hipLaunchKernelGGL((pass_kernel<CWV, 2>), grid, block, 0, s, p); \
+    else hipLaunchKernelGGL((pass_kernel<CWV, 1>), grid, block, 0, s, p);                   \
+  } while (0)
   switch (p.cw) {
-    case 1: hipLaunchKernelGGL(pass_kernel<1>, grid, block, 0, s, p); break;
-    case 2: hipLaunchKernelGGL(pass_kernel<2>, grid, block, 0, s, p); break;
-    case 4: hipLaunchKernelGGL(pass_kernel<4>, grid, block, 0, s, p); break;
-    default: hipLaunchKernelGGL(pass_kernel<8>, grid, block, 0, s, p); break;
+    case 1: ECL_PASS_ROWS(1); break;
+    case 2: ECL_PASS_ROWS(2); break;
+    case 4: ECL_PASS_ROWS(4); break;
+    default: ECL_PASS_ROWS(8); break;
   }
+#undef ECL_PASS_ROWS
   ECL_HIP(hipGetLastError());
 }
 

@@ -49,14 +49,31 @@ __device__ __forceinline__ uint64_t wave_sum_u64(uint64_t v) {
   return v;
 }
 
+// Inclusive wave64 scan with DPP: shifts by 1, 2, 4, 8 lanes inside each
+// row of 16 (lanes that would read across the row start get 0), then the
+// last lane of row 0 / rows 0-1 is broadcast into the later rows
+// (row_bcast:15 into rows 1 and 3, row_bcast:31 into rows 2 and 3).
 __device__ __forceinline__ uint32_t wave_incl_scan_u32(uint32_t v) {
-  const uint32_t lane = lane_id();
-#pragma unroll
-  for (int off = 1; off < 64; off <<= 1) {
-    uint32_t t = __shfl_up(v, off, 64);
-    if (lane >= static_cast<uint32_t>(off)) v += t;
-  }
+  v += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x111, 0xF, 0xF, false));
+  v += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x112, 0xF, 0xF, false));
+  v += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x114, 0xF, 0xF, false));
+  v += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x118, 0xF, 0xF, false));
+  v += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x142, 0xA, 0xF, false));
+  v += static_cast<uint32_t>(__builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x143, 0xC, 0xF, false));
   return v;
+}
+
+// the same scan over two independent 32-bit halves of a u64 (no carry
+// between the halves)
+__device__ __forceinline__ uint64_t wave_incl_scan_2x32(uint64_t v) {
+  const uint32_t lo = wave_incl_scan_u32(static_cast<uint32_t>(v));
+  const uint32_t hi = wave_incl_scan_u32(static_cast<uint32_t>(v >> 32));
+  return (static_cast<uint64_t>(hi) << 32) | lo;
+}
+
+// value of lane 63
+__device__ __forceinline__ uint32_t wave_last_u32(uint32_t v) {
+  return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 63));
 }
 
 // position (0-based) of the r-th (1-based) set bit of x; r <= popcount(x)

@@ -65,6 +65,17 @@ typedef struct ecl_query {
   uint32_t kind;
 } ecl_query;
 
+/* new: one query of a batch over many trees of a forest (same kinds as
+ * ecl_query; `tree` is the tree's index in the forest). */
+typedef struct ecl_forest_query {
+  uint64_t arg;
+  uint32_t symbol;
+  uint16_t kind;
+  uint16_t reserved;
+  uint32_t tree;
+  uint32_t reserved2;
+} ecl_forest_query;
+
 typedef struct ecl_rank_select_info {
   uint64_t size_bits;     /* BitArray::size (bit_array.cuh:274-283) */
   uint64_t offset_words;  /* BitArray::getOffset (bit_array.cuh:160-164) */
@@ -262,6 +273,116 @@ ecl_status ecl_wavelet_tree_query_batch_device(ecl_wavelet_tree wt, const ecl_qu
  * kind 3: ecl_query mixing the three kinds 1:1:1).  Symbols are minimal. */
 ecl_status ecl_wavelet_tree_generate_queries(ecl_wavelet_tree wt, int kind, uint64_t seed,
                                              size_t m, void* d_out, void* stream);
+
+/* ==========================================================================
+ * Forests: many independent trees built and queried together.  No
+ * counterpart in the reference (its trees are built and queried one at a
+ * time, wavelet_tree.cuh:576-867, 956-2079); for callers with thousands of
+ * small texts, where per-tree launches and synchronisations dominate.
+ * ========================================================================== */
+typedef struct ecl_forest_t* ecl_forest;
+typedef struct ecl_graph_t* ecl_graph;
+
+/* Builds one tree per text with one set of launches for all of them.
+ * d_texts: device uint8 texts, text i = d_texts[starts[i] .. starts[i] +
+ * lengths[i]), every start a multiple of 16 and every text non-empty; the
+ * alphabet of text i is 0 .. sigmas[i]-1 (2 <= sigma <= 256).  starts,
+ * lengths and sigmas are host arrays of num_texts entries.  Each tree
+ * answers exactly as ecl_wavelet_tree_create_device over the same text and
+ * alphabet.  tile_rows: 0 = automatic, else 1, 2 or 4 (codes per
+ * construction tile = 8192 * tile_rows). */
+ecl_status ecl_forest_create(const void* d_texts, const uint64_t* starts, const uint64_t* lengths,
+                             const uint64_t* sigmas, size_t num_texts, int device,
+                             uint32_t tile_rows, ecl_forest* out);
+ecl_status ecl_forest_destroy(ecl_forest f);
+ecl_status ecl_forest_num_trees(ecl_forest f, size_t* out);
+/* size, alphabet size and levels of tree i */
+ecl_status ecl_forest_tree_info(ecl_forest f, size_t i, uint64_t* n, uint64_t* sigma,
+                                uint32_t* num_levels);
+/* device view of tree i (detail::TreeView) */
+ecl_status ecl_forest_device_view(ecl_forest f, size_t i, void* view, size_t view_bytes);
+/* One launch over a batch of queries on any trees of the forest (symbols
+ * minimal).  Results as ecl_wavelet_tree_query_batch_device. */
+ecl_status ecl_forest_query_device(ecl_forest f, const ecl_forest_query* d_queries, size_t m,
+                                   uint64_t* d_out, void* stream);
+/* Mixed queries drawn on the device: queries [qoff[i], qoff[i+1]) go to
+ * tree i (qoff: host, num_trees + 1 entries), kind q % 3, fields drawn as
+ * by ecl_wavelet_tree_generate_queries. */
+ecl_status ecl_forest_generate_queries(ecl_forest f, const uint64_t* qoff, uint64_t seed,
+                                       ecl_forest_query* d_out, void* stream);
+/* Times the batch query launch over workgroup sizes and grid sizes on (a
+ * prefix of) the given batch and keeps the fastest for this forest. */
+ecl_status ecl_forest_autotune(ecl_forest f, const ecl_forest_query* d_queries, size_t m,
+                               uint64_t* d_out, uint32_t* block, uint32_t* grid);
+/* The batch launch captured once into a HIP graph and replayed: the
+ * queries and outputs must stay allocated while the graph lives. */
+ecl_status ecl_forest_graph_create(ecl_forest f, const ecl_forest_query* d_queries, size_t m,
+                                   uint64_t* d_out, ecl_graph* out);
+ecl_status ecl_graph_launch(ecl_graph g, void* stream);
+ecl_status ecl_graph_destroy(ecl_graph g);
+
+/* ==========================================================================
+ * Query server: trees stay resident on the device, admitted against an HBM
+ * budget; requests (mixed ecl_query batches against one tree, in host
+ * memory) are coalesced into device batches while earlier batches are still
+ * running (continuous batching).  No counterpart in the reference, whose
+ * host queries run one call at a time (wavelet_tree.cuh:956-1797); for
+ * callers that serve many concurrent requests.
+ * ========================================================================== */
+typedef struct ecl_server_t* ecl_server;
+typedef struct ecl_server_config {
+  uint64_t hbm_budget;    /* bytes the resident trees may occupy (0: 90% of free HBM) */
+  uint64_t max_batch;     /* queries per device batch (0: 2^22) */
+  uint32_t max_inflight;  /* device batches in flight at once (0: 3) */
+  uint32_t max_trees;     /* resident trees (0: 1024) */
+  uint32_t coalesce;      /* 1: coalesce requests into batches; 0: one request per
+                           * batch and one batch on the device at a time (the plain
+                           * per-call form) */
+  uint32_t reserved;
+} ecl_server_config;
+typedef struct ecl_server_stats {
+  uint64_t requests;      /* completed requests since the last reset */
+  uint64_t queries;
+  uint64_t batches;
+  double p50_ms, p99_ms, mean_ms, max_ms;  /* submit -> results in host memory */
+  double elapsed_s;       /* first submit to last completion */
+  double queries_per_s;
+  uint64_t hbm_used;      /* bytes held by the resident trees */
+  uint64_t trees;
+} ecl_server_stats;
+ecl_status ecl_server_create(int device, const ecl_server_config* config, ecl_server* out);
+/* completes every submitted request, then frees the trees */
+ecl_status ecl_server_destroy(ecl_server s);
+/* Builds and admits a tree over a uint8 text (host or device) with the
+ * alphabet 0 .. sigma-1; ECL_ERROR_OUT_OF_MEMORY_TREE when the tree would
+ * exceed the budget. */
+ecl_status ecl_server_add_tree(ecl_server s, const void* text, int text_on_device, size_t n,
+                               uint64_t sigma, uint32_t* tree);
+/* No new requests for the tree; its memory is freed once its requests finish. */
+ecl_status ecl_server_remove_tree(ecl_server s, uint32_t tree);
+/* Queues a request; `queries` must stay valid and `results` (m entries)
+ * writable until ecl_server_wait returns for it. */
+ecl_status ecl_server_submit(ecl_server s, uint32_t tree, const ecl_query* queries, size_t m,
+                             uint64_t* results, uint64_t* request);
+/* Blocks until the request's results are in host memory. */
+ecl_status ecl_server_wait(ecl_server s, uint64_t request, double* latency_ms);
+/* Switches between coalescing (1) and the plain per-call form (0: one
+ * request per batch, one batch on the device at a time) for later batches. */
+ecl_status ecl_server_set_mode(ecl_server s, uint32_t coalesce);
+ecl_status ecl_server_stats_get(ecl_server s, ecl_server_stats* out);
+ecl_status ecl_server_stats_reset(ecl_server s);
+/* Open-loop synthetic load: Poisson arrivals at `rate` requests/s for
+ * `seconds`; request sizes log-uniform in [min_size, max_size] queries
+ * against a uniformly chosen resident tree, taken from per-tree pools of
+ * `pool` device-generated mixed queries.  Returns the stats of the run. */
+typedef struct ecl_load_spec {
+  double rate;
+  double seconds;
+  uint64_t seed;
+  uint64_t min_size, max_size;
+  uint64_t pool;
+} ecl_load_spec;
+ecl_status ecl_server_load(ecl_server s, const ecl_load_spec* spec, ecl_server_stats* out);
 
 #ifdef __cplusplus
 }

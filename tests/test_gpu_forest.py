@@ -1,0 +1,204 @@
+"""Forests (many trees built and queried together) on the GPU vs the oracle:
+every answer of a mixed batch over many trees equals the direct scan of its
+text, equals the answer of the same tree built alone, and is unchanged by
+the construction tile size, the autotuned launch shape and graph replays."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+SIGMAS = [4, 20, 64, 256]
+
+
+def _texts(seed, count, lo_log2=6, hi_log2=15):
+    """count texts, log-uniform lengths, sigma from SIGMAS, uniform or Zipf
+    symbols; returns (texts, sigmas)."""
+    rng = np.random.default_rng(seed)
+    texts, sigmas = [], []
+    for i in range(count):
+        n = int(2 ** rng.uniform(lo_log2, hi_log2))
+        sigma = int(rng.choice(SIGMAS))
+        if rng.random() < 0.5:
+            t = O.gen_uniform(seed * 1000 + i, n, sigma, np.uint8)
+        else:
+            t = O.gen_cdf(seed * 1000 + i, n, O.zipf_thresholds(sigma, 1.1), np.uint8)
+        texts.append(t.astype(np.uint8))
+        sigmas.append(sigma)
+    return texts, sigmas
+
+
+def _forest(texts, sigmas, tile_rows=0):
+    """A forest over texts of any length, each starting on a 16-byte
+    boundary of one device buffer (the gaps hold other bytes)."""
+    from ecl_wavelet_amd import DeviceBuffer, Forest
+    starts, pos = [], 0
+    for t in texts:
+        starts.append(pos)
+        pos += (len(t) + 15) // 16 * 16 + 16
+    buf = np.full(pos, 0xEE, dtype=np.uint8)  # gap bytes outside every alphabet
+    for t, s0 in zip(texts, starts):
+        buf[s0:s0 + len(t)] = t
+    d = DeviceBuffer.from_array(buf)
+    f = Forest(d, np.array(starts, dtype=np.uint64),
+               np.array([len(t) for t in texts], dtype=np.uint64),
+               np.array(sigmas, dtype=np.uint64), tile_rows=tile_rows)
+    return f, d, starts
+
+
+def _padded_texts(seed, count, **kw):
+    return _texts(seed, count, **kw)
+
+
+def _queries(texts, sigmas, per_text, seed):
+    """Mixed queries (kind, tree, arg, symbol) with their direct-scan answers."""
+    from ecl_wavelet_amd import FOREST_QUERY_DTYPE
+    rng = np.random.default_rng(seed)
+    qs, exp = [], []
+    for ti, (t, s) in enumerate(zip(texts, sigmas)):
+        n = len(t)
+        kinds = rng.integers(0, 3, per_text)
+        q = np.zeros(per_text, dtype=FOREST_QUERY_DTYPE)
+        q["kind"] = kinds
+        q["tree"] = ti
+        e = np.zeros(per_text, dtype=np.uint64)
+        a = kinds == 0
+        idx = rng.integers(0, n, a.sum()).astype(np.uint64)
+        q["arg"][a] = idx
+        e[a] = t[idx]
+        r = kinds == 1
+        ri = rng.integers(0, n + 1, r.sum()).astype(np.uint64)
+        rc = rng.integers(0, s, r.sum()).astype(np.uint64)
+        q["arg"][r] = ri
+        q["symbol"][r] = rc
+        e[r] = O.scan_rank_batch(t, ri, rc)
+        sl = kinds == 2
+        sc = rng.integers(0, s, sl.sum()).astype(np.uint64)
+        cnt = np.bincount(t, minlength=256)
+        k = np.array([int(rng.integers(1, cnt[c] + 2)) for c in sc], dtype=np.uint64)
+        q["arg"][sl] = k
+        q["symbol"][sl] = sc
+        e[sl] = O.scan_select_batch(t, k, sc)
+        qs.append(q)
+        exp.append(e)
+    q = np.concatenate(qs)
+    e = np.concatenate(exp)
+    perm = rng.permutation(len(q))  # trees interleaved in the batch
+    return q[perm], e[perm]
+
+
+def _run(f, q):
+    from ecl_wavelet_amd import DeviceBuffer, synchronize
+    dq = DeviceBuffer.from_array(q)
+    dout = DeviceBuffer(len(q) * 8)
+    f.queryDevice(dq, len(q), dout)
+    synchronize()
+    return dout.download(np.uint64, len(q))
+
+
+def test_forest_queries_vs_oracle(gpu):
+    texts, sigmas = _padded_texts(11, 60)
+    f, _, _ = _forest(texts, sigmas)
+    q, e = _queries(texts, sigmas, 300, seed=5)
+    got = _run(f, q)
+    assert np.array_equal(got, e)
+    for i, (t, s) in enumerate(zip(texts, sigmas)):
+        n, sigma, L = f.treeInfo(i)
+        assert (n, sigma, L) == (len(t), s, O.ceil_log2(s))
+
+
+@pytest.mark.parametrize("rows", [1, 2, 4])
+def test_forest_tile_sizes_agree(gpu, rows):
+    texts, sigmas = _padded_texts(12, 40, lo_log2=8, hi_log2=17)
+    q, e = _queries(texts, sigmas, 200, seed=6)
+    f, _, _ = _forest(texts, sigmas, tile_rows=rows)
+    assert np.array_equal(_run(f, q), e)
+
+
+def test_forest_equals_single_trees(gpu):
+    """Every level of every forest tree answers like the tree built alone;
+    checked through queries on both."""
+    from ecl_wavelet_amd import WaveletTree
+    texts, sigmas = _padded_texts(13, 12, lo_log2=10, hi_log2=16)
+    f, _, _ = _forest(texts, sigmas)
+    q, e = _queries(texts, sigmas, 400, seed=7)
+    got = _run(f, q)
+    assert np.array_equal(got, e)
+    for i, (t, s) in enumerate(zip(texts, sigmas)):
+        wt = WaveletTree(t, np.arange(s, dtype=np.uint8))
+        sel = q["tree"] == i
+        qi, gi = q[sel], got[sel]
+        a = qi["kind"] == 0
+        assert np.array_equal(wt.access(qi["arg"][a]).astype(np.uint64), gi[a])
+        r = qi["kind"] == 1
+        rq = wt.make_queries(qi["arg"][r], qi["symbol"][r].astype(np.uint8))
+        assert np.array_equal(wt.rank(rq), gi[r])
+        sl = qi["kind"] == 2
+        sq = wt.make_queries(qi["arg"][sl], qi["symbol"][sl].astype(np.uint8))
+        assert np.array_equal(wt.select(sq), gi[sl])
+
+
+def test_forest_autotune_and_graph_replays(gpu):
+    from ecl_wavelet_amd import DeviceBuffer, synchronize
+    texts, sigmas = _padded_texts(14, 30)
+    f, _, _ = _forest(texts, sigmas)
+    q, e = _queries(texts, sigmas, 500, seed=8)
+    dq = DeviceBuffer.from_array(q)
+    dout = DeviceBuffer(len(q) * 8)
+    block, grid = f.autotune(dq, len(q), dout)
+    assert block in (64, 128, 256, 512) and grid > 0
+    f.queryDevice(dq, len(q), dout)
+    synchronize()
+    assert np.array_equal(dout.download(np.uint64, len(q)), e)
+    g = f.graph(dq, len(q), dout)
+    zeros = np.zeros(len(q), dtype=np.uint64)
+    for _ in range(3):
+        dout.upload(zeros)  # eager work between the replays
+        synchronize()
+        g.launch()
+        synchronize()
+        assert np.array_equal(dout.download(np.uint64, len(q)), e)
+
+
+def test_forest_generated_queries(gpu):
+    """Queries drawn on the device (per-tree counts) are well formed and
+    answered exactly."""
+    from ecl_wavelet_amd import DeviceBuffer, FOREST_QUERY_DTYPE, synchronize
+    texts, sigmas = _padded_texts(15, 25)
+    f, _, _ = _forest(texts, sigmas)
+    per = np.array([max(1, len(t) // 64) for t in texts], dtype=np.uint64)
+    qoff = np.concatenate([[0], np.cumsum(per)]).astype(np.uint64)
+    m = int(qoff[-1])
+    dq = DeviceBuffer(m * FOREST_QUERY_DTYPE.itemsize)
+    f.generateQueries(qoff, 77, dq)
+    synchronize()
+    q = dq.download(FOREST_QUERY_DTYPE, m)
+    assert np.array_equal(np.repeat(np.arange(len(texts)), per.astype(np.int64)), q["tree"])
+    assert np.array_equal(q["kind"], np.arange(m) % 3)
+    got = _run(f, q)
+    for i, t in enumerate(texts):
+        sel = q["tree"] == i
+        qi, gi = q[sel], got[sel]
+        for kind in range(3):
+            s = qi["kind"] == kind
+            if kind == 0:
+                exp = t[qi["arg"][s]]
+            elif kind == 1:
+                exp = O.scan_rank_batch(t, qi["arg"][s], qi["symbol"][s].astype(np.uint64))
+            else:
+                exp = O.scan_select_batch(t, qi["arg"][s], qi["symbol"][s].astype(np.uint64))
+            assert np.array_equal(gi[s], exp.astype(np.uint64))
+
+
+def test_forest_rejects_bad_input(gpu):
+    from ecl_wavelet_amd import DeviceBuffer, Forest
+    from ecl_wavelet_amd._lib import EclError
+    d = DeviceBuffer.from_array(np.zeros(64, dtype=np.uint8))
+    u = np.uint64
+    with pytest.raises(EclError):  # unaligned second text
+        Forest(d, np.array([0, 8], dtype=u), np.array([8, 8], dtype=u), np.array([4, 4], dtype=u))
+    with pytest.raises(EclError):  # empty text
+        Forest(d, np.array([0, 16], dtype=u), np.array([16, 0], dtype=u), np.array([4, 4], dtype=u))
+    with pytest.raises(EclError):  # alphabet too large
+        Forest(d, np.array([0], dtype=u), np.array([64], dtype=u), np.array([300], dtype=u))

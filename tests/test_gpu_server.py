@@ -1,0 +1,132 @@
+"""The query server (continuous batching) on the GPU: every request's results
+are the same when it runs alone and when it runs in a mix of concurrent
+requests of different sizes and trees, and equal the direct scans of the
+texts.  Admission by HBM budget, removal, the plain per-call mode and the
+synthetic load driver are covered too."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _text(seed, n, sigma):
+    return O.gen_uniform(seed, n, sigma, np.uint8)
+
+
+def _requests(text, sigma, sizes, seed):
+    """Mixed-kind requests (MIXED_DTYPE) with their exact answers."""
+    from ecl_wavelet_amd import MIXED_DTYPE
+    rng = np.random.default_rng(seed)
+    n = len(text)
+    cnt = np.bincount(text, minlength=256)
+    out = []
+    for m in sizes:
+        q = np.zeros(m, dtype=MIXED_DTYPE)
+        kinds = rng.integers(0, 3, m)
+        q["kind"] = kinds
+        e = np.zeros(m, dtype=np.uint64)
+        a = kinds == 0
+        idx = rng.integers(0, n, a.sum()).astype(np.uint64)
+        q["arg"][a] = idx
+        e[a] = text[idx]
+        r = kinds == 1
+        ri = rng.integers(0, n + 1, r.sum()).astype(np.uint64)
+        rc = rng.integers(0, sigma, r.sum()).astype(np.uint64)
+        q["arg"][r], q["symbol"][r] = ri, rc
+        e[r] = O.scan_rank_batch(text, ri, rc)
+        s = kinds == 2
+        sc = rng.integers(0, sigma, s.sum()).astype(np.uint64)
+        k = np.array([int(rng.integers(1, cnt[c] + 2)) for c in sc], dtype=np.uint64)
+        q["arg"][s], q["symbol"][s] = k, sc
+        e[s] = O.scan_select_batch(text, k, sc)
+        out.append((q, e))
+    return out
+
+
+@pytest.mark.parametrize("max_batch", [0, 1000])
+def test_server_alone_equals_mix(gpu, max_batch):
+    """max_batch 1000 forces requests to be split across batches."""
+    from ecl_wavelet_amd import Server
+    srv = Server(max_batch=max_batch, max_inflight=3)
+    texts = [(_text(1, 50_000, 4), 4), (_text(2, 120_000, 256), 256), (_text(3, 70_000, 64), 64)]
+    trees = [srv.addTree(t, s) for t, s in texts]
+    rng = np.random.default_rng(9)
+    work = []
+    for j in range(40):
+        ti = int(rng.integers(0, len(trees)))
+        size = int(2 ** rng.uniform(0, 12))
+        (q, e), = _requests(texts[ti][0], texts[ti][1], [size], seed=100 + j)
+        work.append((trees[ti], q, e))
+    # alone: each request waits for the previous one
+    alone = []
+    for tree, q, e in work:
+        out, lat = srv.wait(srv.submit(tree, q))
+        assert np.array_equal(out, e)
+        assert lat >= 0
+        alone.append(out)
+    # in the mix: all submitted before any is waited for
+    ids = [srv.submit(tree, q) for tree, q, _ in work]
+    for rid, a, (_, _, e) in zip(ids, alone, work):
+        out, _ = srv.wait(rid)
+        assert np.array_equal(out, a)
+        assert np.array_equal(out, e)
+    st = srv.stats()
+    assert st["requests"] == 80
+    assert st["queries"] == 2 * sum(len(q) for _, q, _ in work)
+    assert st["p99_ms"] >= st["p50_ms"] > 0
+    assert st["trees"] == 3
+    srv.close()
+
+
+def test_server_plain_mode_same_results(gpu):
+    from ecl_wavelet_amd import Server
+    srv = Server(coalesce=False)
+    t = _text(4, 200_000, 20)
+    tree = srv.addTree(t, 20)
+    reqs = _requests(t, 20, [1, 17, 300, 5000], seed=5)
+    ids = [srv.submit(tree, q) for q, _ in reqs]
+    for rid, (_, e) in zip(ids, reqs):
+        out, _ = srv.wait(rid)
+        assert np.array_equal(out, e)
+    assert srv.stats()["batches"] == 4  # one batch per request
+    srv.close()
+
+
+def test_server_budget_admission_and_removal(gpu):
+    from ecl_wavelet_amd import Server
+    from ecl_wavelet_amd._lib import EclError, ECL_ERROR_OUT_OF_MEMORY_TREE
+    srv = Server(hbm_budget=3 << 20)  # room for about one tree of 2^21 symbols
+    t = _text(6, 1 << 21, 256)
+    tree = srv.addTree(t, 256)
+    with pytest.raises(EclError) as ex:
+        srv.addTree(t, 256)
+    assert ex.value.status == ECL_ERROR_OUT_OF_MEMORY_TREE
+    assert str(ex.value) == "Not enough memory available for the wavelet tree."
+    (q, e), = _requests(t, 256, [1000], seed=3)
+    rid = srv.submit(tree, q)
+    srv.removeTree(tree)  # memory is returned once the request is done
+    out, _ = srv.wait(rid)
+    assert np.array_equal(out, e)
+    with pytest.raises(EclError):
+        srv.submit(tree, q)
+    tree2 = srv.addTree(t, 256)  # fits again
+    out, _ = srv.wait(srv.submit(tree2, q))
+    assert np.array_equal(out, e)
+    srv.close()
+
+
+def test_server_device_text_and_load(gpu):
+    from ecl_wavelet_amd import DeviceBuffer, Server
+    srv = Server()
+    t = _text(7, 1 << 20, 256)
+    d = DeviceBuffer.from_array(t)
+    tree = srv.addTree(d, 256, n=t.size)
+    (q, e), = _requests(t, 256, [4096], seed=8)
+    out, _ = srv.wait(srv.submit(tree, q))
+    assert np.array_equal(out, e)
+    st = srv.load(rate=2000, seconds=0.5, seed=3, min_size=1, max_size=20000, pool=1 << 16)
+    assert st["requests"] > 0 and st["queries"] > 0
+    assert st["p99_ms"] >= st["p50_ms"] > 0
+    srv.close()
