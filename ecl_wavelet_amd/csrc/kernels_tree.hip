@@ -99,16 +99,6 @@ __device__ __forceinline__ uint32_t child_of(const uint4& q, uint32_t k, uint32_
   }
 }
 
-// `nbits` (1..32) bits of an LDS bit string of T bits from bit position x
-__device__ __forceinline__ uint32_t bit_window(const uint32_t* bits, uint32_t x, uint32_t nbits,
-                                               uint32_t T) {
-  const uint32_t w = x >> 5;
-  const uint64_t win = static_cast<uint64_t>(bits[w]) |
-                       (w + 1 < T / 32 ? static_cast<uint64_t>(bits[w + 1]) << 32 : 0ull);
-  const uint32_t val = static_cast<uint32_t>(win >> (x & 31));
-  return nbits < 32 ? val & ((1u << nbits) - 1u) : val;
-}
-
 // the chunk at src[s0 ..] with only the first `valid` codes loaded (rest 0)
 template <typename T>
 __device__ __forceinline__ uint4 load_partial_chunk(const T* src, uint64_t s0, uint32_t valid) {
@@ -386,11 +376,12 @@ __device__ __forceinline__ void wave_excl_scan4(V (&v)[4], V& total) {
 // One pass over k levels of one tile (see the file comment).  Thread tid
 // owns chunks c = r * 512 + tid (r < 4) of 16 bytes; chunk c holds the codes
 // at tile positions [c*E, c*E + E) of the current level order.
-// Waves per SIMD that the tile's LDS allows (two, three, four workgroups of
-// eight waves per CU for 4, 2, 1 chunk rows): the register budget follows.
+// Waves per SIMD that the tile's LDS allows (three, four, four workgroups
+// of eight waves per CU for 4, 2, 1 chunk rows): the register budget
+// follows.
 template <int ROWS> struct PassOcc;
-template <> struct PassOcc<4> { static constexpr int waves = 4; };
-template <> struct PassOcc<2> { static constexpr int waves = 6; };
+template <> struct PassOcc<4> { static constexpr int waves = 6; };
+template <> struct PassOcc<2> { static constexpr int waves = 8; };
 template <> struct PassOcc<1> { static constexpr int waves = 8; };
 
 template <int CW, int ROWS>
@@ -403,7 +394,9 @@ void pass_kernel(PassParams p) {
   constexpr uint32_t E = 16 / CW;                 // codes per chunk
   constexpr uint32_t T = kTileChunks * E;          // codes per tile
   constexpr uint32_t LPW = 32 / E;                 // lanes per 32-bit bit word
-  __shared__ uint4 s_buf[2][kTileChunks];          // 64 KiB: current / next order
+  // the tile in the current level's order; rewritten in place by the
+  // partition (every thread holds its chunks in registers by then)
+  __shared__ uint4 s_buf[kTileChunks];
   __shared__ uint32_t s_bits[kTileChunks * E / 32]; // level bits of the tile
   __shared__ uint16_t s_opre[kTileChunks];         // ones before each chunk in its wave row
   __shared__ uint32_t s_hc[kMaxBins + 1];          // codes of the tile with digit < x
@@ -411,6 +404,7 @@ void pass_kernel(PassParams p) {
   __shared__ uint64_t s_pc[kMaxBins + 1];          // global start of (node, digit x)
   __shared__ uint32_t s_base[kMaxBins];            // partition base per child
   __shared__ uint64_t s_wave[kPassThreads / 64];   // ones per wave, 16-bit field per row
+  __shared__ uint32_t s_gw[kMaxBins / 2 + 1];      // destination words per group (prefix)
   __shared__ uint32_t s_thr[9];
 
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -461,12 +455,11 @@ void pass_kernel(PassParams p) {
     uint4 q = make_uint4(0, 0, 0, 0);
     if (p0 + E <= g.len) q = *reinterpret_cast<const uint4*>(src + g.start + p0);
     else if (p0 < g.len) q = load_partial_chunk<Code>(src, g.start + p0, g.len - p0);
-    s_buf[0][c] = q;
+    s_buf[c] = q;
   }
   // (digits >= nb never occur, so s_hc[x] and s_ec[x] for x >= nb hold the totals)
   __syncthreads();
 
-  uint32_t cur = 0;
   for (uint32_t i = 0; i < k; ++i) {
     const uint32_t npres = s_hc[s_thr[i]];            // tile codes present at level l0+i
     const uint32_t cbit = p.dshift + k - 1 - i;        // code bit of the level
@@ -477,7 +470,7 @@ void pass_kernel(PassParams p) {
 #pragma unroll
     for (uint32_t r = 0; r < kChunkRows; ++r) {
       const uint32_t c = r * kPassThreads + tid;
-      v[r] = s_buf[cur][c];
+      v[r] = s_buf[c];
       uint32_t mm = chunk_mask<Code>(v[r], cbit & 63);
       const uint32_t p0 = c * E;
       if (p0 + E > npres) mm &= (p0 >= npres) ? 0u : ((1u << (npres - p0)) - 1u);
@@ -496,73 +489,86 @@ void pass_kernel(PassParams p) {
       s_opre[r * kPassThreads + tid] = static_cast<uint16_t>(lex >> (16 * r));
     if (lane == 63) s_wave[wave] = incl;
     __syncthreads();
-    // ones before each of my chunks: earlier rows + earlier waves of my row
-    uint32_t rowtot[kChunkRows], wbef[kChunkRows];
-#pragma unroll
-    for (uint32_t r = 0; r < kChunkRows; ++r) rowtot[r] = wbef[r] = 0;
+    // ones before each of my chunks: earlier rows + earlier waves of my row,
+    // summed as packed 16-bit fields (one per row; no field overflows)
+    uint64_t all_waves = 0, waves_before = 0;
+    const uint32_t wv = __builtin_amdgcn_readfirstlane(wave);
 #pragma unroll
     for (uint32_t w = 0; w < kPassThreads / 64; ++w) {
       const uint64_t x = s_wave[w];
-#pragma unroll
-      for (uint32_t r = 0; r < kChunkRows; ++r) {
-        const uint32_t f = static_cast<uint32_t>(x >> (16 * r)) & 0xFFFFu;
-        rowtot[r] += f;
-        wbef[r] += (w < wave) ? f : 0u;
-      }
+      all_waves += x;
+      if (w < wv) waves_before += x;
     }
+    uint32_t rowtot[kChunkRows];
     uint32_t O[kChunkRows];
     uint32_t tot_ones = 0;
+    const uint64_t mine = waves_before + lex;
 #pragma unroll
     for (uint32_t r = 0; r < kChunkRows; ++r) {
-      O[r] = tot_ones + wbef[r] + (static_cast<uint32_t>(lex >> (16 * r)) & 0xFFFFu);
+      rowtot[r] = static_cast<uint32_t>(all_waves >> (16 * r)) & 0xFFFFu;
+      O[r] = tot_ones + (static_cast<uint32_t>(mine >> (16 * r)) & 0xFFFFu);
       tot_ones += rowtot[r];
     }
-    // ---- write the level's runs.  Source word sw of the tile's bit string
-    // starts at most one destination word per run piece inside it; the run
-    // of group q (tile positions [g0, g1)) goes to level bit R + (x - g0).
+    // ---- write the level's runs.  Group q (tile positions [g0, g1)) goes to
+    // level bits [R, R + g1 - g0); wave 0 counts the destination words of
+    // every group, then each thread takes destination words, finds their
+    // group by binary search over the word offsets and assembles the word
+    // from the tile's bit string (plain store when the word lies inside the
+    // run, atomicOr on the two edge words shared with neighbouring runs).
+    const uint32_t ngroups = s_thr[i] >> gshift;
+    if (wave == 0) {
+      uint32_t w2[2];
+#pragma unroll
+      for (int j = 0; j < 2; ++j) {
+        const uint32_t q = 2 * lane + j;
+        w2[j] = 0;
+        if (q < ngroups) {
+          const uint32_t a = q << gshift, b = (q + 1) << gshift;
+          const uint32_t len = s_hc[b] - s_hc[a];
+          const uint64_t R = s_pc[a] + s_ec[b] - s_ec[a];
+          w2[j] = len ? static_cast<uint32_t>(((R & 31) + len + 31) >> 5) : 0u;
+        }
+      }
+      const uint32_t s2 = w2[0] + w2[1];
+      const uint32_t incl = detail::wave_incl_scan_u32(s2);
+      s_gw[2 * lane] = incl - s2;
+      s_gw[2 * lane + 1] = incl - s2 + w2[0];
+      if (lane == 63) s_gw[kMaxBins / 2] = incl;
+    }
+    __syncthreads();
     {
-      const uint32_t nsw = (npres + 31) >> 5;
-      const Code* curc = reinterpret_cast<const Code*>(s_buf[cur]);
+      const uint32_t total_words = s_gw[kMaxBins / 2];
+      const uint32_t* bits32 = s_bits;
       const uint64_t lw = p.node_lw != nullptr ? p.node_lw[g.node * 8ull + i] : p.level_word[i];
       const uint64_t lnw = p.node_lnw != nullptr ? p.node_lnw[g.node * 8ull + i] : p.level_nwords[i];
       uint32_t* words = p.words + lw;
-      const uint32_t ng = 1u << i;
-      for (uint32_t sw = tid; sw < nsw; sw += kPassThreads) {
-        const uint32_t pa0 = sw << 5;
-        const uint32_t pend = (pa0 + 32 < npres) ? pa0 + 32 : npres;
-        uint32_t q = i ? static_cast<uint32_t>(static_cast<uint64_t>(curc[pa0]) >> (cbit + 1)) &
-                             (ng - 1u)
-                       : 0u;
-        uint32_t pa = pa0;
-        while (pa < pend && q < ng) {
-          const uint32_t a = q << gshift, b = (q + 1) << gshift;
-          const uint32_t g0 = s_hc[a], g1 = s_hc[b];
-          if (g1 > pa) {
-            const uint32_t pb = g1 < pend ? g1 : pend;
-            const uint64_t R = s_pc[a] + (s_ec[b] - s_ec[a]);
-            const uint32_t ra = static_cast<uint32_t>(R) & 31u;
-            if (pa == g0 && ra != 0) {  // the run's first word, partly another run's
-              const uint32_t nbits = (32 - ra < g1 - g0) ? 32 - ra : g1 - g0;
-              const uint32_t val = bit_window(s_bits, g0, nbits, T) << ra;
-              const uint64_t D = R >> 5;
-              if (D >= lnw) atomicOr(p.err, 1u);
-              else if (val) atomicOr(&words[D], val);
-            }
-            // destination words start where (R - g0 + x) % 32 == 0
-            const uint32_t delta = (static_cast<uint32_t>(R) - g0) & 31u;
-            const uint32_t x0 = pa + ((32u - ((delta + pa) & 31u)) & 31u);
-            if (x0 < pb) {
-              const uint32_t nbits = (g1 - x0 < 32) ? g1 - x0 : 32u;
-              const uint32_t val = bit_window(s_bits, x0, nbits, T);
-              const uint64_t D = (R + (x0 - g0)) >> 5;
-              if (D >= lnw) atomicOr(p.err, 1u);
-              else if (nbits == 32) words[D] = val;
-              else if (val) atomicOr(&words[D], val);
-            }
-            pa = pb;
-          }
-          ++q;
+      for (uint32_t w = tid; w < total_words; w += kPassThreads) {
+        uint32_t lo = 0, hi = ngroups;
+        while (hi - lo > 1) {
+          const uint32_t mid = (lo + hi) >> 1;
+          if (s_gw[mid] <= w) lo = mid; else hi = mid;
         }
+        const uint32_t q = lo;
+        const uint32_t a = q << gshift, b = (q + 1) << gshift;
+        const uint32_t src0 = s_hc[a];
+        const uint32_t len = s_hc[b] - src0;
+        const uint64_t R = s_pc[a] + s_ec[b] - s_ec[a];
+        const uint64_t D = (R >> 5) + (w - s_gw[q]);
+        const uint64_t db = D * 32;
+        const uint32_t blo = R > db ? static_cast<uint32_t>(R - db) : 0u;
+        const uint64_t rend = R + len;
+        const uint32_t bhi = rend < db + 32 ? static_cast<uint32_t>(rend - db) : 32u;
+        const uint32_t sp = src0 + static_cast<uint32_t>(db + blo - R);
+        const uint32_t sw = sp >> 5;
+        const uint64_t win = static_cast<uint64_t>(bits32[sw]) |
+                             (sw + 1 < T / 32 ? static_cast<uint64_t>(bits32[sw + 1]) << 32 : 0ull);
+        uint32_t val = static_cast<uint32_t>(win >> (sp & 31));
+        const uint32_t nbits = bhi - blo;
+        if (nbits < 32) val &= (1u << nbits) - 1u;
+        val <<= blo;
+        if (D >= lnw) atomicOr(p.err, 1u);
+        else if (nbits == 32) words[D] = val;
+        else if (val) atomicOr(&words[D], val);
       }
     }
     const bool partition = (i + 1 < k) || (p.dst != nullptr);
@@ -572,7 +578,6 @@ void pass_kernel(PassParams p) {
     // entries are indices into both buffers seen as one array of 2T codes
     {
       const uint32_t nchild = 2u << i;
-      const uint32_t boff = (cur ^ 1u) * T;
       for (uint32_t c = tid; c < nchild; c += kPassThreads) {
         const uint32_t q = c >> 1, bit = c & 1;
         const uint32_t ha = s_hc[q << gshift];
@@ -588,13 +593,28 @@ void pass_kernel(PassParams p) {
           Oa += __popc(bits & ((1u << off) - 1u));
         }
         const uint32_t child_lo = c << (gshift - 1);
-        s_base[c] = boff + s_hc[child_lo] - (bit ? Oa : ha - Oa);
+        s_base[c] = s_hc[child_lo] - (bit ? Oa : ha - Oa);
       }
     }
     __syncthreads();
-    Code* both = reinterpret_cast<Code*>(&s_buf[0][0]);
+    Code* both = reinterpret_cast<Code*>(s_buf);
     const uint32_t cwidth = i + 1;  // child index = top i+1 digit bits
-    if (npres == T) {
+    if (npres == T && i == 0) {
+      // one group, two children: the bases stay in registers
+      const uint32_t base0 = s_base[0], base1 = s_base[1];
+#pragma unroll
+      for (uint32_t r = 0; r < kChunkRows; ++r) {
+        const uint32_t p0 = (r * kPassThreads + tid) * E;
+        uint32_t o = O[r], z = p0 - O[r];
+#pragma unroll
+        for (uint32_t j = 0; j < E; ++j) {
+          const uint32_t b = child_of<Code>(v[r], j, cbit, 1);
+          both[(b ? base1 + o : base0 + z)] = chunk_elem<Code>(v[r], j);
+          o += b;
+          z += b ^ 1u;
+        }
+      }
+    } else if (npres == T) {
       // every code of the tile is present at this level: no per-code tests
 #pragma unroll
       for (uint32_t r = 0; r < kChunkRows; ++r) {
@@ -632,7 +652,6 @@ void pass_kernel(PassParams p) {
       }
     }
     __syncthreads();
-    cur ^= 1;
   }
 
   // ---- codes of level l0+k for the next pass, grouped by digit
@@ -644,7 +663,7 @@ void pass_kernel(PassParams p) {
       s_pc[d] = ob[d] + (s_ec[d + 1] - s_ec[d]) - s_hc[d];
     __syncthreads();
     const uint32_t nout = s_hc[s_thr[k]];
-    const Code* cb = reinterpret_cast<const Code*>(s_buf[cur]);
+    const Code* cb = reinterpret_cast<const Code*>(s_buf);
     Code* dst = static_cast<Code*>(p.dst);
     const uint32_t dmask = nb - 1;
     for (uint32_t x = tid; x < nout; x += kPassThreads) {
