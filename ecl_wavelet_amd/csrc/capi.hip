@@ -344,20 +344,18 @@ static void build_directories(ecl_rank_select_t* rs, ecl_bit_array_t* ba, hipStr
     d.cap = rs->cap[i];
     d.total_ones = rs->total_ones.as<uint64_t>() + i;
     d.first_tile = tile_array.size();
-    tile_array.insert(tile_array.end(), rs_build_tiles(d.num_l1),
-                      static_cast<uint32_t>(desc.size()));
+    tile_array.insert(tile_array.end(), d.num_l1, static_cast<uint32_t>(desc.size()));
     desc.push_back(d);
   }
   if (desc.empty()) return;
-  const uint64_t tiles = tile_array.size();
+  const uint64_t blocks = tile_array.size();
   DevBuf d_desc(desc.size() * sizeof(RsArrayDesc));
-  DevBuf d_tiles(tiles * sizeof(uint32_t));
-  DevBuf status((tiles + 1) * sizeof(uint64_t));
+  DevBuf d_blocks(blocks * sizeof(uint32_t));
+  DevBuf d_ones(blocks * sizeof(uint32_t));
   h2d(d_desc.ptr, desc.data(), desc.size());
-  h2d(d_tiles.ptr, tile_array.data(), tiles);
-  launch_fill_u64(status.as<uint64_t>(), tiles + 1, 0, s);
-  launch_rs_build(d_desc.as<RsArrayDesc>(), d_tiles.as<uint32_t>(), tiles, status.as<uint64_t>(),
-                  status.as<uint64_t>() + tiles, s);
+  h2d(d_blocks.ptr, tile_array.data(), blocks);
+  launch_rs_build(d_desc.as<RsArrayDesc>(), desc.size(), d_blocks.as<uint32_t>(), blocks,
+                  d_ones.as<uint32_t>(), s);
   // the buffers are freed on return: finish with them first
   ECL_HIP(hipStreamSynchronize(s));
 }

@@ -333,18 +333,16 @@ ecl_status ecl_forest_create(const void* d_texts, const uint64_t* starts, const 
           d.cap = t.cap[l];
           d.total_ones = reinterpret_cast<uint64_t*>(A + t.ones_at) + l;
           d.first_tile = tile_array.size();
-          tile_array.insert(tile_array.end(), rs_build_tiles(d.num_l1),
-                            static_cast<uint32_t>(desc.size()));
+          tile_array.insert(tile_array.end(), d.num_l1, static_cast<uint32_t>(desc.size()));
           desc.push_back(d);
         }
       }
-      const uint64_t tiles = tile_array.size();
-      DevBuf d_desc(desc.size() * sizeof(RsArrayDesc)), d_ta(tiles * 4), status((tiles + 1) * 8);
+      const uint64_t blocks = tile_array.size();
+      DevBuf d_desc(desc.size() * sizeof(RsArrayDesc)), d_ta(blocks * 4), d_ones(blocks * 4);
       h2d(d_desc.ptr, desc.data(), desc.size());
-      h2d(d_ta.ptr, tile_array.data(), tiles);
-      launch_fill_u64(status.as<uint64_t>(), tiles + 1, 0, s);
-      launch_rs_build(d_desc.as<RsArrayDesc>(), d_ta.as<uint32_t>(), tiles, status.as<uint64_t>(),
-                      status.as<uint64_t>() + tiles, s);
+      h2d(d_ta.ptr, tile_array.data(), blocks);
+      launch_rs_build(d_desc.as<RsArrayDesc>(), desc.size(), d_ta.as<uint32_t>(), blocks,
+                      d_ones.as<uint32_t>(), s);
       ECL_HIP(hipStreamSynchronize(s));
     }
 

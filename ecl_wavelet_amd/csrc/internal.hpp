@@ -122,18 +122,18 @@ void launch_gather_rows(const uint64_t* excl, const uint32_t* rows, uint64_t nro
                         uint32_t nb, uint64_t* out, hipStream_t s);
 
 // One bit array of a directory build: its words, its directory arrays and
-// the first global tile of its look-back chain.
+// the global index of its first L1 block.
 struct RsArrayDesc {
   const uint64_t* words64; uint64_t m;
   uint64_t* l1; uint16_t* l2; uint64_t num_l1, num_l2;
   uint64_t* samples; uint64_t cap;
   uint64_t* total_ones;
-  uint64_t first_tile;
+  uint64_t first_tile;  // first global L1 block of the array
 };
-uint64_t rs_build_tiles(uint64_t num_l1);  // tiles of one array's chain
-// status: tiles + 1 zeroed u64 (the last one is the tile counter)
-void launch_rs_build(const RsArrayDesc* arrays, const uint32_t* tile_array, uint64_t tiles,
-                     uint64_t* status, uint64_t* counter, hipStream_t s);
+// arrays: narrays entries; block_array: array of each of the nblocks global
+// L1 blocks; block_ones: nblocks u32 scratch
+void launch_rs_build(const RsArrayDesc* arrays, uint64_t narrays, const uint32_t* block_array,
+                     uint64_t nblocks, uint32_t* block_ones, hipStream_t s);
 
 void launch_gen_text(void* out, int esize, uint64_t n, uint64_t seed,
                      uint64_t sigma, const uint64_t* thresholds,

@@ -20,73 +20,6 @@ using detail::ballot64;
 using detail::lane_id;
 using detail::mbcnt64;
 
-constexpr uint64_t kFlagAgg = 1ull << 62;
-constexpr uint64_t kFlagPre = 2ull << 62;
-constexpr uint64_t kValMask = (1ull << 62) - 1;
-
-__device__ __forceinline__ void status_store(uint64_t* p, uint64_t v) {
-  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ uint64_t status_load(const uint64_t* p) {
-  return __hip_atomic_load(const_cast<uint64_t*>(p), __ATOMIC_RELAXED,
-                           __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Decoupled look-back, run by all 64 lanes of one wave.  Publishes this
-// tile's aggregate, then reads the status of up to 64*J predecessors per
-// step (lane i covers predecessors 8i+1 .. 8i+J nearest-first) until it
-// meets an inclusive prefix, publishes its own inclusive prefix and returns
-// the exclusive one.  Publication can only advance by one window per
-// status-load latency, so the window is wide and the tiles are large.
-// Tiles take their ids in launch order from an atomic counter: every
-// predecessor is resident or finished, so the wait ends.
-// Tiles of several independent chains may share one status array: the
-// first tile of each chain publishes its inclusive prefix at once, so every
-// later tile of the chain meets an inclusive prefix before leaving it.
-template <int J>
-__device__ uint64_t lookback(uint64_t* status, uint64_t tile, uint64_t agg,
-                             uint64_t first = 0) {
-  const uint32_t lane = lane_id();
-  if (tile == first) {
-    if (lane == 0) status_store(&status[tile], kFlagPre | agg);
-    return 0;
-  }
-  if (lane == 0) status_store(&status[tile], kFlagAgg | agg);
-  uint64_t excl = 0;
-  int64_t top = static_cast<int64_t>(tile) - 1;
-  while (true) {
-    uint64_t v[J];
-#pragma unroll
-    for (int j = 0; j < J; ++j) {
-      const int64_t idx = top - static_cast<int64_t>(lane) * J - j;
-      v[j] = idx >= 0 ? status_load(&status[idx]) : kFlagPre;
-    }
-    int first_pre = J;
-    bool empty = false;
-#pragma unroll
-    for (int j = J - 1; j >= 0; --j)
-      if ((v[j] >> 62) == 2) first_pre = j;
-#pragma unroll
-    for (int j = 0; j < J; ++j)
-      if (j <= first_pre && (v[j] >> 62) == 0) empty = true;
-    const uint64_t pre_mask = ballot64(first_pre < J);
-    const uint32_t stop = pre_mask ? static_cast<uint32_t>(__builtin_ctzll(pre_mask)) : 64u;
-    if (ballot64(lane <= stop && empty)) {
-      __builtin_amdgcn_s_sleep(1);
-      continue;
-    }
-    uint64_t sum = 0;
-#pragma unroll
-    for (int j = 0; j < J; ++j)
-      if (lane < stop || (lane == stop && j <= first_pre)) sum += v[j] & kValMask;
-    excl += detail::wave_sum_u64(sum);
-    if (pre_mask) break;
-    top -= 64 * J;
-  }
-  if (lane == 0) status_store(&status[tile], kFlagPre | (excl + agg));
-  return excl;
-}
-constexpr int kLookbackWords = 8;  // window of 512 tiles
 
 
 // ---------------------------------------------------------------------------
@@ -177,28 +110,28 @@ __global__ __launch_bounds__(256) void histogram_kernel(HistParams p,
 }
 
 // ---------------------------------------------------------------------------
-// RankSelect directory: 8 L1 blocks (8 x 1024 u64 words) per workgroup of
-// 512 threads; wave w owns L1 block 8 * tile + w, 16 words per lane.  One
-// look-back per workgroup gives the ones before its first L1 block.  One
-// launch covers any number of bit arrays: global tile t belongs to array
-// tile_array[t] and is tile t - first_tile of that array's chain.
-constexpr uint32_t kRsBlocksPerTile = 8;
+// RankSelect directory in three steps, none of which waits on another
+// workgroup:
+//  1. rs_local_kernel: one wave per 65536-bit L1 block (16 u64 words per
+//     lane): popcounts, in-block prefix = the block's L2 entries, block total.
+//  2. rs_scan_kernel: one workgroup per bit array: exclusive scan of its block
+//     totals = the L1 entries, and the array's total.
+//  3. rs_samples_kernel: one wave per L1 block, one lane per select sample
+//     that falls inside the block: binary search over the block's L2 entries,
+//     then one 64-byte L2 block is read and the bit is selected in-word.
+// Global L1 block g belongs to array block_array[g], whose first block is
+// arrays[...].first_tile.
+constexpr uint32_t kRsWaves = 8;  // L1 blocks per workgroup
 
-__global__ __launch_bounds__(512) void rs_build_kernel(const RsArrayDesc* __restrict__ arrays,
-                                                       const uint32_t* __restrict__ tile_array,
-                                                       uint64_t* status, uint64_t* counter) {
-  __shared__ uint64_t s_tile;
-  __shared__ uint64_t s_excl;
-  __shared__ uint32_t s_wave[kRsBlocksPerTile];
-  const uint32_t tid = threadIdx.x;
-  const uint32_t lane = tid & 63, wave = tid >> 6;
-  if (tid == 0)
-    s_tile = atomicAdd(reinterpret_cast<unsigned long long*>(counter), 1ull);
-  __syncthreads();
-  const uint64_t gtile = s_tile;
-  const RsArrayDesc p = arrays[tile_array[gtile]];
-  const uint64_t tile = gtile - p.first_tile;
-  const uint64_t kb = tile * kRsBlocksPerTile + wave;  // this wave's L1 block
+__global__ __launch_bounds__(512) void rs_local_kernel(const RsArrayDesc* __restrict__ arrays,
+                                                       const uint32_t* __restrict__ block_array,
+                                                       uint64_t nblocks,
+                                                       uint32_t* __restrict__ block_ones) {
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint64_t gb = static_cast<uint64_t>(blockIdx.x) * kRsWaves + wave;
+  if (gb >= nblocks) return;
+  const RsArrayDesc p = arrays[block_array[gb]];
+  const uint64_t kb = gb - p.first_tile;  // block within its array
   const uint64_t nwords = (p.m + 63) / 64;
   const uint64_t w0 = kb * 1024 + static_cast<uint64_t>(lane) * 16;
   uint64_t w[16];
@@ -223,74 +156,113 @@ __global__ __launch_bounds__(512) void rs_build_kernel(const RsArrayDesc* __rest
     c += pc;
     if (j < 8) c_first += pc;
   }
-  // ones in the L1 block before this lane; lane covers L2 blocks 2l, 2l+1
+  // ones in the block before this lane; the lane covers L2 blocks 2l, 2l+1
   const uint32_t incl = detail::wave_incl_scan_u32(c);
   const uint32_t excl = incl - c;
-  {
-    const uint64_t k2 = kb * detail::kL2PerL1 + 2ull * lane;
-    if (k2 < p.num_l2) p.l2[k2] = static_cast<uint16_t>(excl);
-    if (k2 + 1 < p.num_l2) p.l2[k2 + 1] = static_cast<uint16_t>(excl + c_first);
+  const uint64_t k2 = kb * detail::kL2PerL1 + 2ull * lane;
+  if (k2 < p.num_l2) p.l2[k2] = static_cast<uint16_t>(excl);
+  if (k2 + 1 < p.num_l2) p.l2[k2 + 1] = static_cast<uint16_t>(excl + c_first);
+  if (lane == 63) block_ones[gb] = incl;
+}
+
+// one workgroup (256 threads) per array: L1 = exclusive prefix of the
+// array's block totals; total ones at the end
+__global__ __launch_bounds__(256) void rs_scan_kernel(const RsArrayDesc* __restrict__ arrays,
+                                                      const uint32_t* __restrict__ block_ones) {
+  __shared__ uint64_t s_wave[4];
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const RsArrayDesc p = arrays[blockIdx.x];
+  const uint32_t* cnt = block_ones + p.first_tile;
+  const uint64_t per = (p.num_l1 + 255) / 256;
+  const uint64_t b0 = tid * per;
+  const uint64_t b1 = (b0 + per < p.num_l1) ? b0 + per : p.num_l1;
+  uint64_t sum = 0;
+  for (uint64_t b = b0; b < b1; ++b) sum += cnt[b];
+  uint64_t incl = sum;
+#pragma unroll
+  for (uint32_t off = 1; off < 64; off <<= 1) {
+    const uint64_t y = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += y;
   }
   if (lane == 63) s_wave[wave] = incl;
   __syncthreads();
-  uint32_t wbefore = 0, ttot = 0;
-#pragma unroll
-  for (uint32_t ww = 0; ww < kRsBlocksPerTile; ++ww) {
-    const uint32_t x = s_wave[ww];
-    wbefore += (ww < wave) ? x : 0u;
-    ttot += x;
+  uint64_t run = incl - sum, total = 0;
+  for (uint32_t w = 0; w < 4; ++w) {
+    run += (w < wave) ? s_wave[w] : 0ull;
+    total += s_wave[w];
   }
-  if (wave == 0) {
-    const uint64_t g = lookback<kLookbackWords>(status, gtile, ttot, p.first_tile);
-    if (lane == 0) s_excl = g;
+  for (uint64_t b = b0; b < b1; ++b) {
+    p.l1[b] = run;
+    run += cnt[b];
   }
-  __syncthreads();
-  const uint64_t gb = s_excl + wbefore;  // ones before L1 block kb
-  if (lane == 0 && kb < p.num_l1) p.l1[kb] = gb;
-  if (lane == 0 && kb + 1 == p.num_l1) *p.total_ones = gb + s_wave[wave];
+  if (tid == 0) *p.total_ones = total;
+}
 
-  // select samples: at most one multiple of 4096 ones (and of zeros) falls
-  // inside a lane's 1024 bits
-  const uint64_t b0 = w0 * 64;
-  if (p.cap == 0 || b0 >= p.m) return;
-  const uint64_t b1 = (b0 + 1024 < p.m) ? b0 + 1024 : p.m;
-  const uint64_t ones_before = gb + excl;
-  const uint64_t k1 = (ones_before + c) / detail::kSampleRate;
-  if (k1 > ones_before / detail::kSampleRate) {
-    uint32_t r = static_cast<uint32_t>(k1 * detail::kSampleRate - ones_before);
-    uint64_t pos = b0;
+// position of the r-th (1-based) `value` bit inside the 8 words at q
+template <int Value>
+__device__ __forceinline__ uint32_t select_in_block(const uint64_t* q, uint32_t r,
+                                                    uint64_t valid_bits) {
+  uint32_t pos = 0;
 #pragma unroll 1
-    for (int j = 0; j < 16; ++j) {
-      const uint32_t pc = static_cast<uint32_t>(__popcll(w[j]));
-      if (r <= pc) {
-        pos += detail::select_in_word(w[j], r);
-        break;
-      }
-      r -= pc;
-      pos += 64;
-    }
-    p.samples[k1 - 1] = pos;
+  for (uint32_t j = 0; j < 8; ++j) {
+    uint64_t x = Value ? q[j] : ~q[j];
+    if (valid_bits < 64ull * (j + 1))
+      x &= valid_bits > 64ull * j ? ((1ull << (valid_bits - 64ull * j)) - 1) : 0ull;
+    const uint32_t pc = static_cast<uint32_t>(__popcll(x));
+    if (r <= pc) return pos + detail::select_in_word(x, r);
+    r -= pc;
+    pos += 64;
   }
-  const uint64_t zeros_before = b0 - ones_before;
-  const uint64_t zc = (b1 - b0) - c;
-  const uint64_t k0 = (zeros_before + zc) / detail::kSampleRate;
-  if (k0 > zeros_before / detail::kSampleRate) {
-    uint32_t r = static_cast<uint32_t>(k0 * detail::kSampleRate - zeros_before);
-    uint64_t pos = b0;
-#pragma unroll 1
-    for (int j = 0; j < 16; ++j) {
-      uint64_t z = ~w[j];
-      const uint64_t wb = b0 + 64ull * j;
-      if (wb + 64 > p.m) z &= (p.m > wb) ? ((1ull << (p.m - wb)) - 1) : 0ull;
-      const uint32_t pc = static_cast<uint32_t>(__popcll(z));
-      if (r <= pc) {
-        pos += detail::select_in_word(z, r);
-        break;
-      }
-      r -= pc;
-      pos += 64;
+  return pos;  // not reached for a valid rank
+}
+
+__global__ __launch_bounds__(512) void rs_samples_kernel(const RsArrayDesc* __restrict__ arrays,
+                                                         const uint32_t* __restrict__ block_array,
+                                                         uint64_t nblocks) {
+  __shared__ uint16_t s_l2[kRsWaves][detail::kL2PerL1];
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const uint64_t gb = static_cast<uint64_t>(blockIdx.x) * kRsWaves + wave;
+  if (gb >= nblocks) return;
+  const RsArrayDesc p = arrays[block_array[gb]];
+  if (p.cap == 0) return;
+  const uint64_t kb = gb - p.first_tile;
+  const uint64_t bits0 = kb * detail::kL1Bits;                     // first bit of the block
+  const uint64_t bits = p.m - bits0 < detail::kL1Bits ? p.m - bits0 : detail::kL1Bits;
+  const uint64_t nl2 = (bits + detail::kL2Bits - 1) / detail::kL2Bits;  // L2 blocks here
+  const uint64_t o0 = p.l1[kb];
+  const uint64_t o1 = kb + 1 < p.num_l1 ? p.l1[kb + 1] : *p.total_ones;
+  const uint64_t z0 = bits0 - o0, z1 = bits0 + bits - o1;
+  // samples in this block: ones ranks (o0, o1], zeros ranks (z0, z1]
+  const uint64_t first1 = o0 / detail::kSampleRate + 1, last1 = o1 / detail::kSampleRate;
+  const uint64_t first0 = z0 / detail::kSampleRate + 1, last0 = z1 / detail::kSampleRate;
+  const uint64_t n1 = last1 >= first1 ? last1 - first1 + 1 : 0;
+  const uint64_t n0 = last0 >= first0 ? last0 - first0 + 1 : 0;
+  if (n1 + n0 == 0) return;
+  for (uint32_t x = lane; x < detail::kL2PerL1; x += 64)
+    s_l2[wave][x] = x < nl2 ? p.l2[kb * detail::kL2PerL1 + x] : 0xFFFFu;
+  // the wave's LDS writes complete before its reads (in-order LDS queue)
+  __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
+  __builtin_amdgcn_wave_barrier();
+  // lanes 0 .. n1-1: ones samples; n1 .. n1+n0-1: zeros samples (<= 32 in all)
+  if (lane < n1 + n0) {
+    const bool one = lane < n1;
+    const uint64_t j = one ? first1 + lane : first0 + (lane - n1);  // sample number
+    const uint32_t r = static_cast<uint32_t>(j * detail::kSampleRate - (one ? o0 : z0));
+    // last L2 block whose prefix (of this value) is below r
+    uint32_t lo = 0, hi = static_cast<uint32_t>(nl2);
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) >> 1;
+      const uint32_t before = one ? s_l2[wave][mid] : mid * 512u - s_l2[wave][mid];
+      if (before < r) lo = mid; else hi = mid;
     }
-    p.samples[p.cap - k0] = pos;
+    const uint32_t before = one ? s_l2[wave][lo] : lo * 512u - s_l2[wave][lo];
+    const uint64_t wb = (bits0 >> 6) + lo * 8ull;  // first word of the L2 block
+    const uint64_t valid = p.m - (wb << 6);
+    const uint32_t pos = one ? select_in_block<1>(p.words64 + wb, r - before, valid)
+                             : select_in_block<0>(p.words64 + wb, r - before, valid);
+    const uint64_t at = (wb << 6) + pos;
+    if (one) p.samples[j - 1] = at;
+    else p.samples[p.cap - j] = at;
   }
 }
 
@@ -409,17 +381,21 @@ void launch_histogram(const HistParams& p, hipStream_t s) {
   ECL_HIP(hipGetLastError());
 }
 
-uint64_t rs_build_tiles(uint64_t num_l1) {
-  return (num_l1 + kRsBlocksPerTile - 1) / kRsBlocksPerTile;
-}
-
-void launch_rs_build(const RsArrayDesc* arrays, const uint32_t* tile_array, uint64_t tiles,
-                     uint64_t* status, uint64_t* counter, hipStream_t s) {
-  if (tiles == 0) return;
-  if (tiles > 0x7FFFFFFFull) throw Error(ECL_ERROR_INVALID_ARGUMENT, "bit arrays too large");
+void launch_rs_build(const RsArrayDesc* arrays, uint64_t narrays, const uint32_t* block_array,
+                     uint64_t nblocks, uint32_t* block_ones, hipStream_t s) {
+  if (nblocks == 0) return;
+  const uint64_t groups = (nblocks + kRsWaves - 1) / kRsWaves;
+  if (groups > 0x7FFFFFFFull || narrays > 0x7FFFFFFFull)
+    throw Error(ECL_ERROR_INVALID_ARGUMENT, "bit arrays too large");
   ProfScope ps(K_RS_BUILD, s);
-  hipLaunchKernelGGL(rs_build_kernel, dim3(static_cast<uint32_t>(tiles)), dim3(512), 0, s,
-                     arrays, tile_array, status, counter);
+  hipLaunchKernelGGL(rs_local_kernel, dim3(static_cast<uint32_t>(groups)), dim3(512), 0, s,
+                     arrays, block_array, nblocks, block_ones);
+  ECL_HIP(hipGetLastError());
+  hipLaunchKernelGGL(rs_scan_kernel, dim3(static_cast<uint32_t>(narrays)), dim3(256), 0, s,
+                     arrays, block_ones);
+  ECL_HIP(hipGetLastError());
+  hipLaunchKernelGGL(rs_samples_kernel, dim3(static_cast<uint32_t>(groups)), dim3(512), 0, s,
+                     arrays, block_array, nblocks);
   ECL_HIP(hipGetLastError());
 }
 
