@@ -198,22 +198,36 @@ __global__ __launch_bounds__(256) void rs_scan_kernel(const RsArrayDesc* __restr
   if (tid == 0) *p.total_ones = total;
 }
 
-// position of the r-th (1-based) `value` bit inside the 8 words at q
+// position of the r-th (1-based) `value` bit inside the 8 words at q (one
+// 64-byte L2 block, loaded at once; bits at or past valid_bits ignored)
 template <int Value>
 __device__ __forceinline__ uint32_t select_in_block(const uint64_t* q, uint32_t r,
                                                     uint64_t valid_bits) {
-  uint32_t pos = 0;
-#pragma unroll 1
-  for (uint32_t j = 0; j < 8; ++j) {
-    uint64_t x = Value ? q[j] : ~q[j];
-    if (valid_bits < 64ull * (j + 1))
-      x &= valid_bits > 64ull * j ? ((1ull << (valid_bits - 64ull * j)) - 1) : 0ull;
-    const uint32_t pc = static_cast<uint32_t>(__popcll(x));
-    if (r <= pc) return pos + detail::select_in_word(x, r);
-    r -= pc;
-    pos += 64;
+  const ulonglong2* v = reinterpret_cast<const ulonglong2*>(q);
+  uint64_t x[8];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const ulonglong2 t = v[j];
+    x[2 * j] = t.x;
+    x[2 * j + 1] = t.y;
   }
-  return pos;  // not reached for a valid rank
+  uint32_t pos = 0, found = 0, at = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < 8; ++j) {
+    uint64_t y = Value ? x[j] : ~x[j];
+    if (valid_bits < 64ull * (j + 1))
+      y &= valid_bits > 64ull * j ? ((1ull << (valid_bits - 64ull * j)) - 1) : 0ull;
+    const uint32_t pc = static_cast<uint32_t>(__popcll(y));
+    if (!found && r <= pc) {
+      found = 1;
+      at = pos + detail::select_in_word(y, r);
+    }
+    if (!found) {
+      r -= pc;
+      pos += 64;
+    }
+  }
+  return at;
 }
 
 __global__ __launch_bounds__(512) void rs_samples_kernel(const RsArrayDesc* __restrict__ arrays,
