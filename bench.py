@@ -82,6 +82,19 @@ def pass_bytes(n, sigma):
     return n + L * n / 8
 
 
+def pass_traffic(n, sigma):
+    """HBM bytes per construction-pass launch from the committed counter
+    profile (FETCH_SIZE and WRITE_SIZE scaled by a 1 GiB copy's counts), for
+    the workload that profile covers; None for any other workload."""
+    path = os.path.join(ROOT, "profiles", "wt_pass_counters_u8_n6e9_sigma256.txt")
+    if (n, sigma) != (6_000_000_000, 256) or not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
 def cpu_baseline(n, sigma, seed, args):
     import oracle as O
     t0 = time.time()
@@ -459,7 +472,8 @@ def main():
             "peak": HBM_PEAK_GBPS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBPS,
-            "traffic": None,
+            "traffic": pass_traffic(n, sigma),
+            "traffic_source": "profiles/wt_pass_counters_u8_n6e9_sigma256.txt",
             "bytes_per_launch": pb,
             "launch_ms": pass_ms,
         },
