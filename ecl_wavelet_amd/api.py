@@ -635,6 +635,12 @@ def device_name(device=0):
     return buf.value.decode()
 
 
+def release_cached_memory(device=0):
+    """Frees the construction scratch the library keeps per device between
+    builds."""
+    call("ecl_release_cached_memory", device)
+
+
 def mem_info(device=0):
     fr = C.c_size_t()
     tot = C.c_size_t()

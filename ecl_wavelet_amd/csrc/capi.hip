@@ -56,6 +56,91 @@ void DevBuf::reset() {
   bytes = 0;
 }
 
+// ---- uploads ----------------------------------------------------------------
+void Uploader::clear() {
+  b_.nseg = 0;
+  b_.off[0] = 0;
+}
+void Uploader::flush() {
+  if (b_.nseg) launch_upload(b_, s_);
+  clear();
+}
+void Uploader::add_bytes(void* dst, const void* src, size_t bytes) {
+  if (bytes == 0) return;
+  if (bytes > (64u << 10)) {
+    flush();
+    ECL_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return;
+  }
+  const uint8_t* from = static_cast<const uint8_t*>(src);
+  uint8_t* to = static_cast<uint8_t*>(dst);
+  while (bytes) {
+    if (b_.nseg == kUploadSegs || b_.off[b_.nseg] == kUploadBytes) flush();
+    const uint32_t at = b_.off[b_.nseg];
+    const size_t take = std::min<size_t>(bytes, kUploadBytes - at);
+    std::memcpy(b_.data + at, from, take);
+    b_.dst[b_.nseg] = reinterpret_cast<uint64_t>(to);
+    b_.off[++b_.nseg] = at + static_cast<uint32_t>(take);
+    from += take;
+    to += take;
+    bytes -= take;
+  }
+}
+
+// ---- per-device scratch cache ------------------------------------------------
+namespace {
+struct ScratchCache {
+  std::mutex mu;
+  DevBuf buf;
+};
+constexpr int kMaxDevices = 64;
+// never destroyed: freeing device memory during static destruction may
+// run after the HIP runtime has shut down
+ScratchCache* const g_scratch = new ScratchCache[kMaxDevices];
+}  // namespace
+
+Scratch::Scratch(int device, hipStream_t s, size_t bytes) : device_(device), s_(s) {
+  bytes = std::max<size_t>(units(bytes), 256);
+  if (device >= 0 && device < kMaxDevices && g_scratch[device].mu.try_lock()) {
+    ScratchCache& c = g_scratch[device];
+    try {
+      if (c.buf.bytes < bytes) {
+        c.buf.reset();
+        c.buf = DevBuf(bytes);
+      }
+    } catch (...) {
+      c.mu.unlock();
+      throw;
+    }
+    cached_ = true;
+    base_ = c.buf.as<uint8_t>();
+    cap_ = c.buf.bytes;
+  } else {
+    own_ = DevBuf(bytes);
+    base_ = own_.as<uint8_t>();
+    cap_ = bytes;
+  }
+}
+Scratch::~Scratch() {
+  // kernels that use the buffer finish before another build may take it
+  (void)hipStreamSynchronize(s_);
+  (void)hipGetLastError();
+  if (cached_) g_scratch[device_].mu.unlock();
+}
+size_t cached_scratch_bytes(int device) {
+  if (device < 0 || device >= kMaxDevices) return 0;
+  ScratchCache& c = g_scratch[device];
+  if (!c.mu.try_lock()) return 0;
+  const size_t b = c.buf.bytes;
+  c.mu.unlock();
+  return b;
+}
+void release_scratch_cache(int device) {
+  if (device < 0 || device >= kMaxDevices) return;
+  std::lock_guard<std::mutex> lk(g_scratch[device].mu);
+  g_scratch[device].buf.reset();
+}
+
 static const char* kKernelNames[K_COUNT] = {
     "histogram", "wt_pass", "rs_build", "wt_access", "wt_rank", "wt_select",
     "wt_mixed", "rs_rank", "rs_select", "generate", "fill", "tile_hist", "tile_scan",
@@ -220,8 +305,10 @@ struct ecl_wavelet_tree_t {
 // ===========================================================================
 // BitArray
 // ===========================================================================
+// Allocates the arrays; `up`, when given, carries the size/offset tables
+// (else they are copied before returning).
 static ecl_bit_array_t* make_bit_array(const std::vector<uint64_t>& sizes, int init,
-                                       int device) {
+                                       int device, Uploader* up = nullptr) {
   require(sizes.size() < 255, "a BitArray holds at most 254 arrays");
   auto* ba = new ecl_bit_array_t();
   try {
@@ -240,8 +327,13 @@ static ecl_bit_array_t* make_bit_array(const std::vector<uint64_t>& sizes, int i
     ba->data = DevBuf(total * sizeof(uint32_t));
     ba->d_sizes = DevBuf(std::max<size_t>(1, sizes.size()) * sizeof(uint64_t));
     ba->d_offsets = DevBuf(std::max<size_t>(1, sizes.size()) * sizeof(uint64_t));
-    h2d(ba->d_sizes.ptr, ba->sizes.data(), sizes.size());
-    h2d(ba->d_offsets.ptr, ba->offsets.data(), sizes.size());
+    if (up != nullptr) {
+      up->add(ba->d_sizes.ptr, ba->sizes.data(), sizes.size());
+      up->add(ba->d_offsets.ptr, ba->offsets.data(), sizes.size());
+    } else {
+      h2d(ba->d_sizes.ptr, ba->sizes.data(), sizes.size());
+      h2d(ba->d_offsets.ptr, ba->offsets.data(), sizes.size());
+    }
     if (init >= 0)
       launch_fill_u32(ba->words(), total, init ? 0xFFFFFFFFu : 0u, nullptr);
   } catch (...) {
@@ -290,25 +382,25 @@ static ecl_rank_select_t* alloc_rank_select(ecl_bit_array_t* ba, hipStream_t s) 
   return rs;
 }
 
-// Totals back to the host, reference-shaped device directory and the
-// LevelView table; takes ownership of `ba`.
-static void finalize_rank_select(ecl_rank_select_t* rs, ecl_bit_array_t* ba, hipStream_t s) {
+// Totals back to the host (the directory kernels must have finished),
+// reference-shaped device directory and the LevelView table through `up`;
+// takes ownership of `ba`.
+static void finalize_rank_select(ecl_rank_select_t* rs, ecl_bit_array_t* ba, Uploader& up) {
   const size_t na = ba->sizes.size();
-  ECL_HIP(hipStreamSynchronize(s));
   d2h(rs->ones.data(), rs->total_ones.ptr, na);
-  auto up = [&](DevBuf& b, const std::vector<uint64_t>& v) {
+  auto put = [&](DevBuf& b, const std::vector<uint64_t>& v) {
     b = DevBuf(std::max<size_t>(v.size(), 1) * sizeof(uint64_t));
-    h2d(b.ptr, v.data(), v.size());
+    up.add(b.ptr, v.data(), v.size());
   };
-  up(rs->d_l1_off, rs->l1_off);
-  up(rs->d_l2_off, rs->l2_off);
-  up(rs->d_num_l1, rs->num_l1);
-  up(rs->d_s_off, rs->s_off);
-  up(rs->d_cap, rs->cap);
+  put(rs->d_l1_off, rs->l1_off);
+  put(rs->d_l2_off, rs->l2_off);
+  put(rs->d_num_l1, rs->num_l1);
+  put(rs->d_s_off, rs->s_off);
+  put(rs->d_cap, rs->cap);
   std::vector<uint16_t> last16(na);
   for (size_t i = 0; i < na; ++i) last16[i] = static_cast<uint16_t>(rs->num_last_l2[i]);
   rs->d_num_last_l2 = DevBuf(std::max<size_t>(na, 1) * sizeof(uint16_t));
-  h2d(rs->d_num_last_l2.ptr, last16.data(), na);
+  up.add(rs->d_num_last_l2.ptr, last16.data(), na);
   rs->levels.resize(na);
   for (size_t i = 0; i < na; ++i) {
     LevelView& v = rs->levels[i];
@@ -322,15 +414,27 @@ static void finalize_rank_select(ecl_rank_select_t* rs, ecl_bit_array_t* ba, hip
     v.sample_cap = rs->cap[i];
   }
   rs->d_levels = DevBuf(std::max<size_t>(na, 1) * sizeof(LevelView));
-  h2d(rs->d_levels.ptr, rs->levels.data(), na);
+  up.add(rs->d_levels.ptr, rs->levels.data(), na);
   rs->ba = ba;
 }
 
-// Directory of every non-empty array of `ba` into `rs`, one launch.
-static void build_directories(ecl_rank_select_t* rs, ecl_bit_array_t* ba, hipStream_t s) {
+// Scratch of build_directories for `ba`: the array table and 2 u32 per L1
+// block (upper bound for arrays of the given sizes).
+static size_t directory_scratch_bytes(const std::vector<uint64_t>& sizes) {
+  uint64_t blocks = 0;
+  for (uint64_t m : sizes) blocks += (m + detail::kL1Bits - 1) / detail::kL1Bits;
+  return Scratch::units(std::max<size_t>(sizes.size(), 1) * sizeof(RsArrayDesc)) +
+         Scratch::units(std::max<uint64_t>(blocks, 1) * 8);
+}
+
+// Directory of every non-empty array of `ba` into `rs`: the array table goes
+// up through `up`, then the directory kernels are queued on `s` (nothing
+// waits for them here).
+static void build_directories(ecl_rank_select_t* rs, ecl_bit_array_t* ba, hipStream_t s,
+                              Scratch& scr, Uploader& up) {
   const size_t na = ba->sizes.size();
   std::vector<RsArrayDesc> desc;
-  std::vector<uint32_t> tile_array;
+  uint64_t blocks = 0;
   for (size_t i = 0; i < na; ++i) {
     if (ba->sizes[i] == 0) continue;
     RsArrayDesc d{};
@@ -343,29 +447,29 @@ static void build_directories(ecl_rank_select_t* rs, ecl_bit_array_t* ba, hipStr
     d.samples = rs->samples.as<uint64_t>() + rs->s_off[i];
     d.cap = rs->cap[i];
     d.total_ones = rs->total_ones.as<uint64_t>() + i;
-    d.first_tile = tile_array.size();
-    tile_array.insert(tile_array.end(), d.num_l1, static_cast<uint32_t>(desc.size()));
+    d.first_tile = blocks;
+    blocks += d.num_l1;
     desc.push_back(d);
   }
   if (desc.empty()) return;
-  const uint64_t blocks = tile_array.size();
-  DevBuf d_desc(desc.size() * sizeof(RsArrayDesc));
-  DevBuf d_blocks(blocks * sizeof(uint32_t));
-  DevBuf d_ones(blocks * sizeof(uint32_t));
-  h2d(d_desc.ptr, desc.data(), desc.size());
-  h2d(d_blocks.ptr, tile_array.data(), blocks);
-  launch_rs_build(d_desc.as<RsArrayDesc>(), desc.size(), d_blocks.as<uint32_t>(), blocks,
-                  d_ones.as<uint32_t>(), s);
-  // the buffers are freed on return: finish with them first
-  ECL_HIP(hipStreamSynchronize(s));
+  RsArrayDesc* d_desc = scr.carve<RsArrayDesc>(na);
+  uint32_t* d_blk = scr.carve<uint32_t>(2 * blocks);
+  up.add(d_desc, desc.data(), desc.size());
+  up.flush();
+  launch_rs_build(d_desc, desc.size(), blocks, d_blk, s);
 }
 
 // RankSelect(BitArray&&) (rank_select.cu:50-304): one rs_build pass per array.
 static ecl_rank_select_t* make_rank_select(ecl_bit_array_t* ba, hipStream_t s) {
   ecl_rank_select_t* rs = alloc_rank_select(ba, s);
   try {
-    build_directories(rs, ba, s);
-    finalize_rank_select(rs, ba, s);
+    Scratch scr(ba->device, s, directory_scratch_bytes(ba->sizes));
+    Uploader up(s);
+    build_directories(rs, ba, s, scr, up);
+    ECL_HIP(hipStreamSynchronize(s));
+    finalize_rank_select(rs, ba, up);
+    up.flush();
+    ECL_HIP(hipStreamSynchronize(s));
   } catch (...) {
     rs->ba = nullptr;
     delete rs;
@@ -618,9 +722,11 @@ static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_
     const uint64_t T = pass_tile_codes(cw, rows);   // codes per tile
     const uint64_t E = 16 / cw;               // codes per 16 bytes
 
-    // ---- memory admission (wavelet_tree.cuh:644-689)
+    // ---- memory admission (wavelet_tree.cuh:644-689); the build's scratch
+    // reuses the device's scratch cache when it holds one
     const uint64_t needed = tree_needed_bytes(n, sigma, L, esize, !text_on_device);
-    if (needed > free_device_memory()) throw Error(ECL_ERROR_OUT_OF_MEMORY_TREE, kOomTree);
+    if (needed > free_device_memory() + cached_scratch_bytes(device))
+      throw Error(ECL_ERROR_OUT_OF_MEMORY_TREE, kOomTree);
 
     // ---- the text on the device
     DevBuf own_text;
@@ -635,6 +741,13 @@ static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_
     const bool aligned = (reinterpret_cast<uintptr_t>(d_text) & 15) == 0;
     const bool identity = wt->min_alphabet && wt->pow2 && esize == cw && aligned;
 
+    // Every launch of the build is queued before the host waits for the
+    // device: small tables go up as kernel arguments (Uploader), and the one
+    // wait comes after the directory kernels.  Multi-pass trees and
+    // single-pass trees over a non-power-of-2 alphabet read their digit
+    // counts back after the first scan.
+    Uploader up(s);
+
     // ---- symbol maps
     if (wt->map_kind == 1) {
       const uint64_t range = esize == 1 ? 256 : 65536;
@@ -645,20 +758,21 @@ static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_
         lut[v] = static_cast<uint32_t>(std::min<uint64_t>(j, sigma));
       }
       wt->d_lut = DevBuf(range * 4);
-      h2d(wt->d_lut.ptr, lut.data(), range);
+      up.add(wt->d_lut.ptr, lut.data(), range);
     }
     wt->d_alphabet = DevBuf(sigma * esize);
     {
       std::vector<uint8_t> raw(sigma * esize);
       u64_to_elems(wt->alphabet, raw.data(), esize);
-      h2d(wt->d_alphabet.ptr, raw.data(), sigma * esize);
+      up.add(wt->d_alphabet.ptr, raw.data(), sigma * esize);
     }
     if (!wt->pow2) {
       wt->d_codes = DevBuf(std::max<size_t>(wt->codes.size(), 1) * 8);
-      h2d(wt->d_codes.ptr, wt->codes.data(), wt->codes.size());
+      up.add(wt->d_codes.ptr, wt->codes.data(), wt->codes.size());
       wt->d_code_lens = DevBuf(std::max<size_t>(wt->code_lens.size(), 1));
-      h2d(wt->d_code_lens.ptr, wt->code_lens.data(), wt->code_lens.size());
+      up.add(wt->d_code_lens.ptr, wt->code_lens.data(), wt->code_lens.size());
     }
+    up.flush();
 
     // ---- symbol counts: a multi-pass tree over a non-power-of-2 alphabet
     // needs its level sizes (hence the counts) before the first pass;
@@ -683,7 +797,9 @@ static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_
       d2h(sym_count.data(), d_hist.ptr, sigma);
     }
 
-    // ---- code and level-order buffers; scan scratch for the largest pass
+    // ---- code and level-order buffers; scratch: tile histograms and their
+    // scan for the largest pass, error flag, first pass digit starts and
+    // thresholds, directory tables
     DevBuf buf_x, buf_u, buf_v;
     uint64_t max_tiles = (n + T - 1) / T;
     for (uint32_t q = 1; q < plan.npasses; ++q)
@@ -693,9 +809,21 @@ static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_
     if (!identity) buf_x = DevBuf((plan.npasses > 2 ? pad_cap : n) * cw);
     if (plan.npasses > 1) buf_u = DevBuf(pad_cap * cw);
     if (plan.npasses > 2 && identity) buf_v = DevBuf(pad_cap * cw);
-    DevBuf d_hist_t(max_tiles * (1ull << kmax) * 4);
-    DevBuf d_excl((max_tiles + 1) * (1ull << kmax) * 8);
-    DevBuf d_partial(scan_partial_entries(max_tiles, kmax) * 8);
+    const uint32_t k0 = plan.k[0], nb0 = 1u << k0;
+    const uint64_t tiles0 = (n + T - 1) / T;
+    Scratch scr(device, s,
+                Scratch::units((max_tiles << kmax) * 4) +
+                    Scratch::units(((max_tiles + 1) << kmax) * 8) +
+                    Scratch::units(scan_partial_entries(max_tiles, kmax) * 8) +
+                    Scratch::units(4) + Scratch::units((nb0 + 1) * 8) +
+                    Scratch::units((k0 + 1) * 2) +
+                    directory_scratch_bytes(std::vector<uint64_t>(L, n)));
+    uint32_t* d_hist_t = scr.carve<uint32_t>(max_tiles << kmax);
+    uint64_t* d_excl = scr.carve<uint64_t>((max_tiles + 1) << kmax);
+    uint64_t* d_partial = scr.carve<uint64_t>(scan_partial_entries(max_tiles, kmax));
+    uint32_t* d_err = scr.carve<uint32_t>(1);
+    uint64_t* d_pc0 = scr.carve<uint64_t>(nb0 + 1);
+    uint16_t* d_thr0 = scr.carve<uint16_t>(k0 + 1);
 
     PassParams pp{};
     pp.map_kind = wt->map_kind;
@@ -708,16 +836,13 @@ static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_
     pp.cw = cw;
     pp.tile_rows = rows;
     pp.tile_codes = static_cast<uint32_t>(T);
-    DevBuf d_err(4);
-    launch_fill_u32(d_err.as<uint32_t>(), 1, 0, s);
-    pp.err = d_err.as<uint32_t>();
-    pp.hist = d_hist_t.as<uint32_t>();
-    pp.excl = d_excl.as<uint64_t>();
-    pp.partial = d_partial.as<uint64_t>();
+    launch_fill_u32(d_err, 1, 0, s);
+    pp.err = d_err;
+    pp.hist = d_hist_t;
+    pp.excl = d_excl;
+    pp.partial = d_partial;
 
     // ---- pass 0 digit counts (the whole text is one node)
-    const uint32_t k0 = plan.k[0], nb0 = 1u << k0;
-    const uint64_t tiles0 = (n + T - 1) / T;
     pp.src = d_text;
     pp.src_esize = esize;
     pp.codes_out = identity ? nullptr : buf_x.ptr;
@@ -728,13 +853,22 @@ static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_
     pp.num_tiles = tiles0;
     launch_tile_hist(pp, s);
     launch_tile_scan(pp, s);
+    // A single pass over a power-of-2 alphabet needs no counts on the host
+    // before the end: every level holds n bits, the digit starts are
+    // scanned on the device and the thresholds do not depend on counts.
+    const bool single = plan.npasses == 1;
+    const bool sizes_known = single && wt->pow2;
     // bin counts of the current pass: [node][digit] codes of the pass input
     std::vector<uint64_t> binc(nb0);
-    ECL_HIP(hipStreamSynchronize(s));
-    d2h(binc.data(), d_excl.as<uint64_t>() + tiles0 * nb0, nb0);
-    if (plan.npasses == 1) {  // digits are whole codes
-      sym_count.resize(sigma);
-      for (uint64_t c = 0; c < sigma; ++c) sym_count[c] = binc[code[c]];
+    if (sizes_known) {
+      launch_digit_starts(d_excl + tiles0 * nb0, nb0, d_pc0, s);
+    } else {
+      ECL_HIP(hipStreamSynchronize(s));
+      d2h(binc.data(), d_excl + tiles0 * nb0, nb0);
+      if (single) {  // digits are whole codes
+        sym_count.resize(sigma);
+        for (uint64_t c = 0; c < sigma; ++c) sym_count[c] = binc[code[c]];
+      }
     }
 
     // ---- level sizes and the bit arrays (zeroed: run edges are OR-ed in)
@@ -745,7 +879,7 @@ static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_
       if (tot != n) throw Error(ECL_ERROR_INTERNAL, "histogram does not sum to n");
       level_size = level_sizes(shape, sym_count);
     }
-    ecl_bit_array_t* ba = make_bit_array(level_size, 0, device);
+    ecl_bit_array_t* ba = make_bit_array(level_size, 0, device, &up);
     std::unique_ptr<ecl_bit_array_t> ba_guard(ba);
     ecl_rank_select_t* rs = alloc_rank_select(ba, s);
     std::unique_ptr<ecl_rank_select_t> rs_guard(rs);
@@ -798,7 +932,7 @@ static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_
         launch_tile_scan(pp, s);
         // digit counts per node: differences of the scan rows at node starts
         d_rows = DevBuf((nodes + 1) * nb * 8);
-        launch_gather_rows(d_excl.as<uint64_t>(), d_tfirst.as<uint32_t>(), nodes + 1, nb,
+        launch_gather_rows(d_excl, d_tfirst.as<uint32_t>(), nodes + 1, nb,
                            d_rows.as<uint64_t>(), s);
         std::vector<uint64_t> rows((nodes + 1) * nb);
         ECL_HIP(hipStreamSynchronize(s));
@@ -826,9 +960,19 @@ static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_
                                                        << (k - i));
         }
       }
+      if (single) {
+        up.add(d_thr0, thr.data(), thr.size());
+        pp.thr = d_thr0;
+      } else {
+        d_thr = DevBuf(thr.size() * 2);
+        h2d(d_thr.ptr, thr.data(), thr.size());
+        pp.thr = d_thr.as<uint16_t>();
+      }
       // global start of (node, digit): codes of earlier nodes + earlier digits
-      std::vector<uint64_t> pc(nodes * (nb + 1));
-      {
+      if (sizes_known) {
+        pp.pc = d_pc0;  // scanned on the device
+      } else {
+        std::vector<uint64_t> pc(nodes * (nb + 1));
         uint64_t ns = 0;
         for (uint64_t P = 0; P < nodes; ++P) {
           uint64_t run = ns;
@@ -839,13 +983,15 @@ static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_
           pc[P * (nb + 1) + nb] = run;
           ns += node_size[P];
         }
+        if (single) {
+          up.add(d_pc0, pc.data(), pc.size());
+          pp.pc = d_pc0;
+        } else {
+          d_pc = DevBuf(pc.size() * 8);
+          h2d(d_pc.ptr, pc.data(), pc.size());
+          pp.pc = d_pc.as<uint64_t>();
+        }
       }
-      d_pc = DevBuf(pc.size() * 8);
-      h2d(d_pc.ptr, pc.data(), pc.size());
-      d_thr = DevBuf(thr.size() * 2);
-      h2d(d_thr.ptr, thr.data(), thr.size());
-      pp.pc = d_pc.as<uint64_t>();
-      pp.thr = d_thr.as<uint16_t>();
       // next pass: node sizes and padded starts of level l0+k
       std::vector<uint64_t> next_size, next_pstart;
       void* out_buf = nullptr;
@@ -874,6 +1020,7 @@ static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_
         pp.level_word[i] = ba->offsets[l0 + i];
         pp.level_nwords[i] = (level_size[l0 + i] + 31) / 32;
       }
+      up.flush();
       launch_pass(pp, s);
       if (!last) {
         in_codes = out_buf;
@@ -881,22 +1028,27 @@ static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_
         node_pstart.swap(next_pstart);
       }
     }
+
+    // ---- rank/select directory of every level, queued behind the passes;
+    // then the one wait of the build
+    build_directories(rs, ba, s, scr, up);
+    ECL_HIP(hipStreamSynchronize(s));
+    {
+      uint32_t err = 0;
+      d2h(&err, d_err, 1);
+      if (err) throw Error(ECL_ERROR_INTERNAL, "construction pass wrote outside its arrays");
+    }
+    if (sizes_known) d2h(binc.data(), d_excl + tiles0 * nb0, nb0);
     if (sym_count.empty()) {  // power-of-2 alphabet: the last pass's digits end the codes
       const uint32_t kl = plan.k.back(), nbl = 1u << kl;
       sym_count.resize(sigma);
-      for (uint64_t c = 0; c < sigma; ++c)
+      uint64_t tot = 0;
+      for (uint64_t c = 0; c < sigma; ++c) {
         sym_count[c] = binc[(code[c] >> kl) * nbl + (code[c] & (nbl - 1))];
+        tot += sym_count[c];
+      }
+      if (tot != n) throw Error(ECL_ERROR_INTERNAL, "histogram does not sum to n");
     }
-
-    {
-      uint32_t err = 0;
-      ECL_HIP(hipStreamSynchronize(s));
-      d2h(&err, d_err.ptr, 1);
-      if (err) throw Error(ECL_ERROR_INTERNAL, "construction pass wrote outside its arrays");
-    }
-
-    // ---- rank/select directory of every level
-    build_directories(rs, ba, s);
     own_text.reset();
     buf_x.reset();
     buf_u.reset();
@@ -913,17 +1065,20 @@ static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_
       wt->nodes_until_last_level = meta.nodes_until_last_level;
     }
 
-    finalize_rank_select(rs, ba, s);
+    finalize_rank_select(rs, ba, up);
     ba_guard.release();
     wt->rs = rs_guard.release();
 
-    // ---- device metadata and view
+    // ---- device metadata and view (the tables are on the device when the
+    // build returns)
     wt->d_counts = DevBuf(sigma * 8);
-    h2d(wt->d_counts.ptr, wt->counts.data(), sigma);
+    up.add(wt->d_counts.ptr, wt->counts.data(), sigma);
     wt->d_ranks = DevBuf(std::max<size_t>(wt->ranks.size(), 1) * 8);
-    h2d(wt->d_ranks.ptr, wt->ranks.data(), wt->ranks.size());
+    up.add(wt->d_ranks.ptr, wt->ranks.data(), wt->ranks.size());
     wt->d_nal = DevBuf(std::max<size_t>(wt->nodes_at_level.size(), 1) * 8);
-    h2d(wt->d_nal.ptr, wt->nodes_at_level.data(), wt->nodes_at_level.size());
+    up.add(wt->d_nal.ptr, wt->nodes_at_level.data(), wt->nodes_at_level.size());
+    up.flush();
+    ECL_HIP(hipStreamSynchronize(s));
     TreeView& v = wt->view;
     v.levels = wt->rs->d_levels.as<LevelView>();
     v.counts = wt->d_counts.as<uint64_t>();
@@ -1018,6 +1173,12 @@ ecl_status ecl_device_synchronize(int device) {
   return guard([&] {
     set_device(device);
     ECL_HIP(hipDeviceSynchronize());
+  });
+}
+ecl_status ecl_release_cached_memory(int device) {
+  return guard([&] {
+    set_device(device);
+    release_scratch_cache(device);
   });
 }
 ecl_status ecl_device_mem_info(int device, size_t* fr, size_t* tot) {

@@ -318,7 +318,7 @@ ecl_status ecl_forest_create(const void* d_texts, const uint64_t* starts, const 
     // ---- directories of every level of every tree, one launch
     {
       std::vector<RsArrayDesc> desc;
-      std::vector<uint32_t> tile_array;
+      uint64_t blocks = 0;
       for (auto& t : f->trees) {
         for (uint32_t l = 0; l < t.L; ++l) {
           if (t.level_size[l] == 0) continue;
@@ -332,17 +332,15 @@ ecl_status ecl_forest_create(const void* d_texts, const uint64_t* starts, const 
           d.samples = reinterpret_cast<uint64_t*>(A + t.smp_at[l]);
           d.cap = t.cap[l];
           d.total_ones = reinterpret_cast<uint64_t*>(A + t.ones_at) + l;
-          d.first_tile = tile_array.size();
-          tile_array.insert(tile_array.end(), d.num_l1, static_cast<uint32_t>(desc.size()));
+          d.first_tile = blocks;
+          blocks += d.num_l1;
           desc.push_back(d);
         }
       }
-      const uint64_t blocks = tile_array.size();
-      DevBuf d_desc(desc.size() * sizeof(RsArrayDesc)), d_ta(blocks * 4), d_ones(blocks * 4);
+      DevBuf d_desc(std::max<size_t>(desc.size(), 1) * sizeof(RsArrayDesc));
+      DevBuf d_blk(std::max<uint64_t>(blocks, 1) * 8);
       h2d(d_desc.ptr, desc.data(), desc.size());
-      h2d(d_ta.ptr, tile_array.data(), blocks);
-      launch_rs_build(d_desc.as<RsArrayDesc>(), desc.size(), d_ta.as<uint32_t>(), blocks,
-                      d_ones.as<uint32_t>(), s);
+      launch_rs_build(d_desc.as<RsArrayDesc>(), desc.size(), blocks, d_blk.as<uint32_t>(), s);
       ECL_HIP(hipStreamSynchronize(s));
     }
 

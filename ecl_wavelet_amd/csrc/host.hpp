@@ -50,6 +50,60 @@ void d2h(T* dst, const void* src, size_t count) {
   if (count) ECL_HIP(hipMemcpy(dst, src, count * sizeof(T), hipMemcpyDeviceToHost));
 }
 
+// Collects small host->device copies and issues them on a stream as
+// kernel-argument payloads (launch_upload): the caller's host data may go
+// away after add(), and nothing waits for earlier work of the stream.
+// Copies above 64 KiB go through hipMemcpy after the pending ones.
+class Uploader {
+ public:
+  explicit Uploader(hipStream_t s) : s_(s) { clear(); }
+  template <typename T>
+  void add(void* dst, const T* src, size_t count) {
+    add_bytes(dst, src, count * sizeof(T));
+  }
+  void add_bytes(void* dst, const void* src, size_t bytes);
+  void flush();  // launches what is pending
+
+ private:
+  void clear();
+  hipStream_t s_;
+  UploadBlob b_;
+};
+
+// Device scratch of a build, taken from a per-device cache that keeps the
+// buffer between builds (unmapping a large buffer costs about as much as a
+// construction kernel); a build that finds the cache in use by another
+// thread allocates its own.  Sub-buffers are carved in 256-byte units.  The
+// destructor waits for the stream before giving the buffer back.
+class Scratch {
+ public:
+  Scratch(int device, hipStream_t s, size_t bytes);
+  ~Scratch();
+  Scratch(const Scratch&) = delete;
+  Scratch& operator=(const Scratch&) = delete;
+  template <typename T>
+  T* carve(size_t count) {
+    const size_t b = (count * sizeof(T) + 255) & ~size_t(255);
+    if (used_ + b > cap_) throw Error(ECL_ERROR_INTERNAL, "scratch layout exceeded");
+    T* p = reinterpret_cast<T*>(base_ + used_);
+    used_ += b;
+    return p;
+  }
+  static size_t units(size_t bytes) { return (bytes + 255) & ~size_t(255); }
+
+ private:
+  int device_;
+  hipStream_t s_;
+  bool cached_ = false;
+  DevBuf own_;
+  uint8_t* base_ = nullptr;
+  size_t used_ = 0, cap_ = 0;
+};
+// bytes the device's scratch cache holds (free for a build to reuse)
+size_t cached_scratch_bytes(int device);
+// frees the device's scratch cache (waits for a build that uses it)
+void release_scratch_cache(int device);
+
 // ---- tree shape: minimal codes and the internal nodes of every level
 struct Node { uint64_t lo, hi; };
 struct TreeShape {

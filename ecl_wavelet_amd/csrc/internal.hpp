@@ -56,6 +56,19 @@ struct ProfScope {
   ~ProfScope();
 };
 
+// ---- small host->device copies carried in kernel arguments: the launch
+// copies the bytes, so the host neither stages them nor waits for earlier
+// work of the stream (kernels_build.hip) ---------------------------------------
+constexpr uint32_t kUploadBytes = 2048;
+constexpr uint32_t kUploadSegs = 12;
+struct UploadBlob {
+  uint64_t dst[kUploadSegs];       // destination of segment i
+  uint32_t off[kUploadSegs + 1];   // segment i = data[off[i] .. off[i+1])
+  uint32_t nseg;
+  uint8_t data[kUploadBytes];
+};
+void launch_upload(const UploadBlob& b, hipStream_t s);
+
 // ---- launchers (kernels_build.hip) ------------------------------------------
 void launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t s);
 void launch_fill_u32(uint32_t* p, uint64_t n, uint32_t v, hipStream_t s);
@@ -120,6 +133,9 @@ void launch_tile_scan(const PassParams& p, hipStream_t s);
 void launch_pass(const PassParams& p, hipStream_t s);
 void launch_gather_rows(const uint64_t* excl, const uint32_t* rows, uint64_t nrows,
                         uint32_t nb, uint64_t* out, hipStream_t s);
+// pc[d] = sum of totals[0 .. d) for d <= nb (the start of every digit's codes
+// in a single-node pass), computed on the device from the scan's totals row
+void launch_digit_starts(const uint64_t* totals, uint32_t nb, uint64_t* pc, hipStream_t s);
 
 // One bit array of a directory build: its words, its directory arrays and
 // the global index of its first L1 block.
@@ -130,10 +146,10 @@ struct RsArrayDesc {
   uint64_t* total_ones;
   uint64_t first_tile;  // first global L1 block of the array
 };
-// arrays: narrays entries; block_array: array of each of the nblocks global
-// L1 blocks; block_ones: nblocks u32 scratch
-void launch_rs_build(const RsArrayDesc* arrays, uint64_t narrays, const uint32_t* block_array,
-                     uint64_t nblocks, uint32_t* block_ones, hipStream_t s);
+// arrays: narrays entries (device) covering nblocks global L1 blocks in
+// order; scratch: 2 * nblocks u32 (the array of every block, block totals)
+void launch_rs_build(const RsArrayDesc* arrays, uint64_t narrays, uint64_t nblocks,
+                     uint32_t* scratch, hipStream_t s);
 
 void launch_gen_text(void* out, int esize, uint64_t n, uint64_t seed,
                      uint64_t sigma, const uint64_t* thresholds,

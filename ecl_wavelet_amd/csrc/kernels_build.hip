@@ -119,9 +119,16 @@ __global__ __launch_bounds__(256) void histogram_kernel(HistParams p,
 //  3. rs_samples_kernel: one wave per L1 block, one lane per select sample
 //     that falls inside the block: binary search over the block's L2 entries,
 //     then one 64-byte L2 block is read and the bit is selected in-word.
-// Global L1 block g belongs to array block_array[g], whose first block is
-// arrays[...].first_tile.
+// Global L1 block g belongs to array block_array[g] (written by
+// rs_block_map_kernel), whose first block is arrays[...].first_tile.
 constexpr uint32_t kRsWaves = 8;  // L1 blocks per workgroup
+
+// one workgroup per array: its blocks' entries of block_array
+__global__ __launch_bounds__(256) void rs_block_map_kernel(const RsArrayDesc* __restrict__ arrays,
+                                                           uint32_t* __restrict__ block_array) {
+  const uint64_t first = arrays[blockIdx.x].first_tile, cnt = arrays[blockIdx.x].num_l1;
+  for (uint64_t b = threadIdx.x; b < cnt; b += 256) block_array[first + b] = blockIdx.x;
+}
 
 __global__ __launch_bounds__(512) void rs_local_kernel(const RsArrayDesc* __restrict__ arrays,
                                                        const uint32_t* __restrict__ block_array,
@@ -330,6 +337,15 @@ __global__ void presence_kernel(const T* text, uint64_t n, uint32_t* flags) {
     flags[static_cast<uint64_t>(text[i])] = 1u;
 }
 
+// segment i of the blob to its destination, one workgroup for all segments
+__global__ __launch_bounds__(256) void upload_kernel(UploadBlob b) {
+  for (uint32_t i = 0; i < b.nseg; ++i) {
+    uint8_t* dst = reinterpret_cast<uint8_t*>(b.dst[i]);
+    for (uint32_t x = b.off[i] + threadIdx.x; x < b.off[i + 1]; x += 256)
+      dst[x - b.off[i]] = b.data[x];
+  }
+}
+
 uint32_t grid_for(uint64_t work, uint32_t block, uint32_t max_blocks) {
   uint64_t g = (work + block - 1) / block;
   if (g < 1) g = 1;
@@ -338,6 +354,12 @@ uint32_t grid_for(uint64_t work, uint32_t block, uint32_t max_blocks) {
 }
 
 }  // namespace
+
+void launch_upload(const UploadBlob& b, hipStream_t s) {
+  if (b.nseg == 0) return;
+  hipLaunchKernelGGL(upload_kernel, dim3(1), dim3(256), 0, s, b);
+  ECL_HIP(hipGetLastError());
+}
 
 void launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t s) {
   if (n == 0) return;
@@ -395,13 +417,18 @@ void launch_histogram(const HistParams& p, hipStream_t s) {
   ECL_HIP(hipGetLastError());
 }
 
-void launch_rs_build(const RsArrayDesc* arrays, uint64_t narrays, const uint32_t* block_array,
-                     uint64_t nblocks, uint32_t* block_ones, hipStream_t s) {
+void launch_rs_build(const RsArrayDesc* arrays, uint64_t narrays, uint64_t nblocks,
+                     uint32_t* scratch, hipStream_t s) {
   if (nblocks == 0) return;
   const uint64_t groups = (nblocks + kRsWaves - 1) / kRsWaves;
   if (groups > 0x7FFFFFFFull || narrays > 0x7FFFFFFFull)
     throw Error(ECL_ERROR_INVALID_ARGUMENT, "bit arrays too large");
+  uint32_t* block_array = scratch;
+  uint32_t* block_ones = scratch + nblocks;
   ProfScope ps(K_RS_BUILD, s);
+  hipLaunchKernelGGL(rs_block_map_kernel, dim3(static_cast<uint32_t>(narrays)), dim3(256), 0, s,
+                     arrays, block_array);
+  ECL_HIP(hipGetLastError());
   hipLaunchKernelGGL(rs_local_kernel, dim3(static_cast<uint32_t>(groups)), dim3(512), 0, s,
                      arrays, block_array, nblocks, block_ones);
   ECL_HIP(hipGetLastError());

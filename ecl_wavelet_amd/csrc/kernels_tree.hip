@@ -372,6 +372,23 @@ __device__ __forceinline__ void wave_excl_scan4(V (&v)[4], V& total) {
   }
 }
 
+// one wave: pc[d] = totals[0] + .. + totals[d-1] for d <= nb
+__global__ __launch_bounds__(64) void digit_starts_kernel(const uint64_t* __restrict__ totals,
+                                                          uint32_t nb, uint64_t* __restrict__ pc) {
+  const uint32_t lane = threadIdx.x;
+  uint64_t v[4], tot;
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    const uint32_t d = 4 * lane + j;
+    v[j] = d < nb ? totals[d] : 0ull;
+  }
+  wave_excl_scan4<uint64_t>(v, tot);
+#pragma unroll
+  for (int j = 0; j < 4; ++j)
+    if (4 * lane + j < nb) pc[4 * lane + j] = v[j];
+  if (lane == 0) pc[nb] = tot;
+}
+
 // ---------------------------------------------------------------------------
 // One pass over k levels of one tile (see the file comment).  Thread tid
 // owns chunks c = r * 512 + tid (r < 4) of 16 bytes; chunk c holds the codes
@@ -748,6 +765,12 @@ void launch_gather_rows(const uint64_t* excl, const uint32_t* rows, uint64_t nro
   if (nrows == 0) return;
   hipLaunchKernelGGL(gather_rows_kernel, dim3(static_cast<uint32_t>(nrows)), dim3(256), 0, s,
                      excl, rows, nb, out);
+  ECL_HIP(hipGetLastError());
+}
+
+void launch_digit_starts(const uint64_t* totals, uint32_t nb, uint64_t* pc, hipStream_t s) {
+  if (nb > kMaxBins) throw Error(ECL_ERROR_INTERNAL, "too many digits");
+  hipLaunchKernelGGL(digit_starts_kernel, dim3(1), dim3(64), 0, s, totals, nb, pc);
   ECL_HIP(hipGetLastError());
 }
 

@@ -110,6 +110,11 @@ ecl_status ecl_memcpy_h2d(int device, void* dst, const void* src, size_t bytes);
 ecl_status ecl_memcpy_d2h(int device, void* dst, const void* src, size_t bytes);
 ecl_status ecl_device_synchronize(int device);
 ecl_status ecl_device_mem_info(int device, size_t* free_bytes, size_t* total_bytes);
+/* Construction keeps its device scratch (tile histograms and their scan,
+ * about 0.1 bytes per symbol) cached per device for the next build; this
+ * frees it.  No counterpart in the reference, which allocates temporaries
+ * per build; for callers that need the memory back between builds. */
+ecl_status ecl_release_cached_memory(int device);
 ecl_status ecl_device_name(int device, char* buf, size_t len);
 /* events on `stream` (NULL = default stream); elapsed milliseconds */
 ecl_status ecl_event_create(void** event);

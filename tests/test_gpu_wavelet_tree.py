@@ -296,3 +296,36 @@ def test_oom_message(gpu):
         WaveletTree(d, np.arange(256, dtype=np.uint8), dtype=np.uint8, n=1 << 42)
     assert e.value.status == ECL_ERROR_OUT_OF_MEMORY_TREE
     assert str(e.value) == "Not enough memory available for the wavelet tree."
+
+
+def test_scratch_cache_reuse_release_and_threads(gpu):
+    """Builds reuse the per-device construction scratch (growing, shrinking,
+    released in between); two threads building at once (one of them on a
+    private scratch) both give exact trees."""
+    import threading
+    from ecl_wavelet_amd import WaveletTree, release_cached_memory
+    rng = np.random.default_rng(77)
+    cases = [(3 * 10**6, 256), (10**5, 4), (2 * 10**6, 20), (5 * 10**6, 64)]
+    for j, (n, sigma) in enumerate(cases):
+        text = rng.integers(0, sigma, n).astype(np.uint8)
+        wt = WaveletTree(text, np.arange(sigma, dtype=np.uint8))
+        _check_queries(wt, text, None, nq=1000, seed=j)
+        if j == 1:
+            release_cached_memory()
+    texts = [rng.integers(0, s, n).astype(np.uint8) for n, s in cases]
+    out, errs = {}, []
+
+    def work(i):
+        try:
+            n, sigma = cases[i]
+            out[i] = WaveletTree(texts[i], np.arange(sigma, dtype=np.uint8))
+        except Exception as e:  # noqa: BLE001
+            errs.append(e)
+    ths = [threading.Thread(target=work, args=(i,)) for i in range(len(cases))]
+    for t in ths:
+        t.start()
+    for t in ths:
+        t.join()
+    assert not errs
+    for i, t in enumerate(texts):
+        _check_queries(out[i], t, None, nq=1000, seed=10 + i)
