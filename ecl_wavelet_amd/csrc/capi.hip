@@ -851,13 +851,24 @@ static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_
     pp.dshift = L - k0;
     pp.n = n;
     pp.num_tiles = tiles0;
-    launch_tile_hist(pp, s);
-    launch_tile_scan(pp, s);
     // A single pass over a power-of-2 alphabet needs no counts on the host
     // before the end: every level holds n bits, the digit starts are
     // scanned on the device and the thresholds do not depend on counts.
+    // Its bit arrays exist before the first launch, and the histogram
+    // kernel zeroes them (its reads leave HBM write bandwidth free).
     const bool single = plan.npasses == 1;
     const bool sizes_known = single && wt->pow2;
+    std::vector<uint64_t> level_size(L, n);
+    std::unique_ptr<ecl_bit_array_t> ba_guard;
+    if (sizes_known) {
+      ba_guard.reset(make_bit_array(level_size, -1, device, &up));
+      pp.fill_words = ba_guard->words();
+      pp.fill_count = ba_guard->total_words;
+      pp.fill_per = ((ba_guard->total_words + tiles0 - 1) / tiles0 + 3) & ~3ull;
+    }
+    launch_tile_hist(pp, s);
+    pp.fill_words = nullptr;
+    launch_tile_scan(pp, s);
     // bin counts of the current pass: [node][digit] codes of the pass input
     std::vector<uint64_t> binc(nb0);
     if (sizes_known) {
@@ -872,15 +883,14 @@ static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_
     }
 
     // ---- level sizes and the bit arrays (zeroed: run edges are OR-ed in)
-    std::vector<uint64_t> level_size(L, n);
     if (!sym_count.empty()) {
       uint64_t tot = 0;
       for (uint64_t c = 0; c < sigma; ++c) tot += sym_count[c];
       if (tot != n) throw Error(ECL_ERROR_INTERNAL, "histogram does not sum to n");
       level_size = level_sizes(shape, sym_count);
     }
-    ecl_bit_array_t* ba = make_bit_array(level_size, 0, device, &up);
-    std::unique_ptr<ecl_bit_array_t> ba_guard(ba);
+    if (!ba_guard) ba_guard.reset(make_bit_array(level_size, 0, device, &up));
+    ecl_bit_array_t* ba = ba_guard.get();
     ecl_rank_select_t* rs = alloc_rank_select(ba, s);
     std::unique_ptr<ecl_rank_select_t> rs_guard(rs);
 

@@ -125,6 +125,9 @@ struct PassParams {
   const uint8_t* node_remap;  // [nodes] 1: codes come from codes_out
   uint32_t row_bins;          // entries per hist/excl row (0: 2^k)
   uint32_t thr_stride;        // entries per node in thr (0: k + 1)
+  // tile_hist also zeroes fill_words[0 .. fill_count), fill_per words per
+  // tile (a multiple of 4), when fill_words != nullptr
+  uint32_t* fill_words; uint64_t fill_count, fill_per;
 };
 uint64_t pass_tile_codes(int cw, uint32_t rows);    // codes per tile
 uint64_t scan_partial_entries(uint64_t tiles, uint32_t k);

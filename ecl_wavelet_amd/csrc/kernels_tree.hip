@@ -165,23 +165,39 @@ __device__ __forceinline__ uint64_t map_code(const PassParams& p, uint64_t x) {
 }
 
 // ---------------------------------------------------------------------------
-// Tile digit histograms.  Each wave owns 256 LDS bins split into
-// 256 / 2^k replicas of the 2^k digits (lane l uses replica l % replicas),
-// so small digit ranges do not serialise on a few addresses.
+// Tile digit histograms.  The workgroup's 8192 LDS counters hold
+// C = 8192 / 2^k copies of the 2^k bins, bin d of copy c at d * C + c;
+// thread tid counts into copy tid % C, so the lanes of one atomic add hit
+// distinct banks whatever their digits (C >= 32).  The copies are summed per
+// bin at the end.
+constexpr uint32_t kHistSlots = 8192;
+
 template <typename TIn, int CW, int MapKind, bool Remap>
 __global__ __launch_bounds__(kPassThreads) void tile_hist_kernel(PassParams p) {
   using Code = typename CodeOf<CW>::type;
   const uint32_t T = p.tile_codes;                   // codes per tile
   constexpr uint32_t EI = 16 / sizeof(TIn);          // input symbols per chunk
-  __shared__ uint32_t s_bins[(kPassThreads / 64) * kMaxBins];
-  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const uint32_t nb = 1u << p.k, reps = kMaxBins >> p.k;
+  __shared__ uint32_t s_bins[kHistSlots];
+  __shared__ uint32_t s_tot[kMaxBins];
+  const uint32_t tid = threadIdx.x;
+  const uint32_t nb = 1u << p.k, C = kHistSlots >> p.k;
   const uint32_t dmask = nb - 1;
   const uint32_t row = p.row_bins ? p.row_bins : nb;
-  for (uint32_t i = tid; i < (kPassThreads / 64) * kMaxBins; i += kPassThreads) s_bins[i] = 0;
-  __syncthreads();
-  uint32_t* bins = s_bins + wave * kMaxBins + (lane % reps) * nb;
+  for (uint32_t i = tid; i < kHistSlots / 4; i += kPassThreads)
+    reinterpret_cast<uint4*>(s_bins)[i] = make_uint4(0, 0, 0, 0);
+  if (tid < kMaxBins) s_tot[tid] = 0;
   const uint64_t t = blockIdx.x;
+  // zero this tile's share of the bit arrays (the construction ORs run edges in)
+  if (p.fill_words != nullptr) {
+    const uint64_t w0 = t * p.fill_per;
+    const uint64_t w1 = w0 + p.fill_per < p.fill_count ? w0 + p.fill_per : p.fill_count;
+    for (uint64_t w = w0 + 4ull * tid; w < w1; w += 4ull * kPassThreads) {
+      if (w + 4 <= w1) *reinterpret_cast<uint4*>(p.fill_words + w) = make_uint4(0, 0, 0, 0);
+      else for (uint64_t x = w; x < w1; ++x) p.fill_words[x] = 0u;
+    }
+  }
+  __syncthreads();
+  uint32_t* bins = s_bins + tid % C;
   const TileGeo g = tile_geo(p, t, T);
   const TIn* in = static_cast<const TIn*>(p.src);
   Code* out = static_cast<Code*>(p.codes_out);
@@ -209,7 +225,7 @@ __global__ __launch_bounds__(kPassThreads) void tile_hist_kernel(PassParams p) {
         if (k < valid) {
           const uint64_t x = static_cast<uint64_t>(chunk_elem<TIn>(q, k));
           const uint64_t code = lut != nullptr ? lut[x & 255] : map_code<TIn, MapKind>(p, x);
-          atomicAdd(&bins[(code >> p.dshift) & dmask], 1u);
+          atomicAdd(&bins[((code >> p.dshift) & dmask) * C], 1u);
           if constexpr (CW == 8) {
             o[2 * k] = static_cast<uint32_t>(code);
             o[2 * k + 1] = static_cast<uint32_t>(code >> 32);
@@ -244,20 +260,22 @@ __global__ __launch_bounds__(kPassThreads) void tile_hist_kernel(PassParams p) {
 #pragma unroll
       for (uint32_t k = 0; k < EI; ++k)
         if (k < valid)
-          atomicAdd(&bins[(static_cast<uint64_t>(chunk_elem<TIn>(q, k)) >> p.dshift) & dmask], 1u);
+          atomicAdd(&bins[((static_cast<uint64_t>(chunk_elem<TIn>(q, k)) >> p.dshift) & dmask) * C],
+                    1u);
     }
   }
   __syncthreads();
-  for (uint32_t d = tid; d < row; d += kPassThreads) {
-    if (d >= nb) {
-      p.hist[t * row + d] = 0;
-      continue;
-    }
+  {
+    // bin d summed over its copies by kPassThreads / nb threads; the copy
+    // order is rotated by d so the lanes of a wave read distinct banks
+    const uint32_t per = kPassThreads >> p.k;  // >= 2 (nb <= 256)
+    const uint32_t d = tid & dmask, part = tid >> p.k;
     uint32_t sum = 0;
-    for (uint32_t w = 0; w < kPassThreads / 64; ++w)
-      for (uint32_t r = 0; r < reps; ++r) sum += s_bins[w * kMaxBins + r * nb + d];
-    p.hist[t * row + d] = sum;
+    for (uint32_t j = part; j < C; j += per) sum += s_bins[d * C + ((j + d) & (C - 1))];
+    atomicAdd(&s_tot[d], sum);
   }
+  __syncthreads();
+  for (uint32_t d = tid; d < row; d += kPassThreads) p.hist[t * row + d] = d < nb ? s_tot[d] : 0u;
 }
 
 // ---------------------------------------------------------------------------
