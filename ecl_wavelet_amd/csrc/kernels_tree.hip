@@ -441,6 +441,7 @@ void pass_kernel(PassParams p) {
   __shared__ uint64_t s_wave[kPassThreads / 64];   // ones per wave, 16-bit field per row
   __shared__ uint32_t s_gw[kMaxBins / 2 + 1];      // destination words per group (prefix)
   __shared__ uint32_t s_thr[9];
+  __shared__ uint64_t s_lw[8], s_lnw[8];           // word offset and words per level
 
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint64_t t = blockIdx.x;
@@ -478,7 +479,13 @@ void pass_kernel(PassParams p) {
     const uint64_t* pc = p.pc + static_cast<uint64_t>(g.node) * (row + 1);
     for (uint32_t x = lane; x <= nb; x += 64) s_pc[x] = pc[x];
   } else if (wave == 3) {
+    // read here, before the first bit-array store: a global load inside the
+    // level loop would make the wave wait for its stores of earlier levels
     if (lane <= k) s_thr[lane] = p.thr[static_cast<uint64_t>(g.node) * ts + lane];
+    if (lane < k) {
+      s_lw[lane] = p.node_lw != nullptr ? p.node_lw[g.node * 8ull + lane] : p.level_word[lane];
+      s_lnw[lane] = p.node_lnw != nullptr ? p.node_lnw[g.node * 8ull + lane] : p.level_nwords[lane];
+    }
   }
   // ---- stage the tile
   const Code* src = static_cast<const Code*>(
@@ -574,8 +581,8 @@ void pass_kernel(PassParams p) {
     {
       const uint32_t total_words = s_gw[kMaxBins / 2];
       const uint32_t* bits32 = s_bits;
-      const uint64_t lw = p.node_lw != nullptr ? p.node_lw[g.node * 8ull + i] : p.level_word[i];
-      const uint64_t lnw = p.node_lnw != nullptr ? p.node_lnw[g.node * 8ull + i] : p.level_nwords[i];
+      const uint64_t lw = s_lw[i];
+      const uint64_t lnw = s_lnw[i];
       uint32_t* words = p.words + lw;
       for (uint32_t w = tid; w < total_words; w += kPassThreads) {
         uint32_t lo = 0, hi = ngroups;
