@@ -409,8 +409,16 @@ __global__ __launch_bounds__(64) void digit_starts_kernel(const uint64_t* __rest
 
 // ---------------------------------------------------------------------------
 // One pass over k levels of one tile (see the file comment).  Thread tid
-// owns chunks c = r * 512 + tid (r < 4) of 16 bytes; chunk c holds the codes
-// at tile positions [c*E, c*E + E) of the current level order.
+// owns chunks c = r * 512 + tid (r < ROWS) of 16 bytes; chunk c holds the
+// codes at tile positions [c*E, c*E + E) of the current level order.
+//
+// Everything a level needs besides its bits follows from the tile's digit
+// counts and is tabulated for all levels before the first one: per group
+// (the codes of one node of the level) its destination R in the level's bit
+// array, its tile position and length, its destination words (prefix over
+// the groups), and per child the partition base.  A level then costs two
+// barriers: after its bits and ones counts, and after its partition.
+//
 // Waves per SIMD that the tile's LDS allows (three, four, four workgroups
 // of eight waves per CU for 4, 2, 1 chunk rows): the register budget
 // follows.
@@ -418,6 +426,15 @@ template <int ROWS> struct PassOcc;
 template <> struct PassOcc<4> { static constexpr int waves = 6; };
 template <> struct PassOcc<2> { static constexpr int waves = 8; };
 template <> struct PassOcc<1> { static constexpr int waves = 8; };
+
+// Offsets of level i's entries in the per-level tables: groups (2^i per
+// level), group word prefixes (2^i + 1) and children (2^(i+1)).
+__device__ __forceinline__ uint32_t grp_off(uint32_t i) { return (1u << i) - 1u; }
+__device__ __forceinline__ uint32_t gw_off(uint32_t i) { return (1u << i) - 1u + i; }
+__device__ __forceinline__ uint32_t child_off(uint32_t i) { return (2u << i) - 2u; }
+constexpr uint32_t kLevelGroups = 255;   // groups of levels 0 .. 7
+constexpr uint32_t kLevelGw = 263;       // group word prefixes of levels 0 .. 7
+constexpr uint32_t kLevelChildren = 510; // children of levels 0 .. 7
 
 template <int CW, int ROWS>
 __global__ __launch_bounds__(kPassThreads)
@@ -429,19 +446,21 @@ void pass_kernel(PassParams p) {
   constexpr uint32_t E = 16 / CW;                 // codes per chunk
   constexpr uint32_t T = kTileChunks * E;          // codes per tile
   constexpr uint32_t LPW = 32 / E;                 // lanes per 32-bit bit word
+  static_assert(T <= 65535, "tile positions are kept in 16 bits");
   // the tile in the current level's order; rewritten in place by the
   // partition (every thread holds its chunks in registers by then)
   __shared__ uint4 s_buf[kTileChunks];
   __shared__ uint32_t s_bits[kTileChunks * E / 32]; // level bits of the tile
-  __shared__ uint16_t s_opre[kTileChunks];         // ones before each chunk in its wave row
   __shared__ uint32_t s_hc[kMaxBins + 1];          // codes of the tile with digit < x
   __shared__ uint64_t s_ec[kMaxBins + 1];          // node's codes with digit < x in earlier tiles
   __shared__ uint64_t s_pc[kMaxBins + 1];          // global start of (node, digit x)
-  __shared__ uint32_t s_base[kMaxBins];            // partition base per child
   __shared__ uint64_t s_wave[kPassThreads / 64];   // ones per wave, 16-bit field per row
-  __shared__ uint32_t s_gw[kMaxBins / 2 + 1];      // destination words per group (prefix)
   __shared__ uint32_t s_thr[9];
   __shared__ uint64_t s_lw[8], s_lnw[8];           // word offset and words per level
+  __shared__ uint64_t s_gR[kLevelGroups];          // group destination bit
+  __shared__ uint32_t s_gSL[kLevelGroups];         // group tile start | length << 16
+  __shared__ uint32_t s_gw[kLevelGw];              // destination words before each group
+  __shared__ uint16_t s_base[kLevelChildren];      // partition base per child
 
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const uint64_t t = blockIdx.x;
@@ -502,11 +521,60 @@ void pass_kernel(PassParams p) {
   // (digits >= nb never occur, so s_hc[x] and s_ec[x] for x >= nb hold the totals)
   __syncthreads();
 
+  // ---- level tables: wave i for level i.  Group q of level i covers digits
+  // [a, b) = [q, q+1) << (k - i), its children split at m = a + (b - a) / 2;
+  // it occupies tile positions [s_hc[a], s_hc[b]) of the level order and
+  // level bits [R, R + len) with R = s_pc[a] + s_ec[b] - s_ec[a].  Its ones
+  // (codes of child 2q+1) follow those of the earlier groups, so the
+  // partition bases need no bits: child 2q takes the zeros from Oa (ones
+  // before the group), child 2q+1 the ones from s_hc[m] - Oa.
+  for (uint32_t i = wave; i < k; i += kPassThreads / 64) {
+    const uint32_t gshift = k - i;
+    const uint32_t ngroups = s_thr[i] >> gshift;
+    uint32_t nw[2], ones[2];
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const uint32_t q = 2 * lane + j;
+      nw[j] = 0;
+      ones[j] = 0;
+      if (q < ngroups) {
+        const uint32_t a = q << gshift, b = (q + 1) << gshift, m = a + (1u << (gshift - 1));
+        const uint32_t len = s_hc[b] - s_hc[a];
+        const uint64_t R = s_pc[a] + s_ec[b] - s_ec[a];
+        nw[j] = len ? static_cast<uint32_t>(((R & 31) + len + 31) >> 5) : 0u;
+        ones[j] = s_hc[b] - s_hc[m];
+        s_gR[grp_off(i) + q] = R;
+        s_gSL[grp_off(i) + q] = s_hc[a] | (len << 16);
+      }
+    }
+    // word prefixes in the low half, ones prefixes in the high half
+    const uint64_t v = (static_cast<uint64_t>(ones[0] + ones[1]) << 32) | (nw[0] + nw[1]);
+    const uint64_t incl = detail::wave_incl_scan_2x32(v);
+    const uint32_t wex = static_cast<uint32_t>(incl - v);           // words before q0
+    const uint32_t oex = static_cast<uint32_t>((incl - v) >> 32);   // ones before q0
+    uint32_t* gw = s_gw + gw_off(i);
+    if (2 * lane < ngroups) gw[2 * lane] = wex;
+    if (2 * lane + 1 < ngroups) gw[2 * lane + 1] = wex + nw[0];
+    if (lane == 63) gw[ngroups] = static_cast<uint32_t>(incl);
+    uint16_t* base = s_base + child_off(i);
+#pragma unroll
+    for (int j = 0; j < 2; ++j) {
+      const uint32_t q = 2 * lane + j;
+      const uint32_t Oa = oex + (j ? ones[0] : 0u);
+      if (q < ngroups) {
+        const uint32_t m = (q << gshift) + (1u << (gshift - 1));
+        base[2 * q] = static_cast<uint16_t>(Oa);
+        base[2 * q + 1] = static_cast<uint16_t>(s_hc[m] - Oa);
+      }
+    }
+  }
+  __syncthreads();
+
   for (uint32_t i = 0; i < k; ++i) {
     const uint32_t npres = s_hc[s_thr[i]];            // tile codes present at level l0+i
     const uint32_t cbit = p.dshift + k - 1 - i;        // code bit of the level
     const uint32_t gshift = k - i;                     // digit >> gshift = group
-    // ---- level bits of the four chunks; ones per chunk (16-bit field per row)
+    // ---- level bits of the chunks; ones per chunk (16-bit field per row)
     uint4 v[kChunkRows];
     uint64_t packed = 0;
 #pragma unroll
@@ -526,9 +594,6 @@ void pass_kernel(PassParams p) {
     // wave-local exclusive prefixes (fields stay below 2^16: no carries)
     const uint64_t incl = detail::wave_incl_scan_2x32(packed);
     const uint64_t lex = incl - packed;
-#pragma unroll
-    for (uint32_t r = 0; r < kChunkRows; ++r)
-      s_opre[r * kPassThreads + tid] = static_cast<uint16_t>(lex >> (16 * r));
     if (lane == 63) s_wave[wave] = incl;
     __syncthreads();
     // ones before each of my chunks: earlier rows + earlier waves of my row,
@@ -541,69 +606,49 @@ void pass_kernel(PassParams p) {
       all_waves += x;
       if (w < wv) waves_before += x;
     }
-    uint32_t rowtot[kChunkRows];
     uint32_t O[kChunkRows];
-    uint32_t tot_ones = 0;
-    const uint64_t mine = waves_before + lex;
-#pragma unroll
-    for (uint32_t r = 0; r < kChunkRows; ++r) {
-      rowtot[r] = static_cast<uint32_t>(all_waves >> (16 * r)) & 0xFFFFu;
-      O[r] = tot_ones + (static_cast<uint32_t>(mine >> (16 * r)) & 0xFFFFu);
-      tot_ones += rowtot[r];
-    }
-    // ---- write the level's runs.  Group q (tile positions [g0, g1)) goes to
-    // level bits [R, R + g1 - g0); wave 0 counts the destination words of
-    // every group, then each thread takes destination words, finds their
-    // group by binary search over the word offsets and assembles the word
-    // from the tile's bit string (plain store when the word lies inside the
-    // run, atomicOr on the two edge words shared with neighbouring runs).
-    const uint32_t ngroups = s_thr[i] >> gshift;
-    if (wave == 0) {
-      uint32_t w2[2];
-#pragma unroll
-      for (int j = 0; j < 2; ++j) {
-        const uint32_t q = 2 * lane + j;
-        w2[j] = 0;
-        if (q < ngroups) {
-          const uint32_t a = q << gshift, b = (q + 1) << gshift;
-          const uint32_t len = s_hc[b] - s_hc[a];
-          const uint64_t R = s_pc[a] + s_ec[b] - s_ec[a];
-          w2[j] = len ? static_cast<uint32_t>(((R & 31) + len + 31) >> 5) : 0u;
-        }
-      }
-      const uint32_t s2 = w2[0] + w2[1];
-      const uint32_t incl = detail::wave_incl_scan_u32(s2);
-      s_gw[2 * lane] = incl - s2;
-      s_gw[2 * lane + 1] = incl - s2 + w2[0];
-      if (lane == 63) s_gw[kMaxBins / 2] = incl;
-    }
-    __syncthreads();
     {
-      const uint32_t total_words = s_gw[kMaxBins / 2];
-      const uint32_t* bits32 = s_bits;
-      const uint64_t lw = s_lw[i];
+      uint32_t tot_ones = 0;
+      const uint64_t mine = waves_before + lex;
+#pragma unroll
+      for (uint32_t r = 0; r < kChunkRows; ++r) {
+        O[r] = tot_ones + (static_cast<uint32_t>(mine >> (16 * r)) & 0xFFFFu);
+        tot_ones += static_cast<uint32_t>(all_waves >> (16 * r)) & 0xFFFFu;
+      }
+    }
+    // ---- write the level's runs.  Each thread takes destination words,
+    // finds their group by binary search over the word prefixes and
+    // assembles the word from the tile's bit string (plain store when the
+    // word lies inside the run, atomicOr on the two edge words shared with
+    // neighbouring runs).
+    {
+      const uint32_t ngroups = s_thr[i] >> gshift;
+      const uint32_t* gw = s_gw + gw_off(i);
+      const uint64_t* gR = s_gR + grp_off(i);
+      const uint32_t* gSL = s_gSL + grp_off(i);
+      const uint32_t total_words = gw[ngroups];
       const uint64_t lnw = s_lnw[i];
-      uint32_t* words = p.words + lw;
+      uint32_t* words = p.words + s_lw[i];
       for (uint32_t w = tid; w < total_words; w += kPassThreads) {
-        uint32_t lo = 0, hi = ngroups;
+        uint32_t lo = 0, hi = ngroups, wlo = 0;
         while (hi - lo > 1) {
           const uint32_t mid = (lo + hi) >> 1;
-          if (s_gw[mid] <= w) lo = mid; else hi = mid;
+          const uint32_t x = gw[mid];
+          if (x <= w) { lo = mid; wlo = x; } else { hi = mid; }
         }
-        const uint32_t q = lo;
-        const uint32_t a = q << gshift, b = (q + 1) << gshift;
-        const uint32_t src0 = s_hc[a];
-        const uint32_t len = s_hc[b] - src0;
-        const uint64_t R = s_pc[a] + s_ec[b] - s_ec[a];
-        const uint64_t D = (R >> 5) + (w - s_gw[q]);
-        const uint64_t db = D * 32;
-        const uint32_t blo = R > db ? static_cast<uint32_t>(R - db) : 0u;
-        const uint64_t rend = R + len;
-        const uint32_t bhi = rend < db + 32 ? static_cast<uint32_t>(rend - db) : 32u;
-        const uint32_t sp = src0 + static_cast<uint32_t>(db + blo - R);
+        const uint64_t R = gR[lo];
+        const uint32_t sl = gSL[lo];
+        const uint32_t src0 = sl & 0xFFFFu, len = sl >> 16;
+        const uint32_t j = w - wlo;                   // word of the run
+        const uint32_t r5 = static_cast<uint32_t>(R) & 31u;
+        const uint64_t D = (R >> 5) + j;
+        const uint32_t blo = j == 0 ? r5 : 0u;        // first bit of the word in the run
+        const uint32_t end = r5 + len - 32u * j;      // run end relative to the word
+        const uint32_t bhi = end < 32u ? end : 32u;
+        const uint32_t sp = src0 + 32u * j + blo - r5;
         const uint32_t sw = sp >> 5;
-        const uint64_t win = static_cast<uint64_t>(bits32[sw]) |
-                             (sw + 1 < T / 32 ? static_cast<uint64_t>(bits32[sw + 1]) << 32 : 0ull);
+        const uint64_t win = static_cast<uint64_t>(s_bits[sw]) |
+                             (sw + 1 < T / 32 ? static_cast<uint64_t>(s_bits[sw + 1]) << 32 : 0ull);
         uint32_t val = static_cast<uint32_t>(win >> (sp & 31));
         const uint32_t nbits = bhi - blo;
         if (nbits < 32) val &= (1u << nbits) - 1u;
@@ -615,35 +660,14 @@ void pass_kernel(PassParams p) {
     }
     const bool partition = (i + 1 < k) || (p.dst != nullptr);
     if (!partition) break;
-    // ---- partition bases: child c of group c >> 1 takes its zeros (c even)
-    // or ones (c odd) in order, starting at the child's first position; the
-    // entries are indices into both buffers seen as one array of 2T codes
-    {
-      const uint32_t nchild = 2u << i;
-      for (uint32_t c = tid; c < nchild; c += kPassThreads) {
-        const uint32_t q = c >> 1, bit = c & 1;
-        const uint32_t ha = s_hc[q << gshift];
-        const uint32_t ch = ha / E, off = ha % E;
-        uint32_t Oa = tot_ones;
-        if (ch < kTileChunks) {
-          const uint32_t r = ch / kPassThreads, cw = (ch % kPassThreads) >> 6;
-          Oa = s_opre[ch];
-          for (uint32_t rr = 0; rr < r; ++rr) Oa += rowtot[rr];
-          for (uint32_t w = 0; w < cw; ++w)
-            Oa += static_cast<uint32_t>(s_wave[w] >> (16 * r)) & 0xFFFFu;
-          const uint32_t bits = s_bits[(ch * E) >> 5] >> ((ch * E) & 31);  // chunk's bits
-          Oa += __popc(bits & ((1u << off) - 1u));
-        }
-        const uint32_t child_lo = c << (gshift - 1);
-        s_base[c] = s_hc[child_lo] - (bit ? Oa : ha - Oa);
-      }
-    }
-    __syncthreads();
+    // ---- partition: child c of group c >> 1 takes its zeros (c even) or
+    // ones (c odd) in order from its base (level tables)
+    const uint16_t* s_bs = s_base + child_off(i);
     Code* both = reinterpret_cast<Code*>(s_buf);
     const uint32_t cwidth = i + 1;  // child index = top i+1 digit bits
     if (npres == T && i == 0) {
       // one group, two children: the bases stay in registers
-      const uint32_t base0 = s_base[0], base1 = s_base[1];
+      const uint32_t base0 = s_bs[0], base1 = s_bs[1];
 #pragma unroll
       for (uint32_t r = 0; r < kChunkRows; ++r) {
         const uint32_t p0 = (r * kPassThreads + tid) * E;
@@ -665,7 +689,7 @@ void pass_kernel(PassParams p) {
 #pragma unroll
         for (uint32_t j = 0; j < E; ++j) child[j] = child_of<Code>(v[r], j, cbit, cwidth);
 #pragma unroll
-        for (uint32_t j = 0; j < E; ++j) base[j] = s_base[child[j]];
+        for (uint32_t j = 0; j < E; ++j) base[j] = s_bs[child[j]];
         uint32_t o = O[r], z = p0 - O[r];
 #pragma unroll
         for (uint32_t j = 0; j < E; ++j) {
@@ -686,7 +710,7 @@ void pass_kernel(PassParams p) {
           if (p0 + j < npres) {
             const uint32_t child = child_of<Code>(v[r], j, cbit, cwidth);
             const uint32_t b = child & 1u;
-            both[s_base[child] + (b ? o : z)] = chunk_elem<Code>(v[r], j);
+            both[s_bs[child] + (b ? o : z)] = chunk_elem<Code>(v[r], j);
             o += b;
             z += b ^ 1u;
           }
