@@ -635,6 +635,12 @@ def device_name(device=0):
     return buf.value.decode()
 
 
+def set_build_span(tiles_per_workgroup=0):
+    """Construction tiles each workgroup takes in turn (0: automatic); a
+    tuning knob that does not change results."""
+    call("ecl_set_build_span", tiles_per_workgroup)
+
+
 def release_cached_memory(device=0):
     """Frees the construction scratch the library keeps per device between
     builds."""

@@ -1185,6 +1185,12 @@ ecl_status ecl_device_synchronize(int device) {
     ECL_HIP(hipDeviceSynchronize());
   });
 }
+ecl_status ecl_set_build_span(uint32_t tiles_per_workgroup) {
+  return guard([&] {
+    require(tiles_per_workgroup <= 64, "at most 64 tiles per workgroup");
+    set_pass_tiles_per_wg(tiles_per_workgroup);
+  });
+}
 ecl_status ecl_release_cached_memory(int device) {
   return guard([&] {
     set_device(device);

@@ -128,12 +128,15 @@ struct PassParams {
   // tile_hist also zeroes fill_words[0 .. fill_count), fill_per words per
   // tile (a multiple of 4), when fill_words != nullptr
   uint32_t* fill_words; uint64_t fill_count, fill_per;
+  // construction: consecutive tiles per workgroup (0: chosen by launch_pass)
+  uint32_t tiles_per_wg;
 };
 uint64_t pass_tile_codes(int cw, uint32_t rows);    // codes per tile
 uint64_t scan_partial_entries(uint64_t tiles, uint32_t k);
 void launch_tile_hist(const PassParams& p, hipStream_t s);
 void launch_tile_scan(const PassParams& p, hipStream_t s);
 void launch_pass(const PassParams& p, hipStream_t s);
+void set_pass_tiles_per_wg(uint32_t span);  // 0: automatic
 void launch_gather_rows(const uint64_t* excl, const uint32_t* rows, uint64_t nrows,
                         uint32_t nb, uint64_t* out, hipStream_t s);
 // pc[d] = sum of totals[0 .. d) for d <= nb (the start of every digit's codes

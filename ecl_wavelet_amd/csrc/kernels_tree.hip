@@ -27,6 +27,8 @@
 // nodes) and is cut into tiles of its own.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include "internal.hpp"
 
 namespace ecl {
@@ -419,6 +421,15 @@ __global__ __launch_bounds__(64) void digit_starts_kernel(const uint64_t* __rest
 // the groups), and per child the partition base.  A level then costs two
 // barriers: after its bits and ones counts, and after its partition.
 //
+// A workgroup takes p.tiles_per_wg consecutive tiles in turn.  In one node
+// the runs of consecutive tiles are adjacent, so a run's last, partial word
+// is carried in LDS (per level and group) into the next tile, which
+// completes it: such words are written once with a plain store.  Only a
+// word that a workgroup shares with another (at the ends of its span, or
+// where the tiles' nodes change) is OR-ed in atomically.  One owner thread
+// per group (thread 4q) handles the partial first and last word of the
+// group's run; all other threads write its full words.
+//
 // Waves per SIMD that the tile's LDS allows (three, four, four workgroups
 // of eight waves per CU for 4, 2, 1 chunk rows): the register budget
 // follows.
@@ -435,6 +446,20 @@ __device__ __forceinline__ uint32_t child_off(uint32_t i) { return (2u << i) - 2
 constexpr uint32_t kLevelGroups = 255;   // groups of levels 0 .. 7
 constexpr uint32_t kLevelGw = 263;       // group word prefixes of levels 0 .. 7
 constexpr uint32_t kLevelChildren = 510; // children of levels 0 .. 7
+
+constexpr uint32_t kNoCarry = 0xFF;
+
+__device__ __forceinline__ uint32_t low_mask(uint32_t nbits) {
+  return nbits >= 32 ? 0xFFFFFFFFu : (1u << nbits) - 1u;
+}
+// 32 bits of the tile's bit string from bit sp on (zeros past its end)
+template <uint32_t T>
+__device__ __forceinline__ uint32_t tile_bits(const uint32_t* bits, uint32_t sp) {
+  const uint32_t sw = sp >> 5;
+  const uint64_t win = static_cast<uint64_t>(bits[sw]) |
+                       (sw + 1 < T / 32 ? static_cast<uint64_t>(bits[sw + 1]) << 32 : 0ull);
+  return static_cast<uint32_t>(win >> (sp & 31));
+}
 
 template <int CW, int ROWS>
 __global__ __launch_bounds__(kPassThreads)
@@ -461,9 +486,17 @@ void pass_kernel(PassParams p) {
   __shared__ uint32_t s_gSL[kLevelGroups];         // group tile start | length << 16
   __shared__ uint32_t s_gw[kLevelGw];              // destination words before each group
   __shared__ uint16_t s_base[kLevelChildren];      // partition base per child
+  __shared__ uint64_t s_cD[kLevelGroups];          // carried word (index into p.words)
+  __shared__ uint32_t s_cV[kLevelGroups];          // its bits so far
+  __shared__ uint8_t s_cLo[kLevelGroups];          // first of them; kNoCarry: none
+  __shared__ uint8_t s_cHi[kLevelGroups];          // bit after the last of them
 
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const uint64_t t = blockIdx.x;
+  if (tid < kLevelGroups) s_cLo[tid] = kNoCarry;
+  const uint64_t span = p.tiles_per_wg ? p.tiles_per_wg : 1;
+  const uint64_t tbeg = static_cast<uint64_t>(blockIdx.x) * span;
+  const uint64_t tend = tbeg + span < p.num_tiles ? tbeg + span : p.num_tiles;
+  for (uint64_t t = tbeg; t < tend; ++t) {
   const TileGeo g = tile_geo(p, t, T);
   const uint32_t k = p.node_k != nullptr ? p.node_k[g.node] : p.k;
   const uint32_t nb = 1u << k;
@@ -629,6 +662,7 @@ void pass_kernel(PassParams p) {
       const uint32_t total_words = gw[ngroups];
       const uint64_t lnw = s_lnw[i];
       uint32_t* words = p.words + s_lw[i];
+      // full words of the runs
       for (uint32_t w = tid; w < total_words; w += kPassThreads) {
         uint32_t lo = 0, hi = ngroups, wlo = 0;
         while (hi - lo > 1) {
@@ -641,21 +675,64 @@ void pass_kernel(PassParams p) {
         const uint32_t src0 = sl & 0xFFFFu, len = sl >> 16;
         const uint32_t j = w - wlo;                   // word of the run
         const uint32_t r5 = static_cast<uint32_t>(R) & 31u;
+        if ((j == 0 && r5 != 0) || r5 + len < 32u * (j + 1)) continue;  // partial: owner's
         const uint64_t D = (R >> 5) + j;
-        const uint32_t blo = j == 0 ? r5 : 0u;        // first bit of the word in the run
-        const uint32_t end = r5 + len - 32u * j;      // run end relative to the word
-        const uint32_t bhi = end < 32u ? end : 32u;
-        const uint32_t sp = src0 + 32u * j + blo - r5;
-        const uint32_t sw = sp >> 5;
-        const uint64_t win = static_cast<uint64_t>(s_bits[sw]) |
-                             (sw + 1 < T / 32 ? static_cast<uint64_t>(s_bits[sw + 1]) << 32 : 0ull);
-        uint32_t val = static_cast<uint32_t>(win >> (sp & 31));
-        const uint32_t nbits = bhi - blo;
-        if (nbits < 32) val &= (1u << nbits) - 1u;
-        val <<= blo;
+        const uint32_t sp = src0 + 32u * j - r5;
         if (D >= lnw) atomicOr(p.err, 1u);
-        else if (nbits == 32) words[D] = val;
-        else if (val) atomicOr(&words[D], val);
+        else words[D] = tile_bits<T>(s_bits, sp);
+      }
+      // partial first and last words: group q's owner, thread 4q
+      if ((tid & 3u) == 0 && (tid >> 2) < ngroups) {
+        const uint32_t q = tid >> 2;
+        const uint32_t sl = gSL[q];
+        const uint32_t src0 = sl & 0xFFFFu, len = sl >> 16;
+        if (len != 0) {
+          const uint32_t ci = grp_off(i) + q;
+          const uint64_t R = gR[q];
+          const uint32_t r5 = static_cast<uint32_t>(R) & 31u;
+          const uint64_t D0 = R >> 5;
+          const uint64_t aD0 = s_lw[i] + D0;
+          uint32_t clo = s_cLo[ci], chi = 0, cur = 0;
+          if (clo != kNoCarry) {
+            if (s_cD[ci] == aD0 && s_cHi[ci] == r5) {
+              cur = s_cV[ci];  // this run continues the carried bits
+            } else {           // not continued here: the word is shared
+              if (s_cV[ci]) atomicOr(&p.words[s_cD[ci]], s_cV[ci]);
+              clo = kNoCarry;
+            }
+          }
+          if (r5 != 0 || len < 32u) {
+            // first word: the run's bits [r5, min(32, r5 + len)) join the carry
+            const uint32_t hn = len < 32u - r5 ? len : 32u - r5;
+            cur |= (tile_bits<T>(s_bits, src0) & low_mask(hn)) << r5;
+            if (clo == kNoCarry) clo = r5;
+            if (r5 + len >= 32u) {  // complete
+              if (D0 >= lnw) atomicOr(p.err, 1u);
+              else if (clo == 0) words[D0] = cur;
+              else if (cur) atomicOr(&words[D0], cur);
+              clo = kNoCarry;
+            } else {
+              chi = r5 + len;
+            }
+          }
+          const uint32_t e5 = static_cast<uint32_t>(R + len) & 31u;
+          if (r5 + len > 32u && e5 != 0) {
+            // last word: bits [0, e5), carried into the next tile
+            cur = tile_bits<T>(s_bits, src0 + len - e5) & low_mask(e5);
+            clo = 0;
+            chi = e5;
+            s_cD[ci] = s_lw[i] + ((R + len) >> 5);
+          } else if (clo != kNoCarry) {
+            s_cD[ci] = aD0;
+          }
+          if (clo != kNoCarry && ((R + len) >> 5) >= lnw) {
+            atomicOr(p.err, 1u);
+            clo = kNoCarry;
+          }
+          s_cV[ci] = cur;
+          s_cLo[ci] = static_cast<uint8_t>(clo);
+          s_cHi[ci] = static_cast<uint8_t>(chi);
+        }
       }
     }
     const bool partition = (i + 1 < k) || (p.dst != nullptr);
@@ -740,9 +817,20 @@ void pass_kernel(PassParams p) {
       else atomicOr(p.err, 1u);
     }
   }
+  __syncthreads();  // the next tile rewrites the tables and the buffer
+  }  // tiles of the span
+  // words still carried are shared with the next span
+  if (tid < kLevelGroups && s_cLo[tid] != kNoCarry && s_cV[tid] != 0)
+    atomicOr(&p.words[s_cD[tid]], s_cV[tid]);
 }
 
 }  // namespace
+
+namespace {
+std::atomic<uint32_t> g_pass_span{0};  // 0: automatic
+}  // namespace
+
+void set_pass_tiles_per_wg(uint32_t span) { g_pass_span.store(span); }
 
 uint64_t pass_tile_codes(int cw, uint32_t rows) {
   return kPassThreads * rows * (16u / static_cast<uint32_t>(cw));
@@ -823,10 +911,19 @@ void launch_digit_starts(const uint64_t* totals, uint32_t nb, uint64_t* pc, hipS
   ECL_HIP(hipGetLastError());
 }
 
-void launch_pass(const PassParams& p, hipStream_t s) {
-  if (p.num_tiles == 0) return;
+void launch_pass(const PassParams& params, hipStream_t s) {
+  if (params.num_tiles == 0) return;
   ProfScope ps(K_PASS, s);
-  const dim3 grid(static_cast<uint32_t>(p.num_tiles)), block(kPassThreads);
+  PassParams p = params;
+  if (p.tiles_per_wg == 0) p.tiles_per_wg = g_pass_span.load();
+  if (p.tiles_per_wg == 0) {
+    // spans of up to 8 tiles while the grid still gives every CU about
+    // four rounds of resident workgroups (256 CUs, 3 per CU)
+    const uint64_t span = p.num_tiles / (256 * 3 * 4);
+    p.tiles_per_wg = static_cast<uint32_t>(span < 1 ? 1 : span > 8 ? 8 : span);
+  }
+  const uint64_t span = p.tiles_per_wg;
+  const dim3 grid(static_cast<uint32_t>((p.num_tiles + span - 1) / span)), block(kPassThreads);
   if (p.tile_codes != pass_tile_codes(p.cw, p.tile_rows))
     throw Error(ECL_ERROR_INTERNAL, "tile size does not match the tile rows");
 #define ECL_PASS_ROWS(CWV)                                                                  \

@@ -115,6 +115,10 @@ ecl_status ecl_device_mem_info(int device, size_t* free_bytes, size_t* total_byt
  * frees it.  No counterpart in the reference, which allocates temporaries
  * per build; for callers that need the memory back between builds. */
 ecl_status ecl_release_cached_memory(int device);
+/* Tiles of 32768 codes each construction workgroup takes in turn (0:
+ * automatic, up to 8 for large texts).  A tuning knob with no effect on
+ * results; no counterpart in the reference. */
+ecl_status ecl_set_build_span(uint32_t tiles_per_workgroup);
 ecl_status ecl_device_name(int device, char* buf, size_t len);
 /* events on `stream` (NULL = default stream); elapsed milliseconds */
 ecl_status ecl_event_create(void** event);

@@ -202,3 +202,18 @@ def test_forest_rejects_bad_input(gpu):
         Forest(d, np.array([0, 16], dtype=u), np.array([16, 0], dtype=u), np.array([4, 4], dtype=u))
     with pytest.raises(EclError):  # alphabet too large
         Forest(d, np.array([0], dtype=u), np.array([64], dtype=u), np.array([300], dtype=u))
+
+
+@pytest.mark.parametrize("span", [2, 5])
+def test_forest_build_spans(gpu, span):
+    """Workgroups spanning several tiles (and so several trees) build every
+    tree exactly."""
+    from ecl_wavelet_amd import set_build_span
+    texts, sigmas = _padded_texts(16 + span, 50, lo_log2=6, hi_log2=17)
+    set_build_span(span)
+    try:
+        f, _, _ = _forest(texts, sigmas)
+    finally:
+        set_build_span(0)
+    q, e = _queries(texts, sigmas, 300, seed=9)
+    assert np.array_equal(_run(f, q), e)

@@ -329,3 +329,30 @@ def test_scratch_cache_reuse_release_and_threads(gpu):
     assert not errs
     for i, t in enumerate(texts):
         _check_queries(out[i], t, None, nq=1000, seed=10 + i)
+
+
+@pytest.mark.parametrize("span", [2, 3, 8])
+def test_build_spans_vs_oracle(gpu, span):
+    """Construction workgroups that take several tiles in turn carry the
+    runs' partial words from tile to tile: every level stays bit-exact
+    (single-pass power-of-2 and not, Zipf texts with tiny nodes, multi-pass
+    16-bit trees with node-aligned tiles)."""
+    from ecl_wavelet_amd import WaveletTree, set_build_span
+    rng = np.random.default_rng(300 + span)
+    set_build_span(span)
+    try:
+        for sigma, n in [(256, 1_500_000), (20, 700_000), (5, 400_000), (2, 300_000)]:
+            text = rng.integers(0, sigma, n).astype(np.uint8)
+            wt = WaveletTree(text, np.arange(sigma, dtype=np.uint8))
+            _check_structure_vs_oracle(wt, text, sigma)
+        thr = O.zipf_thresholds(200, 1.3)
+        text = O.gen_cdf(span, 900_000, thr, np.uint8)
+        wt = WaveletTree(text, np.arange(200, dtype=np.uint8))
+        _check_structure_vs_oracle(wt, text, 200)
+        for sigma, n in [(1000, 600_000), (4097, 500_000)]:
+            text = rng.integers(0, sigma, n).astype(np.uint16)
+            wt = WaveletTree(text, np.arange(sigma, dtype=np.uint16))
+            _check_structure_vs_oracle(wt, text, sigma)
+            _check_queries(wt, text, None, nq=1000, seed=span)
+    finally:
+        set_build_span(0)
