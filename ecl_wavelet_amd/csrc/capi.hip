@@ -854,20 +854,14 @@ static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_
     // A single pass over a power-of-2 alphabet needs no counts on the host
     // before the end: every level holds n bits, the digit starts are
     // scanned on the device and the thresholds do not depend on counts.
-    // Its bit arrays exist before the first launch, and the histogram
-    // kernel zeroes them (its reads leave HBM write bandwidth free).
+    // Its bit arrays are not zeroed: the pass stores every word once except
+    // those its workgroups share, which are zeroed just before it.
     const bool single = plan.npasses == 1;
     const bool sizes_known = single && wt->pow2;
     std::vector<uint64_t> level_size(L, n);
     std::unique_ptr<ecl_bit_array_t> ba_guard;
-    if (sizes_known) {
-      ba_guard.reset(make_bit_array(level_size, -1, device, &up));
-      pp.fill_words = ba_guard->words();
-      pp.fill_count = ba_guard->total_words;
-      pp.fill_per = ((ba_guard->total_words + tiles0 - 1) / tiles0 + 3) & ~3ull;
-    }
+    if (sizes_known) ba_guard.reset(make_bit_array(level_size, -1, device, &up));
     launch_tile_hist(pp, s);
-    pp.fill_words = nullptr;
     launch_tile_scan(pp, s);
     // bin counts of the current pass: [node][digit] codes of the pass input
     std::vector<uint64_t> binc(nb0);
@@ -1031,6 +1025,10 @@ static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_
         pp.level_nwords[i] = (level_size[l0 + i] + 31) / 32;
       }
       up.flush();
+      if (sizes_known) {
+        pp.tiles_per_wg = pass_span(pp.num_tiles);
+        launch_span_words(pp, s);
+      }
       launch_pass(pp, s);
       if (!last) {
         in_codes = out_buf;

@@ -125,9 +125,6 @@ struct PassParams {
   const uint8_t* node_remap;  // [nodes] 1: codes come from codes_out
   uint32_t row_bins;          // entries per hist/excl row (0: 2^k)
   uint32_t thr_stride;        // entries per node in thr (0: k + 1)
-  // tile_hist also zeroes fill_words[0 .. fill_count), fill_per words per
-  // tile (a multiple of 4), when fill_words != nullptr
-  uint32_t* fill_words; uint64_t fill_count, fill_per;
   // construction: consecutive tiles per workgroup (0: chosen by launch_pass)
   uint32_t tiles_per_wg;
 };
@@ -137,6 +134,10 @@ void launch_tile_hist(const PassParams& p, hipStream_t s);
 void launch_tile_scan(const PassParams& p, hipStream_t s);
 void launch_pass(const PassParams& p, hipStream_t s);
 void set_pass_tiles_per_wg(uint32_t span);  // 0: automatic
+uint32_t pass_span(uint64_t num_tiles);     // tiles per workgroup of a pass
+// zeroes the words that several workgroups of a single-node pass write
+// (p.tiles_per_wg set); the pass then needs no zeroed bit arrays
+void launch_span_words(const PassParams& p, hipStream_t s);
 void launch_gather_rows(const uint64_t* excl, const uint32_t* rows, uint64_t nrows,
                         uint32_t nb, uint64_t* out, hipStream_t s);
 // pc[d] = sum of totals[0 .. d) for d <= nb (the start of every digit's codes

@@ -172,19 +172,28 @@ __global__ __launch_bounds__(512) void rs_local_kernel(const RsArrayDesc* __rest
   if (lane == 63) block_ones[gb] = incl;
 }
 
-// one workgroup (256 threads) per array: L1 = exclusive prefix of the
-// array's block totals; total ones at the end
-__global__ __launch_bounds__(256) void rs_scan_kernel(const RsArrayDesc* __restrict__ arrays,
-                                                      const uint32_t* __restrict__ block_ones) {
-  __shared__ uint64_t s_wave[4];
+// one workgroup (1024 threads) per array: L1 = exclusive prefix of the
+// array's block totals; total ones at the end.  Each thread takes a
+// contiguous range of blocks and keeps eight loads in flight.
+constexpr uint32_t kRsScanThreads = 1024;
+
+__global__ __launch_bounds__(kRsScanThreads) void rs_scan_kernel(
+    const RsArrayDesc* __restrict__ arrays, const uint32_t* __restrict__ block_ones) {
+  __shared__ uint64_t s_wave[kRsScanThreads / 64];
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const RsArrayDesc p = arrays[blockIdx.x];
   const uint32_t* cnt = block_ones + p.first_tile;
-  const uint64_t per = (p.num_l1 + 255) / 256;
-  const uint64_t b0 = tid * per;
-  const uint64_t b1 = (b0 + per < p.num_l1) ? b0 + per : p.num_l1;
+  const uint64_t per = (p.num_l1 + kRsScanThreads - 1) / kRsScanThreads;
+  const uint64_t b0 = tid * per < p.num_l1 ? tid * per : p.num_l1;
+  const uint64_t b1 = b0 + per < p.num_l1 ? b0 + per : p.num_l1;
   uint64_t sum = 0;
-  for (uint64_t b = b0; b < b1; ++b) sum += cnt[b];
+  for (uint64_t b = b0; b < b1; b += 8) {
+    uint32_t x[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) x[j] = b + j < b1 ? cnt[b + j] : 0u;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) sum += x[j];
+  }
   uint64_t incl = sum;
 #pragma unroll
   for (uint32_t off = 1; off < 64; off <<= 1) {
@@ -194,13 +203,19 @@ __global__ __launch_bounds__(256) void rs_scan_kernel(const RsArrayDesc* __restr
   if (lane == 63) s_wave[wave] = incl;
   __syncthreads();
   uint64_t run = incl - sum, total = 0;
-  for (uint32_t w = 0; w < 4; ++w) {
+  for (uint32_t w = 0; w < kRsScanThreads / 64; ++w) {
     run += (w < wave) ? s_wave[w] : 0ull;
     total += s_wave[w];
   }
-  for (uint64_t b = b0; b < b1; ++b) {
-    p.l1[b] = run;
-    run += cnt[b];
+  for (uint64_t b = b0; b < b1; b += 8) {
+    uint32_t x[8];
+#pragma unroll
+    for (int j = 0; j < 8; ++j) x[j] = b + j < b1 ? cnt[b + j] : 0u;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      if (b + j < b1) p.l1[b + j] = run;
+      run += x[j];
+    }
   }
   if (tid == 0) *p.total_ones = total;
 }
@@ -250,6 +265,9 @@ __global__ __launch_bounds__(512) void rs_samples_kernel(const RsArrayDesc* __re
   const uint64_t bits0 = kb * detail::kL1Bits;                     // first bit of the block
   const uint64_t bits = p.m - bits0 < detail::kL1Bits ? p.m - bits0 : detail::kL1Bits;
   const uint64_t nl2 = (bits + detail::kL2Bits - 1) / detail::kL2Bits;  // L2 blocks here
+  // the block's L2 entries are loaded together with its L1 entries
+  for (uint32_t x = lane; x < detail::kL2PerL1; x += 64)
+    s_l2[wave][x] = x < nl2 ? p.l2[kb * detail::kL2PerL1 + x] : 0xFFFFu;
   const uint64_t o0 = p.l1[kb];
   const uint64_t o1 = kb + 1 < p.num_l1 ? p.l1[kb + 1] : *p.total_ones;
   const uint64_t z0 = bits0 - o0, z1 = bits0 + bits - o1;
@@ -259,8 +277,6 @@ __global__ __launch_bounds__(512) void rs_samples_kernel(const RsArrayDesc* __re
   const uint64_t n1 = last1 >= first1 ? last1 - first1 + 1 : 0;
   const uint64_t n0 = last0 >= first0 ? last0 - first0 + 1 : 0;
   if (n1 + n0 == 0) return;
-  for (uint32_t x = lane; x < detail::kL2PerL1; x += 64)
-    s_l2[wave][x] = x < nl2 ? p.l2[kb * detail::kL2PerL1 + x] : 0xFFFFu;
   // the wave's LDS writes complete before its reads (in-order LDS queue)
   __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0)
   __builtin_amdgcn_wave_barrier();
@@ -432,8 +448,8 @@ void launch_rs_build(const RsArrayDesc* arrays, uint64_t narrays, uint64_t nbloc
   hipLaunchKernelGGL(rs_local_kernel, dim3(static_cast<uint32_t>(groups)), dim3(512), 0, s,
                      arrays, block_array, nblocks, block_ones);
   ECL_HIP(hipGetLastError());
-  hipLaunchKernelGGL(rs_scan_kernel, dim3(static_cast<uint32_t>(narrays)), dim3(256), 0, s,
-                     arrays, block_ones);
+  hipLaunchKernelGGL(rs_scan_kernel, dim3(static_cast<uint32_t>(narrays)), dim3(kRsScanThreads),
+                     0, s, arrays, block_ones);
   ECL_HIP(hipGetLastError());
   hipLaunchKernelGGL(rs_samples_kernel, dim3(static_cast<uint32_t>(groups)), dim3(512), 0, s,
                      arrays, block_array, nblocks);

@@ -189,15 +189,6 @@ __global__ __launch_bounds__(kPassThreads) void tile_hist_kernel(PassParams p) {
     reinterpret_cast<uint4*>(s_bins)[i] = make_uint4(0, 0, 0, 0);
   if (tid < kMaxBins) s_tot[tid] = 0;
   const uint64_t t = blockIdx.x;
-  // zero this tile's share of the bit arrays (the construction ORs run edges in)
-  if (p.fill_words != nullptr) {
-    const uint64_t w0 = t * p.fill_per;
-    const uint64_t w1 = w0 + p.fill_per < p.fill_count ? w0 + p.fill_per : p.fill_count;
-    for (uint64_t w = w0 + 4ull * tid; w < w1; w += 4ull * kPassThreads) {
-      if (w + 4 <= w1) *reinterpret_cast<uint4*>(p.fill_words + w) = make_uint4(0, 0, 0, 0);
-      else for (uint64_t x = w; x < w1; ++x) p.fill_words[x] = 0u;
-    }
-  }
   __syncthreads();
   uint32_t* bins = s_bins + tid % C;
   const TileGeo g = tile_geo(p, t, T);
@@ -407,6 +398,43 @@ __global__ __launch_bounds__(64) void digit_starts_kernel(const uint64_t* __rest
   for (int j = 0; j < 4; ++j)
     if (4 * lane + j < nb) pc[4 * lane + j] = v[j];
   if (lane == 0) pc[nb] = tot;
+}
+
+// Words written by more than one workgroup of a single-node pass whose
+// workgroups take spans of `span` tiles (everything else gets one plain
+// store): for every span start and every group of every level, the word
+// holding the group's first bit in that span (a run carried in from the
+// previous span completes it), and the last word of every level.  One
+// workgroup per span; the bit arrays need no other zeroing.
+__global__ __launch_bounds__(256) void span_words_kernel(PassParams p) {
+  __shared__ uint64_t s_e[kMaxBins + 1];  // codes of digit < x in tiles before the span
+  __shared__ uint64_t s_p[kMaxBins + 1];  // global start of digit x
+  const uint32_t tid = threadIdx.x, lane = tid & 63;
+  const uint64_t tbeg = static_cast<uint64_t>(blockIdx.x) * p.tiles_per_wg;
+  const uint32_t k = p.k, nb = 1u << k;
+  if (tid < 64) {
+    uint64_t v[4], tot;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t d = 4 * lane + j;
+      v[j] = d < nb ? p.excl[tbeg * nb + d] : 0ull;
+    }
+    wave_excl_scan4<uint64_t>(v, tot);
+#pragma unroll
+    for (int j = 0; j < 4; ++j) s_e[4 * lane + j] = v[j];
+    if (lane == 0) s_e[kMaxBins] = tot;
+  }
+  for (uint32_t x = tid; x <= nb; x += 256) s_p[x] = p.pc[x];
+  __syncthreads();
+  // level-groups x = 2^i - 1 + q (level i, group q)
+  for (uint32_t x = tid; x + 1 < nb; x += 256) {
+    const uint32_t i = 31u - __builtin_clz(x + 1), q = x + 1 - (1u << i);
+    const uint32_t gshift = k - i, a = q << gshift, b = (q + 1) << gshift;
+    const uint64_t D = (s_p[a] + s_e[b] - s_e[a]) >> 5;
+    if (D < p.level_nwords[i]) p.words[p.level_word[i] + D] = 0u;
+  }
+  if (blockIdx.x == 0 && tid < k && p.level_nwords[tid] != 0)
+    p.words[p.level_word[tid] + p.level_nwords[tid] - 1] = 0u;
 }
 
 // ---------------------------------------------------------------------------
@@ -832,6 +860,24 @@ std::atomic<uint32_t> g_pass_span{0};  // 0: automatic
 
 void set_pass_tiles_per_wg(uint32_t span) { g_pass_span.store(span); }
 
+uint32_t pass_span(uint64_t num_tiles) {
+  const uint32_t forced = g_pass_span.load();
+  if (forced != 0) return forced;
+  // spans of up to 8 tiles while the grid still gives every CU about four
+  // rounds of resident workgroups (256 CUs, 3 per CU)
+  const uint64_t span = num_tiles / (256 * 3 * 4);
+  return static_cast<uint32_t>(span < 1 ? 1 : span > 8 ? 8 : span);
+}
+
+void launch_span_words(const PassParams& p, hipStream_t s) {
+  if (p.num_tiles == 0) return;
+  if (p.tiles_per_wg == 0 || p.tile_start != nullptr || p.k > 8)
+    throw Error(ECL_ERROR_INTERNAL, "span words need a single-node pass with a set span");
+  const uint64_t spans = (p.num_tiles + p.tiles_per_wg - 1) / p.tiles_per_wg;
+  hipLaunchKernelGGL(span_words_kernel, dim3(static_cast<uint32_t>(spans)), dim3(256), 0, s, p);
+  ECL_HIP(hipGetLastError());
+}
+
 uint64_t pass_tile_codes(int cw, uint32_t rows) {
   return kPassThreads * rows * (16u / static_cast<uint32_t>(cw));
 }
@@ -915,13 +961,7 @@ void launch_pass(const PassParams& params, hipStream_t s) {
   if (params.num_tiles == 0) return;
   ProfScope ps(K_PASS, s);
   PassParams p = params;
-  if (p.tiles_per_wg == 0) p.tiles_per_wg = g_pass_span.load();
-  if (p.tiles_per_wg == 0) {
-    // spans of up to 8 tiles while the grid still gives every CU about
-    // four rounds of resident workgroups (256 CUs, 3 per CU)
-    const uint64_t span = p.num_tiles / (256 * 3 * 4);
-    p.tiles_per_wg = static_cast<uint32_t>(span < 1 ? 1 : span > 8 ? 8 : span);
-  }
+  if (p.tiles_per_wg == 0) p.tiles_per_wg = pass_span(p.num_tiles);
   const uint64_t span = p.tiles_per_wg;
   const dim3 grid(static_cast<uint32_t>((p.num_tiles + span - 1) / span)), block(kPassThreads);
   if (p.tile_codes != pass_tile_codes(p.cw, p.tile_rows))
