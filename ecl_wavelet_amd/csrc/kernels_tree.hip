@@ -518,6 +518,7 @@ void pass_kernel(PassParams p) {
   __shared__ uint32_t s_cV[kLevelGroups];          // its bits so far
   __shared__ uint8_t s_cLo[kLevelGroups];          // first of them; kNoCarry: none
   __shared__ uint8_t s_cHi[kLevelGroups];          // bit after the last of them
+  __shared__ uint32_t s_even[8];                   // level's groups fit 512 >> i threads each
 
   const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   if (tid < kLevelGroups) s_cLo[tid] = kNoCarry;
@@ -613,6 +614,14 @@ void pass_kernel(PassParams p) {
     const uint64_t incl = detail::wave_incl_scan_2x32(v);
     const uint32_t wex = static_cast<uint32_t>(incl - v);           // words before q0
     const uint32_t oex = static_cast<uint32_t>((incl - v) >> 32);   // ones before q0
+    // groups of at most 3 * (512 >> i) words: written by 512 >> i threads each
+    uint32_t mx = nw[0] > nw[1] ? nw[0] : nw[1];
+#pragma unroll
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+      const uint32_t y = __shfl_xor(mx, off, 64);
+      mx = mx > y ? mx : y;
+    }
+    if (lane == 0) s_even[i] = mx <= 3u * (kPassThreads >> i) ? 1u : 0u;
     uint32_t* gw = s_gw + gw_off(i);
     if (2 * lane < ngroups) gw[2 * lane] = wex;
     if (2 * lane + 1 < ngroups) gw[2 * lane + 1] = wex + nw[0];
@@ -691,6 +700,23 @@ void pass_kernel(PassParams p) {
       const uint64_t lnw = s_lnw[i];
       uint32_t* words = p.words + s_lw[i];
       // full words of the runs
+      if (s_even[i]) {
+        // group tid >> (9 - i), its words j = tid mod 2^(9-i) + m * 2^(9-i)
+        const uint32_t tsh = 9u - i, q = tid >> tsh;
+        if (q < ngroups) {
+          const uint32_t nwq = gw[q + 1] - gw[q];
+          const uint64_t R = gR[q];
+          const uint32_t sl = gSL[q];
+          const uint32_t src0 = sl & 0xFFFFu, len = sl >> 16;
+          const uint32_t r5 = static_cast<uint32_t>(R) & 31u;
+          for (uint32_t j = tid & ((1u << tsh) - 1u); j < nwq; j += 1u << tsh) {
+            if ((j == 0 && r5 != 0) || r5 + len < 32u * (j + 1)) continue;  // partial: owner's
+            const uint64_t D = (R >> 5) + j;
+            if (D >= lnw) atomicOr(p.err, 1u);
+            else words[D] = tile_bits<T>(s_bits, src0 + 32u * j - r5);
+          }
+        }
+      } else
       for (uint32_t w = tid; w < total_words; w += kPassThreads) {
         uint32_t lo = 0, hi = ngroups, wlo = 0;
         while (hi - lo > 1) {
