@@ -52,6 +52,9 @@ def parse():
                     help="resident trees of the serving configuration (0 skips it)")
     ap.add_argument("--sv-max-log2", type=float, default=32.0,
                     help="log2 of the largest serving text")
+    ap.add_argument("--bin-log2", type=int, default=34,
+                    help="binary RankSelect section: 2^k bits (0: skip)")
+    ap.add_argument("--bin-queries", type=int, default=100_000_000)
     ap.add_argument("--pmc-workload", action="store_true",
                     help="short run for hardware-counter passes: one 1 GiB "
                          "device-to-device copy (the calibration kernel) and "
@@ -258,6 +261,53 @@ def bench_batched(args):
         "plain_per_tree_p99_ms": float(np.percentile(lat, 99)),
         "autotuned": {"block": block, "grid": grid},
     }
+
+
+def bench_binary(args):
+    """Binary RankSelect configuration: 2^bin_log2 i.i.d. Bernoulli(p) bits
+    drawn on the device, p in {0.5, 0.05}; directory build time, then
+    bin_queries rank1(i) with uniform i and select1(k) with uniform k over
+    the popcount, generated on the device (timed with their generation)."""
+    import ecl_wavelet_amd as E
+    m = 1 << args.bin_log2
+    Q = args.bin_queries
+    timer = E.Timer()
+    out = E.DeviceBuffer(Q * 8)
+    res = {"config": (f"{m} bits i.i.d. Bernoulli(p), drawn on the device; {Q} rank1 and "
+                      f"{Q} select1 queries generated on the device"),
+           "bits": m, "queries": Q}
+    for p, seed in ((0.5, 2), (0.05, 3)):
+        ba = E.BitArray([m])
+        ba.fillRandom(0, seed, min(int(p * 2.0 ** 64), 2 ** 64 - 1))
+        E.synchronize()
+        timer.start()
+        rs = E.RankSelect(ba)
+        build_ms = timer.stop()
+        rs.rankGenerated(0, 1, Q, out, 1)  # warm-up
+        rs.selectGenerated(0, 1, Q, out, 1)
+        E.synchronize()
+        timer.start()
+        rs.rankGenerated(0, 11, Q, out, 1)
+        rank_ms = timer.stop()
+        timer.start()
+        rs.selectGenerated(0, 12, Q, out, 1)
+        select_ms = timer.stop()
+        r = {
+            "build_ms": build_ms,
+            "build_gbps": m / 8 / (build_ms / 1e3) / 1e9,
+            "rank1_qps": Q / (rank_ms / 1e3),
+            "select1_qps": Q / (select_ms / 1e3),
+            # one bitvector line + directory per query (128+16 B rank1,
+            # 256+16 B select1)
+            "rank1_line_gbps": Q * 144 / (rank_ms / 1e3) / 1e9,
+            "select1_line_gbps": Q * 272 / (select_ms / 1e3) / 1e9,
+        }
+        res[f"p={p}"] = r
+        log(f"binary p={p}: build {build_ms:.2f} ms, rank1 {r['rank1_qps']:.3g} q/s, "
+            f"select1 {r['select1_qps']:.3g} q/s")
+        del rs
+    del out
+    return res
 
 
 def bench_serving(args):
@@ -484,6 +534,8 @@ def main():
         res["batched"] = bench_batched(args)
     if args.sv_trees > 0:
         res["serving"] = bench_serving(args)
+    if args.bin_log2 > 0:
+        res["binary_rank_select"] = bench_binary(args)
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(n, sigma, seed, args)
     print(json.dumps(res), flush=True)

@@ -196,3 +196,57 @@ def test_large_generated_bits_and_queries(gpu, p):
     k = np.array([1 + ((O.hash64(78, j) * ones) >> 64) for j in range(q)], dtype=np.uint64)
     one_pos = np.flatnonzero(bits).astype(np.uint64)
     assert np.array_equal(got, one_pos[k - 1])
+
+
+def _rank_from_index(words, l1, l2, idx):
+    """rank1(i) from the oracle's directory: L1 + L2 + ones of the words of
+    the 512-bit block before i (vectorized; independent of the kernels)."""
+    idx = idx.astype(np.uint64)
+    blk = (idx >> np.uint64(9)).astype(np.int64)
+    base = l1[blk >> 7] + l2[blk].astype(np.uint64)
+    w0 = blk * 16
+    out = base.copy()
+    wi = (idx >> np.uint64(5)).astype(np.int64)
+    for j in range(16):
+        w = w0 + j
+        full = w < wi
+        out += np.where(full, np.bitwise_count(words[np.minimum(w, len(words) - 1)]), 0).astype(np.uint64)
+    part = (idx & np.uint64(31)).astype(np.uint32)
+    last = words[np.minimum(wi, len(words) - 1)] & ((np.uint64(1) << part.astype(np.uint64)) - np.uint64(1)).astype(np.uint32)
+    out += np.where(part > 0, np.bitwise_count(last), 0).astype(np.uint64)
+    return out
+
+
+@pytest.mark.parametrize("p", [0.5, 0.05])
+def test_binary_2e34_bits(gpu, p):
+    """The binary RankSelect configuration at its full size: 2^34 Bernoulli
+    bits drawn on the device.  L1, L2 and both sample arrays equal the
+    oracle's directory over the same words; device-generated rank1 queries
+    equal L1 + L2 + in-block ones; every generated select1(k) answer j has
+    bit j set and rank1(j) = k - 1."""
+    from ecl_wavelet_amd import BitArray, DeviceBuffer, RankSelect
+    m = 1 << 34
+    thr = O.bernoulli_threshold(p)
+    ba = BitArray([m])
+    ba.fillRandom(0, 2 if p == 0.5 else 3, thr)
+    rs = RankSelect(ba)
+    words = rs.bit_array_.words(0)
+    l1, l2, s1, s0, ones = O.rs_index(words, m)
+    assert rs.getTotalNumVals(0, 1) == ones
+    assert np.array_equal(rs.l1Entries(0), l1)
+    assert np.array_equal(rs.l2Entries(0), l2)
+    assert np.array_equal(rs.selectSamples(0, 1), s1)
+    assert np.array_equal(rs.selectSamples(0, 0), s0)
+    q = 50_000
+    out = DeviceBuffer(q * 8)
+    rs.rankGenerated(0, 91, q, out, 1)
+    got = out.download(np.uint64)
+    idx = np.array([(O.hash64(91, j) * m) >> 64 for j in range(q)], dtype=np.uint64)
+    assert np.array_equal(got, _rank_from_index(words, l1, l2, idx))
+    rs.selectGenerated(0, 92, q, out, 1)
+    pos = out.download(np.uint64)
+    k = np.array([1 + ((O.hash64(92, j) * ones) >> 64) for j in range(q)], dtype=np.uint64)
+    assert np.all(pos < m)
+    bit = (words[(pos >> np.uint64(5)).astype(np.int64)] >> (pos & np.uint64(31)).astype(np.uint32)) & 1
+    assert np.all(bit == 1)
+    assert np.array_equal(_rank_from_index(words, l1, l2, pos), k - np.uint64(1))
