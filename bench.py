@@ -55,6 +55,9 @@ def parse():
     ap.add_argument("--bin-log2", type=int, default=34,
                     help="binary RankSelect section: 2^k bits (0: skip)")
     ap.add_argument("--bin-queries", type=int, default=100_000_000)
+    ap.add_argument("--full-sweep", type=int, default=1,
+                    help="builds and queries at the other full sizes: u8 sigma 4/16/64 "
+                         "(n=6e9) and u16 sigma 2^16 Zipf(1.1) (n=2e9); 0 skips")
     ap.add_argument("--pmc-workload", action="store_true",
                     help="short run for hardware-counter passes: one 1 GiB "
                          "device-to-device copy (the calibration kernel) and "
@@ -310,6 +313,56 @@ def bench_binary(args):
     return res
 
 
+def bench_full_sweep(args):
+    """The full-size configurations beside the headline one: uint8 texts of
+    6e9 symbols uniform over sigma in {4, 16, 64}, and a uint16 text of 2e9
+    symbols Zipf(1.1) over sigma = 2^16.  Per text: median of two timed
+    builds after one untimed, then 10^7 device-generated queries per kind."""
+    import ecl_wavelet_amd as E
+    timer = E.Timer()
+    Q = 10_000_000
+    d_idx, d_rq, d_sq = E.DeviceBuffer(Q * 8), E.DeviceBuffer(Q * 16), E.DeviceBuffer(Q * 16)
+    d_out = E.DeviceBuffer(Q * 8)
+    cases = [(np.uint8, 6_000_000_000, s, None, 5 + j) for j, s in enumerate((4, 16, 64))]
+    cases.append((np.uint16, 2_000_000_000, 65536, E.zipf_thresholds(65536, 1.1), 8))
+    out = []
+    for dt, n, sigma, thr, seed in cases:
+        d_text = E.generate_text(n, sigma, dt, seed=seed, thresholds=thr)
+        E.synchronize()
+        alphabet = np.arange(sigma, dtype=dt)
+        builds = []
+        for rep in range(3):
+            wt = None
+            timer.start()
+            wt = E.WaveletTree(d_text, alphabet, dtype=dt, n=n)
+            b = timer.stop()
+            if rep:
+                builds.append(b)
+        wt.generateQueries(0, 31, Q, d_idx)
+        wt.generateQueries(1, 32, Q, d_rq)
+        wt.generateQueries(2, 33, Q, d_sq)
+        E.synchronize()
+        t = {}
+        for kind, call_ in (("access", lambda: wt.accessDevice(d_idx, Q, d_out, flags=2)),
+                            ("rank", lambda: wt.rankDevice(d_rq, Q, d_out, flags=1)),
+                            ("select", lambda: wt.selectDevice(d_sq, Q, d_out, flags=1))):
+            timer.start()
+            call_()
+            t[kind] = timer.stop()
+        build_ms = float(np.median(builds))
+        r = {"dtype": np.dtype(dt).name, "n": n, "sigma": sigma,
+             "distribution": "uniform" if thr is None else "zipf(1.1)",
+             "levels": int(wt.getNumLevels()), "build_ms": build_ms,
+             "build_gbps": n * np.dtype(dt).itemsize / (build_ms / 1e3) / 1e9,
+             "access_qps": Q / (t["access"] / 1e3), "rank_qps": Q / (t["rank"] / 1e3),
+             "select_qps": Q / (t["select"] / 1e3)}
+        out.append(r)
+        log(f"sweep {r['dtype']} sigma={sigma} n={n}: build {build_ms:.1f} ms, access "
+            f"{r['access_qps']:.3g}, rank {r['rank_qps']:.3g}, select {r['select_qps']:.3g} q/s")
+        del wt, d_text
+    return out
+
+
 def bench_serving(args):
     """Serving configuration: args.sv_trees resident uint8 trees, lengths
     log-uniform in [2^20, 2^sv_max_log2], sigma in {4, 256}; Poisson request
@@ -536,6 +589,8 @@ def main():
         res["serving"] = bench_serving(args)
     if args.bin_log2 > 0:
         res["binary_rank_select"] = bench_binary(args)
+    if args.full_sweep:
+        res["full_size_sweep"] = bench_full_sweep(args)
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(n, sigma, seed, args)
     print(json.dumps(res), flush=True)
