@@ -58,6 +58,8 @@ def parse():
     ap.add_argument("--full-sweep", type=int, default=1,
                     help="builds and queries at the other full sizes: u8 sigma 4/16/64 "
                          "(n=6e9) and u16 sigma 2^16 Zipf(1.1) (n=2e9); 0 skips")
+    ap.add_argument("--host-queries", type=int, default=100_000_000,
+                    help="host-to-host section: queries per kind from host memory (0 skips)")
     ap.add_argument("--pmc-workload", action="store_true",
                     help="short run for hardware-counter passes: one 1 GiB "
                          "device-to-device copy (the calibration kernel) and "
@@ -264,6 +266,56 @@ def bench_batched(args):
         "plain_per_tree_p99_ms": float(np.percentile(lat, 99)),
         "autotuned": {"block": block, "grid": grid},
     }
+
+
+def bench_host_path(args, d_text, n, sigma):
+    """Host-to-host figures beside the device-resident ones: a build from a
+    host copy of the text (its upload included), then host arrays of Q
+    queries per kind answered through the pinned, chunked upload / kernel /
+    download pipeline.  Wall-clock times around each call."""
+    import ecl_wavelet_amd as E
+    Q = args.host_queries
+    text = d_text.download(np.uint8, n)
+    alphabet = np.arange(sigma, dtype=np.uint8)
+    t0 = time.perf_counter()
+    wt = E.WaveletTree(text, alphabet)
+    build_s = time.perf_counter() - t0
+    del text
+    d_q = E.DeviceBuffer(Q * 16)
+    wt.generateQueries(0, 501, Q, d_q)
+    idx = d_q.download(np.uint64, Q)
+    wt.generateQueries(1, 502, Q, d_q)
+    rq = d_q.download(E.QUERY_DTYPE, Q)
+    wt.generateQueries(2, 503, Q, d_q)
+    sq = d_q.download(E.QUERY_DTYPE, Q)
+    del d_q
+    wt.access(idx[:1000], flags=2)
+    wt.rank(rq[:1000], flags=1)
+    wt.select(sq[:1000], flags=1)
+    t = {}
+    for kind, fn in (("access", lambda: wt.access(idx, flags=2)),
+                     ("rank", lambda: wt.rank(rq, flags=1)),
+                     ("select", lambda: wt.select(sq, flags=1))):
+        t0 = time.perf_counter()
+        fn()
+        t[kind] = time.perf_counter() - t0
+    res = {
+        "config": (f"host uint8 text of {n} symbols (sigma={sigma}) and host arrays of {Q} "
+                   "queries per kind; wall clock, uploads and downloads included"),
+        "build_ms": build_s * 1e3,
+        "build_gbps": n / build_s / 1e9,
+        "access_qps": Q / t["access"],
+        "rank_qps": Q / t["rank"],
+        "select_qps": Q / t["select"],
+        # bytes crossing PCIe per query: 8 B index + 1 B symbol (access),
+        # 16 B query + 8 B result (rank, select)
+        "access_pcie_gbps": Q * 9 / t["access"] / 1e9,
+        "rank_pcie_gbps": Q * 24 / t["rank"] / 1e9,
+        "select_pcie_gbps": Q * 24 / t["select"] / 1e9,
+    }
+    log(f"host path: build {res['build_ms']:.0f} ms, access {res['access_qps']:.3g}, "
+        f"rank {res['rank_qps']:.3g}, select {res['select_qps']:.3g} q/s")
+    return res
 
 
 def bench_binary(args):
@@ -581,8 +633,11 @@ def main():
             "launch_ms": pass_ms,
         },
     }
-    d_text.free()
     del d_idx, d_rq, d_sq, d_mq, d_out
+    wt = None
+    if args.host_queries > 0:
+        res["host_to_host"] = bench_host_path(args, d_text, n, sigma)
+    d_text.free()
     if args.bt_texts > 0:
         res["batched"] = bench_batched(args)
     if args.sv_trees > 0:

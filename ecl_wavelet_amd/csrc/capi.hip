@@ -3,9 +3,14 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
+#include <condition_variable>
 #include <cstring>
+#include <deque>
+#include <functional>
 #include <memory>
 #include <mutex>
+#include <thread>
 #include <new>
 #include <numeric>
 #include <string>
@@ -85,6 +90,110 @@ void Uploader::add_bytes(void* dst, const void* src, size_t bytes) {
     to += take;
     bytes -= take;
   }
+}
+
+// ---- host copy threads -----------------------------------------------------------
+namespace {
+class CopyPool {
+ public:
+  static CopyPool& get() {
+    static CopyPool* pool = new CopyPool();  // never destroyed (threads run to exit)
+    return *pool;
+  }
+  void copy(void* dst, const void* src, size_t bytes) {
+    constexpr size_t kPiece = 4u << 20;
+    const size_t parts = std::min<size_t>(workers_ + 1, bytes / kPiece);
+    if (parts <= 1) {
+      std::memcpy(dst, src, bytes);
+      return;
+    }
+    struct Batch {
+      std::mutex m;
+      std::condition_variable cv;
+      size_t left;
+    } batch;
+    batch.left = parts - 1;
+    const size_t per = (bytes / parts + 63) & ~size_t(63);
+    auto* d = static_cast<uint8_t*>(dst);
+    auto* sp = static_cast<const uint8_t*>(src);
+    {
+      std::lock_guard<std::mutex> lk(mu_);
+      for (size_t i = 1; i < parts; ++i) {
+        const size_t a = i * per, b = std::min(bytes, a + per);
+        tasks_.push_back([=, &batch] {
+          if (a < b) std::memcpy(d + a, sp + a, b - a);
+          std::lock_guard<std::mutex> bl(batch.m);
+          if (--batch.left == 0) batch.cv.notify_one();
+        });
+      }
+    }
+    cv_.notify_all();
+    std::memcpy(d, sp, std::min(per, bytes));
+    std::unique_lock<std::mutex> bl(batch.m);
+    batch.cv.wait(bl, [&] { return batch.left == 0; });
+  }
+
+ private:
+  CopyPool() {
+    const unsigned hw = std::thread::hardware_concurrency();
+    workers_ = std::min<unsigned>(7, hw > 1 ? hw / 2 : 0);
+    for (unsigned i = 0; i < workers_; ++i) std::thread([this] { run(); }).detach();
+  }
+  void run() {
+    for (;;) {
+      std::function<void()> task;
+      {
+        std::unique_lock<std::mutex> lk(mu_);
+        cv_.wait(lk, [&] { return !tasks_.empty(); });
+        task = std::move(tasks_.front());
+        tasks_.pop_front();
+      }
+      task();
+    }
+  }
+  unsigned workers_ = 0;
+  std::mutex mu_;
+  std::condition_variable cv_;
+  std::deque<std::function<void()>> tasks_;
+};
+
+struct Staging {
+  std::mutex mu;
+  void* pin[2] = {nullptr, nullptr};
+  hipEvent_t done[2] = {nullptr, nullptr};
+};
+constexpr size_t kStageBytes = 64u << 20;
+Staging* const g_staging = new Staging();  // never destroyed
+}  // namespace
+
+void parallel_memcpy(void* dst, const void* src, size_t bytes) {
+  CopyPool::get().copy(dst, src, bytes);
+}
+
+void upload_staged(void* d_dst, const void* src, size_t bytes, hipStream_t s) {
+  if (bytes < 2 * kStageBytes) {
+    h2d(d_dst, static_cast<const uint8_t*>(src), bytes);
+    return;
+  }
+  Staging& st = *g_staging;
+  std::lock_guard<std::mutex> lk(st.mu);
+  for (int b = 0; b < 2; ++b) {
+    if (!st.pin[b]) ECL_HIP(hipHostMalloc(&st.pin[b], kStageBytes, hipHostMallocDefault));
+    if (!st.done[b]) ECL_HIP(hipEventCreateWithFlags(&st.done[b], hipEventDisableTiming));
+  }
+  auto* d = static_cast<uint8_t*>(d_dst);
+  auto* sp = static_cast<const uint8_t*>(src);
+  bool used[2] = {false, false};
+  for (size_t off = 0, c = 0; off < bytes; off += kStageBytes, ++c) {
+    const int b = static_cast<int>(c & 1);
+    const size_t len = std::min(kStageBytes, bytes - off);
+    if (used[b]) ECL_HIP(hipEventSynchronize(st.done[b]));  // its last transfer is done
+    parallel_memcpy(st.pin[b], sp + off, len);
+    ECL_HIP(hipMemcpyAsync(d + off, st.pin[b], len, hipMemcpyHostToDevice, s));
+    ECL_HIP(hipEventRecord(st.done[b], s));
+    used[b] = true;
+  }
+  ECL_HIP(hipStreamSynchronize(s));
 }
 
 // ---- per-device scratch cache ------------------------------------------------
@@ -733,7 +842,7 @@ static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_
     const void* d_text = text;
     if (!text_on_device) {
       own_text = DevBuf(n * esize);
-      h2d(own_text.ptr, static_cast<const uint8_t*>(text), n * esize);
+      upload_staged(own_text.ptr, text, n * esize, s);
       d_text = own_text.ptr;
     }
     // the passes read 16-byte vectors: a text that is its own code and is
@@ -1632,15 +1741,15 @@ static void pipelined(ecl_wavelet_tree wt, const void* in, size_t in_elem, void*
   auto drain = [&](int b) {
     if (!pending_len[b]) return;
     ECL_HIP(hipStreamSynchronize(P.stream[b]));
-    std::memcpy(static_cast<uint8_t*>(out) + pending_off[b] * out_elem, P.pin_out[b],
-                pending_len[b] * out_elem);
+    parallel_memcpy(static_cast<uint8_t*>(out) + pending_off[b] * out_elem, P.pin_out[b],
+                    pending_len[b] * out_elem);
     pending_len[b] = 0;
   };
   for (size_t c = 0; c < nchunks; ++c) {
     const int b = static_cast<int>(c & 1);
     const size_t off = c * chunk, len = std::min(chunk, m - off);
     drain(b);
-    std::memcpy(P.pin_in[b], static_cast<const uint8_t*>(in) + off * in_elem, len * in_elem);
+    parallel_memcpy(P.pin_in[b], static_cast<const uint8_t*>(in) + off * in_elem, len * in_elem);
     ECL_HIP(hipMemcpyAsync(P.d_in[b].ptr, P.pin_in[b], len * in_elem, hipMemcpyHostToDevice,
                            P.stream[b]));
     launch(P.d_in[b].ptr, len, P.d_out[b].ptr, P.stream[b]);

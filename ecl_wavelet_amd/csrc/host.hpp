@@ -104,6 +104,15 @@ size_t cached_scratch_bytes(int device);
 // frees the device's scratch cache (waits for a build that uses it)
 void release_scratch_cache(int device);
 
+// memcpy of large host buffers split over a small pool of host threads
+// (one copy thread reaches a fraction of the memory bandwidth that PCIe
+// transfers of pinned buffers need); small copies run on the caller.
+void parallel_memcpy(void* dst, const void* src, size_t bytes);
+// Host -> device copy of a large pageable buffer through two pinned staging
+// buffers (kept for the process), overlapping the host copies with the
+// transfers; returns after the data is on the device.
+void upload_staged(void* d_dst, const void* src, size_t bytes, hipStream_t s);
+
 // ---- tree shape: minimal codes and the internal nodes of every level
 struct Node { uint64_t lo, hi; };
 struct TreeShape {

@@ -356,3 +356,36 @@ def test_build_spans_vs_oracle(gpu, span):
             _check_queries(wt, text, None, nq=1000, seed=span)
     finally:
         set_build_span(0)
+
+
+def test_host_text_staged_upload_and_chunked_host_queries(gpu):
+    """A host text above the staging size (uploaded through the pinned
+    double buffer with parallel host copies) builds the same levels as the
+    same text already on the device; host query arrays spanning several
+    pipeline chunks answer as the device-resident calls."""
+    from ecl_wavelet_amd import DeviceBuffer, WaveletTree
+    rng = np.random.default_rng(42)
+    n = 300_000_000 + 12345
+    text = rng.integers(0, 256, n, dtype=np.uint8)
+    alpha = np.arange(256, dtype=np.uint8)
+    wt_h = WaveletTree(text, alpha)
+    d = DeviceBuffer.from_array(text)
+    wt_d = WaveletTree(d, alpha, dtype=np.uint8, n=n)
+    ba_h, ba_d = wt_h.rank_select_.bit_array_, wt_d.rank_select_.bit_array_
+    for l in range(8):
+        assert np.array_equal(ba_h.words(l), ba_d.words(l)), l
+    m = 9_000_000
+    d_q = DeviceBuffer(m * 16)
+    d_out = DeviceBuffer(m * 8)
+    for kind in (1, 2):
+        wt_d.generateQueries(kind, 70 + kind, m, d_q)
+        q = d_q.download(np.dtype([("index", "<u8"), ("symbol", "<u8")]), m)
+        if kind == 1:
+            wt_d.rankDevice(d_q, m, d_out, flags=1)
+            host = wt_h.rank(q, flags=1)
+        else:
+            wt_d.selectDevice(d_q, m, d_out, flags=1)
+            host = wt_h.select(q, flags=1)
+        assert np.array_equal(host, d_out.download(np.uint64, m))
+    idx = rng.integers(0, n, m).astype(np.uint64)
+    assert np.array_equal(wt_h.access(idx, flags=2), text[idx])
