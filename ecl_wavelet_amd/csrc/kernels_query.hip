@@ -353,7 +353,7 @@ __global__ __launch_bounds__(256) void rs_select_kernel(
     const uint64_t k =
         generated ? (total ? 1 + detail::bounded64(detail::hash64(seed, q), total) : 1)
                   : ks[q];
-    out[q] = detail::level_select<Value>(v, k);
+    out[q] = detail::level_select<Value, true>(v, k);
   }
 }
 

@@ -83,7 +83,7 @@ class RankSelect {
     static_assert(Value == 0 || Value == 1, "Value must be 0 or 1.");
     static_assert(NumThreads > 0 && NumThreads <= WS && (WS % NumThreads) == 0,
                   "NumThreads must divide the wave width");
-    return detail::level_select<Value>(levels_[array_index], i);
+    return detail::level_select<Value, true>(levels_[array_index], i);
   }
 
   // ---- directory accessors (rank_select.cuh:530-701)

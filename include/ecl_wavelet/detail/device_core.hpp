@@ -212,10 +212,16 @@ __device__ __forceinline__ uint64_t level_rank(const LevelView& v,
 }
 
 // Position of the i-th (1-based) Value; size when i exceeds the total
-// (rank_select.cuh:224-523 semantics).  Search: sample pair brackets the
-// answer, binary search over L1 then L2 inside the bracket, then one 64 B
-// block scan.
-template <int Value>
+// (rank_select.cuh:224-523 semantics).  Search: the sample pair brackets
+// the answer; the L1 block is found by binary search inside the bracket
+// (none when the bracket lies in one block); the L2 block by binary search,
+// or with WideL2 by loading all entries of a bracket of up to 16 blocks at
+// once and counting those below the target (counts before a block are
+// monotone); then the 64 B block is loaded at once and the bit selected
+// in-word.  WideL2 shortens the dependent chain of one select (binary
+// rank/select, one level per query) but issues more instructions, which
+// costs the tree descents and mixed batches more than it saves.
+template <int Value, bool WideL2 = false>
 __device__ __forceinline__ uint64_t level_select(const LevelView& v,
                                                  uint64_t i) {
   const uint64_t total = Value ? v.ones : v.size - v.ones;
@@ -244,24 +250,53 @@ __device__ __forceinline__ uint64_t level_select(const LevelView& v,
   const uint64_t first = blk1 << 7, last = first + kL2PerL1 - 1;
   if (k_lo < first) k_lo = first;
   if (k_hi > last) k_hi = last;
-  while (k_lo < k_hi) {
-    uint64_t mid = (k_lo + k_hi + 1) >> 1;
-    uint64_t c = Value ? v.l2[mid] : ((mid - first) << 9) - v.l2[mid];
-    if (c < r) k_lo = mid; else k_hi = mid - 1;
+  uint64_t blk2;
+  if (WideL2 && k_hi - k_lo < 16) {
+    const uint32_t span = static_cast<uint32_t>(k_hi - k_lo);
+    uint32_t below = 0;
+#pragma unroll
+    for (uint32_t u = 1; u < 16; ++u) {
+      if (u <= span) {
+        const uint64_t k = k_lo + u;
+        const uint64_t c = Value ? v.l2[k] : ((k - first) << 9) - v.l2[k];
+        below += c < r ? 1u : 0u;
+      }
+    }
+    blk2 = k_lo + below;
+  } else {
+    while (k_lo < k_hi) {
+      uint64_t mid = (k_lo + k_hi + 1) >> 1;
+      uint64_t c = Value ? v.l2[mid] : ((mid - first) << 9) - v.l2[mid];
+      if (c < r) k_lo = mid; else k_hi = mid - 1;
+    }
+    blk2 = k_lo;
   }
-  const uint64_t blk2 = k_lo;
   r -= Value ? v.l2[blk2] : ((blk2 - first) << 9) - v.l2[blk2];
-  const uint64_t* w = reinterpret_cast<const uint64_t*>(v.words) + (blk2 << 3);
-  uint64_t pos = blk2 << 9;
-#pragma unroll 1
+  // the 64 B block lies inside the array's 128 B padding
+  const uint4* p = reinterpret_cast<const uint4*>(v.words + (blk2 << 4));
+  const uint4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3];
+  const uint64_t w[8] = {
+      (uint64_t)q0.x | ((uint64_t)q0.y << 32), (uint64_t)q0.z | ((uint64_t)q0.w << 32),
+      (uint64_t)q1.x | ((uint64_t)q1.y << 32), (uint64_t)q1.z | ((uint64_t)q1.w << 32),
+      (uint64_t)q2.x | ((uint64_t)q2.y << 32), (uint64_t)q2.z | ((uint64_t)q2.w << 32),
+      (uint64_t)q3.x | ((uint64_t)q3.y << 32), (uint64_t)q3.z | ((uint64_t)q3.w << 32)};
+  uint64_t pos = blk2 << 9, res = v.size;
+  bool found = false;
+#pragma unroll
   for (uint32_t k = 0; k < 8; ++k) {
-    uint64_t x = Value ? w[k] : ~w[k];
-    uint32_t c = static_cast<uint32_t>(__popcll(x));
-    if (r <= c) return pos + select_in_word(x, static_cast<uint32_t>(r));
-    r -= c;
-    pos += 64;
+    const uint64_t x = Value ? w[k] : ~w[k];
+    const uint32_t c = static_cast<uint32_t>(__popcll(x));
+    if (!found) {
+      if (r <= c) {
+        res = pos + select_in_word(x, static_cast<uint32_t>(r));
+        found = true;
+      } else {
+        r -= c;
+        pos += 64;
+      }
+    }
   }
-  return v.size;  // unreachable for a consistent directory
+  return res;  // size only for an inconsistent directory
 }
 
 // ---------------------------------------------------------------------------
