@@ -318,6 +318,24 @@ class WaveletTree:
                 pass
             self._h = None
 
+    # ---- persistence
+    def save(self, path):
+        """Writes the tree (alphabet, symbol counts, level bits) to a file."""
+        call("ecl_wavelet_tree_save", self._h, str(path).encode())
+
+    @classmethod
+    def load(cls, path, GPU_index=0):
+        """A tree read from a file written by save(); its directory is
+        rebuilt on the device."""
+        self = cls.__new__(cls)
+        self.device = GPU_index
+        h = C.c_void_p()
+        call("ecl_wavelet_tree_load", str(path).encode(), GPU_index, C.byref(h))
+        self._h = h
+        self.dtype = np.dtype({1: np.uint8, 2: np.uint16, 4: np.uint32, 8: np.uint64}[
+            int(self.info().element_size)])
+        return self
+
     # ---- accessors (wavelet_tree.cuh:2165-2282)
     def info(self):
         inf = _lib.TreeInfo()

@@ -5,6 +5,7 @@
 #include <algorithm>
 #include <atomic>
 #include <condition_variable>
+#include <cstdio>
 #include <cstring>
 #include <deque>
 #include <functional>
@@ -755,6 +756,107 @@ uint64_t tree_needed_bytes(uint64_t n, uint64_t sigma, uint32_t L, int esize, bo
 
 static const char* kOomTree = "Not enough memory available for the wavelet tree.";
 
+// Fields that follow from the alphabet (wt->alphabet set, sorted): size,
+// levels, minimal alphabet and symbol map (wavelet_tree.cuh:590-638), codes
+// (createMinimalCodes, :2100-2159); returns the tree shape.
+static TreeShape set_alphabet(ecl_wavelet_tree_t* wt) {
+  require(wt->alphabet.size() >= 2, "the alphabet needs at least 2 symbols");
+  const uint64_t sigma = wt->sigma = wt->alphabet.size();
+  wt->L = detail::ceil_log2(sigma);
+  require(wt->L <= 64, "alphabet too large");
+  wt->pow2 = detail::is_pow2(sigma);
+  wt->min_alphabet = true;
+  for (uint64_t i = 0; i < sigma; ++i)
+    if (wt->alphabet[i] != i) { wt->min_alphabet = false; break; }
+  wt->map_kind = wt->min_alphabet ? 0 : (wt->esize <= 2 ? 1 : 2);
+  TreeShape shape = tree_shape(sigma);
+  wt->codes_start = shape.codes_start;
+  wt->codes.clear();
+  wt->code_lens.clear();
+  for (uint64_t c = wt->codes_start; c < sigma; ++c) {
+    wt->codes.push_back(shape.code[c]);
+    wt->code_lens.push_back(shape.len[c]);
+  }
+  return shape;
+}
+
+// Symbol -> index table (8/16-bit symbols), the alphabet and the codes on
+// the device, through `up` (flushed here).
+static void upload_symbol_maps(ecl_wavelet_tree_t* wt, Uploader& up) {
+  const uint64_t sigma = wt->sigma;
+  const int esize = wt->esize;
+  if (wt->map_kind == 1) {
+    const uint64_t range = esize == 1 ? 256 : 65536;
+    std::vector<uint32_t> lut(range);
+    uint64_t j = 0;
+    for (uint64_t v = 0; v < range; ++v) {  // lower_bound
+      while (j < sigma && wt->alphabet[j] < v) ++j;
+      lut[v] = static_cast<uint32_t>(std::min<uint64_t>(j, sigma));
+    }
+    wt->d_lut = DevBuf(range * 4);
+    up.add(wt->d_lut.ptr, lut.data(), range);
+  }
+  wt->d_alphabet = DevBuf(sigma * esize);
+  {
+    std::vector<uint8_t> raw(sigma * esize);
+    u64_to_elems(wt->alphabet, raw.data(), esize);
+    up.add(wt->d_alphabet.ptr, raw.data(), sigma * esize);
+  }
+  if (!wt->pow2) {
+    wt->d_codes = DevBuf(std::max<size_t>(wt->codes.size(), 1) * 8);
+    up.add(wt->d_codes.ptr, wt->codes.data(), wt->codes.size());
+    wt->d_code_lens = DevBuf(std::max<size_t>(wt->code_lens.size(), 1));
+    up.add(wt->d_code_lens.ptr, wt->code_lens.data(), wt->code_lens.size());
+  }
+  up.flush();
+}
+
+// Host metadata from the symbol counts (C, node ranks, nodes per level),
+// the directory totals and level views, the device copies of the metadata
+// and the tree view; the directory kernels have finished.  Takes the bit
+// array and the directory.
+static void finish_tree(ecl_wavelet_tree_t* wt, const TreeShape& shape,
+                        const std::vector<uint64_t>& sym_count,
+                        std::unique_ptr<ecl_bit_array_t>& ba,
+                        std::unique_ptr<ecl_rank_select_t>& rs, Uploader& up, hipStream_t s) {
+  const uint64_t sigma = wt->sigma;
+  {
+    TreeMeta meta = tree_meta(shape, sym_count, wt->n);
+    wt->counts = std::move(meta.counts);
+    wt->ranks = std::move(meta.ranks);
+    wt->node_start = std::move(meta.node_start);
+    wt->node_level = std::move(meta.node_level);
+    wt->nodes_at_level = std::move(meta.nodes_at_level);
+    wt->nodes_until_last_level = meta.nodes_until_last_level;
+  }
+  finalize_rank_select(rs.get(), ba.get(), up);
+  ba.release();
+  wt->rs = rs.release();
+  // device metadata (on the device when the build returns)
+  wt->d_counts = DevBuf(sigma * 8);
+  up.add(wt->d_counts.ptr, wt->counts.data(), sigma);
+  wt->d_ranks = DevBuf(std::max<size_t>(wt->ranks.size(), 1) * 8);
+  up.add(wt->d_ranks.ptr, wt->ranks.data(), wt->ranks.size());
+  wt->d_nal = DevBuf(std::max<size_t>(wt->nodes_at_level.size(), 1) * 8);
+  up.add(wt->d_nal.ptr, wt->nodes_at_level.data(), wt->nodes_at_level.size());
+  up.flush();
+  ECL_HIP(hipStreamSynchronize(s));
+  TreeView& v = wt->view;
+  v.levels = wt->rs->d_levels.as<LevelView>();
+  v.counts = wt->d_counts.as<uint64_t>();
+  v.ranks = wt->d_ranks.as<uint64_t>();
+  v.nodes_at_level = wt->d_nal.as<uint64_t>();
+  v.codes = wt->d_codes.as<uint64_t>();
+  v.code_lens = wt->d_code_lens.as<uint8_t>();
+  v.n = wt->n;
+  v.sigma = sigma;
+  v.codes_start = wt->codes_start;
+  v.nodes_until_last_level = wt->nodes_until_last_level;
+  v.num_ranks = wt->ranks.size();
+  v.num_levels = wt->L;
+  v.pow2 = wt->pow2 ? 1 : 0;
+}
+
 static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_device,
                                       uint64_t n, const void* alphabet, size_t asize,
                                       int device) {
@@ -805,26 +907,11 @@ static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_
       all.erase(std::unique(all.begin(), all.end()), all.end());
       wt->alphabet = std::move(all);
     }
-    require(wt->alphabet.size() >= 2, "the alphabet needs at least 2 symbols");
-    const uint64_t sigma = wt->sigma = wt->alphabet.size();
-    const uint32_t L = wt->L = detail::ceil_log2(sigma);
-    require(L <= 64, "alphabet too large");
-    wt->pow2 = detail::is_pow2(sigma);
-    wt->min_alphabet = true;
-    for (uint64_t i = 0; i < sigma; ++i)
-      if (wt->alphabet[i] != i) { wt->min_alphabet = false; break; }
-    if (!wt->min_alphabet)
-      wt->map_kind = esize <= 2 ? 1 : 2;
-
-    // ---- codes and nodes
-    const TreeShape shape = tree_shape(sigma);
+    const TreeShape shape = set_alphabet(wt);
+    const uint64_t sigma = wt->sigma;
+    const uint32_t L = wt->L;
     const std::vector<uint64_t>& code = shape.code;
     const std::vector<uint8_t>& len = shape.len;
-    wt->codes_start = shape.codes_start;
-    for (uint64_t c = wt->codes_start; c < sigma; ++c) {
-      wt->codes.push_back(code[c]);
-      wt->code_lens.push_back(len[c]);
-    }
     const BuildPlan plan = plan_of(L);
     const int cw = plan.cw;
     const uint32_t rows = 4;
@@ -857,30 +944,7 @@ static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_
     // counts back after the first scan.
     Uploader up(s);
 
-    // ---- symbol maps
-    if (wt->map_kind == 1) {
-      const uint64_t range = esize == 1 ? 256 : 65536;
-      std::vector<uint32_t> lut(range);
-      uint64_t j = 0;
-      for (uint64_t v = 0; v < range; ++v) {  // lower_bound
-        while (j < sigma && wt->alphabet[j] < v) ++j;
-        lut[v] = static_cast<uint32_t>(std::min<uint64_t>(j, sigma));
-      }
-      wt->d_lut = DevBuf(range * 4);
-      up.add(wt->d_lut.ptr, lut.data(), range);
-    }
-    wt->d_alphabet = DevBuf(sigma * esize);
-    {
-      std::vector<uint8_t> raw(sigma * esize);
-      u64_to_elems(wt->alphabet, raw.data(), esize);
-      up.add(wt->d_alphabet.ptr, raw.data(), sigma * esize);
-    }
-    if (!wt->pow2) {
-      wt->d_codes = DevBuf(std::max<size_t>(wt->codes.size(), 1) * 8);
-      up.add(wt->d_codes.ptr, wt->codes.data(), wt->codes.size());
-      wt->d_code_lens = DevBuf(std::max<size_t>(wt->code_lens.size(), 1));
-      up.add(wt->d_code_lens.ptr, wt->code_lens.data(), wt->code_lens.size());
-    }
+    upload_symbol_maps(wt, up);
     up.flush();
 
     // ---- symbol counts: a multi-pass tree over a non-power-of-2 alphabet
@@ -1171,45 +1235,141 @@ static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_
     buf_u.reset();
     buf_v.reset();
 
-    // ---- counts, node ranks, nodes per level (host metadata)
-    {
-      TreeMeta meta = tree_meta(shape, sym_count, n);
-      wt->counts = std::move(meta.counts);
-      wt->ranks = std::move(meta.ranks);
-      wt->node_start = std::move(meta.node_start);
-      wt->node_level = std::move(meta.node_level);
-      wt->nodes_at_level = std::move(meta.nodes_at_level);
-      wt->nodes_until_last_level = meta.nodes_until_last_level;
+    finish_tree(wt, shape, sym_count, ba_guard, rs_guard, up, s);
+  } catch (...) {
+    delete wt;
+    throw;
+  }
+  return wt;
+}
+
+// ===========================================================================
+// persistence: header, alphabet, symbol counts, level bits (the directory and
+// the metadata are rebuilt on load)
+// ===========================================================================
+namespace {
+struct TreeFileHeader {
+  char magic[8];
+  uint32_t version;
+  uint32_t esize;
+  uint64_t n;
+  uint64_t sigma;
+  uint32_t num_levels;
+  uint32_t reserved;
+};
+constexpr char kTreeMagic[8] = {'E', 'C', 'L', 'W', 'T', 'R', 'E', 'E'};
+constexpr uint32_t kTreeFileVersion = 1;
+constexpr size_t kFileChunk = 64u << 20;
+
+struct File {
+  std::FILE* f = nullptr;
+  File(const char* path, const char* mode) : f(std::fopen(path, mode)) {}
+  ~File() { if (f) std::fclose(f); }
+};
+void write_all(std::FILE* f, const void* p, size_t bytes) {
+  if (bytes && std::fwrite(p, 1, bytes, f) != bytes)
+    throw Error(ECL_ERROR_INVALID_ARGUMENT, "writing the tree file failed");
+}
+void read_all(std::FILE* f, void* p, size_t bytes) {
+  if (bytes && std::fread(p, 1, bytes, f) != bytes)
+    throw Error(ECL_ERROR_INVALID_ARGUMENT, "the tree file is truncated");
+}
+}  // namespace
+
+static void save_tree(const ecl_wavelet_tree_t* wt, const char* path) {
+  require(path != nullptr, "path is NULL");
+  set_device(wt->device);
+  File file(path, "wb");
+  if (!file.f) throw Error(ECL_ERROR_INVALID_ARGUMENT, std::string("cannot create ") + path);
+  TreeFileHeader h{};
+  std::memcpy(h.magic, kTreeMagic, 8);
+  h.version = kTreeFileVersion;
+  h.esize = static_cast<uint32_t>(wt->esize);
+  h.n = wt->n;
+  h.sigma = wt->sigma;
+  h.num_levels = wt->L;
+  write_all(file.f, &h, sizeof(h));
+  std::vector<uint8_t> raw(wt->sigma * wt->esize);
+  u64_to_elems(wt->alphabet, raw.data(), wt->esize);
+  write_all(file.f, raw.data(), raw.size());
+  std::vector<uint64_t> cnt(wt->sigma);
+  for (uint64_t c = 0; c < wt->sigma; ++c)
+    cnt[c] = (c + 1 < wt->sigma ? wt->counts[c + 1] : wt->n) - wt->counts[c];
+  write_all(file.f, cnt.data(), cnt.size() * 8);
+  const ecl_bit_array_t* ba = wt->rs->ba;
+  std::vector<uint32_t> buf(kFileChunk / 4);
+  for (uint32_t l = 0; l < wt->L; ++l) {
+    const uint64_t m = ba->sizes[l], nw = (m + 31) / 32;
+    write_all(file.f, &m, 8);
+    for (uint64_t w = 0; w < nw; w += buf.size()) {
+      const uint64_t c = std::min<uint64_t>(buf.size(), nw - w);
+      d2h(buf.data(), ba->words() + ba->offsets[l] + w, c);
+      write_all(file.f, buf.data(), c * 4);
     }
+  }
+  if (std::fflush(file.f) != 0) throw Error(ECL_ERROR_INVALID_ARGUMENT, "writing the tree file failed");
+}
 
-    finalize_rank_select(rs, ba, up);
-    ba_guard.release();
-    wt->rs = rs_guard.release();
-
-    // ---- device metadata and view (the tables are on the device when the
-    // build returns)
-    wt->d_counts = DevBuf(sigma * 8);
-    up.add(wt->d_counts.ptr, wt->counts.data(), sigma);
-    wt->d_ranks = DevBuf(std::max<size_t>(wt->ranks.size(), 1) * 8);
-    up.add(wt->d_ranks.ptr, wt->ranks.data(), wt->ranks.size());
-    wt->d_nal = DevBuf(std::max<size_t>(wt->nodes_at_level.size(), 1) * 8);
-    up.add(wt->d_nal.ptr, wt->nodes_at_level.data(), wt->nodes_at_level.size());
+static ecl_wavelet_tree_t* load_tree(const char* path, int device) {
+  require(path != nullptr, "path is NULL");
+  File file(path, "rb");
+  if (!file.f) throw Error(ECL_ERROR_INVALID_ARGUMENT, std::string("cannot open ") + path);
+  TreeFileHeader h{};
+  read_all(file.f, &h, sizeof(h));
+  if (std::memcmp(h.magic, kTreeMagic, 8) != 0 || h.version != kTreeFileVersion)
+    throw Error(ECL_ERROR_INVALID_ARGUMENT, "not a wavelet-tree file of this version");
+  require(h.esize == 1 || h.esize == 2 || h.esize == 4 || h.esize == 8, "invalid element size");
+  require(h.n > 0 && h.sigma >= 2 && h.sigma <= h.n + (1ull << 32), "invalid tree sizes");
+  set_device(device);
+  hipStream_t s = nullptr;
+  auto* wt = new ecl_wavelet_tree_t();
+  try {
+    wt->device = device;
+    wt->esize = static_cast<int>(h.esize);
+    wt->n = h.n;
+    std::vector<uint8_t> raw(h.sigma * h.esize);
+    read_all(file.f, raw.data(), raw.size());
+    wt->alphabet = elems_to_u64(raw.data(), h.sigma, wt->esize);
+    require(std::is_sorted(wt->alphabet.begin(), wt->alphabet.end()) &&
+                std::adjacent_find(wt->alphabet.begin(), wt->alphabet.end()) ==
+                    wt->alphabet.end(),
+            "the file's alphabet is not sorted and distinct");
+    const TreeShape shape = set_alphabet(wt);
+    require(wt->L == h.num_levels, "the file's level count does not match its alphabet");
+    std::vector<uint64_t> sym_count(h.sigma);
+    read_all(file.f, sym_count.data(), h.sigma * 8);
+    uint64_t tot = 0;
+    for (uint64_t c : sym_count) tot += c;
+    require(tot == h.n, "the file's symbol counts do not sum to its size");
+    const std::vector<uint64_t> level_size = level_sizes(shape, sym_count);
+    if (tree_needed_bytes(h.n, h.sigma, wt->L, wt->esize, false) >
+        free_device_memory() + cached_scratch_bytes(device))
+      throw Error(ECL_ERROR_OUT_OF_MEMORY_TREE, kOomTree);
+    Uploader up(s);
+    upload_symbol_maps(wt, up);
+    std::unique_ptr<ecl_bit_array_t> ba(make_bit_array(level_size, -1, device, &up));
     up.flush();
-    ECL_HIP(hipStreamSynchronize(s));
-    TreeView& v = wt->view;
-    v.levels = wt->rs->d_levels.as<LevelView>();
-    v.counts = wt->d_counts.as<uint64_t>();
-    v.ranks = wt->d_ranks.as<uint64_t>();
-    v.nodes_at_level = wt->d_nal.as<uint64_t>();
-    v.codes = wt->d_codes.as<uint64_t>();
-    v.code_lens = wt->d_code_lens.as<uint8_t>();
-    v.n = n;
-    v.sigma = sigma;
-    v.codes_start = wt->codes_start;
-    v.nodes_until_last_level = wt->nodes_until_last_level;
-    v.num_ranks = wt->ranks.size();
-    v.num_levels = L;
-    v.pow2 = wt->pow2 ? 1 : 0;
+    std::vector<uint32_t> buf(kFileChunk / 4);
+    for (uint32_t l = 0; l < wt->L; ++l) {
+      uint64_t m = 0;
+      read_all(file.f, &m, 8);
+      require(m == level_size[l], "the file's level sizes do not match its symbol counts");
+      const uint64_t nw = (m + 31) / 32;
+      for (uint64_t w = 0; w < nw; w += buf.size()) {
+        const uint64_t c = std::min<uint64_t>(buf.size(), nw - w);
+        read_all(file.f, buf.data(), c * 4);
+        h2d(ba->words() + ba->offsets[l] + w, buf.data(), c);
+      }
+    }
+    if (std::fgetc(file.f) != EOF)
+      throw Error(ECL_ERROR_INVALID_ARGUMENT, "the tree file has trailing bytes");
+    std::unique_ptr<ecl_rank_select_t> rs(alloc_rank_select(ba.get(), s));
+    {
+      Scratch scr(device, s, directory_scratch_bytes(level_size));
+      build_directories(rs.get(), ba.get(), s, scr, up);
+      ECL_HIP(hipStreamSynchronize(s));
+    }
+    finish_tree(wt, shape, sym_count, ba, rs, up, s);
   } catch (...) {
     delete wt;
     throw;
@@ -1655,6 +1815,18 @@ ecl_status ecl_wavelet_tree_destroy(ecl_wavelet_tree wt) {
       set_device(wt->device);
       delete wt;
     }
+  });
+}
+ecl_status ecl_wavelet_tree_save(ecl_wavelet_tree wt, const char* path) {
+  return guard([&] {
+    require(wt != nullptr, "NULL tree");
+    save_tree(wt, path);
+  });
+}
+ecl_status ecl_wavelet_tree_load(const char* path, int device, ecl_wavelet_tree* out) {
+  return guard([&] {
+    require(out != nullptr, "out is NULL");
+    *out = load_tree(path, device);
   });
 }
 ecl_status ecl_wavelet_tree_info_get(ecl_wavelet_tree wt, ecl_wavelet_tree_info* o) {

@@ -230,6 +230,14 @@ ecl_status ecl_wavelet_tree_create_device(ecl_type type, const void* d_text, siz
                                           int device, ecl_wavelet_tree* out);
 /* ~WaveletTree (wavelet_tree.cuh:931-943) */
 ecl_status ecl_wavelet_tree_destroy(ecl_wavelet_tree wt);
+/* Persistence (no counterpart in the reference, which rebuilds every tree
+ * from its text; for warm starts of servers).  The file holds a header, the
+ * alphabet, the symbol counts and every level's bits; loading checks them
+ * for consistency, rebuilds the rank/select directory on the device and
+ * recomputes the metadata, so a loaded tree answers exactly as the saved
+ * one.  Files are little-endian and not portable across format versions. */
+ecl_status ecl_wavelet_tree_save(ecl_wavelet_tree wt, const char* path);
+ecl_status ecl_wavelet_tree_load(const char* path, int device, ecl_wavelet_tree* out);
 ecl_status ecl_wavelet_tree_info_get(ecl_wavelet_tree wt, ecl_wavelet_tree_info* out);
 /* getAlphabet (:2173): alphabet_size elements of type T */
 ecl_status ecl_wavelet_tree_alphabet(ecl_wavelet_tree wt, void* out);

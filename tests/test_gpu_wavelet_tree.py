@@ -389,3 +389,71 @@ def test_host_text_staged_upload_and_chunked_host_queries(gpu):
         assert np.array_equal(host, d_out.download(np.uint64, m))
     idx = rng.integers(0, n, m).astype(np.uint64)
     assert np.array_equal(wt_h.access(idx, flags=2), text[idx])
+
+
+@pytest.mark.parametrize("case", ["u8_256", "u8_20", "u16_1000", "u32_alpha", "u8_zipf"])
+def test_save_load_round_trip(gpu, tmp_path, case):
+    """A tree written to a file and read back has the same levels,
+    directory, counts, node ranks and codes, and answers every query as the
+    original (multi-pass and non-minimal alphabets included)."""
+    from ecl_wavelet_amd import WaveletTree
+    rng = np.random.default_rng(hash(case) % 1000)
+    if case == "u8_256":
+        text = rng.integers(0, 256, 2_000_000).astype(np.uint8)
+        alpha = np.arange(256, dtype=np.uint8)
+    elif case == "u8_20":
+        text = rng.integers(0, 20, 700_000).astype(np.uint8)
+        alpha = np.arange(20, dtype=np.uint8)
+    elif case == "u16_1000":
+        text = rng.integers(0, 1000, 600_000).astype(np.uint16)
+        alpha = np.arange(1000, dtype=np.uint16)
+    elif case == "u32_alpha":
+        alpha = np.sort(rng.choice(1 << 31, 37, replace=False)).astype(np.uint32)
+        text = alpha[rng.integers(0, 37, 300_000)]
+    else:
+        text = O.gen_cdf(5, 800_000, O.zipf_thresholds(200, 1.2), np.uint8)
+        alpha = np.arange(200, dtype=np.uint8)
+    wt = WaveletTree(text, alpha)
+    path = tmp_path / "tree.ecl"
+    wt.save(path)
+    lt = WaveletTree.load(path)
+    assert lt.dtype == wt.dtype
+    assert lt.getNumLevels() == wt.getNumLevels()
+    assert np.array_equal(lt.getAlphabet(), wt.getAlphabet())
+    assert np.array_equal(lt.getCounts(), wt.getCounts())
+    r0, s0, l0 = wt.nodeRanks()
+    r1, s1, l1 = lt.nodeRanks()
+    assert np.array_equal(r0, r1) and np.array_equal(s0, s1) and np.array_equal(l0, l1)
+    rs0, rs1 = wt.rank_select_, lt.rank_select_
+    for l in range(wt.getNumLevels()):
+        assert np.array_equal(rs0.bit_array_.words(l), rs1.bit_array_.words(l))
+        assert np.array_equal(rs0.l1Entries(l), rs1.l1Entries(l))
+        assert np.array_equal(rs0.l2Entries(l), rs1.l2Entries(l))
+        for v in (0, 1):
+            assert np.array_equal(rs0.selectSamples(l, v), rs1.selectSamples(l, v))
+    _check_queries(lt, text, None, nq=2000, seed=9)
+
+
+def test_load_rejects_damaged_files(gpu, tmp_path):
+    from ecl_wavelet_amd import WaveletTree
+    from ecl_wavelet_amd._lib import EclError
+    text = np.arange(100_000, dtype=np.uint8) % 7
+    wt = WaveletTree(text, np.arange(7, dtype=np.uint8))
+    path = tmp_path / "t.ecl"
+    wt.save(path)
+    data = path.read_bytes()
+    cases = {
+        "truncated": data[:-5],
+        "trailing": data + b"x",
+        "magic": b"X" + data[1:],
+        "counts": data[:40 + 7] + bytes([data[47] ^ 1]) + data[48:],  # first symbol count
+    }
+    for name, blob in cases.items():
+        p = tmp_path / f"{name}.ecl"
+        p.write_bytes(blob)
+        with pytest.raises(EclError):
+            WaveletTree.load(p)
+    with pytest.raises(EclError):
+        WaveletTree.load(tmp_path / "missing.ecl")
+    lt = WaveletTree.load(path)  # the intact file still loads
+    _check_queries(lt, text, None, nq=500, seed=1)
