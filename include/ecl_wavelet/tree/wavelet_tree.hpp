@@ -20,6 +20,8 @@
 #include <limits>
 #include <numeric>
 #include <span>
+#include <stdexcept>
+#include <string>
 #include <vector>
 
 #include "ecl_wavelet/bitarray/rank_select.hpp"
@@ -75,6 +77,26 @@ class WaveletTree {
     host_ = new Host();
     host_->gpu = GPU_index;
     bind();
+  }
+  // A tree read from a file written by save() (no counterpart in the
+  // reference); its element size must be sizeof(T).
+  struct FromFile {};
+  __host__ WaveletTree(FromFile, std::string const& path, uint32_t const GPU_index) {
+    utils::hipThrow(hipSetDevice(static_cast<int>(GPU_index)), "hipSetDevice");
+    utils::throwOnError(ecl_wavelet_tree_load(path.c_str(), static_cast<int>(GPU_index),
+                                              &handle_));
+    is_copy_ = false;
+    host_ = new Host();
+    host_->gpu = GPU_index;
+    bind();
+    if (info_.element_size != sizeof(T)) {
+      release();
+      throw std::runtime_error("the tree file holds another element type");
+    }
+  }
+  // writes the tree (alphabet, symbol counts, level bits) to a file
+  __host__ void save(std::string const& path) const {
+    utils::throwOnError(ecl_wavelet_tree_save(handle_, path.c_str()));
   }
   // copy = shallow view (wavelet_tree.cuh:870-887)
   __host__ WaveletTree(WaveletTree const& o)

@@ -13,6 +13,9 @@
 #include <string>
 #include <vector>
 
+#include <stdlib.h>
+#include <unistd.h>
+
 #include "ecl_wavelet/bitarray/rank_select.cuh"
 #include "ecl_wavelet/tree/wavelet_tree.cuh"
 
@@ -410,6 +413,43 @@ static void example_dna() {
   std::printf("example: ok so far (%d failures)\n", g_fail);
 }
 
+static void test_save_load() {
+  // persistence: the loaded tree answers like the saved one
+  std::vector<uint16_t> text(200000);
+  std::mt19937 gen(7);
+  for (auto& x : text) x = static_cast<uint16_t>(gen() % 1000);
+  std::vector<uint16_t> alpha(1000);
+  std::iota(alpha.begin(), alpha.end(), 0);
+  WaveletTree<uint16_t> wt(text.data(), text.size(), std::move(alpha), 0);
+  char path[] = "/tmp/ecl_dropin_treeXXXXXX";
+  const int fd = mkstemp(path);
+  EXPECT_EQ(fd >= 0, true);
+  if (fd >= 0) close(fd);
+  wt.save(path);
+  WaveletTree<uint16_t> lt(WaveletTree<uint16_t>::FromFile{}, path, 0);
+  std::remove(path);
+  std::vector<RankSelectQuery<uint16_t>> q;
+  for (size_t i = 0; i < 3000; ++i)
+    q.push_back({static_cast<size_t>(gen() % text.size()), static_cast<uint16_t>(gen() % 1000)});
+  auto q2 = q;
+  auto a = wt.rank(q.data(), q.size());
+  std::vector<size_t> ra(a.begin(), a.end());
+  auto b = lt.rank(q2.data(), q2.size());
+  for (size_t i = 0; i < q.size(); ++i) EXPECT_EQ(ra[i], b[i]);
+  std::vector<size_t> idx(2000);
+  for (auto& i : idx) i = gen() % text.size();
+  auto acc = lt.access(idx.data(), idx.size());
+  for (size_t i = 0; i < idx.size(); ++i) EXPECT_EQ(acc[i], text[idx[i]]);
+  std::string msg;
+  try {
+    WaveletTree<uint8_t> bad(WaveletTree<uint8_t>::FromFile{}, "/nonexistent/tree", 0);
+  } catch (std::runtime_error const& e) {
+    msg = e.what();
+  }
+  EXPECT_EQ(msg.empty(), false);
+  std::printf("save/load: ok so far (%d failures)\n", g_fail);
+}
+
 int main() {
   gpuErrchk(hipMallocManaged(&g_res, sizeof(size_t)));
   try {
@@ -422,6 +462,7 @@ int main() {
     test_codes_and_nodes();
     test_oom();
     example_dna();
+    test_save_load();
   } catch (std::exception const& e) {
     std::printf("EXCEPTION: %s\n", e.what());
     return 2;
