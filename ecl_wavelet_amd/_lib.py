@@ -143,6 +143,7 @@ _SIGS = {
     "ecl_server_create": (st, [i32, vp, C.POINTER(vp)]),
     "ecl_server_destroy": (st, [vp]),
     "ecl_server_add_tree": (st, [vp, vp, i32, sz, u64, C.POINTER(u32)]),
+    "ecl_server_add_tree_file": (st, [vp, C.c_char_p, C.POINTER(C.c_uint32)]),
     "ecl_server_remove_tree": (st, [vp, u32]),
     "ecl_server_submit": (st, [vp, u32, vp, sz, vp, C.POINTER(u64)]),
     "ecl_server_wait": (st, [vp, u64, C.POINTER(C.c_double)]),

@@ -564,6 +564,13 @@ class Server:
             call("ecl_server_add_tree", self._h, _p(text), 0, text.size, sigma, C.byref(t))
         return t.value
 
+    def addTreeFile(self, path):
+        """Admits a uint8 tree over 0..sigma-1 saved by WaveletTree.save,
+        without rebuilding it."""
+        t = C.c_uint32()
+        call("ecl_server_add_tree_file", self._h, str(path).encode(), C.byref(t))
+        return t.value
+
     def removeTree(self, tree):
         call("ecl_server_remove_tree", self._h, tree)
 

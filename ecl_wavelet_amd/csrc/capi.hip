@@ -1276,6 +1276,23 @@ void read_all(std::FILE* f, void* p, size_t bytes) {
 }
 }  // namespace
 
+namespace ecl {
+namespace lib {
+void read_tree_file_header(const char* path, int* esize, uint64_t* n, uint64_t* sigma) {
+  require(path != nullptr, "path is NULL");
+  File file(path, "rb");
+  if (!file.f) throw Error(ECL_ERROR_INVALID_ARGUMENT, std::string("cannot open ") + path);
+  TreeFileHeader h{};
+  read_all(file.f, &h, sizeof(h));
+  if (std::memcmp(h.magic, kTreeMagic, 8) != 0 || h.version != kTreeFileVersion)
+    throw Error(ECL_ERROR_INVALID_ARGUMENT, "not a wavelet-tree file of this version");
+  *esize = static_cast<int>(h.esize);
+  *n = h.n;
+  *sigma = h.sigma;
+}
+}  // namespace lib
+}  // namespace ecl
+
 static void save_tree(const ecl_wavelet_tree_t* wt, const char* path) {
   require(path != nullptr, "path is NULL");
   set_device(wt->device);

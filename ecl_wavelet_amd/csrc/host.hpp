@@ -113,6 +113,10 @@ void parallel_memcpy(void* dst, const void* src, size_t bytes);
 // transfers; returns after the data is on the device.
 void upload_staged(void* d_dst, const void* src, size_t bytes, hipStream_t s);
 
+// Element size, size and alphabet size of a tree file written by
+// ecl_wavelet_tree_save (its header only; throws for other files).
+void read_tree_file_header(const char* path, int* esize, uint64_t* n, uint64_t* sigma);
+
 // ---- tree shape: minimal codes and the internal nodes of every level
 struct Node { uint64_t lo, hi; };
 struct TreeShape {

@@ -333,6 +333,57 @@ ecl_status ecl_server_destroy(ecl_server s) {
   });
 }
 
+}  // extern "C"
+
+// Bits, directory and metadata of a uint8 tree of n symbols over sigma.
+static uint64_t tree_estimate(uint64_t n, uint64_t sigma) {
+  const uint64_t L = ecl::detail::ceil_log2(sigma);
+  return L * ((n + 1023) / 1024 * 128 + (n + 511) / 512 * 2 + (n + 65535) / 65536 * 8 +
+              n / 4096 * 8 + 1024) +
+         sigma * 64 + 4096;
+}
+
+// Admission: reserves a slot and `estimate` bytes of the budget.
+static uint32_t reserve_slot(ecl_server s, uint64_t estimate) {
+  std::lock_guard<std::mutex> lk(s->mu);
+  if (s->hbm_used + estimate > s->cfg.hbm_budget)
+    throw Error(ECL_ERROR_OUT_OF_MEMORY_TREE, "Not enough memory available for the wavelet tree.");
+  uint32_t slot = 0;
+  while (slot < s->trees.size() && (s->trees[slot].wt || s->trees[slot].active)) ++slot;
+  require(slot < s->trees.size(), "no free tree slot");
+  s->trees[slot].active = false;
+  s->trees[slot].bytes = estimate;
+  s->trees[slot].wt = reinterpret_cast<ecl_wavelet_tree>(1);  // reserved while building
+  s->hbm_used += estimate;
+  return slot;
+}
+
+static void release_slot(ecl_server s, uint32_t slot, uint64_t estimate) {
+  std::lock_guard<std::mutex> lk(s->mu);
+  s->hbm_used -= estimate;
+  s->trees[slot] = TreeEntry{};
+}
+
+// Publishes a built tree in its slot: its view goes to the device table and
+// the budget is charged what the build actually took.
+static void commit_slot(ecl_server s, uint32_t slot, ecl_wavelet_tree wt, uint64_t estimate,
+                        size_t fr0, size_t fr1) {
+  TreeView v{};
+  (void)ecl_wavelet_tree_device_view(wt, &v, sizeof(v));
+  ECL_HIP(hipMemcpyAsync(s->d_views.as<TreeView>() + slot, &v, sizeof(v), hipMemcpyHostToDevice,
+                         s->ctl));
+  ECL_HIP(hipStreamSynchronize(s->ctl));
+  std::lock_guard<std::mutex> lk(s->mu);
+  TreeEntry& e = s->trees[slot];
+  const uint64_t used = fr0 > fr1 ? fr0 - fr1 : estimate;
+  s->hbm_used = s->hbm_used - estimate + used;
+  e.wt = wt;
+  e.bytes = used;
+  e.active = true;
+}
+
+extern "C" {
+
 ecl_status ecl_server_add_tree(ecl_server s, const void* text, int text_on_device, size_t n,
                                uint64_t sigma, uint32_t* tree) {
   return guard([&] {
@@ -341,24 +392,8 @@ ecl_status ecl_server_add_tree(ecl_server s, const void* text, int text_on_devic
     require(n > 0, "empty text");
     set_device(s->device);
     std::lock_guard<std::mutex> build_lk(s->build_mu);
-    // admission: bits and directory of every level plus metadata
-    const uint64_t L = ecl::detail::ceil_log2(sigma);
-    const uint64_t estimate = L * ((n + 1023) / 1024 * 128 + (n + 511) / 512 * 2 +
-                                   (n + 65535) / 65536 * 8 + n / 4096 * 8 + 1024) +
-                              sigma * 64 + 4096;
-    uint32_t slot = 0;
-    {
-      std::lock_guard<std::mutex> lk(s->mu);
-      if (s->hbm_used + estimate > s->cfg.hbm_budget)
-        throw Error(ECL_ERROR_OUT_OF_MEMORY_TREE,
-                    "Not enough memory available for the wavelet tree.");
-      while (slot < s->trees.size() && (s->trees[slot].wt || s->trees[slot].active)) ++slot;
-      require(slot < s->trees.size(), "no free tree slot");
-      s->trees[slot].active = false;
-      s->trees[slot].bytes = estimate;
-      s->trees[slot].wt = reinterpret_cast<ecl_wavelet_tree>(1);  // reserved while building
-      s->hbm_used += estimate;
-    }
+    const uint64_t estimate = tree_estimate(n, sigma);
+    const uint32_t slot = reserve_slot(s, estimate);
     std::vector<uint8_t> alpha(sigma);
     for (uint64_t c = 0; c < sigma; ++c) alpha[c] = static_cast<uint8_t>(c);
     ecl_wavelet_tree wt = nullptr;
@@ -372,23 +407,46 @@ ecl_status ecl_server_add_tree(ecl_server s, const void* text, int text_on_devic
     (void)hipMemGetInfo(&fr1, &tot);
     if (st != ECL_OK) {
       const std::string msg = last_error();
-      std::lock_guard<std::mutex> lk(s->mu);
-      s->hbm_used -= estimate;
-      s->trees[slot] = TreeEntry{};
+      release_slot(s, slot, estimate);
       throw Error(st, msg);
     }
-    TreeView v{};
-    (void)ecl_wavelet_tree_device_view(wt, &v, sizeof(v));
-    ECL_HIP(hipMemcpyAsync(s->d_views.as<TreeView>() + slot, &v, sizeof(v), hipMemcpyHostToDevice,
-                           s->ctl));
-    ECL_HIP(hipStreamSynchronize(s->ctl));
-    std::lock_guard<std::mutex> lk(s->mu);
-    TreeEntry& e = s->trees[slot];
-    const uint64_t used = fr0 > fr1 ? fr0 - fr1 : estimate;
-    s->hbm_used = s->hbm_used - estimate + used;
-    e.wt = wt;
-    e.bytes = used;
-    e.active = true;
+    commit_slot(s, slot, wt, estimate, fr0, fr1);
+    *tree = slot;
+  });
+}
+
+ecl_status ecl_server_add_tree_file(ecl_server s, const char* path, uint32_t* tree) {
+  return guard([&] {
+    require(s && path && tree, "NULL argument");
+    int esize = 0;
+    uint64_t n = 0, sigma = 0;
+    read_tree_file_header(path, &esize, &n, &sigma);
+    require(esize == 1 && sigma >= 2 && sigma <= 256,
+            "the server holds uint8 trees with alphabets of 2 to 256 symbols");
+    set_device(s->device);
+    std::lock_guard<std::mutex> build_lk(s->build_mu);
+    const uint64_t estimate = tree_estimate(n, sigma);
+    const uint32_t slot = reserve_slot(s, estimate);
+    ecl_wavelet_tree wt = nullptr;
+    size_t fr0 = 0, fr1 = 0, tot = 0;
+    (void)hipMemGetInfo(&fr0, &tot);
+    ecl_status st = ecl_wavelet_tree_load(path, s->device, &wt);
+    (void)hipMemGetInfo(&fr1, &tot);
+    std::string msg = st != ECL_OK ? last_error() : std::string();
+    if (st == ECL_OK) {
+      ecl_wavelet_tree_info info{};
+      (void)ecl_wavelet_tree_info_get(wt, &info);
+      if (!info.is_min_alphabet) {
+        (void)ecl_wavelet_tree_destroy(wt);
+        st = ECL_ERROR_INVALID_ARGUMENT;
+        msg = "the server holds trees over the alphabet 0 .. sigma-1";
+      }
+    }
+    if (st != ECL_OK) {
+      release_slot(s, slot, estimate);
+      throw Error(st, msg);
+    }
+    commit_slot(s, slot, wt, estimate, fr0, fr1);
     *tree = slot;
   });
 }

@@ -375,6 +375,10 @@ ecl_status ecl_server_destroy(ecl_server s);
  * exceed the budget. */
 ecl_status ecl_server_add_tree(ecl_server s, const void* text, int text_on_device, size_t n,
                                uint64_t sigma, uint32_t* tree);
+/* Admits a tree read from a file written by ecl_wavelet_tree_save (a uint8
+ * tree over 0 .. sigma-1), without rebuilding it: the warm start of a
+ * server.  Same budget rule and error as ecl_server_add_tree. */
+ecl_status ecl_server_add_tree_file(ecl_server s, const char* path, uint32_t* tree);
 /* No new requests for the tree; its memory is freed once its requests finish. */
 ecl_status ecl_server_remove_tree(ecl_server s, uint32_t tree);
 /* Queues a request; `queries` must stay valid and `results` (m entries)

@@ -130,3 +130,34 @@ def test_server_device_text_and_load(gpu):
     assert st["requests"] > 0 and st["queries"] > 0
     assert st["p99_ms"] >= st["p50_ms"] > 0
     srv.close()
+
+
+def test_server_warm_start_from_files(gpu, tmp_path):
+    """Trees saved to files are admitted without rebuilding and answer as
+    built from their texts; files the server cannot hold are rejected and
+    leave the budget untouched."""
+    from ecl_wavelet_amd import Server, WaveletTree
+    from ecl_wavelet_amd._lib import EclError
+    texts = [(_text(21, 300_000, 256), 256), (_text(22, 90_000, 4), 4)]
+    paths = []
+    for j, (t, sg) in enumerate(texts):
+        p = tmp_path / f"t{j}.ecl"
+        WaveletTree(t, np.arange(sg, dtype=np.uint8)).save(p)
+        paths.append(p)
+    bad_alpha = tmp_path / "alpha.ecl"
+    WaveletTree(np.array([1, 3, 5, 3] * 100, dtype=np.uint8),
+                np.array([1, 3, 5], dtype=np.uint8)).save(bad_alpha)
+    bad_type = tmp_path / "u16.ecl"
+    WaveletTree(np.arange(1000, dtype=np.uint16) % 300, np.arange(300, dtype=np.uint16)).save(bad_type)
+    srv = Server()
+    trees = [srv.addTreeFile(p) for p in paths]
+    used = srv.stats()["hbm_used"]
+    for p in (bad_alpha, bad_type, tmp_path / "missing.ecl"):
+        with pytest.raises(EclError):
+            srv.addTreeFile(p)
+    assert srv.stats()["hbm_used"] == used
+    for tree, (t, sg) in zip(trees, texts):
+        (q, e), = _requests(t, sg, [3000], seed=31)
+        out, _ = srv.wait(srv.submit(tree, q))
+        assert np.array_equal(out, e)
+    srv.close()
