@@ -60,6 +60,9 @@ def parse():
                          "(n=6e9) and u16 sigma 2^16 Zipf(1.1) (n=2e9); 0 skips")
     ap.add_argument("--host-queries", type=int, default=100_000_000,
                     help="host-to-host section: queries per kind from host memory (0 skips)")
+    ap.add_argument("--pmc-queries", action="store_true",
+                    help="counter workload of the queries: the 1 GiB copy calibration, one "
+                         "build, then the bench's access, rank and select batches once each")
     ap.add_argument("--pmc-workload", action="store_true",
                     help="short run for hardware-counter passes: one 1 GiB "
                          "device-to-device copy (the calibration kernel) and "
@@ -486,10 +489,44 @@ def pmc_workload(args):
     log(f"built the tree twice (n={args.n}, sigma={args.sigma})")
 
 
+def pmc_queries(args):
+    """Kernels for the query counter passes: the copy calibration, then one
+    build and 10^7 device-generated queries per kind over the benchmark
+    tree (each batch launched once)."""
+    import ctypes as C
+    import ecl_wavelet_amd as E
+    hip = C.CDLL("libamdhip64.so")
+    nbytes = 1 << 30
+    a, b = E.DeviceBuffer(nbytes), E.DeviceBuffer(nbytes)
+    rc = hip.hipMemcpy(b.ptr, a.ptr, C.c_size_t(nbytes), 3)
+    if rc != 0:
+        raise SystemExit(f"hipMemcpy failed: {rc}")
+    E.synchronize()
+    a.free()
+    b.free()
+    d_text = E.generate_text(args.n, args.sigma, np.uint8, seed=4)
+    wt = E.WaveletTree(d_text, np.arange(args.sigma, dtype=np.uint8), dtype=np.uint8, n=args.n)
+    Q = args.queries
+    d_idx, d_rq, d_sq = E.DeviceBuffer(Q * 8), E.DeviceBuffer(Q * 16), E.DeviceBuffer(Q * 16)
+    d_out = E.DeviceBuffer(Q * 8)
+    wt.generateQueries(0, 100, Q, d_idx)
+    wt.generateQueries(1, 200, Q, d_rq)
+    wt.generateQueries(2, 300, Q, d_sq)
+    E.synchronize()
+    wt.accessDevice(d_idx, Q, d_out, flags=2)
+    wt.rankDevice(d_rq, Q, d_out, flags=1)
+    wt.selectDevice(d_sq, Q, d_out, flags=1)
+    E.synchronize()
+    log(f"queries: {Q} access, rank and select over n={args.n}, sigma={args.sigma}")
+
+
 def main():
     args = parse()
     if args.pmc_workload:
         pmc_workload(args)
+        return
+    if args.pmc_queries:
+        pmc_queries(args)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU per process; "

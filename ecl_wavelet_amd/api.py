@@ -265,6 +265,11 @@ class RankSelect:
         call("ecl_rank_select_select", self._h, i, int(value), _p(k), len(k), _p(out))
         return out
 
+    def buildQueryLines(self):
+        """Second copy of the bits in 128-byte lines headed by their rank:
+        a rank then reads one line."""
+        call("ecl_rank_select_build_query_lines", self._h)
+
     def rankDevice(self, i, d_index, m, d_out, value=1, stream=None):
         call("ecl_rank_select_rank_device", self._h, i, int(value), d_index.ptr, m,
              d_out.ptr, stream)
@@ -317,6 +322,11 @@ class WaveletTree:
             except Exception:  # noqa: BLE001
                 pass
             self._h = None
+
+    def buildQueryLines(self):
+        """Second copy of the level bits in 128-byte lines headed by their
+        rank: rank and access then read one line per level."""
+        call("ecl_wavelet_tree_build_query_lines", self._h)
 
     # ---- persistence
     def save(self, path):

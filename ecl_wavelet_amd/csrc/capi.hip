@@ -344,6 +344,7 @@ struct ecl_rank_select_t {
   ecl_bit_array_t* ba = nullptr;  // owned
   std::vector<uint64_t> num_l1, num_l2, num_last_l2, l1_off, l2_off, s_off, cap, ones;
   DevBuf l1, l2, samples, total_ones;
+  DevBuf lines;  // optional query lines of every array
   DevBuf d_l1_off, d_l2_off, d_num_l1, d_num_last_l2, d_s_off, d_cap, d_levels;
   std::vector<LevelView> levels;
   uint64_t total_l2 = 0;
@@ -1243,6 +1244,33 @@ static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_
   return wt;
 }
 
+// Query lines for every array of a directory (an opt-in copy of the bits,
+// 128/120 of their size, that lets a rank read one line); the level views
+// point to them afterwards.
+static void build_query_lines(ecl_rank_select_t* rs) {
+  if (rs->lines.ptr != nullptr) return;
+  const size_t na = rs->levels.size();
+  std::vector<uint64_t> off(na), nl(na);
+  uint64_t total = 0;
+  for (size_t i = 0; i < na; ++i) {
+    nl[i] = (rs->levels[i].size + detail::kLineBits - 1) / detail::kLineBits;
+    off[i] = total;
+    total += nl[i];
+  }
+  if (total * 128 + (64u << 20) > free_device_memory())
+    throw Error(ECL_ERROR_OUT_OF_MEMORY_TREE, kOomTree);
+  DevBuf lines(std::max<uint64_t>(total, 1) * 128);
+  hipStream_t s = nullptr;
+  for (size_t i = 0; i < na; ++i)
+    launch_query_lines(rs->levels[i], lines.as<uint4>() + off[i] * 8, nl[i], s);
+  ECL_HIP(hipStreamSynchronize(s));
+  std::vector<LevelView> lv = rs->levels;
+  for (size_t i = 0; i < na; ++i) lv[i].lines = nl[i] ? lines.as<uint4>() + off[i] * 8 : nullptr;
+  h2d(rs->d_levels.ptr, lv.data(), na);
+  rs->levels = std::move(lv);
+  rs->lines = std::move(lines);
+}
+
 // ===========================================================================
 // persistence: header, alphabet, symbol counts, level bits (the directory and
 // the metadata are rebuilt on load)
@@ -1832,6 +1860,20 @@ ecl_status ecl_wavelet_tree_destroy(ecl_wavelet_tree wt) {
       set_device(wt->device);
       delete wt;
     }
+  });
+}
+ecl_status ecl_wavelet_tree_build_query_lines(ecl_wavelet_tree wt) {
+  return guard([&] {
+    require(wt != nullptr, "NULL tree");
+    set_device(wt->device);
+    build_query_lines(wt->rs);
+  });
+}
+ecl_status ecl_rank_select_build_query_lines(ecl_rank_select rs) {
+  return guard([&] {
+    require(rs != nullptr, "NULL rank select");
+    set_device(rs->device);
+    build_query_lines(rs);
   });
 }
 ecl_status ecl_wavelet_tree_save(ecl_wavelet_tree wt, const char* path) {

@@ -353,6 +353,45 @@ __global__ void presence_kernel(const T* text, uint64_t n, uint32_t* flags) {
     flags[static_cast<uint64_t>(text[i])] = 1u;
 }
 
+// One thread per 16-byte piece of the query lines of one array: piece 0 of
+// line j holds the ones before the line (from the array's directory) and
+// words 30j, 30j+1; piece k > 0 holds words 30j+4k-2 .. 30j+4k+1 (zeros
+// past the array).  Consecutive threads read and write consecutive memory.
+__global__ __launch_bounds__(256) void query_lines_kernel(LevelView v, uint4* __restrict__ lines,
+                                                          uint64_t nlines) {
+  const uint64_t t = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x;
+  const uint64_t j = t >> 3;
+  const uint32_t k = static_cast<uint32_t>(t & 7);
+  if (j >= nlines) return;
+  const uint64_t nw = (v.size + 31) / 32;
+  const uint64_t first = 30 * j + (k == 0 ? 0 : 4 * k - 2);  // even: 8-byte aligned pairs
+  const uint32_t cnt = k == 0 ? 2 : 4;
+  const uint64_t* w64 = reinterpret_cast<const uint64_t*>(v.words);
+  uint32_t w[4] = {0, 0, 0, 0};
+  if (first + cnt <= nw) {
+    const uint64_t a = w64[first / 2];
+    w[0] = static_cast<uint32_t>(a);
+    w[1] = static_cast<uint32_t>(a >> 32);
+    if (cnt == 4) {
+      const uint64_t b = w64[first / 2 + 1];
+      w[2] = static_cast<uint32_t>(b);
+      w[3] = static_cast<uint32_t>(b >> 32);
+    }
+  } else {
+    for (uint32_t x = 0; x < cnt; ++x) w[x] = first + x < nw ? v.words[first + x] : 0u;
+  }
+  uint4 out;
+  if (k == 0) {
+    const uint64_t s = j * detail::kLineBits;
+    bool unused = false;
+    const uint64_t head = s == 0 ? 0 : detail::rank1_inner(v, s, &unused);
+    out = make_uint4(static_cast<uint32_t>(head), static_cast<uint32_t>(head >> 32), w[0], w[1]);
+  } else {
+    out = make_uint4(w[0], w[1], w[2], w[3]);
+  }
+  lines[j * 8 + k] = out;
+}
+
 // segment i of the blob to its destination, one workgroup for all segments
 __global__ __launch_bounds__(256) void upload_kernel(UploadBlob b) {
   for (uint32_t i = 0; i < b.nseg; ++i) {
@@ -370,6 +409,17 @@ uint32_t grid_for(uint64_t work, uint32_t block, uint32_t max_blocks) {
 }
 
 }  // namespace
+
+void launch_query_lines(const LevelView& v, uint4* lines, uint64_t nlines, hipStream_t s) {
+  if (nlines == 0) return;
+  const uint64_t blocks = (nlines * 8 + 255) / 256;
+  if (blocks > 0x7FFFFFFFull) throw Error(ECL_ERROR_INVALID_ARGUMENT, "bit array too large");
+  LevelView plain = v;
+  plain.lines = nullptr;  // headers come from the directory
+  hipLaunchKernelGGL(query_lines_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(256), 0, s,
+                     plain, lines, nlines);
+  ECL_HIP(hipGetLastError());
+}
 
 void launch_upload(const UploadBlob& b, hipStream_t s) {
   if (b.nseg == 0) return;

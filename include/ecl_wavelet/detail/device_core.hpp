@@ -155,7 +155,12 @@ struct LevelView {
   uint64_t ones;            // total ones
   uint64_t num_l1;
   uint64_t sample_cap;      // floor(size / 4096)
+  // Optional query lines (nullptr when not built): 128-byte lines, each the
+  // u64 count of ones before the line followed by the line's 960 bits, so a
+  // rank reads one line instead of the bit block, its L2 and its L1 entry.
+  const uint4* lines;
 };
+constexpr uint32_t kLineBits = 960;  // bits per query line (30 words)
 
 __device__ __forceinline__ bool level_bit(const LevelView& v, uint64_t i) {
   return (v.words[i >> 5] >> (i & 31)) & 1u;
@@ -190,6 +195,40 @@ __device__ __forceinline__ uint64_t rank1_inner(const LevelView& v,
   return r + acc;
 }
 
+// Ones in [0, index) for index in (0, size) from the query line holding
+// the index: the line's header plus the ones of its bits before the index.
+__device__ __forceinline__ uint64_t rank1_lines(const LevelView& v, uint64_t index, bool* bit) {
+  const uint64_t line = index / kLineBits;
+  const uint32_t off = static_cast<uint32_t>(index - line * kLineBits);
+  const uint4* p = v.lines + line * 8;
+  uint4 q[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) q[k] = p[k];
+  const uint64_t head = (uint64_t)q[0].x | ((uint64_t)q[0].y << 32);
+  // bits as 15 u64 words: (q0.z, q0.w), (q1.x, q1.y), (q1.z, q1.w), ...
+  uint64_t w[15];
+  w[0] = (uint64_t)q[0].z | ((uint64_t)q[0].w << 32);
+#pragma unroll
+  for (int k = 1; k < 8; ++k) {
+    w[2 * k - 1] = (uint64_t)q[k].x | ((uint64_t)q[k].y << 32);
+    if (2 * k < 15) w[2 * k] = (uint64_t)q[k].z | ((uint64_t)q[k].w << 32);
+  }
+  const uint32_t wi = off >> 6, bi = off & 63;
+  uint32_t acc = 0;
+  bool b = false;
+#pragma unroll
+  for (uint32_t k = 0; k < 15; ++k) {
+    uint64_t x = w[k];
+    if (k == wi) {
+      b = (x >> bi) & 1u;
+      x &= (bi == 0) ? 0ull : (~0ull >> (64 - bi));
+    }
+    acc += (k <= wi) ? static_cast<uint32_t>(__popcll(x)) : 0u;
+  }
+  *bit = b;
+  return head + acc;
+}
+
 // rank_Value(index) over [0, index), index in [0, size]; rank_select.cuh:
 // 125-208 semantics (index == size returns the total, bit = false).
 template <int Value>
@@ -205,7 +244,7 @@ __device__ __forceinline__ uint64_t level_rank(const LevelView& v,
     ones = v.ones;
     index = v.size;
   } else {
-    ones = rank1_inner(v, index, &b);
+    ones = v.lines != nullptr ? rank1_lines(v, index, &b) : rank1_inner(v, index, &b);
   }
   if (bit) *bit = b;
   return Value ? ones : index - ones;

@@ -230,6 +230,14 @@ ecl_status ecl_wavelet_tree_create_device(ecl_type type, const void* d_text, siz
                                           int device, ecl_wavelet_tree* out);
 /* ~WaveletTree (wavelet_tree.cuh:931-943) */
 ecl_status ecl_wavelet_tree_destroy(ecl_wavelet_tree wt);
+/* Query lines (no counterpart in the reference; an opt-in space-for-time
+ * structure): a second copy of every level's bits in 128-byte lines of 960
+ * bits, each headed by the count of ones before it, so that rank and access
+ * read one memory line per level instead of three (bit block, L2 and L1
+ * entry).  Costs 128/120 of the bits in memory and one pass over them;
+ * results do not change.  Select keeps the reference layout. */
+ecl_status ecl_wavelet_tree_build_query_lines(ecl_wavelet_tree wt);
+ecl_status ecl_rank_select_build_query_lines(ecl_rank_select rs);
 /* Persistence (no counterpart in the reference, which rebuilds every tree
  * from its text; for warm starts of servers).  The file holds a header, the
  * alphabet, the symbol counts and every level's bits; loading checks them

@@ -250,3 +250,23 @@ def test_binary_2e34_bits(gpu, p):
     bit = (words[(pos >> np.uint64(5)).astype(np.int64)] >> (pos & np.uint64(31)).astype(np.uint32)) & 1
     assert np.all(bit == 1)
     assert np.array_equal(_rank_from_index(words, l1, l2, pos), k - np.uint64(1))
+
+
+def test_query_lines_binary(gpu):
+    """Binary rank1/rank0 through query lines equal the oracle's scans."""
+    from ecl_wavelet_amd import BitArray, RankSelect
+    m = (1 << 22) + 4321
+    thr = O.bernoulli_threshold(0.3)
+    ba = BitArray([m, 960 * 7, 5])
+    for i in range(3):
+        ba.fillRandom(i, 40 + i, thr)
+    rs = RankSelect(ba)
+    words = [rs.bit_array_.words(i) for i in range(3)]
+    sizes = [m, 960 * 7, 5]
+    rs.buildQueryLines()
+    rng = np.random.default_rng(3)
+    for i, mm in enumerate(sizes):
+        idx = np.concatenate([rng.integers(0, mm + 1, 3000), np.arange(0, mm + 1, 960),
+                              np.arange(959, mm, 960), [0, mm]]).astype(np.uint64)
+        for v in (0, 1):
+            assert np.array_equal(rs.rank(i, idx, v), O.scan_bits_batch(words[i], mm, v, idx))

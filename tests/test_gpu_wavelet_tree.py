@@ -457,3 +457,39 @@ def test_load_rejects_damaged_files(gpu, tmp_path):
         WaveletTree.load(tmp_path / "missing.ecl")
     lt = WaveletTree.load(path)  # the intact file still loads
     _check_queries(lt, text, None, nq=500, seed=1)
+
+
+@pytest.mark.parametrize("case", ["u8_256", "u8_20", "u16_1000", "u32_alpha"])
+def test_query_lines_same_answers(gpu, case):
+    """With query lines (one 128-byte line per level and rank) every access,
+    rank and select answer is the one without them, including positions at
+    line boundaries (multiples of 960 and their neighbours)."""
+    from ecl_wavelet_amd import WaveletTree
+    rng = np.random.default_rng(len(case))
+    if case == "u8_256":
+        text = rng.integers(0, 256, 1_500_017).astype(np.uint8)
+        alpha = np.arange(256, dtype=np.uint8)
+    elif case == "u8_20":
+        text = rng.integers(0, 20, 700_001).astype(np.uint8)
+        alpha = np.arange(20, dtype=np.uint8)
+    elif case == "u16_1000":
+        text = rng.integers(0, 1000, 400_003).astype(np.uint16)
+        alpha = np.arange(1000, dtype=np.uint16)
+    else:
+        alpha = np.sort(rng.choice(1 << 31, 29, replace=False)).astype(np.uint32)
+        text = alpha[rng.integers(0, 29, 250_000)]
+    wt = WaveletTree(text, alpha)
+    n = len(text)
+    idx = np.concatenate([rng.integers(0, n, 5000),
+                          np.arange(0, n, 960), np.arange(959, n, 960),
+                          np.arange(961, n, 960), [n - 1]]).astype(np.uint64)
+    ri = np.concatenate([idx, [0, n]]).astype(np.uint64)
+    rc = alpha[rng.integers(0, len(alpha), len(ri))]
+    q = wt.make_queries(ri, rc)
+    before = (wt.access(idx), wt.rank(q.copy()))
+    wt.buildQueryLines()
+    wt.buildQueryLines()  # a second call keeps the lines
+    assert np.array_equal(wt.access(idx), before[0])
+    assert np.array_equal(wt.rank(q.copy()), before[1])
+    assert np.array_equal(wt.access(idx), text[idx])
+    _check_queries(wt, text, None, nq=3000, seed=4)
