@@ -3,14 +3,20 @@
 Workload (the full-size configuration the headline metric is quoted on):
 a uint8 text of 6*10^9 symbols drawn uniformly (counter-based hash, seed 4)
 from sigma = 256 symbols, resident in HBM.  One step = one wavelet-tree
-build over the resident text + Q queries of each kind (access, rank,
-select; uniform positions/symbols, drawn on the device before the timed
-region) through the device-resident entry points, plus the same 3Q queries
-as one mixed-kind batch in a single launch.
+build over the resident text; Q queries of each kind (access, rank, select;
+uniform positions/symbols, drawn on the device before the timed region)
+through the device-resident entry points, plus the same 3Q queries as one
+mixed-kind batch in a single launch; the query lines; the same queries
+again over the lines.
 
 value = aggregate device-resident query throughput of the per-kind calls
-(access+rank+select queries / their kernel time).  build_gbps = text bytes /
-build time.  The roofline object is for the dominant kernel (the construction
+over the tree with its query lines (access+rank+select queries / their
+kernel time); the query lines (a second copy of the level bits in 128-byte
+lines headed by their rank, built on the finished tree) are built inside
+each step and timed on their own (query_lines_build_ms), and the same
+queries over the plain directory, timed in the same step before the lines
+exist, are reported under plain_directory.  build_gbps = text bytes / build
+time (the tree and its directory, without the lines).  The roofline object is for the dominant kernel (the construction
 pass, which writes the bits of every level from one read of the text), with
 per-launch time from HIP events inside the library.
 The CPU baseline runs the oracle's OpenMP direct scans on a bounded sample
@@ -492,7 +498,8 @@ def pmc_workload(args):
 def pmc_queries(args):
     """Kernels for the query counter passes: the copy calibration, then one
     build and 10^7 device-generated queries per kind over the benchmark
-    tree (each batch launched once)."""
+    tree (each batch launched once), then the query lines and the same three
+    batches over them."""
     import ctypes as C
     import ecl_wavelet_amd as E
     hip = C.CDLL("libamdhip64.so")
@@ -517,7 +524,32 @@ def pmc_queries(args):
     wt.rankDevice(d_rq, Q, d_out, flags=1)
     wt.selectDevice(d_sq, Q, d_out, flags=1)
     E.synchronize()
-    log(f"queries: {Q} access, rank and select over n={args.n}, sigma={args.sigma}")
+    # the same batches again over the query lines
+    wt.buildQueryLines()
+    wt.accessDevice(d_idx, Q, d_out, flags=2)
+    wt.rankDevice(d_rq, Q, d_out, flags=1)
+    wt.selectDevice(d_sq, Q, d_out, flags=1)
+    E.synchronize()
+    log(f"queries: {Q} access, rank and select over n={args.n}, sigma={args.sigma}, "
+        f"without and with query lines")
+
+
+def run_queries(wt, timer, Q, d_idx, d_rq, d_sq, d_mq, d_out):
+    """Milliseconds of the three per-kind calls and the mixed batch."""
+    t = {}
+    timer.start()
+    wt.accessDevice(d_idx, Q, d_out, flags=2)
+    t["access"] = timer.stop()
+    timer.start()
+    wt.rankDevice(d_rq, Q, d_out, flags=1)
+    t["rank"] = timer.stop()
+    timer.start()
+    wt.selectDevice(d_sq, Q, d_out, flags=1)
+    t["select"] = timer.stop()
+    timer.start()
+    wt.queryBatchDevice(d_mq, 3 * Q, d_out)
+    t["mixed"] = timer.stop()
+    return t
 
 
 def main():
@@ -553,8 +585,10 @@ def main():
     d_out = E.DeviceBuffer(3 * Q * 8)
     timer = E.Timer()
 
-    build_ms, q_ms, mixed_ms = [], [], []
+    build_ms, lines_ms, q_ms, mixed_ms = [], [], [], []
     kind_ms = {"access": [], "rank": [], "select": []}
+    plain_ms = {"access": [], "rank": [], "select": [], "mixed": []}
+    lines_bytes = 0
     wt = None
     pass_launch_ms = []
     build_kernels = {}
@@ -578,28 +612,28 @@ def main():
         wt.generateQueries(2, 300 + step, Q, d_sq)
         wt.generateQueries(3, 400 + step, 3 * Q, d_mq)
         E.synchronize()
-        times = {}
+        plain = run_queries(wt, timer, Q, d_idx, d_rq, d_sq, d_mq, d_out)
+        free0 = E.mem_info()[0]
         timer.start()
-        wt.accessDevice(d_idx, Q, d_out, flags=2)
-        times["access"] = timer.stop()
-        timer.start()
-        wt.rankDevice(d_rq, Q, d_out, flags=1)
-        times["rank"] = timer.stop()
-        timer.start()
-        wt.selectDevice(d_sq, Q, d_out, flags=1)
-        times["select"] = timer.stop()
-        timer.start()
-        wt.queryBatchDevice(d_mq, 3 * Q, d_out)
-        mx = timer.stop()
+        wt.buildQueryLines()
+        lb = timer.stop()
+        lines_bytes = free0 - E.mem_info()[0]
+        times = run_queries(wt, timer, Q, d_idx, d_rq, d_sq, d_mq, d_out)
+        mx = times.pop("mixed")
         log(f"step {step}{'' if timed else ' (warmup)'}: build {b:.1f} ms, "
-            f"access {times['access']:.1f} ms, rank {times['rank']:.1f} ms, "
-            f"select {times['select']:.1f} ms, mixed {mx:.1f} ms")
+            f"plain access {plain['access']:.2f} ms, rank {plain['rank']:.2f} ms, "
+            f"select {plain['select']:.2f} ms, mixed {plain['mixed']:.2f} ms; lines {lb:.2f} ms; "
+            f"access {times['access']:.2f} ms, rank {times['rank']:.2f} ms, "
+            f"select {times['select']:.2f} ms, mixed {mx:.2f} ms")
         if timed:
             build_ms.append(b)
+            lines_ms.append(lb)
             q_ms.append(sum(times.values()))
             mixed_ms.append(mx)
             for k, v in times.items():
                 kind_ms[k].append(v)
+            for k, v in plain.items():
+                plain_ms[k].append(v)
 
     # per-call latency of a plain small batch vs the same work batched
     lat = []
@@ -640,7 +674,8 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": float(np.mean(np.array(build_ms) + np.array(q_ms))),
+        "ms_per_step": float(np.mean(np.array(build_ms) + np.array(lines_ms) + np.array(q_ms)
+                                     + np.array(mixed_ms))),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -648,11 +683,15 @@ def main():
         "data": "synthetic",
         "config": {
             "workload": (f"wavelet tree over a uint8 text of n={n} uniform over "
-                         f"sigma={sigma} (device-resident); per step: 1 build + "
-                         f"{Q} each of access/rank/select + one mixed batch of {3 * Q}"),
+                         f"sigma={sigma} (device-resident); per step: 1 build, "
+                         f"{Q} each of access/rank/select + one mixed batch of {3 * Q} "
+                         f"over the plain directory, the query lines, the same "
+                         f"queries over the lines"),
             "n": n, "sigma": sigma, "queries_per_kind": Q, "levels": L,
         },
         "build_ms": build,
+        "query_lines_build_ms": float(np.median(lines_ms)),
+        "query_lines_bytes": int(lines_bytes),
         "build_gbps": n / (build / 1e3) / 1e9,
         "build_algorithmic_gbps": build_bytes / (build / 1e3) / 1e9,
         "build_kernels_ms": kern,
@@ -663,6 +702,14 @@ def main():
         "small_call_rank_p50_ms": float(np.percentile(lat, 50)),
         "small_call_rank_p99_ms": float(np.percentile(lat, 99)),
         "rank_kernel_ms": rank_k[1] / max(rank_k[0], 1),
+        "plain_directory": {
+            "access_qps": Q / (np.mean(plain_ms["access"]) / 1e3),
+            "rank_qps": Q / (np.mean(plain_ms["rank"]) / 1e3),
+            "select_qps": Q / (np.mean(plain_ms["select"]) / 1e3),
+            "mixed_batch_qps": 3 * Q / (np.mean(plain_ms["mixed"]) / 1e3),
+            "per_kind_qps": 3 * Q / (np.mean(np.array(plain_ms["access"]) + np.array(plain_ms["rank"])
+                                             + np.array(plain_ms["select"])) / 1e3),
+        },
         "roofline": {
             "kernel": "wt_pass (all levels of the build in one pass)",
             "bound": "hbm",

@@ -73,6 +73,7 @@ _SIGS = {
     "ecl_device_mem_info": (st, [i32, C.POINTER(sz), C.POINTER(sz)]),
     "ecl_release_cached_memory": (st, [i32]),
     "ecl_set_build_span": (st, [C.c_uint32]),
+    "ecl_set_query_lanes": (st, [C.c_uint32]),
     "ecl_device_name": (st, [i32, C.c_char_p, sz]),
     "ecl_event_create": (st, [C.POINTER(vp)]),
     "ecl_event_destroy": (st, [vp]),

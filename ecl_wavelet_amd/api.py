@@ -676,6 +676,14 @@ def set_build_span(tiles_per_workgroup=0):
     call("ecl_set_build_span", tiles_per_workgroup)
 
 
+def set_query_lanes(lanes=0):
+    """Lanes per access/rank query in the tree query kernels (0: automatic,
+    two over query lines and one otherwise; 2: each lane of a pair fetches
+    half of every level's bits; 1: one lane per query); a tuning knob that
+    does not change results."""
+    call("ecl_set_query_lanes", lanes)
+
+
 def release_cached_memory(device=0):
     """Frees the construction scratch the library keeps per device between
     builds."""

@@ -1435,6 +1435,7 @@ static TreeQueryParams query_params(const ecl_wavelet_tree_t* wt, uint32_t flags
   p.alphabet = wt->d_alphabet.ptr;
   p.alphabet_out = (!wt->min_alphabet && !(flags & ECL_RESULTS_MINIMAL)) ? wt->d_alphabet.ptr
                                                                          : nullptr;
+  p.has_lines = wt->rs->lines.ptr != nullptr ? 1 : 0;
   return p;
 }
 
@@ -1501,6 +1502,12 @@ ecl_status ecl_set_build_span(uint32_t tiles_per_workgroup) {
   return guard([&] {
     require(tiles_per_workgroup <= 64, "at most 64 tiles per workgroup");
     set_pass_tiles_per_wg(tiles_per_workgroup);
+  });
+}
+ecl_status ecl_set_query_lanes(uint32_t lanes) {
+  return guard([&] {
+    require(lanes <= 2, "lanes must be 0 (automatic), 1 or 2");
+    set_query_lanes(lanes);
   });
 }
 ecl_status ecl_release_cached_memory(int device) {

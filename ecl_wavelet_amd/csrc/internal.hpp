@@ -137,6 +137,7 @@ void launch_tile_hist(const PassParams& p, hipStream_t s);
 void launch_tile_scan(const PassParams& p, hipStream_t s);
 void launch_pass(const PassParams& p, hipStream_t s);
 void set_pass_tiles_per_wg(uint32_t span);  // 0: automatic
+void set_query_lanes(uint32_t lanes);      // lanes per access/rank query: 0 auto, 1 or 2
 uint32_t pass_span(uint64_t num_tiles);     // tiles per workgroup of a pass
 // zeroes the words that several workgroups of a single-node pass write
 // (p.tiles_per_wg set); the pass then needs no zeroed bit arrays
@@ -177,6 +178,7 @@ struct TreeQueryParams {
   int map_kind; const uint32_t* lut; const void* alphabet;  // symbol map
   const void* alphabet_out;  // access: map result back (nullable = identity)
   uint64_t max_lds_counts;   // unused by callers; tuning knob
+  int has_lines;             // 1: the tree's levels carry query lines
 };
 void launch_tree_access(const TreeQueryParams& p, const uint64_t* idx,
                         uint64_t m, void* out, hipStream_t s);

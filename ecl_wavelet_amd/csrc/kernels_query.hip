@@ -8,6 +8,8 @@
 // HBM traffic left per level.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+
 #include "internal.hpp"
 
 namespace ecl {
@@ -92,39 +94,42 @@ __device__ __forceinline__ uint64_t map_symbol(const TreeQueryParams& p,
   return lo;  // == sigma when x is larger than every symbol
 }
 
-template <typename T>
+// G lanes per query (1, or 2: the adjacent lanes 2k, 2k+1 descend for the
+// same query, each fetching half of every level's bits; lane 2k writes).
+template <typename T, int G>
 __global__ __launch_bounds__(256) void tree_access_kernel(
     TreeQueryParams p, const uint64_t* __restrict__ idx, uint64_t m,
     T* __restrict__ out) {
   extern __shared__ uint8_t s_meta[];
   const Meta mt = stage_meta(p.view, s_meta);
   const T* amap = static_cast<const T*>(p.alphabet_out);
-  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-  for (uint64_t q = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x / G;
+  for (uint64_t q = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) / G;
        q < m; q += stride) {
     const uint64_t i = idx[q];
     if (i >= p.view.n) continue;  // accessKernel skips such indices
     const uint64_t r =
-        detail::tree_access(p.view, mt.lv, mt.counts, mt.ranks, mt.nal, i);
-    out[q] = amap ? amap[r] : static_cast<T>(r);
+        detail::tree_access<G == 2>(p.view, mt.lv, mt.counts, mt.ranks, mt.nal, i);
+    if (G == 1 || (threadIdx.x & 1) == 0) out[q] = amap ? amap[r] : static_cast<T>(r);
   }
 }
 
-template <typename T>
+template <typename T, int G>
 __global__ __launch_bounds__(256) void tree_rank_kernel(
     TreeQueryParams p, const uint8_t* __restrict__ qs, uint64_t m,
     uint64_t* __restrict__ out) {
   extern __shared__ uint8_t s_meta[];
   const Meta mt = stage_meta(p.view, s_meta);
-  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-  for (uint64_t q = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x / G;
+  for (uint64_t q = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) / G;
        q < m; q += stride) {
     const uint8_t* e = qs + q * 16;
     const uint64_t i = *reinterpret_cast<const uint64_t*>(e);
     const uint64_t c = map_symbol<T>(p, load_sym(e, sizeof(T)));
-    out[q] = i > p.view.n ? 0
-                          : detail::tree_rank(p.view, mt.lv, mt.counts,
-                                              mt.ranks, mt.nal, c, i);
+    const uint64_t r = i > p.view.n ? 0
+                                    : detail::tree_rank<G == 2>(p.view, mt.lv, mt.counts,
+                                                                mt.ranks, mt.nal, c, i);
+    if (G == 1 || (threadIdx.x & 1) == 0) out[q] = r;
   }
 }
 
@@ -145,29 +150,97 @@ __global__ __launch_bounds__(256) void tree_select_kernel(
   }
 }
 
+// ---------------------------------------------------------------------------
+// Mixed-kind batches: a workgroup takes the batch in chunks of up to
+// kKindChunk queries (a multiple of its size, sized so that every workgroup
+// of the grid has a chunk) and first lists each chunk's queries by kind in LDS (per wave,
+// three ballots give each lane its place; one LDS atomic per kind and wave
+// reserves the places), then walks the three lists one after the other.  A
+// wave then runs one kind's descent for all its lanes, except the waves at
+// the two list boundaries, instead of running the access, rank and select
+// descents one after the other for the lanes of each kind.  Results are
+// written by query index, so the order of the output is unchanged.
+constexpr uint32_t kKindChunk = 1024;
+
+struct KindLists {
+  uint16_t idx[3][kKindChunk];
+  uint32_t cnt[3];
+};
+
+// kind_of(j) for j < len (<= kKindChunk): the kind of the chunk's query j
+// (0 access, 1 rank, anything else select, as the per-query dispatch reads
+// it).  Every thread of the workgroup calls this; blockDim.x is a multiple
+// of 64.
+template <typename KindOf>
+__device__ __forceinline__ void group_by_kind(KindLists& L, uint32_t len, KindOf kind_of) {
+  if (threadIdx.x < 3) L.cnt[threadIdx.x] = 0;
+  __syncthreads();
+  const uint32_t lane = threadIdx.x & 63u;
+  const uint64_t below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  const uint32_t rounded = (len + blockDim.x - 1) / blockDim.x * blockDim.x;
+  for (uint32_t j = threadIdx.x; j < rounded; j += blockDim.x) {
+    uint32_t k = 3;  // past the chunk: no kind
+    if (j < len) {
+      const uint32_t kk = kind_of(j);
+      k = kk > 2 ? 2u : kk;
+    }
+#pragma unroll
+    for (uint32_t c = 0; c < 3; ++c) {
+      const uint64_t mask = __ballot(k == c);
+      if (mask != 0) {
+        uint32_t base = 0;
+        if (lane == 0) base = atomicAdd(&L.cnt[c], static_cast<uint32_t>(__popcll(mask)));
+        base = __shfl(base, 0);
+        if (k == c) L.idx[c][base + static_cast<uint32_t>(__popcll(mask & below))] = static_cast<uint16_t>(j);
+      }
+    }
+  }
+  __syncthreads();
+}
+
+// Calls run(j, kind) for every query of the chunk, in kind order, spread
+// over the workgroup; then waits for the workgroup (the lists are reused).
+template <typename Run>
+__device__ __forceinline__ void run_by_kind(const KindLists& L, Run run) {
+  const uint32_t c0 = L.cnt[0], c1 = L.cnt[1], tot = c0 + c1 + L.cnt[2];
+  for (uint32_t e = threadIdx.x; e < tot; e += blockDim.x) {
+    uint32_t j, k;
+    if (e < c0) { j = L.idx[0][e]; k = 0; }
+    else if (e < c0 + c1) { j = L.idx[1][e - c0]; k = 1; }
+    else { j = L.idx[2][e - c0 - c1]; k = 2; }
+    run(j, k);
+  }
+  __syncthreads();
+}
+
 __global__ __launch_bounds__(256) void tree_mixed_kernel(
     TreeQueryParams p, const ecl_query* __restrict__ qs, uint64_t m,
-    uint64_t* __restrict__ out) {
+    uint64_t* __restrict__ out, uint32_t chunk) {
   extern __shared__ uint8_t s_meta[];
+  __shared__ KindLists lists;
   const Meta mt = stage_meta(p.view, s_meta);
-  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-  for (uint64_t q = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
-       q < m; q += stride) {
-    const ecl_query e = qs[q];
-    uint64_t r;
-    if (e.kind == 0) {
-      r = e.arg < p.view.n ? detail::tree_access(p.view, mt.lv, mt.counts,
-                                                 mt.ranks, mt.nal, e.arg)
-                           : 0;
-    } else if (e.kind == 1) {
-      r = e.arg > p.view.n ? 0
-                           : detail::tree_rank(p.view, mt.lv, mt.counts,
-                                               mt.ranks, mt.nal, e.symbol, e.arg);
-    } else {
-      r = detail::tree_select(p.view, mt.lv, mt.counts, mt.ranks, mt.nal,
-                              e.symbol, e.arg);
-    }
-    out[q] = r;
+  for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * chunk; base < m;
+       base += static_cast<uint64_t>(gridDim.x) * chunk) {
+    const uint32_t len = static_cast<uint32_t>(m - base < chunk ? m - base : chunk);
+    const ecl_query* cq = qs + base;
+    group_by_kind(lists, len, [&](uint32_t j) { return cq[j].kind; });
+    run_by_kind(lists, [&](uint32_t j, uint32_t k) {
+      const ecl_query e = cq[j];
+      uint64_t r;
+      if (k == 0) {
+        r = e.arg < p.view.n ? detail::tree_access(p.view, mt.lv, mt.counts,
+                                                   mt.ranks, mt.nal, e.arg)
+                             : 0;
+      } else if (k == 1) {
+        r = e.arg > p.view.n ? 0
+                             : detail::tree_rank(p.view, mt.lv, mt.counts,
+                                                 mt.ranks, mt.nal, e.symbol, e.arg);
+      } else {
+        r = detail::tree_select(p.view, mt.lv, mt.counts, mt.ranks, mt.nal,
+                                e.symbol, e.arg);
+      }
+      out[base + j] = r;
+    });
   }
 }
 
@@ -224,30 +297,36 @@ __global__ void tree_gen_kernel(TreeView t, int kind, uint64_t seed,
 // (levels, counts, node ranks) is read from global memory, where the
 // metadata of all trees stays in the L2 / MALL.  A grid of a few
 // workgroups per CU strides over the batch.
+__device__ __forceinline__ uint64_t forest_one(const TreeView* __restrict__ views,
+                                               uint64_t num_views, uint64_t tree, uint32_t k,
+                                               uint64_t arg, uint32_t symbol) {
+  if (tree >= num_views) return 0;
+  const TreeView t = views[tree];
+  if (k == 0) {
+    return arg < t.n ? detail::tree_access(t, t.levels, t.counts, t.ranks, t.nodes_at_level, arg)
+                     : 0;
+  }
+  if (k == 1) {
+    return arg > t.n ? 0
+                     : detail::tree_rank(t, t.levels, t.counts, t.ranks, t.nodes_at_level,
+                                         symbol, arg);
+  }
+  return detail::tree_select(t, t.levels, t.counts, t.ranks, t.nodes_at_level, symbol, arg);
+}
+
 __global__ void forest_query_kernel(const TreeView* __restrict__ views, uint64_t num_trees,
                                     const ecl_forest_query* __restrict__ qs, uint64_t m,
-                                    uint64_t* __restrict__ out) {
-  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-  for (uint64_t q = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
-       q < m; q += stride) {
-    const ecl_forest_query e = qs[q];
-    uint64_t r = 0;
-    if (e.tree < num_trees) {
-      const TreeView t = views[e.tree];
-      if (e.kind == 0) {
-        r = e.arg < t.n ? detail::tree_access(t, t.levels, t.counts, t.ranks,
-                                              t.nodes_at_level, e.arg)
-                        : 0;
-      } else if (e.kind == 1) {
-        r = e.arg > t.n ? 0
-                        : detail::tree_rank(t, t.levels, t.counts, t.ranks,
-                                            t.nodes_at_level, e.symbol, e.arg);
-      } else {
-        r = detail::tree_select(t, t.levels, t.counts, t.ranks, t.nodes_at_level,
-                                e.symbol, e.arg);
-      }
-    }
-    out[q] = r;
+                                    uint64_t* __restrict__ out, uint32_t chunk) {
+  __shared__ KindLists lists;
+  for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * chunk; base < m;
+       base += static_cast<uint64_t>(gridDim.x) * chunk) {
+    const uint32_t len = static_cast<uint32_t>(m - base < chunk ? m - base : chunk);
+    const ecl_forest_query* cq = qs + base;
+    group_by_kind(lists, len, [&](uint32_t j) { return static_cast<uint32_t>(cq[j].kind); });
+    run_by_kind(lists, [&](uint32_t j, uint32_t k) {
+      const ecl_forest_query e = cq[j];
+      out[base + j] = forest_one(views, num_trees, e.tree, k, e.arg, e.symbol);
+    });
   }
 }
 
@@ -257,34 +336,23 @@ __global__ void segment_query_kernel(const TreeView* __restrict__ views, uint64_
                                      const ecl_query* __restrict__ qs, uint64_t m,
                                      const uint64_t* __restrict__ seg_start,
                                      const uint32_t* __restrict__ seg_tree, uint32_t nseg,
-                                     uint64_t* __restrict__ out) {
-  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-  for (uint64_t q = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
-       q < m; q += stride) {
-    uint32_t lo = 0, hi = nseg;  // last segment with seg_start <= q
-    while (hi - lo > 1) {
-      const uint32_t mid = (lo + hi) >> 1;
-      if (seg_start[mid] <= q) lo = mid; else hi = mid;
-    }
-    const uint32_t tree = seg_tree[lo];
-    const ecl_query e = qs[q];
-    uint64_t r = 0;
-    if (tree < num_views) {
-      const TreeView t = views[tree];
-      if (e.kind == 0) {
-        r = e.arg < t.n ? detail::tree_access(t, t.levels, t.counts, t.ranks,
-                                              t.nodes_at_level, e.arg)
-                        : 0;
-      } else if (e.kind == 1) {
-        r = e.arg > t.n ? 0
-                        : detail::tree_rank(t, t.levels, t.counts, t.ranks,
-                                            t.nodes_at_level, e.symbol, e.arg);
-      } else {
-        r = detail::tree_select(t, t.levels, t.counts, t.ranks, t.nodes_at_level,
-                                e.symbol, e.arg);
+                                     uint64_t* __restrict__ out, uint32_t chunk) {
+  __shared__ KindLists lists;
+  for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * chunk; base < m;
+       base += static_cast<uint64_t>(gridDim.x) * chunk) {
+    const uint32_t len = static_cast<uint32_t>(m - base < chunk ? m - base : chunk);
+    const ecl_query* cq = qs + base;
+    group_by_kind(lists, len, [&](uint32_t j) { return cq[j].kind; });
+    run_by_kind(lists, [&](uint32_t j, uint32_t k) {
+      const uint64_t q = base + j;
+      uint32_t lo = 0, hi = nseg;  // last segment with seg_start <= q
+      while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (seg_start[mid] <= q) lo = mid; else hi = mid;
       }
-    }
-    out[q] = r;
+      const ecl_query e = cq[j];
+      out[q] = forest_one(views, num_views, seg_tree[lo], k, e.arg, e.symbol);
+    });
   }
 }
 
@@ -357,6 +425,23 @@ __global__ __launch_bounds__(256) void rs_select_kernel(
   }
 }
 
+// The kind grouping works on whole waves of one workgroup.
+void check_kind_launch(uint32_t block, uint32_t grid) {
+  if (block == 0 || block % 64 != 0 || block > 1024 || grid == 0)
+    throw Error(ECL_ERROR_INVALID_ARGUMENT, "query launch: block must be a multiple of 64 up to 1024");
+}
+
+// Chunk length for a grid of `grid` workgroups of `block` lanes: the
+// smallest multiple of the block that gives every workgroup a chunk, between
+// one block and kKindChunk.
+uint32_t kind_chunk(uint64_t m, uint32_t grid, uint32_t block) {
+  uint64_t c = (m + grid - 1) / grid;
+  c = (c + block - 1) / block * block;
+  if (c < block) c = block;
+  if (c > kKindChunk) c = kKindChunk / block * block;
+  return static_cast<uint32_t>(c);
+}
+
 uint32_t query_grid(uint64_t m) {
   uint64_t g = (m + 255) / 256;
   if (g < 1) g = 1;
@@ -372,18 +457,48 @@ size_t meta_bytes(const TreeView& t) {
 
 }  // namespace
 
+// 0: automatic -- two lanes per query over query lines (a 128 B line split
+// over a pair moves more lines per second), one over the plain directory
+// (its 64 B block and two directory entries gain nothing from the split).
+std::atomic<uint32_t> g_query_lanes{0};
+
+bool pair_lanes(const TreeQueryParams& p) {
+  const uint32_t l = g_query_lanes.load();
+  return l == 0 ? p.has_lines != 0 : l == 2;
+}
+
+template <int G>
+void access_g(const TreeQueryParams& p, const uint64_t* idx, uint64_t m, void* out, hipStream_t s) {
+  const dim3 g(query_grid(m * G)), b(256);
+  const size_t sh = meta_bytes(p.view);
+  switch (p.esize) {
+    case 1: hipLaunchKernelGGL((tree_access_kernel<uint8_t, G>), g, b, sh, s, p, idx, m, static_cast<uint8_t*>(out)); break;
+    case 2: hipLaunchKernelGGL((tree_access_kernel<uint16_t, G>), g, b, sh, s, p, idx, m, static_cast<uint16_t*>(out)); break;
+    case 4: hipLaunchKernelGGL((tree_access_kernel<uint32_t, G>), g, b, sh, s, p, idx, m, static_cast<uint32_t*>(out)); break;
+    default: hipLaunchKernelGGL((tree_access_kernel<uint64_t, G>), g, b, sh, s, p, idx, m, static_cast<uint64_t*>(out)); break;
+  }
+}
+
+template <int G>
+void rank_g(const TreeQueryParams& p, const uint8_t* qq, uint64_t m, uint64_t* out, hipStream_t s) {
+  const dim3 g(query_grid(m * G)), b(256);
+  const size_t sh = meta_bytes(p.view);
+  switch (p.esize) {
+    case 1: hipLaunchKernelGGL((tree_rank_kernel<uint8_t, G>), g, b, sh, s, p, qq, m, out); break;
+    case 2: hipLaunchKernelGGL((tree_rank_kernel<uint16_t, G>), g, b, sh, s, p, qq, m, out); break;
+    case 4: hipLaunchKernelGGL((tree_rank_kernel<uint32_t, G>), g, b, sh, s, p, qq, m, out); break;
+    default: hipLaunchKernelGGL((tree_rank_kernel<uint64_t, G>), g, b, sh, s, p, qq, m, out); break;
+  }
+}
+
+void set_query_lanes(uint32_t lanes) { g_query_lanes.store(lanes); }
+
 void launch_tree_access(const TreeQueryParams& p, const uint64_t* idx,
                         uint64_t m, void* out, hipStream_t s) {
   if (m == 0) return;
   ProfScope ps(K_WT_ACCESS, s);
-  const dim3 g(query_grid(m)), b(256);
-  const size_t sh = meta_bytes(p.view);
-  switch (p.esize) {
-    case 1: hipLaunchKernelGGL(tree_access_kernel<uint8_t>, g, b, sh, s, p, idx, m, static_cast<uint8_t*>(out)); break;
-    case 2: hipLaunchKernelGGL(tree_access_kernel<uint16_t>, g, b, sh, s, p, idx, m, static_cast<uint16_t*>(out)); break;
-    case 4: hipLaunchKernelGGL(tree_access_kernel<uint32_t>, g, b, sh, s, p, idx, m, static_cast<uint32_t*>(out)); break;
-    default: hipLaunchKernelGGL(tree_access_kernel<uint64_t>, g, b, sh, s, p, idx, m, static_cast<uint64_t*>(out)); break;
-  }
+  if (pair_lanes(p)) access_g<2>(p, idx, m, out, s);
+  else access_g<1>(p, idx, m, out, s);
   ECL_HIP(hipGetLastError());
 }
 
@@ -391,15 +506,9 @@ void launch_tree_rank(const TreeQueryParams& p, const void* q, uint64_t m,
                       uint64_t* out, hipStream_t s) {
   if (m == 0) return;
   ProfScope ps(K_WT_RANK, s);
-  const dim3 g(query_grid(m)), b(256);
-  const size_t sh = meta_bytes(p.view);
   const uint8_t* qq = static_cast<const uint8_t*>(q);
-  switch (p.esize) {
-    case 1: hipLaunchKernelGGL(tree_rank_kernel<uint8_t>, g, b, sh, s, p, qq, m, out); break;
-    case 2: hipLaunchKernelGGL(tree_rank_kernel<uint16_t>, g, b, sh, s, p, qq, m, out); break;
-    case 4: hipLaunchKernelGGL(tree_rank_kernel<uint32_t>, g, b, sh, s, p, qq, m, out); break;
-    default: hipLaunchKernelGGL(tree_rank_kernel<uint64_t>, g, b, sh, s, p, qq, m, out); break;
-  }
+  if (pair_lanes(p)) rank_g<2>(p, qq, m, out, s);
+  else rank_g<1>(p, qq, m, out, s);
   ECL_HIP(hipGetLastError());
 }
 
@@ -423,8 +532,9 @@ void launch_tree_mixed(const TreeQueryParams& p, const ecl_query* q,
                        uint64_t m, uint64_t* out, hipStream_t s) {
   if (m == 0) return;
   ProfScope ps(K_WT_MIXED, s);
-  hipLaunchKernelGGL(tree_mixed_kernel, dim3(query_grid(m)), dim3(256),
-                     meta_bytes(p.view), s, p, q, m, out);
+  const uint32_t g = query_grid(m);
+  hipLaunchKernelGGL(tree_mixed_kernel, dim3(g), dim3(256), meta_bytes(p.view), s, p, q, m, out,
+                     kind_chunk(m, g, 256));
   ECL_HIP(hipGetLastError());
 }
 
@@ -433,8 +543,9 @@ void launch_forest_query(const TreeView* views, uint64_t num_trees, const ecl_fo
                          hipStream_t s) {
   if (m == 0) return;
   ProfScope ps(K_FOREST_QUERY, s);
+  check_kind_launch(block, grid);
   hipLaunchKernelGGL(forest_query_kernel, dim3(grid), dim3(block), 0, s, views, num_trees, q,
-                     m, out);
+                     m, out, kind_chunk(m, grid, block));
   ECL_HIP(hipGetLastError());
 }
 
@@ -444,8 +555,9 @@ void launch_segment_query(const TreeView* views, uint64_t num_views, const ecl_q
                           hipStream_t s) {
   if (m == 0) return;
   ProfScope ps(K_FOREST_QUERY, s);
+  check_kind_launch(block, grid);
   hipLaunchKernelGGL(segment_query_kernel, dim3(grid), dim3(block), 0, s, views, num_views, q, m,
-                     seg_start, seg_tree, nseg, out);
+                     seg_start, seg_tree, nseg, out, kind_chunk(m, grid, block));
   ECL_HIP(hipGetLastError());
 }
 

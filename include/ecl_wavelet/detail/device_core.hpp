@@ -250,6 +250,93 @@ __device__ __forceinline__ uint64_t level_rank(const LevelView& v,
   return Value ? ones : index - ones;
 }
 
+// Ones in [0, index) for index in (0, size), computed by the two lanes of
+// an adjacent pair (lanes 2k and 2k+1 of a wave, both active, same
+// arguments), each fetching half of the bits: with query lines, lane 0 the
+// line's header and first 448 bits, lane 1 its other 512 bits; otherwise
+// each lane 256 bits of the 64 B block, and both the L1/L2 entries (the
+// same addresses, one fetch).  A random line read by one lane as eight 16 B
+// loads moves far fewer lines per second than the same line split over two
+// lanes.  Both lanes return the count and the bit at index.
+__device__ __forceinline__ uint64_t rank1_pair(const LevelView& v, uint64_t index, bool* bit) {
+  const uint32_t t = threadIdx.x & 1u;
+  const bool lines = v.lines != nullptr;
+  uint32_t d[16];    // this lane's fetched dwords
+  uint32_t first;    // d[first] is bit 0 of this lane's half
+  uint32_t o, split; // index's offset in the line / block; first bit of lane 1's half
+  uint64_t base;     // ones before the line / block
+  if (lines) {
+    const uint64_t line = index / kLineBits;
+    o = static_cast<uint32_t>(index - line * kLineBits);
+    split = 448;
+    const uint4* p = v.lines + line * 8 + 4 * t;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+      const uint4 q = p[k];
+      d[4 * k] = q.x; d[4 * k + 1] = q.y; d[4 * k + 2] = q.z; d[4 * k + 3] = q.w;
+    }
+    // lane 0's first two dwords are the line's header
+    const unsigned long long h = (uint64_t)d[0] | ((uint64_t)d[1] << 32);
+    const unsigned long long ho = __shfl_xor(h, 1, 2);
+    base = t ? ho : h;
+    first = t ? 0u : 2u;
+  } else {
+    const uint64_t blk = index >> 9;
+    o = static_cast<uint32_t>(index & 511);
+    split = 256;
+    const uint4* p = reinterpret_cast<const uint4*>(v.words + (blk << 4)) + 2 * t;
+#pragma unroll
+    for (int k = 0; k < 2; ++k) {
+      const uint4 q = p[k];
+      d[4 * k] = q.x; d[4 * k + 1] = q.y; d[4 * k + 2] = q.z; d[4 * k + 3] = q.w;
+    }
+#pragma unroll
+    for (int k = 8; k < 16; ++k) d[k] = 0;
+    base = v.l1[index >> 16] + v.l2[blk];
+    first = 0;
+  }
+  // this lane counts its bits before index and reads the bit at index when
+  // index lies in its half (lane 0: o < split, lane 1: o >= split)
+  const bool mine = (t == 0) == (o < split);
+  const uint32_t off = t == 0 ? (o < split ? o : split) : (mine ? o - split : 0u);
+  const uint32_t wi = off >> 5, bi = off & 31;
+  uint32_t acc = 0, b = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < 16; ++k) {
+    if (k >= first) {
+      const uint32_t j = k - first;
+      const uint32_t w = d[k];
+      if (j < wi) acc += __popc(w);
+      else if (j == wi) { acc += __popc(w & ((1u << bi) - 1u)); b = (w >> bi) & 1u; }
+    }
+  }
+  b = mine ? b : 0u;
+  acc += __shfl_xor(acc, 1, 2);
+  b |= __shfl_xor(b, 1, 2);
+  *bit = b != 0;
+  return base + acc;
+}
+
+// level_rank with the pair fetch (rank1_pair): both lanes of the pair call it
+// with the same arguments and get the same result.
+template <int Value>
+__device__ __forceinline__ uint64_t level_rank_pair(const LevelView& v, uint64_t index,
+                                                    bool* bit = nullptr) {
+  bool b = false;
+  uint64_t ones;
+  if (index == 0) {
+    ones = 0;
+    if (bit) b = level_bit(v, 0);
+  } else if (index >= v.size) {
+    ones = v.ones;
+    index = v.size;
+  } else {
+    ones = rank1_pair(v, index, &b);
+  }
+  if (bit) *bit = b;
+  return Value ? ones : index - ones;
+}
+
 // Position of the i-th (1-based) Value; size when i exceeds the total
 // (rank_select.cuh:224-523 semantics).  Search: the sample pair brackets
 // the answer; the L1 block is found by binary search inside the bracket
@@ -391,7 +478,10 @@ __device__ __forceinline__ uint64_t tree_node_pos(const TreeView& t,
   return symbol / node_len + prev - 1;
 }
 
-// access: top-down descent (tree/wavelet_tree.cuh:1815-1877)
+// access: top-down descent (tree/wavelet_tree.cuh:1815-1877).  Pair: the
+// two lanes of an adjacent pair descend for the same index together, each
+// fetching half of every level's bits (rank1_pair).
+template <bool Pair = false>
 __device__ __forceinline__ uint64_t tree_access(const TreeView& t,
                                                 const LevelView* lv,
                                                 const uint64_t* counts,
@@ -408,7 +498,8 @@ __device__ __forceinline__ uint64_t tree_access(const TreeView& t,
     }
     const uint64_t start = lo == 0 ? 0 : ranks[rs + tree_node_pos(t, lo, l)];
     bool bit;
-    const uint64_t pos = level_rank<0>(v, cc + index, &bit);
+    const uint64_t pos = Pair ? level_rank_pair<0>(v, cc + index, &bit)
+                              : level_rank<0>(v, cc + index, &bit);
     const uint64_t diff = prev_pow2(hi - lo + 1);
     if (!bit) {
       index = pos - start;
@@ -423,7 +514,8 @@ __device__ __forceinline__ uint64_t tree_access(const TreeView& t,
 }
 
 // rank: rankKernel early-out + top-down descent (wavelet_tree.cuh:481-486,
-// 1896-1956)
+// 1896-1956).  Pair as for tree_access.
+template <bool Pair = false>
 __device__ __forceinline__ uint64_t tree_rank(const TreeView& t,
                                               const LevelView* lv,
                                               const uint64_t* counts,
@@ -437,7 +529,8 @@ __device__ __forceinline__ uint64_t tree_rank(const TreeView& t,
     if (hi == lo) break;
     const uint64_t cc = counts[lo];
     const uint64_t start = lo == 0 ? 0 : ranks[rs + tree_node_pos(t, lo, l)];
-    const uint64_t pos = level_rank<0>(lv[l], cc + result);
+    const uint64_t pos = Pair ? level_rank_pair<0>(lv[l], cc + result)
+                              : level_rank<0>(lv[l], cc + result);
     const uint64_t split = lo + prev_pow2(hi - lo + 1);
     if (symbol < split) {
       result = pos - start;

@@ -119,6 +119,11 @@ ecl_status ecl_release_cached_memory(int device);
  * automatic, up to 8 for large texts).  A tuning knob with no effect on
  * results; no counterpart in the reference. */
 ecl_status ecl_set_build_span(uint32_t tiles_per_workgroup);
+/* Lanes per access/rank query in the tree query kernels: 0 (default,
+ * automatic: 2 over query lines, 1 otherwise), 1, or 2 (the two lanes of a
+ * pair each fetch half of every level's bits).  A tuning knob with no effect
+ * on results; no counterpart in the reference. */
+ecl_status ecl_set_query_lanes(uint32_t lanes);
 ecl_status ecl_device_name(int device, char* buf, size_t len);
 /* events on `stream` (NULL = default stream); elapsed milliseconds */
 ecl_status ecl_event_create(void** event);

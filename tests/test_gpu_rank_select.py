@@ -252,21 +252,41 @@ def test_binary_2e34_bits(gpu, p):
     assert np.array_equal(_rank_from_index(words, l1, l2, pos), k - np.uint64(1))
 
 
-def test_query_lines_binary(gpu):
-    """Binary rank1/rank0 through query lines equal the oracle's scans."""
+@pytest.mark.parametrize("p", [0.3, 0.02, 0.985])
+def test_query_lines_binary(gpu, p):
+    """Binary rank and select of ones and zeros with query lines built equal
+    the oracle's scans and the answers without them, for balanced (0.3),
+    sparse (0.02) and dense (0.985) arrays.  The select arguments include
+    every sample multiple (4096 j), its neighbours, the total and one past
+    it."""
     from ecl_wavelet_amd import BitArray, RankSelect
     m = (1 << 22) + 4321
-    thr = O.bernoulli_threshold(0.3)
-    ba = BitArray([m, 960 * 7, 5])
-    for i in range(3):
+    thr = O.bernoulli_threshold(p)
+    sizes = [m, 960 * 7, 5, 4096 * 3 + 17]
+    ba = BitArray(sizes)
+    for i in range(len(sizes)):
         ba.fillRandom(i, 40 + i, thr)
     rs = RankSelect(ba)
-    words = [rs.bit_array_.words(i) for i in range(3)]
-    sizes = [m, 960 * 7, 5]
-    rs.buildQueryLines()
+    words = [rs.bit_array_.words(i) for i in range(len(sizes))]
     rng = np.random.default_rng(3)
+    idxs, ks = [], []
     for i, mm in enumerate(sizes):
-        idx = np.concatenate([rng.integers(0, mm + 1, 3000), np.arange(0, mm + 1, 960),
-                              np.arange(959, mm, 960), [0, mm]]).astype(np.uint64)
+        idxs.append(np.concatenate([rng.integers(0, mm + 1, 3000), np.arange(0, mm + 1, 960),
+                                    np.arange(959, mm, 960), [0, mm]]).astype(np.uint64))
+        kv = []
         for v in (0, 1):
-            assert np.array_equal(rs.rank(i, idx, v), O.scan_bits_batch(words[i], mm, v, idx))
+            total = rs.getTotalNumVals(i, v)
+            samp = np.arange(4096, total + 1, 4096, dtype=np.int64)
+            k = np.concatenate([rng.integers(1, total + 2, 3000), samp, samp - 1, samp + 1,
+                                [0, 1, total, total + 1]])
+            kv.append(k[k >= 0].astype(np.uint64))
+        ks.append(kv)
+    before = [[rs.select(i, ks[i][v], v) for v in (0, 1)] for i in range(len(sizes))]
+    rs.buildQueryLines()
+    for i, mm in enumerate(sizes):
+        for v in (0, 1):
+            assert np.array_equal(rs.rank(i, idxs[i], v),
+                                  O.scan_bits_batch(words[i], mm, v, idxs[i]))
+            got = rs.select(i, ks[i][v], v)
+            assert np.array_equal(got, O.scan_bits_batch(words[i], mm, 2 + v, ks[i][v]))
+            assert np.array_equal(got, before[i][v])

@@ -486,10 +486,63 @@ def test_query_lines_same_answers(gpu, case):
     ri = np.concatenate([idx, [0, n]]).astype(np.uint64)
     rc = alpha[rng.integers(0, len(alpha), len(ri))]
     q = wt.make_queries(ri, rc)
-    before = (wt.access(idx), wt.rank(q.copy()))
+    # select: every occurrence (and one past) of two symbols, plus random k
+    syms, cnts = np.unique(text, return_counts=True)
+    sk, sc = [], []
+    for c, cnt in list(zip(syms, cnts))[:2]:
+        sk.append(np.arange(1, cnt + 2, dtype=np.uint64))
+        sc.append(np.full(cnt + 1, c, dtype=text.dtype))
+    c3 = syms[rng.integers(0, len(syms), 4000)]
+    sk.append(np.array([int(rng.integers(1, cnts[np.searchsorted(syms, c)] + 2)) for c in c3],
+                       dtype=np.uint64))
+    sc.append(c3.astype(text.dtype))
+    sq = wt.make_queries(np.concatenate(sk), np.concatenate(sc))
+    before = (wt.access(idx), wt.rank(q.copy()), wt.select(sq.copy()))
     wt.buildQueryLines()
     wt.buildQueryLines()  # a second call keeps the lines
     assert np.array_equal(wt.access(idx), before[0])
     assert np.array_equal(wt.rank(q.copy()), before[1])
+    got = wt.select(sq.copy())
+    assert np.array_equal(got, before[2])
+    assert np.array_equal(got, O.scan_select_batch(text, np.concatenate(sk),
+                                                   np.concatenate(sc).astype(np.uint64)))
     assert np.array_equal(wt.access(idx), text[idx])
     _check_queries(wt, text, None, nq=3000, seed=4)
+
+
+@pytest.mark.parametrize("lanes", [0, 1, 2])
+@pytest.mark.parametrize("case", ["u8_256", "u16_1000", "u32_sparse", "tiny"])
+def test_query_lanes(gpu, lanes, case):
+    """access and rank with one or two lanes per query (the pair fetches half
+    of each level's bits per lane; 0 chooses by the presence of query lines)
+    equal the oracle's scans, over the plain directory and over query lines,
+    including positions at block (512) and line (960) boundaries."""
+    from ecl_wavelet_amd import WaveletTree, set_query_lanes
+    rng = np.random.default_rng(lanes * 7 + len(case))
+    if case == "u8_256":
+        text = rng.integers(0, 256, 900_001).astype(np.uint8)
+    elif case == "u16_1000":
+        text = rng.integers(0, 1000, 300_007).astype(np.uint16)
+    elif case == "u32_sparse":
+        alpha = np.sort(rng.choice(1 << 31, 37, replace=False)).astype(np.uint32)
+        text = alpha[rng.integers(0, 37, 200_003)]
+    else:
+        text = np.array([3, 1, 4, 1, 5, 9, 2, 6, 5], dtype=np.uint8)
+    n = len(text)
+    set_query_lanes(lanes)
+    try:
+        wt = WaveletTree(text, np.unique(text))
+        idx = np.concatenate([rng.integers(0, n, 4000), np.arange(0, n, 512),
+                              np.arange(511, n, 512), np.arange(0, n, 960),
+                              np.arange(959, n, 960), [0, n - 1]]).astype(np.uint64)
+        ri = np.concatenate([idx, [0, n]]).astype(np.uint64)
+        syms = np.unique(text)
+        rc = syms[rng.integers(0, len(syms), len(ri))]
+        exp_r = O.scan_rank_batch(text, np.minimum(ri, n), rc.astype(np.uint64))
+        for lines in (False, True):
+            if lines:
+                wt.buildQueryLines()
+            assert np.array_equal(wt.access(idx), text[idx])
+            assert np.array_equal(wt.rank(wt.make_queries(ri, rc)), exp_r)
+    finally:
+        set_query_lanes(0)
