@@ -1245,27 +1245,43 @@ static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_
 }
 
 // Query lines for every array of a directory (an opt-in copy of the bits,
-// 128/120 of their size, that lets a rank read one line); the level views
-// point to them afterwards.
+// 128/120 of their size, that lets a rank read one line) and the select
+// hints (64 B per 4096 ones or zeros, that let a select read its hint and
+// one block); the level views point to them afterwards.
 static void build_query_lines(ecl_rank_select_t* rs) {
   if (rs->lines.ptr != nullptr) return;
   const size_t na = rs->levels.size();
-  std::vector<uint64_t> off(na), nl(na);
-  uint64_t total = 0;
+  std::vector<uint64_t> off(na), nl(na), hoff(na), nh1(na), nh0(na);
+  uint64_t total = 0, htotal = 0;
   for (size_t i = 0; i < na; ++i) {
-    nl[i] = (rs->levels[i].size + detail::kLineBits - 1) / detail::kLineBits;
+    const LevelView& v = rs->levels[i];
+    nl[i] = (v.size + detail::kLineBits - 1) / detail::kLineBits;
     off[i] = total;
     total += nl[i];
+    // one select hint per sample bracket of each Value (none for an empty array)
+    nh1[i] = v.size ? v.ones / detail::kSampleRate + 1 : 0;
+    nh0[i] = v.size ? (v.size - v.ones) / detail::kSampleRate + 1 : 0;
+    hoff[i] = htotal;
+    htotal += nh1[i] + nh0[i];
   }
-  if (total * 128 + (64u << 20) > free_device_memory())
+  // lines [total x 128 B] | select hints [htotal x 64 B]
+  const uint64_t bytes = total * 128 + htotal * 64;
+  if (bytes + (64u << 20) > free_device_memory())
     throw Error(ECL_ERROR_OUT_OF_MEMORY_TREE, kOomTree);
-  DevBuf lines(std::max<uint64_t>(total, 1) * 128);
+  DevBuf lines(std::max<uint64_t>(bytes, 1));
+  uint4* L = lines.as<uint4>();
+  uint4* H = L + total * 8;
   hipStream_t s = nullptr;
-  for (size_t i = 0; i < na; ++i)
-    launch_query_lines(rs->levels[i], lines.as<uint4>() + off[i] * 8, nl[i], s);
+  for (size_t i = 0; i < na; ++i) {
+    launch_query_lines(rs->levels[i], L + off[i] * 8, nl[i], s);
+    launch_select_hints(rs->levels[i], H + hoff[i] * 4, nh1[i], nh0[i], s);
+  }
   ECL_HIP(hipStreamSynchronize(s));
   std::vector<LevelView> lv = rs->levels;
-  for (size_t i = 0; i < na; ++i) lv[i].lines = nl[i] ? lines.as<uint4>() + off[i] * 8 : nullptr;
+  for (size_t i = 0; i < na; ++i) {
+    lv[i].lines = nl[i] ? L + off[i] * 8 : nullptr;
+    lv[i].sel_hints = nh1[i] + nh0[i] ? H + hoff[i] * 4 : nullptr;
+  }
   h2d(rs->d_levels.ptr, lv.data(), na);
   rs->levels = std::move(lv);
   rs->lines = std::move(lines);

@@ -71,6 +71,8 @@ void launch_upload(const UploadBlob& b, hipStream_t s);
 // query lines of one array (detail::LevelView::lines): nlines =
 // ceil(size / kLineBits) lines of 128 bytes at `lines`
 void launch_query_lines(const LevelView& v, uint4* lines, uint64_t nlines, hipStream_t s);
+void launch_select_hints(const LevelView& v, uint4* hints, uint64_t nh1, uint64_t nh0,
+                         hipStream_t s);
 
 // ---- launchers (kernels_build.hip) ------------------------------------------
 void launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t s);

@@ -410,12 +410,59 @@ uint32_t grid_for(uint64_t work, uint32_t block, uint32_t max_blocks) {
 
 }  // namespace
 
+// One thread per select hint of one array (layout at LevelView::sel_hints):
+// hints [0, nh1) for the ones, [nh1, nh1 + nh0) for the zeros, from the
+// array's samples and directory.
+__global__ __launch_bounds__(256) void select_hints_kernel(LevelView v, uint4* __restrict__ hints,
+                                                           uint64_t nh1, uint64_t nh0) {
+  const uint64_t t = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (t >= nh1 + nh0) return;
+  const bool one = t < nh1;
+  const uint64_t j = one ? t : t - nh1;
+  const uint64_t total = one ? v.ones : v.size - v.ones;
+  const uint64_t nsamp = total / detail::kSampleRate;
+  const uint64_t lo = j == 0 ? 0 : (one ? v.samples[j - 1] : v.samples[v.sample_cap - j]);
+  const uint64_t hi = j < nsamp ? (one ? v.samples[j] : v.samples[v.sample_cap - 1 - j])
+                                : v.size - 1;
+  const uint64_t b0 = lo >> 9;
+  const uint64_t span = (hi >> 9) - b0;
+  auto before = [&](uint64_t blk) -> uint64_t {  // Values before block blk of the array
+    const uint64_t ones = v.l1[blk >> 7] + v.l2[blk];
+    return one ? ones : (blk << 9) - ones;
+  };
+  const uint64_t c0 = before(b0);
+  uint32_t e[24];
+  e[0] = span > detail::kHintBlocks ? 0xFFFFu : static_cast<uint32_t>(span);
+  for (uint32_t u = 1; u < 24; ++u)
+    e[u] = u <= span ? static_cast<uint32_t>(before(b0 + u) - c0) : 0xFFFFu;
+  uint4* o = hints + t * 4;
+  o[0] = make_uint4(static_cast<uint32_t>(lo), static_cast<uint32_t>(lo >> 32),
+                    static_cast<uint32_t>(c0), static_cast<uint32_t>(c0 >> 32));
+  for (int k = 0; k < 3; ++k)
+    o[1 + k] = make_uint4(e[8 * k] | (e[8 * k + 1] << 16), e[8 * k + 2] | (e[8 * k + 3] << 16),
+                          e[8 * k + 4] | (e[8 * k + 5] << 16), e[8 * k + 6] | (e[8 * k + 7] << 16));
+}
+
+void launch_select_hints(const LevelView& v, uint4* hints, uint64_t nh1, uint64_t nh0,
+                         hipStream_t s) {
+  if (nh1 + nh0 == 0) return;
+  const uint64_t blocks = (nh1 + nh0 + 255) / 256;
+  if (blocks > 0x7FFFFFFFull) throw Error(ECL_ERROR_INVALID_ARGUMENT, "bit array too large");
+  LevelView plain = v;
+  plain.lines = nullptr;
+  plain.sel_hints = nullptr;
+  hipLaunchKernelGGL(select_hints_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(256), 0, s,
+                     plain, hints, nh1, nh0);
+  ECL_HIP(hipGetLastError());
+}
+
 void launch_query_lines(const LevelView& v, uint4* lines, uint64_t nlines, hipStream_t s) {
   if (nlines == 0) return;
   const uint64_t blocks = (nlines * 8 + 255) / 256;
   if (blocks > 0x7FFFFFFFull) throw Error(ECL_ERROR_INVALID_ARGUMENT, "bit array too large");
   LevelView plain = v;
   plain.lines = nullptr;  // headers come from the directory
+  plain.sel_hints = nullptr;
   hipLaunchKernelGGL(query_lines_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(256), 0, s,
                      plain, lines, nlines);
   ECL_HIP(hipGetLastError());

@@ -159,8 +159,18 @@ struct LevelView {
   // u64 count of ones before the line followed by the line's 960 bits, so a
   // rank reads one line instead of the bit block, its L2 and its L1 entry.
   const uint4* lines;
+  // With the lines: one 64-byte select hint per sample bracket, ones then
+  // zeros (hint j of a Value covers the bracket from its j-th sample, or bit
+  // 0, to the next sample, or the last bit):
+  //   u64 position of sample j (0 for j = 0), u64 Values before the 512-bit
+  //   block holding it (block b0), then 24 u16: the bracket's last block
+  //   minus b0 (0xFFFF above kHintBlocks), and for u = 1..23 the Values
+  //   before block b0 + u minus the u64 count (0xFFFF past the bracket).
+  // A select then reads its hint and the one block that holds the answer.
+  const uint4* sel_hints;
 };
 constexpr uint32_t kLineBits = 960;  // bits per query line (30 words)
+constexpr uint32_t kHintBlocks = 23;  // blocks after b0 one hint covers
 
 __device__ __forceinline__ bool level_bit(const LevelView& v, uint64_t i) {
   return (v.words[i >> 5] >> (i & 31)) & 1u;
@@ -347,6 +357,38 @@ __device__ __forceinline__ uint64_t level_rank_pair(const LevelView& v, uint64_t
 // in-word.  WideL2 shortens the dependent chain of one select (binary
 // rank/select, one level per query) but issues more instructions, which
 // costs the tree descents and mixed batches more than it saves.
+// Position of the r-th (1-based) Value inside 512-bit block blk, which
+// holds it: the 64 B block is loaded at once and the bit selected in-word.
+template <int Value>
+__device__ __forceinline__ uint64_t select_in_block(const LevelView& v, uint64_t blk2,
+                                                   uint64_t r) {
+  // the 64 B block lies inside the array's 128 B padding
+  const uint4* p = reinterpret_cast<const uint4*>(v.words + (blk2 << 4));
+  const uint4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3];
+  const uint64_t w[8] = {
+      (uint64_t)q0.x | ((uint64_t)q0.y << 32), (uint64_t)q0.z | ((uint64_t)q0.w << 32),
+      (uint64_t)q1.x | ((uint64_t)q1.y << 32), (uint64_t)q1.z | ((uint64_t)q1.w << 32),
+      (uint64_t)q2.x | ((uint64_t)q2.y << 32), (uint64_t)q2.z | ((uint64_t)q2.w << 32),
+      (uint64_t)q3.x | ((uint64_t)q3.y << 32), (uint64_t)q3.z | ((uint64_t)q3.w << 32)};
+  uint64_t pos = blk2 << 9, res = v.size;
+  bool found = false;
+#pragma unroll
+  for (uint32_t k = 0; k < 8; ++k) {
+    const uint64_t x = Value ? w[k] : ~w[k];
+    const uint32_t c = static_cast<uint32_t>(__popcll(x));
+    if (!found) {
+      if (r <= c) {
+        res = pos + select_in_word(x, static_cast<uint32_t>(r));
+        found = true;
+      } else {
+        r -= c;
+        pos += 64;
+      }
+    }
+  }
+  return res;  // size only for an inconsistent directory
+}
+
 template <int Value, bool WideL2 = false>
 __device__ __forceinline__ uint64_t level_select(const LevelView& v,
                                                  uint64_t i) {
@@ -354,6 +396,32 @@ __device__ __forceinline__ uint64_t level_select(const LevelView& v,
   if (i == 0 || i > total) return v.size;
   const uint64_t j = i / kSampleRate;
   const uint64_t nsamp = total / kSampleRate;
+  if (v.sel_hints != nullptr) {
+    // the bracket's hint: sample position, Values before its first block,
+    // and the counts before the next kHintBlocks blocks
+    const uint4* h = v.sel_hints + (Value ? j : v.ones / kSampleRate + 1 + j) * 4;
+    const uint4 h0 = h[0], h1 = h[1], h2 = h[2], h3 = h[3];
+    const uint64_t lo_pos = (uint64_t)h0.x | ((uint64_t)h0.y << 32);
+    if (j > 0 && i == j * kSampleRate) return lo_pos;
+    const uint32_t span = h1.x & 0xFFFFu;
+    if (span <= kHintBlocks) {
+      const uint64_t c0 = (uint64_t)h0.z | ((uint64_t)h0.w << 32);
+      const uint32_t hw[12] = {h1.x, h1.y, h1.z, h1.w, h2.x, h2.y, h2.z, h2.w,
+                               h3.x, h3.y, h3.z, h3.w};
+      // blocks b0 + u (u <= span) with fewer than i Values before them; the
+      // counts grow with u, so the last such block holds the answer
+      uint32_t below = 0, eb = 0;
+#pragma unroll
+      for (uint32_t u = 1; u <= kHintBlocks; ++u) {
+        const uint32_t e = (hw[u >> 1] >> ((u & 1) * 16)) & 0xFFFFu;
+        if (u <= span && c0 + e < i) {
+          below = u;
+          eb = e;
+        }
+      }
+      return select_in_block<Value>(v, (lo_pos >> 9) + below, i - c0 - eb);
+    }
+  }
   uint64_t lo_pos = 0, hi_pos = v.size - 1;
   if (j > 0) {
     lo_pos = Value ? v.samples[j - 1] : v.samples[v.sample_cap - j];
@@ -398,31 +466,7 @@ __device__ __forceinline__ uint64_t level_select(const LevelView& v,
     blk2 = k_lo;
   }
   r -= Value ? v.l2[blk2] : ((blk2 - first) << 9) - v.l2[blk2];
-  // the 64 B block lies inside the array's 128 B padding
-  const uint4* p = reinterpret_cast<const uint4*>(v.words + (blk2 << 4));
-  const uint4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3];
-  const uint64_t w[8] = {
-      (uint64_t)q0.x | ((uint64_t)q0.y << 32), (uint64_t)q0.z | ((uint64_t)q0.w << 32),
-      (uint64_t)q1.x | ((uint64_t)q1.y << 32), (uint64_t)q1.z | ((uint64_t)q1.w << 32),
-      (uint64_t)q2.x | ((uint64_t)q2.y << 32), (uint64_t)q2.z | ((uint64_t)q2.w << 32),
-      (uint64_t)q3.x | ((uint64_t)q3.y << 32), (uint64_t)q3.z | ((uint64_t)q3.w << 32)};
-  uint64_t pos = blk2 << 9, res = v.size;
-  bool found = false;
-#pragma unroll
-  for (uint32_t k = 0; k < 8; ++k) {
-    const uint64_t x = Value ? w[k] : ~w[k];
-    const uint32_t c = static_cast<uint32_t>(__popcll(x));
-    if (!found) {
-      if (r <= c) {
-        res = pos + select_in_word(x, static_cast<uint32_t>(r));
-        found = true;
-      } else {
-        r -= c;
-        pos += 64;
-      }
-    }
-  }
-  return res;  // size only for an inconsistent directory
+  return select_in_block<Value>(v, blk2, r);
 }
 
 // ---------------------------------------------------------------------------

@@ -254,11 +254,13 @@ def test_binary_2e34_bits(gpu, p):
 
 @pytest.mark.parametrize("p", [0.3, 0.02, 0.985])
 def test_query_lines_binary(gpu, p):
-    """Binary rank and select of ones and zeros with query lines built equal
-    the oracle's scans and the answers without them, for balanced (0.3),
-    sparse (0.02) and dense (0.985) arrays.  The select arguments include
-    every sample multiple (4096 j), its neighbours, the total and one past
-    it."""
+    """Binary rank and select of ones and zeros with query lines and select
+    hints built equal the oracle's scans and the answers without them, for
+    balanced (0.3), sparse (0.02) and dense (0.985) arrays: at 0.3 every
+    sample bracket fits one hint (at most 24 blocks); the rare Value of the
+    sparse and dense arrays has brackets wider than that and takes the
+    directory search.  The select arguments include every sample multiple
+    (4096 j), its neighbours, the total and one past it."""
     from ecl_wavelet_amd import BitArray, RankSelect
     m = (1 << 22) + 4321
     thr = O.bernoulli_threshold(p)
