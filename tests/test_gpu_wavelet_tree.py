@@ -515,8 +515,9 @@ def test_query_lines_same_answers(gpu, case):
 def test_query_lanes(gpu, lanes, case):
     """access and rank with one or two lanes per query (the pair fetches half
     of each level's bits per lane; 0 chooses by the presence of query lines)
-    equal the oracle's scans, over the plain directory and over query lines,
-    including positions at block (512) and line (960) boundaries."""
+    and select (one lane) equal the oracle's scans, over the plain directory
+    and over query lines with select hints, including positions at block
+    (512) and line (960) boundaries and every occurrence of one symbol."""
     from ecl_wavelet_amd import WaveletTree, set_query_lanes
     rng = np.random.default_rng(lanes * 7 + len(case))
     if case == "u8_256":
@@ -539,10 +540,20 @@ def test_query_lanes(gpu, lanes, case):
         syms = np.unique(text)
         rc = syms[rng.integers(0, len(syms), len(ri))]
         exp_r = O.scan_rank_batch(text, np.minimum(ri, n), rc.astype(np.uint64))
+        # select: every occurrence (and one past) of one symbol, plus random k
+        cnt = {int(a): int(c) for a, c in zip(*np.unique(text, return_counts=True))}
+        s0 = syms[0]
+        sc = np.concatenate([np.full(cnt[int(s0)] + 1, s0), syms[rng.integers(0, len(syms), 3000)]])
+        sk = np.concatenate([np.arange(1, cnt[int(s0)] + 2),
+                             [int(rng.integers(1, cnt[int(c)] + 2)) for c in sc[cnt[int(s0)] + 1:]]])
+        sc = sc.astype(text.dtype)
+        sk = sk.astype(np.uint64)
+        exp_s = O.scan_select_batch(text, sk, sc.astype(np.uint64))
         for lines in (False, True):
             if lines:
                 wt.buildQueryLines()
             assert np.array_equal(wt.access(idx), text[idx])
             assert np.array_equal(wt.rank(wt.make_queries(ri, rc)), exp_r)
+            assert np.array_equal(wt.select(wt.make_queries(sk, sc)), exp_s)
     finally:
         set_query_lanes(0)
