@@ -239,8 +239,12 @@ ecl_status ecl_wavelet_tree_destroy(ecl_wavelet_tree wt);
  * structure): a second copy of every level's bits in 128-byte lines of 960
  * bits, each headed by the count of ones before it, so that rank and access
  * read one memory line per level instead of three (bit block, L2 and L1
- * entry).  Costs 128/120 of the bits in memory and one pass over them;
- * results do not change.  Select keeps the reference layout. */
+ * entry); and one 64-byte select hint per select-sample bracket (the
+ * sample's position and the counts before the bracket's first 24 blocks),
+ * so that a select reads its hint and one block.  Costs 128/120 + 1/64 of
+ * the bits in memory and one pass over them; results do not change.  The
+ * reference-shaped directory and samples stay as they are.  A second call
+ * keeps the existing lines. */
 ecl_status ecl_wavelet_tree_build_query_lines(ecl_wavelet_tree wt);
 ecl_status ecl_rank_select_build_query_lines(ecl_rank_select rs);
 /* Persistence (no counterpart in the reference, which rebuilds every tree
