@@ -347,7 +347,8 @@ def bench_binary(args):
     """Binary RankSelect configuration: 2^bin_log2 i.i.d. Bernoulli(p) bits
     drawn on the device, p in {0.5, 0.05}; directory build time, then
     bin_queries rank1(i) with uniform i and select1(k) with uniform k over
-    the popcount, generated on the device (timed with their generation)."""
+    the popcount, generated on the device (timed with their generation);
+    then the same queries over the query lines and select hints."""
     import ecl_wavelet_amd as E
     m = 1 << args.bin_log2
     Q = args.bin_queries
@@ -376,6 +377,19 @@ def bench_binary(args):
         timer.start()
         rs.selectGenerated(0, 12, Q, out, 1)
         select_ms = timer.stop()
+        # the same queries over the query lines and select hints
+        timer.start()
+        rs.buildQueryLines()
+        lines_ms = timer.stop()
+        rs.rankGenerated(0, 1, Q, out, 1)  # warm-up
+        rs.selectGenerated(0, 1, Q, out, 1)
+        E.synchronize()
+        timer.start()
+        rs.rankGenerated(0, 11, Q, out, 1)
+        rank_l_ms = timer.stop()
+        timer.start()
+        rs.selectGenerated(0, 12, Q, out, 1)
+        select_l_ms = timer.stop()
         r = {
             "build_ms": build_ms,
             "build_gbps": m / 8 / (build_ms / 1e3) / 1e9,
@@ -385,10 +399,14 @@ def bench_binary(args):
             # 256+16 B select1)
             "rank1_line_gbps": Q * 144 / (rank_ms / 1e3) / 1e9,
             "select1_line_gbps": Q * 272 / (select_ms / 1e3) / 1e9,
+            "query_lines_build_ms": lines_ms,
+            "rank1_qps_query_lines": Q / (rank_l_ms / 1e3),
+            "select1_qps_query_lines": Q / (select_l_ms / 1e3),
         }
         res[f"p={p}"] = r
         log(f"binary p={p}: build {build_ms:.2f} ms, rank1 {r['rank1_qps']:.3g} q/s, "
-            f"select1 {r['select1_qps']:.3g} q/s")
+            f"select1 {r['select1_qps']:.3g} q/s; lines {lines_ms:.2f} ms, rank1 "
+            f"{r['rank1_qps_query_lines']:.3g} q/s, select1 {r['select1_qps_query_lines']:.3g} q/s")
         del rs
     del out
     return res

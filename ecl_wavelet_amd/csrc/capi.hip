@@ -1272,15 +1272,24 @@ static void build_query_lines(ecl_rank_select_t* rs) {
   uint4* L = lines.as<uint4>();
   uint4* H = L + total * 8;
   hipStream_t s = nullptr;
+  const size_t fit_bytes = std::max<size_t>(na, 1) * 2 * sizeof(unsigned long long);
+  DevBuf fit(fit_bytes);
+  ECL_HIP(hipMemsetAsync(fit.ptr, 0, fit_bytes, s));
   for (size_t i = 0; i < na; ++i) {
     launch_query_lines(rs->levels[i], L + off[i] * 8, nl[i], s);
-    launch_select_hints(rs->levels[i], H + hoff[i] * 4, nh1[i], nh0[i], s);
+    launch_select_hints(rs->levels[i], H + hoff[i] * 4, nh1[i], nh0[i],
+                        fit.as<unsigned long long>() + 2 * i, s);
   }
+  std::vector<unsigned long long> fits(std::max<size_t>(na, 1) * 2);
+  ECL_HIP(hipMemcpyAsync(fits.data(), fit.ptr, fit_bytes, hipMemcpyDeviceToHost, s));
   ECL_HIP(hipStreamSynchronize(s));
   std::vector<LevelView> lv = rs->levels;
   for (size_t i = 0; i < na; ++i) {
     lv[i].lines = nl[i] ? L + off[i] * 8 : nullptr;
     lv[i].sel_hints = nh1[i] + nh0[i] ? H + hoff[i] * 4 : nullptr;
+    // a Value reads the hints where at least half of its brackets fit one
+    lv[i].hint_values = (2 * fits[2 * i + 1] >= nh1[i] ? 2u : 0u) |
+                        (2 * fits[2 * i] >= nh0[i] ? 1u : 0u);
   }
   h2d(rs->d_levels.ptr, lv.data(), na);
   rs->levels = std::move(lv);
@@ -1794,7 +1803,7 @@ ecl_status ecl_rank_select_rank_device(ecl_rank_select rs, size_t i, int value,
     require(rs && i < rs->ba->sizes.size(), "invalid argument");
     set_device(rs->device);
     launch_rs_rank(rs->d_levels.as<LevelView>() + i, value, d_index, m, d_out, 0, 0,
-                   as_stream(stream));
+                   rs->lines.ptr != nullptr, as_stream(stream));
   });
 }
 ecl_status ecl_rank_select_select_device(ecl_rank_select rs, size_t i, int value,
@@ -1818,7 +1827,7 @@ static void rs_host(ecl_rank_select rs, size_t i, int value, const uint64_t* in,
     h2d(din.ptr, in + off, c);
     if (is_rank)
       launch_rs_rank(rs->d_levels.as<LevelView>() + i, value, din.as<uint64_t>(), c,
-                     dout.as<uint64_t>(), 0, 0, nullptr);
+                     dout.as<uint64_t>(), 0, 0, rs->lines.ptr != nullptr, nullptr);
     else
       launch_rs_select(rs->d_levels.as<LevelView>() + i, value, din.as<uint64_t>(), c,
                        dout.as<uint64_t>(), 0, 0, nullptr);
@@ -1840,7 +1849,7 @@ ecl_status ecl_rank_select_rank_generated(ecl_rank_select rs, size_t i, int valu
     require(rs && i < rs->ba->sizes.size(), "invalid argument");
     set_device(rs->device);
     launch_rs_rank(rs->d_levels.as<LevelView>() + i, value, nullptr, m, d_out, 1, seed,
-                   as_stream(stream));
+                   rs->lines.ptr != nullptr, as_stream(stream));
   });
 }
 ecl_status ecl_rank_select_select_generated(ecl_rank_select rs, size_t i, int value,

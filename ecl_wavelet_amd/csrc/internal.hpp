@@ -72,7 +72,7 @@ void launch_upload(const UploadBlob& b, hipStream_t s);
 // ceil(size / kLineBits) lines of 128 bytes at `lines`
 void launch_query_lines(const LevelView& v, uint4* lines, uint64_t nlines, hipStream_t s);
 void launch_select_hints(const LevelView& v, uint4* hints, uint64_t nh1, uint64_t nh0,
-                         hipStream_t s);
+                         unsigned long long* fit, hipStream_t s);  // fit[2] zeroed by the caller
 
 // ---- launchers (kernels_build.hip) ------------------------------------------
 void launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t s);
@@ -204,7 +204,7 @@ void launch_forest_gen(const TreeView* views, uint64_t num_trees, const uint64_t
                        uint64_t seed, uint64_t m, ecl_forest_query* out, hipStream_t s);
 void launch_rs_rank(const LevelView* lv, int value, const uint64_t* idx,
                     uint64_t m, uint64_t* out, int generated, uint64_t seed,
-                    hipStream_t s);
+                    int has_lines, hipStream_t s);
 void launch_rs_select(const LevelView* lv, int value, const uint64_t* k,
                       uint64_t m, uint64_t* out, int generated, uint64_t seed,
                       hipStream_t s);

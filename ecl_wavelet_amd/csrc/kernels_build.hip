@@ -413,9 +413,31 @@ uint32_t grid_for(uint64_t work, uint32_t block, uint32_t max_blocks) {
 // One thread per select hint of one array (layout at LevelView::sel_hints):
 // hints [0, nh1) for the ones, [nh1, nh1 + nh0) for the zeros, from the
 // array's samples and directory.
+// fit[1] / fit[0] count the hints of the ones / zeros whose bracket fits
+// (at most kHintBlocks blocks after b0).
 __global__ __launch_bounds__(256) void select_hints_kernel(LevelView v, uint4* __restrict__ hints,
-                                                           uint64_t nh1, uint64_t nh0) {
+                                                           uint64_t nh1, uint64_t nh0,
+                                                           unsigned long long* __restrict__ fit) {
   const uint64_t t = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x;
+  // per wave: how many of its hints fit, for each Value (one atomic per wave and Value)
+  bool fits1 = false, fits0 = false;
+  if (t < nh1 + nh0) {
+    const bool one = t < nh1;
+    const uint64_t j = one ? t : t - nh1;
+    const uint64_t total = one ? v.ones : v.size - v.ones;
+    const uint64_t nsamp = total / detail::kSampleRate;
+    const uint64_t lo = j == 0 ? 0 : (one ? v.samples[j - 1] : v.samples[v.sample_cap - j]);
+    const uint64_t hi = j < nsamp ? (one ? v.samples[j] : v.samples[v.sample_cap - 1 - j])
+                                  : v.size - 1;
+    const bool fits = (hi >> 9) - (lo >> 9) <= detail::kHintBlocks;
+    fits1 = one && fits;
+    fits0 = !one && fits;
+  }
+  const uint64_t m1 = __ballot(fits1), m0 = __ballot(fits0);
+  if ((threadIdx.x & 63) == 0) {
+    if (m1) atomicAdd(&fit[1], static_cast<unsigned long long>(__popcll(m1)));
+    if (m0) atomicAdd(&fit[0], static_cast<unsigned long long>(__popcll(m0)));
+  }
   if (t >= nh1 + nh0) return;
   const bool one = t < nh1;
   const uint64_t j = one ? t : t - nh1;
@@ -444,7 +466,7 @@ __global__ __launch_bounds__(256) void select_hints_kernel(LevelView v, uint4* _
 }
 
 void launch_select_hints(const LevelView& v, uint4* hints, uint64_t nh1, uint64_t nh0,
-                         hipStream_t s) {
+                         unsigned long long* fit, hipStream_t s) {
   if (nh1 + nh0 == 0) return;
   const uint64_t blocks = (nh1 + nh0 + 255) / 256;
   if (blocks > 0x7FFFFFFFull) throw Error(ECL_ERROR_INVALID_ARGUMENT, "bit array too large");
@@ -452,7 +474,7 @@ void launch_select_hints(const LevelView& v, uint4* hints, uint64_t nh1, uint64_
   plain.lines = nullptr;
   plain.sel_hints = nullptr;
   hipLaunchKernelGGL(select_hints_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(256), 0, s,
-                     plain, hints, nh1, nh0);
+                     plain, hints, nh1, nh0, fit);
   ECL_HIP(hipGetLastError());
 }
 

@@ -395,17 +395,21 @@ __global__ void forest_gen_kernel(const TreeView* __restrict__ views, uint64_t n
   }
 }
 
-template <int Value>
+// G lanes per query as in tree_access_kernel (2: the pair fetch over query
+// lines).
+template <int Value, int G>
 __global__ __launch_bounds__(256) void rs_rank_kernel(
     const LevelView* lvp, const uint64_t* __restrict__ idx, uint64_t m,
     uint64_t* __restrict__ out, int generated, uint64_t seed) {
   const LevelView v = *lvp;
-  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
-  for (uint64_t q = blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x / G;
+  for (uint64_t q = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) / G;
        q < m; q += stride) {
     const uint64_t i =
         generated ? detail::bounded64(detail::hash64(seed, q), v.size) : idx[q];
-    out[q] = detail::level_rank<Value>(v, i);
+    const uint64_t r = G == 2 ? detail::level_rank_pair<Value>(v, i)
+                              : detail::level_rank<Value>(v, i);
+    if (G == 1 || (threadIdx.x & 1) == 0) out[q] = r;
   }
 }
 
@@ -464,10 +468,12 @@ size_t meta_bytes(const TreeView& t) {
 // (5.7 vs 3.8 ms per 10^7 at n = 6e9, sigma = 256).
 std::atomic<uint32_t> g_query_lanes{0};
 
-bool pair_lanes(const TreeQueryParams& p) {
+bool pair_lanes_for(int has_lines) {
   const uint32_t l = g_query_lanes.load();
-  return l == 0 ? p.has_lines != 0 : l == 2;
+  return l == 0 ? has_lines != 0 : l == 2;
 }
+
+bool pair_lanes(const TreeQueryParams& p) { return pair_lanes_for(p.has_lines); }
 
 template <int G>
 void access_g(const TreeQueryParams& p, const uint64_t* idx, uint64_t m, void* out, hipStream_t s) {
@@ -584,14 +590,22 @@ void launch_tree_gen_queries(const TreeView& v, const uint64_t*, int kind,
 
 void launch_rs_rank(const LevelView* lv, int value, const uint64_t* idx,
                     uint64_t m, uint64_t* out, int generated, uint64_t seed,
-                    hipStream_t s) {
+                    int has_lines, hipStream_t s) {
   if (m == 0) return;
   ProfScope ps(K_RS_RANK, s);
-  const dim3 g(query_grid(m)), b(256);
-  if (value)
-    hipLaunchKernelGGL(rs_rank_kernel<1>, g, b, 0, s, lv, idx, m, out, generated, seed);
-  else
-    hipLaunchKernelGGL(rs_rank_kernel<0>, g, b, 0, s, lv, idx, m, out, generated, seed);
+  if (pair_lanes_for(has_lines)) {
+    const dim3 g(query_grid(2 * m)), b(256);
+    if (value)
+      hipLaunchKernelGGL((rs_rank_kernel<1, 2>), g, b, 0, s, lv, idx, m, out, generated, seed);
+    else
+      hipLaunchKernelGGL((rs_rank_kernel<0, 2>), g, b, 0, s, lv, idx, m, out, generated, seed);
+  } else {
+    const dim3 g(query_grid(m)), b(256);
+    if (value)
+      hipLaunchKernelGGL((rs_rank_kernel<1, 1>), g, b, 0, s, lv, idx, m, out, generated, seed);
+    else
+      hipLaunchKernelGGL((rs_rank_kernel<0, 1>), g, b, 0, s, lv, idx, m, out, generated, seed);
+  }
   ECL_HIP(hipGetLastError());
 }
 

@@ -168,6 +168,10 @@ struct LevelView {
   //   before block b0 + u minus the u64 count (0xFFFF past the bracket).
   // A select then reads its hint and the one block that holds the answer.
   const uint4* sel_hints;
+  // bit v: a select of Value v reads the hints (set where at least half of
+  // that Value's brackets fit one hint; otherwise every select would read a
+  // hint only to fall back to the directory search)
+  uint32_t hint_values;
 };
 constexpr uint32_t kLineBits = 960;  // bits per query line (30 words)
 constexpr uint32_t kHintBlocks = 23;  // blocks after b0 one hint covers
@@ -396,7 +400,7 @@ __device__ __forceinline__ uint64_t level_select(const LevelView& v,
   if (i == 0 || i > total) return v.size;
   const uint64_t j = i / kSampleRate;
   const uint64_t nsamp = total / kSampleRate;
-  if (v.sel_hints != nullptr) {
+  if (v.sel_hints != nullptr && ((v.hint_values >> Value) & 1u)) {
     // the bracket's hint: sample position, Values before its first block,
     // and the counts before the next kHintBlocks blocks
     const uint4* h = v.sel_hints + (Value ? j : v.ones / kSampleRate + 1 + j) * 4;
