@@ -1264,34 +1264,41 @@ static void build_query_lines(ecl_rank_select_t* rs) {
     hoff[i] = htotal;
     htotal += nh1[i] + nh0[i];
   }
-  // lines [total x 128 B] | select hints [htotal x 64 B]
+  // lines [total x 128 B] | select hints [htotal x 64 B]; scratch for the
+  // hints' fit counts (2 u32 per hint workgroup of the largest array)
   const uint64_t bytes = total * 128 + htotal * 64;
-  if (bytes + (64u << 20) > free_device_memory())
+  uint64_t fit_words = 0;
+  for (size_t i = 0; i < na; ++i)
+    fit_words = std::max<uint64_t>(fit_words, 2 * select_hint_blocks(nh1[i], nh0[i]));
+  if (bytes + fit_words * 4 + (64u << 20) > free_device_memory())
     throw Error(ECL_ERROR_OUT_OF_MEMORY_TREE, kOomTree);
   DevBuf lines(std::max<uint64_t>(bytes, 1));
+  DevBuf fit(std::max<uint64_t>(fit_words, 1) * 4);
   uint4* L = lines.as<uint4>();
   uint4* H = L + total * 8;
-  hipStream_t s = nullptr;
-  const size_t fit_bytes = std::max<size_t>(na, 1) * 2 * sizeof(unsigned long long);
-  DevBuf fit(fit_bytes);
-  ECL_HIP(hipMemsetAsync(fit.ptr, 0, fit_bytes, s));
-  for (size_t i = 0; i < na; ++i) {
-    launch_query_lines(rs->levels[i], L + off[i] * 8, nl[i], s);
-    launch_select_hints(rs->levels[i], H + hoff[i] * 4, nh1[i], nh0[i],
-                        fit.as<unsigned long long>() + 2 * i, s);
-  }
-  std::vector<unsigned long long> fits(std::max<size_t>(na, 1) * 2);
-  ECL_HIP(hipMemcpyAsync(fits.data(), fit.ptr, fit_bytes, hipMemcpyDeviceToHost, s));
-  ECL_HIP(hipStreamSynchronize(s));
+  // the views point to the lines and hints first; each array's hint kernels
+  // then set its hint_values in the device view
   std::vector<LevelView> lv = rs->levels;
   for (size_t i = 0; i < na; ++i) {
     lv[i].lines = nl[i] ? L + off[i] * 8 : nullptr;
     lv[i].sel_hints = nh1[i] + nh0[i] ? H + hoff[i] * 4 : nullptr;
-    // a Value reads the hints where at least half of its brackets fit one
-    lv[i].hint_values = (2 * fits[2 * i + 1] >= nh1[i] ? 2u : 0u) |
-                        (2 * fits[2 * i] >= nh0[i] ? 1u : 0u);
+    lv[i].hint_values = 0;
   }
-  h2d(rs->d_levels.ptr, lv.data(), na);
+  hipStream_t s = nullptr;
+  ECL_HIP(hipMemcpyAsync(rs->d_levels.ptr, lv.data(), na * sizeof(LevelView),
+                         hipMemcpyHostToDevice, s));
+  for (size_t i = 0; i < na; ++i) {
+    launch_query_lines(rs->levels[i], L + off[i] * 8, nl[i], s);
+    // the fit scratch is reused array after array (same stream, in order)
+    launch_select_hints(rs->levels[i], H + hoff[i] * 4, nh1[i], nh0[i], fit.as<uint32_t>(),
+                        rs->d_levels.as<LevelView>() + i, s);
+  }
+  // the host copies of the views get the flags the device computed
+  std::vector<LevelView> dv(na);
+  if (na) ECL_HIP(hipMemcpyAsync(dv.data(), rs->d_levels.ptr, na * sizeof(LevelView),
+                                 hipMemcpyDeviceToHost, s));
+  ECL_HIP(hipStreamSynchronize(s));
+  for (size_t i = 0; i < na; ++i) lv[i].hint_values = dv[i].hint_values;
   rs->levels = std::move(lv);
   rs->lines = std::move(lines);
 }

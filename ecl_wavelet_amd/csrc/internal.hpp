@@ -71,8 +71,11 @@ void launch_upload(const UploadBlob& b, hipStream_t s);
 // query lines of one array (detail::LevelView::lines): nlines =
 // ceil(size / kLineBits) lines of 128 bytes at `lines`
 void launch_query_lines(const LevelView& v, uint4* lines, uint64_t nlines, hipStream_t s);
+// the hints of one array and its hint_values, set in its device view d_view
+// (block_fit: 2 x select_hint_blocks(nh1, nh0) u32 of scratch)
+uint64_t select_hint_blocks(uint64_t nh1, uint64_t nh0);
 void launch_select_hints(const LevelView& v, uint4* hints, uint64_t nh1, uint64_t nh0,
-                         unsigned long long* fit, hipStream_t s);  // fit[2] zeroed by the caller
+                         uint32_t* block_fit, LevelView* d_view, hipStream_t s);
 
 // ---- launchers (kernels_build.hip) ------------------------------------------
 void launch_fill_u64(uint64_t* p, uint64_t n, uint64_t v, hipStream_t s);

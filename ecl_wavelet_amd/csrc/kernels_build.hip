@@ -413,13 +413,15 @@ uint32_t grid_for(uint64_t work, uint32_t block, uint32_t max_blocks) {
 // One thread per select hint of one array (layout at LevelView::sel_hints):
 // hints [0, nh1) for the ones, [nh1, nh1 + nh0) for the zeros, from the
 // array's samples and directory.
-// fit[1] / fit[0] count the hints of the ones / zeros whose bracket fits
-// (at most kHintBlocks blocks after b0).
+// block_fit[2 b + 1] / block_fit[2 b] count the hints of workgroup b for
+// the ones / zeros whose bracket fits (at most kHintBlocks blocks after b0).
 __global__ __launch_bounds__(256) void select_hints_kernel(LevelView v, uint4* __restrict__ hints,
                                                            uint64_t nh1, uint64_t nh0,
-                                                           unsigned long long* __restrict__ fit) {
+                                                           uint32_t* __restrict__ block_fit) {
+  __shared__ uint32_t s_fit[2];
+  if (threadIdx.x < 2) s_fit[threadIdx.x] = 0;
+  __syncthreads();
   const uint64_t t = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x;
-  // per wave: how many of its hints fit, for each Value (one atomic per wave and Value)
   bool fits1 = false, fits0 = false;
   if (t < nh1 + nh0) {
     const bool one = t < nh1;
@@ -435,9 +437,11 @@ __global__ __launch_bounds__(256) void select_hints_kernel(LevelView v, uint4* _
   }
   const uint64_t m1 = __ballot(fits1), m0 = __ballot(fits0);
   if ((threadIdx.x & 63) == 0) {
-    if (m1) atomicAdd(&fit[1], static_cast<unsigned long long>(__popcll(m1)));
-    if (m0) atomicAdd(&fit[0], static_cast<unsigned long long>(__popcll(m0)));
+    atomicAdd(&s_fit[1], static_cast<uint32_t>(__popcll(m1)));
+    atomicAdd(&s_fit[0], static_cast<uint32_t>(__popcll(m0)));
   }
+  __syncthreads();
+  if (threadIdx.x < 2) block_fit[2 * blockIdx.x + threadIdx.x] = s_fit[threadIdx.x];
   if (t >= nh1 + nh0) return;
   const bool one = t < nh1;
   const uint64_t j = one ? t : t - nh1;
@@ -465,16 +469,50 @@ __global__ __launch_bounds__(256) void select_hints_kernel(LevelView v, uint4* _
                           e[8 * k + 4] | (e[8 * k + 5] << 16), e[8 * k + 6] | (e[8 * k + 7] << 16));
 }
 
+// One workgroup: sums the per-workgroup fit counts of one array's hints and
+// sets the array's hint_values in its device view (a Value reads the hints
+// where at least half of its brackets fit one).
+__global__ __launch_bounds__(256) void hint_values_kernel(const uint32_t* __restrict__ block_fit,
+                                                          uint64_t nblocks, uint64_t nh1,
+                                                          uint64_t nh0, LevelView* d_view) {
+  __shared__ uint64_t s_sum[2][4];
+  uint64_t f1 = 0, f0 = 0;
+  for (uint64_t b = threadIdx.x; b < nblocks; b += 256) {
+    f1 += block_fit[2 * b + 1];
+    f0 += block_fit[2 * b];
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    f1 += __shfl_xor(static_cast<unsigned long long>(f1), off, 64);
+    f0 += __shfl_xor(static_cast<unsigned long long>(f0), off, 64);
+  }
+  if ((threadIdx.x & 63) == 0) {
+    s_sum[1][threadIdx.x >> 6] = f1;
+    s_sum[0][threadIdx.x >> 6] = f0;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    const uint64_t t1 = s_sum[1][0] + s_sum[1][1] + s_sum[1][2] + s_sum[1][3];
+    const uint64_t t0 = s_sum[0][0] + s_sum[0][1] + s_sum[0][2] + s_sum[0][3];
+    d_view->hint_values = (2 * t1 >= nh1 ? 2u : 0u) | (2 * t0 >= nh0 ? 1u : 0u);
+  }
+}
+
+uint64_t select_hint_blocks(uint64_t nh1, uint64_t nh0) { return (nh1 + nh0 + 255) / 256; }
+
 void launch_select_hints(const LevelView& v, uint4* hints, uint64_t nh1, uint64_t nh0,
-                         unsigned long long* fit, hipStream_t s) {
+                         uint32_t* block_fit, LevelView* d_view, hipStream_t s) {
   if (nh1 + nh0 == 0) return;
-  const uint64_t blocks = (nh1 + nh0 + 255) / 256;
+  const uint64_t blocks = select_hint_blocks(nh1, nh0);
   if (blocks > 0x7FFFFFFFull) throw Error(ECL_ERROR_INVALID_ARGUMENT, "bit array too large");
   LevelView plain = v;
   plain.lines = nullptr;
   plain.sel_hints = nullptr;
   hipLaunchKernelGGL(select_hints_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(256), 0, s,
-                     plain, hints, nh1, nh0, fit);
+                     plain, hints, nh1, nh0, block_fit);
+  ECL_HIP(hipGetLastError());
+  hipLaunchKernelGGL(hint_values_kernel, dim3(1), dim3(256), 0, s, block_fit, blocks, nh1, nh0,
+                     d_view);
   ECL_HIP(hipGetLastError());
 }
 

@@ -557,3 +557,43 @@ def test_query_lanes(gpu, lanes, case):
             assert np.array_equal(wt.select(wt.make_queries(sk, sc)), exp_s)
     finally:
         set_query_lanes(0)
+
+
+@pytest.mark.parametrize("m", [1, 3, 777, 70001])
+def test_mixed_batch_query_lines(gpu, m):
+    """A mixed batch (kinds grouped per chunk; with query lines, access and
+    rank on two lanes each) answers like the oracle before and after the
+    query lines are built, for odd batch sizes and for balanced, one-sided
+    and kind > 2 (read as select) mixes."""
+    from ecl_wavelet_amd import MIXED_DTYPE, DeviceBuffer, WaveletTree
+    rng = np.random.default_rng(m)
+    text = rng.integers(0, 256, 300_007).astype(np.uint8)
+    n = len(text)
+    wt = WaveletTree(text, np.arange(256, dtype=np.uint8))
+    cnt = np.bincount(text, minlength=256)
+    for mix in ("balanced", "access_rank", "select_high"):
+        q = np.zeros(m, dtype=MIXED_DTYPE)
+        if mix == "balanced":
+            kinds = rng.integers(0, 3, m)
+        elif mix == "access_rank":
+            kinds = rng.integers(0, 2, m)
+        else:
+            kinds = rng.choice([0, 2, 7], m)
+        sym = rng.integers(0, 256, m)
+        arg = np.where(kinds == 0, rng.integers(0, n, m),
+                       np.where(kinds == 1, rng.integers(0, n + 1, m),
+                                [int(rng.integers(1, cnt[c] + 2)) for c in sym]))
+        q["kind"], q["symbol"], q["arg"] = kinds, sym, arg
+        exp = np.empty(m, dtype=np.uint64)
+        a, s = q["arg"], q["symbol"].astype(np.uint64)
+        k0, k1, k2 = kinds == 0, kinds == 1, kinds >= 2
+        exp[k0] = text[a[k0]]
+        exp[k1] = O.scan_rank_batch(text, a[k1], s[k1])
+        exp[k2] = O.scan_select_batch(text, a[k2], s[k2])
+        d_q, d_out = DeviceBuffer(m * 16), DeviceBuffer(m * 8)
+        d_q.upload(q)
+        for lines in (False, True):
+            if lines:
+                wt.buildQueryLines()
+            wt.queryBatchDevice(d_q, m, d_out)
+            assert np.array_equal(d_out.download(np.uint64, m), exp), (mix, lines)
