@@ -180,7 +180,9 @@ def bench_batched(args):
     symbols.  Batched form: one forest build, then all queries (per-text
     count proportional to length, mixed 1:1:1) as one launch replayed from a
     HIP graph.  Plain form: one build per text through the reference-shaped
-    API and three device calls (access, rank, select) per tree."""
+    API and three device calls (access, rank, select) per tree.  Both over the
+    plain directory; the batched graph is then replayed over the forest's
+    query lines and select hints (reported beside)."""
     import ecl_wavelet_amd as E
     rng = np.random.default_rng(9)
     N = args.bt_texts
@@ -227,6 +229,20 @@ def bench_batched(args):
     batched_ms = float(np.median(runs))
     log(f"batched: build {forest_build_ms:.1f} ms, {m} queries in {batched_ms:.2f} ms "
         f"(block {block}, grid {grid})")
+    # the same graph over the forest's query lines and select hints
+    timer.start()
+    forest.buildQueryLines()
+    lines_ms = timer.stop()
+    g.launch()
+    E.synchronize()
+    runs = []
+    for _ in range(3):
+        timer.start()
+        g.launch()
+        runs.append(timer.stop())
+    batched_lines_ms = float(np.median(runs))
+    log(f"batched: query lines built in {lines_ms:.1f} ms, {m} queries in "
+        f"{batched_lines_ms:.2f} ms over them")
     del g, forest
     # ---- plain: one tree per text, per-tree per-kind calls
     alph = {int(x): np.arange(int(x), dtype=np.uint8) for x in (4, 20, 64, 256)}
@@ -285,6 +301,9 @@ def bench_batched(args):
         "plain_qps": m / (plain_ms / 1e3),
         "query_speedup": plain_ms / batched_ms,
         "batched_launch_ms": batched_ms,
+        "query_lines_build_ms": lines_ms,
+        "batched_qps_query_lines": m / (batched_lines_ms / 1e3),
+        "batched_launch_ms_query_lines": batched_lines_ms,
         "plain_per_tree_p50_ms": float(np.percentile(lat, 50)),
         "plain_per_tree_p99_ms": float(np.percentile(lat, 99)),
         "autotuned": {"block": block, "grid": grid},

@@ -134,6 +134,7 @@ _SIGS = {
     "ecl_wavelet_tree_generate_queries": (st, [vp, i32, u64, sz, vp, vp]),
     "ecl_forest_create": (st, [vp, vp, vp, vp, sz, i32, u32, C.POINTER(vp)]),
     "ecl_forest_destroy": (st, [vp]),
+    "ecl_forest_build_query_lines": (st, [vp]),
     "ecl_forest_num_trees": (st, [vp, C.POINTER(sz)]),
     "ecl_forest_tree_info": (st, [vp, sz, C.POINTER(u64), C.POINTER(u64), C.POINTER(u32)]),
     "ecl_forest_device_view": (st, [vp, sz, vp, sz]),

@@ -475,6 +475,12 @@ class Forest:
         except Exception:  # noqa: BLE001 - interpreter shutdown
             pass
 
+    def buildQueryLines(self):
+        """Query lines and select hints for every tree (one 128-byte line per
+        level for rank and access, a hint and one block for select); later
+        queries and graphs read them, with the same answers."""
+        call("ecl_forest_build_query_lines", self._h)
+
     def treeInfo(self, i):
         n, s = C.c_uint64(), C.c_uint64()
         L = C.c_uint32()

@@ -68,6 +68,9 @@ struct ecl_forest_t {
   DevBuf d_views;                // TreeView per tree
   std::vector<TreeView> views;
   uint32_t q_block = 256, q_grid = 0;  // launch shape of the batch query kernel
+  std::vector<LevelView> levels;       // every tree's level views, tree by tree
+  std::vector<LevelView*> d_levels;    // where each of them lives in the arena
+  DevBuf lines;                        // query lines and select hints (opt-in)
 };
 
 extern "C" {
@@ -381,6 +384,10 @@ ecl_status ecl_forest_create(const void* d_texts, const uint64_t* starts, const 
         v.sample_cap = t.cap[l];
       }
       put(t.lv_at, lv.data(), lv.size() * sizeof(LevelView));
+      for (uint32_t l = 0; l < t.L; ++l) {
+        f->levels.push_back(lv[l]);
+        f->d_levels.push_back(reinterpret_cast<LevelView*>(A + t.lv_at) + l);
+      }
       TreeView& v = f->views[i];
       v = TreeView{};
       v.levels = reinterpret_cast<const LevelView*>(A + t.lv_at);
@@ -402,6 +409,61 @@ ecl_status ecl_forest_create(const void* d_texts, const uint64_t* starts, const 
     h2d(f->d_views.ptr, f->views.data(), N);
     f->q_grid = static_cast<uint32_t>(cu_count(device)) * 8;
     *out = f.release();
+  });
+}
+
+ecl_status ecl_forest_build_query_lines(ecl_forest f) {
+  return guard([&] {
+    require(f, "NULL argument");
+    if (f->lines.ptr != nullptr) return;
+    set_device(f->device);
+    const size_t na = f->levels.size();
+    require(na < (1ull << 32), "too many bit arrays");
+    std::vector<LinesJob> jobs(na);
+    std::vector<uint64_t> starts(2 * na);  // line starts | hint starts
+    uint64_t tl = 0, th = 0;
+    for (size_t a = 0; a < na; ++a) {
+      const LevelView& v = f->levels[a];
+      LinesJob& J = jobs[a];
+      J.v = v;
+      J.v.lines = nullptr;
+      J.v.sel_hints = nullptr;
+      J.v.hint_values = 0;
+      J.d_view = f->d_levels[a];
+      J.nl = (v.size + detail::kLineBits - 1) / detail::kLineBits;
+      J.nh1 = v.size ? v.ones / detail::kSampleRate + 1 : 0;
+      J.nh0 = v.size ? (v.size - v.ones) / detail::kSampleRate + 1 : 0;
+      starts[a] = tl;
+      starts[na + a] = th;
+      tl += J.nl;
+      th += J.nh1 + J.nh0;
+    }
+    const uint64_t bytes = tl * 128 + th * 64;
+    const uint64_t meta = na * (sizeof(LinesJob) + 16 + 8);
+    if (bytes + meta + (64u << 20) > free_device_memory())
+      throw Error(ECL_ERROR_OUT_OF_MEMORY_TREE, "Not enough memory available for the wavelet tree.");
+    DevBuf lines(std::max<uint64_t>(bytes, 1));
+    DevBuf d_jobs(std::max<size_t>(na, 1) * sizeof(LinesJob));
+    DevBuf d_starts(std::max<size_t>(na, 1) * 16);
+    DevBuf d_fit(std::max<size_t>(na, 1) * 8);
+    hipStream_t s = nullptr;
+    h2d(d_jobs.ptr, jobs.data(), na);
+    h2d(d_starts.ptr, starts.data(), 2 * na);
+    ECL_HIP(hipMemsetAsync(d_fit.ptr, 0, std::max<size_t>(na, 1) * 8, s));
+    uint4* L = lines.as<uint4>();
+    uint4* H = L + tl * 8;
+    launch_multi_lines(d_jobs.as<LinesJob>(), d_starts.as<uint64_t>(), d_starts.as<uint64_t>() + na,
+                       static_cast<uint32_t>(na), L, tl, H, th, d_fit.as<uint32_t>(), s);
+    std::vector<uint32_t> fit(2 * na);
+    if (na) d2h(fit.data(), d_fit.ptr, 2 * na);  // waits for the launches
+    for (size_t a = 0; a < na; ++a) {
+      LevelView& v = f->levels[a];
+      v.lines = jobs[a].nl ? L + starts[a] * 8 : nullptr;
+      v.sel_hints = jobs[a].nh1 + jobs[a].nh0 ? H + starts[na + a] * 4 : nullptr;
+      v.hint_values = (2ull * fit[2 * a + 1] >= jobs[a].nh1 ? 2u : 0u) |
+                      (2ull * fit[2 * a] >= jobs[a].nh0 ? 1u : 0u);
+    }
+    f->lines = std::move(lines);
   });
 }
 

@@ -71,6 +71,19 @@ void launch_upload(const UploadBlob& b, hipStream_t s);
 // query lines of one array (detail::LevelView::lines): nlines =
 // ceil(size / kLineBits) lines of 128 bytes at `lines`
 void launch_query_lines(const LevelView& v, uint4* lines, uint64_t nlines, hipStream_t s);
+// Query lines and select hints of many arrays in three launches: job a's
+// lines and hints start at line_start[a] / hint_start[a] of the shared
+// buffers, and its device view d_view gets their addresses and hint_values
+// (fit: 2 x njobs u32, zeroed by the caller).
+struct LinesJob {
+  LevelView v;        // the array's view without lines or hints
+  LevelView* d_view;  // its view in device memory
+  uint64_t nl, nh1, nh0;
+};
+void launch_multi_lines(const LinesJob* jobs, const uint64_t* line_start,
+                        const uint64_t* hint_start, uint32_t njobs, uint4* lines,
+                        uint64_t total_lines, uint4* hints, uint64_t total_hints, uint32_t* fit,
+                        hipStream_t s);
 // the hints of one array and its hint_values, set in its device view d_view
 // (block_fit: 2 x select_hint_blocks(nh1, nh0) u32 of scratch)
 uint64_t select_hint_blocks(uint64_t nh1, uint64_t nh0);

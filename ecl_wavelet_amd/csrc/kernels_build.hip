@@ -353,16 +353,10 @@ __global__ void presence_kernel(const T* text, uint64_t n, uint32_t* flags) {
     flags[static_cast<uint64_t>(text[i])] = 1u;
 }
 
-// One thread per 16-byte piece of the query lines of one array: piece 0 of
-// line j holds the ones before the line (from the array's directory) and
-// words 30j, 30j+1; piece k > 0 holds words 30j+4k-2 .. 30j+4k+1 (zeros
-// past the array).  Consecutive threads read and write consecutive memory.
-__global__ __launch_bounds__(256) void query_lines_kernel(LevelView v, uint4* __restrict__ lines,
-                                                          uint64_t nlines) {
-  const uint64_t t = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x;
-  const uint64_t j = t >> 3;
-  const uint32_t k = static_cast<uint32_t>(t & 7);
-  if (j >= nlines) return;
+// Piece k (0..7) of query line j of an array: piece 0 holds the ones
+// before the line (from the array's directory) and words 30j, 30j+1; piece
+// k > 0 holds words 30j+4k-2 .. 30j+4k+1 (zeros past the array).
+__device__ __forceinline__ uint4 query_line_piece(const LevelView& v, uint64_t j, uint32_t k) {
   const uint64_t nw = (v.size + 31) / 32;
   const uint64_t first = 30 * j + (k == 0 ? 0 : 4 * k - 2);  // even: 8-byte aligned pairs
   const uint32_t cnt = k == 0 ? 2 : 4;
@@ -380,16 +374,23 @@ __global__ __launch_bounds__(256) void query_lines_kernel(LevelView v, uint4* __
   } else {
     for (uint32_t x = 0; x < cnt; ++x) w[x] = first + x < nw ? v.words[first + x] : 0u;
   }
-  uint4 out;
   if (k == 0) {
-    const uint64_t s = j * detail::kLineBits;
+    const uint64_t s = j * detail::kLineBits;  // < size for every line of the array
     bool unused = false;
     const uint64_t head = s == 0 ? 0 : detail::rank1_inner(v, s, &unused);
-    out = make_uint4(static_cast<uint32_t>(head), static_cast<uint32_t>(head >> 32), w[0], w[1]);
-  } else {
-    out = make_uint4(w[0], w[1], w[2], w[3]);
+    return make_uint4(static_cast<uint32_t>(head), static_cast<uint32_t>(head >> 32), w[0], w[1]);
   }
-  lines[j * 8 + k] = out;
+  return make_uint4(w[0], w[1], w[2], w[3]);
+}
+
+// One thread per 16-byte piece of the query lines of one array; consecutive
+// threads read and write consecutive memory.
+__global__ __launch_bounds__(256) void query_lines_kernel(LevelView v, uint4* __restrict__ lines,
+                                                          uint64_t nlines) {
+  const uint64_t t = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x;
+  const uint64_t j = t >> 3;
+  if (j >= nlines) return;
+  lines[t] = query_line_piece(v, j, static_cast<uint32_t>(t & 7));
 }
 
 // segment i of the blob to its destination, one workgroup for all segments
@@ -410,41 +411,13 @@ uint32_t grid_for(uint64_t work, uint32_t block, uint32_t max_blocks) {
 
 }  // namespace
 
-// One thread per select hint of one array (layout at LevelView::sel_hints):
-// hints [0, nh1) for the ones, [nh1, nh1 + nh0) for the zeros, from the
-// array's samples and directory.
-// block_fit[2 b + 1] / block_fit[2 b] count the hints of workgroup b for
-// the ones / zeros whose bracket fits (at most kHintBlocks blocks after b0).
-__global__ __launch_bounds__(256) void select_hints_kernel(LevelView v, uint4* __restrict__ hints,
-                                                           uint64_t nh1, uint64_t nh0,
-                                                           uint32_t* __restrict__ block_fit) {
-  __shared__ uint32_t s_fit[2];
-  if (threadIdx.x < 2) s_fit[threadIdx.x] = 0;
-  __syncthreads();
-  const uint64_t t = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x;
-  bool fits1 = false, fits0 = false;
-  if (t < nh1 + nh0) {
-    const bool one = t < nh1;
-    const uint64_t j = one ? t : t - nh1;
-    const uint64_t total = one ? v.ones : v.size - v.ones;
-    const uint64_t nsamp = total / detail::kSampleRate;
-    const uint64_t lo = j == 0 ? 0 : (one ? v.samples[j - 1] : v.samples[v.sample_cap - j]);
-    const uint64_t hi = j < nsamp ? (one ? v.samples[j] : v.samples[v.sample_cap - 1 - j])
-                                  : v.size - 1;
-    const bool fits = (hi >> 9) - (lo >> 9) <= detail::kHintBlocks;
-    fits1 = one && fits;
-    fits0 = !one && fits;
-  }
-  const uint64_t m1 = __ballot(fits1), m0 = __ballot(fits0);
-  if ((threadIdx.x & 63) == 0) {
-    atomicAdd(&s_fit[1], static_cast<uint32_t>(__popcll(m1)));
-    atomicAdd(&s_fit[0], static_cast<uint32_t>(__popcll(m0)));
-  }
-  __syncthreads();
-  if (threadIdx.x < 2) block_fit[2 * blockIdx.x + threadIdx.x] = s_fit[threadIdx.x];
-  if (t >= nh1 + nh0) return;
-  const bool one = t < nh1;
-  const uint64_t j = one ? t : t - nh1;
+// Select hint h of an array (h < nh1: the ones' hint h; else the zeros'
+// hint h - nh1; layout at LevelView::sel_hints), written to out[0..3];
+// returns whether its bracket fits (at most kHintBlocks blocks after b0).
+__device__ __forceinline__ bool select_hint(const LevelView& v, uint64_t h, uint64_t nh1,
+                                            uint4* __restrict__ out) {
+  const bool one = h < nh1;
+  const uint64_t j = one ? h : h - nh1;
   const uint64_t total = one ? v.ones : v.size - v.ones;
   const uint64_t nsamp = total / detail::kSampleRate;
   const uint64_t lo = j == 0 ? 0 : (one ? v.samples[j - 1] : v.samples[v.sample_cap - j]);
@@ -461,12 +434,37 @@ __global__ __launch_bounds__(256) void select_hints_kernel(LevelView v, uint4* _
   e[0] = span > detail::kHintBlocks ? 0xFFFFu : static_cast<uint32_t>(span);
   for (uint32_t u = 1; u < 24; ++u)
     e[u] = u <= span ? static_cast<uint32_t>(before(b0 + u) - c0) : 0xFFFFu;
-  uint4* o = hints + t * 4;
-  o[0] = make_uint4(static_cast<uint32_t>(lo), static_cast<uint32_t>(lo >> 32),
-                    static_cast<uint32_t>(c0), static_cast<uint32_t>(c0 >> 32));
+  out[0] = make_uint4(static_cast<uint32_t>(lo), static_cast<uint32_t>(lo >> 32),
+                      static_cast<uint32_t>(c0), static_cast<uint32_t>(c0 >> 32));
   for (int k = 0; k < 3; ++k)
-    o[1 + k] = make_uint4(e[8 * k] | (e[8 * k + 1] << 16), e[8 * k + 2] | (e[8 * k + 3] << 16),
-                          e[8 * k + 4] | (e[8 * k + 5] << 16), e[8 * k + 6] | (e[8 * k + 7] << 16));
+    out[1 + k] = make_uint4(e[8 * k] | (e[8 * k + 1] << 16), e[8 * k + 2] | (e[8 * k + 3] << 16),
+                            e[8 * k + 4] | (e[8 * k + 5] << 16), e[8 * k + 6] | (e[8 * k + 7] << 16));
+  return span <= detail::kHintBlocks;
+}
+
+// One thread per select hint of one array: hints [0, nh1) for the ones,
+// [nh1, nh1 + nh0) for the zeros.  block_fit[2 b + 1] / block_fit[2 b]
+// count the hints of workgroup b for the ones / zeros whose bracket fits.
+__global__ __launch_bounds__(256) void select_hints_kernel(LevelView v, uint4* __restrict__ hints,
+                                                           uint64_t nh1, uint64_t nh0,
+                                                           uint32_t* __restrict__ block_fit) {
+  __shared__ uint32_t s_fit[2];
+  if (threadIdx.x < 2) s_fit[threadIdx.x] = 0;
+  __syncthreads();
+  const uint64_t t = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x;
+  bool fits1 = false, fits0 = false;
+  if (t < nh1 + nh0) {
+    const bool fits = select_hint(v, t, nh1, hints + t * 4);
+    fits1 = t < nh1 && fits;
+    fits0 = t >= nh1 && fits;
+  }
+  const uint64_t m1 = __ballot(fits1), m0 = __ballot(fits0);
+  if ((threadIdx.x & 63) == 0) {
+    atomicAdd(&s_fit[1], static_cast<uint32_t>(__popcll(m1)));
+    atomicAdd(&s_fit[0], static_cast<uint32_t>(__popcll(m0)));
+  }
+  __syncthreads();
+  if (threadIdx.x < 2) block_fit[2 * blockIdx.x + threadIdx.x] = s_fit[threadIdx.x];
 }
 
 // One workgroup: sums the per-workgroup fit counts of one array's hints and
@@ -513,6 +511,80 @@ void launch_select_hints(const LevelView& v, uint4* hints, uint64_t nh1, uint64_
   ECL_HIP(hipGetLastError());
   hipLaunchKernelGGL(hint_values_kernel, dim3(1), dim3(256), 0, s, block_fit, blocks, nh1, nh0,
                      d_view);
+  ECL_HIP(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// Query lines and select hints of many arrays (the levels of a forest) in
+// three launches.  Array a owns lines [line_start[a], line_start[a + 1]) and
+// hints [hint_start[a], hint_start[a + 1]) of the shared buffers; a thread
+// finds its array by binary search over the starts (empty arrays share their
+// start with the next array, and the search takes the last array whose start
+// is not above the item, which owns it).
+__device__ __forceinline__ uint32_t owner_job(const uint64_t* __restrict__ start, uint32_t njobs,
+                                              uint64_t x) {
+  uint32_t lo = 0, hi = njobs;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (start[mid] <= x) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void multi_query_lines_kernel(
+    const LinesJob* __restrict__ jobs, const uint64_t* __restrict__ line_start, uint32_t njobs,
+    uint4* __restrict__ lines, uint64_t total_lines) {
+  const uint64_t t = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x;
+  const uint64_t j = t >> 3;
+  if (j >= total_lines) return;
+  const uint32_t a = owner_job(line_start, njobs, j);
+  lines[t] = query_line_piece(jobs[a].v, j - line_start[a], static_cast<uint32_t>(t & 7));
+}
+
+// fit[2 a + 1] / fit[2 a]: hints of array a for the ones / zeros that fit
+__global__ __launch_bounds__(256) void multi_select_hints_kernel(
+    const LinesJob* __restrict__ jobs, const uint64_t* __restrict__ hint_start, uint32_t njobs,
+    uint4* __restrict__ hints, uint64_t total_hints, uint32_t* __restrict__ fit) {
+  const uint64_t t = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (t >= total_hints) return;
+  const uint32_t a = owner_job(hint_start, njobs, t);
+  const uint64_t h = t - hint_start[a];
+  const uint64_t nh1 = jobs[a].nh1;
+  if (select_hint(jobs[a].v, h, nh1, hints + t * 4))
+    atomicAdd(&fit[2 * static_cast<uint64_t>(a) + (h < nh1 ? 1 : 0)], 1u);
+}
+
+// one thread per array: its device view gets the lines, the hints and the
+// Values that read the hints (at least half of their brackets fit one)
+__global__ __launch_bounds__(256) void multi_lines_views_kernel(
+    const LinesJob* __restrict__ jobs, const uint64_t* __restrict__ line_start,
+    const uint64_t* __restrict__ hint_start, uint32_t njobs, const uint4* lines,
+    const uint4* hints, const uint32_t* __restrict__ fit) {
+  const uint64_t a = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (a >= njobs) return;
+  const LinesJob& J = jobs[a];
+  LevelView* d = J.d_view;
+  d->lines = J.nl ? lines + line_start[a] * 8 : nullptr;
+  d->sel_hints = J.nh1 + J.nh0 ? hints + hint_start[a] * 4 : nullptr;
+  d->hint_values = (2ull * fit[2 * a + 1] >= J.nh1 ? 2u : 0u) | (2ull * fit[2 * a] >= J.nh0 ? 1u : 0u);
+}
+
+void launch_multi_lines(const LinesJob* jobs, const uint64_t* line_start,
+                        const uint64_t* hint_start, uint32_t njobs, uint4* lines,
+                        uint64_t total_lines, uint4* hints, uint64_t total_hints, uint32_t* fit,
+                        hipStream_t s) {
+  if (njobs == 0) return;
+  const uint64_t lb = (total_lines * 8 + 255) / 256, hb = (total_hints + 255) / 256;
+  if (lb > 0x7FFFFFFFull || hb > 0x7FFFFFFFull)
+    throw Error(ECL_ERROR_INVALID_ARGUMENT, "bit arrays too large");
+  if (lb)
+    hipLaunchKernelGGL(multi_query_lines_kernel, dim3(static_cast<uint32_t>(lb)), dim3(256), 0, s,
+                       jobs, line_start, njobs, lines, total_lines);
+  if (hb)
+    hipLaunchKernelGGL(multi_select_hints_kernel, dim3(static_cast<uint32_t>(hb)), dim3(256), 0, s,
+                       jobs, hint_start, njobs, hints, total_hints, fit);
+  hipLaunchKernelGGL(multi_lines_views_kernel, dim3((njobs + 255) / 256), dim3(256), 0, s, jobs,
+                     line_start, hint_start, njobs, lines, hints, fit);
   ECL_HIP(hipGetLastError());
 }
 

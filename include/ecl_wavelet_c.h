@@ -329,6 +329,11 @@ ecl_status ecl_forest_create(const void* d_texts, const uint64_t* starts, const 
                              const uint64_t* sigmas, size_t num_texts, int device,
                              uint32_t tile_rows, ecl_forest* out);
 ecl_status ecl_forest_destroy(ecl_forest f);
+/* Query lines and select hints (see ecl_wavelet_tree_build_query_lines) for
+ * every tree of the forest, in three launches; later forest queries and
+ * graphs (also graphs captured before) read them.  No counterpart in the
+ * reference. */
+ecl_status ecl_forest_build_query_lines(ecl_forest f);
 ecl_status ecl_forest_num_trees(ecl_forest f, size_t* out);
 /* size, alphabet size and levels of tree i */
 ecl_status ecl_forest_tree_info(ecl_forest f, size_t i, uint64_t* n, uint64_t* sigma,

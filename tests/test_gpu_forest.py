@@ -217,3 +217,27 @@ def test_forest_build_spans(gpu, span):
         set_build_span(0)
     q, e = _queries(texts, sigmas, 300, seed=9)
     assert np.array_equal(_run(f, q), e)
+
+
+def test_forest_query_lines(gpu):
+    """With query lines and select hints built for every tree (three
+    launches over all levels), a mixed batch answers like the oracle, and a
+    graph captured before the lines existed replays with the same answers."""
+    from ecl_wavelet_amd import DeviceBuffer, synchronize
+    texts, sigmas = _padded_texts(21, 40, lo_log2=4, hi_log2=18)
+    f, _, _ = _forest(texts, sigmas)
+    q, e = _queries(texts, sigmas, 400, seed=9)
+    dq = DeviceBuffer.from_array(q)
+    dout = DeviceBuffer(len(q) * 8)
+    g = f.graph(dq, len(q), dout)
+    assert np.array_equal(_run(f, q), e)
+    f.buildQueryLines()
+    f.buildQueryLines()  # a second call keeps the lines
+    assert np.array_equal(_run(f, q), e)
+    zeros = np.zeros(len(q), dtype=np.uint64)
+    for _ in range(2):
+        dout.upload(zeros)
+        synchronize()
+        g.launch()
+        synchronize()
+        assert np.array_equal(dout.download(np.uint64, len(q)), e)
