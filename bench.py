@@ -485,7 +485,9 @@ def bench_serving(args):
     """Serving configuration: args.sv_trees resident uint8 trees, lengths
     log-uniform in [2^20, 2^sv_max_log2], sigma in {4, 256}; Poisson request
     streams of mixed batches of log-uniform size in [1, 10^6] against a
-    random tree, served with coalescing and in the plain per-call form."""
+    random tree, served with coalescing and in the plain per-call form; then
+    the same streams served by a server whose trees carry query lines and
+    select hints."""
     import ecl_wavelet_amd as E
     rng = np.random.default_rng(10)
     T = args.sv_trees
@@ -517,6 +519,23 @@ def bench_serving(args):
             log(f"serving {mode} @ {rate:g} req/s: {st['queries_per_s']:.3g} q/s, "
                 f"p50 {st['p50_ms']:.2f} ms, p99 {st['p99_ms']:.2f} ms, {st['batches']} batches")
         out["rates"].append(row)
+    srv.close()
+    # the same trees and request streams, served over query lines and hints
+    srv = E.Server(max_batch=1 << 22, max_inflight=3, query_lines=True)
+    timer.start()
+    for i in range(T):
+        d = E.generate_text(int(lens[i]), int(sig[i]), np.uint8, seed=200_000 + i)
+        srv.addTree(d, int(sig[i]), n=int(lens[i]))
+        d.free()
+    out["admit_ms_query_lines"] = timer.stop()
+    out["hbm_used_gb_query_lines"] = srv.stats()["hbm_used"] / 1e9
+    for row in out["rates"]:
+        rate, secs = row["rate_req_s"], row["seconds"]
+        st = srv.load(rate, secs, seed=int(rate), min_size=1, max_size=1_000_000, pool=1 << 20)
+        row["served_query_lines"] = {k: st[k] for k in ("requests", "queries", "batches",
+                                                        "p50_ms", "p99_ms", "queries_per_s")}
+        log(f"serving over query lines @ {rate:g} req/s: {st['queries_per_s']:.3g} q/s, "
+            f"p50 {st['p50_ms']:.2f} ms, p99 {st['p99_ms']:.2f} ms")
     srv.close()
     return out
 

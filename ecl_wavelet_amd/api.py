@@ -527,7 +527,7 @@ class Graph:
 class _ServerConfig(C.Structure):
     _fields_ = [("hbm_budget", C.c_uint64), ("max_batch", C.c_uint64),
                 ("max_inflight", C.c_uint32), ("max_trees", C.c_uint32),
-                ("coalesce", C.c_uint32), ("reserved", C.c_uint32)]
+                ("coalesce", C.c_uint32), ("query_lines", C.c_uint32)]
 
 
 class _ServerStats(C.Structure):
@@ -551,9 +551,9 @@ class Server:
     admitted by an HBM budget, requests coalesced into device batches."""
 
     def __init__(self, device=0, hbm_budget=0, max_batch=0, max_inflight=0, max_trees=0,
-                 coalesce=True):
+                 coalesce=True, query_lines=False):
         cfg = _ServerConfig(hbm_budget, max_batch, max_inflight, max_trees,
-                            1 if coalesce else 0, 0)
+                            1 if coalesce else 0, 1 if query_lines else 0)
         h = C.c_void_p()
         call("ecl_server_create", device, C.byref(cfg), C.byref(h))
         self._h = h

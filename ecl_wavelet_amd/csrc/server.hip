@@ -335,12 +335,30 @@ ecl_status ecl_server_destroy(ecl_server s) {
 
 }  // extern "C"
 
-// Bits, directory and metadata of a uint8 tree of n symbols over sigma.
-static uint64_t tree_estimate(uint64_t n, uint64_t sigma) {
+// Bits, directory and metadata of a uint8 tree of n symbols over sigma, and
+// with `lines` its query lines (128 B per 960 bits) and select hints (64 B
+// per 4096 ones or zeros, plus one per Value) of every level.
+static uint64_t tree_estimate(uint64_t n, uint64_t sigma, bool lines) {
   const uint64_t L = ecl::detail::ceil_log2(sigma);
-  return L * ((n + 1023) / 1024 * 128 + (n + 511) / 512 * 2 + (n + 65535) / 65536 * 8 +
-              n / 4096 * 8 + 1024) +
-         sigma * 64 + 4096;
+  uint64_t b = L * ((n + 1023) / 1024 * 128 + (n + 511) / 512 * 2 + (n + 65535) / 65536 * 8 +
+                    n / 4096 * 8 + 1024) +
+               sigma * 64 + 4096;
+  if (lines)
+    b += L * ((n + ecl::detail::kLineBits - 1) / ecl::detail::kLineBits * 128 +
+              (n / ecl::detail::kSampleRate + 2) * 64);
+  return b;
+}
+
+// The tree's query lines when the server is configured for them; on failure
+// the tree is destroyed and the error returned.
+static ecl_status add_query_lines(ecl_server s, ecl_wavelet_tree wt, std::string* msg) {
+  if (!s->cfg.query_lines) return ECL_OK;
+  const ecl_status st = ecl_wavelet_tree_build_query_lines(wt);
+  if (st != ECL_OK) {
+    *msg = last_error();
+    (void)ecl_wavelet_tree_destroy(wt);
+  }
+  return st;
 }
 
 // Admission: reserves a slot and `estimate` bytes of the budget.
@@ -392,21 +410,22 @@ ecl_status ecl_server_add_tree(ecl_server s, const void* text, int text_on_devic
     require(n > 0, "empty text");
     set_device(s->device);
     std::lock_guard<std::mutex> build_lk(s->build_mu);
-    const uint64_t estimate = tree_estimate(n, sigma);
+    const uint64_t estimate = tree_estimate(n, sigma, s->cfg.query_lines != 0);
     const uint32_t slot = reserve_slot(s, estimate);
     std::vector<uint8_t> alpha(sigma);
     for (uint64_t c = 0; c < sigma; ++c) alpha[c] = static_cast<uint8_t>(c);
     ecl_wavelet_tree wt = nullptr;
     size_t fr0 = 0, fr1 = 0, tot = 0;
     (void)hipMemGetInfo(&fr0, &tot);
-    const ecl_status st =
+    ecl_status st =
         text_on_device ? ecl_wavelet_tree_create_device(ECL_U8, text, n, alpha.data(), sigma,
                                                         s->device, &wt)
                        : ecl_wavelet_tree_create(ECL_U8, text, n, alpha.data(), sigma, s->device,
                                                  &wt);
+    std::string msg = st != ECL_OK ? last_error() : std::string();
+    if (st == ECL_OK) st = add_query_lines(s, wt, &msg);
     (void)hipMemGetInfo(&fr1, &tot);
     if (st != ECL_OK) {
-      const std::string msg = last_error();
       release_slot(s, slot, estimate);
       throw Error(st, msg);
     }
@@ -425,13 +444,12 @@ ecl_status ecl_server_add_tree_file(ecl_server s, const char* path, uint32_t* tr
             "the server holds uint8 trees with alphabets of 2 to 256 symbols");
     set_device(s->device);
     std::lock_guard<std::mutex> build_lk(s->build_mu);
-    const uint64_t estimate = tree_estimate(n, sigma);
+    const uint64_t estimate = tree_estimate(n, sigma, s->cfg.query_lines != 0);
     const uint32_t slot = reserve_slot(s, estimate);
     ecl_wavelet_tree wt = nullptr;
     size_t fr0 = 0, fr1 = 0, tot = 0;
     (void)hipMemGetInfo(&fr0, &tot);
     ecl_status st = ecl_wavelet_tree_load(path, s->device, &wt);
-    (void)hipMemGetInfo(&fr1, &tot);
     std::string msg = st != ECL_OK ? last_error() : std::string();
     if (st == ECL_OK) {
       ecl_wavelet_tree_info info{};
@@ -442,6 +460,8 @@ ecl_status ecl_server_add_tree_file(ecl_server s, const char* path, uint32_t* tr
         msg = "the server holds trees over the alphabet 0 .. sigma-1";
       }
     }
+    if (st == ECL_OK) st = add_query_lines(s, wt, &msg);
+    (void)hipMemGetInfo(&fr1, &tot);
     if (st != ECL_OK) {
       release_slot(s, slot, estimate);
       throw Error(st, msg);

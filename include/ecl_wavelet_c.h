@@ -377,7 +377,9 @@ typedef struct ecl_server_config {
   uint32_t coalesce;      /* 1: coalesce requests into batches; 0: one request per
                            * batch and one batch on the device at a time (the plain
                            * per-call form) */
-  uint32_t reserved;
+  uint32_t query_lines;   /* 1: every admitted tree also gets its query lines and
+                           * select hints (ecl_wavelet_tree_build_query_lines; about
+                           * 1.08x its bits more memory, charged to the budget) */
 } ecl_server_config;
 typedef struct ecl_server_stats {
   uint64_t requests;      /* completed requests since the last reset */

@@ -161,3 +161,21 @@ def test_server_warm_start_from_files(gpu, tmp_path):
         out, _ = srv.wait(srv.submit(tree, q))
         assert np.array_equal(out, e)
     srv.close()
+
+
+def test_server_query_lines(gpu, tmp_path):
+    """A server configured for query lines builds them for every admitted
+    tree (from a text and from a saved file) and answers every request like
+    the direct scans."""
+    from ecl_wavelet_amd import Server, WaveletTree
+    t0, t1 = _text(31, 300_001, 256), _text(32, 150_000, 4)
+    WaveletTree(t1, np.arange(4, dtype=np.uint8)).save(tmp_path / "t1.ewt")
+    srv = Server(query_lines=True)
+    trees = [srv.addTree(t0, 256), srv.addTreeFile(tmp_path / "t1.ewt")]
+    work = [(trees[0], *r) for r in _requests(t0, 256, [1, 999, 20000], seed=41)]
+    work += [(trees[1], *r) for r in _requests(t1, 4, [7, 3000], seed=42)]
+    ids = [srv.submit(tree, q) for tree, q, _ in work]
+    for rid, (_, _, e) in zip(ids, work):
+        out, _ = srv.wait(rid)
+        assert np.array_equal(out, e)
+    srv.close()
