@@ -297,62 +297,113 @@ __global__ void tree_gen_kernel(TreeView t, int kind, uint64_t seed,
 // (levels, counts, node ranks) is read from global memory, where the
 // metadata of all trees stays in the L2 / MALL.  A grid of a few
 // workgroups per CU strides over the batch.
+// One query of kind k against tree t with its metadata at mt (LDS or the
+// tree's own arrays in global memory).
+__device__ __forceinline__ uint64_t tree_one(const TreeView& t, const Meta& mt, uint32_t k,
+                                             uint64_t arg, uint32_t symbol) {
+  if (k == 0)
+    return arg < t.n ? detail::tree_access(t, mt.lv, mt.counts, mt.ranks, mt.nal, arg) : 0;
+  if (k == 1)
+    return arg > t.n ? 0 : detail::tree_rank(t, mt.lv, mt.counts, mt.ranks, mt.nal, symbol, arg);
+  return detail::tree_select(t, mt.lv, mt.counts, mt.ranks, mt.nal, symbol, arg);
+}
+
 __device__ __forceinline__ uint64_t forest_one(const TreeView* __restrict__ views,
                                                uint64_t num_views, uint64_t tree, uint32_t k,
                                                uint64_t arg, uint32_t symbol) {
   if (tree >= num_views) return 0;
   const TreeView t = views[tree];
-  if (k == 0) {
-    return arg < t.n ? detail::tree_access(t, t.levels, t.counts, t.ranks, t.nodes_at_level, arg)
-                     : 0;
-  }
-  if (k == 1) {
-    return arg > t.n ? 0
-                     : detail::tree_rank(t, t.levels, t.counts, t.ranks, t.nodes_at_level,
-                                         symbol, arg);
-  }
-  return detail::tree_select(t, t.levels, t.counts, t.ranks, t.nodes_at_level, symbol, arg);
+  return tree_one(t, Meta{t.levels, t.counts, t.ranks, t.nodes_at_level}, k, arg, symbol);
 }
 
+// LDS for one tree's staged metadata in the forest and served-batch kernels
+// (level views, nodes per level, C array, node ranks; under 5 KB for the
+// uint8 trees of 2..256 symbols they hold).
+constexpr uint32_t kTreeMetaLds = 8192;
+
+// A chunk whose queries all name one tree stages that tree's metadata in
+// LDS, as the single-tree kernels do, so the descents read it there instead
+// of from global memory at every level; other chunks read each query's tree
+// from global memory.
 __global__ void forest_query_kernel(const TreeView* __restrict__ views, uint64_t num_trees,
                                     const ecl_forest_query* __restrict__ qs, uint64_t m,
                                     uint64_t* __restrict__ out, uint32_t chunk) {
   __shared__ KindLists lists;
+  __shared__ __align__(16) uint8_t s_meta[kTreeMetaLds];
   for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * chunk; base < m;
        base += static_cast<uint64_t>(gridDim.x) * chunk) {
     const uint32_t len = static_cast<uint32_t>(m - base < chunk ? m - base : chunk);
     const ecl_forest_query* cq = qs + base;
     group_by_kind(lists, len, [&](uint32_t j) { return static_cast<uint32_t>(cq[j].kind); });
-    run_by_kind(lists, [&](uint32_t j, uint32_t k) {
-      const ecl_forest_query e = cq[j];
-      out[base + j] = forest_one(views, num_trees, e.tree, k, e.arg, e.symbol);
-    });
+    const uint32_t t0 = cq[0].tree;
+    bool other = false;
+    for (uint32_t j = threadIdx.x; j < len; j += blockDim.x) other |= cq[j].tree != t0;
+    const bool one_tree = !__syncthreads_or(other) && t0 < num_trees &&
+                          meta_bytes_of(views[t0]) <= kTreeMetaLds;
+    if (one_tree) {
+      const TreeView t = views[t0];
+      const Meta mt = stage_meta(t, s_meta);
+      run_by_kind(lists, [&](uint32_t j, uint32_t k) {
+        const ecl_forest_query e = cq[j];
+        out[base + j] = tree_one(t, mt, k, e.arg, e.symbol);
+      });
+    } else {
+      run_by_kind(lists, [&](uint32_t j, uint32_t k) {
+        const ecl_forest_query e = cq[j];
+        out[base + j] = forest_one(views, num_trees, e.tree, k, e.arg, e.symbol);
+      });
+    }
   }
 }
 
 // A served batch: requests' queries back to back, segment j (queries from
 // seg_start[j] on) belonging to the tree in view slot seg_tree[j].
+__device__ __forceinline__ uint32_t segment_of(const uint64_t* __restrict__ seg_start,
+                                               uint32_t nseg, uint64_t q) {
+  uint32_t lo = 0, hi = nseg;  // last segment with seg_start <= q
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (seg_start[mid] <= q) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// Chunks whose segments all name one tree stage its metadata in LDS (as in
+// forest_query_kernel) and skip the per-query segment search.
 __global__ void segment_query_kernel(const TreeView* __restrict__ views, uint64_t num_views,
                                      const ecl_query* __restrict__ qs, uint64_t m,
                                      const uint64_t* __restrict__ seg_start,
                                      const uint32_t* __restrict__ seg_tree, uint32_t nseg,
                                      uint64_t* __restrict__ out, uint32_t chunk) {
   __shared__ KindLists lists;
+  __shared__ __align__(16) uint8_t s_meta[kTreeMetaLds];
   for (uint64_t base = static_cast<uint64_t>(blockIdx.x) * chunk; base < m;
        base += static_cast<uint64_t>(gridDim.x) * chunk) {
     const uint32_t len = static_cast<uint32_t>(m - base < chunk ? m - base : chunk);
     const ecl_query* cq = qs + base;
     group_by_kind(lists, len, [&](uint32_t j) { return cq[j].kind; });
-    run_by_kind(lists, [&](uint32_t j, uint32_t k) {
-      const uint64_t q = base + j;
-      uint32_t lo = 0, hi = nseg;  // last segment with seg_start <= q
-      while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (seg_start[mid] <= q) lo = mid; else hi = mid;
-      }
-      const ecl_query e = cq[j];
-      out[q] = forest_one(views, num_views, seg_tree[lo], k, e.arg, e.symbol);
-    });
+    const uint32_t s0 = segment_of(seg_start, nseg, base);
+    const uint32_t s1 = segment_of(seg_start, nseg, base + len - 1);
+    const uint32_t t0 = seg_tree[s0];
+    bool other = false;
+    for (uint32_t x = s0 + 1 + threadIdx.x; x <= s1; x += blockDim.x) other |= seg_tree[x] != t0;
+    const bool one_tree = !__syncthreads_or(other) && t0 < num_views &&
+                          meta_bytes_of(views[t0]) <= kTreeMetaLds;
+    if (one_tree) {
+      const TreeView t = views[t0];
+      const Meta mt = stage_meta(t, s_meta);
+      run_by_kind(lists, [&](uint32_t j, uint32_t k) {
+        const ecl_query e = cq[j];
+        out[base + j] = tree_one(t, mt, k, e.arg, e.symbol);
+      });
+    } else {
+      run_by_kind(lists, [&](uint32_t j, uint32_t k) {
+        const uint64_t q = base + j;
+        const ecl_query e = cq[j];
+        out[q] = forest_one(views, num_views, seg_tree[segment_of(seg_start, nseg, q)], k, e.arg,
+                            e.symbol);
+      });
+    }
   }
 }
 
