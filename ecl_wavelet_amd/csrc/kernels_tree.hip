@@ -182,10 +182,14 @@ __global__ __launch_bounds__(kPassThreads) void tile_hist_kernel(PassParams p) {
   __shared__ uint32_t s_bins[kHistSlots];
   __shared__ uint32_t s_tot[kMaxBins];
   const uint32_t tid = threadIdx.x;
-  const uint32_t nb = 1u << p.k, C = kHistSlots >> p.k;
+  // C copies of the nb bins, lane-banked (thread tid counts into copy
+  // tid % C); never more copies than threads, so small alphabets zero and
+  // reduce only the copies that are used
+  const uint32_t nb = 1u << p.k;
+  const uint32_t C = (kHistSlots >> p.k) < kPassThreads ? (kHistSlots >> p.k) : kPassThreads;
   const uint32_t dmask = nb - 1;
   const uint32_t row = p.row_bins ? p.row_bins : nb;
-  for (uint32_t i = tid; i < kHistSlots / 4; i += kPassThreads)
+  for (uint32_t i = tid; i < nb * C / 4; i += kPassThreads)
     reinterpret_cast<uint4*>(s_bins)[i] = make_uint4(0, 0, 0, 0);
   if (tid < kMaxBins) s_tot[tid] = 0;
   const uint64_t t = blockIdx.x;
@@ -203,7 +207,32 @@ __global__ __launch_bounds__(kPassThreads) void tile_hist_kernel(PassParams p) {
     }
   }
   const uint32_t nchunks = (g.len + EI - 1) / EI;
-  for (uint32_t c = tid; c < nchunks; c += kPassThreads) {
+  uint32_t c0 = tid;
+  if constexpr (!Remap) {
+    // four chunks per thread at a time, all loads issued before the counts
+    // (one load per thread in flight would leave HBM idle between them)
+    constexpr uint32_t U = 4;
+    for (; c0 + (U - 1) * kPassThreads < nchunks; c0 += U * kPassThreads) {
+      uint4 q[U];
+      uint32_t valid[U];
+#pragma unroll
+      for (uint32_t u = 0; u < U; ++u) {
+        const uint32_t c = c0 + u * kPassThreads;
+        const uint64_t e0 = g.start + static_cast<uint64_t>(c) * EI;
+        valid[u] = (g.len - c * EI < EI) ? g.len - c * EI : EI;
+        q[u] = valid[u] == EI ? *reinterpret_cast<const uint4*>(in + e0)
+                              : load_partial_chunk<TIn>(in, e0, valid[u]);
+      }
+#pragma unroll
+      for (uint32_t u = 0; u < U; ++u)
+#pragma unroll
+        for (uint32_t k = 0; k < EI; ++k)
+          if (k < valid[u])
+            atomicAdd(&bins[((static_cast<uint64_t>(chunk_elem<TIn>(q[u], k)) >> p.dshift) & dmask) * C],
+                      1u);
+    }
+  }
+  for (uint32_t c = c0; c < nchunks; c += kPassThreads) {
     const uint64_t e0 = g.start + static_cast<uint64_t>(c) * EI;
     const uint32_t valid = (g.len - c * EI < EI) ? g.len - c * EI : EI;
     uint4 q;
