@@ -725,6 +725,10 @@ def main():
 
     L = wt.getNumLevels()
     del wt
+    # the queries' roofline: random dependent 128-byte lines per second on
+    # this device (probe over a buffer the size of the tree's query lines,
+    # L lines per probe query, two lanes per line like the query kernels)
+    line_peak = E.probe_random_lines(int(lines_bytes), Q, steps=L, lanes=2)
     build = float(np.median(build_ms))
     qtime = float(np.mean(q_ms)) / 1e3
     value = 3 * Q / qtime
@@ -774,6 +778,17 @@ def main():
         "small_call_rank_p50_ms": float(np.percentile(lat, 50)),
         "small_call_rank_p99_ms": float(np.percentile(lat, 99)),
         "rank_kernel_ms": rank_k[1] / max(rank_k[0], 1),
+        "query_roofline": {
+            "bound": "random dependent 128-byte line reads",
+            "unit": "lines/s",
+            "peak": line_peak,
+            "peak_source": (f"ecl_probe_random_lines over {int(lines_bytes)} bytes, {Q} probe "
+                            f"queries of {L} dependent lines, two lanes per line"),
+            **{kind: {"lines_per_query": lpq,
+                      "achieved": Q / (np.mean(kind_ms[kind]) / 1e3) * lpq,
+                      "frac": Q / (np.mean(kind_ms[kind]) / 1e3) * lpq / line_peak}
+               for kind, lpq in (("access", L), ("rank", L), ("select", 2 * L))},
+        },
         "plain_directory": {
             "access_qps": Q / (np.mean(plain_ms["access"]) / 1e3),
             "rank_qps": Q / (np.mean(plain_ms["rank"]) / 1e3),

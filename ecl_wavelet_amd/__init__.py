@@ -8,6 +8,6 @@ class names in Python (``api``).
 from .api import (BitArray, DeviceBuffer, Forest, Graph, Profiler, RankSelect,  # noqa: F401
                   Server, Timer, WaveletTree, device_name, generate_text,
                   generate_text_into, mem_info, release_cached_memory, set_build_span,
-                  set_query_lanes,
+                  set_query_lanes, probe_random_lines,
                   zipf_thresholds,
                   synchronize, FOREST_QUERY_DTYPE, MIXED_DTYPE, QUERY_DTYPE)

@@ -74,6 +74,8 @@ _SIGS = {
     "ecl_release_cached_memory": (st, [i32]),
     "ecl_set_build_span": (st, [C.c_uint32]),
     "ecl_set_query_lanes": (st, [C.c_uint32]),
+    "ecl_probe_random_lines": (st, [C.c_int, C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32,
+                                    C.POINTER(C.c_double)]),
     "ecl_device_name": (st, [i32, C.c_char_p, sz]),
     "ecl_event_create": (st, [C.POINTER(vp)]),
     "ecl_event_destroy": (st, [vp]),

@@ -690,6 +690,14 @@ def set_query_lanes(lanes=0):
     call("ecl_set_query_lanes", lanes)
 
 
+def probe_random_lines(nbytes, queries, steps=8, lanes=2, device=0):
+    """Random, dependent 128-byte line reads per second on the device (the
+    bound of the tree queries: one line per level, two for select)."""
+    r = C.c_double()
+    call("ecl_probe_random_lines", device, nbytes, queries, steps, lanes, C.byref(r))
+    return r.value
+
+
 def release_cached_memory(device=0):
     """Frees the construction scratch the library keeps per device between
     builds."""

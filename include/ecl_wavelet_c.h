@@ -135,6 +135,14 @@ ecl_status ecl_event_elapsed(void* start, void* stop, float* ms);
 ecl_status ecl_profile_enable(int on);
 ecl_status ecl_profile_reset(void);
 ecl_status ecl_profile_get(const char* kernel, uint64_t* launches, double* total_ms);
+/* The device's rate of random, dependent 128-byte line reads (lines per
+ * second): `queries` probe queries over a buffer of `bytes`, each reading
+ * `steps` lines one after the other at positions derived from the previous
+ * line, with `lanes` (1, 2, 4 or 8) lanes per line.  The bound of the tree
+ * queries, which read one such line per level (two for select); no
+ * counterpart in the reference. */
+ecl_status ecl_probe_random_lines(int device, uint64_t bytes, uint64_t queries, uint32_t steps,
+                                  uint32_t lanes, double* lines_per_s);
 /* deterministic synthetic inputs on the device: text[i] = bounded(hash(seed,
  * i), sigma), or the inverse CDF of `thresholds` (host array of sigma u64)
  * when non-NULL; bits i.i.d. with P(1) = threshold / 2^64 */
