@@ -114,7 +114,9 @@ _SIGS = {
     "ecl_wavelet_tree_create_device": (st, [i32, vp, sz, vp, sz, i32, C.POINTER(vp)]),
     "ecl_wavelet_tree_destroy": (st, [vp]),
     "ecl_wavelet_tree_build_query_lines": (st, [vp]),
+    "ecl_wavelet_tree_build_select_lines": (st, [vp]),
     "ecl_rank_select_build_query_lines": (st, [vp]),
+    "ecl_rank_select_build_select_lines": (st, [vp]),
     "ecl_wavelet_tree_save": (st, [vp, C.c_char_p]),
     "ecl_wavelet_tree_load": (st, [C.c_char_p, i32, C.POINTER(vp)]),
     "ecl_wavelet_tree_info_get": (st, [vp, C.POINTER(TreeInfo)]),

@@ -270,6 +270,11 @@ class RankSelect:
         a rank then reads one line."""
         call("ecl_rank_select_build_query_lines", self._h)
 
+    def buildSelectLines(self):
+        """A 128-byte line every r ones (zeros): its position and the 960 bits
+        from there; a select then reads one line."""
+        call("ecl_rank_select_build_select_lines", self._h)
+
     def rankDevice(self, i, d_index, m, d_out, value=1, stream=None):
         call("ecl_rank_select_rank_device", self._h, i, int(value), d_index.ptr, m,
              d_out.ptr, stream)
@@ -327,6 +332,12 @@ class WaveletTree:
         """Second copy of the level bits in 128-byte lines headed by their
         rank: rank and access then read one line per level."""
         call("ecl_wavelet_tree_build_query_lines", self._h)
+
+    def buildSelectLines(self):
+        """Per level, a 128-byte line every r ones (zeros) holding its
+        position and the 960 bits from there: a select then reads one line
+        per level."""
+        call("ecl_wavelet_tree_build_select_lines", self._h)
 
     # ---- persistence
     def save(self, path):

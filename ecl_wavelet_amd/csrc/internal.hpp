@@ -71,6 +71,10 @@ void launch_upload(const UploadBlob& b, hipStream_t s);
 // query lines of one array (detail::LevelView::lines): nlines =
 // ceil(size / kLineBits) lines of 128 bytes at `lines`
 void launch_query_lines(const LevelView& v, uint4* lines, uint64_t nlines, hipStream_t s);
+// select lines of one array: n1 for the ones at rate r1, then n0 for the
+// zeros at rate r0 (layout at LevelView::sel_lines)
+void launch_select_lines(const LevelView& v, uint4* out, uint64_t n1, uint64_t n0, uint32_t r1,
+                         uint32_t r0, hipStream_t s);
 // Query lines and select hints of many arrays in three launches: job a's
 // lines and hints start at line_start[a] / hint_start[a] of the shared
 // buffers, and its device view d_view gets their addresses and hint_values

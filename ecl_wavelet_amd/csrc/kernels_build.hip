@@ -588,6 +588,43 @@ void launch_multi_lines(const LinesJob* jobs, const uint64_t* line_start,
   ECL_HIP(hipGetLastError());
 }
 
+// One thread per select line of one array (layout at LevelView::sel_lines):
+// lines [0, n1) for the ones at rate r1, [n1, n1 + n0) for the zeros at r0.
+// The line's start comes from a select over the array's directory (and its
+// hints when they exist); its 960 bits are copied from the 64-bit word that
+// holds it (zeros past the array).
+__global__ __launch_bounds__(256) void select_lines_kernel(LevelView v, uint4* __restrict__ out,
+                                                           uint64_t n1, uint64_t n0, uint32_t r1,
+                                                           uint32_t r0) {
+  const uint64_t t = static_cast<uint64_t>(blockIdx.x) * 256 + threadIdx.x;
+  if (t >= n1 + n0) return;
+  const bool one = t < n1;
+  const uint64_t j = one ? t : t - n1;
+  uint64_t s = 0;
+  if (j != 0) s = one ? detail::level_select<1, true>(v, j * r1) : detail::level_select<0, true>(v, j * r0);
+  const uint64_t nw = (v.size + 31) / 32;
+  const uint64_t w0 = (s >> 6) * 2;
+  uint32_t w[30];
+#pragma unroll
+  for (uint32_t k = 0; k < 30; ++k) w[k] = w0 + k < nw ? v.words[w0 + k] : 0u;
+  uint4* o = out + t * 8;
+  o[0] = make_uint4(static_cast<uint32_t>(s), static_cast<uint32_t>(s >> 32), w[0], w[1]);
+#pragma unroll
+  for (uint32_t k = 1; k < 8; ++k) o[k] = make_uint4(w[4 * k - 2], w[4 * k - 1], w[4 * k], w[4 * k + 1]);
+}
+
+void launch_select_lines(const LevelView& v, uint4* out, uint64_t n1, uint64_t n0, uint32_t r1,
+                         uint32_t r0, hipStream_t s) {
+  if (n1 + n0 == 0) return;
+  const uint64_t blocks = (n1 + n0 + 255) / 256;
+  if (blocks > 0x7FFFFFFFull) throw Error(ECL_ERROR_INVALID_ARGUMENT, "bit array too large");
+  LevelView plain = v;
+  plain.sel_lines = nullptr;  // starts come from the directory (and hints)
+  hipLaunchKernelGGL(select_lines_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(256), 0, s,
+                     plain, out, n1, n0, r1, r0);
+  ECL_HIP(hipGetLastError());
+}
+
 void launch_query_lines(const LevelView& v, uint4* lines, uint64_t nlines, hipStream_t s) {
   if (nlines == 0) return;
   const uint64_t blocks = (nlines * 8 + 255) / 256;

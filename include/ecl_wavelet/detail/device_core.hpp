@@ -172,6 +172,15 @@ struct LevelView {
   // that Value's brackets fit one hint; otherwise every select would read a
   // hint only to fall back to the directory search)
   uint32_t hint_values;
+  // Optional select lines (nullptr when not built): for Value v with rate
+  // r = sel_rate[v] > 0, line j (j = 0 .. total_v / r; the zeros' lines from
+  // line sel_zero_line on) is the u64 position s_j of the (j r)-th Value (0
+  // for j = 0) followed by the 960 bits from bit s_j & ~63.  r follows the
+  // Value's density, so the next r Values lie in those bits; a select of
+  // Value v then reads one line.
+  const uint4* sel_lines;
+  uint64_t sel_zero_line;
+  uint32_t sel_rate[2];
 };
 constexpr uint32_t kLineBits = 960;  // bits per query line (30 words)
 constexpr uint32_t kHintBlocks = 23;  // blocks after b0 one hint covers
@@ -393,11 +402,56 @@ __device__ __forceinline__ uint64_t select_in_block(const LevelView& v, uint64_t
   return res;  // size only for an inconsistent directory
 }
 
+// The i-th Value (0 < i <= total) from its select line; size when the
+// answer lies past the line's bits (the caller then searches otherwise).
+template <int Value>
+__device__ __forceinline__ uint64_t select_from_line(const LevelView& v, uint64_t i,
+                                                     uint32_t rate) {
+  const uint64_t jj = i / rate;
+  const uint4* p = v.sel_lines + ((Value ? 0 : v.sel_zero_line) + jj) * 8;
+  uint4 q[8];
+#pragma unroll
+  for (int k = 0; k < 8; ++k) q[k] = p[k];
+  const uint64_t s = (uint64_t)q[0].x | ((uint64_t)q[0].y << 32);
+  uint64_t w[15];
+  w[0] = (uint64_t)q[0].z | ((uint64_t)q[0].w << 32);
+#pragma unroll
+  for (int k = 1; k < 8; ++k) {
+    w[2 * k - 1] = (uint64_t)q[k].x | ((uint64_t)q[k].y << 32);
+    if (2 * k < 15) w[2 * k] = (uint64_t)q[k].z | ((uint64_t)q[k].w << 32);
+  }
+  // counted from s (the (jj rate)-th Value itself, or bit 0 for jj = 0)
+  uint64_t r = jj ? i - jj * rate + 1 : i;
+  const uint32_t off = static_cast<uint32_t>(s & 63);
+  uint64_t pos = s & ~63ull, res = v.size;
+  bool found = false;
+#pragma unroll
+  for (uint32_t k = 0; k < 15; ++k) {
+    uint64_t x = Value ? w[k] : ~w[k];
+    if (k == 0) x &= ~0ull << off;
+    const uint32_t c = static_cast<uint32_t>(__popcll(x));
+    if (!found) {
+      if (r <= c) {
+        res = pos + select_in_word(x, static_cast<uint32_t>(r));
+        found = true;
+      } else {
+        r -= c;
+        pos += 64;
+      }
+    }
+  }
+  return res;
+}
+
 template <int Value, bool WideL2 = false>
 __device__ __forceinline__ uint64_t level_select(const LevelView& v,
                                                  uint64_t i) {
   const uint64_t total = Value ? v.ones : v.size - v.ones;
   if (i == 0 || i > total) return v.size;
+  if (v.sel_lines != nullptr && v.sel_rate[Value] != 0) {
+    const uint64_t r = select_from_line<Value>(v, i, v.sel_rate[Value]);
+    if (r < v.size) return r;
+  }
   const uint64_t j = i / kSampleRate;
   const uint64_t nsamp = total / kSampleRate;
   if (v.sel_hints != nullptr && ((v.hint_values >> Value) & 1u)) {

@@ -255,6 +255,15 @@ ecl_status ecl_wavelet_tree_destroy(ecl_wavelet_tree wt);
  * keeps the existing lines. */
 ecl_status ecl_wavelet_tree_build_query_lines(ecl_wavelet_tree wt);
 ecl_status ecl_rank_select_build_query_lines(ecl_rank_select rs);
+/* Select lines (no counterpart in the reference; opt-in, independent of the
+ * query lines): for ones and zeros of every array, one 128-byte line every r
+ * of them, holding the position of that one (or zero) and the 960 bits from
+ * the 64-bit word that holds it, with r = max(1, floor(800 x density)) so
+ * that the next r lie in those bits.  A select then reads one memory line
+ * per level.  Costs about 0.32 bytes per bit (2.7x the bits); results do not
+ * change.  A second call keeps the existing lines. */
+ecl_status ecl_wavelet_tree_build_select_lines(ecl_wavelet_tree wt);
+ecl_status ecl_rank_select_build_select_lines(ecl_rank_select rs);
 /* Persistence (no counterpart in the reference, which rebuilds every tree
  * from its text; for warm starts of servers).  The file holds a header, the
  * alphabet, the symbol counts and every level's bits; loading checks them

@@ -252,15 +252,19 @@ def test_binary_2e34_bits(gpu, p):
     assert np.array_equal(_rank_from_index(words, l1, l2, pos), k - np.uint64(1))
 
 
+@pytest.mark.parametrize("order", ["query_lines_first", "select_lines_first"])
 @pytest.mark.parametrize("p", [0.3, 0.02, 0.985])
-def test_query_lines_binary(gpu, p):
+def test_query_lines_binary(gpu, p, order):
     """Binary rank and select of ones and zeros with query lines and select
     hints built equal the oracle's scans and the answers without them, for
     balanced (0.3), sparse (0.02) and dense (0.985) arrays: at 0.3 every
     sample bracket fits one hint (at most 24 blocks); the rare Value of the
     sparse and dense arrays has brackets wider than that and takes the
-    directory search.  The select arguments include every sample multiple
-    (4096 j), its neighbours, the total and one past it."""
+    directory search.  Select lines (a line every r = floor(800 density)
+    ones or zeros) are built before or after the query lines and give the
+    same answers.  The select arguments include every sample multiple
+    (4096 j) and every multiple of r, their neighbours, the total and one
+    past it."""
     from ecl_wavelet_amd import BitArray, RankSelect
     m = (1 << 22) + 4321
     thr = O.bernoulli_threshold(p)
@@ -279,16 +283,22 @@ def test_query_lines_binary(gpu, p):
         for v in (0, 1):
             total = rs.getTotalNumVals(i, v)
             samp = np.arange(4096, total + 1, 4096, dtype=np.int64)
+            r = max(1, 800 * total // mm) if total else 1
+            lines = np.arange(r, total + 1, r, dtype=np.int64)[:4000]
             k = np.concatenate([rng.integers(1, total + 2, 3000), samp, samp - 1, samp + 1,
-                                [0, 1, total, total + 1]])
+                                lines, lines - 1, lines + 1, [0, 1, total, total + 1]])
             kv.append(k[k >= 0].astype(np.uint64))
         ks.append(kv)
     before = [[rs.select(i, ks[i][v], v) for v in (0, 1)] for i in range(len(sizes))]
-    rs.buildQueryLines()
-    for i, mm in enumerate(sizes):
-        for v in (0, 1):
-            assert np.array_equal(rs.rank(i, idxs[i], v),
-                                  O.scan_bits_batch(words[i], mm, v, idxs[i]))
-            got = rs.select(i, ks[i][v], v)
-            assert np.array_equal(got, O.scan_bits_batch(words[i], mm, 2 + v, ks[i][v]))
-            assert np.array_equal(got, before[i][v])
+    steps = [rs.buildQueryLines, rs.buildSelectLines]
+    if order == "select_lines_first":
+        steps.reverse()
+    for step in steps:
+        step()
+        for i, mm in enumerate(sizes):
+            for v in (0, 1):
+                assert np.array_equal(rs.rank(i, idxs[i], v),
+                                      O.scan_bits_batch(words[i], mm, v, idxs[i]))
+                got = rs.select(i, ks[i][v], v)
+                assert np.array_equal(got, O.scan_bits_batch(words[i], mm, 2 + v, ks[i][v]))
+                assert np.array_equal(got, before[i][v])

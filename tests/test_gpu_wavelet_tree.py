@@ -506,6 +506,10 @@ def test_query_lines_same_answers(gpu, case):
     assert np.array_equal(got, before[2])
     assert np.array_equal(got, O.scan_select_batch(text, np.concatenate(sk),
                                                    np.concatenate(sc).astype(np.uint64)))
+    wt.buildSelectLines()
+    wt.buildSelectLines()  # a second call keeps the lines
+    assert np.array_equal(wt.select(sq.copy()), before[2])
+    assert np.array_equal(wt.access(idx), before[0])
     assert np.array_equal(wt.access(idx), text[idx])
     _check_queries(wt, text, None, nq=3000, seed=4)
 
