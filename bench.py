@@ -10,12 +10,14 @@ mixed-kind batch in a single launch; the query lines; the same queries
 again over the lines.
 
 value = aggregate device-resident query throughput of the per-kind calls
-over the tree with its query lines (access+rank+select queries / their
-kernel time); the query lines (a second copy of the level bits in 128-byte
-lines headed by their rank, built on the finished tree) are built inside
-each step and timed on their own (query_lines_build_ms), and the same
-queries over the plain directory, timed in the same step before the lines
-exist, are reported under plain_directory.  build_gbps = text bytes / build
+over the tree with its query structures (access+rank+select queries / their
+kernel time).  The query lines (the level bits in 128-byte lines headed by
+their rank, with select hints) and the select lines (a 128-byte line every r
+ones or zeros: its position and the 960 bits from there) are built on the
+finished tree inside each step and timed on their own
+(query_lines_build_ms, select_lines_build_ms; ms_per_step includes them);
+the same queries over the plain directory, timed in the same step before
+those exist, are reported under plain_directory.  build_gbps = text bytes / build
 time (the tree and its directory, without the lines).
 
 The roofline object is for the dominant kernel (the construction pass,
@@ -655,10 +657,10 @@ def main():
     d_out = E.DeviceBuffer(3 * Q * 8)
     timer = E.Timer()
 
-    build_ms, lines_ms, q_ms, mixed_ms = [], [], [], []
+    build_ms, lines_ms, sel_ms, q_ms, mixed_ms = [], [], [], [], []
     kind_ms = {"access": [], "rank": [], "select": []}
     plain_ms = {"access": [], "rank": [], "select": [], "mixed": []}
-    lines_bytes = 0
+    lines_bytes = sel_bytes = 0
     wt = None
     pass_launch_ms = []
     build_kernels = {}
@@ -687,17 +689,24 @@ def main():
         timer.start()
         wt.buildQueryLines()
         lb = timer.stop()
-        lines_bytes = free0 - E.mem_info()[0]
+        free1 = E.mem_info()[0]
+        lines_bytes = free0 - free1
+        timer.start()
+        wt.buildSelectLines()
+        sb = timer.stop()
+        sel_bytes = free1 - E.mem_info()[0]
         times = run_queries(wt, timer, Q, d_idx, d_rq, d_sq, d_mq, d_out)
         mx = times.pop("mixed")
         log(f"step {step}{'' if timed else ' (warmup)'}: build {b:.1f} ms, "
             f"plain access {plain['access']:.2f} ms, rank {plain['rank']:.2f} ms, "
-            f"select {plain['select']:.2f} ms, mixed {plain['mixed']:.2f} ms; lines {lb:.2f} ms; "
+            f"select {plain['select']:.2f} ms, mixed {plain['mixed']:.2f} ms; lines {lb:.2f} ms, "
+            f"select lines {sb:.2f} ms; "
             f"access {times['access']:.2f} ms, rank {times['rank']:.2f} ms, "
             f"select {times['select']:.2f} ms, mixed {mx:.2f} ms")
         if timed:
             build_ms.append(b)
             lines_ms.append(lb)
+            sel_ms.append(sb)
             q_ms.append(sum(times.values()))
             mixed_ms.append(mx)
             for k, v in times.items():
@@ -750,8 +759,8 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": float(np.mean(np.array(build_ms) + np.array(lines_ms) + np.array(q_ms)
-                                     + np.array(mixed_ms))),
+        "ms_per_step": float(np.mean(np.array(build_ms) + np.array(lines_ms) + np.array(sel_ms)
+                                     + np.array(q_ms) + np.array(mixed_ms))),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -761,13 +770,15 @@ def main():
             "workload": (f"wavelet tree over a uint8 text of n={n} uniform over "
                          f"sigma={sigma} (device-resident); per step: 1 build, "
                          f"{Q} each of access/rank/select + one mixed batch of {3 * Q} "
-                         f"over the plain directory, the query lines, the same "
-                         f"queries over the lines"),
+                         f"over the plain directory, the query lines and select "
+                         f"lines, the same queries over them"),
             "n": n, "sigma": sigma, "queries_per_kind": Q, "levels": L,
         },
         "build_ms": build,
         "query_lines_build_ms": float(np.median(lines_ms)),
         "query_lines_bytes": int(lines_bytes),
+        "select_lines_build_ms": float(np.median(sel_ms)),
+        "select_lines_bytes": int(sel_bytes),
         "build_gbps": n / (build / 1e3) / 1e9,
         "build_algorithmic_gbps": build_bytes / (build / 1e3) / 1e9,
         "build_kernels_ms": kern,
@@ -787,7 +798,7 @@ def main():
             **{kind: {"lines_per_query": lpq,
                       "achieved": Q / (np.mean(kind_ms[kind]) / 1e3) * lpq,
                       "frac": Q / (np.mean(kind_ms[kind]) / 1e3) * lpq / line_peak}
-               for kind, lpq in (("access", L), ("rank", L), ("select", 2 * L))},
+               for kind, lpq in (("access", L), ("rank", L), ("select", L))},
         },
         "plain_directory": {
             "access_qps": Q / (np.mean(plain_ms["access"]) / 1e3),

@@ -1515,6 +1515,7 @@ static TreeQueryParams query_params(const ecl_wavelet_tree_t* wt, uint32_t flags
   p.alphabet_out = (!wt->min_alphabet && !(flags & ECL_RESULTS_MINIMAL)) ? wt->d_alphabet.ptr
                                                                          : nullptr;
   p.has_lines = wt->rs->lines.ptr != nullptr ? 1 : 0;
+  p.has_sel_lines = wt->rs->sel_lines.ptr != nullptr ? 1 : 0;
   return p;
 }
 

@@ -201,6 +201,7 @@ struct TreeQueryParams {
   const void* alphabet_out;  // access: map result back (nullable = identity)
   uint64_t max_lds_counts;   // unused by callers; tuning knob
   int has_lines;             // 1: the tree's levels carry query lines
+  int has_sel_lines;         // 1: the tree's levels carry select lines
 };
 void launch_tree_access(const TreeQueryParams& p, const uint64_t* idx,
                         uint64_t m, void* out, hipStream_t s);

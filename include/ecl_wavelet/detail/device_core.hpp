@@ -443,12 +443,15 @@ __device__ __forceinline__ uint64_t select_from_line(const LevelView& v, uint64_
   return res;
 }
 
-template <int Value, bool WideL2 = false>
+// SelLines: consult the select lines when the array has them (the query
+// kernels of trees without them compile the path out, which keeps their
+// register use and occupancy).
+template <int Value, bool WideL2 = false, bool SelLines = true>
 __device__ __forceinline__ uint64_t level_select(const LevelView& v,
                                                  uint64_t i) {
   const uint64_t total = Value ? v.ones : v.size - v.ones;
   if (i == 0 || i > total) return v.size;
-  if (v.sel_lines != nullptr && v.sel_rate[Value] != 0) {
+  if (SelLines && v.sel_lines != nullptr && v.sel_rate[Value] != 0) {
     const uint64_t r = select_from_line<Value>(v, i, v.sel_rate[Value]);
     if (r < v.size) return r;
   }
@@ -663,7 +666,8 @@ __device__ __forceinline__ uint64_t tree_prev_char_start(
 }
 
 // select: selectKernel early-out + bottom-up ascent (wavelet_tree.cuh:
-// 532-535, 2007-2079)
+// 532-535, 2007-2079).  SelLines as for level_select.
+template <bool SelLines = true>
 __device__ __forceinline__ uint64_t tree_select(const TreeView& t,
                                                 const LevelView* lv,
                                                 const uint64_t* counts,
@@ -696,12 +700,12 @@ __device__ __forceinline__ uint64_t tree_select(const TreeView& t,
       start = cs == 0 ? 0
                       : counts[cs] - ranks[rs + tree_node_pos(
                                                     t, cs, static_cast<uint32_t>(l))];
-      result = level_select<1>(v, start + result) + 1;
+      result = level_select<1, false, SelLines>(v, start + result) + 1;
     } else {
       start = cs == 0
                   ? 0
                   : ranks[rs + tree_node_pos(t, cs, static_cast<uint32_t>(l))];
-      result = level_select<0>(v, start + result) + 1;
+      result = level_select<0, false, SelLines>(v, start + result) + 1;
     }
     if (static_cast<uint32_t>(l) == clen - 1 && result > v.size) {
       result = t.n + 1;

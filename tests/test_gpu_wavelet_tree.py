@@ -518,10 +518,11 @@ def test_query_lines_same_answers(gpu, case):
 @pytest.mark.parametrize("case", ["u8_256", "u16_1000", "u32_sparse", "tiny"])
 def test_query_lanes(gpu, lanes, case):
     """access and rank with one or two lanes per query (the pair fetches half
-    of each level's bits per lane; 0 chooses by the presence of query lines)
-    and select (one lane) equal the oracle's scans, over the plain directory
-    and over query lines with select hints, including positions at block
-    (512) and line (960) boundaries and every occurrence of one symbol."""
+    of each level's line per lane; 0 chooses two lanes over query lines) and
+    select (one lane) equal the oracle's scans over the plain directory, over
+    query lines with select hints, and with select lines too, including
+    positions at block (512) and line (960) boundaries and every occurrence
+    of one symbol."""
     from ecl_wavelet_amd import WaveletTree, set_query_lanes
     rng = np.random.default_rng(lanes * 7 + len(case))
     if case == "u8_256":
@@ -553,12 +554,14 @@ def test_query_lanes(gpu, lanes, case):
         sc = sc.astype(text.dtype)
         sk = sk.astype(np.uint64)
         exp_s = O.scan_select_batch(text, sk, sc.astype(np.uint64))
-        for lines in (False, True):
-            if lines:
+        for stage in ("plain", "query_lines", "select_lines"):
+            if stage == "query_lines":
                 wt.buildQueryLines()
-            assert np.array_equal(wt.access(idx), text[idx])
-            assert np.array_equal(wt.rank(wt.make_queries(ri, rc)), exp_r)
-            assert np.array_equal(wt.select(wt.make_queries(sk, sc)), exp_s)
+            elif stage == "select_lines":
+                wt.buildSelectLines()
+            assert np.array_equal(wt.access(idx), text[idx]), stage
+            assert np.array_equal(wt.rank(wt.make_queries(ri, rc)), exp_r), stage
+            assert np.array_equal(wt.select(wt.make_queries(sk, sc)), exp_s), stage
     finally:
         set_query_lanes(0)
 
