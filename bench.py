@@ -369,7 +369,8 @@ def bench_binary(args):
     drawn on the device, p in {0.5, 0.05}; directory build time, then
     bin_queries rank1(i) with uniform i and select1(k) with uniform k over
     the popcount, generated on the device (timed with their generation);
-    then the same queries over the query lines and select hints."""
+    then the same queries over the query lines and select hints, and select
+    over select lines."""
     import ecl_wavelet_amd as E
     m = 1 << args.bin_log2
     Q = args.bin_queries
@@ -424,10 +425,21 @@ def bench_binary(args):
             "rank1_qps_query_lines": Q / (rank_l_ms / 1e3),
             "select1_qps_query_lines": Q / (select_l_ms / 1e3),
         }
+        # and select over select lines
+        timer.start()
+        rs.buildSelectLines()
+        r["select_lines_build_ms"] = timer.stop()
+        rs.selectGenerated(0, 1, Q, out, 1)  # warm-up
+        E.synchronize()
+        timer.start()
+        rs.selectGenerated(0, 12, Q, out, 1)
+        r["select1_qps_select_lines"] = Q / (timer.stop() / 1e3)
         res[f"p={p}"] = r
         log(f"binary p={p}: build {build_ms:.2f} ms, rank1 {r['rank1_qps']:.3g} q/s, "
             f"select1 {r['select1_qps']:.3g} q/s; lines {lines_ms:.2f} ms, rank1 "
-            f"{r['rank1_qps_query_lines']:.3g} q/s, select1 {r['select1_qps_query_lines']:.3g} q/s")
+            f"{r['rank1_qps_query_lines']:.3g} q/s, select1 {r['select1_qps_query_lines']:.3g} q/s; "
+            f"select lines {r['select_lines_build_ms']:.2f} ms, select1 "
+            f"{r['select1_qps_select_lines']:.3g} q/s")
         del rs
     del out
     return res
