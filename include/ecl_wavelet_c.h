@@ -252,7 +252,8 @@ ecl_status ecl_wavelet_tree_destroy(ecl_wavelet_tree wt);
  * so that a select reads its hint and one block.  Costs 128/120 + 1/64 of
  * the bits in memory and one pass over them; results do not change.  The
  * reference-shaped directory and samples stay as they are.  A second call
- * keeps the existing lines. */
+ * keeps the existing lines.  The call rewrites the tree's device level views:
+ * make it while no query on this tree is in flight. */
 ecl_status ecl_wavelet_tree_build_query_lines(ecl_wavelet_tree wt);
 ecl_status ecl_rank_select_build_query_lines(ecl_rank_select rs);
 /* Select lines (no counterpart in the reference; opt-in, independent of the
@@ -261,7 +262,8 @@ ecl_status ecl_rank_select_build_query_lines(ecl_rank_select rs);
  * the 64-bit word that holds it, with r = max(1, floor(800 x density)) so
  * that the next r lie in those bits.  A select then reads one memory line
  * per level.  Costs about 0.32 bytes per bit (2.7x the bits); results do not
- * change.  A second call keeps the existing lines. */
+ * change.  A second call keeps the existing lines.  Like the query lines,
+ * build them while no query on this tree is in flight. */
 ecl_status ecl_wavelet_tree_build_select_lines(ecl_wavelet_tree wt);
 ecl_status ecl_rank_select_build_select_lines(ecl_rank_select rs);
 /* Persistence (no counterpart in the reference, which rebuilds every tree
@@ -348,7 +350,8 @@ ecl_status ecl_forest_create(const void* d_texts, const uint64_t* starts, const 
 ecl_status ecl_forest_destroy(ecl_forest f);
 /* Query lines and select hints (see ecl_wavelet_tree_build_query_lines) for
  * every tree of the forest, in three launches; later forest queries and
- * graphs (also graphs captured before) read them.  No counterpart in the
+ * graphs (also graphs captured before) read them.  Make the call while no
+ * forest query or graph replay is in flight.  No counterpart in the
  * reference. */
 ecl_status ecl_forest_build_query_lines(ecl_forest f);
 ecl_status ecl_forest_num_trees(ecl_forest f, size_t* out);
