@@ -583,7 +583,7 @@ def pmc_queries(args):
     """Kernels for the query counter passes: the copy calibration, then one
     build and 10^7 device-generated queries per kind over the benchmark
     tree (each batch launched once), then the query lines and the same three
-    batches over them."""
+    batches over them, then the select lines and the select batch again."""
     import ctypes as C
     import ecl_wavelet_amd as E
     hip = C.CDLL("libamdhip64.so")
@@ -608,14 +608,17 @@ def pmc_queries(args):
     wt.rankDevice(d_rq, Q, d_out, flags=1)
     wt.selectDevice(d_sq, Q, d_out, flags=1)
     E.synchronize()
-    # the same batches again over the query lines
+    # the same batches again over the query lines, then select over select lines
     wt.buildQueryLines()
     wt.accessDevice(d_idx, Q, d_out, flags=2)
     wt.rankDevice(d_rq, Q, d_out, flags=1)
     wt.selectDevice(d_sq, Q, d_out, flags=1)
     E.synchronize()
+    wt.buildSelectLines()
+    wt.selectDevice(d_sq, Q, d_out, flags=1)
+    E.synchronize()
     log(f"queries: {Q} access, rank and select over n={args.n}, sigma={args.sigma}, "
-        f"without and with query lines")
+        f"without and with query lines; select with select lines")
 
 
 def run_queries(wt, timer, Q, d_idx, d_rq, d_sq, d_mq, d_out):
