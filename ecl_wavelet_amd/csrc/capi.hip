@@ -1304,6 +1304,9 @@ static void build_query_lines(ecl_rank_select_t* rs) {
   rs->lines = std::move(lines);
 }
 
+// The multiplier that turns i / r into a multiply-high (LevelView::sel_magic).
+static uint64_t select_line_magic(uint32_t r) { return r > 1 ? ~0ull / r + 1 : 0; }
+
 // Select lines for every array of a directory (opt-in): for each Value a
 // line every r Values, r = max(1, floor(800 x density)) so that the next r
 // Values lie in the line's 960 bits; the level views point to them
@@ -1344,6 +1347,8 @@ static void build_select_lines(ecl_rank_select_t* rs) {
     lv[i].sel_zero_line = n1[i];
     lv[i].sel_rate[1] = r1[i];
     lv[i].sel_rate[0] = r0[i];
+    lv[i].sel_magic[1] = select_line_magic(r1[i]);
+    lv[i].sel_magic[0] = select_line_magic(r0[i]);
   }
   h2d(rs->d_levels.ptr, lv.data(), na);
   rs->levels = std::move(lv);

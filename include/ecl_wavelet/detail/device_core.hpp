@@ -181,6 +181,10 @@ struct LevelView {
   const uint4* sel_lines;
   uint64_t sel_zero_line;
   uint32_t sel_rate[2];
+  // floor((2^64 - 1) / r) + 1 for r = sel_rate[v] > 1: i / r is then the
+  // high word of i * sel_magic[v] for every i < 2^64 / r (the counts here are
+  // below 2^44), which spares a 64-bit division per level
+  uint64_t sel_magic[2];
 };
 constexpr uint32_t kLineBits = 960;  // bits per query line (30 words)
 constexpr uint32_t kHintBlocks = 23;  // blocks after b0 one hint covers
@@ -407,7 +411,7 @@ __device__ __forceinline__ uint64_t select_in_block(const LevelView& v, uint64_t
 template <int Value>
 __device__ __forceinline__ uint64_t select_from_line(const LevelView& v, uint64_t i,
                                                      uint32_t rate) {
-  const uint64_t jj = i / rate;
+  const uint64_t jj = rate == 1 ? i : __umul64hi(i, v.sel_magic[Value]);  // i / rate
   const uint4* p = v.sel_lines + ((Value ? 0 : v.sel_zero_line) + jj) * 8;
   uint4 q[8];
 #pragma unroll
@@ -561,7 +565,14 @@ __device__ __forceinline__ uint64_t tree_nodes_at(const TreeView& t,
   return t.pow2 ? (1ull << l) - 1 : nal[l];
 }
 
-// getNodePosAtLevel (tree/wavelet_tree.cuh:2241-2268)
+// x / d for a divisor d that is a power of two (a shift; d == 0 keeps the
+// division, which valid trees never reach)
+__device__ __forceinline__ uint64_t div_pow2(uint64_t x, uint64_t d) {
+  return d != 0 ? x >> __builtin_ctzll(d) : x / d;
+}
+
+// getNodePosAtLevel (tree/wavelet_tree.cuh:2241-2268); the node lengths are
+// powers of two (a power of two shifted right), so the divisions are shifts
 __device__ __forceinline__ uint64_t tree_node_pos(const TreeView& t,
                                                   uint64_t symbol,
                                                   uint32_t level) {
@@ -576,11 +587,11 @@ __device__ __forceinline__ uint64_t tree_node_pos(const TreeView& t,
       node_len = sub >> i;
       if (symbol <= sub) break;
       remaining -= sub;
-      prev += sub / node_len;
+      prev += div_pow2(sub, node_len);
       symbol -= sub;
     }
   }
-  return symbol / node_len + prev - 1;
+  return div_pow2(symbol, node_len) + prev - 1;
 }
 
 // access: top-down descent (tree/wavelet_tree.cuh:1815-1877).  Pair: the
