@@ -603,10 +603,20 @@ __global__ __launch_bounds__(256) void select_lines_kernel(LevelView v, uint4* _
   uint64_t s = 0;
   if (j != 0) s = one ? detail::level_select<1, true>(v, j * r1) : detail::level_select<0, true>(v, j * r0);
   const uint64_t nw = (v.size + 31) / 32;
-  const uint64_t w0 = (s >> 6) * 2;
+  const uint64_t w0 = (s >> 6) * 2;  // even: the 120 bytes start 8-byte aligned
   uint32_t w[30];
+  if (w0 + 30 <= nw) {
+    const uint2* p2 = reinterpret_cast<const uint2*>(v.words + w0);
 #pragma unroll
-  for (uint32_t k = 0; k < 30; ++k) w[k] = w0 + k < nw ? v.words[w0 + k] : 0u;
+    for (uint32_t k = 0; k < 15; ++k) {
+      const uint2 x = p2[k];
+      w[2 * k] = x.x;
+      w[2 * k + 1] = x.y;
+    }
+  } else {
+#pragma unroll
+    for (uint32_t k = 0; k < 30; ++k) w[k] = w0 + k < nw ? v.words[w0 + k] : 0u;
+  }
   uint4* o = out + t * 8;
   o[0] = make_uint4(static_cast<uint32_t>(s), static_cast<uint32_t>(s >> 32), w[0], w[1]);
 #pragma unroll
