@@ -7,7 +7,8 @@
  *
  * Every function restates the behaviour of the reference library
  * (mfranzreb/ECL_wavelet); the file:line it follows is cited at each
- * definition in ecl_oracle.c.  Parity pins are listed in DESIGN.md.
+ * definition in ecl_oracle.c.  The checks that pin each function are in
+ * tests/test_oracle.py.
  */
 #ifndef ECL_ORACLE_H
 #define ECL_ORACLE_H
