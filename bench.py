@@ -449,7 +449,8 @@ def bench_full_sweep(args):
     """The full-size configurations beside the headline one: uint8 texts of
     6e9 symbols uniform over sigma in {4, 16, 64}, and a uint16 text of 2e9
     symbols Zipf(1.1) over sigma = 2^16.  Per text: median of two timed
-    builds after one untimed, then 10^7 device-generated queries per kind."""
+    builds after one untimed, then 10^7 device-generated queries per kind,
+    over the plain directory and again over the query and select lines."""
     import ecl_wavelet_amd as E
     timer = E.Timer()
     Q = 10_000_000
@@ -474,23 +475,39 @@ def bench_full_sweep(args):
         wt.generateQueries(1, 32, Q, d_rq)
         wt.generateQueries(2, 33, Q, d_sq)
         E.synchronize()
+        calls = (("access", lambda: wt.accessDevice(d_idx, Q, d_out, flags=2)),
+                 ("rank", lambda: wt.rankDevice(d_rq, Q, d_out, flags=1)),
+                 ("select", lambda: wt.selectDevice(d_sq, Q, d_out, flags=1)))
         t = {}
-        for kind, call_ in (("access", lambda: wt.accessDevice(d_idx, Q, d_out, flags=2)),
-                            ("rank", lambda: wt.rankDevice(d_rq, Q, d_out, flags=1)),
-                            ("select", lambda: wt.selectDevice(d_sq, Q, d_out, flags=1))):
+        for kind, call_ in calls:
             timer.start()
             call_()
             t[kind] = timer.stop()
+        # the same queries over query lines, select hints and select lines
+        timer.start()
+        wt.buildQueryLines()
+        wt.buildSelectLines()
+        lines_ms = timer.stop()
+        tl = {}
+        for kind, call_ in calls:
+            timer.start()
+            call_()
+            tl[kind] = timer.stop()
         build_ms = float(np.median(builds))
         r = {"dtype": np.dtype(dt).name, "n": n, "sigma": sigma,
              "distribution": "uniform" if thr is None else "zipf(1.1)",
              "levels": int(wt.getNumLevels()), "build_ms": build_ms,
              "build_gbps": n * np.dtype(dt).itemsize / (build_ms / 1e3) / 1e9,
              "access_qps": Q / (t["access"] / 1e3), "rank_qps": Q / (t["rank"] / 1e3),
-             "select_qps": Q / (t["select"] / 1e3)}
+             "select_qps": Q / (t["select"] / 1e3),
+             "query_and_select_lines_build_ms": lines_ms,
+             "access_qps_lines": Q / (tl["access"] / 1e3), "rank_qps_lines": Q / (tl["rank"] / 1e3),
+             "select_qps_lines": Q / (tl["select"] / 1e3)}
         out.append(r)
         log(f"sweep {r['dtype']} sigma={sigma} n={n}: build {build_ms:.1f} ms, access "
-            f"{r['access_qps']:.3g}, rank {r['rank_qps']:.3g}, select {r['select_qps']:.3g} q/s")
+            f"{r['access_qps']:.3g}, rank {r['rank_qps']:.3g}, select {r['select_qps']:.3g} q/s; "
+            f"with lines ({lines_ms:.1f} ms): access {r['access_qps_lines']:.3g}, rank "
+            f"{r['rank_qps_lines']:.3g}, select {r['select_qps_lines']:.3g} q/s")
         del wt, d_text
     return out
 
