@@ -6,8 +6,8 @@ from sigma = 256 symbols, resident in HBM.  One step = one wavelet-tree
 build over the resident text; Q queries of each kind (access, rank, select;
 uniform positions/symbols, drawn on the device before the timed region)
 through the device-resident entry points, plus the same 3Q queries as one
-mixed-kind batch in a single launch; the query lines; the same queries
-again over the lines.
+mixed-kind batch in a single launch; the query lines and the select lines;
+the same queries again over them.
 
 value = aggregate device-resident query throughput of the per-kind calls
 over the tree with its query structures (access+rank+select queries / their
@@ -17,8 +17,10 @@ ones or zeros: its position and the 960 bits from there) are built on the
 finished tree inside each step and timed on their own
 (query_lines_build_ms, select_lines_build_ms; ms_per_step includes them);
 the same queries over the plain directory, timed in the same step before
-those exist, are reported under plain_directory.  build_gbps = text bytes / build
-time (the tree and its directory, without the lines).
+those exist, are reported under plain_directory.  build_gbps = text bytes /
+build time (the tree and its directory, without the lines).  query_roofline
+prices the per-kind rates against the device's measured rate of random,
+dependent 128-byte line reads (one line per level and query).
 
 The roofline object is for the dominant kernel (the construction pass,
 which writes the bits of every level from one read of the text), with
