@@ -137,12 +137,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void t
   }
 }
 
-// Occupancy target: with select lines, 6 waves per SIMD (80 VGPRs, no
-// spills, instead of the compiler's 123 and 4 waves): 2.79 vs 3.00 ms per
-// 10^7 at n = 6e9, sigma = 256.  Without them the compiler's choice is kept:
-// more waves made the hint and directory searches slower (4.71 vs 3.94 ms).
+// Occupancy target: with select lines (and without the hint path, below),
+// 7 waves per SIMD (68 VGPRs, no spills, instead of the compiler's 123 and 4
+// waves): 2.7 vs 3.0 ms per 10^7 at n = 6e9, sigma = 256.  Without them the
+// compiler's choice is kept: more waves made the hint and directory searches
+// slower (4.71 vs 3.94 ms).
 template <typename T, bool SelLines>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SelLines ? 6 : 1))) void tree_select_kernel(
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SelLines ? 7 : 1))) void tree_select_kernel(
     TreeQueryParams p, const uint8_t* __restrict__ qs, uint64_t m,
     uint64_t* __restrict__ out) {
   extern __shared__ uint8_t s_meta[];
@@ -153,8 +154,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SelLines ? 
     const uint8_t* e = qs + q * 16;
     const uint64_t k = *reinterpret_cast<const uint64_t*>(e);
     const uint64_t c = map_symbol<T>(p, load_sym(e, sizeof(T)));
-    out[q] = detail::tree_select<SelLines>(p.view, mt.lv, mt.counts, mt.ranks, mt.nal,
-                                           c, k);
+    // with select lines the hints would only serve the rare overflow of a
+    // line: the directory search does that, with fewer registers
+    out[q] = detail::tree_select<SelLines, !SelLines>(p.view, mt.lv, mt.counts, mt.ranks,
+                                                      mt.nal, c, k);
   }
 }
 

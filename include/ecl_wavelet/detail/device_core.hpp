@@ -450,7 +450,7 @@ __device__ __forceinline__ uint64_t select_from_line(const LevelView& v, uint64_
 // SelLines: consult the select lines when the array has them (the query
 // kernels of trees without them compile the path out, which keeps their
 // register use and occupancy).
-template <int Value, bool WideL2 = false, bool SelLines = true>
+template <int Value, bool WideL2 = false, bool SelLines = true, bool Hints = true>
 __device__ __forceinline__ uint64_t level_select(const LevelView& v,
                                                  uint64_t i) {
   const uint64_t total = Value ? v.ones : v.size - v.ones;
@@ -461,7 +461,7 @@ __device__ __forceinline__ uint64_t level_select(const LevelView& v,
   }
   const uint64_t j = i / kSampleRate;
   const uint64_t nsamp = total / kSampleRate;
-  if (v.sel_hints != nullptr && ((v.hint_values >> Value) & 1u)) {
+  if (Hints && v.sel_hints != nullptr && ((v.hint_values >> Value) & 1u)) {
     // the bracket's hint: sample position, Values before its first block,
     // and the counts before the next kHintBlocks blocks
     const uint4* h = v.sel_hints + (Value ? j : v.ones / kSampleRate + 1 + j) * 4;
@@ -678,7 +678,7 @@ __device__ __forceinline__ uint64_t tree_prev_char_start(
 
 // select: selectKernel early-out + bottom-up ascent (wavelet_tree.cuh:
 // 532-535, 2007-2079).  SelLines as for level_select.
-template <bool SelLines = true>
+template <bool SelLines = true, bool Hints = true>
 __device__ __forceinline__ uint64_t tree_select(const TreeView& t,
                                                 const LevelView* lv,
                                                 const uint64_t* counts,
@@ -711,12 +711,12 @@ __device__ __forceinline__ uint64_t tree_select(const TreeView& t,
       start = cs == 0 ? 0
                       : counts[cs] - ranks[rs + tree_node_pos(
                                                     t, cs, static_cast<uint32_t>(l))];
-      result = level_select<1, false, SelLines>(v, start + result) + 1;
+      result = level_select<1, false, SelLines, Hints>(v, start + result) + 1;
     } else {
       start = cs == 0
                   ? 0
                   : ranks[rs + tree_node_pos(t, cs, static_cast<uint32_t>(l))];
-      result = level_select<0, false, SelLines>(v, start + result) + 1;
+      result = level_select<0, false, SelLines, Hints>(v, start + result) + 1;
     }
     if (static_cast<uint32_t>(l) == clen - 1 && result > v.size) {
       result = t.n + 1;
