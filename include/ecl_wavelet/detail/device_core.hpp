@@ -387,6 +387,8 @@ __device__ __forceinline__ uint64_t select_in_block(const LevelView& v, uint64_t
       (uint64_t)q1.x | ((uint64_t)q1.y << 32), (uint64_t)q1.z | ((uint64_t)q1.w << 32),
       (uint64_t)q2.x | ((uint64_t)q2.y << 32), (uint64_t)q2.z | ((uint64_t)q2.w << 32),
       (uint64_t)q3.x | ((uint64_t)q3.y << 32), (uint64_t)q3.z | ((uint64_t)q3.w << 32)};
+  // (selecting once after the loop, as select_from_line does, made the hint
+  // and directory searches slower: 130 instead of 116 VGPRs)
   uint64_t pos = blk2 << 9, res = v.size;
   bool found = false;
 #pragma unroll
@@ -427,24 +429,26 @@ __device__ __forceinline__ uint64_t select_from_line(const LevelView& v, uint64_
   // counted from s (the (jj rate)-th Value itself, or bit 0 for jj = 0)
   uint64_t r = jj ? i - jj * rate + 1 : i;
   const uint32_t off = static_cast<uint32_t>(s & 63);
-  uint64_t pos = s & ~63ull, res = v.size;
-  bool found = false;
+  // the word first, then one in-word select: a select under each of the
+  // unrolled word tests runs for every word that any lane of the wave stops at
+  uint32_t kw = 15;
+  uint64_t xw = 0;
 #pragma unroll
   for (uint32_t k = 0; k < 15; ++k) {
     uint64_t x = Value ? w[k] : ~w[k];
     if (k == 0) x &= ~0ull << off;
     const uint32_t c = static_cast<uint32_t>(__popcll(x));
-    if (!found) {
+    if (kw == 15) {
       if (r <= c) {
-        res = pos + select_in_word(x, static_cast<uint32_t>(r));
-        found = true;
+        kw = k;
+        xw = x;
       } else {
         r -= c;
-        pos += 64;
       }
     }
   }
-  return res;
+  if (kw == 15) return v.size;
+  return (s & ~63ull) + 64ull * kw + select_in_word(xw, static_cast<uint32_t>(r));
 }
 
 // SelLines: consult the select lines when the array has them (the query
