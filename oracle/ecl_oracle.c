@@ -618,3 +618,151 @@ void orc_histogram(const void* text, int esize, uint64_t n, uint64_t sigma,
     if (c < sigma) hist[c]++;
   }
 }
+
+/* ------------------------------------------------------------------------
+ * Sweep rank: the answers of many rank queries from one pass over the text.
+ * out[j] = count of sym[j] in [0, pos[j]) -- the direct scan's rank
+ * (test_wavelet_tree.cu:617-619) -- for positions given in ascending order
+ * (pos[j] <= n).  The text is cut into one range per thread; each range's
+ * histogram is counted, the histograms are prefixed over the ranges, and
+ * each thread then walks its range with a running histogram that starts at
+ * that prefix, answering its queries as it reaches their positions.  This
+ * is how the by-construction queries of test_benchmark_utils.cuh:417-528
+ * get their answers (a running histogram while the text is generated),
+ * here for arbitrary (symbol, position) pairs.  O(n + m) time.
+ * ---------------------------------------------------------------------- */
+static uint64_t lower_pos(const uint64_t* pos, uint64_t m, uint64_t x) {
+  uint64_t lo = 0, hi = m;
+  while (lo < hi) {
+    uint64_t mid = (lo + hi) / 2;
+    if (pos[mid] < x) lo = mid + 1; else hi = mid;
+  }
+  return lo;
+}
+
+int orc_sweep_rank(const void* text, int esize, uint64_t n, uint64_t sigma,
+                   const uint64_t* pos, const uint64_t* sym, uint64_t m,
+                   uint64_t* out) {
+  int T = orc_max_threads();
+  if ((uint64_t)T > n / 4096 + 1) T = (int)(n / 4096 + 1);
+  uint64_t* hist = (uint64_t*)calloc((size_t)(T + 1) * sigma, sizeof(uint64_t));
+  if (!hist) return -1;
+  /* range t covers [t*n/T, (t+1)*n/T); row t+1 of hist first gets range t's
+   * histogram, then the prefix turns row t into the counts before range t */
+#pragma omp parallel for num_threads(T) schedule(static, 1)
+  for (int t = 0; t < T; ++t) {
+    uint64_t lo = (uint64_t)t * n / T, hi = (uint64_t)(t + 1) * n / T;
+    uint64_t* h = hist + (size_t)(t + 1) * sigma;
+    for (uint64_t i = lo; i < hi; ++i) {
+      uint64_t c = get_elem(text, esize, i);
+      if (c < sigma) h[c]++;
+    }
+  }
+  for (int t = 1; t <= T; ++t)
+    for (uint64_t c = 0; c < sigma; ++c)
+      hist[(size_t)t * sigma + c] += hist[(size_t)(t - 1) * sigma + c];
+  /* queries at position n take the totals (row T) */
+  uint64_t first_n = lower_pos(pos, m, n);
+  for (uint64_t j = first_n; j < m; ++j)
+    out[j] = sym[j] < sigma ? hist[(size_t)T * sigma + sym[j]] : 0;
+#pragma omp parallel for num_threads(T) schedule(static, 1)
+  for (int t = 0; t < T; ++t) {
+    uint64_t lo = (uint64_t)t * n / T, hi = (uint64_t)(t + 1) * n / T;
+    uint64_t* h = hist + (size_t)t * sigma; /* counts before lo */
+    uint64_t q = lower_pos(pos, m, lo), qe = lower_pos(pos, m, hi);
+    uint64_t i = lo;
+    for (; q < qe; ++q) {
+      for (; i < pos[q]; ++i) {
+        uint64_t c = get_elem(text, esize, i);
+        if (c < sigma) h[c]++;
+      }
+      out[q] = sym[q] < sigma ? h[sym[q]] : 0;
+    }
+  }
+  free(hist);
+  return T;
+}
+
+/* ------------------------------------------------------------------------
+ * Blocked oracle (CPU baseline, variant B): the direct scans' answers
+ * (test_wavelet_tree.cu:617-678, test_utils.cu:81-91) from per-symbol
+ * prefix counts every 2^16 symbols plus a scan of at most one block.
+ * table has (ceil(n / 2^16) + 1) rows of sigma u64: row b holds the counts
+ * of every symbol in [0, min(b * 2^16, n)), so the last row holds the
+ * totals.  Same answers as variant A, including n for a select past the
+ * last occurrence.
+ * ---------------------------------------------------------------------- */
+#define ORC_BLOCK_LOG2 16
+uint64_t orc_blocked_rows(uint64_t n) {
+  return ((n + ((1ull << ORC_BLOCK_LOG2) - 1)) >> ORC_BLOCK_LOG2) + 1;
+}
+
+void orc_blocked_build(const void* text, int esize, uint64_t n, uint64_t sigma,
+                       uint64_t* table) {
+  const uint64_t rows = orc_blocked_rows(n);
+  memset(table, 0, sigma * sizeof(uint64_t));
+  /* row b + 1 first holds block b's own histogram */
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int64_t b = 0; b < (int64_t)(rows - 1); ++b) {
+    uint64_t* h = table + (size_t)(b + 1) * sigma;
+    memset(h, 0, sigma * sizeof(uint64_t));
+    uint64_t lo = (uint64_t)b << ORC_BLOCK_LOG2;
+    uint64_t hi = lo + (1ull << ORC_BLOCK_LOG2);
+    if (hi > n) hi = n;
+    for (uint64_t i = lo; i < hi; ++i) {
+      uint64_t c = get_elem(text, esize, i);
+      if (c < sigma) h[c]++;
+    }
+  }
+  /* prefix over the rows, one symbol column per iteration */
+#pragma omp parallel for schedule(static)
+  for (int64_t c = 0; c < (int64_t)sigma; ++c)
+    for (uint64_t b = 1; b < rows; ++b)
+      table[b * sigma + c] += table[(b - 1) * sigma + c];
+}
+
+uint64_t orc_blocked_rank(const void* text, int esize, uint64_t n, uint64_t sigma,
+                          const uint64_t* table, uint64_t c, uint64_t i) {
+  if (c >= sigma) return 0;
+  if (i > n) i = n;
+  const uint64_t b = i >> ORC_BLOCK_LOG2;
+  uint64_t r = table[b * sigma + c];
+  for (uint64_t j = b << ORC_BLOCK_LOG2; j < i; ++j) r += get_elem(text, esize, j) == c;
+  return r;
+}
+
+uint64_t orc_blocked_select(const void* text, int esize, uint64_t n, uint64_t sigma,
+                            const uint64_t* table, uint64_t c, uint64_t k) {
+  const uint64_t rows = orc_blocked_rows(n);
+  if (k == 0 || c >= sigma || k > table[(rows - 1) * sigma + c]) return n;
+  /* largest block b with fewer than k occurrences before it */
+  uint64_t lo = 0, hi = rows - 2;
+  while (lo < hi) {
+    uint64_t mid = (lo + hi + 1) / 2;
+    if (table[mid * sigma + c] < k) lo = mid; else hi = mid - 1;
+  }
+  uint64_t seen = table[lo * sigma + c];
+  uint64_t end = (lo + 1) << ORC_BLOCK_LOG2;
+  if (end > n) end = n;
+  for (uint64_t j = lo << ORC_BLOCK_LOG2; j < end; ++j)
+    if (get_elem(text, esize, j) == c && ++seen == k) return j;
+  return n; /* unreachable for a consistent table */
+}
+
+int orc_blocked_rank_batch(const void* text, int esize, uint64_t n, uint64_t sigma,
+                           const uint64_t* table, const uint64_t* idx,
+                           const uint64_t* sym, uint64_t m, uint64_t* out) {
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t q = 0; q < (int64_t)m; ++q)
+    out[q] = orc_blocked_rank(text, esize, n, sigma, table, sym[q], idx[q]);
+  return orc_max_threads();
+}
+
+int orc_blocked_select_batch(const void* text, int esize, uint64_t n, uint64_t sigma,
+                             const uint64_t* table, const uint64_t* k,
+                             const uint64_t* sym, uint64_t m, uint64_t* out) {
+#pragma omp parallel for schedule(dynamic, 64)
+  for (int64_t q = 0; q < (int64_t)m; ++q)
+    out[q] = orc_blocked_select(text, esize, n, sigma, table, sym[q], k[q]);
+  return orc_max_threads();
+}

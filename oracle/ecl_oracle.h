@@ -118,6 +118,30 @@ int orc_max_threads(void);
 void orc_histogram(const void* text, int esize, uint64_t n, uint64_t sigma,
                    uint64_t* hist);
 
+/* Sweep rank (by-construction answers, test_benchmark_utils.cuh:417-528):
+ * out[j] = count of sym[j] in [0, pos[j]) for positions in ascending order,
+ * pos[j] <= n; one pass over the text.  Returns threads used (-1: no
+ * memory). */
+int orc_sweep_rank(const void* text, int esize, uint64_t n, uint64_t sigma,
+                   const uint64_t* pos, const uint64_t* sym, uint64_t m,
+                   uint64_t* out);
+/* Blocked oracle (CPU baseline, variant B): per-symbol counts before every
+ * 2^16-symbol block, orc_blocked_rows(n) rows of sigma u64, plus a scan of
+ * at most one block per query.  Same answers as the direct scans. */
+uint64_t orc_blocked_rows(uint64_t n);
+void orc_blocked_build(const void* text, int esize, uint64_t n, uint64_t sigma,
+                       uint64_t* table);
+uint64_t orc_blocked_rank(const void* text, int esize, uint64_t n, uint64_t sigma,
+                          const uint64_t* table, uint64_t c, uint64_t i);
+uint64_t orc_blocked_select(const void* text, int esize, uint64_t n, uint64_t sigma,
+                            const uint64_t* table, uint64_t c, uint64_t k);
+int orc_blocked_rank_batch(const void* text, int esize, uint64_t n, uint64_t sigma,
+                           const uint64_t* table, const uint64_t* idx,
+                           const uint64_t* sym, uint64_t m, uint64_t* out);
+int orc_blocked_select_batch(const void* text, int esize, uint64_t n, uint64_t sigma,
+                             const uint64_t* table, const uint64_t* k,
+                             const uint64_t* sym, uint64_t m, uint64_t* out);
+
 #ifdef __cplusplus
 }
 #endif

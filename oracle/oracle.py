@@ -64,6 +64,13 @@ def lib():
             "orc_scan_bits_batch": (C.c_int, [vp, u64, C.c_int, vp, u64, vp]),
             "orc_max_threads": (C.c_int, []),
             "orc_histogram": (None, [vp, C.c_int, u64, u64, vp]),
+            "orc_sweep_rank": (C.c_int, [vp, C.c_int, u64, u64, vp, vp, u64, vp]),
+            "orc_blocked_rows": (u64, [u64]),
+            "orc_blocked_build": (None, [vp, C.c_int, u64, u64, vp]),
+            "orc_blocked_rank": (u64, [vp, C.c_int, u64, u64, vp, u64, u64]),
+            "orc_blocked_select": (u64, [vp, C.c_int, u64, u64, vp, u64, u64]),
+            "orc_blocked_rank_batch": (C.c_int, [vp, C.c_int, u64, u64, vp, vp, vp, u64, vp]),
+            "orc_blocked_select_batch": (C.c_int, [vp, C.c_int, u64, u64, vp, vp, vp, u64, vp]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -303,3 +310,53 @@ def histogram(text, sigma):
 
 def max_threads():
     return lib().orc_max_threads()
+
+
+def sweep_rank(text, pos, sym, sigma=None):
+    """count of sym[j] in text[0:pos[j]] for every j (pos[j] <= len(text)),
+    from one pass over the text (orc_sweep_rank); any order of queries."""
+    pos = np.ascontiguousarray(pos, dtype=np.uint64)
+    sym = np.ascontiguousarray(sym, dtype=np.uint64)
+    if sigma is None:
+        sigma = 1 << (8 * text.itemsize) if text.itemsize <= 2 else int(text.max()) + 1
+    order = np.argsort(pos, kind="stable")
+    ps, ss = pos[order], sym[order]
+    out_s = np.empty(len(pos), dtype=np.uint64)
+    r = lib().orc_sweep_rank(_p(text), text.itemsize, len(text), sigma, _p(ps), _p(ss),
+                             len(ps), _p(out_s))
+    if r < 0:
+        raise MemoryError("orc_sweep_rank")
+    out = np.empty_like(out_s)
+    out[order] = out_s
+    return out
+
+
+class BlockedOracle:
+    """Variant B of the CPU baseline: per-symbol counts before every 2^16
+    symbols plus a scan of one block (orc_blocked_*)."""
+
+    def __init__(self, text, sigma):
+        self.text = np.ascontiguousarray(text)
+        self.sigma = int(sigma)
+        rows = lib().orc_blocked_rows(len(self.text))
+        self.table = np.empty(rows * self.sigma, dtype=np.uint64)
+        lib().orc_blocked_build(_p(self.text), self.text.itemsize, len(self.text),
+                                self.sigma, _p(self.table))
+
+    def rank_batch(self, idx, sym):
+        idx = np.ascontiguousarray(idx, dtype=np.uint64)
+        sym = np.ascontiguousarray(sym, dtype=np.uint64)
+        out = np.empty(len(idx), dtype=np.uint64)
+        lib().orc_blocked_rank_batch(_p(self.text), self.text.itemsize, len(self.text),
+                                     self.sigma, _p(self.table), _p(idx), _p(sym), len(idx),
+                                     _p(out))
+        return out
+
+    def select_batch(self, k, sym):
+        k = np.ascontiguousarray(k, dtype=np.uint64)
+        sym = np.ascontiguousarray(sym, dtype=np.uint64)
+        out = np.empty(len(k), dtype=np.uint64)
+        lib().orc_blocked_select_batch(_p(self.text), self.text.itemsize, len(self.text),
+                                       self.sigma, _p(self.table), _p(k), _p(sym), len(k),
+                                       _p(out))
+        return out

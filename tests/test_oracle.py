@@ -346,3 +346,96 @@ def test_rs_samples_kat():
             _, _, s1, s0, ones = O.rs_index(w, m)
             s = s1 if val else s0
             assert list(s) == [j * 4096 - 1 for j in range(1, m // 4096 + 1)]
+
+
+# --- sweep rank and the blocked oracle (variant B) ---------------------------
+def _numpy_rank(text, pos, sym):
+    """count of sym in text[:pos] from per-symbol prefix sums (numpy)."""
+    out = np.empty(len(pos), dtype=np.uint64)
+    for c in np.unique(sym):
+        pref = np.concatenate([[0], np.cumsum(text == c, dtype=np.int64)])
+        sel = sym == c
+        out[sel] = pref[pos[sel].astype(np.int64)]
+    return out
+
+
+def _numpy_select(text, k, sym):
+    n = len(text)
+    out = np.empty(len(k), dtype=np.uint64)
+    for c in np.unique(sym):
+        occ = np.flatnonzero(text == c)
+        sel = sym == c
+        kk = k[sel].astype(np.int64)
+        ok = (kk >= 1) & (kk <= len(occ))
+        r = np.full(len(kk), n, dtype=np.uint64)
+        r[ok] = occ[kk[ok] - 1]
+        out[sel] = r
+    return out
+
+
+def _edge_positions(n, rng, extra):
+    b = np.arange(0, n + 1, 65536, dtype=np.int64)
+    e = np.concatenate([[0, 1, n - 1, n], b, b - 1, b + 1,
+                        rng.integers(0, n + 1, extra)])
+    return np.unique(np.clip(e, 0, n)).astype(np.uint64)
+
+
+@pytest.mark.parametrize("case", [("u8", 4, 300_001), ("u8", 256, 200_000), ("u8", 20, 131_072),
+                                  ("u16", 1000, 150_000), ("u8", 7, 5000), ("u8", 3, 1)])
+def test_sweep_rank_vs_prefix_sums(case):
+    """orc_sweep_rank equals numpy prefix sums for every (symbol, position),
+    including position 0, n, block and range boundaries, repeated positions
+    and symbols that do not occur."""
+    dt, sigma, n = case
+    rng = np.random.default_rng(sigma * 7 + n)
+    text = rng.integers(0, sigma, n).astype(np.uint8 if dt == "u8" else np.uint16)
+    pos = _edge_positions(n, rng, 3000)
+    pos = np.concatenate([pos, pos[:50]])  # repeated positions
+    sym = rng.integers(0, sigma + 2, len(pos)).astype(np.uint64)  # includes absent symbols
+    sym = np.minimum(sym, sigma - 1) if sigma > 250 else sym
+    got = O.sweep_rank(text, pos, sym, sigma=sigma + 2)
+    assert np.array_equal(got, _numpy_rank(text, pos, sym))
+
+
+@pytest.mark.parametrize("case", [("u8", 4, 300_001), ("u8", 256, 400_000), ("u8", 20, 131_072),
+                                  ("u16", 1000, 200_000), ("u8", 5, 65_536), ("u8", 2, 3)])
+def test_blocked_oracle_vs_numpy(case):
+    """Variant B (per-symbol block counts + one-block scan) answers rank and
+    select exactly as numpy prefix sums / occurrence lists, at block
+    boundaries, position n, k = count + 1 (gives n) and absent symbols."""
+    dt, sigma, n = case
+    rng = np.random.default_rng(sigma * 11 + n)
+    text = rng.integers(0, sigma, n).astype(np.uint8 if dt == "u8" else np.uint16)
+    B = O.BlockedOracle(text, sigma + 1)  # symbol sigma never occurs
+    pos = _edge_positions(n, rng, 2000)
+    sym = rng.integers(0, sigma + 1, len(pos)).astype(np.uint64)
+    assert np.array_equal(B.rank_batch(pos, sym), _numpy_rank(text, pos, sym))
+    hist = np.bincount(text, minlength=sigma + 1)
+    sym = rng.integers(0, sigma + 1, 3000).astype(np.uint64)
+    k = np.array([rng.integers(0, hist[c] + 2) for c in sym], dtype=np.uint64)
+    # every block's first and last occurrence of symbol 0
+    occ0 = np.flatnonzero(text == 0)
+    if len(occ0):
+        ks = np.unique(np.searchsorted(occ0, np.arange(0, n + 65536, 65536)))
+        ks = ks[(ks >= 0) & (ks < len(occ0))] + 1
+        k = np.concatenate([k, ks.astype(np.uint64), ks.astype(np.uint64) + 1])
+        sym = np.concatenate([sym, np.zeros(2 * len(ks), dtype=np.uint64)])
+    assert np.array_equal(B.select_batch(k, sym), _numpy_select(text, k, sym))
+    # the direct scans (variant A) agree on a sample
+    assert np.array_equal(B.select_batch(k[:200], sym[:200]),
+                          O.scan_select_batch(text, k[:200], sym[:200]))
+
+
+def test_by_construction_round_trip():
+    """The by-construction pairs of test_benchmark_utils.cuh:417-528: for a
+    position p with c = text[p], rank(c, p) is the number of earlier c, and
+    select(c, rank(c, p) + 1) is p."""
+    rng = np.random.default_rng(5)
+    n = 250_000
+    text = O.gen_cdf(3, n, O.zipf_thresholds(64, 1.1), np.uint8)
+    p = rng.integers(0, n, 4000).astype(np.uint64)
+    c = text[p].astype(np.uint64)
+    r = O.sweep_rank(text, p, c, sigma=64)
+    B = O.BlockedOracle(text, 64)
+    assert np.array_equal(B.select_batch(r + 1, c), p)
+    assert np.array_equal(r, B.rank_batch(p, c))
