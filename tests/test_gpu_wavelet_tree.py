@@ -437,24 +437,54 @@ def test_save_load_round_trip(gpu, tmp_path, case):
 def test_load_rejects_damaged_files(gpu, tmp_path):
     from ecl_wavelet_amd import WaveletTree
     from ecl_wavelet_amd._lib import EclError
-    text = np.arange(100_000, dtype=np.uint8) % 7
+    text = (np.arange(100_003) % 7).astype(np.uint8)
     wt = WaveletTree(text, np.arange(7, dtype=np.uint8))
     path = tmp_path / "t.ecl"
     wt.save(path)
     data = path.read_bytes()
+    # layout: 40-byte header, 7 alphabet bytes, 7 u64 counts, then per level
+    # a u64 size and its words.  Level 1 holds node 0..3 at [0, C(4)) and
+    # node 4..6 at [C(4), n).
+    n = len(text)
+    lvl0 = 40 + 7 + 56 + 8
+    lvl1 = lvl0 + (n + 31) // 32 * 4 + 8
+    bits1 = np.unpackbits(np.frombuffer(data[lvl1:lvl1 + (n + 31) // 32 * 4], np.uint8),
+                          bitorder="little")[:n]
+    c4 = int(np.count_nonzero(text < 4))
+    one_left = int(np.flatnonzero(bits1[:c4] == 1)[0])
+    zero_right = c4 + int(np.flatnonzero(bits1[c4:] == 0)[0])
+
+    def flip(blob, base, bit):
+        b = bytearray(blob)
+        b[base + bit // 8] ^= 1 << (bit % 8)
+        return bytes(b)
+
     cases = {
         "truncated": data[:-5],
         "trailing": data + b"x",
         "magic": b"X" + data[1:],
         "counts": data[:40 + 7] + bytes([data[47] ^ 1]) + data[48:],  # first symbol count
+        # one level-0 bit flipped: the level's ones no longer match the counts
+        "bit": flip(data, lvl0, 12345),
+        # a one moved from node 0..3 into node 4..6 of level 1: the level's
+        # total is unchanged, but both nodes' ones are wrong, and a descent
+        # into the right node could index past the last level's array
+        "moved_bit": flip(flip(data, lvl1, one_left), lvl1, zero_right),
     }
     for name, blob in cases.items():
         p = tmp_path / f"{name}.ecl"
         p.write_bytes(blob)
-        with pytest.raises(EclError):
+        with pytest.raises(EclError, match="level bits do not match" if "bit" in name else None):
             WaveletTree.load(p)
     with pytest.raises(EclError):
         WaveletTree.load(tmp_path / "missing.ecl")
+    # set bits past n in the last word of level 0 are not part of the array:
+    # the file loads and answers as the intact one
+    last = lvl0 + ((n + 31) // 32 - 1) * 4
+    p = tmp_path / "pad.ecl"
+    assert n % 32
+    p.write_bytes(flip(data, last, 31))
+    _check_queries(WaveletTree.load(p), text, None, nq=500, seed=2)
     lt = WaveletTree.load(path)  # the intact file still loads
     _check_queries(lt, text, None, nq=500, seed=1)
 
