@@ -38,28 +38,151 @@ void hip_check(hipError_t e, const char* what) {
     throw Error(ECL_ERROR_HIP, std::string(what) + ": " + hipGetErrorString(e));
 }
 
+// ---- per-device block cache -----------------------------------------------
+// hipMalloc and hipFree of multi-GB blocks take from milliseconds to most of
+// a second while the runtime maps and unmaps the memory, and the library
+// repeats the same sizes (a tree, its directory and its query and select
+// lines when trees of one shape are rebuilt; a forest's arena; a server's
+// trees).  Freed blocks of at least kPoolMinBytes are therefore kept per
+// device and handed out again for a request of the same size class (the
+// smallest cached block of at least the request and at most 1/8 plus one
+// grain more).  A release synchronizes the block's device first, as hipFree
+// does, so no kernel still uses a block when it is handed out again.  The
+// cache keeps at most kPoolCapFraction of the device's memory (oldest
+// blocks are freed first); when an allocation fails, the whole cache is
+// freed and the allocation tried once more.
+namespace {
+constexpr size_t kPoolGrain = size_t(2) << 20;
+constexpr int kPoolDevices = 64;
+constexpr double kPoolCapFraction = 0.4;
+struct PoolBlock { void* ptr; size_t cap; uint64_t seq; };
+struct BlockPool {
+  std::mutex mu;
+  std::vector<PoolBlock> blocks;
+  size_t cached = 0;
+  size_t limit = 0;  // 0 until the device's memory size is known
+  uint64_t seq = 0;
+};
+// never destroyed: freeing device memory during static destruction may run
+// after the HIP runtime has shut down
+BlockPool* const g_pool = new BlockPool[kPoolDevices];
+
+struct DeviceScope {  // makes `dev` current for a scope
+  int prev = -1;
+  explicit DeviceScope(int dev) {
+    if (hipGetDevice(&prev) != hipSuccess) prev = -1;
+    if (prev != dev) (void)hipSetDevice(dev);
+  }
+  ~DeviceScope() {
+    int cur = -1;
+    if (prev >= 0 && hipGetDevice(&cur) == hipSuccess && cur != prev) (void)hipSetDevice(prev);
+  }
+};
+
+void pool_free_all(BlockPool& P) {  // P.mu held, device current
+  for (const PoolBlock& b : P.blocks) (void)hipFree(b.ptr);
+  P.blocks.clear();
+  P.cached = 0;
+}
+}  // namespace
+
+size_t pooled_bytes(int device) {
+  if (device < 0 || device >= kPoolDevices) return 0;
+  std::lock_guard<std::mutex> lk(g_pool[device].mu);
+  return g_pool[device].cached;
+}
+void release_block_pool(int device) {
+  if (device < 0 || device >= kPoolDevices) return;
+  DeviceScope ds(device);
+  std::lock_guard<std::mutex> lk(g_pool[device].mu);
+  pool_free_all(g_pool[device]);
+}
+
 DevBuf::DevBuf(size_t b) : bytes(b) {
   if (b == 0) return;
+  ECL_HIP(hipGetDevice(&dev));
+  if (b >= kPoolMinBytes && dev >= 0 && dev < kPoolDevices) {
+    const size_t want = (b + kPoolGrain - 1) / kPoolGrain * kPoolGrain;
+    BlockPool& P = g_pool[dev];
+    std::lock_guard<std::mutex> lk(P.mu);
+    size_t best = SIZE_MAX;
+    for (size_t i = 0; i < P.blocks.size(); ++i) {
+      const size_t c = P.blocks[i].cap;
+      if (c >= want && c <= want + want / 8 + kPoolGrain &&
+          (best == SIZE_MAX || c < P.blocks[best].cap))
+        best = i;
+    }
+    if (best != SIZE_MAX) {
+      ptr = P.blocks[best].ptr;
+      cap = P.blocks[best].cap;
+      P.cached -= cap;
+      P.blocks.erase(P.blocks.begin() + static_cast<std::ptrdiff_t>(best));
+      return;
+    }
+    hipError_t e = hipMalloc(&ptr, want);
+    if (e != hipSuccess && !P.blocks.empty()) {
+      (void)hipGetLastError();
+      pool_free_all(P);
+      e = hipMalloc(&ptr, want);
+    }
+    if (e != hipSuccess) {
+      (void)hipGetLastError();
+      ptr = nullptr;
+      throw Error(ECL_ERROR_HIP, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    }
+    cap = want;
+    return;
+  }
   hipError_t e = hipMalloc(&ptr, b);
   if (e != hipSuccess) {
     (void)hipGetLastError();
     ptr = nullptr;
     throw Error(ECL_ERROR_HIP, std::string("hipMalloc failed: ") + hipGetErrorString(e));
   }
+  cap = b;
 }
 DevBuf::~DevBuf() { reset(); }
 DevBuf& DevBuf::operator=(DevBuf&& o) noexcept {
   if (this != &o) {
     reset();
-    ptr = o.ptr; bytes = o.bytes;
-    o.ptr = nullptr; o.bytes = 0;
+    ptr = o.ptr; bytes = o.bytes; cap = o.cap; dev = o.dev;
+    o.ptr = nullptr; o.bytes = 0; o.cap = 0; o.dev = -1;
   }
   return *this;
 }
 void DevBuf::reset() {
-  if (ptr) (void)hipFree(ptr);
+  if (ptr) {
+    if (cap >= kPoolMinBytes && dev >= 0 && dev < kPoolDevices) {
+      DeviceScope ds(dev);
+      (void)hipDeviceSynchronize();  // hipFree's implicit wait
+      (void)hipGetLastError();
+      BlockPool& P = g_pool[dev];
+      std::lock_guard<std::mutex> lk(P.mu);
+      if (P.limit == 0) {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) == hipSuccess)
+          P.limit = static_cast<size_t>(static_cast<double>(tot) * kPoolCapFraction);
+        else
+          (void)hipGetLastError();
+      }
+      P.blocks.push_back({ptr, cap, P.seq++});
+      P.cached += cap;
+      while (P.cached > P.limit && !P.blocks.empty()) {
+        size_t old = 0;
+        for (size_t i = 1; i < P.blocks.size(); ++i)
+          if (P.blocks[i].seq < P.blocks[old].seq) old = i;
+        (void)hipFree(P.blocks[old].ptr);
+        P.cached -= P.blocks[old].cap;
+        P.blocks.erase(P.blocks.begin() + static_cast<std::ptrdiff_t>(old));
+      }
+    } else {
+      (void)hipFree(ptr);
+    }
+  }
   ptr = nullptr;
   bytes = 0;
+  cap = 0;
+  dev = -1;
 }
 
 // ---- uploads ----------------------------------------------------------------
@@ -316,10 +439,14 @@ void set_device(int device) {
   ECL_HIP(hipSetDevice(device));
 }
 
+// free device memory as the library sees it: what the runtime reports plus
+// the blocks the library's cache holds for reuse
 size_t free_device_memory() {
   size_t fr = 0, tot = 0;
   ECL_HIP(hipMemGetInfo(&fr, &tot));
-  return fr;
+  int dev = -1;
+  ECL_HIP(hipGetDevice(&dev));
+  return fr + pooled_bytes(dev);
 }
 
 }  // namespace lib
@@ -1478,16 +1605,49 @@ static ecl_wavelet_tree_t* load_tree(const char* path, int device) {
     upload_symbol_maps(wt, up);
     std::unique_ptr<ecl_bit_array_t> ba(make_bit_array(level_size, -1, device, &up));
     up.flush();
+    // Level bits are checked against the counts while they stream in: the
+    // internal nodes of level l tile its array in symbol order (node lo..hi
+    // at positions [C(lo), C(hi + 1))), and a node's ones are the codes of
+    // its right child, C(hi + 1) - C(mid).  Queries rely on that: a descent
+    // stays inside its node only if every node holds exactly these ones.
+    std::vector<uint64_t> prefix(h.sigma + 1, 0);
+    for (uint64_t c = 0; c < h.sigma; ++c) prefix[c + 1] = prefix[c] + sym_count[c];
     std::vector<uint32_t> buf(kFileChunk / 4);
     for (uint32_t l = 0; l < wt->L; ++l) {
       uint64_t m = 0;
       read_all(file.f, &m, 8);
       require(m == level_size[l], "the file's level sizes do not match its symbol counts");
+      const std::vector<Node>& nodes = shape.nodes[l];
+      std::vector<uint64_t> bound;  // node starts, then m
+      for (const Node& nd : nodes) bound.push_back(prefix[nd.lo]);
+      bound.push_back(m);
+      require(bound.front() == 0, "the file's level sizes do not match its symbol counts");
+      std::vector<uint64_t> ones_at(bound.size(), 0);  // ones before each bound
+      size_t bi = 0;
+      uint64_t ones = 0;
       const uint64_t nw = (m + 31) / 32;
       for (uint64_t w = 0; w < nw; w += buf.size()) {
         const uint64_t c = std::min<uint64_t>(buf.size(), nw - w);
         read_all(file.f, buf.data(), c * 4);
+        // bits past m in the last word are not part of the array
+        if (w + c == nw && (m & 31)) buf[c - 1] &= (1u << (m & 31)) - 1u;
+        for (uint64_t k = 0; k < c; ++k) {
+          const uint64_t word_bit = (w + k) * 32;
+          while (bi < bound.size() && bound[bi] < word_bit + 32) {
+            const uint32_t b = static_cast<uint32_t>(bound[bi] - word_bit);
+            ones_at[bi++] = ones + __builtin_popcount(b ? buf[k] & ((1u << b) - 1u) : 0u);
+          }
+          ones += __builtin_popcount(buf[k]);
+        }
         h2d(ba->words() + ba->offsets[l] + w, buf.data(), c);
+      }
+      while (bi < bound.size()) ones_at[bi++] = ones;  // bound == m at a word edge
+      for (size_t j = 0; j < nodes.size(); ++j) {
+        const Node& nd = nodes[j];
+        const uint64_t mid = nd.lo + detail::prev_pow2(nd.hi - nd.lo + 1);
+        require(bound[j + 1] == prefix[nd.hi + 1] &&
+                    ones_at[j + 1] - ones_at[j] == prefix[nd.hi + 1] - prefix[mid],
+                "the file's level bits do not match its symbol counts");
       }
     }
     if (std::fgetc(file.f) != EOF)
@@ -1556,7 +1716,13 @@ ecl_status ecl_device_alloc(int device, size_t bytes, void** ptr) {
     require(ptr != nullptr, "ptr is NULL");
     set_device(device);
     *ptr = nullptr;
-    if (bytes) ECL_HIP(hipMalloc(ptr, bytes));
+    if (!bytes) return;
+    if (hipMalloc(ptr, bytes) != hipSuccess) {
+      // the library's cached blocks go back to the runtime, then once more
+      (void)hipGetLastError();
+      release_block_pool(device);
+      ECL_HIP(hipMalloc(ptr, bytes));
+    }
   });
 }
 ecl_status ecl_device_free(int device, void* ptr) {
@@ -1599,12 +1765,15 @@ ecl_status ecl_release_cached_memory(int device) {
   return guard([&] {
     set_device(device);
     release_scratch_cache(device);
+    release_block_pool(device);
   });
 }
 ecl_status ecl_device_mem_info(int device, size_t* fr, size_t* tot) {
   return guard([&] {
+    require(fr != nullptr && tot != nullptr, "NULL output");
     set_device(device);
     ECL_HIP(hipMemGetInfo(fr, tot));
+    *fr += pooled_bytes(device);
   });
 }
 ecl_status ecl_device_name(int device, char* buf, size_t len) {

@@ -29,19 +29,32 @@ void hip_check(hipError_t e, const char* what);
 #define ECL_HIP(call) ::ecl::lib::hip_check((call), #call)
 
 // Device allocation that records itself for freeing (RAII).
+// Device memory owned by the library.  Blocks of kPoolMinBytes or more come
+// from and go back to a per-device block cache (see capi.hip): cap is the
+// block's size, dev the device it lives on.
 struct DevBuf {
   void* ptr = nullptr;
   size_t bytes = 0;
+  size_t cap = 0;
+  int dev = -1;
   DevBuf() = default;
   explicit DevBuf(size_t b);
   ~DevBuf();
   DevBuf(const DevBuf&) = delete;
   DevBuf& operator=(const DevBuf&) = delete;
-  DevBuf(DevBuf&& o) noexcept : ptr(o.ptr), bytes(o.bytes) { o.ptr = nullptr; o.bytes = 0; }
+  DevBuf(DevBuf&& o) noexcept : ptr(o.ptr), bytes(o.bytes), cap(o.cap), dev(o.dev) {
+    o.ptr = nullptr; o.bytes = 0; o.cap = 0; o.dev = -1;
+  }
   DevBuf& operator=(DevBuf&& o) noexcept;
   void reset();
   template <typename T> T* as() const { return static_cast<T*>(ptr); }
 };
+constexpr size_t kPoolMinBytes = size_t(1) << 20;
+// bytes of freed blocks the device's block cache holds (available to the
+// library's next allocations; freed by ecl_release_cached_memory)
+size_t pooled_bytes(int device);
+// frees every cached block of the device
+void release_block_pool(int device);
 
 // ---- kernel profiling (events around selected launches) -------------------
 enum KernelId : int {

@@ -109,11 +109,16 @@ ecl_status ecl_device_free(int device, void* ptr);
 ecl_status ecl_memcpy_h2d(int device, void* dst, const void* src, size_t bytes);
 ecl_status ecl_memcpy_d2h(int device, void* dst, const void* src, size_t bytes);
 ecl_status ecl_device_synchronize(int device);
+/* free bytes: the runtime's free memory plus the library's cached blocks
+ * (below), which the library reuses and frees when an allocation needs them */
 ecl_status ecl_device_mem_info(int device, size_t* free_bytes, size_t* total_bytes);
-/* Construction keeps its device scratch (tile histograms and their scan,
- * about 0.1 bytes per symbol) cached per device for the next build; this
- * frees it.  No counterpart in the reference, which allocates temporaries
- * per build; for callers that need the memory back between builds. */
+/* The library keeps freed device blocks of 1 MiB or more per device and
+ * hands them out again for requests of the same size class (up to 40% of
+ * the device's memory; freed when an allocation would fail), and keeps the
+ * construction scratch (tile histograms and their scan, about 0.1 bytes per
+ * symbol) for the next build.  This frees both.  No counterpart in the
+ * reference, which allocates and frees per call; for callers that need the
+ * memory back outside the library. */
 ecl_status ecl_release_cached_memory(int device);
 /* Tiles of 32768 codes each construction workgroup takes in turn (0:
  * automatic, up to 8 for large texts).  A tuning knob with no effect on
