@@ -142,6 +142,25 @@ __host__ __device__ __forceinline__ uint64_t bounded64(uint64_t h,
 #endif
 }
 
+// Element i of an array in device memory, read with a global load.  The
+// level views sit in LDS in the query kernels, so the pointers read from
+// them are generic to the compiler, which would issue flat loads: a flat
+// load counts on both the vector-memory and the LDS/scalar counters, and
+// the compiler waits for every outstanding LDS and flat access before using
+// one.  Every array a LevelView points to is device memory the library
+// allocated, so these pointers are always global.
+template <typename T>
+__device__ __forceinline__ T gld(const T* p, uint64_t i) {
+  return ((const __attribute__((address_space(1))) T*)p)[i];
+}
+// uint4 is a class type without a copy from another address space: load
+// the 16 bytes as a plain vector
+__device__ __forceinline__ uint4 gld(const uint4* p, uint64_t i) {
+  typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+  const u32x4 v = ((const __attribute__((address_space(1))) u32x4*)p)[i];
+  return make_uint4(v.x, v.y, v.z, v.w);
+}
+
 // ---------------------------------------------------------------------------
 // One bit array with its rank/select directory.  Trivially copyable; the
 // library keeps one per array in device memory.
@@ -190,7 +209,7 @@ constexpr uint32_t kLineBits = 960;  // bits per query line (30 words)
 constexpr uint32_t kHintBlocks = 23;  // blocks after b0 one hint covers
 
 __device__ __forceinline__ bool level_bit(const LevelView& v, uint64_t i) {
-  return (v.words[i >> 5] >> (i & 31)) & 1u;
+  return (gld(v.words, i >> 5) >> (i & 31)) & 1u;
 }
 
 // Ones in [0, index) for index in (0, size): L1 + L2 + popcount of the
@@ -200,8 +219,8 @@ __device__ __forceinline__ uint64_t rank1_inner(const LevelView& v,
                                                 uint64_t index, bool* bit) {
   const uint64_t blk = index >> 9;
   const uint4* p = reinterpret_cast<const uint4*>(v.words + (blk << 4));
-  uint4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3];
-  uint64_t r = v.l1[index >> 16] + v.l2[blk];
+  uint4 q0 = gld(p, 0), q1 = gld(p, 1), q2 = gld(p, 2), q3 = gld(p, 3);
+  uint64_t r = gld(v.l1, index >> 16) + gld(v.l2, blk);
   uint64_t w[8] = {
       (uint64_t)q0.x | ((uint64_t)q0.y << 32), (uint64_t)q0.z | ((uint64_t)q0.w << 32),
       (uint64_t)q1.x | ((uint64_t)q1.y << 32), (uint64_t)q1.z | ((uint64_t)q1.w << 32),
@@ -230,7 +249,7 @@ __device__ __forceinline__ uint64_t rank1_lines(const LevelView& v, uint64_t ind
   const uint4* p = v.lines + line * 8;
   uint4 q[8];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) q[k] = p[k];
+  for (int k = 0; k < 8; ++k) q[k] = gld(p, k);
   const uint64_t head = (uint64_t)q[0].x | ((uint64_t)q[0].y << 32);
   // bits as 15 u64 words: (q0.z, q0.w), (q1.x, q1.y), (q1.z, q1.w), ...
   uint64_t w[15];
@@ -299,7 +318,7 @@ __device__ __forceinline__ uint64_t rank1_pair(const LevelView& v, uint64_t inde
     const uint4* p = v.lines + line * 8 + 4 * t;
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-      const uint4 q = p[k];
+      const uint4 q = gld(p, k);
       d[4 * k] = q.x; d[4 * k + 1] = q.y; d[4 * k + 2] = q.z; d[4 * k + 3] = q.w;
     }
     // lane 0's first two dwords are the line's header
@@ -314,12 +333,12 @@ __device__ __forceinline__ uint64_t rank1_pair(const LevelView& v, uint64_t inde
     const uint4* p = reinterpret_cast<const uint4*>(v.words + (blk << 4)) + 2 * t;
 #pragma unroll
     for (int k = 0; k < 2; ++k) {
-      const uint4 q = p[k];
+      const uint4 q = gld(p, k);
       d[4 * k] = q.x; d[4 * k + 1] = q.y; d[4 * k + 2] = q.z; d[4 * k + 3] = q.w;
     }
 #pragma unroll
     for (int k = 8; k < 16; ++k) d[k] = 0;
-    base = v.l1[index >> 16] + v.l2[blk];
+    base = gld(v.l1, index >> 16) + gld(v.l2, blk);
     first = 0;
   }
   // this lane counts its bits before index and reads the bit at index when
@@ -381,7 +400,7 @@ __device__ __forceinline__ uint64_t select_in_block(const LevelView& v, uint64_t
                                                    uint64_t r) {
   // the 64 B block lies inside the array's 128 B padding
   const uint4* p = reinterpret_cast<const uint4*>(v.words + (blk2 << 4));
-  const uint4 q0 = p[0], q1 = p[1], q2 = p[2], q3 = p[3];
+  const uint4 q0 = gld(p, 0), q1 = gld(p, 1), q2 = gld(p, 2), q3 = gld(p, 3);
   const uint64_t w[8] = {
       (uint64_t)q0.x | ((uint64_t)q0.y << 32), (uint64_t)q0.z | ((uint64_t)q0.w << 32),
       (uint64_t)q1.x | ((uint64_t)q1.y << 32), (uint64_t)q1.z | ((uint64_t)q1.w << 32),
@@ -417,7 +436,7 @@ __device__ __forceinline__ uint64_t select_from_line(const LevelView& v, uint64_
   const uint4* p = v.sel_lines + ((Value ? 0 : v.sel_zero_line) + jj) * 8;
   uint4 q[8];
 #pragma unroll
-  for (int k = 0; k < 8; ++k) q[k] = p[k];
+  for (int k = 0; k < 8; ++k) q[k] = gld(p, k);
   const uint64_t s = (uint64_t)q[0].x | ((uint64_t)q[0].y << 32);
   uint64_t w[15];
   w[0] = (uint64_t)q[0].z | ((uint64_t)q[0].w << 32);
@@ -469,7 +488,7 @@ __device__ __forceinline__ uint64_t level_select(const LevelView& v,
     // the bracket's hint: sample position, Values before its first block,
     // and the counts before the next kHintBlocks blocks
     const uint4* h = v.sel_hints + (Value ? j : v.ones / kSampleRate + 1 + j) * 4;
-    const uint4 h0 = h[0], h1 = h[1], h2 = h[2], h3 = h[3];
+    const uint4 h0 = gld(h, 0), h1 = gld(h, 1), h2 = gld(h, 2), h3 = gld(h, 3);
     const uint64_t lo_pos = (uint64_t)h0.x | ((uint64_t)h0.y << 32);
     if (j > 0 && i == j * kSampleRate) return lo_pos;
     const uint32_t span = h1.x & 0xFFFFu;
@@ -493,19 +512,21 @@ __device__ __forceinline__ uint64_t level_select(const LevelView& v,
   }
   uint64_t lo_pos = 0, hi_pos = v.size - 1;
   if (j > 0) {
-    lo_pos = Value ? v.samples[j - 1] : v.samples[v.sample_cap - j];
+    lo_pos = gld(v.samples, Value ? j - 1 : v.sample_cap - j);
     if (i == j * kSampleRate) return lo_pos;
   }
-  if (j < nsamp) hi_pos = Value ? v.samples[j] : v.samples[v.sample_cap - 1 - j];
+  if (j < nsamp) hi_pos = gld(v.samples, Value ? j : v.sample_cap - 1 - j);
   // L1: largest block b in [lo_pos>>16, hi_pos>>16] with count_before(b) < i
   uint64_t b_lo = lo_pos >> 16, b_hi = hi_pos >> 16;
   while (b_lo < b_hi) {
     uint64_t mid = (b_lo + b_hi + 1) >> 1;
-    uint64_t c = Value ? v.l1[mid] : (mid << 16) - v.l1[mid];
+    const uint64_t e = gld(v.l1, mid);
+    uint64_t c = Value ? e : (mid << 16) - e;
     if (c < i) b_lo = mid; else b_hi = mid - 1;
   }
   const uint64_t blk1 = b_lo;
-  const uint64_t c1 = Value ? v.l1[blk1] : (blk1 << 16) - v.l1[blk1];
+  const uint64_t e1 = gld(v.l1, blk1);
+  const uint64_t c1 = Value ? e1 : (blk1 << 16) - e1;
   uint64_t r = i - c1;  // rank inside the L1 block (1-based)
   // L2: largest k in [max(lo_pos>>9, blk1*128), min(hi_pos>>9, end)] with
   // count_before(k) < r
@@ -521,7 +542,8 @@ __device__ __forceinline__ uint64_t level_select(const LevelView& v,
     for (uint32_t u = 1; u < 16; ++u) {
       if (u <= span) {
         const uint64_t k = k_lo + u;
-        const uint64_t c = Value ? v.l2[k] : ((k - first) << 9) - v.l2[k];
+        const uint64_t e = gld(v.l2, k);
+        const uint64_t c = Value ? e : ((k - first) << 9) - e;
         below += c < r ? 1u : 0u;
       }
     }
@@ -529,12 +551,14 @@ __device__ __forceinline__ uint64_t level_select(const LevelView& v,
   } else {
     while (k_lo < k_hi) {
       uint64_t mid = (k_lo + k_hi + 1) >> 1;
-      uint64_t c = Value ? v.l2[mid] : ((mid - first) << 9) - v.l2[mid];
+      const uint64_t e = gld(v.l2, mid);
+      uint64_t c = Value ? e : ((mid - first) << 9) - e;
       if (c < r) k_lo = mid; else k_hi = mid - 1;
     }
     blk2 = k_lo;
   }
-  r -= Value ? v.l2[blk2] : ((blk2 - first) << 9) - v.l2[blk2];
+  const uint64_t e2 = gld(v.l2, blk2);
+  r -= Value ? e2 : ((blk2 - first) << 9) - e2;
   return select_in_block<Value>(v, blk2, r);
 }
 
