@@ -56,8 +56,11 @@ def parse():
     ap.add_argument("--sigma", type=int, default=256)
     ap.add_argument("--queries", type=int, default=10_000_000,
                     help="queries of each kind per step")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0,
+    ap.add_argument("--cpu-seconds", type=float, default=25.0,
                     help="budget of the CPU-baseline sample")
+    ap.add_argument("--cpu-check-queries", type=int, default=1_000_000,
+                    help="rank and select queries (each) whose GPU answers the CPU "
+                         "baseline's variant B checks (variant A takes a prefix)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--latency-calls", type=int, default=200)
     ap.add_argument("--bt-texts", type=int, default=10_000,
@@ -130,52 +133,94 @@ def pass_traffic(n, sigma):
     return None
 
 
-def cpu_baseline(n, sigma, seed, args):
+def cpu_baseline(n, sigma, seed, args, sample):
+    """The oracle on the host cores, over the same 6e9-symbol text (the same
+    counter-based hash, generated on the host) and the first queries of the
+    GPU's own device-generated query arrays (`sample`, with the GPU's answers
+    to them, which both variants check).
+
+    Variant A is the literal oracle of the reference's tests: access reads
+    text[i]; rank counts c over [0, i); select walks to the k-th c (n when
+    absent); OpenMP over queries.  Rank and select cost O(n) per query, so
+    they run on a bounded sample sized to --cpu-seconds.  Variant B answers
+    the same queries from per-symbol counts before every 2^16 symbols plus a
+    scan of one block (a realistic CPU index; the table build is timed
+    separately).  `value` is variant A's 1:1:1 mixed rate."""
     import oracle as O
     t0 = time.time()
     text = O.gen_uniform(seed, n, sigma, np.uint8)
     log(f"cpu: generated host text in {time.time() - t0:.1f}s")
-    rng = np.random.default_rng(123)
     threads = O.max_threads()
-    # access: O(1) per query
-    na = 2_000_000
-    idx = rng.integers(0, n, na).astype(np.uint64)
+    try:
+        affinity = len(os.sched_getaffinity(0))
+    except AttributeError:
+        affinity = None
+    idx, a_gpu = sample["access_idx"], sample["access_gpu"]
+    rq, r_gpu = sample["rank_q"], sample["rank_gpu"]
+    sq, s_gpu = sample["select_q"], sample["select_gpu"]
+    # ---- variant A
     t0 = time.time()
-    O.scan_access_batch(text, idx)
-    ta = (time.time() - t0) / na
-    # rank / select: O(n) scans per query; sample sized to the budget
-    per = max(threads, 8)
-    c = rng.integers(0, sigma, per).astype(np.uint64)
-    i = rng.integers(0, n, per).astype(np.uint64)
+    a_cpu = O.scan_access_batch(text, idx)
+    ta = (time.time() - t0) / len(idx)
+    # rank/select: a first batch of 2 per thread estimates the per-query
+    # cost; the sample then grows to the time budget (at most the download)
+    per = 2 * threads
     t0 = time.time()
-    O.scan_rank_batch(text, i, c)
-    tr = (time.time() - t0) / per
-    k = np.array([1 + rng.integers(0, max(1, n // sigma)) for _ in range(per)], np.uint64)
-    t0 = time.time()
-    O.scan_select_batch(text, k, c)
-    ts = (time.time() - t0) / per
-    spent = ta * na + (tr + ts) * per
-    # extend the rank/select sample while inside the budget
-    extra = int(max(0, args.cpu_seconds - spent) / max(tr + ts, 1e-9) / 2) // per * per
-    if extra:
-        c2 = rng.integers(0, sigma, extra).astype(np.uint64)
-        i2 = rng.integers(0, n, extra).astype(np.uint64)
+    r_a = O.scan_rank_batch(text, rq["index"][:per], rq["symbol"][:per])
+    s_a = O.scan_select_batch(text, sq["index"][:per], sq["symbol"][:per])
+    first = time.time() - t0
+    budget = max(args.cpu_seconds - ta * len(idx), 0.0)
+    m_a = int(min(len(rq), max(per, per * budget / max(first, 1e-9)))) // per * per
+    t_rank = t_sel = 0.0
+    if m_a > per:
         t0 = time.time()
-        O.scan_rank_batch(text, i2, c2)
-        tr = (tr * per + (time.time() - t0)) / (per + extra)
-        k2 = np.array([1 + rng.integers(0, max(1, n // sigma)) for _ in range(extra)], np.uint64)
+        r_a = np.concatenate([r_a, O.scan_rank_batch(text, rq["index"][per:m_a],
+                                                     rq["symbol"][per:m_a])])
+        t_rank = time.time() - t0
         t0 = time.time()
-        O.scan_select_batch(text, k2, c2)
-        ts = (ts * per + (time.time() - t0)) / (per + extra)
-    del text
-    mixed = 3.0 / (ta + tr + ts)
+        s_a = np.concatenate([s_a, O.scan_select_batch(text, sq["index"][per:m_a],
+                                                       sq["symbol"][per:m_a])])
+        t_sel = time.time() - t0
+        tr, ts = t_rank / (m_a - per), t_sel / (m_a - per)
+    else:
+        tr = ts = first / (2 * per)
+    # ---- variant B
+    t0 = time.time()
+    B = O.BlockedOracle(text, sigma)
+    table_s = time.time() - t0
+    t0 = time.time()
+    r_b = B.rank_batch(rq["index"], rq["symbol"])
+    trb = (time.time() - t0) / len(rq)
+    t0 = time.time()
+    s_b = B.select_batch(sq["index"], sq["symbol"])
+    tsb = (time.time() - t0) / len(sq)
+    ok_a = (np.array_equal(a_cpu.astype(np.uint8), a_gpu) and np.array_equal(r_a, r_gpu[:len(r_a)])
+            and np.array_equal(s_a, s_gpu[:len(s_a)]))
+    ok_b = np.array_equal(r_b, r_gpu) and np.array_equal(s_b, s_gpu)
+    del text, B
     return {
-        "value": mixed, "unit": "queries/s", "cores": threads, "kind": "port",
-        "sample": (f"oracle direct scans (OpenMP, {threads} threads) over the same "
-                   f"6e9-symbol host text: {na} access, {per + extra} rank, "
-                   f"{per + extra} select queries; value = 1:1:1 mixed rate"),
+        "value": 3.0 / (ta + tr + ts), "unit": "queries/s", "cores": threads, "kind": "port",
+        "sample": (f"variant A, the oracle's direct scans (OpenMP, {threads} threads) over the "
+                   f"same 6e9-symbol text, generated on the host: the first {len(idx)} access, "
+                   f"{len(r_a)} rank and {len(s_a)} select queries of the GPU's query arrays; "
+                   f"value = 1:1:1 mixed rate"),
+        "omp_threads": threads, "nproc": os.cpu_count(), "affinity_cpus": affinity,
         "access_qps": 1.0 / ta, "rank_qps": 1.0 / tr, "select_qps": 1.0 / ts,
+        "rank_select_sample": len(r_a),
+        "variant_b": {
+            "kind": "port (blocked: per-symbol counts every 2^16 symbols + one-block scan)",
+            "table_build_s": table_s, "table_bytes": int(B_bytes(n, sigma)),
+            "access_qps": 1.0 / ta, "rank_qps": 1.0 / trb, "select_qps": 1.0 / tsb,
+            "mixed_qps": 3.0 / (ta + trb + tsb), "queries_per_kind": len(rq),
+        },
+        "gpu_answers_equal_variant_a": bool(ok_a),
+        "gpu_answers_equal_variant_b": bool(ok_b),
     }
+
+
+def B_bytes(n, sigma):
+    """Bytes of variant B's count table: one row of sigma u64 per 2^16 symbols."""
+    return ((n + 65535) // 65536 + 1) * sigma * 8
 
 
 def bench_batched(args):
@@ -766,6 +811,20 @@ def main():
     E.Profiler.enable(False)
     rank_k = E.Profiler.get("wt_rank")
 
+    # the first queries of each kind (the last step's device-generated
+    # arrays) and the GPU's answers to them: the CPU baseline runs on these
+    # and checks the answers
+    ka, kr = min(Q, 2_000_000), min(Q, args.cpu_check_queries)
+    sample = {"access_idx": d_idx.download(np.uint64, ka),
+              "rank_q": d_rq.download(E.QUERY_DTYPE, kr),
+              "select_q": d_sq.download(E.QUERY_DTYPE, kr)}
+    wt.accessDevice(d_idx, ka, d_out, flags=2)
+    sample["access_gpu"] = d_out.download(np.uint8, ka)
+    wt.rankDevice(d_rq, kr, d_out, flags=1)
+    sample["rank_gpu"] = d_out.download(np.uint64, kr)
+    wt.selectDevice(d_sq, kr, d_out, flags=1)
+    sample["select_gpu"] = d_out.download(np.uint64, kr)
+
     L = wt.getNumLevels()
     del wt
     # the queries' roofline: random dependent 128-byte lines per second on
@@ -876,7 +935,7 @@ def main():
     if args.full_sweep:
         res["full_size_sweep"] = bench_full_sweep(args)
     if not args.no_cpu:
-        res["cpu_baseline"] = cpu_baseline(n, sigma, seed, args)
+        res["cpu_baseline"] = cpu_baseline(n, sigma, seed, args, sample)
     print(json.dumps(res), flush=True)
 
 

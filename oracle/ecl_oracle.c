@@ -525,14 +525,42 @@ uint64_t orc_wt_select(const orc_wt* wt, uint64_t c, uint64_t k) {
  * select = position of the k-th occurrence via find_if (test_utils.cu:
  * 81-91), n when absent (test_wavelet_tree.cu:674-678).
  * ---------------------------------------------------------------------- */
+/* occurrences of byte c in t[0, len): 16-byte compares accumulated in byte
+ * lanes, 255 rounds of 64 bytes at a time (the count of one lane stays
+ * below 256), then a byte loop for the tail -- a vectorizable form of the
+ * same count */
+typedef uint8_t orc_v16 __attribute__((vector_size(16)));
+#define ORC_COUNT_BLOCK (64u * 255u)
+static uint64_t count_u8(const uint8_t* t, uint64_t len, uint8_t c) {
+  orc_v16 cv;
+  memset(&cv, c, sizeof(cv));
+  uint64_t r = 0, j = 0;
+  while (j + ORC_COUNT_BLOCK <= len) {
+    orc_v16 a0 = {0}, a1 = {0}, a2 = {0}, a3 = {0};
+    for (unsigned k = 0; k < 255; ++k, j += 64) {
+      orc_v16 x0, x1, x2, x3;
+      memcpy(&x0, t + j, 16);
+      memcpy(&x1, t + j + 16, 16);
+      memcpy(&x2, t + j + 32, 16);
+      memcpy(&x3, t + j + 48, 16);
+      a0 -= (orc_v16)(x0 == cv);
+      a1 -= (orc_v16)(x1 == cv);
+      a2 -= (orc_v16)(x2 == cv);
+      a3 -= (orc_v16)(x3 == cv);
+    }
+    for (int k = 0; k < 16; ++k) r += (uint64_t)a0[k] + a1[k] + a2[k] + a3[k];
+  }
+  for (; j < len; ++j) r += t[j] == c;
+  return r;
+}
+
 uint64_t orc_scan_rank(const void* text, int esize, uint64_t n, uint64_t c,
                        uint64_t i) {
   uint64_t r = 0;
   if (i > n) i = n;
   switch (esize) {
     case 1: {
-      const uint8_t* t = (const uint8_t*)text;
-      for (uint64_t j = 0; j < i; ++j) r += t[j] == c;
+      if (c < 256) r = count_u8((const uint8_t*)text, i, (uint8_t)c);
       break;
     }
     case 2: {
@@ -551,8 +579,18 @@ uint64_t orc_scan_select(const void* text, int esize, uint64_t n, uint64_t c,
   if (k == 0) return n;
   switch (esize) {
     case 1: {
+      /* whole blocks are counted while the k-th occurrence lies past them;
+       * the block that holds it is walked byte by byte */
       const uint8_t* t = (const uint8_t*)text;
-      for (uint64_t j = 0; j < n; ++j)
+      if (c >= 256) break;
+      uint64_t j = 0;
+      while (j + ORC_COUNT_BLOCK <= n) {
+        const uint64_t b = count_u8(t + j, ORC_COUNT_BLOCK, (uint8_t)c);
+        if (seen + b >= k) break;
+        seen += b;
+        j += ORC_COUNT_BLOCK;
+      }
+      for (; j < n; ++j)
         if (t[j] == c && ++seen == k) return j;
       break;
     }

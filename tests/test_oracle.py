@@ -439,3 +439,24 @@ def test_by_construction_round_trip():
     B = O.BlockedOracle(text, 64)
     assert np.array_equal(B.select_batch(r + 1, c), p)
     assert np.array_equal(r, B.rank_batch(p, c))
+
+
+@pytest.mark.parametrize("sigma", [2, 4, 256])
+def test_direct_scans_at_count_block_edges(sigma):
+    """The u8 direct scans count whole 16320-byte blocks at a time: rank at
+    and around block edges, and select of occurrences that are the first or
+    last byte of a block, equal numpy prefix sums / occurrence lists."""
+    rng = np.random.default_rng(sigma)
+    blk = 64 * 255
+    n = 5 * blk + 77
+    text = rng.integers(0, sigma, n).astype(np.uint8)
+    edges = np.array([j * blk + d for j in range(6) for d in (-1, 0, 1)] + [n], np.int64)
+    edges = np.unique(np.clip(edges, 0, n)).astype(np.uint64)
+    for c in range(min(sigma, 4)):
+        sym = np.full(len(edges), c, np.uint64)
+        assert np.array_equal(O.scan_rank_batch(text, edges, sym), _numpy_rank(text, edges, sym))
+        occ = np.flatnonzero(text == c)
+        at = [q for q in range(len(occ)) if occ[q] % blk in (0, blk - 1)][:20]
+        k = np.array([q + 1 for q in at] + [len(occ), len(occ) + 1, 1], np.uint64)
+        sym = np.full(len(k), c, np.uint64)
+        assert np.array_equal(O.scan_select_batch(text, k, sym), _numpy_select(text, k, sym))
