@@ -559,12 +559,14 @@ class _LoadSpec(C.Structure):
 
 class Server:
     """Query server with continuous batching (ecl_server_*): resident trees
-    admitted by an HBM budget, requests coalesced into device batches."""
+    admitted by an HBM budget, requests coalesced into device batches.
+    query_lines: 0/False none, 1/True query lines and select hints for every
+    admitted tree, 2 also select lines."""
 
     def __init__(self, device=0, hbm_budget=0, max_batch=0, max_inflight=0, max_trees=0,
-                 coalesce=True, query_lines=False):
+                 coalesce=True, query_lines=0):
         cfg = _ServerConfig(hbm_budget, max_batch, max_inflight, max_trees,
-                            1 if coalesce else 0, 1 if query_lines else 0)
+                            1 if coalesce else 0, int(query_lines))
         h = C.c_void_p()
         call("ecl_server_create", device, C.byref(cfg), C.byref(h))
         self._h = h

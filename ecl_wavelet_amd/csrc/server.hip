@@ -271,6 +271,7 @@ ecl_status ecl_server_create(int device, const ecl_server_config* config, ecl_se
     if (s->cfg.max_trees == 0) s->cfg.max_trees = 1024;
     if (s->cfg.hbm_budget == 0) s->cfg.hbm_budget = free_device_memory() / 10 * 9;
     require(s->cfg.max_inflight <= 16, "at most 16 batches in flight");
+    require(s->cfg.query_lines <= 2, "query_lines must be 0, 1 or 2");
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
         cus > 0)
@@ -335,25 +336,40 @@ ecl_status ecl_server_destroy(ecl_server s) {
 
 }  // extern "C"
 
-// Bits, directory and metadata of a uint8 tree of n symbols over sigma, and
-// with `lines` its query lines (128 B per 960 bits) and select hints (64 B
-// per 4096 ones or zeros, plus one per Value) of every level.
-static uint64_t tree_estimate(uint64_t n, uint64_t sigma, bool lines) {
+// Bits, directory and metadata of a uint8 tree of n symbols over sigma;
+// with lines >= 1 its query lines (128 B per 960 bits) and select hints (64 B
+// per 4096 ones or zeros, plus one per Value) of every level; with lines == 2
+// also its select lines (per array and Value one 128 B line every r of them,
+// r = max(1, floor(800 x density)): at most 4 / 800 lines per bit).
+static uint64_t tree_estimate(uint64_t n, uint64_t sigma, uint32_t lines) {
   const uint64_t L = ecl::detail::ceil_log2(sigma);
   uint64_t b = L * ((n + 1023) / 1024 * 128 + (n + 511) / 512 * 2 + (n + 65535) / 65536 * 8 +
                     n / 4096 * 8 + 1024) +
                sigma * 64 + 4096;
-  if (lines)
+  if (lines >= 1)
     b += L * ((n + ecl::detail::kLineBits - 1) / ecl::detail::kLineBits * 128 +
               (n / ecl::detail::kSampleRate + 2) * 64);
+  if (lines >= 2) b += L * ((n / 200 + 4) * 128);
   return b;
 }
 
-// The tree's query lines when the server is configured for them; on failure
-// the tree is destroyed and the error returned.
+// free device memory as the library sees it (the runtime's free bytes plus
+// the library's cached blocks, which its allocations reuse)
+static size_t lib_free_memory(int device) {
+  size_t fr = 0, tot = 0;
+  if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+    (void)hipGetLastError();
+    return 0;
+  }
+  return fr + pooled_bytes(device);
+}
+
+// The tree's query lines (and select lines) when the server is configured
+// for them; on failure the tree is destroyed and the error returned.
 static ecl_status add_query_lines(ecl_server s, ecl_wavelet_tree wt, std::string* msg) {
   if (!s->cfg.query_lines) return ECL_OK;
-  const ecl_status st = ecl_wavelet_tree_build_query_lines(wt);
+  ecl_status st = ecl_wavelet_tree_build_query_lines(wt);
+  if (st == ECL_OK && s->cfg.query_lines >= 2) st = ecl_wavelet_tree_build_select_lines(wt);
   if (st != ECL_OK) {
     *msg = last_error();
     (void)ecl_wavelet_tree_destroy(wt);
@@ -410,13 +426,12 @@ ecl_status ecl_server_add_tree(ecl_server s, const void* text, int text_on_devic
     require(n > 0, "empty text");
     set_device(s->device);
     std::lock_guard<std::mutex> build_lk(s->build_mu);
-    const uint64_t estimate = tree_estimate(n, sigma, s->cfg.query_lines != 0);
+    const uint64_t estimate = tree_estimate(n, sigma, s->cfg.query_lines);
     const uint32_t slot = reserve_slot(s, estimate);
     std::vector<uint8_t> alpha(sigma);
     for (uint64_t c = 0; c < sigma; ++c) alpha[c] = static_cast<uint8_t>(c);
     ecl_wavelet_tree wt = nullptr;
-    size_t fr0 = 0, fr1 = 0, tot = 0;
-    (void)hipMemGetInfo(&fr0, &tot);
+    const size_t fr0 = lib_free_memory(s->device);
     ecl_status st =
         text_on_device ? ecl_wavelet_tree_create_device(ECL_U8, text, n, alpha.data(), sigma,
                                                         s->device, &wt)
@@ -424,7 +439,7 @@ ecl_status ecl_server_add_tree(ecl_server s, const void* text, int text_on_devic
                                                  &wt);
     std::string msg = st != ECL_OK ? last_error() : std::string();
     if (st == ECL_OK) st = add_query_lines(s, wt, &msg);
-    (void)hipMemGetInfo(&fr1, &tot);
+    const size_t fr1 = lib_free_memory(s->device);
     if (st != ECL_OK) {
       release_slot(s, slot, estimate);
       throw Error(st, msg);
@@ -444,11 +459,10 @@ ecl_status ecl_server_add_tree_file(ecl_server s, const char* path, uint32_t* tr
             "the server holds uint8 trees with alphabets of 2 to 256 symbols");
     set_device(s->device);
     std::lock_guard<std::mutex> build_lk(s->build_mu);
-    const uint64_t estimate = tree_estimate(n, sigma, s->cfg.query_lines != 0);
+    const uint64_t estimate = tree_estimate(n, sigma, s->cfg.query_lines);
     const uint32_t slot = reserve_slot(s, estimate);
     ecl_wavelet_tree wt = nullptr;
-    size_t fr0 = 0, fr1 = 0, tot = 0;
-    (void)hipMemGetInfo(&fr0, &tot);
+    const size_t fr0 = lib_free_memory(s->device);
     ecl_status st = ecl_wavelet_tree_load(path, s->device, &wt);
     std::string msg = st != ECL_OK ? last_error() : std::string();
     if (st == ECL_OK) {
@@ -461,7 +475,7 @@ ecl_status ecl_server_add_tree_file(ecl_server s, const char* path, uint32_t* tr
       }
     }
     if (st == ECL_OK) st = add_query_lines(s, wt, &msg);
-    (void)hipMemGetInfo(&fr1, &tot);
+    const size_t fr1 = lib_free_memory(s->device);
     if (st != ECL_OK) {
       release_slot(s, slot, estimate);
       throw Error(st, msg);

@@ -404,7 +404,9 @@ typedef struct ecl_server_config {
                            * per-call form) */
   uint32_t query_lines;   /* 1: every admitted tree also gets its query lines and
                            * select hints (ecl_wavelet_tree_build_query_lines; about
-                           * 1.08x its bits more memory, charged to the budget) */
+                           * 1.08x its bits more memory); 2: and its select lines
+                           * (ecl_wavelet_tree_build_select_lines; about 2.7x its
+                           * bits more).  Charged to the budget. */
 } ecl_server_config;
 typedef struct ecl_server_stats {
   uint64_t requests;      /* completed requests since the last reset */

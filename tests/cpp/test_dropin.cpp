@@ -8,6 +8,7 @@
 
 #include <algorithm>
 #include <cstdio>
+#include <limits>
 #include <numeric>
 #include <random>
 #include <string>
@@ -35,6 +36,16 @@ static int g_checks = 0;
   } while (0)
 
 namespace ecl {
+
+// ---- utils in device code (test_utils.cu:17-57) -----------------------------
+// One launch evaluates the device helpers for every i below the limit.
+template <typename T>
+__global__ void utilsKernel(uint32_t limit, uint64_t* prev, uint64_t* clog) {
+  for (uint32_t i = threadIdx.x; i < limit; i += blockDim.x) {
+    prev[i] = utils::getPrevPowTwo<T>(static_cast<T>(i));
+    clog[i] = i ? utils::ceilLog2<T>(static_cast<T>(i)) : 0;
+  }
+}
 
 // ---- probe kernels (test_rank_select.cu:21-111) ------------------------------
 __global__ void writeWordKernel(BitArray ba, size_t a, size_t i, uint32_t v) { ba.writeWord(a, i, v); }
@@ -450,9 +461,41 @@ static void test_save_load() {
   std::printf("save/load: ok so far (%d failures)\n", g_fail);
 }
 
+// getPrevPowTwo (largest power of two strictly below i, 0 for 0 and 1) and
+// ceilLog2 (ceilLog2(1) = 0) for every i < min(1024, max(T)), in a kernel
+// and, for ceilLog2, on the host, as the reference's UtilsTest does.
+template <typename T>
+static void test_utils_type() {
+  const uint32_t limit =
+      static_cast<uint32_t>(std::min<uint64_t>(1024, std::numeric_limits<T>::max()));
+  uint64_t *prev = nullptr, *clog = nullptr;
+  gpuErrchk(hipMallocManaged(&prev, 1024 * sizeof(uint64_t)));
+  gpuErrchk(hipMallocManaged(&clog, 1024 * sizeof(uint64_t)));
+  utilsKernel<T><<<1, 256>>>(limit, prev, clog);
+  kernelCheck();
+  uint64_t expected = 0;
+  for (uint32_t i = 0; i < limit; ++i) {
+    EXPECT_EQ(prev[i], expected);
+    if (utils::isPowTwo(static_cast<T>(i))) expected = i;
+  }
+  expected = 0;
+  for (uint32_t i = 1; i < limit; ++i) {
+    EXPECT_EQ(static_cast<uint64_t>(utils::ceilLog2<T>(static_cast<T>(i))), expected);
+    EXPECT_EQ(clog[i], expected);
+    if (utils::isPowTwo(static_cast<T>(i))) ++expected;
+  }
+  gpuErrchk(hipFree(prev));
+  gpuErrchk(hipFree(clog));
+  std::printf("utils<%zu>: ok so far (%d failures)\n", sizeof(T), g_fail);
+}
+
 int main() {
   gpuErrchk(hipMallocManaged(&g_res, sizeof(size_t)));
   try {
+    test_utils_type<uint8_t>();
+    test_utils_type<uint16_t>();
+    test_utils_type<uint32_t>();
+    test_utils_type<uint64_t>();
     test_bit_array();
     test_rank_select();
     test_tree_type<uint8_t>();

@@ -298,6 +298,43 @@ def test_oom_message(gpu):
     assert str(e.value) == "Not enough memory available for the wavelet tree."
 
 
+def test_query_oom_message(gpu):
+    """Host queries whose device staging does not fit raise the reference's
+    exact message for their kind (wavelet_tree.cuh:972-973, 1268-1269,
+    1587-1588); with the memory back, the same calls answer."""
+    from ecl_wavelet_amd import DeviceBuffer, WaveletTree, mem_info, release_cached_memory
+    from ecl_wavelet_amd._lib import EclError, ECL_ERROR_OUT_OF_MEMORY_QUERIES
+    rng = np.random.default_rng(3)
+    text = rng.integers(0, 4, 100_000).astype(np.uint8)
+    wt = WaveletTree(text, np.arange(4, dtype=np.uint8))
+    m = 1 << 22  # one full staging chunk: 2 x 2^22 x (16 + 8) bytes on the device
+    q = wt.make_queries(rng.integers(0, len(text), m), rng.integers(0, 4, m))
+    idx = q["index"].copy()
+    release_cached_memory()
+    fillers = []
+    try:
+        # leave less than the 75 MB a chunk pair of access queries needs
+        # (rank and select need 201 MB)
+        take = 64 << 30
+        while mem_info()[0] > (64 << 20) and take >= (8 << 20):
+            want = min(take, mem_info()[0] - (48 << 20))
+            try:
+                fillers.append(DeviceBuffer(want))
+            except EclError:
+                take //= 2
+        assert mem_info()[0] < (72 << 20)
+        for kind, call in (("access", lambda: wt.access(idx)), ("rank", lambda: wt.rank(q)),
+                           ("select", lambda: wt.select(q))):
+            with pytest.raises(EclError) as e:
+                call()
+            assert e.value.status == ECL_ERROR_OUT_OF_MEMORY_QUERIES
+            assert str(e.value) == f"Not enough GPU memory available for the {kind} queries."
+    finally:
+        for f in fillers:
+            f.free()
+    _check_queries(wt, text, None, nq=500, seed=4)
+
+
 def test_scratch_cache_reuse_release_and_threads(gpu):
     """Builds reuse the per-device construction scratch (growing, shrinking,
     released in between); two threads building at once (one of them on a
