@@ -110,6 +110,9 @@ _SIGS = {
     "ecl_rank_select_rank_generated": (st, [vp, sz, i32, u64, sz, vp, vp]),
     "ecl_rank_select_select_generated": (st, [vp, sz, i32, u64, sz, vp, vp]),
     "ecl_rank_select_needed_gpu_memory": (sz, [sz, C.c_uint8]),
+    "ecl_rank_select_directory_sizes": (st, [u64, C.POINTER(u64), C.POINTER(u64),
+                                             C.POINTER(u64)]),
+    "ecl_rank_select_build_directory_noalloc": (st, [i32, vp, u64, vp, vp, vp, vp, vp]),
     "ecl_wavelet_tree_create": (st, [i32, vp, sz, vp, sz, i32, C.POINTER(vp)]),
     "ecl_wavelet_tree_create_device": (st, [i32, vp, sz, vp, sz, i32, C.POINTER(vp)]),
     "ecl_wavelet_tree_destroy": (st, [vp]),

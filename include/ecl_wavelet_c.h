@@ -234,6 +234,34 @@ ecl_status ecl_rank_select_select_generated(ecl_rank_select rs, size_t array_ind
                                             uint64_t* d_out, void* stream);
 /* RankSelect::getNeededGPUMemory (rank_select.cu:382-409) */
 size_t ecl_rank_select_needed_gpu_memory(size_t size, uint8_t num_arrays);
+/* Entry counts of the directory of one bit array of size_bits bits, as the
+ * RankSelect constructor lays it out (rank_select.cu:57-117; the same values
+ * as getNumL1Blocks / getNumL2Blocks and floor(size / 4096) sample slots):
+ * the sizes of the arrays ecl_rank_select_build_directory_noalloc writes. */
+ecl_status ecl_rank_select_directory_sizes(uint64_t size_bits, uint64_t* num_l1,
+                                           uint64_t* num_l2, uint64_t* sample_slots);
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): the directory that ecl_rank_select_create builds for one
+ * bit array -- L1 entries (num_l1 u64), L2 entries (num_l2 u16), select
+ * samples (sample_slots u64: ones samples from the front, zeros samples from
+ * the back; slots between the two are left as they are) and the total of
+ * ones (one u64) -- written into the caller's device arrays on `stream`
+ * (NULL = default stream), with the same kernels and the same values.
+ * d_words holds the bits LSB-first in u32 words and must be readable up to
+ * the next multiple of 64 bytes (a BitArray's arrays are: they are padded to
+ * 128 bytes); bits at or past size_bits are ignored.  The call allocates
+ * nothing: its scratch (the array's descriptor and 8 bytes per 65536 bits)
+ * is a fixed 4 MiB + 256 B device array of the library, one per GPU, held
+ * until the process ends; size_bits above ECL_NOALLOC_MAX_BITS is rejected
+ * with ECL_ERROR_INVALID_ARGUMENT.  Every call on a GPU shares that GPU's
+ * scratch: calls that overlap on one GPU (other streams, other threads)
+ * race and must be serialised by the caller. */
+#define ECL_NOALLOC_MAX_BITS (1ull << 35)
+ecl_status ecl_rank_select_build_directory_noalloc(int device, const uint32_t* d_words,
+                                                   uint64_t size_bits, uint64_t* d_l1,
+                                                   uint16_t* d_l2, uint64_t* d_samples,
+                                                   uint64_t* d_total_ones, void* stream);
 
 /* ---- WaveletTree<T> (tree/wavelet_tree.cuh) ----------------------------- */
 /* WaveletTree(data, data_size, alphabet&&, GPU_index) (wavelet_tree.cuh:

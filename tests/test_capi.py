@@ -55,3 +55,24 @@ def test_null_handle_is_invalid_argument(name):
     rc = f(None, None, 0, None, 0) if name == "ecl_wavelet_tree_access" else f(
         None, 0, 1, None, 0, None)
     assert rc == _lib.ECL_ERROR_INVALID_ARGUMENT
+
+
+def test_directory_sizes_match_the_oracle():
+    """ecl_rank_select_directory_sizes (host only) gives the RankSelect
+    layout's entry counts (rank_select.cu:57-117) for edge sizes."""
+    import oracle as O
+    lib = _lib.load()
+    for m in [0, 1, 511, 512, 513, 4095, 4096, 65535, 65536, 65537, 131071, 2**34 + 7]:
+        a, b, c = ctypes.c_uint64(), ctypes.c_uint64(), ctypes.c_uint64()
+        assert lib.ecl_rank_select_directory_sizes(m, ctypes.byref(a), ctypes.byref(b),
+                                                   ctypes.byref(c)) == 0
+        n1, n2 = O.rs_sizes(m)[:2] if m else (0, 0)
+        assert (a.value, b.value, c.value) == (n1, n2, m // 4096), m
+
+
+def test_noalloc_rejects_oversize_before_any_device_call():
+    lib = _lib.load()
+    rc = lib.ecl_rank_select_build_directory_noalloc(0, None, 2**35 + 1, None, None, None,
+                                                     None, None)
+    assert rc == _lib.ECL_ERROR_INVALID_ARGUMENT
+    assert b"ECL_NOALLOC_MAX_BITS" in lib.ecl_last_error()
