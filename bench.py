@@ -559,62 +559,79 @@ def bench_full_sweep(args):
     return out
 
 
+SERVE_RATES = (1e3, 2e3, 5e3, 1e4, 2e4, 5e4, 1e5, 2e5)
+SERVE_P99_TARGETS_MS = (2.0, 5.0, 10.0, 20.0, 50.0, 100.0, 300.0)
+
+
+def _serve_sweep(srv, mode, coalesce):
+    """Open-loop Poisson load at every rate of SERVE_RATES (at least about
+    4000 requests each), stopping once p99 passes one second (the form is
+    saturated); returns one row per rate."""
+    srv.setMode(coalesce)
+    rows = []
+    for rate in SERVE_RATES:
+        secs = max(0.2, 4000.0 / rate)
+        st = srv.load(rate, secs, seed=int(rate), min_size=1, max_size=1_000_000, pool=1 << 20)
+        row = {"rate_req_s": rate, "seconds": secs,
+               **{k: st[k] for k in ("requests", "queries", "batches", "p50_ms", "p99_ms",
+                                     "queries_per_s")}}
+        rows.append(row)
+        log(f"serving {mode} @ {rate:g} req/s: {st['queries_per_s']:.3g} q/s, "
+            f"p50 {st['p50_ms']:.2f} ms, p99 {st['p99_ms']:.2f} ms, {st['batches']} batches")
+        if st["p99_ms"] > 1000.0:
+            break
+    return rows
+
+
+def _best_under(rows, p99_ms):
+    """Highest delivered throughput among the rates whose p99 stays within
+    p99_ms (0 when none does)."""
+    ok = [r["queries_per_s"] for r in rows if r["p99_ms"] <= p99_ms]
+    return max(ok) if ok else 0.0
+
+
 def bench_serving(args):
     """Serving configuration: args.sv_trees resident uint8 trees, lengths
     log-uniform in [2^20, 2^sv_max_log2], sigma in {4, 256}; Poisson request
     streams of mixed batches of log-uniform size in [1, 10^6] against a
-    random tree, served with coalescing and in the plain per-call form; then
-    the same streams served by a server whose trees carry query lines and
-    select hints."""
+    random tree, swept over offered rates, served with coalescing and in the
+    plain per-call form (one request per batch, one batch at a time).  The
+    two forms are compared at matched p99: for each p99 target, the highest
+    throughput each form delivers at a rate whose p99 stays within it.  The
+    sweep runs over the trees' reference-shaped directory, then over a
+    server whose trees also carry query lines, select hints and select
+    lines (the layout of the headline numbers)."""
     import ecl_wavelet_amd as E
     rng = np.random.default_rng(10)
     T = args.sv_trees
     lens = np.floor(2.0 ** rng.uniform(20, args.sv_max_log2, T)).astype(np.uint64)
     sig = rng.choice([4, 256], T)
-    srv = E.Server(max_batch=1 << 22, max_inflight=3)
-    timer = E.Timer()
-    timer.start()
-    for i in range(T):
-        d = E.generate_text(int(lens[i]), int(sig[i]), np.uint8, seed=200_000 + i)
-        srv.addTree(d, int(sig[i]), n=int(lens[i]))
-        d.free()
-    admit_ms = timer.stop()
-    st0 = srv.stats()
-    log(f"serving: {T} trees, {int(lens.sum())} symbols, admitted in {admit_ms:.0f} ms, "
-        f"{st0['hbm_used'] / 1e9:.1f} GB resident")
     out = {"config": (f"{T} resident uint8 trees, lengths log-uniform in [2^20, "
                       f"2^{args.sv_max_log2:g}] (total {int(lens.sum())}), sigma in {{4, 256}}; "
                       f"Poisson requests of log-uniform size in [1, 10^6] mixed queries"),
-           "admit_ms": admit_ms, "hbm_used_gb": st0["hbm_used"] / 1e9, "rates": []}
-    for rate, secs in ((1e3, 3.0), (1e4, 1.0), (1e5, 0.1)):
-        row = {"rate_req_s": rate, "seconds": secs}
-        for mode, coalesce in (("served", True), ("plain", False)):
-            srv.setMode(coalesce)
-            st = srv.load(rate, secs, seed=int(rate), min_size=1, max_size=1_000_000,
-                          pool=1 << 20)
-            row[mode] = {k: st[k] for k in ("requests", "queries", "batches", "p50_ms",
-                                            "p99_ms", "queries_per_s")}
-            log(f"serving {mode} @ {rate:g} req/s: {st['queries_per_s']:.3g} q/s, "
-                f"p50 {st['p50_ms']:.2f} ms, p99 {st['p99_ms']:.2f} ms, {st['batches']} batches")
-        out["rates"].append(row)
-    srv.close()
-    # the same trees and request streams, served over query lines and hints
-    srv = E.Server(max_batch=1 << 22, max_inflight=3, query_lines=True)
-    timer.start()
-    for i in range(T):
-        d = E.generate_text(int(lens[i]), int(sig[i]), np.uint8, seed=200_000 + i)
-        srv.addTree(d, int(sig[i]), n=int(lens[i]))
-        d.free()
-    out["admit_ms_query_lines"] = timer.stop()
-    out["hbm_used_gb_query_lines"] = srv.stats()["hbm_used"] / 1e9
-    for row in out["rates"]:
-        rate, secs = row["rate_req_s"], row["seconds"]
-        st = srv.load(rate, secs, seed=int(rate), min_size=1, max_size=1_000_000, pool=1 << 20)
-        row["served_query_lines"] = {k: st[k] for k in ("requests", "queries", "batches",
-                                                        "p50_ms", "p99_ms", "queries_per_s")}
-        log(f"serving over query lines @ {rate:g} req/s: {st['queries_per_s']:.3g} q/s, "
-            f"p50 {st['p50_ms']:.2f} ms, p99 {st['p99_ms']:.2f} ms")
-    srv.close()
+           "p99_targets_ms": list(SERVE_P99_TARGETS_MS)}
+    timer = E.Timer()
+    for layout, lines in (("directory", 0), ("lines", 2)):
+        srv = E.Server(max_batch=1 << 22, max_inflight=3, query_lines=lines)
+        timer.start()
+        for i in range(T):
+            d = E.generate_text(int(lens[i]), int(sig[i]), np.uint8, seed=200_000 + i)
+            srv.addTree(d, int(sig[i]), n=int(lens[i]))
+            d.free()
+        admit_ms = timer.stop()
+        st0 = srv.stats()
+        log(f"serving ({layout}): {T} trees, {int(lens.sum())} symbols, admitted in "
+            f"{admit_ms:.0f} ms, {st0['hbm_used'] / 1e9:.1f} GB resident")
+        served = _serve_sweep(srv, "served", True)
+        plain = _serve_sweep(srv, "plain", False)
+        srv.close()
+        matched = []
+        for tgt in SERVE_P99_TARGETS_MS:
+            a, b = _best_under(served, tgt), _best_under(plain, tgt)
+            matched.append({"p99_ms": tgt, "served_qps": a, "plain_qps": b,
+                            "ratio": (a / b) if b > 0 else None})
+        out[layout] = {"admit_ms": admit_ms, "hbm_used_gb": st0["hbm_used"] / 1e9,
+                       "served": served, "plain": plain, "matched_p99": matched}
     return out
 
 

@@ -6,7 +6,8 @@ C ABI in ``include/ecl_wavelet_c.h``).  This package builds it
 class names in Python (``api``).
 """
 from .api import (BitArray, DeviceBuffer, Forest, Graph, Profiler, RankSelect,  # noqa: F401
-                  Server, Timer, WaveletTree, device_name, generate_text,
+                  Server, Timer, WaveletTree, convert_data_to_min_alphabet, device_name,
+                  generate_text, read_data_from_file,
                   generate_text_into, mem_info, release_cached_memory, set_build_span,
                   set_query_lanes, probe_random_lines,
                   zipf_thresholds,

@@ -115,6 +115,12 @@ _SIGS = {
     "ecl_rank_select_build_directory_noalloc": (st, [i32, vp, u64, vp, vp, vp, vp, vp]),
     "ecl_wavelet_tree_create": (st, [i32, vp, sz, vp, sz, i32, C.POINTER(vp)]),
     "ecl_wavelet_tree_create_device": (st, [i32, vp, sz, vp, sz, i32, C.POINTER(vp)]),
+    "ecl_read_text_file": (st, [C.c_char_p, i32, sz, vp]),
+    "ecl_convert_to_min_alphabet": (st, [i32, vp, sz, vp, sz, vp, C.POINTER(sz)]),
+    "ecl_wavelet_tree_create_from_file": (st, [i32, C.c_char_p, sz, vp, sz, i32,
+                                               C.POINTER(vp)]),
+    "ecl_server_add_tree_text_file": (st, [vp, C.c_char_p, sz, vp, C.POINTER(u64),
+                                           C.POINTER(u32)]),
     "ecl_wavelet_tree_destroy": (st, [vp]),
     "ecl_wavelet_tree_build_query_lines": (st, [vp]),
     "ecl_wavelet_tree_build_select_lines": (st, [vp]),

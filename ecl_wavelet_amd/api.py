@@ -115,6 +115,36 @@ def generate_text(n, sigma, dtype=np.uint8, seed=1, thresholds=None, device=0):
     return buf
 
 
+_ESIZE_DTYPE = {1: np.uint8, 2: np.uint16, 4: np.uint32, 8: np.uint64}
+
+
+def read_data_from_file(path, num_symbols, dtype=np.uint8):
+    """readDataFromFile<T> (utils/test_benchmark_utils.cuh:588-619): the first
+    num_symbols elements of a raw text file, with the reference's checks."""
+    dtype = np.dtype(dtype)
+    out = np.empty(int(num_symbols), dtype=dtype)
+    call("ecl_read_text_file", str(path).encode(), dtype.itemsize, out.size, _p(out))
+    return out
+
+
+def convert_data_to_min_alphabet(data, alphabet=None, return_alphabet=False):
+    """convertDataToMinAlphabet<T> (utils/test_benchmark_utils.cuh:628-679):
+    replaces every element of the contiguous numpy array `data` in place by
+    its index in the sorted distinct elements (or, by lower bound, in the
+    sorted `alphabet`); returns the alphabet size (and the alphabet)."""
+    assert isinstance(data, np.ndarray) and data.flags.c_contiguous
+    dt = data.dtype
+    a = None if alphabet is None else np.ascontiguousarray(alphabet, dtype=dt)
+    cap = (1 << (8 * dt.itemsize)) if dt.itemsize <= 2 else max(data.size, 1)
+    out = np.empty(cap if a is None else len(a), dtype=dt)
+    sigma = C.c_size_t()
+    call("ecl_convert_to_min_alphabet", dt.itemsize, _p(data), data.size,
+         None if a is None else _p(a), 0 if a is None else len(a), _p(out), C.byref(sigma))
+    if return_alphabet:
+        return sigma.value, out[:sigma.value].copy()
+    return sigma.value
+
+
 class BitArray:
     """ecl::BitArray (bitarray/bit_array.cuh)."""
 
@@ -343,6 +373,21 @@ class WaveletTree:
     def save(self, path):
         """Writes the tree (alphabet, symbol counts, level bits) to a file."""
         call("ecl_wavelet_tree_save", self._h, str(path).encode())
+
+    @classmethod
+    def fromFile(cls, path, num_symbols, dtype=np.uint8, alphabet=None, GPU_index=0):
+        """A tree over the first num_symbols elements of a raw text file
+        (ecl_wavelet_tree_create_from_file; no alphabet discovers it)."""
+        self = cls.__new__(cls)
+        self.device = GPU_index
+        self.dtype = np.dtype(dtype)
+        a = (np.zeros(0, self.dtype) if alphabet is None
+             else np.ascontiguousarray(alphabet, dtype=self.dtype))
+        h = C.c_void_p()
+        call("ecl_wavelet_tree_create_from_file", self.dtype.itemsize, str(path).encode(),
+             int(num_symbols), _p(a), len(a), GPU_index, C.byref(h))
+        self._h = h
+        return self
 
     @classmethod
     def load(cls, path, GPU_index=0):
@@ -599,6 +644,16 @@ class Server:
         t = C.c_uint32()
         call("ecl_server_add_tree_file", self._h, str(path).encode(), C.byref(t))
         return t.value
+
+    def addTreeTextFile(self, path, num_symbols):
+        """Admits a tree over the first num_symbols bytes of a raw uint8 text
+        file, converted to its minimal alphabet; returns (tree, alphabet):
+        requests name symbols by their index in that alphabet."""
+        t, sg = C.c_uint32(), C.c_uint64()
+        alpha = np.zeros(256, np.uint8)
+        call("ecl_server_add_tree_text_file", self._h, str(path).encode(), int(num_symbols),
+             _p(alpha), C.byref(sg), C.byref(t))
+        return t.value, alpha[:sg.value].copy()
 
     def removeTree(self, tree):
         call("ecl_server_remove_tree", self._h, tree)

@@ -16,7 +16,8 @@ CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(ROOT, "_build")
 LIB = os.path.join(PKG, "libecl_wavelet_hip.so")
 SOURCES = ["capi.hip", "forest.hip", "server.hip", "kernels_build.hip", "kernels_tree.hip",
-           "kernels_query.hip", "probe.hip", "noalloc.hip"]
+           "kernels_query.hip", "probe.hip", "noalloc.hip",
+           "textio.hip"]
 ARCH = os.environ.get("ECL_OFFLOAD_ARCH", "gfx950")
 
 

@@ -225,7 +225,7 @@ class CopyPool {
     return *pool;
   }
   void copy(void* dst, const void* src, size_t bytes) {
-    constexpr size_t kPiece = 4u << 20;
+    constexpr size_t kPiece = 1u << 20;  // copies of 2 MiB and more are split
     const size_t parts = std::min<size_t>(workers_ + 1, bytes / kPiece);
     if (parts <= 1) {
       std::memcpy(dst, src, bytes);

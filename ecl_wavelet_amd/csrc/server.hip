@@ -34,6 +34,9 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 constexpr uint32_t kMaxSegments = 65536;  // requests per batch
+// smallest batch the coalescing scheduler cuts when it splits the queued
+// work over the free slots (1 MiB of queries)
+constexpr uint64_t kMinBatch = 1ull << 16;
 
 struct Request {
   uint64_t id = 0;
@@ -88,6 +91,7 @@ struct ecl_server_t {
   std::mutex build_mu;         // one admission at a time (memory accounting)
   std::condition_variable cv_sched, cv_done, cv_wait;
   std::deque<Request*> queue;
+  uint64_t queued = 0;         // queries in the queue not yet placed in a batch
   std::vector<Slot*> free_slots;
   std::deque<Slot*> inflight;
   std::unordered_map<uint64_t, std::unique_ptr<Request>> requests;
@@ -119,23 +123,37 @@ void ecl_server_t::scheduler_loop() {
     Slot* sl = free_slots.back();
     free_slots.pop_back();
     sl->parts.clear();
+    // Batch size: with coalescing, the queued work is split over the batch
+    // slots (at least kMinBatch queries a batch, at most max_batch), so that
+    // a lone large request runs as several batches whose copies overlap the
+    // kernels of the others instead of one batch that copies, computes and
+    // copies back in turn; under load the batches grow to max_batch.  The
+    // plain form takes one whole request (up to max_batch) at a time.
+    uint64_t target = cfg.max_batch;
+    if (cfg.coalesce) {
+      const uint64_t share = (queued + cfg.max_inflight - 1) / cfg.max_inflight;
+      target = std::min<uint64_t>(cfg.max_batch, std::max<uint64_t>(kMinBatch, share));
+    }
     uint64_t count = 0;
-    while (!queue.empty() && count < cfg.max_batch && sl->parts.size() < kMaxSegments) {
+    while (!queue.empty() && count < target && sl->parts.size() < kMaxSegments) {
       Request* r = queue.front();
-      const uint64_t c = std::min<uint64_t>(r->m - r->taken, cfg.max_batch - count);
+      const uint64_t c = std::min<uint64_t>(r->m - r->taken, target - count);
       sl->parts.push_back({r, r->taken, count, c});
       r->taken += c;
       count += c;
       if (r->taken == r->m) queue.pop_front();
       if (!cfg.coalesce) break;
     }
+    queued -= count;
     sl->count = count;
     lk.unlock();
     // fill the pinned batch and launch copy-in, kernel, copy-out
     const uint32_t nseg = static_cast<uint32_t>(sl->parts.size());
     for (uint32_t j = 0; j < nseg; ++j) {
       const Part& pt = sl->parts[j];
-      std::memcpy(sl->pin_q + pt.batch_off, pt.r->q + pt.req_off, pt.count * sizeof(ecl_query));
+      // large parts are copied by the library's host copy threads
+      parallel_memcpy(sl->pin_q + pt.batch_off, pt.r->q + pt.req_off,
+                      pt.count * sizeof(ecl_query));
       sl->pin_seg[j] = pt.batch_off;
       sl->pin_tree[j] = pt.r->tree;
     }
@@ -190,7 +208,7 @@ void ecl_server_t::completer_loop() {
     }
     (void)hipEventSynchronize(sl->done);
     for (const Part& pt : sl->parts)
-      std::memcpy(pt.r->out + pt.req_off, sl->pin_out + pt.batch_off, pt.count * 8ull);
+      parallel_memcpy(pt.r->out + pt.req_off, sl->pin_out + pt.batch_off, pt.count * 8ull);
     const Clock::time_point now = Clock::now();
     std::lock_guard<std::mutex> lk(mu);
     for (const Part& pt : sl->parts) {
@@ -242,6 +260,7 @@ ecl_status submit_impl(ecl_server s, uint32_t tree, const ecl_query* q, size_t m
     Request* raw = r.get();
     s->requests.emplace(r->id, std::move(r));
     s->queue.push_back(raw);
+    s->queued += m;
     if (id_out) *id_out = raw->id;
     s->cv_sched.notify_one();
   });

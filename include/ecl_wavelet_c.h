@@ -274,6 +274,25 @@ ecl_status ecl_wavelet_tree_create(ecl_type type, const void* text, size_t n,
 ecl_status ecl_wavelet_tree_create_device(ecl_type type, const void* d_text, size_t n,
                                           const void* alphabet, size_t alphabet_size,
                                           int device, ecl_wavelet_tree* out);
+/* readDataFromFile<T> (utils/test_benchmark_utils.cuh:588-619): the first
+ * num_symbols elements of a raw little-endian file of T into host memory,
+ * with the reference's checks and messages ("Failed to open file: <path>",
+ * "Data size is larger than file size", "File size is not a multiple of the
+ * type size", "Failed to read file: <path>"; ECL_ERROR_INVALID_ARGUMENT). */
+ecl_status ecl_read_text_file(const char* path, ecl_type type, size_t num_symbols, void* out);
+/* convertDataToMinAlphabet<T> (utils/test_benchmark_utils.cuh:628-679): every
+ * element of the host array becomes its index in the sorted distinct
+ * elements (alphabet == NULL) or, by lower bound, in the given sorted
+ * alphabet; *sigma = the alphabet size.  alphabet_out (may be NULL)
+ * receives the sorted alphabet (sigma elements of T). */
+ecl_status ecl_convert_to_min_alphabet(ecl_type type, void* data, size_t n, const void* alphabet,
+                                       size_t alphabet_size, void* alphabet_out, size_t* sigma);
+/* new: WaveletTree over the first num_symbols elements of a raw text file
+ * (read as ecl_read_text_file, then as ecl_wavelet_tree_create; an empty
+ * alphabet discovers it) */
+ecl_status ecl_wavelet_tree_create_from_file(ecl_type type, const char* path, size_t num_symbols,
+                                             const void* alphabet, size_t alphabet_size,
+                                             int device, ecl_wavelet_tree* out);
 /* ~WaveletTree (wavelet_tree.cuh:931-943) */
 ecl_status ecl_wavelet_tree_destroy(ecl_wavelet_tree wt);
 /* Query lines (no counterpart in the reference; an opt-in space-for-time
@@ -458,6 +477,14 @@ ecl_status ecl_server_add_tree(ecl_server s, const void* text, int text_on_devic
  * tree over 0 .. sigma-1), without rebuilding it: the warm start of a
  * server.  Same budget rule and error as ecl_server_add_tree. */
 ecl_status ecl_server_add_tree_file(ecl_server s, const char* path, uint32_t* tree);
+/* Admits a tree over the first num_symbols bytes of a raw uint8 text file,
+ * converted to its minimal alphabet first (ecl_read_text_file, then
+ * ecl_convert_to_min_alphabet): requests name symbols by their index in
+ * that alphabet.  alphabet_out (256 bytes, may be NULL) receives the
+ * alphabet, *sigma its size. */
+ecl_status ecl_server_add_tree_text_file(ecl_server s, const char* path, size_t num_symbols,
+                                         uint8_t* alphabet_out, uint64_t* sigma,
+                                         uint32_t* tree);
 /* No new requests for the tree; its memory is freed once its requests finish. */
 ecl_status ecl_server_remove_tree(ecl_server s, uint32_t tree);
 /* Queues a request; `queries` must stay valid and `results` (m entries)

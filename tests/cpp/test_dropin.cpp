@@ -19,6 +19,7 @@
 
 #include "ecl_wavelet/bitarray/rank_select.cuh"
 #include "ecl_wavelet/tree/wavelet_tree.cuh"
+#include "ecl_wavelet/utils/test_benchmark_utils.cuh"
 
 static int g_fail = 0;
 static int g_checks = 0;
@@ -489,6 +490,45 @@ static void test_utils_type() {
   std::printf("utils<%zu>: ok so far (%d failures)\n", sizeof(T), g_fail);
 }
 
+// readDataFromFile + convertDataToMinAlphabet (test_benchmark_utils.cuh:
+// 588-679) feeding a tree: a raw u16 file over a sparse alphabet, a prefix
+// of it converted to the minimal alphabet and built with the alphabet
+// 0 .. sigma-1 answers like the direct scan of the converted text.
+static void test_text_file() {
+  char path[] = "/tmp/ecl_dropin_textXXXXXX";
+  const int fd = mkstemp(path);
+  EXPECT_EQ(fd >= 0, true);
+  std::mt19937_64 gen(5);
+  std::vector<uint16_t> raw(200000);
+  for (auto& x : raw) x = static_cast<uint16_t>(1000 + 7 * (gen() % 50));
+  EXPECT_EQ(write(fd, raw.data(), raw.size() * 2), static_cast<ssize_t>(raw.size() * 2));
+  close(fd);
+  auto data = utils::readDataFromFile<uint16_t>(path, 150000);
+  EXPECT_EQ(data.size(), size_t(150000));
+  EXPECT_EQ(data[149999], raw[149999]);
+  std::string msg;
+  try {
+    (void)utils::readDataFromFile<uint16_t>(path, 200001);
+  } catch (std::runtime_error const& e) {
+    msg = e.what();
+  }
+  EXPECT_EQ(msg == "Data size is larger than file size", true);
+  const size_t sigma = utils::convertDataToMinAlphabet(data.data(), data.size());
+  EXPECT_EQ(sigma, size_t(50));
+  std::vector<uint16_t> alphabet(sigma);
+  std::iota(alphabet.begin(), alphabet.end(), 0);
+  WaveletTree<uint16_t> wt(data.data(), data.size(), std::move(alphabet), 0);
+  std::vector<RankSelectQuery<uint16_t>> q(500);
+  for (auto& x : q) x = {gen() % data.size(), static_cast<uint16_t>(gen() % sigma)};
+  std::vector<size_t> exp(q.size());
+  for (size_t i = 0; i < q.size(); ++i)
+    exp[i] = std::count(data.begin(), data.begin() + q[i].index_, q[i].symbol_);
+  auto r = wt.rank(q.data(), q.size());
+  for (size_t i = 0; i < q.size(); ++i) EXPECT_EQ(r[i], exp[i]);
+  unlink(path);
+  std::printf("text file: ok so far (%d failures)\n", g_fail);
+}
+
 int main() {
   gpuErrchk(hipMallocManaged(&g_res, sizeof(size_t)));
   try {
@@ -506,6 +546,7 @@ int main() {
     test_oom();
     example_dna();
     test_save_load();
+    test_text_file();
   } catch (std::exception const& e) {
     std::printf("EXCEPTION: %s\n", e.what());
     return 2;

@@ -179,3 +179,24 @@ def test_server_query_lines(gpu, tmp_path):
         out, _ = srv.wait(rid)
         assert np.array_equal(out, e)
     srv.close()
+
+
+def test_server_tree_from_raw_text_file(gpu, tmp_path):
+    """A raw uint8 text file over a sparse alphabet is admitted after its
+    conversion to the minimal alphabet (convertDataToMinAlphabet,
+    test_benchmark_utils.cuh:628-679); requests over the minimal symbols
+    answer like the direct scans of the converted text."""
+    from ecl_wavelet_amd import Server
+    rng = np.random.default_rng(21)
+    symbols = np.array([3, 50, 51, 200, 255], np.uint8)
+    raw = symbols[rng.integers(0, 5, 300_000)]
+    path = tmp_path / "raw.u8"
+    raw.tofile(path)
+    srv = Server(max_inflight=2)
+    tree, alpha = srv.addTreeTextFile(path, 250_000)
+    assert np.array_equal(alpha, np.unique(raw[:250_000]))
+    text = np.searchsorted(alpha, raw[:250_000]).astype(np.uint8)
+    for q, e in _requests(text, len(alpha), [1, 700, 5000], seed=3):
+        out, _ = srv.wait(srv.submit(tree, q))
+        assert np.array_equal(out, e)
+    srv.close()

@@ -671,3 +671,28 @@ def test_mixed_batch_query_lines(gpu, m):
                 wt.buildQueryLines()
             wt.queryBatchDevice(d_q, m, d_out)
             assert np.array_equal(d_out.download(np.uint64, m), exp), (mix, lines)
+
+
+def test_tree_from_raw_text_file(gpu, tmp_path):
+    """A tree over a prefix of a raw text file (readDataFromFile,
+    test_benchmark_utils.cuh:588-619) with its alphabet discovered equals the
+    tree over the same text given in memory, and answers every query like
+    the direct scans; a file that is too short raises the reference's
+    message."""
+    from ecl_wavelet_amd import WaveletTree
+    from ecl_wavelet_amd._lib import EclError
+    rng = np.random.default_rng(17)
+    alpha = np.sort(rng.choice(60000, 300, replace=False)).astype(np.uint16)
+    full = alpha[rng.integers(0, 300, 400_000)]
+    path = tmp_path / "text.u16"
+    full.tofile(path)
+    n = 399_990
+    wt = WaveletTree.fromFile(path, n, np.uint16)
+    ref = WaveletTree(full[:n], np.unique(full[:n]))
+    assert np.array_equal(wt.getAlphabet(), ref.getAlphabet())
+    for l in range(wt.getNumLevels()):
+        assert np.array_equal(wt.rank_select_.bit_array_.words(l),
+                              ref.rank_select_.bit_array_.words(l))
+    _check_queries(wt, full[:n], None, nq=1000, seed=5)
+    with pytest.raises(EclError, match="Data size is larger than file size"):
+        WaveletTree.fromFile(path, 400_001, np.uint16)
