@@ -258,6 +258,20 @@ def bench_batched(args):
     timer.start()
     forest = E.Forest(buf, starts, lens, sig)
     forest_build_ms = timer.stop()
+    # the construction as one graph (tile size autotuned), replayed
+    tile_rows, tuned_ms = forest.autotuneBuild()
+    gb = forest.buildGraph()
+    gb.launch()
+    E.synchronize()
+    runs = []
+    for _ in range(3):
+        timer.start()
+        gb.launch()
+        runs.append(timer.stop())
+    build_graph_ms = float(np.median(runs))
+    del gb
+    log(f"batched: build {forest_build_ms:.1f} ms (create), {build_graph_ms:.1f} ms as one graph "
+        f"(tile rows {tile_rows})")
     # ---- batched queries
     per = np.maximum(1, np.round(args.bt_queries * lens.astype(np.float64) / total)).astype(np.uint64)
     qoff = np.concatenate([[0], np.cumsum(per)]).astype(np.uint64)
@@ -344,8 +358,11 @@ def bench_batched(args):
                    f"sigma in {{4, 20, 64, 256}}, uniform or Zipf(1.1); {m} mixed queries "
                    f"(per-text count proportional to length)"),
         "batched_build_ms": forest_build_ms,
+        "batched_build_graph_ms": build_graph_ms,
+        "build_tile_rows": tile_rows,
         "plain_build_ms": plain_build_ms,
         "build_speedup": plain_build_ms / forest_build_ms,
+        "build_graph_speedup": plain_build_ms / build_graph_ms,
         "batched_qps": m / (batched_ms / 1e3),
         "plain_qps": m / (plain_ms / 1e3),
         "query_speedup": plain_ms / batched_ms,
@@ -355,7 +372,7 @@ def bench_batched(args):
         "batched_launch_ms_query_lines": batched_lines_ms,
         "plain_per_tree_p50_ms": float(np.percentile(lat, 50)),
         "plain_per_tree_p99_ms": float(np.percentile(lat, 99)),
-        "autotuned": {"block": block, "grid": grid},
+        "autotuned": {"block": block, "grid": grid, "build_tile_rows": tile_rows},
     }
 
 

@@ -147,6 +147,8 @@ _SIGS = {
     "ecl_wavelet_tree_generate_queries": (st, [vp, i32, u64, sz, vp, vp]),
     "ecl_forest_create": (st, [vp, vp, vp, vp, sz, i32, u32, C.POINTER(vp)]),
     "ecl_forest_destroy": (st, [vp]),
+    "ecl_forest_build_graph": (st, [vp, C.POINTER(vp)]),
+    "ecl_forest_autotune_build": (st, [vp, C.POINTER(u32), C.POINTER(C.c_float)]),
     "ecl_forest_build_query_lines": (st, [vp]),
     "ecl_forest_num_trees": (st, [vp, C.POINTER(sz)]),
     "ecl_forest_tree_info": (st, [vp, sz, C.POINTER(u64), C.POINTER(u64), C.POINTER(u32)]),

@@ -558,13 +558,29 @@ class Forest:
     def graph(self, d_q, m, d_out):
         return Graph(self, d_q, m, d_out)
 
+    def buildGraph(self):
+        """The construction as one HIP graph: launch() rebuilds every tree
+        from the current contents of the texts (ecl_forest_build_graph)."""
+        return Graph(self, build=True)
+
+    def autotuneBuild(self):
+        """Times the build with each tile size, keeps the fastest for later
+        build graphs; returns (tile_rows, ms)."""
+        r, t = C.c_uint32(), C.c_float()
+        call("ecl_forest_autotune_build", self._h, C.byref(r), C.byref(t))
+        return r.value, t.value
+
 
 class Graph:
-    """A forest batch launch captured into a HIP graph (ecl_forest_graph_create)."""
+    """A forest batch launch (ecl_forest_graph_create), or with build=True the
+    forest's construction (ecl_forest_build_graph), captured into a HIP graph."""
 
-    def __init__(self, forest, d_q, m, d_out):
+    def __init__(self, forest, d_q=None, m=0, d_out=None, build=False):
         h = C.c_void_p()
-        call("ecl_forest_graph_create", forest._h, d_q.ptr, m, d_out.ptr, C.byref(h))
+        if build:
+            call("ecl_forest_build_graph", forest._h, C.byref(h))
+        else:
+            call("ecl_forest_graph_create", forest._h, d_q.ptr, m, d_out.ptr, C.byref(h))
         self._h = h
         self._keep = (forest, d_q, d_out)
 

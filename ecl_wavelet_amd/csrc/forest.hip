@@ -25,17 +25,22 @@ using detail::TreeView;
 namespace {
 
 constexpr uint64_t kAlign = 256;
+constexpr uint32_t kRow = 256;  // digits per text: 8-bit codes, one pass
 uint64_t align_up(uint64_t x, uint64_t a) { return (x + a - 1) / a * a; }
 
-// One tree of the forest: its shape, metadata and where its arrays live in
-// the forest's arena (byte offsets).
+// One tree of the forest: where its arrays live in the forest's arena (byte
+// offsets).  Every level gets the capacity of a full level (n bits and the
+// directory of n bits): the level sizes of a non-power-of-two alphabet
+// depend on the symbol counts, which only the device knows during the build,
+// and a layout that does not depend on them lets the whole build run
+// without the host waiting in between.
 struct TreeSlot {
   uint64_t n = 0, sigma = 0;
   uint32_t L = 0;
   const TreeShape* shape = nullptr;
-  TreeMeta meta;
-  std::vector<uint64_t> level_size, level_word, level_nwords;  // bits, word offset, words
-  std::vector<uint64_t> num_l1, num_l2, cap, l1_at, l2_at, smp_at;
+  uint32_t shape_id = 0, num_ranks = 0;
+  std::vector<uint64_t> level_word;                 // word offset of each level
+  std::vector<uint64_t> l1_at, l2_at, smp_at;      // directory byte offsets
   uint64_t ones_at = 0, lv_at = 0, counts_at = 0, ranks_at = 0, nal_at = 0, codes_at = 0,
            lens_at = 0;
 };
@@ -48,12 +53,128 @@ int cu_count(int device) {
   return cus;
 }
 
+// ---- device-side metadata ---------------------------------------------------
+// One distinct alphabet size: its minimal codes (as 8-bit digits) and code
+// lengths from code_off, its internal nodes in (level, symbol) order from
+// node_off (lo, hi and level each).
+struct ShapeDev {
+  uint32_t sigma, L, nodes, code_off, node_off;
+};
+struct MetaJob {
+  uint64_t n;
+  uint64_t counts_at, ranks_at;  // arena byte offsets of the C array and node ranks
+  uint32_t shape, desc0;         // shape index; first directory descriptor (L of them)
+};
+
+__device__ __forceinline__ uint64_t rs_num_l2(uint64_t m, uint64_t num_l1) {
+  uint64_t last = ((m % detail::kL1Bits) + detail::kL2Bits - 1) / detail::kL2Bits;
+  if (last == 0 && m) last = detail::kL2PerL1;
+  return m ? (num_l1 - 1) * detail::kL2PerL1 + last : 0;
+}
+
+// One thread per tree: from the digit-count rows at the text boundaries,
+// the tree's digit starts (pc, for the pass), its C array and node ranks
+// (wavelet_tree.cuh:773-774, 332-352: the zeros of level l before each node
+// start, in (level, symbol) order) and the sizes of its levels
+// (wavelet_tree.cuh:776-790) in its directory descriptors.
+__global__ void forest_meta_kernel(const uint64_t* __restrict__ rows, const MetaJob* __restrict__ jobs,
+                                   uint32_t N, const ShapeDev* __restrict__ shapes,
+                                   const uint8_t* __restrict__ sh_code,
+                                   const uint8_t* __restrict__ sh_len,
+                                   const uint16_t* __restrict__ sh_node,
+                                   const uint8_t* __restrict__ sh_node_level,
+                                   uint8_t* __restrict__ arena, uint64_t* __restrict__ pc,
+                                   RsArrayDesc* __restrict__ desc) {
+  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= N) return;
+  const MetaJob J = jobs[i];
+  const ShapeDev S = shapes[J.shape];
+  const uint64_t* r0 = rows + static_cast<uint64_t>(i) * kRow;
+  const uint64_t* r1 = r0 + kRow;
+  uint64_t* P = pc + static_cast<uint64_t>(i) * (kRow + 1);
+  uint64_t run = 0;
+  for (uint32_t d = 0; d < kRow; ++d) {
+    P[d] = run;
+    run += r1[d] - r0[d];
+  }
+  P[kRow] = run;
+  uint64_t* C = reinterpret_cast<uint64_t*>(arena + J.counts_at);
+  uint64_t m[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  uint64_t acc = 0;
+  for (uint32_t c = 0; c < S.sigma; ++c) {
+    const uint8_t code = sh_code[S.code_off + c];
+    const uint64_t cnt = r1[code] - r0[code];
+    C[c] = acc;
+    acc += cnt;
+    const uint32_t len = sh_len[S.code_off + c];
+    for (uint32_t l = 0; l < 8; ++l)
+      if (l < len) m[l] += cnt;
+  }
+  uint64_t* R = reinterpret_cast<uint64_t*>(arena + J.ranks_at);
+  uint32_t k = 0, cur = 0xFFFFFFFFu;
+  uint64_t zb = 0;
+  for (uint32_t j = 0; j < S.nodes; ++j) {
+    const uint32_t lo = sh_node[2 * (S.node_off + j)], hi = sh_node[2 * (S.node_off + j) + 1];
+    const uint32_t l = sh_node_level[S.node_off + j];
+    if (l != cur) {
+      cur = l;
+      zb = 0;
+    }
+    const uint32_t mid = lo + static_cast<uint32_t>(detail::prev_pow2(hi - lo + 1));
+    if (l >= 1 && lo >= 1) R[k++] = zb;
+    const uint64_t cmid = mid < S.sigma ? C[mid] : J.n;
+    zb += cmid - C[lo];
+  }
+  for (uint32_t l = 0; l < S.L; ++l) {
+    RsArrayDesc& d = desc[J.desc0 + l];
+    d.m = m[l];
+    d.num_l1 = (m[l] + detail::kL1Bits - 1) / detail::kL1Bits;
+    d.num_l2 = rs_num_l2(m[l], d.num_l1);
+    d.cap = m[l] / detail::kSampleRate;
+  }
+}
+
+// One thread per level of every tree: its view, from its directory
+// descriptor and the total the directory build wrote.  Query lines are not
+// part of a build: a rebuilt forest answers over its directory until they
+// are built again.
+__global__ void forest_views_kernel(const RsArrayDesc* __restrict__ desc, uint32_t ndesc,
+                                    LevelView* const* __restrict__ lv_ptr) {
+  const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= ndesc) return;
+  const RsArrayDesc d = desc[j];
+  LevelView v{};
+  v.words = reinterpret_cast<const uint32_t*>(d.words64);
+  v.l1 = d.l1;
+  v.l2 = d.l2;
+  v.samples = d.samples;
+  v.size = d.m;
+  v.ones = *d.total_ones;
+  v.num_l1 = d.num_l1;
+  v.sample_cap = d.cap;
+  *lv_ptr[j] = v;
+}
+
 }  // namespace
+
+// The device buffers and launch parameters of one forest build: everything
+// the launch sequence reads besides the texts and the arena.  Kept by the
+// forest while a build graph (ecl_forest_build_graph) may replay it.
+struct ForestBuild {
+  uint32_t rows = 1;
+  uint64_t ntiles = 0, blocks = 0, meta_at = 0;
+  uint32_t N = 0, ndesc = 0;
+  DevBuf d_x, d_tstart, d_tlen, d_tnode, d_tfirst, d_remap, d_lut, d_hist, d_excl, d_partial,
+      d_err, d_rows, d_pc, d_thr, d_nk, d_lw, d_lnw, d_jobs, d_shapes, d_code, d_len, d_node,
+      d_node_level, d_desc, d_blk, d_lvptr;
+  PassParams pp{};
+};
 
 struct ecl_graph_t {
   int device = 0;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
+  std::shared_ptr<ForestBuild> keep;  // a build graph's buffers
   ~ecl_graph_t() {
     if (exec) (void)hipGraphExecDestroy(exec);
     if (graph) (void)hipGraphDestroy(graph);
@@ -62,16 +183,274 @@ struct ecl_graph_t {
 
 struct ecl_forest_t {
   int device = 0;
+  const void* d_texts = nullptr;
+  std::vector<uint64_t> starts, lengths, sigmas;
   std::map<uint64_t, TreeShape> shapes;
   std::vector<TreeSlot> trees;
   DevBuf arena;                  // bits, directories and metadata of every tree
   DevBuf d_views;                // TreeView per tree
   std::vector<TreeView> views;
   uint32_t q_block = 256, q_grid = 0;  // launch shape of the batch query kernel
-  std::vector<LevelView> levels;       // every tree's level views, tree by tree
-  std::vector<LevelView*> d_levels;    // where each of them lives in the arena
+  uint32_t tile_rows = 0;              // construction tile (0: by padding)
+  std::vector<LevelView*> d_levels;    // where each level's view lives in the arena
   DevBuf lines;                        // query lines and select hints (opt-in)
+  std::shared_ptr<ForestBuild> build;  // buffers of a build graph, while one lives
 };
+
+namespace {
+
+// Tiles of `rows` 16-byte chunks per thread (0: the largest whose padding
+// stays within 1/4 of the text).
+uint32_t choose_rows(const ecl_forest_t* f, uint32_t rows) {
+  if (rows != 0) return rows;
+  uint64_t total = 0;
+  for (uint64_t n : f->lengths) total += n;
+  for (uint32_t r : {4u, 2u}) {
+    const uint64_t T = pass_tile_codes(1, r);
+    uint64_t padded = 0;
+    for (uint64_t n : f->lengths) padded += (n + T - 1) / T * T;
+    if (padded * 4 <= total * 5) return r;
+  }
+  return 1;
+}
+
+// The device buffers and tables of a build of forest f with construction
+// tiles of `rows` chunks per thread (the arena must exist).
+std::shared_ptr<ForestBuild> make_build(ecl_forest_t* f, uint32_t rows) {
+  auto B = std::make_shared<ForestBuild>();
+  const size_t N = f->trees.size();
+  B->N = static_cast<uint32_t>(N);
+  B->rows = rows;
+  const uint64_t T = pass_tile_codes(1, rows);
+  std::vector<uint64_t> tstart;
+  std::vector<uint32_t> tlen, tnode, tfirst(N + 1);
+  uint64_t span = 0;
+  for (size_t i = 0; i < N; ++i) {
+    tfirst[i] = static_cast<uint32_t>(tstart.size());
+    const uint64_t n = f->lengths[i];
+    span = std::max(span, f->starts[i] + n);
+    for (uint64_t o = 0; o < n; o += T) {
+      tstart.push_back(f->starts[i] + o);
+      tlen.push_back(static_cast<uint32_t>(std::min<uint64_t>(T, n - o)));
+      tnode.push_back(static_cast<uint32_t>(i));
+    }
+  }
+  B->ntiles = tstart.size();
+  tfirst[N] = static_cast<uint32_t>(B->ntiles);
+  require(B->ntiles < (1ull << 31), "too many construction tiles");
+  // non-power-of-2 alphabets read mapped codes (same positions)
+  bool any_remap = false;
+  std::vector<uint8_t> remap(N, 0), lut(N * 256, 0);
+  for (size_t i = 0; i < N; ++i) {
+    const TreeShape& sh = *f->trees[i].shape;
+    if (sh.pow2) continue;
+    remap[i] = 1;
+    any_remap = true;
+    for (uint32_t x = 0; x < 256; ++x)
+      lut[i * 256 + x] = static_cast<uint8_t>(sh.code[std::min<uint64_t>(x, sh.sigma - 1)]);
+  }
+  B->d_x = DevBuf(any_remap ? span : 0);
+  B->d_tstart = DevBuf(B->ntiles * 8);
+  B->d_tlen = DevBuf(B->ntiles * 4);
+  B->d_tnode = DevBuf(B->ntiles * 4);
+  B->d_tfirst = DevBuf((N + 1) * 4);
+  B->d_remap = DevBuf(N);
+  B->d_lut = DevBuf(N * 256);
+  h2d(B->d_tstart.ptr, tstart.data(), B->ntiles);
+  h2d(B->d_tlen.ptr, tlen.data(), B->ntiles);
+  h2d(B->d_tnode.ptr, tnode.data(), B->ntiles);
+  h2d(B->d_tfirst.ptr, tfirst.data(), N + 1);
+  h2d(B->d_remap.ptr, remap.data(), N);
+  h2d(B->d_lut.ptr, lut.data(), N * 256);
+  B->d_hist = DevBuf(B->ntiles * kRow * 4);
+  B->d_excl = DevBuf((B->ntiles + 1) * kRow * 8);
+  B->d_partial = DevBuf(scan_partial_entries(B->ntiles, 8) * 8);
+  B->d_err = DevBuf(4);
+  B->d_rows = DevBuf((N + 1) * kRow * 8);
+  B->d_pc = DevBuf(N * (kRow + 1) * 8);
+  // ---- per-text pass tables that do not depend on the counts
+  std::vector<uint64_t> lw(N * 8, 0), lnw(N * 8, 0);
+  std::vector<uint16_t> thr(N * 9, 0);
+  std::vector<uint8_t> nk(N);
+  for (size_t i = 0; i < N; ++i) {
+    const TreeSlot& t = f->trees[i];
+    const TreeShape& sh = *t.shape;
+    const uint32_t L = t.L, nb = 1u << L;
+    nk[i] = static_cast<uint8_t>(L);
+    for (uint32_t l = 0; l < L; ++l) {
+      lw[i * 8 + l] = t.level_word[l];
+      lnw[i * 8 + l] = (t.n + 31) / 32;  // the level's capacity (bounds checks only)
+    }
+    if (sh.pow2) {
+      for (uint32_t x = 0; x <= L; ++x) thr[i * 9 + x] = static_cast<uint16_t>(nb);
+    } else {
+      // first absent code per level, aligned to the level's groups
+      int32_t maxd[9];
+      for (uint32_t x = 0; x <= L; ++x) maxd[x] = -1;
+      for (uint64_t c = 0; c < sh.sigma; ++c)
+        for (uint32_t x = 0; x <= L; ++x)
+          if (sh.len[c] > x) maxd[x] = std::max<int32_t>(maxd[x], static_cast<int32_t>(sh.code[c]));
+      for (uint32_t x = 0; x <= L; ++x)
+        thr[i * 9 + x] = maxd[x] < 0 ? 0
+                                     : static_cast<uint16_t>(
+                                           ((static_cast<uint32_t>(maxd[x]) >> (L - x)) + 1)
+                                           << (L - x));
+    }
+  }
+  B->d_thr = DevBuf(thr.size() * 2);
+  B->d_nk = DevBuf(N);
+  B->d_lw = DevBuf(N * 64);
+  B->d_lnw = DevBuf(N * 64);
+  h2d(B->d_thr.ptr, thr.data(), thr.size());
+  h2d(B->d_nk.ptr, nk.data(), N);
+  h2d(B->d_lw.ptr, lw.data(), lw.size());
+  h2d(B->d_lnw.ptr, lnw.data(), lnw.size());
+  // ---- shapes, metadata jobs, directory descriptors (sizes filled on the
+  // device), view slots
+  std::vector<ShapeDev> shp;
+  std::vector<uint8_t> codes8, lens8, node_level;
+  std::vector<uint16_t> nodes;
+  std::map<uint64_t, uint32_t> shape_index;
+  for (auto& kv : f->shapes) {
+    const TreeShape& sh = kv.second;
+    ShapeDev d{};
+    d.sigma = static_cast<uint32_t>(sh.sigma);
+    d.L = sh.L;
+    d.code_off = static_cast<uint32_t>(codes8.size());
+    d.node_off = static_cast<uint32_t>(node_level.size());
+    for (uint64_t c = 0; c < sh.sigma; ++c) {
+      codes8.push_back(static_cast<uint8_t>(sh.code[c]));
+      lens8.push_back(sh.len[c]);
+    }
+    for (uint32_t l = 0; l < sh.L; ++l)
+      for (const Node& nd : sh.nodes[l]) {
+        nodes.push_back(static_cast<uint16_t>(nd.lo));
+        nodes.push_back(static_cast<uint16_t>(nd.hi));
+        node_level.push_back(static_cast<uint8_t>(l));
+      }
+    d.nodes = static_cast<uint32_t>(node_level.size()) - d.node_off;
+    shape_index[kv.first] = static_cast<uint32_t>(shp.size());
+    shp.push_back(d);
+  }
+  uint8_t* A = f->arena.as<uint8_t>();
+  std::vector<MetaJob> jobs(N);
+  std::vector<RsArrayDesc> desc;
+  std::vector<LevelView*> lvptr;
+  uint64_t blocks = 0;
+  for (size_t i = 0; i < N; ++i) {
+    const TreeSlot& t = f->trees[i];
+    MetaJob& J = jobs[i];
+    J.n = t.n;
+    J.counts_at = t.counts_at;
+    J.ranks_at = t.ranks_at;
+    J.shape = shape_index.at(t.sigma);
+    J.desc0 = static_cast<uint32_t>(desc.size());
+    const uint64_t cap_l1 = (t.n + detail::kL1Bits - 1) / detail::kL1Bits;
+    for (uint32_t l = 0; l < t.L; ++l) {
+      RsArrayDesc d{};
+      d.words64 = reinterpret_cast<const uint64_t*>(A + t.level_word[l] * 4);
+      d.l1 = reinterpret_cast<uint64_t*>(A + t.l1_at[l]);
+      d.l2 = reinterpret_cast<uint16_t*>(A + t.l2_at[l]);
+      d.samples = reinterpret_cast<uint64_t*>(A + t.smp_at[l]);
+      d.total_ones = reinterpret_cast<uint64_t*>(A + t.ones_at) + l;
+      d.first_tile = blocks;
+      d.cap_l1 = cap_l1;
+      blocks += cap_l1;
+      desc.push_back(d);
+      lvptr.push_back(reinterpret_cast<LevelView*>(A + t.lv_at) + l);
+    }
+  }
+  B->blocks = blocks;
+  B->ndesc = static_cast<uint32_t>(desc.size());
+  B->d_jobs = DevBuf(N * sizeof(MetaJob));
+  B->d_shapes = DevBuf(shp.size() * sizeof(ShapeDev));
+  B->d_code = DevBuf(codes8.size());
+  B->d_len = DevBuf(lens8.size());
+  B->d_node = DevBuf(std::max<size_t>(nodes.size(), 1) * 2);
+  B->d_node_level = DevBuf(std::max<size_t>(node_level.size(), 1));
+  B->d_desc = DevBuf(desc.size() * sizeof(RsArrayDesc));
+  B->d_blk = DevBuf(std::max<uint64_t>(blocks, 1) * 8);
+  B->d_lvptr = DevBuf(lvptr.size() * sizeof(LevelView*));
+  h2d(B->d_jobs.ptr, jobs.data(), N);
+  h2d(B->d_shapes.ptr, shp.data(), shp.size());
+  h2d(B->d_code.ptr, codes8.data(), codes8.size());
+  h2d(B->d_len.ptr, lens8.data(), lens8.size());
+  h2d(B->d_node.ptr, nodes.data(), nodes.size());
+  h2d(B->d_node_level.ptr, node_level.data(), node_level.size());
+  h2d(B->d_desc.ptr, desc.data(), desc.size());
+  h2d(B->d_lvptr.ptr, lvptr.data(), lvptr.size());
+  // ---- pass parameters
+  PassParams& pp = B->pp;
+  pp = PassParams{};
+  pp.src = f->d_texts;
+  pp.src_esize = 1;
+  pp.codes_out = any_remap ? B->d_x.ptr : nullptr;
+  pp.map_kind = 0;
+  pp.sigma = 256;
+  pp.pow2 = 1;
+  pp.codes = f->d_texts;
+  pp.cw = 1;
+  pp.k = 8;
+  pp.dshift = 0;
+  pp.tile_rows = rows;
+  pp.tile_codes = static_cast<uint32_t>(T);
+  pp.num_tiles = B->ntiles;
+  pp.tile_start = B->d_tstart.as<uint64_t>();
+  pp.tile_len = B->d_tlen.as<uint32_t>();
+  pp.tile_node = B->d_tnode.as<uint32_t>();
+  pp.node_first_tile = B->d_tfirst.as<uint32_t>();
+  pp.hist = B->d_hist.as<uint32_t>();
+  pp.excl = B->d_excl.as<uint64_t>();
+  pp.partial = B->d_partial.as<uint64_t>();
+  pp.err = B->d_err.as<uint32_t>();
+  pp.node_remap = any_remap ? B->d_remap.as<uint8_t>() : nullptr;
+  pp.node_lut = B->d_lut.as<uint8_t>();
+  pp.row_bins = kRow;
+  pp.thr_stride = 9;
+  pp.pc = B->d_pc.as<uint64_t>();
+  pp.thr = B->d_thr.as<uint16_t>();
+  pp.node_k = B->d_nk.as<uint8_t>();
+  pp.node_lw = B->d_lw.as<uint64_t>();
+  pp.node_lnw = B->d_lnw.as<uint64_t>();
+  pp.words = f->arena.as<uint32_t>();
+  pp.dst = nullptr;
+  B->meta_at = f->trees.empty() ? 0 : f->trees.front().ones_at;
+  return B;
+}
+
+// The build's launches, queued on s with no host wait in between (and so
+// capturable into one graph): clear the bits and directories, tile digit
+// histograms and their scan, the digit counts of every text, the metadata
+// kernel, the level pass, the directories of every level, the level views.
+void enqueue_build(ecl_forest_t* f, ForestBuild& B, hipStream_t s) {
+  launch_fill_u32(B.d_err.as<uint32_t>(), 1, 0, s);
+  launch_fill_u64(f->arena.as<uint64_t>(), B.meta_at / 8, 0, s);
+  launch_tile_hist(B.pp, s);
+  launch_tile_scan(B.pp, s);
+  launch_gather_rows(B.d_excl.as<uint64_t>(), B.d_tfirst.as<uint32_t>(), B.N + 1, kRow,
+                     B.d_rows.as<uint64_t>(), s);
+  hipLaunchKernelGGL(forest_meta_kernel, dim3((B.N + 127) / 128), dim3(128), 0, s,
+                     B.d_rows.as<uint64_t>(), B.d_jobs.as<MetaJob>(), B.N,
+                     B.d_shapes.as<ShapeDev>(), B.d_code.as<uint8_t>(), B.d_len.as<uint8_t>(),
+                     B.d_node.as<uint16_t>(), B.d_node_level.as<uint8_t>(),
+                     f->arena.as<uint8_t>(), B.d_pc.as<uint64_t>(), B.d_desc.as<RsArrayDesc>());
+  ECL_HIP(hipGetLastError());
+  launch_pass(B.pp, s);
+  launch_rs_build(B.d_desc.as<RsArrayDesc>(), B.ndesc, B.blocks, B.d_blk.as<uint32_t>(), s);
+  hipLaunchKernelGGL(forest_views_kernel, dim3((B.ndesc + 255) / 256), dim3(256), 0, s,
+                     B.d_desc.as<RsArrayDesc>(), B.ndesc, B.d_lvptr.as<LevelView*>());
+  ECL_HIP(hipGetLastError());
+}
+
+// Waits for a build on s and checks the pass's bounds flag.
+void finish_build(ForestBuild& B, hipStream_t s) {
+  ECL_HIP(hipStreamSynchronize(s));
+  uint32_t err = 0;
+  d2h(&err, B.d_err.ptr, 1);
+  if (err) throw Error(ECL_ERROR_INTERNAL, "construction pass wrote outside its arrays");
+}
+
+}  // namespace
 
 extern "C" {
 
@@ -90,110 +469,18 @@ ecl_status ecl_forest_create(const void* d_texts, const uint64_t* starts, const 
     const size_t N = num_texts;
     auto f = std::make_unique<ecl_forest_t>();
     f->device = device;
-    uint64_t total = 0;
-    uint64_t span = 0;  // bytes of d_texts the texts reach
+    f->d_texts = d_texts;
+    f->starts.assign(starts, starts + N);
+    f->lengths.assign(lengths, lengths + N);
+    f->sigmas.assign(sigmas, sigmas + N);
     for (size_t i = 0; i < N; ++i) {
       require(starts[i] % 16 == 0, "every text must start on a 16-byte boundary");
       require(lengths[i] > 0, "every text must be non-empty");
-      span = std::max(span, starts[i] + lengths[i]);
       require(sigmas[i] >= 2 && sigmas[i] <= 256, "alphabet sizes must be in [2, 256]");
-      total += lengths[i];
       if (!f->shapes.count(sigmas[i])) f->shapes.emplace(sigmas[i], tree_shape(sigmas[i]));
     }
-    // ---- tiles: the largest tile whose padding stays within 1/4 of the text
-    uint32_t rows = tile_rows;
-    if (rows == 0) {
-      rows = 1;
-      for (uint32_t r : {4u, 2u}) {
-        const uint64_t T = pass_tile_codes(1, r);
-        uint64_t padded = 0;
-        for (size_t i = 0; i < N; ++i) padded += (lengths[i] + T - 1) / T * T;
-        if (padded * 4 <= total * 5) {
-          rows = r;
-          break;
-        }
-      }
-    }
-    const uint64_t T = pass_tile_codes(1, rows);
-    std::vector<uint64_t> tstart;
-    std::vector<uint32_t> tlen, tnode, tfirst(N + 1);
-    for (size_t i = 0; i < N; ++i) {
-      tfirst[i] = static_cast<uint32_t>(tstart.size());
-      const uint64_t n = lengths[i];
-      for (uint64_t o = 0; o < n; o += T) {
-        tstart.push_back(starts[i] + o);
-        tlen.push_back(static_cast<uint32_t>(std::min<uint64_t>(T, n - o)));
-        tnode.push_back(static_cast<uint32_t>(i));
-      }
-    }
-    const uint64_t ntiles = tstart.size();
-    tfirst[N] = static_cast<uint32_t>(ntiles);
-    require(ntiles < (1ull << 31), "too many construction tiles");
-    // ---- non-power-of-2 alphabets read mapped codes (same positions)
-    bool any_remap = false;
-    std::vector<uint8_t> remap(N, 0), lut(N * 256, 0);
-    for (size_t i = 0; i < N; ++i) {
-      const TreeShape& sh = f->shapes.at(sigmas[i]);
-      if (sh.pow2) continue;
-      remap[i] = 1;
-      any_remap = true;
-      for (uint32_t x = 0; x < 256; ++x)
-        lut[i * 256 + x] = static_cast<uint8_t>(sh.code[std::min<uint64_t>(x, sh.sigma - 1)]);
-    }
-    DevBuf d_x(any_remap ? span : 0);
-    DevBuf d_tstart(ntiles * 8), d_tlen(ntiles * 4), d_tnode(ntiles * 4), d_tfirst((N + 1) * 4);
-    DevBuf d_remap(N), d_lut(N * 256);
-    h2d(d_tstart.ptr, tstart.data(), ntiles);
-    h2d(d_tlen.ptr, tlen.data(), ntiles);
-    h2d(d_tnode.ptr, tnode.data(), ntiles);
-    h2d(d_tfirst.ptr, tfirst.data(), N + 1);
-    h2d(d_remap.ptr, remap.data(), N);
-    h2d(d_lut.ptr, lut.data(), N * 256);
-    constexpr uint32_t kRow = 256;
-    DevBuf d_hist(ntiles * kRow * 4), d_excl((ntiles + 1) * kRow * 8);
-    DevBuf d_partial(scan_partial_entries(ntiles, 8) * 8);
-    DevBuf d_err(4);
-    launch_fill_u32(d_err.as<uint32_t>(), 1, 0, s);
-
-    PassParams pp{};
-    pp.src = d_texts;
-    pp.src_esize = 1;
-    pp.codes_out = any_remap ? d_x.ptr : nullptr;
-    pp.map_kind = 0;
-    pp.sigma = 256;
-    pp.pow2 = 1;
-    pp.codes = d_texts;
-    pp.cw = 1;
-    pp.k = 8;
-    pp.dshift = 0;
-    pp.tile_rows = rows;
-    pp.tile_codes = static_cast<uint32_t>(T);
-    pp.num_tiles = ntiles;
-    pp.tile_start = d_tstart.as<uint64_t>();
-    pp.tile_len = d_tlen.as<uint32_t>();
-    pp.tile_node = d_tnode.as<uint32_t>();
-    pp.node_first_tile = d_tfirst.as<uint32_t>();
-    pp.hist = d_hist.as<uint32_t>();
-    pp.excl = d_excl.as<uint64_t>();
-    pp.partial = d_partial.as<uint64_t>();
-    pp.err = d_err.as<uint32_t>();
-    pp.node_remap = any_remap ? d_remap.as<uint8_t>() : nullptr;
-    pp.node_lut = d_lut.as<uint8_t>();
-    pp.row_bins = kRow;
-    pp.thr_stride = 9;
-    launch_tile_hist(pp, s);
-    launch_tile_scan(pp, s);
-    // digit (= code) counts of every text: scan rows at the text boundaries
-    std::vector<uint64_t> rows_h((N + 1) * kRow);
-    {
-      DevBuf d_rows((N + 1) * kRow * 8);
-      launch_gather_rows(d_excl.as<uint64_t>(), d_tfirst.as<uint32_t>(), N + 1, kRow,
-                         d_rows.as<uint64_t>(), s);
-      ECL_HIP(hipStreamSynchronize(s));
-      d2h(rows_h.data(), d_rows.ptr, rows_h.size());
-    }
-
-    // ---- per tree: counts, metadata, level sizes, arena layout
+    f->tile_rows = tile_rows;
+    // ---- arena layout: every level at the capacity of a full level
     f->trees.resize(N);
     uint64_t at = 0;  // bits and directories first, then the metadata block
     for (size_t i = 0; i < N; ++i) {
@@ -203,49 +490,45 @@ ecl_status ecl_forest_create(const void* d_texts, const uint64_t* starts, const 
       t.sigma = sh.sigma;
       t.L = sh.L;
       t.shape = &sh;
-      std::vector<uint64_t> sym(sh.sigma);
-      for (uint64_t c = 0; c < sh.sigma; ++c)
-        sym[c] = rows_h[(i + 1) * kRow + sh.code[c]] - rows_h[i * kRow + sh.code[c]];
-      t.meta = tree_meta(sh, sym, t.n);
-      t.level_size = level_sizes(sh, sym);
       // bit array: each level padded to 32 words (bit_array.cu:57-72)
       at = align_up(at, kAlign);
+      const uint64_t nwords = (t.n + 31) / 32;
       uint64_t w = 0;
       t.level_word.resize(t.L);
-      t.level_nwords.resize(t.L);
       for (uint32_t l = 0; l < t.L; ++l) {
         t.level_word[l] = at / 4 + w;
-        t.level_nwords[l] = (t.level_size[l] + 31) / 32;
-        w += (t.level_nwords[l] + 31) & ~31ull;
+        w += (nwords + 31) & ~31ull;
       }
       w += w % 2;
       at += std::max<uint64_t>(w, 2) * 4;
       // rank/select directory per level (rank_select.cu:57-143 sizes)
-      t.num_l1.resize(t.L); t.num_l2.resize(t.L); t.cap.resize(t.L);
+      const uint64_t n1 = (t.n + detail::kL1Bits - 1) / detail::kL1Bits;
+      uint64_t last = ((t.n % detail::kL1Bits) + detail::kL2Bits - 1) / detail::kL2Bits;
+      if (last == 0) last = detail::kL2PerL1;
+      const uint64_t n2 = (n1 - 1) * detail::kL2PerL1 + last, cap = t.n / detail::kSampleRate;
       t.l1_at.resize(t.L); t.l2_at.resize(t.L); t.smp_at.resize(t.L);
       for (uint32_t l = 0; l < t.L; ++l) {
-        const uint64_t m = t.level_size[l];
-        t.num_l1[l] = (m + detail::kL1Bits - 1) / detail::kL1Bits;
-        uint64_t last = ((m % detail::kL1Bits) + detail::kL2Bits - 1) / detail::kL2Bits;
-        if (last == 0 && m) last = detail::kL2PerL1;
-        t.num_l2[l] = m ? (t.num_l1[l] - 1) * detail::kL2PerL1 + last : 0;
-        t.cap[l] = m / detail::kSampleRate;
         at = align_up(at, 8);
-        t.l1_at[l] = at; at += t.num_l1[l] * 8;
-        t.smp_at[l] = at; at += t.cap[l] * 8;
-        t.l2_at[l] = at; at += t.num_l2[l] * 2;
+        t.l1_at[l] = at; at += n1 * 8;
+        t.smp_at[l] = at; at += cap * 8;
+        t.l2_at[l] = at; at += n2 * 2;
       }
     }
     // metadata block: total ones, counts, ranks, nodes per level, codes,
-    // code lengths and level views of every tree, uploaded in one copy
+    // code lengths and level views of every tree
     at = align_up(at, kAlign);
     const uint64_t meta_at = at;
     for (auto& t : f->trees) { t.ones_at = at; at += t.L * 8ull; }
-    const uint64_t ones_end = at;
+    std::map<uint64_t, TreeMeta> shape_meta;  // node counts per shape (from unit counts)
+    for (auto& kv : f->shapes)
+      shape_meta.emplace(kv.first, tree_meta(kv.second, std::vector<uint64_t>(kv.first, 1),
+                                             kv.first));
     for (auto& t : f->trees) {
+      const TreeMeta& sm = shape_meta.at(t.sigma);
+      t.num_ranks = static_cast<uint32_t>(sm.ranks.size());
       t.counts_at = at; at += t.sigma * 8;
-      t.ranks_at = at; at += t.meta.ranks.size() * 8;
-      t.nal_at = at; at += t.meta.nodes_at_level.size() * 8;
+      t.ranks_at = at; at += t.num_ranks * 8ull;
+      t.nal_at = at; at += sm.nodes_at_level.size() * 8;
       const uint64_t ncodes = t.shape->pow2 ? 0 : t.sigma - t.shape->codes_start;
       t.codes_at = at; at += ncodes * 8;
       t.lens_at = at; at += align_up(ncodes, 8);
@@ -257,99 +540,9 @@ ecl_status ecl_forest_create(const void* d_texts, const uint64_t* starts, const 
       throw Error(ECL_ERROR_OUT_OF_MEMORY_TREE, "Not enough memory available for the wavelet tree.");
     f->arena = DevBuf(arena_bytes);
     uint8_t* A = f->arena.as<uint8_t>();
-    // bits are OR-ed in at run edges and the totals accumulate: zero all
-    launch_fill_u64(f->arena.as<uint64_t>(), arena_bytes / 8, 0, s);
-
-    // ---- the level pass of every tree
-    {
-      std::vector<uint64_t> pc(N * (kRow + 1)), lw(N * 8, 0), lnw(N * 8, 0);
-      std::vector<uint16_t> thr(N * 9, 0);
-      std::vector<uint8_t> nk(N);
-      for (size_t i = 0; i < N; ++i) {
-        const TreeSlot& t = f->trees[i];
-        const TreeShape& sh = *t.shape;
-        const uint32_t L = t.L, nb = 1u << L;
-        nk[i] = static_cast<uint8_t>(L);
-        uint64_t run = 0;
-        for (uint32_t d = 0; d <= kRow; ++d) {
-          pc[i * (kRow + 1) + d] = run;
-          if (d < kRow) run += rows_h[(i + 1) * kRow + d] - rows_h[i * kRow + d];
-        }
-        for (uint32_t l = 0; l < L; ++l) {
-          lw[i * 8 + l] = t.level_word[l];
-          lnw[i * 8 + l] = t.level_nwords[l];
-        }
-        if (sh.pow2) {
-          for (uint32_t x = 0; x <= L; ++x) thr[i * 9 + x] = static_cast<uint16_t>(nb);
-        } else {
-          // first absent code per level, aligned to the level's groups
-          int32_t maxd[9];
-          for (uint32_t x = 0; x <= L; ++x) maxd[x] = -1;
-          for (uint64_t c = 0; c < sh.sigma; ++c)
-            for (uint32_t x = 0; x <= L; ++x)
-              if (sh.len[c] > x) maxd[x] = std::max<int32_t>(maxd[x], static_cast<int32_t>(sh.code[c]));
-          for (uint32_t x = 0; x <= L; ++x)
-            thr[i * 9 + x] = maxd[x] < 0 ? 0
-                                         : static_cast<uint16_t>(
-                                               ((static_cast<uint32_t>(maxd[x]) >> (L - x)) + 1)
-                                               << (L - x));
-        }
-      }
-      DevBuf d_pc(pc.size() * 8), d_thr(thr.size() * 2), d_nk(N), d_lw(N * 64), d_lnw(N * 64);
-      h2d(d_pc.ptr, pc.data(), pc.size());
-      h2d(d_thr.ptr, thr.data(), thr.size());
-      h2d(d_nk.ptr, nk.data(), N);
-      h2d(d_lw.ptr, lw.data(), lw.size());
-      h2d(d_lnw.ptr, lnw.data(), lnw.size());
-      pp.pc = d_pc.as<uint64_t>();
-      pp.thr = d_thr.as<uint16_t>();
-      pp.node_k = d_nk.as<uint8_t>();
-      pp.node_lw = d_lw.as<uint64_t>();
-      pp.node_lnw = d_lnw.as<uint64_t>();
-      pp.words = f->arena.as<uint32_t>();
-      pp.dst = nullptr;
-      launch_pass(pp, s);
-      uint32_t err = 0;
-      ECL_HIP(hipStreamSynchronize(s));
-      d2h(&err, d_err.ptr, 1);
-      if (err) throw Error(ECL_ERROR_INTERNAL, "construction pass wrote outside its arrays");
-    }
-    d_x.reset();
-    d_hist.reset();
-    d_excl.reset();
-
-    // ---- directories of every level of every tree, one launch
-    {
-      std::vector<RsArrayDesc> desc;
-      uint64_t blocks = 0;
-      for (auto& t : f->trees) {
-        for (uint32_t l = 0; l < t.L; ++l) {
-          if (t.level_size[l] == 0) continue;
-          RsArrayDesc d{};
-          d.words64 = reinterpret_cast<const uint64_t*>(A + t.level_word[l] * 4);
-          d.m = t.level_size[l];
-          d.l1 = reinterpret_cast<uint64_t*>(A + t.l1_at[l]);
-          d.l2 = reinterpret_cast<uint16_t*>(A + t.l2_at[l]);
-          d.num_l1 = t.num_l1[l];
-          d.num_l2 = t.num_l2[l];
-          d.samples = reinterpret_cast<uint64_t*>(A + t.smp_at[l]);
-          d.cap = t.cap[l];
-          d.total_ones = reinterpret_cast<uint64_t*>(A + t.ones_at) + l;
-          d.first_tile = blocks;
-          blocks += d.num_l1;
-          desc.push_back(d);
-        }
-      }
-      DevBuf d_desc(std::max<size_t>(desc.size(), 1) * sizeof(RsArrayDesc));
-      DevBuf d_blk(std::max<uint64_t>(blocks, 1) * 8);
-      h2d(d_desc.ptr, desc.data(), desc.size());
-      launch_rs_build(d_desc.as<RsArrayDesc>(), desc.size(), blocks, d_blk.as<uint32_t>(), s);
-      ECL_HIP(hipStreamSynchronize(s));
-    }
-
-    // ---- metadata block and views
+    // ---- the static part of the metadata block (the build's kernels write
+    // the counts, node ranks, totals and level views) and the tree views
     std::vector<uint8_t> img(arena_bytes - meta_at, 0);
-    d2h(img.data(), A + meta_at, ones_end - meta_at);  // total ones per level
     auto put = [&](uint64_t where, const void* src, size_t bytes) {
       if (bytes) std::memcpy(img.data() + (where - meta_at), src, bytes);
     };
@@ -357,9 +550,8 @@ ecl_status ecl_forest_create(const void* d_texts, const uint64_t* starts, const 
     for (size_t i = 0; i < N; ++i) {
       TreeSlot& t = f->trees[i];
       const TreeShape& sh = *t.shape;
-      put(t.counts_at, t.meta.counts.data(), t.sigma * 8);
-      put(t.ranks_at, t.meta.ranks.data(), t.meta.ranks.size() * 8);
-      put(t.nal_at, t.meta.nodes_at_level.data(), t.meta.nodes_at_level.size() * 8);
+      const TreeMeta& sm = shape_meta.at(t.sigma);
+      put(t.nal_at, sm.nodes_at_level.data(), sm.nodes_at_level.size() * 8);
       if (!sh.pow2) {
         std::vector<uint64_t> codes;
         std::vector<uint8_t> lens;
@@ -370,24 +562,8 @@ ecl_status ecl_forest_create(const void* d_texts, const uint64_t* starts, const 
         put(t.codes_at, codes.data(), codes.size() * 8);
         put(t.lens_at, lens.data(), lens.size());
       }
-      std::vector<LevelView> lv(t.L);
-      const uint64_t* ones = reinterpret_cast<const uint64_t*>(img.data() + (t.ones_at - meta_at));
-      for (uint32_t l = 0; l < t.L; ++l) {
-        LevelView& v = lv[l];
-        v.words = reinterpret_cast<const uint32_t*>(A + t.level_word[l] * 4);
-        v.l1 = reinterpret_cast<const uint64_t*>(A + t.l1_at[l]);
-        v.l2 = reinterpret_cast<const uint16_t*>(A + t.l2_at[l]);
-        v.samples = reinterpret_cast<const uint64_t*>(A + t.smp_at[l]);
-        v.size = t.level_size[l];
-        v.ones = ones[l];
-        v.num_l1 = t.num_l1[l];
-        v.sample_cap = t.cap[l];
-      }
-      put(t.lv_at, lv.data(), lv.size() * sizeof(LevelView));
-      for (uint32_t l = 0; l < t.L; ++l) {
-        f->levels.push_back(lv[l]);
+      for (uint32_t l = 0; l < t.L; ++l)
         f->d_levels.push_back(reinterpret_cast<LevelView*>(A + t.lv_at) + l);
-      }
       TreeView& v = f->views[i];
       v = TreeView{};
       v.levels = reinterpret_cast<const LevelView*>(A + t.lv_at);
@@ -399,31 +575,121 @@ ecl_status ecl_forest_create(const void* d_texts, const uint64_t* starts, const 
       v.n = t.n;
       v.sigma = t.sigma;
       v.codes_start = sh.codes_start;
-      v.nodes_until_last_level = t.meta.nodes_until_last_level;
-      v.num_ranks = t.meta.ranks.size();
+      v.nodes_until_last_level = sm.nodes_until_last_level;
+      v.num_ranks = t.num_ranks;
       v.num_levels = t.L;
       v.pow2 = sh.pow2 ? 1 : 0;
     }
     h2d(A + meta_at, img.data(), img.size());
     f->d_views = DevBuf(N * sizeof(TreeView));
     h2d(f->d_views.ptr, f->views.data(), N);
+    // ---- the build: one queue of launches, one wait at the end
+    {
+      std::shared_ptr<ForestBuild> B = make_build(f.get(), choose_rows(f.get(), tile_rows));
+      enqueue_build(f.get(), *B, s);
+      finish_build(*B, s);
+    }
     f->q_grid = static_cast<uint32_t>(cu_count(device)) * 8;
     *out = f.release();
+  });
+}
+
+// new: the forest's construction as one graph that rebuilds every tree from
+// the current contents of its texts (same buffer, starts, lengths and
+// alphabets); the graph keeps the build's device buffers.
+ecl_status ecl_forest_build_graph(ecl_forest f, ecl_graph* out) {
+  return guard([&] {
+    require(f && out, "NULL argument");
+    set_device(f->device);
+    if (!f->build) f->build = make_build(f, choose_rows(f, f->tile_rows));
+    auto g = std::make_unique<ecl_graph_t>();
+    g->device = f->device;
+    g->keep = f->build;
+    hipStream_t s = nullptr;
+    ECL_HIP(hipStreamCreate(&s));
+    try {
+      ECL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+      enqueue_build(f, *f->build, s);
+      ECL_HIP(hipStreamEndCapture(s, &g->graph));
+      ECL_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
+    } catch (...) {
+      hipGraph_t dummy = nullptr;
+      (void)hipStreamEndCapture(s, &dummy);
+      if (dummy) (void)hipGraphDestroy(dummy);
+      (void)hipStreamDestroy(s);
+      throw;
+    }
+    ECL_HIP(hipStreamDestroy(s));
+    *out = g.release();
+  });
+}
+
+// new: times the build (the queued launches) with tiles of 1, 2 and 4
+// chunks per thread on this forest's texts, keeps the fastest for later
+// build graphs, and leaves the forest built (every tile size builds the
+// same trees).
+ecl_status ecl_forest_autotune_build(ecl_forest f, uint32_t* tile_rows, float* ms) {
+  return guard([&] {
+    require(f, "NULL argument");
+    set_device(f->device);
+    hipStream_t s = nullptr;
+    hipEvent_t e0, e1;
+    ECL_HIP(hipEventCreate(&e0));
+    ECL_HIP(hipEventCreate(&e1));
+    float best = 1e30f;
+    uint32_t best_rows = choose_rows(f, 0);
+    try {
+      for (uint32_t r : {1u, 2u, 4u}) {
+        std::shared_ptr<ForestBuild> B = make_build(f, r);
+        enqueue_build(f, *B, s);  // warm
+        finish_build(*B, s);
+        ECL_HIP(hipEventRecord(e0, s));
+        enqueue_build(f, *B, s);
+        ECL_HIP(hipEventRecord(e1, s));
+        finish_build(*B, s);
+        float t = 0.f;
+        ECL_HIP(hipEventElapsedTime(&t, e0, e1));
+        if (t < best) {
+          best = t;
+          best_rows = r;
+        }
+      }
+    } catch (...) {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    f->tile_rows = best_rows;
+    f->build.reset();
+    if (tile_rows) *tile_rows = best_rows;
+    if (ms) *ms = best;
   });
 }
 
 ecl_status ecl_forest_build_query_lines(ecl_forest f) {
   return guard([&] {
     require(f, "NULL argument");
-    if (f->lines.ptr != nullptr) return;
     set_device(f->device);
-    const size_t na = f->levels.size();
+    // the level views as the last build left them (a rebuild from a build
+    // graph drops the lines of the trees it rebuilt)
+    const size_t na = f->d_levels.size();
+    // (the views of all trees are consecutive in the arena: one copy)
+    std::vector<LevelView> views(na);
+    if (na)
+      d2h(views.data(),
+          reinterpret_cast<const LevelView*>(f->arena.as<uint8_t>() + f->trees.front().lv_at), na);
+    bool have = na > 0;
+    for (const LevelView& v : views) have = have && (v.lines != nullptr || v.size == 0);
+    if (have && f->lines.ptr != nullptr) return;
+    f->lines.reset();
     require(na < (1ull << 32), "too many bit arrays");
     std::vector<LinesJob> jobs(na);
     std::vector<uint64_t> starts(2 * na);  // line starts | hint starts
     uint64_t tl = 0, th = 0;
     for (size_t a = 0; a < na; ++a) {
-      const LevelView& v = f->levels[a];
+      const LevelView& v = views[a];
       LinesJob& J = jobs[a];
       J.v = v;
       J.v.lines = nullptr;
@@ -449,20 +715,12 @@ ecl_status ecl_forest_build_query_lines(ecl_forest f) {
     hipStream_t s = nullptr;
     h2d(d_jobs.ptr, jobs.data(), na);
     h2d(d_starts.ptr, starts.data(), 2 * na);
-    ECL_HIP(hipMemsetAsync(d_fit.ptr, 0, std::max<size_t>(na, 1) * 8, s));
+    launch_fill_u32(d_fit.as<uint32_t>(), std::max<size_t>(na, 1) * 2, 0, s);
     uint4* L = lines.as<uint4>();
     uint4* H = L + tl * 8;
     launch_multi_lines(d_jobs.as<LinesJob>(), d_starts.as<uint64_t>(), d_starts.as<uint64_t>() + na,
                        static_cast<uint32_t>(na), L, tl, H, th, d_fit.as<uint32_t>(), s);
-    std::vector<uint32_t> fit(2 * na);
-    if (na) d2h(fit.data(), d_fit.ptr, 2 * na);  // waits for the launches
-    for (size_t a = 0; a < na; ++a) {
-      LevelView& v = f->levels[a];
-      v.lines = jobs[a].nl ? L + starts[a] * 8 : nullptr;
-      v.sel_hints = jobs[a].nh1 + jobs[a].nh0 ? H + starts[na + a] * 4 : nullptr;
-      v.hint_values = (2ull * fit[2 * a + 1] >= jobs[a].nh1 ? 2u : 0u) |
-                      (2ull * fit[2 * a] >= jobs[a].nh0 ? 1u : 0u);
-    }
+    ECL_HIP(hipStreamSynchronize(s));
     f->lines = std::move(lines);
   });
 }

@@ -191,6 +191,10 @@ struct RsArrayDesc {
   uint64_t* samples; uint64_t cap;
   uint64_t* total_ones;
   uint64_t first_tile;  // first global L1 block of the array
+  // L1 blocks reserved for the array in the global numbering (>= num_l1;
+  // 0 means num_l1): blocks past num_l1 do nothing, so a launch laid out by
+  // capacity serves arrays whose sizes are only known on the device
+  uint64_t cap_l1;
 };
 // arrays: narrays entries (device) covering nblocks global L1 blocks in
 // order; scratch: 2 * nblocks u32 (the array of every block, block totals)

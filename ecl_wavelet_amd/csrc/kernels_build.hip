@@ -126,7 +126,8 @@ constexpr uint32_t kRsWaves = 8;  // L1 blocks per workgroup
 // one workgroup per array: its blocks' entries of block_array
 __global__ __launch_bounds__(256) void rs_block_map_kernel(const RsArrayDesc* __restrict__ arrays,
                                                            uint32_t* __restrict__ block_array) {
-  const uint64_t first = arrays[blockIdx.x].first_tile, cnt = arrays[blockIdx.x].num_l1;
+  const RsArrayDesc& a = arrays[blockIdx.x];
+  const uint64_t first = a.first_tile, cnt = a.cap_l1 ? a.cap_l1 : a.num_l1;
   for (uint64_t b = threadIdx.x; b < cnt; b += 256) block_array[first + b] = blockIdx.x;
 }
 
@@ -139,6 +140,7 @@ __global__ __launch_bounds__(512) void rs_local_kernel(const RsArrayDesc* __rest
   if (gb >= nblocks) return;
   const RsArrayDesc p = arrays[block_array[gb]];
   const uint64_t kb = gb - p.first_tile;  // block within its array
+  if (kb >= p.num_l1) return;             // reserved past the array's size
   const uint64_t nwords = (p.m + 63) / 64;
   const uint64_t w0 = kb * 1024 + static_cast<uint64_t>(lane) * 16;
   uint64_t w[16];
@@ -262,6 +264,7 @@ __global__ __launch_bounds__(512) void rs_samples_kernel(const RsArrayDesc* __re
   const RsArrayDesc p = arrays[block_array[gb]];
   if (p.cap == 0) return;
   const uint64_t kb = gb - p.first_tile;
+  if (kb >= p.num_l1) return;  // reserved past the array's size
   const uint64_t bits0 = kb * detail::kL1Bits;                     // first bit of the block
   const uint64_t bits = p.m - bits0 < detail::kL1Bits ? p.m - bits0 : detail::kL1Bits;
   const uint64_t nl2 = (bits + detail::kL2Bits - 1) / detail::kL2Bits;  // L2 blocks here

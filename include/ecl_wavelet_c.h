@@ -400,6 +400,22 @@ ecl_status ecl_forest_create(const void* d_texts, const uint64_t* starts, const 
                              const uint64_t* sigmas, size_t num_texts, int device,
                              uint32_t tile_rows, ecl_forest* out);
 ecl_status ecl_forest_destroy(ecl_forest f);
+/* The forest's construction as one HIP graph: replaying it (ecl_graph_launch)
+ * rebuilds every tree from the current contents of the texts passed to
+ * ecl_forest_create (same buffer, starts, lengths and alphabets), with no
+ * host involvement between its launches: tile histograms and their scan,
+ * per-tree counts, C arrays, node ranks and level sizes on the device, the
+ * level pass, every directory and every level view.  A rebuild drops the
+ * query lines (build them again).  The graph keeps the build's device
+ * buffers (about 2.6 bytes per symbol, plus one byte per symbol when an
+ * alphabet is not a power of two) until it is destroyed; destroy it before
+ * the forest, and launch it while no query on the forest is in flight. */
+ecl_status ecl_forest_build_graph(ecl_forest f, ecl_graph* out);
+/* Times the construction with tiles of 8192, 16384 and 32768 codes on this
+ * forest's texts and keeps the fastest for later build graphs (the trees
+ * are the same with any tile); *tile_rows gets 1, 2 or 4 and *ms the
+ * fastest build time.  Leaves the forest built. */
+ecl_status ecl_forest_autotune_build(ecl_forest f, uint32_t* tile_rows, float* ms);
 /* Query lines and select hints (see ecl_wavelet_tree_build_query_lines) for
  * every tree of the forest, in three launches; later forest queries and
  * graphs (also graphs captured before) read them.  Make the call while no

@@ -241,3 +241,61 @@ def test_forest_query_lines(gpu):
         g.launch()
         synchronize()
         assert np.array_equal(dout.download(np.uint64, len(q)), e)
+
+
+def _rewrite(d, starts, texts):
+    """The texts written in place into the forest's device buffer."""
+    for t, s0 in zip(texts, starts):
+        dv = d.view(s0, len(t))
+        dv.upload(t)
+
+
+def test_forest_build_graph_rebuilds_from_new_texts(gpu):
+    """The build captured as one graph: every replay rebuilds all trees from
+    the texts the buffer holds at that moment (same lengths and alphabets,
+    new symbols, so the non-power-of-two level sizes change), and the trees
+    then answer like the oracle over those texts.  Eager work between the
+    replays; query lines built before a replay are dropped by it and built
+    again after."""
+    from ecl_wavelet_amd import synchronize
+    texts, sigmas = _padded_texts(31, 45, lo_log2=5, hi_log2=17)
+    f, d, starts = _forest(texts, sigmas)
+    q, e = _queries(texts, sigmas, 200, seed=31)
+    assert np.array_equal(_run(f, q), e)
+    g = f.buildGraph()
+    rng = np.random.default_rng(32)
+    for rep in range(3):
+        new = []
+        for t, s in zip(texts, sigmas):
+            if rep == 1:  # skewed: some symbols absent, level sizes move
+                x = O.gen_cdf(100 + rep, len(t), O.zipf_thresholds(s, 2.0), np.uint8)
+            else:
+                x = rng.integers(0, s, len(t)).astype(np.uint8)
+            new.append(x)
+        _rewrite(d, starts, new)
+        synchronize()
+        if rep == 2:
+            f.buildQueryLines()
+        g.launch()
+        synchronize()
+        q, e = _queries(new, sigmas, 200, seed=40 + rep)
+        assert np.array_equal(_run(f, q), e), rep
+        f.buildQueryLines()
+        assert np.array_equal(_run(f, q), e), rep
+    del g
+
+
+def test_forest_autotune_build(gpu):
+    """The build autotuner picks a tile size, leaves the forest built (same
+    answers), and later build graphs use the chosen tile."""
+    from ecl_wavelet_amd import synchronize
+    texts, sigmas = _padded_texts(33, 30, lo_log2=8, hi_log2=18)
+    f, d, starts = _forest(texts, sigmas)
+    q, e = _queries(texts, sigmas, 200, seed=33)
+    rows, ms = f.autotuneBuild()
+    assert rows in (1, 2, 4) and ms > 0
+    assert np.array_equal(_run(f, q), e)
+    g = f.buildGraph()
+    g.launch()
+    synchronize()
+    assert np.array_equal(_run(f, q), e)
