@@ -505,6 +505,18 @@ class WaveletTree:
     def generateQueries(self, kind, seed, m, d_out, stream=None):
         call("ecl_wavelet_tree_generate_queries", self._h, kind, seed, m, d_out.ptr, stream)
 
+    def setQueryLanes(self, lanes):
+        """Lanes per access/rank query for this tree (0: library-wide)."""
+        call("ecl_wavelet_tree_set_query_lanes", self._h, lanes)
+
+    def autotune(self, m=1 << 22, seed=1):
+        """Times one and two lanes per access/rank query on m generated
+        queries per kind and keeps the faster; returns (lanes, [ms1, ms2])."""
+        lanes = C.c_uint32()
+        ms = (C.c_float * 2)()
+        call("ecl_wavelet_tree_autotune", self._h, m, seed, C.byref(lanes), ms)
+        return lanes.value, [ms[0], ms[1]]
+
 
 class Forest:
     """Many trees over uint8 texts built together (ecl_forest_create): text

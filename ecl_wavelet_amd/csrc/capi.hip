@@ -537,6 +537,7 @@ struct ecl_wavelet_tree_t {
   DevBuf d_counts, d_ranks, d_nal, d_codes, d_code_lens, d_alphabet, d_lut;
   int map_kind = 0;
   TreeView view{};
+  uint32_t lanes = 0;  // lanes per access/rank query (0: the library-wide setting)
   std::unique_ptr<HostPipe> pipe{new HostPipe()};
   ~ecl_wavelet_tree_t() { delete rs; }
 };
@@ -1681,6 +1682,7 @@ static TreeQueryParams query_params(const ecl_wavelet_tree_t* wt, uint32_t flags
                                                                          : nullptr;
   p.has_lines = wt->rs->lines.ptr != nullptr ? 1 : 0;
   p.has_sel_lines = wt->rs->sel_lines.ptr != nullptr ? 1 : 0;
+  p.lanes = wt->lanes;
   return p;
 }
 
@@ -2347,6 +2349,62 @@ ecl_status ecl_wavelet_tree_generate_queries(ecl_wavelet_tree wt, int kind, uint
     set_device(wt->device);
     launch_tree_gen_queries(wt->view, nullptr, kind, seed, m, d_out, wt->esize,
                             as_stream(stream));
+  });
+}
+
+ecl_status ecl_wavelet_tree_set_query_lanes(ecl_wavelet_tree wt, uint32_t lanes) {
+  return guard([&] {
+    require(wt != nullptr, "NULL tree");
+    require(lanes <= 2, "lanes must be 0, 1 or 2");
+    wt->lanes = lanes;
+  });
+}
+
+// Times access and rank with one and with two lanes per query on m
+// device-generated queries of each kind over the tree as it is now (with or
+// without query lines), and keeps the faster setting for the tree.
+ecl_status ecl_wavelet_tree_autotune(ecl_wavelet_tree wt, uint64_t m, uint64_t seed,
+                                     uint32_t* lanes, float* ms) {
+  return guard([&] {
+    require(wt != nullptr, "NULL tree");
+    require(m > 0, "m must be positive");
+    set_device(wt->device);
+    hipStream_t s = nullptr;
+    DevBuf d_idx(m * 8), d_rq(m * 16), d_out(m * 8);
+    launch_tree_gen_queries(wt->view, nullptr, 0, seed, m, d_idx.ptr, wt->esize, s);
+    launch_tree_gen_queries(wt->view, nullptr, 1, seed + 1, m, d_rq.ptr, wt->esize, s);
+    hipEvent_t e0, e1;
+    ECL_HIP(hipEventCreate(&e0));
+    ECL_HIP(hipEventCreate(&e1));
+    float t[2] = {0.f, 0.f};
+    try {
+      for (uint32_t g = 1; g <= 2; ++g) {
+        TreeQueryParams p = query_params(wt, ECL_SYMBOLS_MINIMAL | ECL_RESULTS_MINIMAL);
+        p.lanes = g;
+        for (int rep = 0; rep < 3; ++rep) {  // the first round warms up
+          ECL_HIP(hipEventRecord(e0, s));
+          launch_tree_access(p, d_idx.as<uint64_t>(), m, d_out.ptr, s);
+          launch_tree_rank(p, d_rq.ptr, m, d_out.as<uint64_t>(), s);
+          ECL_HIP(hipEventRecord(e1, s));
+          ECL_HIP(hipEventSynchronize(e1));
+          float x = 0.f;
+          ECL_HIP(hipEventElapsedTime(&x, e0, e1));
+          if (rep == 1 || (rep == 2 && x < t[g - 1])) t[g - 1] = x;
+        }
+      }
+    } catch (...) {
+      (void)hipEventDestroy(e0);
+      (void)hipEventDestroy(e1);
+      throw;
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    wt->lanes = t[1] < t[0] ? 2 : 1;
+    if (lanes) *lanes = wt->lanes;
+    if (ms) {
+      ms[0] = t[0];
+      ms[1] = t[1];
+    }
   });
 }
 

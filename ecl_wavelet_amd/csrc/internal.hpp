@@ -219,6 +219,8 @@ struct TreeQueryParams {
   uint64_t max_lds_counts;   // unused by callers; tuning knob
   int has_lines;             // 1: the tree's levels carry query lines
   int has_sel_lines;         // 1: the tree's levels carry select lines
+  uint32_t lanes;            // lanes per access/rank query: 0 (the library-wide
+                             // setting), 1 or 2 (the tree's own, from its tuner)
 };
 void launch_tree_access(const TreeQueryParams& p, const uint64_t* idx,
                         uint64_t m, void* out, hipStream_t s);

@@ -538,7 +538,9 @@ bool pair_lanes_for(int has_lines) {
   return l == 0 ? has_lines != 0 : l == 2;
 }
 
-bool pair_lanes(const TreeQueryParams& p) { return pair_lanes_for(p.has_lines); }
+bool pair_lanes(const TreeQueryParams& p) {
+  return p.lanes ? p.lanes == 2 : pair_lanes_for(p.has_lines);
+}
 
 template <int G>
 void access_g(const TreeQueryParams& p, const uint64_t* idx, uint64_t m, void* out, hipStream_t s) {

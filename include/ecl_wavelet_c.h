@@ -378,6 +378,16 @@ ecl_status ecl_wavelet_tree_query_batch_device(ecl_wavelet_tree wt, const ecl_qu
  * kind 3: ecl_query mixing the three kinds 1:1:1).  Symbols are minimal. */
 ecl_status ecl_wavelet_tree_generate_queries(ecl_wavelet_tree wt, int kind, uint64_t seed,
                                              size_t m, void* d_out, void* stream);
+/* new: lanes per access/rank query for this tree's kernels: 0 (default: the
+ * library-wide ecl_set_query_lanes setting), 1, or 2.  No effect on results. */
+ecl_status ecl_wavelet_tree_set_query_lanes(ecl_wavelet_tree wt, uint32_t lanes);
+/* new: the tree's query autotuner.  Times access and rank with one and with
+ * two lanes per query over m device-generated queries of each kind (seeded)
+ * on the tree as it is now (with or without query lines) and keeps the
+ * faster for the tree; *lanes gets 1 or 2, ms[0] and ms[1] (may be NULL)
+ * the time of one access and one rank launch with each setting. */
+ecl_status ecl_wavelet_tree_autotune(ecl_wavelet_tree wt, uint64_t m, uint64_t seed,
+                                     uint32_t* lanes, float* ms);
 
 /* ==========================================================================
  * Forests: many independent trees built and queried together.  No

@@ -696,3 +696,22 @@ def test_tree_from_raw_text_file(gpu, tmp_path):
     _check_queries(wt, full[:n], None, nq=1000, seed=5)
     with pytest.raises(EclError, match="Data size is larger than file size"):
         WaveletTree.fromFile(path, 400_001, np.uint16)
+
+
+def test_tree_autotune_lanes(gpu):
+    """The tree's query autotuner picks one or two lanes per access/rank
+    query (with and without query lines) and keeps it for the tree; every
+    setting answers like the direct scans."""
+    from ecl_wavelet_amd import WaveletTree
+    rng = np.random.default_rng(61)
+    text = rng.integers(0, 256, 3_000_000).astype(np.uint8)
+    wt = WaveletTree(text, np.arange(256, dtype=np.uint8))
+    for lines in (False, True):
+        if lines:
+            wt.buildQueryLines()
+        lanes, ms = wt.autotune(m=1 << 20, seed=3)
+        assert lanes in (1, 2) and ms[0] > 0 and ms[1] > 0
+        _check_queries(wt, text, None, nq=2000, seed=62 + lines)
+        for forced in (1, 2, 0):
+            wt.setQueryLanes(forced)
+            _check_queries(wt, text, None, nq=1000, seed=70 + forced)
