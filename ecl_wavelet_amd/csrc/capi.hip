@@ -674,7 +674,7 @@ static void build_directories(ecl_rank_select_t* rs, ecl_bit_array_t* ba, hipStr
                               Scratch& scr, Uploader& up) {
   const size_t na = ba->sizes.size();
   std::vector<RsArrayDesc> desc;
-  uint64_t blocks = 0;
+  uint64_t blocks = 0, chunks = 0;
   for (size_t i = 0; i < na; ++i) {
     if (ba->sizes[i] == 0) continue;
     RsArrayDesc d{};
@@ -688,7 +688,9 @@ static void build_directories(ecl_rank_select_t* rs, ecl_bit_array_t* ba, hipStr
     d.cap = rs->cap[i];
     d.total_ones = rs->total_ones.as<uint64_t>() + i;
     d.first_tile = blocks;
+    d.first_chunk = chunks;
     blocks += d.num_l1;
+    chunks += rs_scan_chunks(d.num_l1);
     desc.push_back(d);
   }
   if (desc.empty()) return;
@@ -696,7 +698,7 @@ static void build_directories(ecl_rank_select_t* rs, ecl_bit_array_t* ba, hipStr
   uint32_t* d_blk = scr.carve<uint32_t>(2 * blocks);
   up.add(d_desc, desc.data(), desc.size());
   up.flush();
-  launch_rs_build(d_desc, desc.size(), blocks, d_blk, s);
+  launch_rs_build(d_desc, desc.size(), blocks, chunks, d_blk, s);
 }
 
 // RankSelect(BitArray&&) (rank_select.cu:50-304): one rs_build pass per array.

@@ -162,7 +162,7 @@ __global__ void forest_views_kernel(const RsArrayDesc* __restrict__ desc, uint32
 // forest while a build graph (ecl_forest_build_graph) may replay it.
 struct ForestBuild {
   uint32_t rows = 1;
-  uint64_t ntiles = 0, blocks = 0, meta_at = 0;
+  uint64_t ntiles = 0, blocks = 0, chunks = 0, meta_at = 0;
   uint32_t N = 0, ndesc = 0;
   DevBuf d_x, d_tstart, d_tlen, d_tnode, d_tfirst, d_remap, d_lut, d_hist, d_excl, d_partial,
       d_err, d_rows, d_pc, d_thr, d_nk, d_lw, d_lnw, d_jobs, d_shapes, d_code, d_len, d_node,
@@ -336,7 +336,7 @@ std::shared_ptr<ForestBuild> make_build(ecl_forest_t* f, uint32_t rows) {
   std::vector<MetaJob> jobs(N);
   std::vector<RsArrayDesc> desc;
   std::vector<LevelView*> lvptr;
-  uint64_t blocks = 0;
+  uint64_t blocks = 0, chunks = 0;
   for (size_t i = 0; i < N; ++i) {
     const TreeSlot& t = f->trees[i];
     MetaJob& J = jobs[i];
@@ -355,12 +355,15 @@ std::shared_ptr<ForestBuild> make_build(ecl_forest_t* f, uint32_t rows) {
       d.total_ones = reinterpret_cast<uint64_t*>(A + t.ones_at) + l;
       d.first_tile = blocks;
       d.cap_l1 = cap_l1;
+      d.first_chunk = chunks;
       blocks += cap_l1;
+      chunks += rs_scan_chunks(cap_l1);
       desc.push_back(d);
       lvptr.push_back(reinterpret_cast<LevelView*>(A + t.lv_at) + l);
     }
   }
   B->blocks = blocks;
+  B->chunks = chunks;
   B->ndesc = static_cast<uint32_t>(desc.size());
   B->d_jobs = DevBuf(N * sizeof(MetaJob));
   B->d_shapes = DevBuf(shp.size() * sizeof(ShapeDev));
@@ -436,7 +439,8 @@ void enqueue_build(ecl_forest_t* f, ForestBuild& B, hipStream_t s) {
                      f->arena.as<uint8_t>(), B.d_pc.as<uint64_t>(), B.d_desc.as<RsArrayDesc>());
   ECL_HIP(hipGetLastError());
   launch_pass(B.pp, s);
-  launch_rs_build(B.d_desc.as<RsArrayDesc>(), B.ndesc, B.blocks, B.d_blk.as<uint32_t>(), s);
+  launch_rs_build(B.d_desc.as<RsArrayDesc>(), B.ndesc, B.blocks, B.chunks,
+                  B.d_blk.as<uint32_t>(), s);
   hipLaunchKernelGGL(forest_views_kernel, dim3((B.ndesc + 255) / 256), dim3(256), 0, s,
                      B.d_desc.as<RsArrayDesc>(), B.ndesc, B.d_lvptr.as<LevelView*>());
   ECL_HIP(hipGetLastError());

@@ -195,11 +195,19 @@ struct RsArrayDesc {
   // 0 means num_l1): blocks past num_l1 do nothing, so a launch laid out by
   // capacity serves arrays whose sizes are only known on the device
   uint64_t cap_l1;
+  // first chunk of kRsScanChunk L1 blocks of the array in the global
+  // numbering of the L1 scan (the arrays' chunks follow each other)
+  uint64_t first_chunk;
 };
-// arrays: narrays entries (device) covering nblocks global L1 blocks in
-// order; scratch: 2 * nblocks u32 (the array of every block, block totals)
+constexpr uint64_t kRsScanChunk = 2048;  // L1 blocks per chunk of the L1 scan
+__host__ __device__ inline uint64_t rs_scan_chunks(uint64_t l1_blocks) {
+  return (l1_blocks + kRsScanChunk - 1) / kRsScanChunk;
+}
+// arrays: narrays entries (device) covering nblocks global L1 blocks and
+// nchunks scan chunks in order; scratch: 2 * nblocks u32 (the array of every
+// block, block totals)
 void launch_rs_build(const RsArrayDesc* arrays, uint64_t narrays, uint64_t nblocks,
-                     uint32_t* scratch, hipStream_t s);
+                     uint64_t nchunks, uint32_t* scratch, hipStream_t s);
 
 void launch_gen_text(void* out, int esize, uint64_t n, uint64_t seed,
                      uint64_t sigma, const uint64_t* thresholds,

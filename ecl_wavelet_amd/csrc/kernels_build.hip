@@ -114,8 +114,9 @@ __global__ __launch_bounds__(256) void histogram_kernel(HistParams p,
 // workgroup:
 //  1. rs_local_kernel: one wave per 65536-bit L1 block (16 u64 words per
 //     lane): popcounts, in-block prefix = the block's L2 entries, block total.
-//  2. rs_scan_kernel: one workgroup per bit array: exclusive scan of its block
-//     totals = the L1 entries, and the array's total.
+//  2. rs_chunk_sum / rs_chunk_scan / rs_l1_write kernels: the exclusive scan
+//     of every array's block totals = its L1 entries, and its total (chunk
+//     sums, their scan per array, chunk scans from their bases).
 //  3. rs_samples_kernel: one wave per L1 block, one lane per select sample
 //     that falls inside the block: binary search over the block's L2 entries,
 //     then one 64-byte L2 block is read and the bit is selected in-word.
@@ -174,52 +175,96 @@ __global__ __launch_bounds__(512) void rs_local_kernel(const RsArrayDesc* __rest
   if (lane == 63) block_ones[gb] = incl;
 }
 
-// one workgroup (1024 threads) per array: L1 = exclusive prefix of the
-// array's block totals; total ones at the end.  Each thread takes a
-// contiguous range of blocks and keeps eight loads in flight.
-constexpr uint32_t kRsScanThreads = 1024;
+// L1 entries = exclusive prefix of each array's block totals, in three
+// launches with every workgroup busy: the sum of every chunk of
+// kRsScanChunk blocks (kept for the moment in the L1 entry of the chunk's
+// first block), the exclusive scan of each array's chunk sums in place (and
+// the array's total), then every chunk's own scan from its base.
 
-__global__ __launch_bounds__(kRsScanThreads) void rs_scan_kernel(
-    const RsArrayDesc* __restrict__ arrays, const uint32_t* __restrict__ block_ones) {
-  __shared__ uint64_t s_wave[kRsScanThreads / 64];
-  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const RsArrayDesc p = arrays[blockIdx.x];
-  const uint32_t* cnt = block_ones + p.first_tile;
-  const uint64_t per = (p.num_l1 + kRsScanThreads - 1) / kRsScanThreads;
-  const uint64_t b0 = tid * per < p.num_l1 ? tid * per : p.num_l1;
-  const uint64_t b1 = b0 + per < p.num_l1 ? b0 + per : p.num_l1;
-  uint64_t sum = 0;
-  for (uint64_t b = b0; b < b1; b += 8) {
-    uint32_t x[8];
-#pragma unroll
-    for (int j = 0; j < 8; ++j) x[j] = b + j < b1 ? cnt[b + j] : 0u;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) sum += x[j];
+// the array of global scan chunk j: the last whose first chunk is <= j
+__device__ __forceinline__ uint32_t rs_chunk_array(const RsArrayDesc* __restrict__ arrays,
+                                                   uint32_t narrays, uint64_t j) {
+  uint32_t lo = 0, hi = narrays;
+  while (hi - lo > 1) {
+    const uint32_t mid = (lo + hi) >> 1;
+    if (arrays[mid].first_chunk <= j) lo = mid; else hi = mid;
   }
-  uint64_t incl = sum;
+  return lo;
+}
+
+__global__ __launch_bounds__(256) void rs_chunk_sum_kernel(const RsArrayDesc* __restrict__ arrays,
+                                                           uint32_t narrays,
+                                                           const uint32_t* __restrict__ block_ones) {
+  __shared__ uint32_t s_w[4];
+  const uint32_t tid = threadIdx.x;
+  const RsArrayDesc& p = arrays[rs_chunk_array(arrays, narrays, blockIdx.x)];
+  const uint64_t b0 = (blockIdx.x - p.first_chunk) * kRsScanChunk;
+  if (b0 >= p.num_l1) return;  // reserved past the array's size
+  const uint32_t n = static_cast<uint32_t>(
+      p.num_l1 - b0 < kRsScanChunk ? p.num_l1 - b0 : kRsScanChunk);
+  const uint32_t* cnt = block_ones + p.first_tile + b0;
+  uint32_t sum = 0;
+  for (uint32_t i = tid; i < n; i += 256) sum += cnt[i];
 #pragma unroll
-  for (uint32_t off = 1; off < 64; off <<= 1) {
-    const uint64_t y = __shfl_up(incl, off, 64);
-    if (lane >= off) incl += y;
-  }
-  if (lane == 63) s_wave[wave] = incl;
+  for (uint32_t off = 32; off > 0; off >>= 1) sum += __shfl_xor(sum, off, 64);
+  if ((tid & 63) == 0) s_w[tid >> 6] = sum;
   __syncthreads();
-  uint64_t run = incl - sum, total = 0;
-  for (uint32_t w = 0; w < kRsScanThreads / 64; ++w) {
-    run += (w < wave) ? s_wave[w] : 0ull;
-    total += s_wave[w];
-  }
-  for (uint64_t b = b0; b < b1; b += 8) {
-    uint32_t x[8];
+  if (tid == 0) p.l1[b0] = static_cast<uint64_t>(s_w[0]) + s_w[1] + s_w[2] + s_w[3];
+}
+
+// one wave per array: chunk sums -> chunk bases (exclusive), and the total
+__global__ __launch_bounds__(64) void rs_chunk_scan_kernel(const RsArrayDesc* __restrict__ arrays) {
+  const RsArrayDesc p = arrays[blockIdx.x];
+  const uint32_t lane = threadIdx.x;
+  const uint64_t nch = rs_scan_chunks(p.num_l1);
+  uint64_t run = 0;
+  for (uint64_t base = 0; base < nch; base += 64) {
+    const uint64_t c = base + lane;
+    const uint64_t v = c < nch ? p.l1[c * kRsScanChunk] : 0ull;
+    uint64_t incl = v;
 #pragma unroll
-    for (int j = 0; j < 8; ++j) x[j] = b + j < b1 ? cnt[b + j] : 0u;
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      if (b + j < b1) p.l1[b + j] = run;
-      run += x[j];
+    for (uint32_t off = 1; off < 64; off <<= 1) {
+      const uint64_t y = __shfl_up(incl, off, 64);
+      if (lane >= off) incl += y;
     }
+    if (c < nch) p.l1[c * kRsScanChunk] = run + incl - v;
+    run += __shfl(incl, 63, 64);
   }
-  if (tid == 0) *p.total_ones = total;
+  if (lane == 0) *p.total_ones = run;
+}
+
+// one workgroup per chunk: the chunk's L1 entries from its base, eight
+// consecutive blocks per thread
+__global__ __launch_bounds__(256) void rs_l1_write_kernel(const RsArrayDesc* __restrict__ arrays,
+                                                          uint32_t narrays,
+                                                          const uint32_t* __restrict__ block_ones) {
+  __shared__ uint64_t s_base;
+  __shared__ uint32_t s_w[4];
+  const uint32_t tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const RsArrayDesc& p = arrays[rs_chunk_array(arrays, narrays, blockIdx.x)];
+  const uint64_t b0 = (blockIdx.x - p.first_chunk) * kRsScanChunk;
+  if (b0 >= p.num_l1) return;
+  const uint32_t n = static_cast<uint32_t>(
+      p.num_l1 - b0 < kRsScanChunk ? p.num_l1 - b0 : kRsScanChunk);
+  if (tid == 0) s_base = p.l1[b0];
+  const uint32_t* cnt = block_ones + p.first_tile + b0;
+  const uint32_t i0 = tid * 8;
+  uint32_t x[8], mine = 0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    x[j] = i0 + j < n ? cnt[i0 + j] : 0u;
+    mine += x[j];
+  }
+  const uint32_t incl = detail::wave_incl_scan_u32(mine);
+  if (lane == 63) s_w[wave] = incl;
+  __syncthreads();  // the base is read before any entry of the chunk is written
+  uint64_t run = s_base + incl - mine;
+  for (uint32_t w = 0; w < wave; ++w) run += s_w[w];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) {
+    if (i0 + j < n) p.l1[b0 + i0 + j] = run;
+    run += x[j];
+  }
 }
 
 // position of the r-th (1-based) `value` bit inside the 8 words at q (one
@@ -713,10 +758,10 @@ void launch_histogram(const HistParams& p, hipStream_t s) {
 }
 
 void launch_rs_build(const RsArrayDesc* arrays, uint64_t narrays, uint64_t nblocks,
-                     uint32_t* scratch, hipStream_t s) {
+                     uint64_t nchunks, uint32_t* scratch, hipStream_t s) {
   if (nblocks == 0) return;
   const uint64_t groups = (nblocks + kRsWaves - 1) / kRsWaves;
-  if (groups > 0x7FFFFFFFull || narrays > 0x7FFFFFFFull)
+  if (groups > 0x7FFFFFFFull || narrays > 0x7FFFFFFFull || nchunks > 0x7FFFFFFFull)
     throw Error(ECL_ERROR_INVALID_ARGUMENT, "bit arrays too large");
   uint32_t* block_array = scratch;
   uint32_t* block_ones = scratch + nblocks;
@@ -727,8 +772,12 @@ void launch_rs_build(const RsArrayDesc* arrays, uint64_t narrays, uint64_t nbloc
   hipLaunchKernelGGL(rs_local_kernel, dim3(static_cast<uint32_t>(groups)), dim3(512), 0, s,
                      arrays, block_array, nblocks, block_ones);
   ECL_HIP(hipGetLastError());
-  hipLaunchKernelGGL(rs_scan_kernel, dim3(static_cast<uint32_t>(narrays)), dim3(kRsScanThreads),
-                     0, s, arrays, block_ones);
+  const uint32_t na = static_cast<uint32_t>(narrays);
+  hipLaunchKernelGGL(rs_chunk_sum_kernel, dim3(static_cast<uint32_t>(nchunks)), dim3(256), 0, s,
+                     arrays, na, block_ones);
+  hipLaunchKernelGGL(rs_chunk_scan_kernel, dim3(na), dim3(64), 0, s, arrays);
+  hipLaunchKernelGGL(rs_l1_write_kernel, dim3(static_cast<uint32_t>(nchunks)), dim3(256), 0, s,
+                     arrays, na, block_ones);
   ECL_HIP(hipGetLastError());
   hipLaunchKernelGGL(rs_samples_kernel, dim3(static_cast<uint32_t>(groups)), dim3(512), 0, s,
                      arrays, block_array, nblocks);

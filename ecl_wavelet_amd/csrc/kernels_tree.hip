@@ -300,6 +300,76 @@ __global__ __launch_bounds__(kPassThreads) void tile_hist_kernel(PassParams p) {
   for (uint32_t d = tid; d < row; d += kPassThreads) p.hist[t * row + d] = d < nb ? s_tot[d] : 0u;
 }
 
+// Tile histogram of 1- or 2-bit digits of a uint8 text that is its own code
+// (identity map, digit = symbol & (2^k - 1)): per 32-bit word the digit's
+// low and high bits are bit 0 and bit 1 of every byte, so the counts of
+// digits 1, 2 and 3 are popcounts of (lo & ~hi), (hi & ~lo) and (hi & lo);
+// digit 0 takes the rest of the tile.  Counts stay in registers (no LDS
+// atomics) until one wave reduction per workgroup.
+__global__ __launch_bounds__(kPassThreads) void tile_hist_small_kernel(PassParams p) {
+  __shared__ uint32_t s_tot[4];
+  const uint32_t tid = threadIdx.x, lane = tid & 63;
+  if (tid < 4) s_tot[tid] = 0;
+  const uint64_t t = blockIdx.x;
+  const TileGeo g = tile_geo(p, t, p.tile_codes);
+  const uint8_t* in = static_cast<const uint8_t*>(p.src);
+  const uint32_t hi_mask = p.k >= 2 ? 0x02020202u : 0u;
+  uint32_t c1 = 0, c2 = 0, c3 = 0;
+  auto count = [&](const uint4 q) {
+    const uint32_t w[4] = {q.x, q.y, q.z, q.w};
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const uint32_t lo = w[j] & 0x01010101u, hi = (w[j] & hi_mask) >> 1;
+      c1 += __popc(lo & ~hi);
+      c2 += __popc(hi & ~lo);
+      c3 += __popc(hi & lo);
+    }
+  };
+  const uint32_t nchunks = (g.len + 15) / 16;
+  constexpr uint32_t U = 4;
+  uint32_t c0 = tid;
+  for (; c0 + (U - 1) * kPassThreads < nchunks; c0 += U * kPassThreads) {
+    uint4 q[U];
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) {
+      const uint32_t c = c0 + u * kPassThreads;
+      const uint64_t e0 = g.start + static_cast<uint64_t>(c) * 16;
+      const uint32_t valid = (g.len - c * 16 < 16) ? g.len - c * 16 : 16;
+      q[u] = valid == 16 ? *reinterpret_cast<const uint4*>(in + e0)
+                         : load_partial_chunk<uint8_t>(in, e0, valid);
+    }
+#pragma unroll
+    for (uint32_t u = 0; u < U; ++u) count(q[u]);
+  }
+  for (uint32_t c = c0; c < nchunks; c += kPassThreads) {
+    const uint64_t e0 = g.start + static_cast<uint64_t>(c) * 16;
+    const uint32_t valid = (g.len - c * 16 < 16) ? g.len - c * 16 : 16;
+    count(valid == 16 ? *reinterpret_cast<const uint4*>(in + e0)
+                      : load_partial_chunk<uint8_t>(in, e0, valid));
+  }
+#pragma unroll
+  for (uint32_t off = 32; off > 0; off >>= 1) {
+    c1 += __shfl_xor(c1, off, 64);
+    c2 += __shfl_xor(c2, off, 64);
+    c3 += __shfl_xor(c3, off, 64);
+  }
+  __syncthreads();
+  if (lane == 0) {
+    atomicAdd(&s_tot[1], c1);
+    atomicAdd(&s_tot[2], c2);
+    atomicAdd(&s_tot[3], c3);
+  }
+  __syncthreads();
+  const uint32_t nb = 1u << p.k;
+  const uint32_t row = p.row_bins ? p.row_bins : nb;
+  for (uint32_t d = tid; d < row; d += kPassThreads) {
+    uint32_t v = 0;
+    if (d == 0) v = g.len - s_tot[1] - s_tot[2] - s_tot[3];
+    else if (d < nb) v = s_tot[d];
+    p.hist[t * row + d] = v;
+  }
+}
+
 // ---------------------------------------------------------------------------
 // Exclusive column scan of the tile histograms, in blocks of 256 tiles:
 //  partial[b][d] = sum of block b  ->  exclusive over blocks  ->  excl rows.
@@ -918,10 +988,12 @@ void set_pass_tiles_per_wg(uint32_t span) { g_pass_span.store(span); }
 uint32_t pass_span(uint64_t num_tiles) {
   const uint32_t forced = g_pass_span.load();
   if (forced != 0) return forced;
-  // spans of up to 8 tiles while the grid still gives every CU about four
-  // rounds of resident workgroups (256 CUs, 3 per CU)
+  // spans of up to 16 tiles while the grid still gives every CU about four
+  // rounds of resident workgroups (256 CUs, 3 per CU); at 6e9 symbols 16
+  // tiles took the two-level pass from 3.99 to 3.79 ms and left the
+  // eight-level pass as it was (8 -> 16: 15.06 -> 15.15 ms)
   const uint64_t span = num_tiles / (256 * 3 * 4);
-  return static_cast<uint32_t>(span < 1 ? 1 : span > 8 ? 8 : span);
+  return static_cast<uint32_t>(span < 1 ? 1 : span > 16 ? 16 : span);
 }
 
 void launch_span_words(const PassParams& p, hipStream_t s) {
@@ -943,6 +1015,12 @@ void launch_tile_hist(const PassParams& p, hipStream_t s) {
   ProfScope ps(K_TILE_HIST, s);
   const dim3 grid(static_cast<uint32_t>(p.num_tiles)), block(kPassThreads);
   const bool remap = p.codes_out != nullptr;
+  if (p.k <= 2 && p.dshift == 0 && p.src_esize == 1 && p.cw == 1 && !remap && p.map_kind == 0 &&
+      p.node_remap == nullptr) {
+    hipLaunchKernelGGL(tile_hist_small_kernel, grid, block, 0, s, p);
+    ECL_HIP(hipGetLastError());
+    return;
+  }
 #define ECL_TH(TIN, CWV, MK)                                                                 \
   do {                                                                                       \
     if (remap) hipLaunchKernelGGL((tile_hist_kernel<TIN, CWV, MK, true>), grid, block, 0, s, p); \

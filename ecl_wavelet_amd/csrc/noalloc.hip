@@ -100,13 +100,14 @@ ecl_status ecl_rank_select_build_directory_noalloc(int device, const uint32_t* d
     d.cap = slots;
     d.total_ones = d_total_ones;
     d.first_tile = 0;
+    d.first_chunk = 0;
     RsArrayDesc* d_desc = reinterpret_cast<RsArrayDesc*>(base);
     uint32_t* d_blk = reinterpret_cast<uint32_t*>(base + kDescBytes);
     // the descriptor travels as a kernel argument (no staging allocation)
     Uploader up(s);
     up.add(d_desc, &d, 1);
     up.flush();
-    launch_rs_build(d_desc, 1, num_l1, d_blk, s);
+    launch_rs_build(d_desc, 1, num_l1, rs_scan_chunks(num_l1), d_blk, s);
   });
 }
 

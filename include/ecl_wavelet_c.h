@@ -121,7 +121,7 @@ ecl_status ecl_device_mem_info(int device, size_t* free_bytes, size_t* total_byt
  * memory back outside the library. */
 ecl_status ecl_release_cached_memory(int device);
 /* Tiles of 32768 codes each construction workgroup takes in turn (0:
- * automatic, up to 8 for large texts).  A tuning knob with no effect on
+ * automatic, up to 16 for large texts).  A tuning knob with no effect on
  * results; no counterpart in the reference. */
 ecl_status ecl_set_build_span(uint32_t tiles_per_workgroup);
 /* Lanes per access/rank query in the tree query kernels: 0 (default,
