@@ -5,8 +5,10 @@
 // select samples of one array share a buffer of floor(size/4096) slots, ones
 // from the front and zeros from the back, which getSelectSample hides.
 // rank/select take the reference's template arguments; NumThreads may be
-// any power of two up to the wave width (64): the lanes of a group compute
-// the answer together and every lane returns it.
+// any power of two up to the wave width (64): a group is NumThreads
+// consecutive lanes of one wave, all active and calling with the same
+// arguments; each lane reads its share of the block (or line), the group
+// combines the counts over shuffles, and every lane returns the answer.
 #pragma once
 
 #include "ecl_wavelet/bitarray/bit_array.hpp"
@@ -68,7 +70,8 @@ class RankSelect {
     static_assert(NumThreads > 0 && NumThreads <= WS && (WS % NumThreads) == 0,
                   "NumThreads must divide the wave width");
     bool bit = false;
-    const size_t r = detail::level_rank<Value>(levels_[array_index], index, GetBit ? &bit : nullptr);
+    const size_t r = detail::level_rank_coop<Value, NumThreads>(levels_[array_index], index,
+                                                                GetBit ? &bit : nullptr);
     if constexpr (GetBit) {
       return RankResult{r, bit};
     } else {
@@ -83,7 +86,7 @@ class RankSelect {
     static_assert(Value == 0 || Value == 1, "Value must be 0 or 1.");
     static_assert(NumThreads > 0 && NumThreads <= WS && (WS % NumThreads) == 0,
                   "NumThreads must divide the wave width");
-    return detail::level_select<Value, true>(levels_[array_index], i);
+    return detail::level_select_coop<Value, NumThreads>(levels_[array_index], i);
   }
 
   // ---- directory accessors (rank_select.cuh:530-701)

@@ -563,6 +563,199 @@ __device__ __forceinline__ uint64_t level_select(const LevelView& v,
 }
 
 // ---------------------------------------------------------------------------
+// Cooperative forms for groups of G lanes: G is a power of two up to the
+// wave width, a group is G consecutive lanes of one wave (threadIdx.x % G
+// numbers a lane inside it), as the reference's NumThreads groups
+// (rank_select.cuh:125-208).  Every lane of a group calls with the same
+// arguments and gets the same result; each lane reads only its share of the
+// 64-byte block or 128-byte line, and the group combines the counts.
+// ---------------------------------------------------------------------------
+template <int G>
+__device__ __forceinline__ uint32_t group_lane() {
+  return static_cast<uint32_t>(threadIdx.x) & (G - 1);
+}
+template <int G, typename V>
+__device__ __forceinline__ V group_sum(V v) {
+#pragma unroll
+  for (int off = 1; off < G; off <<= 1) v += __shfl_xor(v, off, 64);
+  return v;
+}
+template <int G>
+__device__ __forceinline__ uint32_t group_excl_scan(uint32_t v) {
+  const uint32_t t = group_lane<G>();
+  uint32_t incl = v;
+#pragma unroll
+  for (int off = 1; off < G; off <<= 1) {
+    const uint32_t y = __shfl_up(incl, off, G);
+    if (t >= static_cast<uint32_t>(off)) incl += y;
+  }
+  return incl - v;
+}
+// the value of the group's lane where `owner` holds (the first one), and
+// whether there is one
+template <int G>
+__device__ __forceinline__ uint64_t group_from(bool owner, uint64_t val, bool* any) {
+  const uint64_t m = static_cast<uint64_t>(__ballot(owner));
+  const uint32_t base = (static_cast<uint32_t>(threadIdx.x) & 63u) & ~static_cast<uint32_t>(G - 1);
+  const uint64_t gm = G == 64 ? m : (m >> base) & ((1ull << (G & 63)) - 1ull);
+  *any = gm != 0;
+  const int src = static_cast<int>(base) + (gm ? __ffsll(static_cast<long long>(gm)) - 1 : 0);
+  return __shfl(val, src, 64);
+}
+
+// ones in [0, index) for index in (0, size), lanes splitting the words
+template <int G>
+__device__ __forceinline__ uint64_t rank1_coop(const LevelView& v, uint64_t index, bool* bit) {
+  const uint32_t t = group_lane<G>();
+  const uint64_t* w;
+  uint32_t nw, wi, bi;
+  uint64_t base;
+  if (v.lines != nullptr) {
+    const uint64_t line = index / kLineBits;
+    const uint32_t off = static_cast<uint32_t>(index - line * kLineBits);
+    const uint64_t* lp = reinterpret_cast<const uint64_t*>(v.lines + line * 8);
+    base = gld(lp, 0);
+    w = lp + 1;
+    nw = 15;
+    wi = off >> 6;
+    bi = off & 63;
+  } else {
+    const uint64_t blk = index >> 9;
+    w = reinterpret_cast<const uint64_t*>(v.words + (blk << 4));
+    nw = 8;
+    wi = static_cast<uint32_t>((index >> 6) & 7);
+    bi = static_cast<uint32_t>(index & 63);
+    base = gld(v.l1, index >> 16) + gld(v.l2, blk);
+  }
+  uint32_t acc = 0, b = 0;
+  for (uint32_t k = t; k < nw && k <= wi; k += G) {
+    uint64_t x = gld(w, k);
+    if (k == wi) {
+      b = static_cast<uint32_t>((x >> bi) & 1u);
+      x &= bi == 0 ? 0ull : (~0ull >> (64 - bi));
+    }
+    acc += static_cast<uint32_t>(__popcll(x));
+  }
+  acc = group_sum<G>(acc);
+  b = group_sum<G>(b);
+  *bit = b != 0;
+  return base + acc;
+}
+
+// level_rank<Value> by a group of G lanes
+template <int Value, int G>
+__device__ __forceinline__ uint64_t level_rank_coop(const LevelView& v, uint64_t index,
+                                                    bool* bit = nullptr) {
+  if constexpr (G == 1) return level_rank<Value>(v, index, bit);
+  bool b = false;
+  uint64_t ones;
+  if (index == 0) {
+    ones = 0;
+    if (bit) b = level_bit(v, 0);
+  } else if (index >= v.size) {
+    ones = v.ones;
+    index = v.size;
+  } else {
+    ones = rank1_coop<G>(v, index, &b);
+  }
+  if (bit) *bit = b;
+  return Value ? ones : index - ones;
+}
+
+// position of the r-th (1-based) Value in the nw words from w (the first
+// word's bits below off dropped), the words split over the group in
+// contiguous shares; `none` when the words hold fewer than r
+template <int Value, int G>
+__device__ __forceinline__ uint64_t group_select_words(const uint64_t* w, uint32_t nw,
+                                                       uint32_t off, uint64_t r,
+                                                       uint64_t pos0, uint64_t none) {
+  const uint32_t t = group_lane<G>();
+  const uint32_t k0 = t * nw / G, k1 = (t + 1) * nw / G;
+  uint32_t cnt = 0;
+  for (uint32_t k = k0; k < k1; ++k) {
+    uint64_t x = Value ? gld(w, k) : ~gld(w, k);
+    if (k == 0) x &= ~0ull << off;
+    cnt += static_cast<uint32_t>(__popcll(x));
+  }
+  const uint32_t ex = group_excl_scan<G>(cnt);
+  const bool own = ex < r && r <= static_cast<uint64_t>(ex) + cnt;
+  uint64_t pos = none;
+  if (own) {
+    uint64_t rr = r - ex;
+    for (uint32_t k = k0; k < k1; ++k) {
+      uint64_t x = Value ? gld(w, k) : ~gld(w, k);
+      if (k == 0) x &= ~0ull << off;
+      const uint32_t c = static_cast<uint32_t>(__popcll(x));
+      if (rr <= c) {
+        pos = pos0 + 64ull * k + select_in_word(x, static_cast<uint32_t>(rr));
+        break;
+      }
+      rr -= c;
+    }
+  }
+  bool any = false;
+  pos = group_from<G>(own, pos, &any);
+  return any ? pos : none;
+}
+
+// level_select<Value> by a group of G lanes: the select line when the array
+// has one (its 960 bits split over the lanes), otherwise the directory: the
+// sample bracket and the L1 search in every lane, the bracket's L2 entries
+// and the final 64-byte block split over the lanes
+template <int Value, int G>
+__device__ __forceinline__ uint64_t level_select_coop(const LevelView& v, uint64_t i) {
+  if constexpr (G == 1) return level_select<Value, true>(v, i);
+  const uint64_t total = Value ? v.ones : v.size - v.ones;
+  if (i == 0 || i > total) return v.size;
+  const uint32_t rate = v.sel_rate[Value];
+  if (v.sel_lines != nullptr && rate != 0) {
+    const uint64_t jj = rate == 1 ? i : __umul64hi(i, v.sel_magic[Value]);
+    const uint64_t* lp =
+        reinterpret_cast<const uint64_t*>(v.sel_lines + ((Value ? 0 : v.sel_zero_line) + jj) * 8);
+    const uint64_t s = gld(lp, 0);
+    const uint64_t r = jj ? i - jj * rate + 1 : i;
+    const uint64_t res = group_select_words<Value, G>(lp + 1, 15, static_cast<uint32_t>(s & 63),
+                                                      r, s & ~63ull, v.size);
+    if (res < v.size) return res;
+  }
+  const uint64_t j = i / kSampleRate;
+  const uint64_t nsamp = total / kSampleRate;
+  uint64_t lo_pos = 0, hi_pos = v.size - 1;
+  if (j > 0) {
+    lo_pos = gld(v.samples, Value ? j - 1 : v.sample_cap - j);
+    if (i == j * kSampleRate) return lo_pos;
+  }
+  if (j < nsamp) hi_pos = gld(v.samples, Value ? j : v.sample_cap - 1 - j);
+  uint64_t b_lo = lo_pos >> 16, b_hi = hi_pos >> 16;
+  while (b_lo < b_hi) {
+    const uint64_t mid = (b_lo + b_hi + 1) >> 1;
+    const uint64_t e = gld(v.l1, mid);
+    const uint64_t c = Value ? e : (mid << 16) - e;
+    if (c < i) b_lo = mid; else b_hi = mid - 1;
+  }
+  const uint64_t blk1 = b_lo;
+  const uint64_t e1 = gld(v.l1, blk1);
+  const uint64_t r = i - (Value ? e1 : (blk1 << 16) - e1);
+  uint64_t k_lo = lo_pos >> 9, k_hi = hi_pos >> 9;
+  const uint64_t first = blk1 << 7, last = first + kL2PerL1 - 1;
+  if (k_lo < first) k_lo = first;
+  if (k_hi > last) k_hi = last;
+  // blocks k_lo + u (0 < u <= span) with fewer than r Values before them
+  // (the counts grow with u): counted by the lanes together
+  uint32_t below = 0;
+  for (uint64_t u = 1 + group_lane<G>(); u <= k_hi - k_lo; u += G) {
+    const uint64_t k = k_lo + u;
+    const uint64_t e = gld(v.l2, k);
+    below += (Value ? e : ((k - first) << 9) - e) < r ? 1u : 0u;
+  }
+  const uint64_t blk2 = k_lo + group_sum<G>(below);
+  const uint64_t e2 = gld(v.l2, blk2);
+  const uint64_t rr = r - (Value ? e2 : ((blk2 - first) << 9) - e2);
+  return group_select_words<Value, G>(reinterpret_cast<const uint64_t*>(v.words + (blk2 << 4)),
+                                      8, 0, rr, blk2 << 9, v.size);
+}
+
+// ---------------------------------------------------------------------------
 // Wavelet tree view (minimal-alphabet symbols), trivially copyable.
 // ---------------------------------------------------------------------------
 struct TreeView {
@@ -625,7 +818,7 @@ __device__ __forceinline__ uint64_t tree_node_pos(const TreeView& t,
 // access: top-down descent (tree/wavelet_tree.cuh:1815-1877).  Pair: the
 // two lanes of an adjacent pair descend for the same index together, each
 // fetching half of every level's bits (rank1_pair).
-template <bool Pair = false>
+template <bool Pair = false, int G = 1>
 __device__ __forceinline__ uint64_t tree_access(const TreeView& t,
                                                 const LevelView* lv,
                                                 const uint64_t* counts,
@@ -643,7 +836,7 @@ __device__ __forceinline__ uint64_t tree_access(const TreeView& t,
     const uint64_t start = lo == 0 ? 0 : ranks[rs + tree_node_pos(t, lo, l)];
     bool bit;
     const uint64_t pos = Pair ? level_rank_pair<0>(v, cc + index, &bit)
-                              : level_rank<0>(v, cc + index, &bit);
+                              : level_rank_coop<0, G>(v, cc + index, &bit);
     const uint64_t diff = prev_pow2(hi - lo + 1);
     if (!bit) {
       index = pos - start;
@@ -659,7 +852,7 @@ __device__ __forceinline__ uint64_t tree_access(const TreeView& t,
 
 // rank: rankKernel early-out + top-down descent (wavelet_tree.cuh:481-486,
 // 1896-1956).  Pair as for tree_access.
-template <bool Pair = false>
+template <bool Pair = false, int G = 1>
 __device__ __forceinline__ uint64_t tree_rank(const TreeView& t,
                                               const LevelView* lv,
                                               const uint64_t* counts,
@@ -674,7 +867,7 @@ __device__ __forceinline__ uint64_t tree_rank(const TreeView& t,
     const uint64_t cc = counts[lo];
     const uint64_t start = lo == 0 ? 0 : ranks[rs + tree_node_pos(t, lo, l)];
     const uint64_t pos = Pair ? level_rank_pair<0>(lv[l], cc + result)
-                              : level_rank<0>(lv[l], cc + result);
+                              : level_rank_coop<0, G>(lv[l], cc + result);
     const uint64_t split = lo + prev_pow2(hi - lo + 1);
     if (symbol < split) {
       result = pos - start;
@@ -706,7 +899,7 @@ __device__ __forceinline__ uint64_t tree_prev_char_start(
 
 // select: selectKernel early-out + bottom-up ascent (wavelet_tree.cuh:
 // 532-535, 2007-2079).  SelLines as for level_select.
-template <bool SelLines = true, bool Hints = true>
+template <bool SelLines = true, bool Hints = true, int G = 1>
 __device__ __forceinline__ uint64_t tree_select(const TreeView& t,
                                                 const LevelView* lv,
                                                 const uint64_t* counts,
@@ -739,12 +932,14 @@ __device__ __forceinline__ uint64_t tree_select(const TreeView& t,
       start = cs == 0 ? 0
                       : counts[cs] - ranks[rs + tree_node_pos(
                                                     t, cs, static_cast<uint32_t>(l))];
-      result = level_select<1, false, SelLines, Hints>(v, start + result) + 1;
+      if constexpr (G > 1) result = level_select_coop<1, G>(v, start + result) + 1;
+      else result = level_select<1, false, SelLines, Hints>(v, start + result) + 1;
     } else {
       start = cs == 0
                   ? 0
                   : ranks[rs + tree_node_pos(t, cs, static_cast<uint32_t>(l))];
-      result = level_select<0, false, SelLines, Hints>(v, start + result) + 1;
+      if constexpr (G > 1) result = level_select_coop<0, G>(v, start + result) + 1;
+      else result = level_select<0, false, SelLines, Hints>(v, start + result) + 1;
     }
     if (static_cast<uint32_t>(l) == clen - 1 && result > v.size) {
       result = t.n + 1;

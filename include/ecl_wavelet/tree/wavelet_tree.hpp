@@ -11,7 +11,9 @@
 //  * errors other than out-of-memory throw std::runtime_error instead of
 //    printing and calling exit();
 //  * ThreadsPerQuery may be any power of two dividing the wave width (64);
-//    the group's lanes compute the answer together and all return it;
+//    a group is that many consecutive lanes of one wave, all active and
+//    calling with the same query; its lanes split each level's rank or
+//    select and all return the answer;
 //  * the `offsets` LDS argument of deviceAccess / deviceRank is accepted and
 //    ignored (level descriptors carry pointers, not word offsets).
 #pragma once
@@ -151,23 +153,25 @@ class WaveletTree {
   __device__ T deviceAccess(size_t index, bool const /*is_pow_two*/, size_t* counts = nullptr,
                             size_t* /*offsets*/ = nullptr, size_t* ranks = nullptr) {
     check_tpq<ThreadsPerQuery>();
-    return static_cast<T>(detail::tree_access(view_, view_.levels, counts_or(counts),
-                                              ranks_or(ranks), view_.nodes_at_level, index));
+    return static_cast<T>(detail::tree_access<false, ThreadsPerQuery>(
+        view_, view_.levels, counts_or(counts), ranks_or(ranks), view_.nodes_at_level, index));
   }
   template <int ThreadsPerQuery>
   __device__ size_t deviceRank(RankSelectQuery<T> const& query, bool const /*is_pow_two*/,
                                size_t* counts = nullptr, size_t* /*offsets*/ = nullptr,
                                size_t* ranks = nullptr) {
     check_tpq<ThreadsPerQuery>();
-    return detail::tree_rank(view_, view_.levels, counts_or(counts), ranks_or(ranks),
-                             view_.nodes_at_level, query.symbol_, query.index_);
+    return detail::tree_rank<false, ThreadsPerQuery>(view_, view_.levels, counts_or(counts),
+                                                     ranks_or(ranks), view_.nodes_at_level,
+                                                     query.symbol_, query.index_);
   }
   template <int ThreadsPerQuery>
   __device__ size_t deviceSelect(RankSelectQuery<T> const& query, bool const /*is_pow_two*/,
                                  size_t* ranks = nullptr) {
     check_tpq<ThreadsPerQuery>();
-    return detail::tree_select(view_, view_.levels, view_.counts, ranks_or(ranks),
-                               view_.nodes_at_level, query.symbol_, query.index_);
+    return detail::tree_select<true, true, ThreadsPerQuery>(view_, view_.levels, view_.counts,
+                                                            ranks_or(ranks), view_.nodes_at_level,
+                                                            query.symbol_, query.index_);
   }
 
   // encode (wavelet_tree.cuh:2087-2091)

@@ -53,15 +53,32 @@ __global__ void writeWordKernel(BitArray ba, size_t a, size_t i, uint32_t v) { b
 __global__ void accessKernel(BitArray ba, size_t a, size_t i, size_t* out) { *out = ba.access(a, i); }
 __global__ void twoWordsKernel(BitArray ba, size_t a, size_t i, size_t* out) { *out = ba.twoWords(a, i); }
 __global__ void partialWordKernel(BitArray ba, uint32_t w, uint8_t b, size_t* out) { *out = ba.partialWord(w, b); }
+// one group of NT lanes; every lane writes what it got
 template <int NT, int V>
 __global__ void rankKernel(RankSelect rs, uint32_t a, size_t i, size_t* out) {
   RankResult r = rs.rank<NT, true, V>(a, i, rs.bit_array_.getOffset(a));
-  if (threadIdx.x == 0) *out = r.rank;
+  out[threadIdx.x] = r.rank;
 }
 template <int NT, int V>
 __global__ void selectKernel(RankSelect rs, uint32_t a, size_t i, size_t* out) {
-  size_t r = rs.select<V, NT>(a, i, rs.bit_array_.getOffset(a));
-  if (threadIdx.x == 0) *out = r;
+  out[threadIdx.x] = rs.select<V, NT>(a, i, rs.bit_array_.getOffset(a));
+}
+// many groups of NT lanes, group g answering query g (test_rank_select.cu:
+// 21-111 launch one group per query the same way); lane t of group g writes
+// out[g * NT + t]
+template <int NT, int V>
+__global__ void rankGroups(RankSelect rs, uint32_t a, const size_t* idx, size_t m, size_t* out) {
+  const size_t step = size_t(gridDim.x) * blockDim.x / NT;
+  for (size_t g = (size_t(blockIdx.x) * blockDim.x + threadIdx.x) / NT; g < m; g += step) {
+    RankResult r = rs.rank<NT, true, V>(a, idx[g], rs.bit_array_.getOffset(a));
+    out[g * NT + threadIdx.x % NT] = r.rank;
+  }
+}
+template <int NT, int V>
+__global__ void selectGroups(RankSelect rs, uint32_t a, const size_t* idx, size_t m, size_t* out) {
+  const size_t step = size_t(gridDim.x) * blockDim.x / NT;
+  for (size_t g = (size_t(blockIdx.x) * blockDim.x + threadIdx.x) / NT; g < m; g += step)
+    out[g * NT + threadIdx.x % NT] = rs.select<V, NT>(a, idx[g], rs.bit_array_.getOffset(a));
 }
 __global__ void l1Kernel(RankSelect rs, uint32_t a, size_t i, size_t* out) { *out = rs.getL1Entry(a, i); }
 __global__ void l2Kernel(RankSelect rs, uint32_t a, size_t i, size_t* out) { *out = rs.getL2Entry(a, i); }
@@ -74,21 +91,28 @@ __global__ void sampleKernel(RankSelect rs, uint32_t a, size_t j, bool v, size_t
 __global__ void totalKernel(RankSelect rs, uint32_t a, bool v, size_t* out) {
   *out = v ? rs.getTotalNumVals<1>(a) : rs.getTotalNumVals<0>(a);
 }
-// user kernels over a WaveletTree passed by value (README.md:5)
-template <typename T>
+// user kernels over a WaveletTree passed by value (README.md:5), TPQ lanes
+// per query: lane t of the group answering query g writes out[g * TPQ + t]
+template <typename T, int TPQ>
 __global__ void userAccess(WaveletTree<T> wt, const size_t* idx, size_t m, size_t* out) {
-  for (size_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x)
-    out[i] = wt.template deviceAccess<1>(idx[i], utils::isPowTwo(wt.getAlphabetSize()));
+  const size_t step = size_t(gridDim.x) * blockDim.x / TPQ;
+  for (size_t g = (size_t(blockIdx.x) * blockDim.x + threadIdx.x) / TPQ; g < m; g += step)
+    out[g * TPQ + threadIdx.x % TPQ] =
+        wt.template deviceAccess<TPQ>(idx[g], utils::isPowTwo(wt.getAlphabetSize()));
 }
-template <typename T>
+template <typename T, int TPQ>
 __global__ void userRank(WaveletTree<T> wt, const RankSelectQuery<T>* q, size_t m, size_t* out) {
-  for (size_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x)
-    out[i] = wt.template deviceRank<1>(q[i], utils::isPowTwo(wt.getAlphabetSize()));
+  const size_t step = size_t(gridDim.x) * blockDim.x / TPQ;
+  for (size_t g = (size_t(blockIdx.x) * blockDim.x + threadIdx.x) / TPQ; g < m; g += step)
+    out[g * TPQ + threadIdx.x % TPQ] =
+        wt.template deviceRank<TPQ>(q[g], utils::isPowTwo(wt.getAlphabetSize()));
 }
-template <typename T>
+template <typename T, int TPQ>
 __global__ void userSelect(WaveletTree<T> wt, const RankSelectQuery<T>* q, size_t m, size_t* out) {
-  for (size_t i = blockIdx.x * blockDim.x + threadIdx.x; i < m; i += gridDim.x * blockDim.x)
-    out[i] = wt.template deviceSelect<1>(q[i], utils::isPowTwo(wt.getAlphabetSize()));
+  const size_t step = size_t(gridDim.x) * blockDim.x / TPQ;
+  for (size_t g = (size_t(blockIdx.x) * blockDim.x + threadIdx.x) / TPQ; g < m; g += step)
+    out[g * TPQ + threadIdx.x % TPQ] =
+        wt.template deviceSelect<TPQ>(q[g], utils::isPowTwo(wt.getAlphabetSize()));
 }
 template <typename T, bool P2>
 __global__ void nodePosKernel(WaveletTree<T> wt, T s, uint8_t l, size_t* out) {
@@ -103,29 +127,65 @@ using namespace ecl;
 
 static size_t* g_res = nullptr;
 
-static size_t run_rank(RankSelect& rs, int nt, int v, uint32_t a, size_t i) {
-  switch (nt * 2 + v) {
-    case 2: rankKernel<1, 0><<<1, 1>>>(rs, a, i, g_res); break;
-    case 3: rankKernel<1, 1><<<1, 1>>>(rs, a, i, g_res); break;
-    case 64: rankKernel<32, 0><<<1, 32>>>(rs, a, i, g_res); break;
-    case 65: rankKernel<32, 1><<<1, 32>>>(rs, a, i, g_res); break;
-    case 128: rankKernel<64, 0><<<1, 64>>>(rs, a, i, g_res); break;
-    default: rankKernel<64, 1><<<1, 64>>>(rs, a, i, g_res); break;
+static const int kGroupSizes[] = {1, 2, 4, 8, 16, 32, 64};
+
+// one group of nt lanes; every lane must return the same answer
+template <int NT>
+static void launch_one(bool sel, RankSelect& rs, int v, uint32_t a, size_t i) {
+  if (sel) {
+    if (v) selectKernel<NT, 1><<<1, NT>>>(rs, a, i, g_res);
+    else selectKernel<NT, 0><<<1, NT>>>(rs, a, i, g_res);
+  } else {
+    if (v) rankKernel<NT, 1><<<1, NT>>>(rs, a, i, g_res);
+    else rankKernel<NT, 0><<<1, NT>>>(rs, a, i, g_res);
+  }
+}
+static size_t run_one(bool sel, RankSelect& rs, int nt, int v, uint32_t a, size_t i) {
+  switch (nt) {
+    case 1: launch_one<1>(sel, rs, v, a, i); break;
+    case 2: launch_one<2>(sel, rs, v, a, i); break;
+    case 4: launch_one<4>(sel, rs, v, a, i); break;
+    case 8: launch_one<8>(sel, rs, v, a, i); break;
+    case 16: launch_one<16>(sel, rs, v, a, i); break;
+    case 32: launch_one<32>(sel, rs, v, a, i); break;
+    default: launch_one<64>(sel, rs, v, a, i); break;
   }
   kernelCheck();
-  return *g_res;
+  for (int t = 1; t < nt; ++t) EXPECT_EQ(g_res[t], g_res[0]);
+  return g_res[0];
+}
+static size_t run_rank(RankSelect& rs, int nt, int v, uint32_t a, size_t i) {
+  return run_one(false, rs, nt, v, a, i);
 }
 static size_t run_select(RankSelect& rs, int nt, int v, uint32_t a, size_t i) {
-  switch (nt * 2 + v) {
-    case 2: selectKernel<1, 0><<<1, 1>>>(rs, a, i, g_res); break;
-    case 3: selectKernel<1, 1><<<1, 1>>>(rs, a, i, g_res); break;
-    case 64: selectKernel<32, 0><<<1, 32>>>(rs, a, i, g_res); break;
-    case 65: selectKernel<32, 1><<<1, 32>>>(rs, a, i, g_res); break;
-    case 128: selectKernel<64, 0><<<1, 64>>>(rs, a, i, g_res); break;
-    default: selectKernel<64, 1><<<1, 64>>>(rs, a, i, g_res); break;
+  return run_one(true, rs, nt, v, a, i);
+}
+
+// m queries in groups of nt lanes over 16 workgroups of 64 lanes; out holds
+// m * nt answers
+template <int NT>
+static void launch_groups(bool sel, RankSelect& rs, int v, const size_t* d_idx, size_t m,
+                          size_t* d_out) {
+  if (sel) {
+    if (v) selectGroups<NT, 1><<<16, 64>>>(rs, 0, d_idx, m, d_out);
+    else selectGroups<NT, 0><<<16, 64>>>(rs, 0, d_idx, m, d_out);
+  } else {
+    if (v) rankGroups<NT, 1><<<16, 64>>>(rs, 0, d_idx, m, d_out);
+    else rankGroups<NT, 0><<<16, 64>>>(rs, 0, d_idx, m, d_out);
+  }
+}
+static void run_groups(bool sel, RankSelect& rs, int nt, int v, const size_t* d_idx, size_t m,
+                       size_t* d_out) {
+  switch (nt) {
+    case 1: launch_groups<1>(sel, rs, v, d_idx, m, d_out); break;
+    case 2: launch_groups<2>(sel, rs, v, d_idx, m, d_out); break;
+    case 4: launch_groups<4>(sel, rs, v, d_idx, m, d_out); break;
+    case 8: launch_groups<8>(sel, rs, v, d_idx, m, d_out); break;
+    case 16: launch_groups<16>(sel, rs, v, d_idx, m, d_out); break;
+    case 32: launch_groups<32>(sel, rs, v, d_idx, m, d_out); break;
+    default: launch_groups<64>(sel, rs, v, d_idx, m, d_out); break;
   }
   kernelCheck();
-  return *g_res;
 }
 
 static void test_bit_array() {
@@ -182,7 +242,7 @@ static void test_rank_select() {
       size_t m = rs.bit_array_.size(i);
       EXPECT_EQ(m, sizes[i]);
       std::uniform_int_distribution<size_t> dis(1, m - 1);
-      for (int nt : {1, 32, 64}) {
+      for (int nt : kGroupSizes) {
         for (int q = 0; q < 10; ++q) {
           size_t pos = dis(gen);
           EXPECT_EQ(run_rank(rs, nt, val, i, pos), pos);
@@ -210,26 +270,104 @@ static void test_rank_select() {
       }
     }
   }
-  {  // random words vs a prefix-count helper (RankSelectOperationsRandom)
-    std::mt19937 gen(11);
-    const size_t m = 300001;
+  // random words vs prefix counts (RankSelectOperationsRandom), at density
+  // 1/2 and 1/16 (select brackets of ~16 and ~128 blocks), every group size,
+  // each query answered by its own group among many in a launch, on the
+  // directory, then with the query lines, then with the select lines too
+  for (int sparse = 0; sparse < 2; ++sparse) {
+    std::mt19937 gen(11 + sparse);
+    const size_t m = sparse ? 2000003 : 300001;
     std::vector<uint32_t> words((m + 31) / 32);
-    for (auto& w : words) w = gen();
+    for (auto& w : words) w = sparse ? gen() & gen() & gen() & gen() : gen();
     BitArray ba(std::vector<size_t>{m}, false);
     ba.writeWordsFromHost(0, 0, words.data(), words.size());
     RankSelect rs(std::move(ba), 0);
     std::vector<size_t> pref(m + 1, 0);
     for (size_t i = 0; i < m; ++i) pref[i + 1] = pref[i] + ((words[i / 32] >> (i % 32)) & 1);
+    const size_t ones = pref[m], zeros = m - ones;
+    // rank positions (ends included) and select ranks of both Values
+    const size_t nq = 3000;
+    std::vector<size_t> rpos(nq), sel1(nq), sel0(nq);
     std::uniform_int_distribution<size_t> dis(0, m);
-    for (int q = 0; q < 40; ++q) {
-      size_t i = dis(gen);
-      EXPECT_EQ(run_rank(rs, 1, 1, 0, i), pref[i]);
-      EXPECT_EQ(run_rank(rs, 64, 0, 0, i), i - pref[i]);
-      size_t k = pref[i];
-      if (k) EXPECT_EQ(run_select(rs, 1, 1, 0, k), size_t(std::lower_bound(pref.begin(), pref.end(), k) - pref.begin() - 1));
+    for (size_t j = 0; j < nq; ++j) {
+      rpos[j] = j == 0 ? 0 : j == 1 ? m : j == 2 ? m - 1 : dis(gen);
+      sel1[j] = j == 0 ? 1 : j == 1 ? ones : j == 2 ? ones + 1 : 1 + dis(gen) % ones;
+      sel0[j] = j == 0 ? 1 : j == 1 ? zeros : j == 2 ? zeros + 1 : 1 + dis(gen) % zeros;
     }
+    // expected answers: rank by prefix counts, select by a sweep over the bits
+    std::vector<size_t> pos1(ones), pos0(zeros);
+    for (size_t i = 0, a1 = 0, a0 = 0; i < m; ++i) {
+      if ((words[i / 32] >> (i % 32)) & 1) pos1[a1++] = i;
+      else pos0[a0++] = i;
+    }
+    auto sel_exp = [&](const std::vector<size_t>& pos, size_t k) {
+      return k >= 1 && k <= pos.size() ? pos[k - 1] : m;
+    };
+    size_t *d_idx, *d_out;
+    gpuErrchk(hipMalloc(&d_idx, nq * sizeof(size_t)));
+    gpuErrchk(hipMalloc(&d_out, nq * 64 * sizeof(size_t)));
+    std::vector<size_t> out(nq * 64);
+    for (int layout = 0; layout < 3; ++layout) {
+      if (layout == 1) utils::throwOnError(ecl_rank_select_build_query_lines(rs.handle()));
+      if (layout == 2) utils::throwOnError(ecl_rank_select_build_select_lines(rs.handle()));
+      for (int nt : kGroupSizes) {
+        for (int op = 0; op < 4; ++op) {  // rank1, rank0, select1, select0
+          const bool sel = op >= 2;
+          const int v = op == 0 || op == 2;
+          const std::vector<size_t>& in = !sel ? rpos : v ? sel1 : sel0;
+          gpuErrchk(hipMemcpy(d_idx, in.data(), nq * sizeof(size_t), hipMemcpyHostToDevice));
+          run_groups(sel, rs, nt, v, d_idx, nq, d_out);
+          gpuErrchk(hipMemcpy(out.data(), d_out, nq * nt * sizeof(size_t),
+                              hipMemcpyDeviceToHost));
+          int bad = 0;
+          for (size_t j = 0; j < nq; ++j) {
+            const size_t e = !sel ? (v ? pref[in[j]] : in[j] - pref[in[j]])
+                                  : sel_exp(v ? pos1 : pos0, in[j]);
+            for (int t = 0; t < nt; ++t) bad += out[j * nt + t] != e;
+          }
+          if (bad)
+            std::printf("rank/select groups: density 1/%d layout %d nt %d op %d: %d wrong\n",
+                        sparse ? 16 : 2, layout, nt, op, bad);
+          EXPECT_EQ(bad, 0);
+        }
+      }
+      // single-group kernels agree on the same arrays
+      for (int nt : {1, 4, 64}) {
+        for (int q = 0; q < 20; ++q) {
+          const size_t i = rpos[q], k = sel1[q], z = sel0[q];
+          EXPECT_EQ(run_rank(rs, nt, 1, 0, i), pref[i]);
+          EXPECT_EQ(run_rank(rs, nt, 0, 0, i), i - pref[i]);
+          EXPECT_EQ(run_select(rs, nt, 1, 0, k), sel_exp(pos1, k));
+          EXPECT_EQ(run_select(rs, nt, 0, 0, z), sel_exp(pos0, z));
+        }
+      }
+    }
+    gpuErrchk(hipFree(d_idx));
+    gpuErrchk(hipFree(d_out));
   }
   std::printf("rank_select: ok so far (%d failures)\n", g_fail);
+}
+
+template <typename T, int TPQ>
+static void launch_user(WaveletTree<T>& wt, int op, const size_t* d_idx,
+                        const RankSelectQuery<T>* d_q, size_t m, size_t* d_out) {
+  if (op == 0) userAccess<T, TPQ><<<8, 64>>>(wt, d_idx, m, d_out);
+  else if (op == 1) userRank<T, TPQ><<<8, 64>>>(wt, d_q, m, d_out);
+  else userSelect<T, TPQ><<<8, 64>>>(wt, d_q, m, d_out);
+}
+template <typename T>
+static void run_user(WaveletTree<T>& wt, int tpq, int op, const size_t* d_idx,
+                     const RankSelectQuery<T>* d_q, size_t m, size_t* d_out) {
+  switch (tpq) {
+    case 1: launch_user<T, 1>(wt, op, d_idx, d_q, m, d_out); break;
+    case 2: launch_user<T, 2>(wt, op, d_idx, d_q, m, d_out); break;
+    case 4: launch_user<T, 4>(wt, op, d_idx, d_q, m, d_out); break;
+    case 8: launch_user<T, 8>(wt, op, d_idx, d_q, m, d_out); break;
+    case 16: launch_user<T, 16>(wt, op, d_idx, d_q, m, d_out); break;
+    case 32: launch_user<T, 32>(wt, op, d_idx, d_q, m, d_out); break;
+    default: launch_user<T, 64>(wt, op, d_idx, d_q, m, d_out); break;
+  }
+  kernelCheck();
 }
 
 template <typename T>
@@ -289,45 +427,66 @@ static void test_tree_type() {
     for (size_t i = 0; i < idx.size(); ++i) EXPECT_EQ(size_t(res[i]), size_t(data[idx[i]]));
     // host rank rewrites symbols to minimal indices (wavelet_tree.cuh:1426-1436)
     std::vector<RankSelectQuery<T>> q;
-    for (int j = 0; j < 300; ++j) q.push_back({gen() % n, alphabet[gen() % alphabet.size()]});
+    for (int j = 0; j < 2000; ++j) q.push_back({gen() % n, alphabet[gen() % alphabet.size()]});
     std::vector<size_t> exp;
     for (auto& x : q) exp.push_back(std::count(data.begin(), data.begin() + x.index_, x.symbol_));
     auto rr = wt.rank(q.data(), q.size());
     for (size_t j = 0; j < q.size(); ++j) EXPECT_EQ(rr[j], exp[j]);
     for (auto& x : q) EXPECT_EQ(x.symbol_ < alphabet.size(), true);
-    // device API: minimal-alphabet symbols and indices
-    size_t *d_idx, *d_out;
-    RankSelectQuery<T>* d_q;
-    gpuErrchk(hipMalloc(&d_idx, idx.size() * 8));
-    gpuErrchk(hipMalloc(&d_out, idx.size() * 8));
-    gpuErrchk(hipMalloc(&d_q, q.size() * sizeof(RankSelectQuery<T>)));
-    gpuErrchk(hipMemcpy(d_idx, idx.data(), idx.size() * 8, hipMemcpyHostToDevice));
-    userAccess<T><<<8, 64>>>(wt, d_idx, idx.size(), d_out);
-    kernelCheck();
-    std::vector<size_t> out(idx.size());
-    gpuErrchk(hipMemcpy(out.data(), d_out, idx.size() * 8, hipMemcpyDeviceToHost));
+    // device API from user kernels: minimal-alphabet symbols and indices,
+    // every ThreadsPerQuery, on the directory, then with the query lines,
+    // then with the select lines too
+    std::vector<size_t> aexp(idx.size());
     for (size_t i = 0; i < idx.size(); ++i)
-      EXPECT_EQ(out[i], size_t(std::lower_bound(alphabet.begin(), alphabet.end(), data[idx[i]]) - alphabet.begin()));
-    gpuErrchk(hipMemcpy(d_q, q.data(), q.size() * sizeof(RankSelectQuery<T>), hipMemcpyHostToDevice));
-    userRank<T><<<4, 64>>>(wt, d_q, q.size(), d_out);
-    kernelCheck();
-    gpuErrchk(hipMemcpy(out.data(), d_out, q.size() * 8, hipMemcpyDeviceToHost));
-    for (size_t j = 0; j < q.size(); ++j) EXPECT_EQ(out[j], exp[j]);
-    // select the k-th occurrence via the device API
+      aexp[i] = std::lower_bound(alphabet.begin(), alphabet.end(), data[idx[i]]) - alphabet.begin();
+    // select the k-th occurrence, plus one past the last of a symbol
     std::vector<RankSelectQuery<T>> sq;
     std::vector<size_t> sexp;
-    for (int j = 0; j < 200; ++j) {
+    for (int j = 0; j < 1000; ++j) {
       size_t p = gen() % n;
       T c = data[p];
       size_t k = std::count(data.begin(), data.begin() + p + 1, c);
       sq.push_back({k, static_cast<T>(std::lower_bound(alphabet.begin(), alphabet.end(), c) - alphabet.begin())});
       sexp.push_back(p);
     }
-    gpuErrchk(hipMemcpy(d_q, sq.data(), sq.size() * sizeof(RankSelectQuery<T>), hipMemcpyHostToDevice));
-    userSelect<T><<<4, 64>>>(wt, d_q, sq.size(), d_out);
-    kernelCheck();
-    gpuErrchk(hipMemcpy(out.data(), d_out, sq.size() * 8, hipMemcpyDeviceToHost));
-    for (size_t j = 0; j < sq.size(); ++j) EXPECT_EQ(out[j], sexp[j]);
+    sq.push_back({size_t(std::count(data.begin(), data.end(), alphabet[3])) + 1, T(3)});
+    sexp.push_back(n);
+    // rank at both ends
+    q.push_back({0, T(5)});
+    exp.push_back(0);
+    q.push_back({n, T(5)});
+    exp.push_back(std::count(data.begin(), data.end(), alphabet[5]));
+    const size_t mq = std::max({idx.size(), q.size(), sq.size()});
+    size_t *d_idx, *d_out;
+    RankSelectQuery<T>* d_q;
+    gpuErrchk(hipMalloc(&d_idx, idx.size() * 8));
+    gpuErrchk(hipMalloc(&d_out, mq * 64 * 8));
+    gpuErrchk(hipMalloc(&d_q, mq * sizeof(RankSelectQuery<T>)));
+    gpuErrchk(hipMemcpy(d_idx, idx.data(), idx.size() * 8, hipMemcpyHostToDevice));
+    std::vector<size_t> out(mq * 64);
+    for (int layout = 0; layout < 3; ++layout) {
+      if (layout == 1) utils::throwOnError(ecl_wavelet_tree_build_query_lines(wt.handle()));
+      if (layout == 2) utils::throwOnError(ecl_wavelet_tree_build_select_lines(wt.handle()));
+      for (int tpq : kGroupSizes) {
+        for (int op = 0; op < 3; ++op) {  // access, rank, select
+          const size_t cnt = op == 0 ? idx.size() : op == 1 ? q.size() : sq.size();
+          if (op == 1)
+            gpuErrchk(hipMemcpy(d_q, q.data(), cnt * sizeof(RankSelectQuery<T>), hipMemcpyHostToDevice));
+          if (op == 2)
+            gpuErrchk(hipMemcpy(d_q, sq.data(), cnt * sizeof(RankSelectQuery<T>), hipMemcpyHostToDevice));
+          run_user<T>(wt, tpq, op, d_idx, d_q, cnt, d_out);
+          gpuErrchk(hipMemcpy(out.data(), d_out, cnt * tpq * 8, hipMemcpyDeviceToHost));
+          const std::vector<size_t>& e = op == 0 ? aexp : op == 1 ? exp : sexp;
+          int bad = 0;
+          for (size_t j = 0; j < cnt; ++j)
+            for (int t = 0; t < tpq; ++t) bad += out[j * tpq + t] != e[j];
+          if (bad)
+            std::printf("device API <%zu>: layout %d TPQ %d op %d: %d wrong\n", sizeof(T),
+                        layout, tpq, op, bad);
+          EXPECT_EQ(bad, 0);
+        }
+      }
+    }
     gpuErrchk(hipFree(d_idx));
     gpuErrchk(hipFree(d_out));
     gpuErrchk(hipFree(d_q));
@@ -530,7 +689,7 @@ static void test_text_file() {
 }
 
 int main() {
-  gpuErrchk(hipMallocManaged(&g_res, sizeof(size_t)));
+  gpuErrchk(hipMallocManaged(&g_res, 64 * sizeof(size_t)));
   try {
     test_utils_type<uint8_t>();
     test_utils_type<uint16_t>();
