@@ -9,6 +9,7 @@ from .api import (BitArray, DeviceBuffer, Forest, Graph, Profiler, RankSelect,  
                   Server, Timer, WaveletTree, convert_data_to_min_alphabet, device_name,
                   generate_text, read_data_from_file,
                   generate_text_into, mem_info, release_cached_memory, set_build_span,
+                  set_device_memory_cap, tree_build_memory,
                   set_query_lanes, probe_random_lines,
                   zipf_thresholds,
                   synchronize, FOREST_QUERY_DTYPE, MIXED_DTYPE, QUERY_DTYPE)

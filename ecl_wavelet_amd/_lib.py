@@ -72,6 +72,9 @@ _SIGS = {
     "ecl_device_synchronize": (st, [i32]),
     "ecl_device_mem_info": (st, [i32, C.POINTER(sz), C.POINTER(sz)]),
     "ecl_release_cached_memory": (st, [i32]),
+    "ecl_set_device_memory_cap": (st, [i32, u64]),
+    "ecl_wavelet_tree_build_memory": (st, [i32, sz, sz, i32, C.POINTER(u64), C.POINTER(u64)]),
+    "ecl_wavelet_tree_build_host_bytes": (st, [vp, C.POINTER(u64)]),
     "ecl_set_build_span": (st, [C.c_uint32]),
     "ecl_set_query_lanes": (st, [C.c_uint32]),
     "ecl_probe_random_lines": (st, [C.c_int, C.c_uint64, C.c_uint64, C.c_uint32, C.c_uint32,

@@ -505,6 +505,13 @@ class WaveletTree:
     def generateQueries(self, kind, seed, m, d_out, stream=None):
         call("ecl_wavelet_tree_generate_queries", self._h, kind, seed, m, d_out.ptr, stream)
 
+    def buildHostBytes(self):
+        """Bytes of the build's buffers that were placed in pinned host
+        memory because the device could not hold them (0: none)."""
+        b = C.c_uint64()
+        call("ecl_wavelet_tree_build_host_bytes", self._h, C.byref(b))
+        return b.value
+
     def setQueryLanes(self, lanes):
         """Lanes per access/rank query for this tree (0: library-wide)."""
         call("ecl_wavelet_tree_set_query_lanes", self._h, lanes)
@@ -798,6 +805,23 @@ def release_cached_memory(device=0):
     """Frees the construction scratch the library keeps per device between
     builds."""
     call("ecl_release_cached_memory", device)
+
+
+def set_device_memory_cap(device, nbytes):
+    """Caps the free device memory the library's admission checks assume
+    (0 removes the cap)."""
+    call("ecl_set_device_memory_cap", device, int(nbytes))
+
+
+def tree_build_memory(dtype, n, sigma, host_text=True):
+    """(resident, staging) upper bounds in bytes for a tree build: resident
+    must fit the device; each staging buffer goes to pinned host memory when
+    the device cannot hold it after the resident part."""
+    r = C.c_uint64()
+    g = C.c_uint64()
+    call("ecl_wavelet_tree_build_memory", np.dtype(dtype).itemsize, n, sigma,
+         1 if host_text else 0, C.byref(r), C.byref(g))
+    return r.value, g.value
 
 
 def mem_info(device=0):

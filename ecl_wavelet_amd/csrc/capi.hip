@@ -439,14 +439,38 @@ void set_device(int device) {
   ECL_HIP(hipSetDevice(device));
 }
 
+// per-device cap on the free memory the admission checks assume (0: none)
+constexpr int kCapDevices = 64;
+std::atomic<uint64_t> g_mem_cap[kCapDevices];
+
 // free device memory as the library sees it: what the runtime reports plus
-// the blocks the library's cache holds for reuse
+// the blocks the library's cache holds for reuse, at most the device's cap
 size_t free_device_memory() {
   size_t fr = 0, tot = 0;
   ECL_HIP(hipMemGetInfo(&fr, &tot));
   int dev = -1;
   ECL_HIP(hipGetDevice(&dev));
-  return fr + pooled_bytes(dev);
+  const size_t f = fr + pooled_bytes(dev);
+  const uint64_t cap = dev >= 0 && dev < kCapDevices ? g_mem_cap[dev].load() : 0;
+  return cap ? std::min<size_t>(f, cap) : f;
+}
+
+// MemAvailable of /proc/meminfo in bytes (getAvailableMemoryLinux,
+// utils.cu:122-139); 0 when it cannot be read
+uint64_t available_host_memory() {
+  std::FILE* f = std::fopen("/proc/meminfo", "r");
+  if (!f) return 0;
+  char line[256];
+  uint64_t kb = 0;
+  while (std::fgets(line, sizeof line, f)) {
+    unsigned long long v = 0;
+    if (std::sscanf(line, "MemAvailable: %llu kB", &v) == 1) {
+      kb = v;
+      break;
+    }
+  }
+  std::fclose(f);
+  return kb * 1024;
 }
 
 }  // namespace lib
@@ -538,6 +562,7 @@ struct ecl_wavelet_tree_t {
   int map_kind = 0;
   TreeView view{};
   uint32_t lanes = 0;  // lanes per access/rank query (0: the library-wide setting)
+  uint64_t build_host_bytes = 0;  // build buffers placed in pinned host memory
   std::unique_ptr<HostPipe> pipe{new HostPipe()};
   ~ecl_wavelet_tree_t() { delete rs; }
 };
@@ -866,23 +891,71 @@ BuildPlan plan_of(uint32_t L) {
   return b;
 }
 
-// Memory a tree of n symbols needs beyond its text (upper bound, bytes):
-// bits + L2/L1/samples per level, the mapped codes and the level-order
-// buffers of multi-pass trees, the tile histograms and their scan, a host
-// text's device copy, metadata and the reference's 10M-query headroom
-// (wavelet_tree.cuh:644-648).
-uint64_t tree_needed_bytes(uint64_t n, uint64_t sigma, uint32_t L, int esize, bool host_text) {
+// Memory a tree of n symbols needs beyond its text (upper bound, bytes), in
+// two parts.  resident: what the build keeps on the device in any case --
+// bits + L2/L1/samples per level, the tile histograms and their scan,
+// metadata and the reference's 10M-query headroom (wavelet_tree.cuh:
+// 644-648).  staging: a host text's copy and the code buffers (the mapped
+// codes and the level-order buffers of multi-pass trees), each of which goes
+// to pinned host memory when the device cannot hold it after the resident
+// part (the reference's unified-memory fallback, wavelet_tree.cuh:655-689).
+struct TreeMemory {
+  uint64_t resident = 0;
+  uint64_t text = 0;     // the device copy of a host text
+  uint64_t x = 0, u = 0, v = 0;  // mapped codes, level-order buffers
+  uint64_t staging() const { return text + x + u + v; }
+};
+TreeMemory tree_memory(uint64_t n, uint64_t sigma, uint32_t L, int esize, bool host_text,
+                       bool identity) {
   const BuildPlan b = plan_of(L);
   const uint64_t per_level = ((n + 1023) / 1024) * 128 + 256 /* bits, padded */ +
                              ((n + 511) / 512) * 2 + ((n + 65535) / 65536) * 8 +
                              (n / 4096) * 8 + 64;
-  const uint64_t codes = n * b.cw + 4 * sigma * 16;
-  const uint64_t buffers = (b.npasses > 1 ? 2 : 1) * codes;
   const uint64_t tiles = n / pass_tile_codes(b.cw, 4) + 2 * sigma + 2;
   const uint64_t scan = tiles * 256 * 12 + (tiles / 256 + 1) * 256 * 8;
-  return per_level * L + buffers + scan + (host_text ? n * esize : 0) + sigma * 64 +
-         (n / 4096 + 4096) * 8 + 10'000'000ull * 16;
+  TreeMemory m;
+  m.resident = per_level * L + scan + sigma * 64 + (n / 4096 + 4096) * 8 +
+               10'000'000ull * 16;
+  // level order with padded nodes (the build's pad_cap)
+  const uint64_t pad = (n + (2 * sigma + 2) * (16 / b.cw)) * b.cw;
+  m.text = host_text ? n * esize : 0;
+  m.x = identity ? 0 : (b.npasses > 2 ? pad : n * b.cw);
+  m.u = b.npasses > 1 ? pad : 0;
+  m.v = b.npasses > 2 && identity ? pad : 0;
+  return m;
 }
+uint64_t tree_needed_bytes(uint64_t n, uint64_t sigma, uint32_t L, int esize, bool host_text) {
+  return tree_memory(n, sigma, L, esize, host_text, false).resident +
+         tree_memory(n, sigma, L, esize, host_text, false).staging();
+}
+
+// A build buffer on the device, or in pinned host memory that the kernels
+// read and write over the host link when the device cannot hold it.
+struct StagingBuf {
+  DevBuf dev;
+  void* host = nullptr;
+  void* ptr = nullptr;
+  StagingBuf() = default;
+  StagingBuf(const StagingBuf&) = delete;
+  StagingBuf& operator=(const StagingBuf&) = delete;
+  ~StagingBuf() { reset(); }
+  void alloc(uint64_t bytes, bool on_host) {
+    reset();
+    if (bytes == 0) return;
+    if (on_host) {
+      ECL_HIP(hipHostMalloc(&host, bytes, hipHostMallocDefault));
+      ptr = host;
+    } else {
+      dev = DevBuf(bytes);
+      ptr = dev.ptr;
+    }
+  }
+  void reset() {
+    dev.reset();
+    if (host) (void)hipHostFree(host);
+    host = ptr = nullptr;
+  }
+};
 
 }  // namespace
 
@@ -1050,24 +1123,47 @@ static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_
     const uint64_t T = pass_tile_codes(cw, rows);   // codes per tile
     const uint64_t E = 16 / cw;               // codes per 16 bytes
 
-    // ---- memory admission (wavelet_tree.cuh:644-689); the build's scratch
-    // reuses the device's scratch cache when it holds one
-    const uint64_t needed = tree_needed_bytes(n, sigma, L, esize, !text_on_device);
-    if (needed > free_device_memory() + cached_scratch_bytes(device))
-      throw Error(ECL_ERROR_OUT_OF_MEMORY_TREE, kOomTree);
+    // the passes read 16-byte vectors: a text that is its own code and is
+    // aligned is read in place, anything else is mapped to codes first (a
+    // host text's copy is aligned)
+    const bool aligned =
+        !text_on_device || (reinterpret_cast<uintptr_t>(text) & 15) == 0;
+    const bool identity = wt->min_alphabet && wt->pow2 && esize == cw && aligned;
 
-    // ---- the text on the device
-    DevBuf own_text;
+    // ---- memory admission (wavelet_tree.cuh:644-689): the resident part
+    // must fit the device (the build's scratch reuses the device's scratch
+    // cache when it holds one); then the text copy and the code buffers, in
+    // that order, each stay on the device while they fit and otherwise go
+    // to pinned host memory, which must fit the host's available memory
+    const TreeMemory mem = tree_memory(n, sigma, L, esize, !text_on_device, identity);
+    const uint64_t avail = free_device_memory() + cached_scratch_bytes(device);
+    if (mem.resident > avail) throw Error(ECL_ERROR_OUT_OF_MEMORY_TREE, kOomTree);
+    uint64_t room = avail - mem.resident, host_bytes = 0;
+    auto on_host = [&](uint64_t bytes) {
+      if (bytes <= room) {
+        room -= bytes;
+        return false;
+      }
+      host_bytes += bytes;
+      return true;
+    };
+    const bool text_host = on_host(mem.text);
+    const bool x_host = on_host(mem.x);
+    const bool u_host = on_host(mem.u);
+    const bool v_host = on_host(mem.v);
+    if (host_bytes > available_host_memory())
+      throw Error(ECL_ERROR_OUT_OF_MEMORY_TREE, kOomTree);
+    wt->build_host_bytes = host_bytes;
+
+    // ---- the text on the device (or in pinned host memory)
+    StagingBuf own_text;
     const void* d_text = text;
     if (!text_on_device) {
-      own_text = DevBuf(n * esize);
-      upload_staged(own_text.ptr, text, n * esize, s);
+      own_text.alloc(n * esize, text_host);
+      if (text_host) parallel_memcpy(own_text.ptr, text, n * esize);
+      else upload_staged(own_text.ptr, text, n * esize, s);
       d_text = own_text.ptr;
     }
-    // the passes read 16-byte vectors: a text that is its own code and is
-    // aligned is read in place, anything else is mapped to codes first
-    const bool aligned = (reinterpret_cast<uintptr_t>(d_text) & 15) == 0;
-    const bool identity = wt->min_alphabet && wt->pow2 && esize == cw && aligned;
 
     // Every launch of the build is queued before the host waits for the
     // device: small tables go up as kernel arguments (Uploader), and the one
@@ -1105,15 +1201,19 @@ static ecl_wavelet_tree_t* build_tree(int esize, const void* text, bool text_on_
     // ---- code and level-order buffers; scratch: tile histograms and their
     // scan for the largest pass, error flag, first pass digit starts and
     // thresholds, directory tables
-    DevBuf buf_x, buf_u, buf_v;
+    StagingBuf buf_x, buf_u, buf_v;
     uint64_t max_tiles = (n + T - 1) / T;
     for (uint32_t q = 1; q < plan.npasses; ++q)
       max_tiles = std::max<uint64_t>(max_tiles, n / T + (1ull << plan.l0[q]) + 1);
     const uint32_t kmax = plan.k[0];
     const uint64_t pad_cap = n + (2 * sigma + 2) * E;  // level order with padded nodes
-    if (!identity) buf_x = DevBuf((plan.npasses > 2 ? pad_cap : n) * cw);
-    if (plan.npasses > 1) buf_u = DevBuf(pad_cap * cw);
-    if (plan.npasses > 2 && identity) buf_v = DevBuf(pad_cap * cw);
+    if (!identity) buf_x.alloc((plan.npasses > 2 ? pad_cap : n) * cw, x_host);
+    if (plan.npasses > 1) buf_u.alloc(pad_cap * cw, u_host);
+    if (plan.npasses > 2 && identity) buf_v.alloc(pad_cap * cw, v_host);
+    if ((!identity && (plan.npasses > 2 ? pad_cap : n) * cw != mem.x) ||
+        (plan.npasses > 1 ? pad_cap * cw : 0) != mem.u ||
+        (plan.npasses > 2 && identity ? pad_cap * cw : 0) != mem.v)
+      throw Error(ECL_ERROR_INTERNAL, "code buffer sizes differ from the memory estimate");
     const uint32_t k0 = plan.k[0], nb0 = 1u << k0;
     const uint64_t tiles0 = (n + T - 1) / T;
     Scratch scr(device, s,
@@ -1763,6 +1863,33 @@ ecl_status ecl_set_query_lanes(uint32_t lanes) {
   return guard([&] {
     require(lanes <= 2, "lanes must be 0 (automatic), 1 or 2");
     set_query_lanes(lanes);
+  });
+}
+ecl_status ecl_set_device_memory_cap(int device, uint64_t bytes) {
+  return guard([&] {
+    require(device >= 0 && device < kCapDevices, "invalid device");
+    g_mem_cap[device].store(bytes);
+  });
+}
+ecl_status ecl_wavelet_tree_build_memory(ecl_type type, size_t n, size_t sigma,
+                                         int host_text, uint64_t* resident,
+                                         uint64_t* staging) {
+  return guard([&] {
+    require(resident != nullptr && staging != nullptr, "NULL output");
+    const int esize = static_cast<int>(type);
+    require(esize == 1 || esize == 2 || esize == 4 || esize == 8, "invalid element type");
+    require(sigma >= 2, "the alphabet needs at least 2 symbols");
+    const uint32_t L = detail::ceil_log2(sigma);
+    // an upper bound: the text is taken not to be its own code
+    const TreeMemory m = tree_memory(n, sigma, L, esize, host_text != 0, false);
+    *resident = m.resident;
+    *staging = m.staging();
+  });
+}
+ecl_status ecl_wavelet_tree_build_host_bytes(ecl_wavelet_tree wt, uint64_t* bytes) {
+  return guard([&] {
+    require(wt != nullptr && bytes != nullptr, "NULL argument");
+    *bytes = wt->build_host_bytes;
   });
 }
 ecl_status ecl_release_cached_memory(int device) {

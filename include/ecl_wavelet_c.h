@@ -120,6 +120,28 @@ ecl_status ecl_device_mem_info(int device, size_t* free_bytes, size_t* total_byt
  * reference, which allocates and frees per call; for callers that need the
  * memory back outside the library. */
 ecl_status ecl_release_cached_memory(int device);
+/* Caps the free device memory the library's admission checks assume on a
+ * device (tree builds, query batches, server budgets): they see at most
+ * `bytes` free; 0 removes the cap.  No counterpart in the reference; for
+ * sharing a device with other work, and to exercise the host-memory
+ * fallback of the build (below) on a device that has room. */
+ecl_status ecl_set_device_memory_cap(int device, uint64_t bytes);
+/* Memory a tree build needs (upper bounds, bytes): `resident` must fit the
+ * device -- bits, directory, construction scratch, metadata and the
+ * reference's 10M-query headroom (wavelet_tree.cuh:644-648); otherwise the
+ * build fails with ECL_ERROR_OUT_OF_MEMORY_TREE.  `staging` is a host text's
+ * copy and the code buffers of the passes: the build places each on the
+ * device while it fits after `resident`, and otherwise in pinned host memory
+ * that the kernels read and write over the host link, as the reference
+ * moves its data, sorted copy and sort scratch to unified memory
+ * (wavelet_tree.cuh:655-689, 735-802); the host part must fit the host's
+ * available memory (MemAvailable).  Results do not depend on the placement;
+ * a build that stages on the host runs at the host link's bandwidth. */
+ecl_status ecl_wavelet_tree_build_memory(ecl_type type, size_t n, size_t sigma, int host_text,
+                                         uint64_t* resident, uint64_t* staging);
+/* Bytes of the tree's build buffers that were placed in host memory (0 when
+ * the whole build ran in device memory). */
+ecl_status ecl_wavelet_tree_build_host_bytes(ecl_wavelet_tree wt, uint64_t* bytes);
 /* Tiles of 32768 codes each construction workgroup takes in turn (0:
  * automatic, up to 16 for large texts).  A tuning knob with no effect on
  * results; no counterpart in the reference. */

@@ -3,6 +3,7 @@ No compute call: runs without a GPU."""
 import ctypes
 import os
 
+import numpy as np
 import pytest
 
 from ecl_wavelet_amd import _lib
@@ -76,3 +77,21 @@ def test_noalloc_rejects_oversize_before_any_device_call():
                                                      None, None)
     assert rc == _lib.ECL_ERROR_INVALID_ARGUMENT
     assert b"ECL_NOALLOC_MAX_BITS" in lib.ecl_last_error()
+
+
+def test_tree_build_memory_estimate():
+    """The build's memory split: the resident part does not depend on where
+    the text is; a host text adds its copy to the staging part; multi-pass
+    trees add a level-order buffer."""
+    from ecl_wavelet_amd import tree_build_memory
+    n = 10**7
+    r_h, s_h = tree_build_memory(np.uint8, n, 200, host_text=True)
+    r_d, s_d = tree_build_memory(np.uint8, n, 200, host_text=False)
+    assert r_h == r_d and r_h > n  # bits of 8 levels + directory + headroom
+    assert s_h - s_d == n
+    assert s_d >= n  # the mapped codes
+    r16, s16 = tree_build_memory(np.uint16, n, 65536, host_text=False)
+    assert s16 >= 2 * n * 2  # codes and one level-order buffer of 2-byte codes
+    assert r16 > r_d
+    with pytest.raises(Exception):
+        tree_build_memory(np.uint8, n, 1)

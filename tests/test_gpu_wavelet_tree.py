@@ -1,6 +1,8 @@
 """WaveletTree<T> on the GPU vs the oracle.  Mirrors the reference's
 test_wavelet_tree.cu: structure KATs, codes, node ranks, histogram, and
 randomized access/rank/select against direct scans of the text."""
+import gc
+
 import numpy as np
 import pytest
 
@@ -298,6 +300,61 @@ def test_oom_message(gpu):
     assert str(e.value) == "Not enough memory available for the wavelet tree."
 
 
+@pytest.mark.parametrize("case", ["u8_256", "u8_200", "u16_1000", "u16_65536", "dev_200"])
+@pytest.mark.parametrize("where", ["host", "split"])
+def test_build_stages_on_host_when_device_is_short(gpu, case, where):
+    """With the device's free memory capped just above the resident part of a
+    build (wavelet_tree.cuh:655-689: the text, its sorted copy and the sort
+    scratch go to unified memory when they do not fit), the text copy and the
+    code buffers go to pinned host memory -- all of them ("host"), or all but
+    the text copy ("split") -- and the tree equals the oracle's; below the
+    resident part the build fails with the reference's message."""
+    from ecl_wavelet_amd import (DeviceBuffer, WaveletTree, release_cached_memory,
+                                 set_device_memory_cap, tree_build_memory)
+    from ecl_wavelet_amd._lib import EclError, ECL_ERROR_OUT_OF_MEMORY_TREE
+    kind, sigma = case.split("_")
+    sigma = int(sigma)
+    dt = np.uint16 if kind == "u16" else np.uint8
+    rng = np.random.default_rng(sigma + (7 if where == "split" else 0))
+    n = 3 * 10**6 + int(rng.integers(0, 1000))
+    text = rng.integers(0, sigma, n).astype(dt)
+    on_device = kind == "dev"
+    resident, staging = tree_build_memory(dt, n, sigma, host_text=not on_device)
+    text_bytes = 0 if on_device else n * np.dtype(dt).itemsize
+    d = DeviceBuffer.from_array(text) if on_device else None
+    alphabet = np.arange(sigma, dtype=dt)
+
+    def build():
+        if on_device:
+            return WaveletTree(d, alphabet, dtype=dt, n=n)
+        return WaveletTree(text, alphabet)
+
+    try:
+        release_cached_memory()
+        set_device_memory_cap(0, resident - 1)
+        with pytest.raises(EclError) as e:
+            build()
+        assert e.value.status == ECL_ERROR_OUT_OF_MEMORY_TREE
+        assert str(e.value) == "Not enough memory available for the wavelet tree."
+        del e  # its traceback holds this frame (and the trees) in a cycle
+        release_cached_memory()
+        extra = (text_bytes if where == "split" else 0) + (1 << 20)
+        set_device_memory_cap(0, resident + extra)
+        wt = build()
+    finally:
+        set_device_memory_cap(0, 0)
+    hb = wt.buildHostBytes()
+    if case == "u8_256":
+        # its own code: the text copy is the only staging buffer
+        assert hb == (0 if where == "split" else n)
+    else:
+        assert 0 < hb <= staging - (text_bytes if where == "split" else 0)
+    _check_structure_vs_oracle(wt, text.astype(np.uint64), sigma)
+    _check_queries(wt, text, None, nq=2000, seed=sigma)
+    # without the cap the same text builds on the device alone
+    assert build().buildHostBytes() == 0
+
+
 def test_query_oom_message(gpu):
     """Host queries whose device staging does not fit raise the reference's
     exact message for their kind (wavelet_tree.cuh:972-973, 1268-1269,
@@ -310,6 +367,9 @@ def test_query_oom_message(gpu):
     m = 1 << 22  # one full staging chunk: 2 x 2^22 x (16 + 8) bytes on the device
     q = wt.make_queries(rng.integers(0, len(text), m), rng.integers(0, 4, m))
     idx = q["index"].copy()
+    # objects of earlier tests that only the cycle collector frees would
+    # otherwise give their device memory back in the middle of this test
+    gc.collect()
     release_cached_memory()
     fillers = []
     try:
