@@ -951,7 +951,7 @@ def main():
             "compulsory_bytes_per_launch": pb,
             "compulsory_gbps": compulsory,
             "compulsory_frac": compulsory / HBM_PEAK_GBPS,
-            "limiter": "VALU issue (0.93 of peak in the committed counter profile)",
+            "limiter": "VALU issue (0.94 of peak in the committed counter profile)",
             "launch_ms": pass_ms,
         },
     }
