@@ -34,8 +34,8 @@ namespace {
 
 using Clock = std::chrono::steady_clock;
 constexpr uint32_t kMaxSegments = 65536;  // requests per batch
-// smallest batch the coalescing scheduler cuts when it splits the queued
-// work over the free slots (1 MiB of queries)
+// smallest batch the coalescing scheduler cuts when it splits a backlog
+// over the free slots (1 MiB of queries)
 constexpr uint64_t kMinBatch = 1ull << 16;
 
 struct Request {
@@ -123,16 +123,21 @@ void ecl_server_t::scheduler_loop() {
     Slot* sl = free_slots.back();
     free_slots.pop_back();
     sl->parts.clear();
-    // Batch size: with coalescing, the queued work is split over the batch
-    // slots (at least kMinBatch queries a batch, at most max_batch), so that
-    // a lone large request runs as several batches whose copies overlap the
-    // kernels of the others instead of one batch that copies, computes and
-    // copies back in turn; under load the batches grow to max_batch.  The
+    // Batch size.  With coalescing, a backlog is split over the batch slots
+    // (at least kMinBatch queries a batch), so that the first part of it
+    // completes while the rest is staged: at 10^4 requests/s this gave a
+    // p99 of 2.5 ms against 6.6 ms for batches that take everything queued.
+    // The request at the head is not cut below its own size, though: a
+    // lone request split over the slots has its parts' staging copies,
+    // launches and copy-outs run one after another on the scheduler and
+    // completion threads (a 10^6-query request took 1.43 ms as three
+    // batches against 1.04 ms as one).  At most max_batch either way.  The
     // plain form takes one whole request (up to max_batch) at a time.
     uint64_t target = cfg.max_batch;
     if (cfg.coalesce) {
       const uint64_t share = (queued + cfg.max_inflight - 1) / cfg.max_inflight;
-      target = std::min<uint64_t>(cfg.max_batch, std::max<uint64_t>(kMinBatch, share));
+      const uint64_t head = queue.front()->m - queue.front()->taken;
+      target = std::min<uint64_t>(cfg.max_batch, std::max({kMinBatch, share, head}));
     }
     uint64_t count = 0;
     while (!queue.empty() && count < target && sl->parts.size() < kMaxSegments) {

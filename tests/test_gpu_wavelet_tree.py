@@ -693,15 +693,31 @@ def test_query_lanes(gpu, lanes, case):
         set_query_lanes(0)
 
 
+def _skewed_text(rng, n):
+    """Halves whose level-0 ones density is ~0.02 and ~0.98 (98% of the
+    symbols below 128, then 98% at or above): the level's density is ~0.5,
+    so its select lines (one every ~400 Values) overflow their 960 bits
+    throughout, and the selects fall back to the hints and the directory."""
+    half = n // 2
+    lo = np.where(rng.random(half) < 0.98, rng.integers(0, 128, half), rng.integers(128, 256, half))
+    hi = np.where(rng.random(n - half) < 0.98, rng.integers(128, 256, n - half),
+                  rng.integers(0, 128, n - half))
+    return np.concatenate([lo, hi]).astype(np.uint8)
+
+
+@pytest.mark.parametrize("text_kind", ["uniform", "skewed"])
 @pytest.mark.parametrize("m", [1, 3, 777, 70001])
-def test_mixed_batch_query_lines(gpu, m):
+def test_mixed_batch_query_lines(gpu, m, text_kind):
     """A mixed batch (kinds grouped per chunk; with query lines, access and
-    rank on two lanes each) answers like the oracle before and after the
-    query lines are built, for odd batch sizes and for balanced, one-sided
-    and kind > 2 (read as select) mixes."""
+    rank on two lanes each) answers like the oracle over the directory, then
+    with the query lines and hints, then with the select lines too, for odd
+    batch sizes and for balanced, one-sided and kind > 2 (read as select)
+    mixes; on a skewed text the select lines overflow and the mixed kernel's
+    select goes from the line to the hint and to the directory search."""
     from ecl_wavelet_amd import MIXED_DTYPE, DeviceBuffer, WaveletTree
     rng = np.random.default_rng(m)
-    text = rng.integers(0, 256, 300_007).astype(np.uint8)
+    text = (rng.integers(0, 256, 300_007).astype(np.uint8) if text_kind == "uniform"
+            else _skewed_text(rng, 300_007))
     n = len(text)
     wt = WaveletTree(text, np.arange(256, dtype=np.uint8))
     cnt = np.bincount(text, minlength=256)
@@ -726,11 +742,13 @@ def test_mixed_batch_query_lines(gpu, m):
         exp[k2] = O.scan_select_batch(text, a[k2], s[k2])
         d_q, d_out = DeviceBuffer(m * 16), DeviceBuffer(m * 8)
         d_q.upload(q)
-        for lines in (False, True):
-            if lines:
+        for stage in ("directory", "query lines", "select lines"):
+            if stage == "query lines":
                 wt.buildQueryLines()
+            elif stage == "select lines":
+                wt.buildSelectLines()
             wt.queryBatchDevice(d_q, m, d_out)
-            assert np.array_equal(d_out.download(np.uint64, m), exp), (mix, lines)
+            assert np.array_equal(d_out.download(np.uint64, m), exp), (mix, stage)
 
 
 def test_tree_from_raw_text_file(gpu, tmp_path):
