@@ -514,7 +514,8 @@ def bench_full_sweep(args):
     6e9 symbols uniform over sigma in {4, 16, 64}, and a uint16 text of 2e9
     symbols Zipf(1.1) over sigma = 2^16.  Per text: median of two timed
     builds after one untimed, then 10^7 device-generated queries per kind,
-    over the plain directory and again over the query and select lines."""
+    over the plain directory, again over the query and select lines, and
+    access and rank over the quad lines."""
     import ecl_wavelet_amd as E
     timer = E.Timer()
     Q = 10_000_000
@@ -557,6 +558,16 @@ def bench_full_sweep(args):
             timer.start()
             call_()
             tl[kind] = timer.stop()
+        # access and rank over the quad lines (every sweep alphabet is a power
+        # of two with an even number of levels)
+        timer.start()
+        wt.buildQuadLines()
+        quad_ms = timer.stop()
+        tq = {}
+        for kind, call_ in calls[:2]:
+            timer.start()
+            call_()
+            tq[kind] = timer.stop()
         build_ms = float(np.median(builds))
         r = {"dtype": np.dtype(dt).name, "n": n, "sigma": sigma,
              "distribution": "uniform" if thr is None else "zipf(1.1)",
@@ -566,12 +577,16 @@ def bench_full_sweep(args):
              "select_qps": Q / (t["select"] / 1e3),
              "query_and_select_lines_build_ms": lines_ms,
              "access_qps_lines": Q / (tl["access"] / 1e3), "rank_qps_lines": Q / (tl["rank"] / 1e3),
-             "select_qps_lines": Q / (tl["select"] / 1e3)}
+             "select_qps_lines": Q / (tl["select"] / 1e3),
+             "quad_lines_build_ms": quad_ms,
+             "access_qps_quad": Q / (tq["access"] / 1e3), "rank_qps_quad": Q / (tq["rank"] / 1e3)}
         out.append(r)
         log(f"sweep {r['dtype']} sigma={sigma} n={n}: build {build_ms:.1f} ms, access "
             f"{r['access_qps']:.3g}, rank {r['rank_qps']:.3g}, select {r['select_qps']:.3g} q/s; "
             f"with lines ({lines_ms:.1f} ms): access {r['access_qps_lines']:.3g}, rank "
-            f"{r['rank_qps_lines']:.3g}, select {r['select_qps_lines']:.3g} q/s")
+            f"{r['rank_qps_lines']:.3g}, select {r['select_qps_lines']:.3g} q/s; "
+            f"quad lines ({quad_ms:.1f} ms): access {r['access_qps_quad']:.3g}, rank "
+            f"{r['rank_qps_quad']:.3g} q/s")
         del wt, d_text
     return out
 
@@ -771,7 +786,11 @@ def main():
     timer = E.Timer()
 
     build_ms, lines_ms, sel_ms, q_ms, mixed_ms = [], [], [], [], []
+    quad_ms, plain_total_ms = [], []
     kind_ms = {"access": [], "rank": [], "select": []}
+    quad_kind_ms = {"access": [], "rank": []}
+    # quad lines: power-of-two alphabets with an even number of levels
+    use_quad = sigma >= 4 and (sigma & (sigma - 1)) == 0 and (sigma.bit_length() - 1) % 2 == 0
     plain_ms = {"access": [], "rank": [], "select": [], "mixed": []}
     lines_bytes = sel_bytes = 0
     wt = None
@@ -810,22 +829,41 @@ def main():
         sel_bytes = free1 - E.mem_info()[0]
         times = run_queries(wt, timer, Q, d_idx, d_rq, d_sq, d_mq, d_out)
         mx = times.pop("mixed")
+        qt = {}
+        if use_quad:
+            # quad lines, then access and rank over them (one line per two levels)
+            timer.start()
+            wt.buildQuadLines()
+            qb = timer.stop()
+            timer.start()
+            wt.accessDevice(d_idx, Q, d_out, flags=2)
+            qt["access"] = timer.stop()
+            timer.start()
+            wt.rankDevice(d_rq, Q, d_out, flags=1)
+            qt["rank"] = timer.stop()
         log(f"step {step}{'' if timed else ' (warmup)'}: build {b:.1f} ms, "
             f"plain access {plain['access']:.2f} ms, rank {plain['rank']:.2f} ms, "
             f"select {plain['select']:.2f} ms, mixed {plain['mixed']:.2f} ms; lines {lb:.2f} ms, "
             f"select lines {sb:.2f} ms; "
             f"access {times['access']:.2f} ms, rank {times['rank']:.2f} ms, "
-            f"select {times['select']:.2f} ms, mixed {mx:.2f} ms")
+            f"select {times['select']:.2f} ms, mixed {mx:.2f} ms"
+            + (f"; quad lines {qb:.2f} ms, access {qt['access']:.2f} ms, "
+               f"rank {qt['rank']:.2f} ms" if use_quad else ""))
         if timed:
             build_ms.append(b)
             lines_ms.append(lb)
             sel_ms.append(sb)
             q_ms.append(sum(times.values()))
             mixed_ms.append(mx)
+            plain_total_ms.append(sum(plain.values()))
             for k, v in times.items():
                 kind_ms[k].append(v)
             for k, v in plain.items():
                 plain_ms[k].append(v)
+            if use_quad:
+                quad_ms.append(qb + qt["access"] + qt["rank"])
+                for k, v in qt.items():
+                    quad_kind_ms[k].append(v)
 
     # per-call latency of a plain small batch vs the same work batched
     lat = []
@@ -865,9 +903,18 @@ def main():
     # this device (probe over a buffer the size of the tree's query lines,
     # L lines per probe query, two lanes per line like the query kernels)
     line_peak = E.probe_random_lines(int(lines_bytes), Q, steps=L, lanes=2)
+    line_peak_half = (E.probe_random_lines(int(lines_bytes), Q, steps=L // 2, lanes=2)
+                      if use_quad else None)
     build = float(np.median(build_ms))
     qtime = float(np.mean(q_ms)) / 1e3
-    value = 3 * Q / qtime
+    lines_value = 3 * Q / qtime
+    # value: access and rank over the quad lines where the tree has them,
+    # select over the select lines
+    kind_best = {k: float(np.mean(v)) for k, v in kind_ms.items()}
+    if use_quad:
+        for k, v in quad_kind_ms.items():
+            kind_best[k] = float(np.mean(v))
+    value = 3 * Q / (sum(kind_best.values()) / 1e3)
     pb = pass_bytes(n, sigma)
     pab = pass_algorithmic_bytes(n, sigma)
     pass_ms = float(np.median(pass_launch_ms))
@@ -886,8 +933,10 @@ def main():
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": float(np.mean(np.array(build_ms) + np.array(lines_ms) + np.array(sel_ms)
-                                     + np.array(q_ms) + np.array(mixed_ms))),
+        "ms_per_step": float(np.mean(np.array(build_ms) + np.array(plain_total_ms)
+                                     + np.array(lines_ms) + np.array(sel_ms)
+                                     + np.array(q_ms) + np.array(mixed_ms)
+                                     + (np.array(quad_ms) if use_quad else 0.0))),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -898,7 +947,8 @@ def main():
                          f"sigma={sigma} (device-resident); per step: 1 build, "
                          f"{Q} each of access/rank/select + one mixed batch of {3 * Q} "
                          f"over the plain directory, the query lines and select "
-                         f"lines, the same queries over them"),
+                         f"lines, the same queries over them, then the quad lines and "
+                         f"access/rank over them"),
             "n": n, "sigma": sigma, "queries_per_kind": Q, "levels": L,
         },
         "build_ms": build,
@@ -909,9 +959,22 @@ def main():
         "build_gbps": n / (build / 1e3) / 1e9,
         "build_algorithmic_gbps": build_bytes / (build / 1e3) / 1e9,
         "build_kernels_ms": kern,
-        "access_qps": Q / (np.mean(kind_ms["access"]) / 1e3),
-        "rank_qps": Q / (np.mean(kind_ms["rank"]) / 1e3),
-        "select_qps": Q / (np.mean(kind_ms["select"]) / 1e3),
+        "access_qps": Q / (kind_best["access"] / 1e3),
+        "rank_qps": Q / (kind_best["rank"] / 1e3),
+        "select_qps": Q / (kind_best["select"] / 1e3),
+        "layouts": {
+            "access_rank": "quad lines" if use_quad else "query lines",
+            "select": "select lines",
+        },
+        "quad_lines_build_ms": (float(np.median([t - a - r for t, a, r in zip(
+            quad_ms, quad_kind_ms["access"], quad_kind_ms["rank"])])) if use_quad else None),
+        "quad_lines_bytes": (L // 2) * (n // 448 + 1) * 128 if use_quad else None,
+        "query_lines_layout": {
+            "access_qps": Q / (np.mean(kind_ms["access"]) / 1e3),
+            "rank_qps": Q / (np.mean(kind_ms["rank"]) / 1e3),
+            "select_qps": Q / (np.mean(kind_ms["select"]) / 1e3),
+            "per_kind_qps": lines_value,
+        },
         "mixed_batch_qps": 3 * Q / (np.mean(mixed_ms) / 1e3),
         "small_call_rank_p50_ms": float(np.percentile(lat, 50)),
         "small_call_rank_p99_ms": float(np.percentile(lat, 99)),
@@ -922,10 +985,19 @@ def main():
             "peak": line_peak,
             "peak_source": (f"ecl_probe_random_lines over {int(lines_bytes)} bytes, {Q} probe "
                             f"queries of {L} dependent lines, two lanes per line"),
-            **{kind: {"lines_per_query": lpq,
-                      "achieved": Q / (np.mean(kind_ms[kind]) / 1e3) * lpq,
-                      "frac": Q / (np.mean(kind_ms[kind]) / 1e3) * lpq / line_peak}
-               for kind, lpq in (("access", L), ("rank", L), ("select", L))},
+            "peak_half_chains": line_peak_half,
+            "peak_half_chains_source": (f"the same probe with {L // 2} dependent lines per "
+                                        f"query (the quad lines' chains)" if use_quad else None),
+            **{kind: {"layout": lay, "lines_per_query": lpq,
+                      "achieved": Q / (kind_best[kind] / 1e3) * lpq,
+                      "peak": pk,
+                      "frac": Q / (kind_best[kind] / 1e3) * lpq / pk}
+               for kind, lay, lpq, pk in (
+                   ("access", "quad lines" if use_quad else "query lines",
+                    L // 2 if use_quad else L, line_peak_half if use_quad else line_peak),
+                   ("rank", "quad lines" if use_quad else "query lines",
+                    L // 2 if use_quad else L, line_peak_half if use_quad else line_peak),
+                   ("select", "select lines", L, line_peak))},
         },
         "plain_directory": {
             "access_qps": Q / (np.mean(plain_ms["access"]) / 1e3),

@@ -505,6 +505,12 @@ class WaveletTree:
     def generateQueries(self, kind, seed, m, d_out, stream=None):
         call("ecl_wavelet_tree_generate_queries", self._h, kind, seed, m, d_out.ptr, stream)
 
+    def buildQuadLines(self):
+        """Quad lines (power-of-two alphabets with an even number of levels):
+        access and rank read one memory line per two levels; results do not
+        change."""
+        call("ecl_wavelet_tree_build_quad_lines", self._h)
+
     def buildHostBytes(self):
         """Bytes of the build's buffers that were placed in pinned host
         memory because the device could not hold them (0: none)."""

@@ -563,6 +563,8 @@ struct ecl_wavelet_tree_t {
   TreeView view{};
   uint32_t lanes = 0;  // lanes per access/rank query (0: the library-wide setting)
   uint64_t build_host_bytes = 0;  // build buffers placed in pinned host memory
+  DevBuf quad_lines, quad_base;   // optional quad lines and their bases
+  QuadView quad{};
   std::unique_ptr<HostPipe> pipe{new HostPipe()};
   ~ecl_wavelet_tree_t() { delete rs; }
 };
@@ -1541,6 +1543,85 @@ static uint64_t select_line_magic(uint32_t r) { return r > 1 ? ~0ull / r + 1 : 0
 // line every r Values, r = max(1, floor(800 x density)) so that the next r
 // Values lie in the line's 960 bits; the level views point to them
 // afterwards.  About 0.32 bytes per bit of every array.
+// Quad lines of a tree (QuadView): for quad level q, from the symbol
+// counts, per node N of level 2q (4^q nodes, N = the codes' top 2q bits) and
+// digit d: cnt[N][d] = the symbols with top 2q + 2 bits 4N + d; S_N = C of
+// the node's first symbol; CD[N][d] = the d's before the node (exclusive
+// prefix of cnt over N); base[4N + d] = C of child 4N + d minus CD[N][d].
+// The lines themselves come from the binary levels 2q and 2q + 1 on the
+// device.
+static void build_quad_lines(ecl_wavelet_tree_t* wt) {
+  if (wt->quad.levels != 0) return;
+  const uint32_t L = wt->L;
+  if (!wt->pow2 || L % 2 != 0 || L < 2 || L > 2 * kMaxQuadLevels)
+    throw Error(ECL_ERROR_INVALID_ARGUMENT,
+                "quad lines need a power-of-two alphabet with an even number of levels "
+                "(at most 16)");
+  const uint32_t Q = L / 2;
+  const uint64_t n = wt->n, sigma = wt->sigma;
+  // C[c] for c in [0, sigma], C[sigma] = n
+  std::vector<uint64_t> C(sigma + 1);
+  for (uint64_t c = 0; c < sigma; ++c) C[c] = wt->counts[c];
+  C[sigma] = n;
+  // host tables: bases of all levels; per level S and CD for the build
+  std::vector<uint64_t> base, build_tab;
+  std::vector<uint64_t> off(Q), s_off(Q), cd_off(Q), nodes(Q);
+  for (uint32_t q = 0; q < Q; ++q) {
+    const uint64_t nn = 1ull << (2 * q);  // nodes of level 2q
+    const uint32_t shift = L - 2 * q - 2;  // child 4N + d spans symbols [(4N+d) << shift, ...)
+    nodes[q] = nn;
+    off[q] = base.size();
+    s_off[q] = build_tab.size();
+    for (uint64_t N = 0; N <= nn; ++N) build_tab.push_back(C[std::min<uint64_t>(N << (shift + 2), sigma)]);
+    cd_off[q] = build_tab.size();
+    uint64_t acc[4] = {0, 0, 0, 0};
+    for (uint64_t N = 0; N <= nn; ++N) {
+      for (uint32_t d = 0; d < 4; ++d) build_tab.push_back(acc[d]);  // CD[N][d]
+      if (N == nn) break;
+      for (uint32_t d = 0; d < 4; ++d) {
+        const uint64_t lo = (4 * N + d) << shift, hi = (4 * N + d + 1) << shift;
+        const uint64_t cnt = C[hi] - C[lo];
+        base.push_back(C[lo] - acc[d]);  // base[4N + d]
+        acc[d] += cnt;
+      }
+    }
+  }
+  // lines: n / 448 + 1 per level
+  const uint64_t nlines = n / kQuadDigits + 1;
+  const uint64_t bytes = Q * nlines * 128;
+  const uint64_t tab_bytes = (base.size() + build_tab.size()) * 8;
+  if (bytes + tab_bytes + (64u << 20) > free_device_memory())
+    throw Error(ECL_ERROR_OUT_OF_MEMORY_TREE, kOomTree);
+  DevBuf lines(bytes);
+  DevBuf dbase(base.size() * 8);
+  DevBuf dtab(build_tab.size() * 8);
+  h2d(dbase.ptr, base.data(), base.size());
+  h2d(dtab.ptr, build_tab.data(), build_tab.size());
+  hipStream_t s = nullptr;
+  QuadView qv{};
+  for (uint32_t q = 0; q < Q; ++q) {
+    QuadBuildParams qp{};
+    qp.a = wt->rs->levels[2 * q];
+    qp.b = wt->rs->levels[2 * q + 1];
+    qp.S = dtab.as<uint64_t>() + s_off[q];
+    qp.CD = dtab.as<uint64_t>() + cd_off[q];
+    qp.nodes = nodes[q];
+    qp.n = n;
+    qp.nlines = nlines;
+    qp.out = lines.as<uint4>() + q * nlines * 8;
+    launch_quad_lines(qp, s);
+    qv.lines[q] = qp.out;
+    qv.off[q] = off[q];
+  }
+  ECL_HIP(hipStreamSynchronize(s));
+  qv.base = dbase.as<uint64_t>();
+  qv.nbase = base.size();
+  qv.levels = Q;
+  wt->quad_lines = std::move(lines);
+  wt->quad_base = std::move(dbase);
+  wt->quad = qv;
+}
+
 static void build_select_lines(ecl_rank_select_t* rs) {
   if (rs->sel_lines.ptr != nullptr) return;
   const size_t na = rs->levels.size();
@@ -1785,6 +1866,7 @@ static TreeQueryParams query_params(const ecl_wavelet_tree_t* wt, uint32_t flags
   p.has_lines = wt->rs->lines.ptr != nullptr ? 1 : 0;
   p.has_sel_lines = wt->rs->sel_lines.ptr != nullptr ? 1 : 0;
   p.lanes = wt->lanes;
+  p.quad = wt->quad;
   return p;
 }
 
@@ -2259,6 +2341,13 @@ ecl_status ecl_wavelet_tree_build_query_lines(ecl_wavelet_tree wt) {
     require(wt != nullptr, "NULL tree");
     set_device(wt->device);
     build_query_lines(wt->rs);
+  });
+}
+ecl_status ecl_wavelet_tree_build_quad_lines(ecl_wavelet_tree wt) {
+  return guard([&] {
+    require(wt != nullptr, "NULL tree");
+    set_device(wt->device);
+    build_quad_lines(wt);
   });
 }
 ecl_status ecl_wavelet_tree_build_select_lines(ecl_wavelet_tree wt) {

@@ -218,8 +218,42 @@ void launch_presence(const void* text, int esize, uint64_t n, uint32_t* flags,
                      hipStream_t s);
 
 // ---- launchers (kernels_query.hip) -------------------------------------------
+// Quad lines (opt-in; power-of-two alphabets with an even number of levels,
+// at most 16): quad level q holds the 2-bit digits of the codes (bits 2q and
+// 2q+1 from the top, the first one high) in the order of binary level 2q,
+// in 128-byte lines: the counts of digits 0, 1 and 2 before the line as
+// three 40-bit fields in the first 16 bytes, then 448 digits, digit i at
+// bits 2i..2i+1 of the 112 bytes.  With them access and rank read one line
+// per two levels.  Descending from position p in node N of quad level q by
+// digit d leads to position base[off[q] + 4N + d] + (d's before p) of quad
+// level q + 1 (base = the child's start minus the d's before node N).
+constexpr uint32_t kQuadDigits = 448;
+constexpr uint32_t kMaxQuadLevels = 8;
+struct QuadView {
+  const uint4* lines[kMaxQuadLevels];
+  const uint64_t* base;          // every level's bases
+  uint64_t off[kMaxQuadLevels];  // level q's bases start at base + off[q]
+  uint64_t nbase;                // entries of base
+  uint32_t levels;               // 0: no quad lines
+};
+
+// One quad level's build: binary levels a = 2q and b = 2q + 1, the level-2q
+// node starts S[0..nodes] (S[nodes] = n) and per node and digit the digits
+// before the node, CD[4N + d] for N in [0, nodes] (row nodes: the totals).
+struct QuadBuildParams {
+  LevelView a, b;
+  const uint64_t* S;
+  const uint64_t* CD;
+  uint64_t nodes;
+  uint64_t n;
+  uint64_t nlines;  // n / 448 + 1: position n has a line (its header: the totals)
+  uint4* out;
+};
+void launch_quad_lines(const QuadBuildParams& qp, hipStream_t s);
+
 struct TreeQueryParams {
   TreeView view;
+  QuadView quad;             // quad.levels == 0: none
   int esize;
   int symbols_minimal;       // 1: query symbols already minimal indices
   int map_kind; const uint32_t* lut; const void* alphabet;  // symbol map

@@ -781,6 +781,147 @@ void launch_select_lines(const LevelView& v, uint4* out, uint64_t n1, uint64_t n
   ECL_HIP(hipGetLastError());
 }
 
+// ---------------------------------------------------------------------------
+// Quad lines of one quad level q, from binary levels a = 2q and b = 2q + 1.
+// In node N of level 2q (positions [S_N, S_N+1)) the codes with bit 0 go,
+// in order, to positions [S_N, S_N + Z_N) of level 2q + 1 and those with bit
+// 1 to [S_N + Z_N, S_N+1), Z_N the node's zeros.  So the code at position p
+// with o ones and z zeros of level 2q before it in its node has its level
+// 2q + 1 bit at S_N + z (bit 0) or S_N + Z_N + o (bit 1): a word of 32
+// positions inside one node takes its second bits from two contiguous runs
+// of level 2q + 1.  Line headers: the d's before position P are the d's
+// before its node (CD) plus, for d = 1 and 3, the ones of level 2q + 1 in
+// the two runs up to P (two ranks), and for d = 0 and 2 the rest of the
+// node's zeros and ones before P.
+// A workgroup builds kQuadLinesPerWg lines in LDS and writes them out
+// contiguously.
+constexpr uint32_t kQuadLinesPerWg = 64;
+constexpr uint32_t kQuadWordsPerLine = kQuadDigits / 32;  // 14 words of 32 positions
+
+// last node N with S[N] <= pos (pos < n)
+__device__ __forceinline__ uint64_t quad_node(const uint64_t* S, uint64_t nodes, uint64_t pos) {
+  uint64_t lo = 0, hi = nodes;  // S[lo] <= pos < S[hi]
+  while (hi - lo > 1) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (detail::gld(S, mid) <= pos) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+// bits [x, x + 32) of a bit array of nw words (zeros past them)
+__device__ __forceinline__ uint32_t bits32(const uint32_t* w, uint64_t nw, uint64_t x) {
+  const uint64_t k = x >> 5;
+  const uint32_t sh = static_cast<uint32_t>(x & 31);
+  const uint32_t lo = k < nw ? detail::gld(w, k) : 0u;
+  if (sh == 0) return lo;
+  const uint32_t hi = k + 1 < nw ? detail::gld(w, k + 1) : 0u;
+  return (lo >> sh) | (hi << (32 - sh));
+}
+
+__global__ __launch_bounds__(256) void quad_lines_kernel(QuadBuildParams qp) {
+  __shared__ __align__(16) uint32_t s_lines[kQuadLinesPerWg * 32];
+  const uint32_t tid = threadIdx.x;
+  const uint64_t line0 = static_cast<uint64_t>(blockIdx.x) * kQuadLinesPerWg;
+  const uint64_t nl = qp.nlines - line0 < kQuadLinesPerWg ? qp.nlines - line0 : kQuadLinesPerWg;
+  const uint64_t n = qp.n, nw = (n + 31) / 32;
+  const uint32_t* A = qp.a.words;
+  const uint32_t* B = qp.b.words;
+  auto cd = [&](uint64_t N, uint32_t d) { return detail::gld(qp.CD, 4 * N + d); };
+  // digits of 32 positions per thread and round
+  for (uint32_t w = tid; w < nl * kQuadWordsPerLine; w += blockDim.x) {
+    const uint64_t k = line0 * kQuadWordsPerLine + w;  // word of positions [32k, 32k + 32)
+    const uint64_t P0 = 32 * k;
+    uint64_t D = 0;
+    if (P0 < n) {
+      const uint32_t valid = n - P0 < 32 ? static_cast<uint32_t>(n - P0) : 32u;
+      const uint32_t a = detail::gld(A, k);  // bits past n are zero
+      const uint64_t rA = detail::level_rank<1>(qp.a, P0);  // ones of level 2q before P0
+      uint64_t N = quad_node(qp.S, qp.nodes, P0);
+      uint64_t SN = detail::gld(qp.S, N), SN1 = detail::gld(qp.S, N + 1);
+      if (P0 + valid <= SN1) {
+        // one node: the second bits come from two runs of level 2q + 1
+        const uint64_t o0 = rA - (cd(N, 2) + cd(N, 3));
+        const uint64_t z0 = (P0 - SN) - o0;
+        const uint64_t ZN = (cd(N + 1, 0) - cd(N, 0)) + (cd(N + 1, 1) - cd(N, 1));
+        const uint32_t zb = bits32(B, nw, SN + z0);
+        const uint32_t ob = bits32(B, nw, SN + ZN + o0);
+        uint32_t iz = 0, io = 0;
+#pragma unroll 8
+        for (uint32_t j = 0; j < valid; ++j) {
+          const uint32_t aj = (a >> j) & 1u;
+          const uint32_t bj = aj ? (ob >> io) & 1u : (zb >> iz) & 1u;
+          io += aj;
+          iz += aj ^ 1u;
+          D |= static_cast<uint64_t>((aj << 1) | bj) << (2 * j);
+        }
+      } else {
+        // the word crosses node starts: position by position
+        uint32_t before = 0;  // ones of the word before position j
+        for (uint32_t j = 0; j < valid; ++j) {
+          const uint64_t pos = P0 + j;
+          while (pos >= SN1) {
+            ++N;
+            SN = SN1;
+            SN1 = detail::gld(qp.S, N + 1);
+          }
+          const uint64_t o = rA + before - (cd(N, 2) + cd(N, 3));
+          const uint64_t z = (pos - SN) - o;
+          const uint64_t ZN = (cd(N + 1, 0) - cd(N, 0)) + (cd(N + 1, 1) - cd(N, 1));
+          const uint32_t aj = (a >> j) & 1u;
+          const uint64_t pb = SN + (aj ? ZN + o : z);
+          const uint32_t bj = (detail::gld(B, pb >> 5) >> (pb & 31)) & 1u;
+          D |= static_cast<uint64_t>((aj << 1) | bj) << (2 * j);
+          before += aj;
+        }
+      }
+    }
+    const uint32_t li = w / kQuadWordsPerLine, wi = w - li * kQuadWordsPerLine;
+    uint32_t* dst = s_lines + li * 32 + 4 + 2 * wi;
+    dst[0] = static_cast<uint32_t>(D);
+    dst[1] = static_cast<uint32_t>(D >> 32);
+  }
+  // headers: the counts of digits 0, 1, 2 before the line's first position
+  if (tid < nl) {
+    const uint64_t P = (line0 + tid) * kQuadDigits;
+    uint64_t c0, c1, c2;
+    if (P >= n) {
+      c0 = cd(qp.nodes, 0);
+      c1 = cd(qp.nodes, 1);
+      c2 = cd(qp.nodes, 2);
+    } else {
+      const uint64_t N = quad_node(qp.S, qp.nodes, P);
+      const uint64_t SN = detail::gld(qp.S, N);
+      const uint64_t o = detail::level_rank<1>(qp.a, P) - (cd(N, 2) + cd(N, 3));
+      const uint64_t z = (P - SN) - o;
+      const uint64_t cnt1 = cd(N + 1, 1) - cd(N, 1);
+      const uint64_t ZN = (cd(N + 1, 0) - cd(N, 0)) + cnt1;
+      const uint64_t rB = cd(N, 1) + cd(N, 3);  // ones of level 2q + 1 before S_N
+      const uint64_t d1 = detail::level_rank<1>(qp.b, SN + z) - rB;
+      const uint64_t d3 = detail::level_rank<1>(qp.b, SN + ZN + o) - (rB + cnt1);
+      c0 = cd(N, 0) + (z - d1);
+      c1 = cd(N, 1) + d1;
+      c2 = cd(N, 2) + (o - d3);
+    }
+    uint32_t* h = s_lines + tid * 32;
+    const uint64_t h0 = c0 | (c1 << 40);
+    const uint64_t h1 = (c1 >> 24) | (c2 << 16);
+    h[0] = static_cast<uint32_t>(h0);
+    h[1] = static_cast<uint32_t>(h0 >> 32);
+    h[2] = static_cast<uint32_t>(h1);
+    h[3] = static_cast<uint32_t>(h1 >> 32);
+  }
+  __syncthreads();
+  uint4* out = qp.out + line0 * 8;
+  for (uint32_t i = tid; i < nl * 8; i += blockDim.x) out[i] = reinterpret_cast<const uint4*>(s_lines)[i];
+}
+
+void launch_quad_lines(const QuadBuildParams& qp, hipStream_t s) {
+  if (qp.nlines == 0) return;
+  const uint64_t blocks = (qp.nlines + kQuadLinesPerWg - 1) / kQuadLinesPerWg;
+  if (blocks > 0x7FFFFFFFull) throw Error(ECL_ERROR_INVALID_ARGUMENT, "bit array too large");
+  hipLaunchKernelGGL(quad_lines_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(256), 0, s, qp);
+  ECL_HIP(hipGetLastError());
+}
+
 void launch_query_lines(const LevelView& v, uint4* lines, uint64_t nlines, hipStream_t s) {
   if (nlines == 0) return;
   const uint64_t blocks = (nlines * 8 + 255) / 256;

@@ -137,6 +137,144 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void t
   }
 }
 
+// ---------------------------------------------------------------------------
+// Access and rank over quad lines (see QuadView): one 128-byte line per two
+// levels.  Two adjacent lanes share a query: lane 0 loads the line's header
+// and digit words 0..5, lane 1 words 6..13; each counts the digit in its
+// words before the offset, and one xor-shuffle adds the counts.  The bases
+// sit in LDS when they fit.
+constexpr uint64_t kQuadLdsBase = 4096;  // u64 entries
+
+__host__ __device__ __forceinline__ size_t quad_lds_bytes(const QuadView& q) {
+  return q.nbase <= kQuadLdsBase ? q.nbase * sizeof(uint64_t) : 0;
+}
+__device__ __forceinline__ const uint64_t* stage_quad_base(const QuadView& q, uint64_t* lds) {
+  if (q.nbase > kQuadLdsBase) return q.base;
+  for (uint64_t i = threadIdx.x; i < q.nbase; i += blockDim.x) lds[i] = q.base[i];
+  __syncthreads();
+  return lds;
+}
+
+// matches of digit d among the 32 digits of w, as one bit per digit (bit 2i)
+__device__ __forceinline__ uint64_t digit_matches(uint64_t w, uint32_t d) {
+  const uint64_t y = w ^ (0x5555555555555555ull * d);
+  return ~(y | (y >> 1)) & 0x5555555555555555ull;
+}
+
+// one quad level for the pair: the digit at offset o of line `line` (when
+// want_digit; otherwise d is given) and the d's before position line*448+o
+struct QuadStep {
+  uint32_t d;
+  uint64_t rank;
+};
+__device__ __forceinline__ QuadStep quad_step(const uint4* lines, uint64_t p, bool want_digit,
+                                              uint32_t d_given) {
+  const uint32_t h = threadIdx.x & 1u;
+  // p / 448 = (p / 64) / 7, the division by 7 as a multiply-high
+  const uint64_t line = __umul64hi(p >> 6, 0x2492492492492493ull);
+  const uint32_t o = static_cast<uint32_t>(p - line * kQuadDigits);
+  const uint4* lp = lines + line * 8 + 4 * h;
+  const uint4 x0 = detail::gld(lp, 0), x1 = detail::gld(lp, 1), x2 = detail::gld(lp, 2),
+              x3 = detail::gld(lp, 3);
+  // this lane's digit words: lane 0 words 0..5 (x1..x3), lane 1 words 6..13
+  uint64_t w[8];
+  w[0] = h ? ((uint64_t)x0.x | ((uint64_t)x0.y << 32)) : ((uint64_t)x1.x | ((uint64_t)x1.y << 32));
+  w[1] = h ? ((uint64_t)x0.z | ((uint64_t)x0.w << 32)) : ((uint64_t)x1.z | ((uint64_t)x1.w << 32));
+  w[2] = h ? ((uint64_t)x1.x | ((uint64_t)x1.y << 32)) : ((uint64_t)x2.x | ((uint64_t)x2.y << 32));
+  w[3] = h ? ((uint64_t)x1.z | ((uint64_t)x1.w << 32)) : ((uint64_t)x2.z | ((uint64_t)x2.w << 32));
+  w[4] = h ? ((uint64_t)x2.x | ((uint64_t)x2.y << 32)) : ((uint64_t)x3.x | ((uint64_t)x3.y << 32));
+  w[5] = h ? ((uint64_t)x2.z | ((uint64_t)x2.w << 32)) : ((uint64_t)x3.z | ((uint64_t)x3.w << 32));
+  w[6] = h ? ((uint64_t)x3.x | ((uint64_t)x3.y << 32)) : 0ull;
+  w[7] = h ? ((uint64_t)x3.z | ((uint64_t)x3.w << 32)) : 0ull;
+  const uint32_t kb = h ? 6u : 0u;   // first word of this lane
+  const uint32_t ko = o >> 5;        // word holding offset o
+  const uint32_t sh = 2 * (o & 31);  // its bit
+  uint32_t d = d_given;
+  if (want_digit) {
+    uint64_t sel = 0;
+#pragma unroll
+    for (uint32_t j = 0; j < 8; ++j)
+      if (kb + j == ko) sel = w[j];
+    const uint32_t mine = static_cast<uint32_t>((sel >> sh) & 3u);
+    const uint32_t owner = ko < 6 ? 0u : 1u;
+    // the owner's digit to both lanes of the pair
+    const uint32_t other = __shfl_xor(mine, 1, 64);
+    d = owner == h ? mine : other;
+  }
+  uint32_t cnt = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < 8; ++j) {
+    const uint32_t k = kb + j;
+    if (k <= ko && (h || j < 6)) {
+      uint64_t m = digit_matches(w[j], d);
+      if (k == ko) m &= sh == 0 ? 0ull : (~0ull >> (64 - sh));
+      cnt += static_cast<uint32_t>(__popcll(m));
+    }
+  }
+  cnt += __shfl_xor(cnt, 1, 64);
+  // lane 0 holds the header: counts of digits 0, 1, 2 before the line
+  uint64_t hdr = 0;
+  if (h == 0) {
+    const uint64_t h0 = (uint64_t)x0.x | ((uint64_t)x0.y << 32);
+    const uint64_t h1 = (uint64_t)x0.z | ((uint64_t)x0.w << 32);
+    const uint64_t m40 = (1ull << 40) - 1;
+    const uint64_t c0 = h0 & m40, c1 = (h0 >> 40) | ((h1 & 0xFFFFull) << 24),
+                   c2 = (h1 >> 16) & m40;
+    hdr = d == 0 ? c0 : d == 1 ? c1 : d == 2 ? c2 : line * kQuadDigits - c0 - c1 - c2;
+  }
+  hdr = __shfl(hdr, static_cast<int>((threadIdx.x & 63u) & ~1u), 64);
+  return QuadStep{d, hdr + cnt};
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void tree_access_quad_kernel(
+    TreeQueryParams p, const uint64_t* __restrict__ idx, uint64_t m, T* __restrict__ out) {
+  extern __shared__ uint64_t s_qbase[];
+  const uint64_t* base = stage_quad_base(p.quad, s_qbase);
+  const T* amap = static_cast<const T*>(p.alphabet_out);
+  const uint32_t Q = p.quad.levels;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x / 2;
+  for (uint64_t q = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) / 2;
+       q < m; q += stride) {
+    const uint64_t i = idx[q];
+    if (i >= p.view.n) continue;  // accessKernel skips such indices (both lanes)
+    uint64_t pos = i, N = 0;
+    for (uint32_t l = 0; l < Q; ++l) {
+      const QuadStep st = quad_step(p.quad.lines[l], pos, true, 0);
+      pos = base[p.quad.off[l] + 4 * N + st.d] + st.rank;
+      N = 4 * N + st.d;
+    }
+    if ((threadIdx.x & 1) == 0) out[q] = amap ? amap[N] : static_cast<T>(N);
+  }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void tree_rank_quad_kernel(
+    TreeQueryParams p, const uint8_t* __restrict__ qs, uint64_t m, uint64_t* __restrict__ out) {
+  extern __shared__ uint64_t s_qbase[];
+  const uint64_t* base = stage_quad_base(p.quad, s_qbase);
+  const uint32_t Q = p.quad.levels;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x / 2;
+  for (uint64_t q = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) / 2;
+       q < m; q += stride) {
+    const uint8_t* e = qs + q * 16;
+    const uint64_t i = *reinterpret_cast<const uint64_t*>(e);
+    const uint64_t c = map_symbol<T>(p, load_sym(e, sizeof(T)));
+    uint64_t r = 0;
+    if (i <= p.view.n && c < p.view.sigma) {
+      uint64_t pos = i, N = 0;
+      for (uint32_t l = 0; l < Q; ++l) {
+        const uint32_t d = static_cast<uint32_t>(c >> (2 * (Q - 1 - l))) & 3u;
+        const QuadStep st = quad_step(p.quad.lines[l], pos, false, d);
+        pos = base[p.quad.off[l] + 4 * N + d] + st.rank;
+        N = 4 * N + d;
+      }
+      r = pos - detail::gld(p.view.counts, c);
+    }
+    if ((threadIdx.x & 1) == 0) out[q] = r;
+  }
+}
+
 // Occupancy target: with select lines (and without the hint path, below),
 // 7 waves per SIMD (68 VGPRs, no spills, instead of the compiler's 123 and 4
 // waves): 2.7 vs 3.0 ms per 10^7 at n = 6e9, sigma = 256.  Without them the
@@ -572,7 +710,16 @@ void launch_tree_access(const TreeQueryParams& p, const uint64_t* idx,
                         uint64_t m, void* out, hipStream_t s) {
   if (m == 0) return;
   ProfScope ps(K_WT_ACCESS, s);
-  if (pair_lanes(p)) access_g<2>(p, idx, m, out, s);
+  if (p.quad.levels) {
+    const dim3 g(query_grid(2 * m)), b(256);
+    const size_t sh = quad_lds_bytes(p.quad);
+    switch (p.esize) {
+      case 1: hipLaunchKernelGGL((tree_access_quad_kernel<uint8_t>), g, b, sh, s, p, idx, m, static_cast<uint8_t*>(out)); break;
+      case 2: hipLaunchKernelGGL((tree_access_quad_kernel<uint16_t>), g, b, sh, s, p, idx, m, static_cast<uint16_t*>(out)); break;
+      case 4: hipLaunchKernelGGL((tree_access_quad_kernel<uint32_t>), g, b, sh, s, p, idx, m, static_cast<uint32_t*>(out)); break;
+      default: hipLaunchKernelGGL((tree_access_quad_kernel<uint64_t>), g, b, sh, s, p, idx, m, static_cast<uint64_t*>(out)); break;
+    }
+  } else if (pair_lanes(p)) access_g<2>(p, idx, m, out, s);
   else access_g<1>(p, idx, m, out, s);
   ECL_HIP(hipGetLastError());
 }
@@ -582,7 +729,16 @@ void launch_tree_rank(const TreeQueryParams& p, const void* q, uint64_t m,
   if (m == 0) return;
   ProfScope ps(K_WT_RANK, s);
   const uint8_t* qq = static_cast<const uint8_t*>(q);
-  if (pair_lanes(p)) rank_g<2>(p, qq, m, out, s);
+  if (p.quad.levels) {
+    const dim3 g(query_grid(2 * m)), b(256);
+    const size_t sh = quad_lds_bytes(p.quad);
+    switch (p.esize) {
+      case 1: hipLaunchKernelGGL((tree_rank_quad_kernel<uint8_t>), g, b, sh, s, p, qq, m, out); break;
+      case 2: hipLaunchKernelGGL((tree_rank_quad_kernel<uint16_t>), g, b, sh, s, p, qq, m, out); break;
+      case 4: hipLaunchKernelGGL((tree_rank_quad_kernel<uint32_t>), g, b, sh, s, p, qq, m, out); break;
+      default: hipLaunchKernelGGL((tree_rank_quad_kernel<uint64_t>), g, b, sh, s, p, qq, m, out); break;
+    }
+  } else if (pair_lanes(p)) rank_g<2>(p, qq, m, out, s);
   else rank_g<1>(p, qq, m, out, s);
   ECL_HIP(hipGetLastError());
 }

@@ -340,6 +340,18 @@ ecl_status ecl_rank_select_build_query_lines(ecl_rank_select rs);
  * build them while no query on this tree is in flight. */
 ecl_status ecl_wavelet_tree_build_select_lines(ecl_wavelet_tree wt);
 ecl_status ecl_rank_select_build_select_lines(ecl_rank_select rs);
+/* Quad lines (no counterpart in the reference; opt-in, for trees over a
+ * power-of-two alphabet with an even number of levels, at most 16: sigma =
+ * 4, 16, 64, 256, ..., 65536): for every pair of levels (2q, 2q + 1), the
+ * codes' 2-bit digits in the order of level 2q, in 128-byte lines of 448
+ * digits headed by the counts of digits 0, 1 and 2 before the line.  Access
+ * and rank then read one memory line per two levels (the tree's host and
+ * device access/rank entry points use them; select, mixed batches and the
+ * device API keep the binary levels).  Costs 128/112 of the bits (about one
+ * byte per symbol at sigma = 256); results do not change.  Other trees get
+ * ECL_ERROR_INVALID_ARGUMENT.  A second call keeps the existing lines; build
+ * them while no query on the tree is in flight. */
+ecl_status ecl_wavelet_tree_build_quad_lines(ecl_wavelet_tree wt);
 /* Persistence (no counterpart in the reference, which rebuilds every tree
  * from its text; for warm starts of servers).  The file holds a header, the
  * alphabet, the symbol counts and every level's bits; loading checks them

@@ -751,6 +751,75 @@ def test_mixed_batch_query_lines(gpu, m, text_kind):
             assert np.array_equal(d_out.download(np.uint64, m), exp), (mix, stage)
 
 
+QUAD_CASES = ["u8_4", "u8_16", "u8_64", "u8_256", "u8_256_skewed", "u16_16_alpha",
+              "u16_65536", "u8_256_n448k"]
+
+
+def _quad_text(case, rng):
+    """(text, alphabet) for a quad-lines case: power-of-two alphabets with an
+    even number of levels, uniform or skewed (with absent symbols), minimal
+    or not, n a multiple of 448 or not."""
+    if case == "u8_256_skewed":
+        text = np.minimum(rng.zipf(1.3, 1_000_003) - 1, 255).astype(np.uint8)
+        text[text == 7] = 8  # symbol 7 absent
+        return text, np.arange(256, dtype=np.uint8)
+    if case == "u16_16_alpha":
+        alpha = np.sort(rng.choice(60000, 16, replace=False)).astype(np.uint16)
+        return alpha[rng.integers(0, 16, 777_777)], alpha
+    if case == "u16_65536":
+        return rng.integers(0, 65536, 2_000_011).astype(np.uint16), np.arange(65536, dtype=np.uint16)
+    if case == "u8_256_n448k":
+        return rng.integers(0, 256, 448 * 1000).astype(np.uint8), np.arange(256, dtype=np.uint8)
+    sigma = int(case.split("_")[1])
+    return rng.integers(0, sigma, 1_234_567).astype(np.uint8), np.arange(sigma, dtype=np.uint8)
+
+
+@pytest.mark.parametrize("case", QUAD_CASES)
+def test_quad_lines_same_answers(gpu, case):
+    """Access and rank over the quad lines (2-bit digits of two levels per
+    128-byte line of 448) equal the direct scans of the text: random
+    queries, every line boundary (448 j - 1, 448 j, 448 j + 1), every node
+    boundary of every quad level (the C array entries), positions 0, n - 1
+    and n, and every symbol including absent ones; select keeps its own
+    path and stays exact."""
+    from ecl_wavelet_amd import WaveletTree
+    rng = np.random.default_rng(sum(map(ord, case)))
+    text, alpha = _quad_text(case, rng)
+    n = len(text)
+    wt = WaveletTree(text, alpha)
+    wt.buildQuadLines()
+    wt.buildQuadLines()  # a second call keeps the lines
+    _check_queries(wt, text, None, nq=4000, seed=len(case))
+    # access at line boundaries and node starts
+    lb = np.arange(0, n + 448, 448, dtype=np.int64)
+    pos = np.concatenate([lb - 1, lb, lb + 1, [0, n - 1], wt.getCounts().astype(np.int64)])
+    pos = np.unique(pos[(pos >= 0) & (pos < n)]).astype(np.uint64)
+    assert np.array_equal(wt.access(pos), text[pos])
+    # rank at the same positions and n, for a spread of symbols (absent ones too)
+    ri = np.unique(np.concatenate([pos, [n]])).astype(np.uint64)
+    syms = alpha[rng.integers(0, len(alpha), 8)]
+    for c in np.unique(np.concatenate([syms, alpha[:1], alpha[-1:]])):
+        cs = np.full(len(ri), c, dtype=alpha.dtype)
+        exp = O.scan_rank_batch(text, ri, cs.astype(np.uint64))
+        assert np.array_equal(wt.rank(wt.make_queries(ri, cs)), exp), int(c)
+
+
+@pytest.mark.parametrize("sigma", [2, 8, 20, 128])
+def test_quad_lines_reject_other_trees(gpu, sigma):
+    """Alphabets that are not a power of two, or whose level count is odd,
+    get no quad lines, and their queries keep working."""
+    from ecl_wavelet_amd import WaveletTree
+    from ecl_wavelet_amd._lib import EclError, ECL_ERROR_INVALID_ARGUMENT
+    rng = np.random.default_rng(sigma)
+    text = rng.integers(0, sigma, 100_003).astype(np.uint8)
+    wt = WaveletTree(text, np.arange(sigma, dtype=np.uint8))
+    with pytest.raises(EclError) as e:
+        wt.buildQuadLines()
+    assert e.value.status == ECL_ERROR_INVALID_ARGUMENT
+    del e
+    _check_queries(wt, text, None, nq=500, seed=sigma)
+
+
 def test_tree_from_raw_text_file(gpu, tmp_path):
     """A tree over a prefix of a raw text file (readDataFromFile,
     test_benchmark_utils.cuh:588-619) with its alphabet discovered equals the
