@@ -1595,6 +1595,20 @@ static void build_quad_lines(ecl_wavelet_tree_t* wt) {
   DevBuf lines(bytes);
   DevBuf dbase(base.size() * 8);
   DevBuf dtab(build_tab.size() * 8);
+  DevBuf wg_info(quad_build_workgroups(nlines) * kQuadWgInfo * 8);  // reused level by level
+  // the 4-position deposit table (QuadBuildParams::lut)
+  std::vector<uint8_t> lut(4096);
+  for (uint32_t x = 0; x < 4096; ++x) {
+    const uint32_t a = x & 15, z = (x >> 4) & 15, o = x >> 8;
+    uint32_t iz = 0, io = 0, v = 0;
+    for (uint32_t i = 0; i < 4; ++i) {
+      const uint32_t ai = (a >> i) & 1, bi = ai ? (o >> io++) & 1 : (z >> iz++) & 1;
+      v |= ((ai << 1) | bi) << (2 * i);
+    }
+    lut[x] = static_cast<uint8_t>(v);
+  }
+  DevBuf dlut(4096);
+  h2d(dlut.ptr, lut.data(), 4096);
   h2d(dbase.ptr, base.data(), base.size());
   h2d(dtab.ptr, build_tab.data(), build_tab.size());
   hipStream_t s = nullptr;
@@ -1609,6 +1623,8 @@ static void build_quad_lines(ecl_wavelet_tree_t* wt) {
     qp.n = n;
     qp.nlines = nlines;
     qp.out = lines.as<uint4>() + q * nlines * 8;
+    qp.wg_info = wg_info.as<uint64_t>();
+    qp.lut = dlut.as<uint8_t>();
     launch_quad_lines(qp, s);
     qv.lines[q] = qp.out;
     qv.off[q] = off[q];

@@ -248,7 +248,14 @@ struct QuadBuildParams {
   uint64_t n;
   uint64_t nlines;  // n / 448 + 1: position n has a line (its header: the totals)
   uint4* out;
+  uint64_t* wg_info;  // scratch: kQuadWgInfo u64 per build workgroup
+  // 4096 bytes: entry a | z << 4 | o << 8 holds the 4 digits of 4
+  // positions whose first bits are a (bit i: position i), whose zeros take
+  // their second bits in order from z and whose ones from o
+  const uint8_t* lut;
 };
+constexpr uint32_t kQuadWgInfo = 6;
+uint64_t quad_build_workgroups(uint64_t nlines);
 void launch_quad_lines(const QuadBuildParams& qp, hipStream_t s);
 
 struct TreeQueryParams {

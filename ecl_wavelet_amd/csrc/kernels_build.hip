@@ -817,42 +817,135 @@ __device__ __forceinline__ uint32_t bits32(const uint32_t* w, uint64_t nw, uint6
   return (lo >> sh) | (hi << (32 - sh));
 }
 
+// node holding pos, searched forward from a node N0 with S[N0] <= pos
+__device__ __forceinline__ uint64_t quad_node_from(const uint64_t* S, uint64_t nodes, uint64_t N0,
+                                                   uint64_t pos) {
+  if (pos < detail::gld(S, N0 + 1)) return N0;
+  uint64_t lo = N0 + 1, hi = nodes;  // S[lo] <= pos < S[hi]
+  while (hi - lo > 1) {
+    const uint64_t mid = (lo + hi) >> 1;
+    if (detail::gld(S, mid) <= pos) lo = mid; else hi = mid;
+  }
+  return lo;
+}
+
+// Thread t takes the 4 words (128 positions) from word 4t of the
+// workgroup's range; a block scan of their ones gives every word's rank from
+// one directory rank at the range's start, and the node search runs once per
+// workgroup (a thread searches only past a node start).
 __global__ __launch_bounds__(256) void quad_lines_kernel(QuadBuildParams qp) {
+  constexpr uint32_t kWords = kQuadLinesPerWg * kQuadWordsPerLine;  // 896
   __shared__ __align__(16) uint32_t s_lines[kQuadLinesPerWg * 32];
-  const uint32_t tid = threadIdx.x;
+  __shared__ uint32_t s_wrank[kWords];  // ones of level 2q before each word, from the range start
+  __shared__ uint32_t s_wave[4];
+  // from quad_wg_kernel: ones of level 2q before the range, the node holding
+  // its start (node0), node0's start and end, the ones of level 2q before
+  // node0 and the zeros in it
+  __shared__ uint64_t s_info[kQuadWgInfo];
+  __shared__ __align__(16) uint8_t s_lut[4096];
+  reinterpret_cast<uint4*>(s_lut)[threadIdx.x] =
+      detail::gld(reinterpret_cast<const uint4*>(qp.lut), threadIdx.x);
+  const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
   const uint64_t line0 = static_cast<uint64_t>(blockIdx.x) * kQuadLinesPerWg;
   const uint64_t nl = qp.nlines - line0 < kQuadLinesPerWg ? qp.nlines - line0 : kQuadLinesPerWg;
+  const uint32_t nwords = static_cast<uint32_t>(nl) * kQuadWordsPerLine;
   const uint64_t n = qp.n, nw = (n + 31) / 32;
+  const uint64_t k0 = line0 * kQuadWordsPerLine;  // first word (a multiple of 4)
   const uint32_t* A = qp.a.words;
   const uint32_t* B = qp.b.words;
   auto cd = [&](uint64_t N, uint32_t d) { return detail::gld(qp.CD, 4 * N + d); };
-  // digits of 32 positions per thread and round
-  for (uint32_t w = tid; w < nl * kQuadWordsPerLine; w += blockDim.x) {
-    const uint64_t k = line0 * kQuadWordsPerLine + w;  // word of positions [32k, 32k + 32)
-    const uint64_t P0 = 32 * k;
+  // this thread's words (zero past the array) and their ones
+  uint32_t a4[4] = {0u, 0u, 0u, 0u};
+  const uint64_t kt = k0 + 4ull * tid;
+  if (4 * tid < nwords && kt < nw) {
+    if (kt + 4 <= nw) {
+      const uint4 q = detail::gld(reinterpret_cast<const uint4*>(A + kt), 0);
+      a4[0] = q.x; a4[1] = q.y; a4[2] = q.z; a4[3] = q.w;
+    } else {
+      for (uint32_t i = 0; i < 4; ++i) a4[i] = kt + i < nw ? detail::gld(A, kt + i) : 0u;
+    }
+  }
+  const uint32_t pc = __popc(a4[0]) + __popc(a4[1]) + __popc(a4[2]) + __popc(a4[3]);
+  uint32_t incl = pc;
+#pragma unroll
+  for (uint32_t off = 1; off < 64; off <<= 1) {
+    const uint32_t y = __shfl_up(incl, off, 64);
+    if (lane >= off) incl += y;
+  }
+  if (lane == 63) s_wave[wave] = incl;
+  if (tid < kQuadWgInfo) s_info[tid] = detail::gld(qp.wg_info, blockIdx.x * kQuadWgInfo + tid);
+  __syncthreads();
+  uint32_t run = incl - pc;
+  for (uint32_t w = 0; w < wave; ++w) run += s_wave[w];
+  for (uint32_t i = 0; i < 4; ++i) {
+    if (4 * tid + i < nwords) s_wrank[4 * tid + i] = run;
+    run += __popc(a4[i]);
+  }
+  const uint64_t rank0 = s_info[0], node0 = s_info[1];
+  const uint64_t SN0 = s_info[2], SN10 = s_info[3], OB0 = s_info[4], ZN0 = s_info[5];
+  __syncthreads();
+  // digits: 32 positions per word.  Words inside node0 (nearly all) take its
+  // parameters from LDS and fetch their two runs of level 2q + 1 first, all
+  // four words' runs at once; the others go the general way.
+  uint32_t zb[4], ob[4];
+  bool fast[4];
+#pragma unroll
+  for (uint32_t i = 0; i < 4; ++i) {
+    const uint32_t w = 4 * tid + i;
+    const uint64_t P0 = 32 * (k0 + w);
+    fast[i] = false;
+    zb[i] = ob[i] = 0;
+    if (w < nwords && P0 < n) {
+      const uint32_t valid = n - P0 < 32 ? static_cast<uint32_t>(n - P0) : 32u;
+      if (P0 + valid <= SN10) {
+        const uint64_t o0 = rank0 + s_wrank[w] - OB0;
+        const uint64_t z0 = (P0 - SN0) - o0;
+        zb[i] = bits32(B, nw, SN0 + z0);
+        ob[i] = bits32(B, nw, SN0 + ZN0 + o0);
+        fast[i] = true;
+      }
+    }
+  }
+  for (uint32_t i = 0; i < 4; ++i) {
+    const uint32_t w = 4 * tid + i;
+    if (w >= nwords) break;
+    const uint64_t P0 = 32 * (k0 + w);
     uint64_t D = 0;
     if (P0 < n) {
       const uint32_t valid = n - P0 < 32 ? static_cast<uint32_t>(n - P0) : 32u;
-      const uint32_t a = detail::gld(A, k);  // bits past n are zero
-      const uint64_t rA = detail::level_rank<1>(qp.a, P0);  // ones of level 2q before P0
-      uint64_t N = quad_node(qp.S, qp.nodes, P0);
-      uint64_t SN = detail::gld(qp.S, N), SN1 = detail::gld(qp.S, N + 1);
-      if (P0 + valid <= SN1) {
-        // one node: the second bits come from two runs of level 2q + 1
-        const uint64_t o0 = rA - (cd(N, 2) + cd(N, 3));
-        const uint64_t z0 = (P0 - SN) - o0;
-        const uint64_t ZN = (cd(N + 1, 0) - cd(N, 0)) + (cd(N + 1, 1) - cd(N, 1));
-        const uint32_t zb = bits32(B, nw, SN + z0);
-        const uint32_t ob = bits32(B, nw, SN + ZN + o0);
-        uint32_t iz = 0, io = 0;
-#pragma unroll 8
-        for (uint32_t j = 0; j < valid; ++j) {
-          const uint32_t aj = (a >> j) & 1u;
-          const uint32_t bj = aj ? (ob >> io) & 1u : (zb >> iz) & 1u;
-          io += aj;
-          iz += aj ^ 1u;
-          D |= static_cast<uint64_t>((aj << 1) | bj) << (2 * j);
+      const uint32_t a = a4[i];  // bits past n are zero
+      const uint64_t rA = rank0 + s_wrank[w];  // ones of level 2q before P0
+      uint32_t zbi = zb[i], obi = ob[i];
+      bool one_node = fast[i];
+      uint64_t N = node0, SN = SN0, SN1 = SN10;
+      if (!one_node) {
+        N = quad_node_from(qp.S, qp.nodes, node0, P0);
+        SN = detail::gld(qp.S, N);
+        SN1 = detail::gld(qp.S, N + 1);
+        if (P0 + valid <= SN1) {
+          const uint64_t o0 = rA - (cd(N, 2) + cd(N, 3));
+          const uint64_t z0 = (P0 - SN) - o0;
+          const uint64_t ZN = (cd(N + 1, 0) - cd(N, 0)) + (cd(N + 1, 1) - cd(N, 1));
+          zbi = bits32(B, nw, SN + z0);
+          obi = bits32(B, nw, SN + ZN + o0);
+          one_node = true;
         }
+      }
+      if (one_node) {
+        // one node: the second bits come from the two runs, 4 positions per
+        // table lookup (at most 28 of a run's 32 bits precede a nibble)
+        uint32_t iz = 0, io = 0;
+#pragma unroll
+        for (uint32_t k = 0; k < 8; ++k) {
+          const uint32_t a4 = (a >> (4 * k)) & 15u;
+          const uint32_t z4 = (zbi >> iz) & 15u, o4 = (obi >> io) & 15u;
+          D |= static_cast<uint64_t>(s_lut[a4 | (z4 << 4) | (o4 << 8)]) << (8 * k);
+          const uint32_t c = __popc(a4);
+          io += c;
+          iz += 4u - c;
+        }
+        // positions past n (the last word) hold no digits
+        if (valid < 32) D &= (1ull << (2 * valid)) - 1;
       } else {
         // the word crosses node starts: position by position
         uint32_t before = 0;  // ones of the word before position j
@@ -888,9 +981,9 @@ __global__ __launch_bounds__(256) void quad_lines_kernel(QuadBuildParams qp) {
       c1 = cd(qp.nodes, 1);
       c2 = cd(qp.nodes, 2);
     } else {
-      const uint64_t N = quad_node(qp.S, qp.nodes, P);
+      const uint64_t N = quad_node_from(qp.S, qp.nodes, node0, P);
       const uint64_t SN = detail::gld(qp.S, N);
-      const uint64_t o = detail::level_rank<1>(qp.a, P) - (cd(N, 2) + cd(N, 3));
+      const uint64_t o = rank0 + s_wrank[tid * kQuadWordsPerLine] - (cd(N, 2) + cd(N, 3));
       const uint64_t z = (P - SN) - o;
       const uint64_t cnt1 = cd(N + 1, 1) - cd(N, 1);
       const uint64_t ZN = (cd(N + 1, 0) - cd(N, 0)) + cnt1;
@@ -914,10 +1007,36 @@ __global__ __launch_bounds__(256) void quad_lines_kernel(QuadBuildParams qp) {
   for (uint32_t i = tid; i < nl * 8; i += blockDim.x) out[i] = reinterpret_cast<const uint4*>(s_lines)[i];
 }
 
+// Per build workgroup g (positions from P = 448 * 64 g): the ones of level
+// 2q before P, the node N holding P (the last position when P >= n), N's
+// start and end, the ones of level 2q before N and the zeros in N -- the
+// chain of dependent loads (directory rank, node search, tables) that would
+// otherwise hold up every build workgroup, here one thread each, all at once.
+__global__ __launch_bounds__(256) void quad_wg_kernel(QuadBuildParams qp, uint64_t nwg) {
+  const uint64_t g = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (g >= nwg) return;
+  const uint64_t P = g * kQuadLinesPerWg * kQuadDigits;
+  const uint64_t N = quad_node(qp.S, qp.nodes, P < qp.n ? P : qp.n - 1);
+  auto cd = [&](uint64_t M, uint32_t d) { return detail::gld(qp.CD, 4 * M + d); };
+  uint64_t* o = qp.wg_info + g * kQuadWgInfo;
+  o[0] = detail::level_rank<1>(qp.a, P);
+  o[1] = N;
+  o[2] = detail::gld(qp.S, N);
+  o[3] = detail::gld(qp.S, N + 1);
+  o[4] = cd(N, 2) + cd(N, 3);
+  o[5] = (cd(N + 1, 0) - cd(N, 0)) + (cd(N + 1, 1) - cd(N, 1));
+}
+
+uint64_t quad_build_workgroups(uint64_t nlines) {
+  return (nlines + kQuadLinesPerWg - 1) / kQuadLinesPerWg;
+}
+
 void launch_quad_lines(const QuadBuildParams& qp, hipStream_t s) {
   if (qp.nlines == 0) return;
-  const uint64_t blocks = (qp.nlines + kQuadLinesPerWg - 1) / kQuadLinesPerWg;
+  const uint64_t blocks = quad_build_workgroups(qp.nlines);
   if (blocks > 0x7FFFFFFFull) throw Error(ECL_ERROR_INVALID_ARGUMENT, "bit array too large");
+  hipLaunchKernelGGL(quad_wg_kernel, dim3(static_cast<uint32_t>((blocks + 255) / 256)), dim3(256),
+                     0, s, qp, blocks);
   hipLaunchKernelGGL(quad_lines_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(256), 0, s, qp);
   ECL_HIP(hipGetLastError());
 }
