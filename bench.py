@@ -788,11 +788,12 @@ def main():
     build_ms, lines_ms, sel_ms, q_ms, mixed_ms = [], [], [], [], []
     quad_ms, plain_total_ms = [], []
     kind_ms = {"access": [], "rank": [], "select": []}
-    quad_kind_ms = {"access": [], "rank": []}
+    quad_kind_ms = {"access": [], "rank": [], "select": []}
+    quad_build_ms, quad_sel_build_ms = [], []
     # quad lines: power-of-two alphabets with an even number of levels
     use_quad = sigma >= 4 and (sigma & (sigma - 1)) == 0 and (sigma.bit_length() - 1) % 2 == 0
     plain_ms = {"access": [], "rank": [], "select": [], "mixed": []}
-    lines_bytes = sel_bytes = 0
+    lines_bytes = sel_bytes = qsel_bytes = 0
     wt = None
     pass_launch_ms = []
     build_kernels = {}
@@ -841,6 +842,15 @@ def main():
             timer.start()
             wt.rankDevice(d_rq, Q, d_out, flags=1)
             qt["rank"] = timer.stop()
+            # quad select lines, then select over them
+            free2 = E.mem_info()[0]
+            timer.start()
+            wt.buildQuadSelectLines()
+            qsb = timer.stop()
+            qsel_bytes = free2 - E.mem_info()[0]
+            timer.start()
+            wt.selectDevice(d_sq, Q, d_out, flags=1)
+            qt["select"] = timer.stop()
         log(f"step {step}{'' if timed else ' (warmup)'}: build {b:.1f} ms, "
             f"plain access {plain['access']:.2f} ms, rank {plain['rank']:.2f} ms, "
             f"select {plain['select']:.2f} ms, mixed {plain['mixed']:.2f} ms; lines {lb:.2f} ms, "
@@ -848,7 +858,8 @@ def main():
             f"access {times['access']:.2f} ms, rank {times['rank']:.2f} ms, "
             f"select {times['select']:.2f} ms, mixed {mx:.2f} ms"
             + (f"; quad lines {qb:.2f} ms, access {qt['access']:.2f} ms, "
-               f"rank {qt['rank']:.2f} ms" if use_quad else ""))
+               f"rank {qt['rank']:.2f} ms; quad select lines {qsb:.2f} ms, "
+               f"select {qt['select']:.2f} ms" if use_quad else ""))
         if timed:
             build_ms.append(b)
             lines_ms.append(lb)
@@ -861,7 +872,9 @@ def main():
             for k, v in plain.items():
                 plain_ms[k].append(v)
             if use_quad:
-                quad_ms.append(qb + qt["access"] + qt["rank"])
+                quad_ms.append(qb + qsb + qt["access"] + qt["rank"] + qt["select"])
+                quad_build_ms.append(qb)
+                quad_sel_build_ms.append(qsb)
                 for k, v in qt.items():
                     quad_kind_ms[k].append(v)
 
@@ -948,7 +961,7 @@ def main():
                          f"{Q} each of access/rank/select + one mixed batch of {3 * Q} "
                          f"over the plain directory, the query lines and select "
                          f"lines, the same queries over them, then the quad lines and "
-                         f"access/rank over them"),
+                         f"quad select lines and the three kinds over them"),
             "n": n, "sigma": sigma, "queries_per_kind": Q, "levels": L,
         },
         "build_ms": build,
@@ -964,11 +977,12 @@ def main():
         "select_qps": Q / (kind_best["select"] / 1e3),
         "layouts": {
             "access_rank": "quad lines" if use_quad else "query lines",
-            "select": "select lines",
+            "select": "quad select lines" if use_quad else "select lines",
         },
-        "quad_lines_build_ms": (float(np.median([t - a - r for t, a, r in zip(
-            quad_ms, quad_kind_ms["access"], quad_kind_ms["rank"])])) if use_quad else None),
+        "quad_lines_build_ms": float(np.median(quad_build_ms)) if use_quad else None,
         "quad_lines_bytes": (L // 2) * (n // 448 + 1) * 128 if use_quad else None,
+        "quad_select_lines_build_ms": float(np.median(quad_sel_build_ms)) if use_quad else None,
+        "quad_select_lines_bytes": int(qsel_bytes) if use_quad else None,
         "query_lines_layout": {
             "access_qps": Q / (np.mean(kind_ms["access"]) / 1e3),
             "rank_qps": Q / (np.mean(kind_ms["rank"]) / 1e3),
@@ -997,7 +1011,8 @@ def main():
                     L // 2 if use_quad else L, line_peak_half if use_quad else line_peak),
                    ("rank", "quad lines" if use_quad else "query lines",
                     L // 2 if use_quad else L, line_peak_half if use_quad else line_peak),
-                   ("select", "select lines", L, line_peak))},
+                   ("select", "quad select lines" if use_quad else "select lines",
+                    L // 2 if use_quad else L, line_peak_half if use_quad else line_peak))},
         },
         "plain_directory": {
             "access_qps": Q / (np.mean(plain_ms["access"]) / 1e3),

@@ -511,6 +511,11 @@ class WaveletTree:
         change."""
         call("ecl_wavelet_tree_build_quad_lines", self._h)
 
+    def buildQuadSelectLines(self):
+        """Quad select lines (builds the quad lines first when absent): select
+        reads one memory line per two levels; results do not change."""
+        call("ecl_wavelet_tree_build_quad_select_lines", self._h)
+
     def buildHostBytes(self):
         """Bytes of the build's buffers that were placed in pinned host
         memory because the device could not hold them (0: none)."""

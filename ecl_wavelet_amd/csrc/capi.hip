@@ -563,7 +563,8 @@ struct ecl_wavelet_tree_t {
   TreeView view{};
   uint32_t lanes = 0;  // lanes per access/rank query (0: the library-wide setting)
   uint64_t build_host_bytes = 0;  // build buffers placed in pinned host memory
-  DevBuf quad_lines, quad_base;   // optional quad lines and their bases
+  DevBuf quad_lines, quad_base, quad_tab;  // optional quad lines, bases, node tables
+  DevBuf quad_sel;                          // optional quad select lines
   QuadView quad{};
   std::unique_ptr<HostPipe> pipe{new HostPipe()};
   ~ecl_wavelet_tree_t() { delete rs; }
@@ -1633,9 +1634,81 @@ static void build_quad_lines(ecl_wavelet_tree_t* wt) {
   qv.base = dbase.as<uint64_t>();
   qv.nbase = base.size();
   qv.levels = Q;
+  qv.tab = dtab.as<uint64_t>();
+  qv.ntab = build_tab.size();
+  for (uint32_t q = 0; q < Q; ++q) {
+    qv.s_off[q] = s_off[q];
+    qv.cd_off[q] = cd_off[q];
+  }
   wt->quad_lines = std::move(lines);
   wt->quad_base = std::move(dbase);
+  wt->quad_tab = std::move(dtab);
   wt->quad = qv;
+}
+
+// Quad select lines (QuadView): for each quad level and digit d with t
+// occurrences, a line every r = max(1, floor(340 t / n)) of them (448
+// digits hold the next r with a margin of about 2.4 standard deviations at
+// density 1/4), built from the level's quad lines.  Builds the quad lines
+// first when the tree has none.
+static void build_quad_select_lines(ecl_wavelet_tree_t* wt) {
+  build_quad_lines(wt);
+  QuadView& qv = wt->quad;
+  if (qv.has_sel) return;
+  const uint32_t Q = qv.levels;
+  const uint64_t n = wt->n;
+  const uint64_t nql = n / kQuadDigits + 1;
+  // totals of every level's digits: the last row of its CD table
+  std::vector<uint64_t> tab(qv.ntab);
+  d2h(tab.data(), qv.tab, qv.ntab);
+  uint64_t total_lines = 0;
+  uint64_t first[kMaxQuadLevels][4] = {};
+  uint64_t cnt[kMaxQuadLevels][4] = {};
+  uint32_t rate[kMaxQuadLevels][4] = {};
+  uint64_t level_first[kMaxQuadLevels] = {};
+  for (uint32_t q = 0; q < Q; ++q) {
+    const uint64_t nodes = 1ull << (2 * q);
+    level_first[q] = total_lines;
+    uint64_t run = 0;
+    for (uint32_t d = 0; d < 4; ++d) {
+      const uint64_t t = tab[qv.cd_off[q] + 4 * nodes + d];
+      cnt[q][d] = t;
+      const uint64_t r = t ? std::max<uint64_t>(1, kQuadSelSpan * t / n) : 0;
+      rate[q][d] = static_cast<uint32_t>(r);
+      first[q][d] = run;
+      run += r ? t / r + 1 : 0;
+    }
+    total_lines += run;
+  }
+  if (total_lines * 128 + (64u << 20) > free_device_memory())
+    throw Error(ECL_ERROR_OUT_OF_MEMORY_TREE, kOomTree);
+  DevBuf lines(std::max<uint64_t>(total_lines, 1) * 128);
+  hipStream_t s = nullptr;
+  for (uint32_t q = 0; q < Q; ++q) {
+    uint4* level_lines = lines.as<uint4>() + level_first[q] * 8;
+    for (uint32_t d = 0; d < 4; ++d) {
+      if (rate[q][d] == 0) continue;
+      QuadSelBuildParams sp{};
+      sp.qlines = qv.lines[q];
+      sp.nql = nql;
+      sp.total = cnt[q][d];
+      sp.d = d;
+      sp.r = rate[q][d];
+      sp.magic = rate[q][d] > 1 ? ~0ull / rate[q][d] + 1 : 0;
+      sp.nlines = cnt[q][d] / rate[q][d] + 1;
+      sp.out = level_lines + first[q][d] * 8;
+      launch_quad_select_lines(sp, s);
+    }
+    qv.sel_lines[q] = level_lines;
+    for (uint32_t d = 0; d < 4; ++d) {
+      qv.sel_first[q][d] = first[q][d];
+      qv.sel_rate[q][d] = rate[q][d];
+      qv.sel_magic[q][d] = rate[q][d] > 1 ? ~0ull / rate[q][d] + 1 : 0;
+    }
+  }
+  ECL_HIP(hipStreamSynchronize(s));
+  wt->quad_sel = std::move(lines);
+  qv.has_sel = 1;
 }
 
 static void build_select_lines(ecl_rank_select_t* rs) {
@@ -2364,6 +2437,13 @@ ecl_status ecl_wavelet_tree_build_quad_lines(ecl_wavelet_tree wt) {
     require(wt != nullptr, "NULL tree");
     set_device(wt->device);
     build_quad_lines(wt);
+  });
+}
+ecl_status ecl_wavelet_tree_build_quad_select_lines(ecl_wavelet_tree wt) {
+  return guard([&] {
+    require(wt != nullptr, "NULL tree");
+    set_device(wt->device);
+    build_quad_select_lines(wt);
   });
 }
 ecl_status ecl_wavelet_tree_build_select_lines(ecl_wavelet_tree wt) {

@@ -228,6 +228,11 @@ void launch_presence(const void* text, int esize, uint64_t n, uint32_t* flags,
 // digit d leads to position base[off[q] + 4N + d] + (d's before p) of quad
 // level q + 1 (base = the child's start minus the d's before node N).
 constexpr uint32_t kQuadDigits = 448;
+// matches of digit d among the 32 digits of w, one bit per digit (bit 2i)
+__device__ __forceinline__ uint64_t quad_match(uint64_t w, uint32_t d) {
+  const uint64_t y = w ^ (0x5555555555555555ull * d);
+  return ~(y | (y >> 1)) & 0x5555555555555555ull;
+}
 constexpr uint32_t kMaxQuadLevels = 8;
 struct QuadView {
   const uint4* lines[kMaxQuadLevels];
@@ -235,7 +240,35 @@ struct QuadView {
   uint64_t off[kMaxQuadLevels];  // level q's bases start at base + off[q]
   uint64_t nbase;                // entries of base
   uint32_t levels;               // 0: no quad lines
+  // Node tables of every level (for select), in one array of ntab u64:
+  // tab[s_off[q] + N] = start of node N of level 2q (N in [0, 4^q], the
+  // last one n) and tab[cd_off[q] + 4N + d] = the d's before node N.
+  const uint64_t* tab;
+  uint64_t ntab;
+  uint64_t s_off[kMaxQuadLevels], cd_off[kMaxQuadLevels];
+  // Optional quad select lines: for level q and digit d with rate
+  // r = sel_rate[q][d] > 0, line j (from sel_first[q][d] in sel_lines[q])
+  // is the position s_j of the (j r)-th d of the level (0 for j = 0), 8
+  // unused bytes, then the 448 digits from the 32-digit word holding s_j.
+  const uint4* sel_lines[kMaxQuadLevels];
+  uint64_t sel_first[kMaxQuadLevels][4];
+  uint32_t sel_rate[kMaxQuadLevels][4];
+  uint64_t sel_magic[kMaxQuadLevels][4];
+  uint32_t has_sel;
 };
+constexpr uint32_t kQuadSelSpan = 340;  // digits the r digits of a select line span on average
+
+// One digit's quad select lines of one level, from that level's quad lines.
+struct QuadSelBuildParams {
+  const uint4* qlines;  // the level's quad lines
+  uint64_t nql;         // their number (n / 448 + 1)
+  uint64_t total;       // the level's d's
+  uint32_t d, r;
+  uint64_t magic;
+  uint64_t nlines;      // total / r + 1
+  uint4* out;
+};
+void launch_quad_select_lines(const QuadSelBuildParams& qp, hipStream_t s);
 
 // One quad level's build: binary levels a = 2q and b = 2q + 1, the level-2q
 // node starts S[0..nodes] (S[nodes] = n) and per node and digit the digits

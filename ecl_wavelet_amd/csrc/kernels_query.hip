@@ -155,12 +155,6 @@ __device__ __forceinline__ const uint64_t* stage_quad_base(const QuadView& q, ui
   return lds;
 }
 
-// matches of digit d among the 32 digits of w, as one bit per digit (bit 2i)
-__device__ __forceinline__ uint64_t digit_matches(uint64_t w, uint32_t d) {
-  const uint64_t y = w ^ (0x5555555555555555ull * d);
-  return ~(y | (y >> 1)) & 0x5555555555555555ull;
-}
-
 // one quad level for the pair: the digit at offset o of line `line` (when
 // want_digit; otherwise d is given) and the d's before position line*448+o
 struct QuadStep {
@@ -206,7 +200,7 @@ __device__ __forceinline__ QuadStep quad_step(const uint4* lines, uint64_t p, bo
   for (uint32_t j = 0; j < 8; ++j) {
     const uint32_t k = kb + j;
     if (k <= ko && (h || j < 6)) {
-      uint64_t m = digit_matches(w[j], d);
+      uint64_t m = quad_match(w[j], d);
       if (k == ko) m &= sh == 0 ? 0ull : (~0ull >> (64 - sh));
       cnt += static_cast<uint32_t>(__popcll(m));
     }
@@ -272,6 +266,132 @@ __global__ __launch_bounds__(256) void tree_rank_quad_kernel(
       r = pos - detail::gld(p.view.counts, c);
     }
     if ((threadIdx.x & 1) == 0) out[q] = r;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Select over quad select lines: the ascent takes two levels a step.  At
+// quad level q, the kk-th code of child 4N + d of node N is the
+// (CD[N][d] + kk)-th d of the level, found in one select line read by the
+// pair (lane 0: position and digit words 0..5, lane 1: words 6..13); its
+// index in node N is its position minus the node's start, plus one.  When
+// the d lies past the line's 448 digits, the step takes the two binary
+// levels' selects instead.
+constexpr uint64_t kQuadLdsTab = 4096;  // u64 entries
+
+__device__ uint64_t quad_select_fallback(const TreeQueryParams& p, const uint64_t* tab,
+                                         uint32_t ql, uint32_t d, uint64_t N, uint64_t kk) {
+  const uint64_t* S = tab + p.quad.s_off[ql];
+  const uint64_t* CD = tab + p.quad.cd_off[ql];
+  const uint64_t SN = S[N];
+  const uint64_t cnt0 = CD[4 * (N + 1)] - CD[4 * N], cnt1 = CD[4 * (N + 1) + 1] - CD[4 * N + 1];
+  const uint32_t hi = d >> 1, lo = d & 1;
+  // level 2q + 1: node 2N + hi starts at SM; its ones before it
+  const uint64_t SM = SN + (hi ? cnt0 + cnt1 : 0);
+  const uint64_t onesM = CD[4 * N + 1] + CD[4 * N + 3] + (hi ? cnt1 : 0);
+  const LevelView v1 = p.view.levels[2 * ql + 1];
+  const uint64_t p1 = lo ? detail::level_select<1, false, true, true>(v1, onesM + kk)
+                         : detail::level_select<0, false, true, true>(v1, (SM - onesM) + kk);
+  const uint64_t kk1 = p1 - SM + 1;
+  // level 2q: node N, bit hi
+  const uint64_t onesN = CD[4 * N + 2] + CD[4 * N + 3];
+  const LevelView v0 = p.view.levels[2 * ql];
+  return hi ? detail::level_select<1, false, true, true>(v0, onesN + kk1)
+            : detail::level_select<0, false, true, true>(v0, (SN - onesN) + kk1);
+}
+
+// position of the g-th (1-based) d of quad level ql, for the pair
+__device__ __forceinline__ uint64_t quad_select_line(const TreeQueryParams& p, uint32_t ql,
+                                                     uint32_t d, uint64_t g, bool* ok) {
+  const uint32_t h = threadIdx.x & 1u;
+  const int pair0 = static_cast<int>((threadIdx.x & 63u) & ~1u);
+  const uint32_t r = p.quad.sel_rate[ql][d];
+  const uint64_t j = r == 1 ? g : __umul64hi(g, p.quad.sel_magic[ql][d]);
+  const uint4* lp = p.quad.sel_lines[ql] + (p.quad.sel_first[ql][d] + j) * 8 + 4 * h;
+  const uint4 x0 = detail::gld(lp, 0), x1 = detail::gld(lp, 1), x2 = detail::gld(lp, 2),
+              x3 = detail::gld(lp, 3);
+  uint64_t w[8];
+  w[0] = h ? ((uint64_t)x0.x | ((uint64_t)x0.y << 32)) : ((uint64_t)x1.x | ((uint64_t)x1.y << 32));
+  w[1] = h ? ((uint64_t)x0.z | ((uint64_t)x0.w << 32)) : ((uint64_t)x1.z | ((uint64_t)x1.w << 32));
+  w[2] = h ? ((uint64_t)x1.x | ((uint64_t)x1.y << 32)) : ((uint64_t)x2.x | ((uint64_t)x2.y << 32));
+  w[3] = h ? ((uint64_t)x1.z | ((uint64_t)x1.w << 32)) : ((uint64_t)x2.z | ((uint64_t)x2.w << 32));
+  w[4] = h ? ((uint64_t)x2.x | ((uint64_t)x2.y << 32)) : ((uint64_t)x3.x | ((uint64_t)x3.y << 32));
+  w[5] = h ? ((uint64_t)x2.z | ((uint64_t)x2.w << 32)) : ((uint64_t)x3.z | ((uint64_t)x3.w << 32));
+  w[6] = h ? ((uint64_t)x3.x | ((uint64_t)x3.y << 32)) : 0ull;
+  w[7] = h ? ((uint64_t)x3.z | ((uint64_t)x3.w << 32)) : 0ull;
+  const uint64_t s = __shfl(static_cast<unsigned long long>((uint64_t)x0.x | ((uint64_t)x0.y << 32)),
+                            pair0, 64);
+  const uint64_t rem = j ? g - j * r + 1 : g;  // counted from s itself
+  const uint32_t nw = h ? 8u : 6u;
+  uint64_t m[8];
+  uint32_t cnt = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < 8; ++k) {
+    m[k] = k < nw ? quad_match(w[k], d) : 0ull;
+    if (h == 0 && k == 0) m[k] &= ~0ull << (2 * (s & 31));  // digits before s
+    cnt += static_cast<uint32_t>(__popcll(m[k]));
+  }
+  const uint32_t c0 = __shfl(cnt, pair0, 64);
+  const uint32_t owner = rem <= c0 ? 0u : 1u;
+  uint64_t pos = 0;
+  bool found = false;
+  if (owner == h) {
+    uint32_t need = static_cast<uint32_t>(h ? rem - c0 : rem);
+#pragma unroll
+    for (uint32_t k = 0; k < 8; ++k) {
+      const uint32_t c = static_cast<uint32_t>(__popcll(m[k]));
+      if (!found && k < nw) {
+        if (need <= c) {
+          pos = (s & ~31ull) + 32ull * (k + (h ? 6u : 0u)) + (detail::select_in_word(m[k], need) >> 1);
+          found = true;
+        } else {
+          need -= c;
+        }
+      }
+    }
+  }
+  pos = __shfl(static_cast<unsigned long long>(pos), pair0 + static_cast<int>(owner), 64);
+  *ok = __shfl(found ? 1 : 0, pair0 + static_cast<int>(owner), 64) != 0;
+  return pos;
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void tree_select_quad_kernel(
+    TreeQueryParams p, const uint8_t* __restrict__ qs, uint64_t m, uint64_t* __restrict__ out) {
+  extern __shared__ uint64_t s_qtab[];
+  const uint64_t* tab = p.quad.tab;
+  if (p.quad.ntab <= kQuadLdsTab) {
+    for (uint64_t i = threadIdx.x; i < p.quad.ntab; i += blockDim.x) s_qtab[i] = p.quad.tab[i];
+    __syncthreads();
+    tab = s_qtab;
+  }
+  const uint32_t Q = p.quad.levels;
+  const uint64_t n = p.view.n, sigma = p.view.sigma;
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x / 2;
+  for (uint64_t q = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) / 2;
+       q < m; q += stride) {
+    const uint8_t* e = qs + q * 16;
+    const uint64_t k = *reinterpret_cast<const uint64_t*>(e);
+    const uint64_t c = map_symbol<T>(p, load_sym(e, sizeof(T)));
+    uint64_t res = n;
+    if (c < sigma && k > 0) {
+      const uint64_t cc = detail::gld(p.view.counts, c);
+      const uint64_t cnt = (c + 1 < sigma ? detail::gld(p.view.counts, c + 1) : n) - cc;
+      if (k <= cnt) {
+        uint64_t kk = k;
+        for (int32_t ql = static_cast<int32_t>(Q) - 1; ql >= 0; --ql) {
+          const uint64_t N = c >> (2 * (Q - ql));
+          const uint32_t d = static_cast<uint32_t>(c >> (2 * (Q - 1 - ql))) & 3u;
+          const uint64_t g = tab[p.quad.cd_off[ql] + 4 * N + d] + kk;
+          bool ok = false;
+          uint64_t pos = quad_select_line(p, static_cast<uint32_t>(ql), d, g, &ok);
+          if (!ok) pos = quad_select_fallback(p, tab, static_cast<uint32_t>(ql), d, N, kk);
+          kk = pos - tab[p.quad.s_off[ql] + N] + 1;
+        }
+        res = kk - 1;
+      }
+    }
+    if ((threadIdx.x & 1) == 0) out[q] = res;
   }
 }
 
@@ -761,7 +881,16 @@ void launch_tree_select(const TreeQueryParams& p, const void* q, uint64_t m,
   if (m == 0) return;
   ProfScope ps(K_WT_SELECT, s);
   const uint8_t* qq = static_cast<const uint8_t*>(q);
-  if (p.has_sel_lines) select_sl<true>(p, qq, m, out, s);
+  if (p.quad.has_sel) {
+    const dim3 g(query_grid(2 * m)), b(256);
+    const size_t sh = p.quad.ntab <= kQuadLdsTab ? p.quad.ntab * sizeof(uint64_t) : 0;
+    switch (p.esize) {
+      case 1: hipLaunchKernelGGL((tree_select_quad_kernel<uint8_t>), g, b, sh, s, p, qq, m, out); break;
+      case 2: hipLaunchKernelGGL((tree_select_quad_kernel<uint16_t>), g, b, sh, s, p, qq, m, out); break;
+      case 4: hipLaunchKernelGGL((tree_select_quad_kernel<uint32_t>), g, b, sh, s, p, qq, m, out); break;
+      default: hipLaunchKernelGGL((tree_select_quad_kernel<uint64_t>), g, b, sh, s, p, qq, m, out); break;
+    }
+  } else if (p.has_sel_lines) select_sl<true>(p, qq, m, out, s);
   else select_sl<false>(p, qq, m, out, s);
   ECL_HIP(hipGetLastError());
 }

@@ -804,6 +804,37 @@ def test_quad_lines_same_answers(gpu, case):
         assert np.array_equal(wt.rank(wt.make_queries(ri, cs)), exp), int(c)
 
 
+@pytest.mark.parametrize("case", QUAD_CASES)
+def test_quad_select_lines_same_answers(gpu, case):
+    """Select over the quad select lines (a line every r occurrences of each
+    digit of each quad level; an occurrence past a line's 448 digits takes
+    the two binary levels' select) equals the text's occurrence positions:
+    random queries, and every occurrence of the rarest and the most frequent
+    symbol and of a random one (each line boundary, both ends), plus
+    count + 1 (n).  The skewed text has digits rare enough that their lines
+    overflow."""
+    from ecl_wavelet_amd import WaveletTree
+    rng = np.random.default_rng(sum(map(ord, case)) + 1)
+    text, alpha = _quad_text(case, rng)
+    n = len(text)
+    wt = WaveletTree(text, alpha)
+    wt.buildQuadSelectLines()  # builds the quad lines too
+    wt.buildQuadSelectLines()
+    _check_queries(wt, text, None, nq=4000, seed=len(case) + 1)
+    vals, cnts = np.unique(text, return_counts=True)
+    picks = {vals[np.argmin(cnts)], vals[np.argmax(cnts)], vals[rng.integers(0, len(vals))]}
+    for c in picks:
+        occ = np.flatnonzero(text == c).astype(np.uint64)
+        if len(occ) > 300_000:
+            occ = occ[:300_000]
+        k = np.arange(1, len(occ) + 2, dtype=np.uint64)
+        exp = np.concatenate([occ, [n]]).astype(np.uint64)
+        if len(occ) == 300_000:
+            k, exp = k[:-1], exp[:-1]
+        cs = np.full(len(k), c, dtype=alpha.dtype)
+        assert np.array_equal(wt.select(wt.make_queries(k, cs)), exp), int(c)
+
+
 @pytest.mark.parametrize("sigma", [2, 8, 20, 128])
 def test_quad_lines_reject_other_trees(gpu, sigma):
     """Alphabets that are not a power of two, or whose level count is odd,
@@ -813,10 +844,11 @@ def test_quad_lines_reject_other_trees(gpu, sigma):
     rng = np.random.default_rng(sigma)
     text = rng.integers(0, sigma, 100_003).astype(np.uint8)
     wt = WaveletTree(text, np.arange(sigma, dtype=np.uint8))
-    with pytest.raises(EclError) as e:
-        wt.buildQuadLines()
-    assert e.value.status == ECL_ERROR_INVALID_ARGUMENT
-    del e
+    for build in (wt.buildQuadLines, wt.buildQuadSelectLines):
+        with pytest.raises(EclError) as e:
+            build()
+        assert e.value.status == ECL_ERROR_INVALID_ARGUMENT
+        del e
     _check_queries(wt, text, None, nq=500, seed=sigma)
 
 
