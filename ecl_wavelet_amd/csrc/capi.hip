@@ -1686,19 +1686,17 @@ static void build_quad_select_lines(ecl_wavelet_tree_t* wt) {
   hipStream_t s = nullptr;
   for (uint32_t q = 0; q < Q; ++q) {
     uint4* level_lines = lines.as<uint4>() + level_first[q] * 8;
+    QuadSelBuildParams sp{};
+    sp.qlines = qv.lines[q];
+    sp.nql = nql;
     for (uint32_t d = 0; d < 4; ++d) {
-      if (rate[q][d] == 0) continue;
-      QuadSelBuildParams sp{};
-      sp.qlines = qv.lines[q];
-      sp.nql = nql;
-      sp.total = cnt[q][d];
-      sp.d = d;
-      sp.r = rate[q][d];
-      sp.magic = rate[q][d] > 1 ? ~0ull / rate[q][d] + 1 : 0;
-      sp.nlines = cnt[q][d] / rate[q][d] + 1;
-      sp.out = level_lines + first[q][d] * 8;
-      launch_quad_select_lines(sp, s);
+      sp.total[d] = cnt[q][d];
+      sp.r[d] = rate[q][d];
+      sp.magic[d] = rate[q][d] > 1 ? ~0ull / rate[q][d] + 1 : 0;
+      sp.nlines[d] = rate[q][d] ? cnt[q][d] / rate[q][d] + 1 : 0;
+      sp.out[d] = level_lines + first[q][d] * 8;
     }
+    launch_quad_select_lines(sp, s);
     qv.sel_lines[q] = level_lines;
     for (uint32_t d = 0; d < 4; ++d) {
       qv.sel_first[q][d] = first[q][d];

@@ -258,15 +258,16 @@ struct QuadView {
 };
 constexpr uint32_t kQuadSelSpan = 340;  // digits the r digits of a select line span on average
 
-// One digit's quad select lines of one level, from that level's quad lines.
+// One quad level's select lines of every digit, from its quad lines; digit
+// d with r[d] == 0 (absent) has none.
 struct QuadSelBuildParams {
   const uint4* qlines;  // the level's quad lines
   uint64_t nql;         // their number (n / 448 + 1)
-  uint64_t total;       // the level's d's
-  uint32_t d, r;
-  uint64_t magic;
-  uint64_t nlines;      // total / r + 1
-  uint4* out;
+  uint64_t total[4];    // the level's d's
+  uint32_t r[4];
+  uint64_t magic[4];
+  uint64_t nlines[4];   // total / r + 1
+  uint4* out[4];
 };
 void launch_quad_select_lines(const QuadSelBuildParams& qp, hipStream_t s);
 

@@ -1068,72 +1068,79 @@ __global__ __launch_bounds__(256) void quad_select_lines_kernel(QuadSelBuildPara
     s_ql[i] = gi < qp.nql * 8 ? detail::gld(qp.qlines, gi) : make_uint4(0, 0, 0, 0);
   }
   __syncthreads();
-  const uint32_t d = qp.d, r = qp.r;
-  // d's before staged quad line l (the total past the last one)
-  auto before = [&](uint32_t l) -> uint64_t {
-    if (L0 + l >= qp.nql) return qp.total;
-    const uint4 h = s_ql[l * 8];
-    const uint64_t h0 = (uint64_t)h.x | ((uint64_t)h.y << 32);
-    const uint64_t h1 = (uint64_t)h.z | ((uint64_t)h.w << 32);
-    const uint64_t m40 = (1ull << 40) - 1;
-    const uint64_t c0 = h0 & m40, c1 = (h0 >> 40) | ((h1 & 0xFFFFull) << 24), c2 = (h1 >> 16) & m40;
-    return d == 0 ? c0 : d == 1 ? c1 : d == 2 ? c2 : (L0 + l) * kQuadDigits - c0 - c1 - c2;
-  };
   const uint32_t lend = static_cast<uint32_t>(qp.nql - L0 < kQsQuadLines ? qp.nql - L0 : kQsQuadLines);
-  const uint64_t first = L0 == 0 ? 0 : div_rate(before(0), r, qp.magic) + 1;
-  const uint64_t T = div_rate(before(lend), r, qp.magic) + 1 - first;
-  uint64_t jlo = 1, jend = 0, cb = 0;  // empty
-  if (tid < lend) {
-    cb = before(tid);
-    jlo = L0 + tid == 0 ? 0 : div_rate(cb, r, qp.magic) + 1;
-    jend = div_rate(before(tid + 1), r, qp.magic);
-  }
-  for (uint64_t r0 = 0; r0 < T; r0 += kSlLines) {
-    for (uint64_t j = jlo; j <= jend; ++j) {
-      const uint64_t slot = j - first;
-      if (slot < r0) continue;
-      if (slot >= r0 + kSlLines) break;
-      uint64_t pos = 0;
-      if (j != 0) {
-        uint32_t rank = static_cast<uint32_t>(j * r - cb);  // 1-based, <= 448
-        for (uint32_t k = 0; k < kQuadDigits / 32; ++k) {
-          const uint64_t m = quad_match(qs_digit_word(s_ql, tid, k), d);
-          const uint32_t c = static_cast<uint32_t>(__popcll(m));
-          if (rank <= c) {
-            pos = (L0 + tid) * kQuadDigits + 32 * k + (detail::select_in_word(m, rank) >> 1);
-            break;
+  // the four digits' lines in turn, over the same staged quad lines
+  for (uint32_t d = 0; d < 4; ++d) {
+    const uint32_t r = qp.r[d];
+    if (r == 0) continue;
+    const uint64_t magic = qp.magic[d];
+    // d's before staged quad line l (the total past the last one)
+    auto before = [&](uint32_t l) -> uint64_t {
+      if (L0 + l >= qp.nql) return qp.total[d];
+      const uint4 h = s_ql[l * 8];
+      const uint64_t h0 = (uint64_t)h.x | ((uint64_t)h.y << 32);
+      const uint64_t h1 = (uint64_t)h.z | ((uint64_t)h.w << 32);
+      const uint64_t m40 = (1ull << 40) - 1;
+      const uint64_t c0 = h0 & m40, c1 = (h0 >> 40) | ((h1 & 0xFFFFull) << 24),
+                     c2 = (h1 >> 16) & m40;
+      return d == 0 ? c0 : d == 1 ? c1 : d == 2 ? c2 : (L0 + l) * kQuadDigits - c0 - c1 - c2;
+    };
+    const uint64_t first = L0 == 0 ? 0 : div_rate(before(0), r, magic) + 1;
+    const uint64_t T = div_rate(before(lend), r, magic) + 1 - first;
+    uint64_t jlo = 1, jend = 0, cb = 0;  // empty
+    if (tid < lend) {
+      cb = before(tid);
+      jlo = L0 + tid == 0 ? 0 : div_rate(cb, r, magic) + 1;
+      jend = div_rate(before(tid + 1), r, magic);
+    }
+    for (uint64_t r0 = 0; r0 < T; r0 += kSlLines) {
+      for (uint64_t j = jlo; j <= jend; ++j) {
+        const uint64_t slot = j - first;
+        if (slot < r0) continue;
+        if (slot >= r0 + kSlLines) break;
+        uint64_t pos = 0;
+        if (j != 0) {
+          uint32_t rank = static_cast<uint32_t>(j * r - cb);  // 1-based, <= 448
+          for (uint32_t k = 0; k < kQuadDigits / 32; ++k) {
+            const uint64_t m = quad_match(qs_digit_word(s_ql, tid, k), d);
+            const uint32_t c = static_cast<uint32_t>(__popcll(m));
+            if (rank <= c) {
+              pos = (L0 + tid) * kQuadDigits + 32 * k + (detail::select_in_word(m, rank) >> 1);
+              break;
+            }
+            rank -= c;
           }
-          rank -= c;
         }
+        s_pos[slot - r0] = pos;
       }
-      s_pos[slot - r0] = pos;
-    }
-    __syncthreads();
-    // piece 0: the position; piece k > 0: digit words 2k - 2, 2k - 1 from the
-    // word holding the position
-    const uint64_t nl = T - r0 < kSlLines ? T - r0 : kSlLines;
-    uint4* dst = qp.out + (first + r0) * 8;
-    for (uint32_t i = tid; i < nl * 8; i += blockDim.x) {
-      const uint64_t pos = s_pos[i >> 3];
-      const uint32_t k = i & 7;
-      uint4 q = make_uint4(static_cast<uint32_t>(pos), static_cast<uint32_t>(pos >> 32), 0u, 0u);
-      if (k != 0) {
-        const uint64_t w0 = (pos >> 5) + 2 * (k - 1);  // global digit word
-        const uint64_t a = w0 / 14, b = (w0 + 1) / 14;
-        const uint64_t x = qs_digit_word(s_ql, static_cast<uint32_t>(a - L0), static_cast<uint32_t>(w0 - a * 14));
-        const uint64_t y = qs_digit_word(s_ql, static_cast<uint32_t>(b - L0),
-                                         static_cast<uint32_t>(w0 + 1 - b * 14));
-        q = make_uint4(static_cast<uint32_t>(x), static_cast<uint32_t>(x >> 32),
-                       static_cast<uint32_t>(y), static_cast<uint32_t>(y >> 32));
+      __syncthreads();
+      // piece 0: the position; piece k > 0: digit words 2k - 2, 2k - 1 from
+      // the word holding the position
+      const uint64_t nl = T - r0 < kSlLines ? T - r0 : kSlLines;
+      uint4* dst = qp.out[d] + (first + r0) * 8;
+      for (uint32_t i = tid; i < nl * 8; i += blockDim.x) {
+        const uint64_t pos = s_pos[i >> 3];
+        const uint32_t k = i & 7;
+        uint4 q = make_uint4(static_cast<uint32_t>(pos), static_cast<uint32_t>(pos >> 32), 0u, 0u);
+        if (k != 0) {
+          const uint64_t w0 = (pos >> 5) + 2 * (k - 1);  // global digit word
+          const uint64_t a = w0 / 14, b = (w0 + 1) / 14;
+          const uint64_t x = qs_digit_word(s_ql, static_cast<uint32_t>(a - L0),
+                                           static_cast<uint32_t>(w0 - a * 14));
+          const uint64_t y = qs_digit_word(s_ql, static_cast<uint32_t>(b - L0),
+                                           static_cast<uint32_t>(w0 + 1 - b * 14));
+          q = make_uint4(static_cast<uint32_t>(x), static_cast<uint32_t>(x >> 32),
+                         static_cast<uint32_t>(y), static_cast<uint32_t>(y >> 32));
+        }
+        dst[i] = q;
       }
-      dst[i] = q;
+      __syncthreads();
     }
-    __syncthreads();
   }
 }
 
 void launch_quad_select_lines(const QuadSelBuildParams& qp, hipStream_t s) {
-  if (qp.nlines == 0 || qp.nql == 0) return;
+  if (qp.nql == 0) return;
   const uint64_t blocks = (qp.nql + kQsQuadLines - 1) / kQsQuadLines;
   if (blocks > 0x7FFFFFFFull) throw Error(ECL_ERROR_INVALID_ARGUMENT, "bit array too large");
   hipLaunchKernelGGL(quad_select_lines_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(256), 0, s, qp);

@@ -7,8 +7,9 @@ each kind run in single launches of the shipped kernels (8192 workgroups,
 grid-stride), over the reference-shaped directory, then with the query lines
 and select hints, then with the select lines, then with the quad lines (which
 access and rank then read: all three alphabets are powers of two with an even
-number of levels); the mixed-kind kernel runs the same 3 * 10^7 queries in
-one launch in every layout.
+number of levels), then with the quad select lines (which select then
+reads); the mixed-kind kernel runs the same 3 * 10^7 queries in one launch in
+every layout.
 
 Every answer is checked, over the whole launch:
   * access(i) against the text copied back from the device;
@@ -36,7 +37,7 @@ CASES = {
     "u8_sigma4": dict(n=6_000_000_000, sigma=4, dtype=np.uint8, seed=5, zipf=None),
     "u16_zipf": dict(n=2_000_000_000, sigma=1 << 16, dtype=np.uint16, seed=8, zipf=1.1),
 }
-LAYOUTS = ["plain", "query_lines", "select_lines", "quad_lines"]
+LAYOUTS = ["plain", "query_lines", "select_lines", "quad_lines", "quad_select"]
 
 
 class Full:
@@ -107,12 +108,14 @@ def full(request, gpu):
 
 
 def _ensure(F, layout):
-    if layout in ("query_lines", "select_lines", "quad_lines"):
+    if layout in ("query_lines", "select_lines", "quad_lines", "quad_select"):
         F.wt.buildQueryLines()
-    if layout in ("select_lines", "quad_lines"):
+    if layout in ("select_lines", "quad_lines", "quad_select"):
         F.wt.buildSelectLines()
-    if layout == "quad_lines":
+    if layout in ("quad_lines", "quad_select"):
         F.wt.buildQuadLines()
+    if layout == "quad_select":
+        F.wt.buildQuadSelectLines()
 
 
 def _first_bad(got, exp):
