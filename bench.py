@@ -789,7 +789,7 @@ def main():
     quad_ms, plain_total_ms = [], []
     kind_ms = {"access": [], "rank": [], "select": []}
     quad_kind_ms = {"access": [], "rank": [], "select": []}
-    quad_build_ms, quad_sel_build_ms = [], []
+    quad_build_ms, quad_sel_build_ms, quad_mixed_ms = [], [], []
     # quad lines: power-of-two alphabets with an even number of levels
     use_quad = sigma >= 4 and (sigma & (sigma - 1)) == 0 and (sigma.bit_length() - 1) % 2 == 0
     plain_ms = {"access": [], "rank": [], "select": [], "mixed": []}
@@ -851,6 +851,10 @@ def main():
             timer.start()
             wt.selectDevice(d_sq, Q, d_out, flags=1)
             qt["select"] = timer.stop()
+            # the mixed batch over them (a pair of lanes per query)
+            timer.start()
+            wt.queryBatchDevice(d_mq, 3 * Q, d_out)
+            qmx = timer.stop()
         log(f"step {step}{'' if timed else ' (warmup)'}: build {b:.1f} ms, "
             f"plain access {plain['access']:.2f} ms, rank {plain['rank']:.2f} ms, "
             f"select {plain['select']:.2f} ms, mixed {plain['mixed']:.2f} ms; lines {lb:.2f} ms, "
@@ -859,7 +863,7 @@ def main():
             f"select {times['select']:.2f} ms, mixed {mx:.2f} ms"
             + (f"; quad lines {qb:.2f} ms, access {qt['access']:.2f} ms, "
                f"rank {qt['rank']:.2f} ms; quad select lines {qsb:.2f} ms, "
-               f"select {qt['select']:.2f} ms" if use_quad else ""))
+               f"select {qt['select']:.2f} ms, mixed {qmx:.2f} ms" if use_quad else ""))
         if timed:
             build_ms.append(b)
             lines_ms.append(lb)
@@ -872,7 +876,8 @@ def main():
             for k, v in plain.items():
                 plain_ms[k].append(v)
             if use_quad:
-                quad_ms.append(qb + qsb + qt["access"] + qt["rank"] + qt["select"])
+                quad_ms.append(qb + qsb + qt["access"] + qt["rank"] + qt["select"] + qmx)
+                quad_mixed_ms.append(qmx)
                 quad_build_ms.append(qb)
                 quad_sel_build_ms.append(qsb)
                 for k, v in qt.items():
@@ -988,8 +993,9 @@ def main():
             "rank_qps": Q / (np.mean(kind_ms["rank"]) / 1e3),
             "select_qps": Q / (np.mean(kind_ms["select"]) / 1e3),
             "per_kind_qps": lines_value,
+            "mixed_batch_qps": 3 * Q / (np.mean(mixed_ms) / 1e3),
         },
-        "mixed_batch_qps": 3 * Q / (np.mean(mixed_ms) / 1e3),
+        "mixed_batch_qps": 3 * Q / (np.mean(quad_mixed_ms if use_quad else mixed_ms) / 1e3),
         "small_call_rank_p50_ms": float(np.percentile(lat, 50)),
         "small_call_rank_p99_ms": float(np.percentile(lat, 99)),
         "rank_kernel_ms": rank_k[1] / max(rank_k[0], 1),

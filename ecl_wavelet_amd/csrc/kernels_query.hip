@@ -220,24 +220,43 @@ __device__ __forceinline__ QuadStep quad_step(const uint4* lines, uint64_t p, bo
   return QuadStep{d, hdr + cnt};
 }
 
+// access(i) over the quad lines: the symbol (minimal), for the pair
+__device__ __forceinline__ uint64_t quad_access(const TreeQueryParams& p, const uint64_t* base,
+                                                uint64_t i) {
+  uint64_t pos = i, N = 0;
+  for (uint32_t l = 0; l < p.quad.levels; ++l) {
+    const QuadStep st = quad_step(p.quad.lines[l], pos, true, 0);
+    pos = base[p.quad.off[l] + 4 * N + st.d] + st.rank;
+    N = 4 * N + st.d;
+  }
+  return N;
+}
+// rank_c(i) over the quad lines, c < sigma and i <= n, for the pair
+__device__ __forceinline__ uint64_t quad_rank(const TreeQueryParams& p, const uint64_t* base,
+                                              uint64_t c, uint64_t i) {
+  const uint32_t Q = p.quad.levels;
+  uint64_t pos = i, N = 0;
+  for (uint32_t l = 0; l < Q; ++l) {
+    const uint32_t d = static_cast<uint32_t>(c >> (2 * (Q - 1 - l))) & 3u;
+    const QuadStep st = quad_step(p.quad.lines[l], pos, false, d);
+    pos = base[p.quad.off[l] + 4 * N + d] + st.rank;
+    N = 4 * N + d;
+  }
+  return pos - detail::gld(p.view.counts, c);
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void tree_access_quad_kernel(
     TreeQueryParams p, const uint64_t* __restrict__ idx, uint64_t m, T* __restrict__ out) {
   extern __shared__ uint64_t s_qbase[];
   const uint64_t* base = stage_quad_base(p.quad, s_qbase);
   const T* amap = static_cast<const T*>(p.alphabet_out);
-  const uint32_t Q = p.quad.levels;
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x / 2;
   for (uint64_t q = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) / 2;
        q < m; q += stride) {
     const uint64_t i = idx[q];
     if (i >= p.view.n) continue;  // accessKernel skips such indices (both lanes)
-    uint64_t pos = i, N = 0;
-    for (uint32_t l = 0; l < Q; ++l) {
-      const QuadStep st = quad_step(p.quad.lines[l], pos, true, 0);
-      pos = base[p.quad.off[l] + 4 * N + st.d] + st.rank;
-      N = 4 * N + st.d;
-    }
+    const uint64_t N = quad_access(p, base, i);
     if ((threadIdx.x & 1) == 0) out[q] = amap ? amap[N] : static_cast<T>(N);
   }
 }
@@ -247,24 +266,13 @@ __global__ __launch_bounds__(256) void tree_rank_quad_kernel(
     TreeQueryParams p, const uint8_t* __restrict__ qs, uint64_t m, uint64_t* __restrict__ out) {
   extern __shared__ uint64_t s_qbase[];
   const uint64_t* base = stage_quad_base(p.quad, s_qbase);
-  const uint32_t Q = p.quad.levels;
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x / 2;
   for (uint64_t q = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) / 2;
        q < m; q += stride) {
     const uint8_t* e = qs + q * 16;
     const uint64_t i = *reinterpret_cast<const uint64_t*>(e);
     const uint64_t c = map_symbol<T>(p, load_sym(e, sizeof(T)));
-    uint64_t r = 0;
-    if (i <= p.view.n && c < p.view.sigma) {
-      uint64_t pos = i, N = 0;
-      for (uint32_t l = 0; l < Q; ++l) {
-        const uint32_t d = static_cast<uint32_t>(c >> (2 * (Q - 1 - l))) & 3u;
-        const QuadStep st = quad_step(p.quad.lines[l], pos, false, d);
-        pos = base[p.quad.off[l] + 4 * N + d] + st.rank;
-        N = 4 * N + d;
-      }
-      r = pos - detail::gld(p.view.counts, c);
-    }
+    const uint64_t r = i <= p.view.n && c < p.view.sigma ? quad_rank(p, base, c, i) : 0;
     if ((threadIdx.x & 1) == 0) out[q] = r;
   }
 }
@@ -355,42 +363,47 @@ __device__ __forceinline__ uint64_t quad_select_line(const TreeQueryParams& p, u
   return pos;
 }
 
+// select_c(k) over the quad select lines (n when absent), for the pair
+__device__ __forceinline__ uint64_t quad_select(const TreeQueryParams& p, const uint64_t* tab,
+                                                uint64_t c, uint64_t k) {
+  const uint32_t Q = p.quad.levels;
+  const uint64_t n = p.view.n, sigma = p.view.sigma;
+  if (c >= sigma || k == 0) return n;
+  const uint64_t cc = detail::gld(p.view.counts, c);
+  const uint64_t cnt = (c + 1 < sigma ? detail::gld(p.view.counts, c + 1) : n) - cc;
+  if (k > cnt) return n;
+  uint64_t kk = k;
+  for (int32_t ql = static_cast<int32_t>(Q) - 1; ql >= 0; --ql) {
+    const uint64_t N = c >> (2 * (Q - ql));
+    const uint32_t d = static_cast<uint32_t>(c >> (2 * (Q - 1 - ql))) & 3u;
+    const uint64_t g = tab[p.quad.cd_off[ql] + 4 * N + d] + kk;
+    bool ok = false;
+    uint64_t pos = quad_select_line(p, static_cast<uint32_t>(ql), d, g, &ok);
+    if (!ok) pos = quad_select_fallback(p, tab, static_cast<uint32_t>(ql), d, N, kk);
+    kk = pos - tab[p.quad.s_off[ql] + N] + 1;
+  }
+  return kk - 1;
+}
+
+__device__ __forceinline__ const uint64_t* stage_quad_tab(const QuadView& q, uint64_t* lds) {
+  if (q.ntab > kQuadLdsTab) return q.tab;
+  for (uint64_t i = threadIdx.x; i < q.ntab; i += blockDim.x) lds[i] = q.tab[i];
+  __syncthreads();
+  return lds;
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void tree_select_quad_kernel(
     TreeQueryParams p, const uint8_t* __restrict__ qs, uint64_t m, uint64_t* __restrict__ out) {
   extern __shared__ uint64_t s_qtab[];
-  const uint64_t* tab = p.quad.tab;
-  if (p.quad.ntab <= kQuadLdsTab) {
-    for (uint64_t i = threadIdx.x; i < p.quad.ntab; i += blockDim.x) s_qtab[i] = p.quad.tab[i];
-    __syncthreads();
-    tab = s_qtab;
-  }
-  const uint32_t Q = p.quad.levels;
-  const uint64_t n = p.view.n, sigma = p.view.sigma;
+  const uint64_t* tab = stage_quad_tab(p.quad, s_qtab);
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x / 2;
   for (uint64_t q = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) / 2;
        q < m; q += stride) {
     const uint8_t* e = qs + q * 16;
     const uint64_t k = *reinterpret_cast<const uint64_t*>(e);
     const uint64_t c = map_symbol<T>(p, load_sym(e, sizeof(T)));
-    uint64_t res = n;
-    if (c < sigma && k > 0) {
-      const uint64_t cc = detail::gld(p.view.counts, c);
-      const uint64_t cnt = (c + 1 < sigma ? detail::gld(p.view.counts, c + 1) : n) - cc;
-      if (k <= cnt) {
-        uint64_t kk = k;
-        for (int32_t ql = static_cast<int32_t>(Q) - 1; ql >= 0; --ql) {
-          const uint64_t N = c >> (2 * (Q - ql));
-          const uint32_t d = static_cast<uint32_t>(c >> (2 * (Q - 1 - ql))) & 3u;
-          const uint64_t g = tab[p.quad.cd_off[ql] + 4 * N + d] + kk;
-          bool ok = false;
-          uint64_t pos = quad_select_line(p, static_cast<uint32_t>(ql), d, g, &ok);
-          if (!ok) pos = quad_select_fallback(p, tab, static_cast<uint32_t>(ql), d, N, kk);
-          kk = pos - tab[p.quad.s_off[ql] + N] + 1;
-        }
-        res = kk - 1;
-      }
-    }
+    const uint64_t res = quad_select(p, tab, c, k);
     if ((threadIdx.x & 1) == 0) out[q] = res;
   }
 }
@@ -513,6 +526,43 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void t
       }
       out[base + j] = r;
     });
+  }
+}
+
+// A mixed batch over the quad lines and quad select lines: the chunk's
+// queries grouped by kind as above, then the three lists walked with a pair
+// of lanes per query.  Symbols are minimal indices.
+__global__ __launch_bounds__(256) void tree_mixed_quad_kernel(
+    TreeQueryParams p, const ecl_query* __restrict__ qs, uint64_t m,
+    uint64_t* __restrict__ out, uint32_t chunk) {
+  extern __shared__ uint64_t s_qmix[];
+  __shared__ KindLists lists;
+  const uint64_t* base = stage_quad_base(p.quad, s_qmix);
+  const uint64_t* tab = stage_quad_tab(p.quad, s_qmix + (p.quad.nbase <= kQuadLdsBase ? p.quad.nbase : 0));
+  const uint32_t pair = threadIdx.x >> 1, npairs = blockDim.x >> 1;
+  for (uint64_t b0 = static_cast<uint64_t>(blockIdx.x) * chunk; b0 < m;
+       b0 += static_cast<uint64_t>(gridDim.x) * chunk) {
+    const uint32_t len = static_cast<uint32_t>(m - b0 < chunk ? m - b0 : chunk);
+    const ecl_query* cq = qs + b0;
+    group_by_kind(lists, len, [&](uint32_t j) { return cq[j].kind; });
+    const uint32_t c0 = lists.cnt[0], c1 = lists.cnt[1], tot = c0 + c1 + lists.cnt[2];
+    for (uint32_t e = pair; e < tot; e += npairs) {
+      uint32_t j, k;
+      if (e < c0) { j = lists.idx[0][e]; k = 0; }
+      else if (e < c0 + c1) { j = lists.idx[1][e - c0]; k = 1; }
+      else { j = lists.idx[2][e - c0 - c1]; k = 2; }
+      const ecl_query q = cq[j];
+      uint64_t r;
+      if (k == 0) {
+        r = q.arg < p.view.n ? quad_access(p, base, q.arg) : 0;
+      } else if (k == 1) {
+        r = q.arg <= p.view.n && q.symbol < p.view.sigma ? quad_rank(p, base, q.symbol, q.arg) : 0;
+      } else {
+        r = quad_select(p, tab, q.symbol, q.arg);
+      }
+      if ((threadIdx.x & 1) == 0) out[b0 + j] = r;
+    }
+    __syncthreads();
   }
 }
 
@@ -899,9 +949,17 @@ void launch_tree_mixed(const TreeQueryParams& p, const ecl_query* q,
                        uint64_t m, uint64_t* out, hipStream_t s) {
   if (m == 0) return;
   ProfScope ps(K_WT_MIXED, s);
-  const uint32_t g = query_grid(m);
-  hipLaunchKernelGGL(tree_mixed_kernel, dim3(g), dim3(256), meta_bytes(p.view), s, p, q, m, out,
-                     kind_chunk(m, g, 256));
+  if (p.quad.levels && p.quad.has_sel) {
+    const uint32_t g = query_grid(2 * m);
+    const size_t sh = ((p.quad.nbase <= kQuadLdsBase ? p.quad.nbase : 0) +
+                       (p.quad.ntab <= kQuadLdsTab ? p.quad.ntab : 0)) * sizeof(uint64_t);
+    hipLaunchKernelGGL(tree_mixed_quad_kernel, dim3(g), dim3(256), sh, s, p, q, m, out,
+                       kind_chunk(m, g, 256));
+  } else {
+    const uint32_t g = query_grid(m);
+    hipLaunchKernelGGL(tree_mixed_kernel, dim3(g), dim3(256), meta_bytes(p.view), s, p, q, m, out,
+                       kind_chunk(m, g, 256));
+  }
   ECL_HIP(hipGetLastError());
 }
 

@@ -710,10 +710,12 @@ def _skewed_text(rng, n):
 def test_mixed_batch_query_lines(gpu, m, text_kind):
     """A mixed batch (kinds grouped per chunk; with query lines, access and
     rank on two lanes each) answers like the oracle over the directory, then
-    with the query lines and hints, then with the select lines too, for odd
-    batch sizes and for balanced, one-sided and kind > 2 (read as select)
-    mixes; on a skewed text the select lines overflow and the mixed kernel's
-    select goes from the line to the hint and to the directory search."""
+    with the query lines and hints, then with the select lines too, then
+    with the quad lines and quad select lines (every query on a pair of
+    lanes), for odd batch sizes and for balanced, one-sided and kind > 2
+    (read as select) mixes; on a skewed text the select lines overflow and
+    the mixed kernel's select goes from the line to the hint and to the
+    directory search, and rare digits overflow their quad select lines."""
     from ecl_wavelet_amd import MIXED_DTYPE, DeviceBuffer, WaveletTree
     rng = np.random.default_rng(m)
     text = (rng.integers(0, 256, 300_007).astype(np.uint8) if text_kind == "uniform"
@@ -742,11 +744,13 @@ def test_mixed_batch_query_lines(gpu, m, text_kind):
         exp[k2] = O.scan_select_batch(text, a[k2], s[k2])
         d_q, d_out = DeviceBuffer(m * 16), DeviceBuffer(m * 8)
         d_q.upload(q)
-        for stage in ("directory", "query lines", "select lines"):
+        for stage in ("directory", "query lines", "select lines", "quad lines"):
             if stage == "query lines":
                 wt.buildQueryLines()
             elif stage == "select lines":
                 wt.buildSelectLines()
+            elif stage == "quad lines":
+                wt.buildQuadSelectLines()  # with the quad lines: the pair-lane mixed kernel
             wt.queryBatchDevice(d_q, m, d_out)
             assert np.array_equal(d_out.download(np.uint64, m), exp), (mix, stage)
 
