@@ -696,7 +696,9 @@ def pmc_queries(args):
     """Kernels for the query counter passes: the copy calibration, then one
     build and 10^7 device-generated queries per kind over the benchmark
     tree (each batch launched once), then the query lines and the same three
-    batches over them, then the select lines and the select batch again."""
+    batches over them, then the select lines and the select batch again,
+    then the quad lines and quad select lines and the three batches over
+    them."""
     import ctypes as C
     import ecl_wavelet_amd as E
     hip = C.CDLL("libamdhip64.so")
@@ -730,8 +732,15 @@ def pmc_queries(args):
     wt.buildSelectLines()
     wt.selectDevice(d_sq, Q, d_out, flags=1)
     E.synchronize()
+    # access and rank over the quad lines, select over the quad select lines
+    wt.buildQuadSelectLines()
+    wt.accessDevice(d_idx, Q, d_out, flags=2)
+    wt.rankDevice(d_rq, Q, d_out, flags=1)
+    wt.selectDevice(d_sq, Q, d_out, flags=1)
+    E.synchronize()
     log(f"queries: {Q} access, rank and select over n={args.n}, sigma={args.sigma}, "
-        f"without and with query lines; select with select lines")
+        f"without and with query lines; select with select lines; the three over the quad "
+        f"lines and quad select lines")
 
 
 def run_queries(wt, timer, Q, d_idx, d_rq, d_sq, d_mq, d_out):
