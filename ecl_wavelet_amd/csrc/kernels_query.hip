@@ -155,6 +155,18 @@ __device__ __forceinline__ const uint64_t* stage_quad_base(const QuadView& q, ui
   return lds;
 }
 
+// The partner's value within a pair of lanes (2k, 2k + 1): a DPP quad_perm
+// [1, 0, 3, 2] move, one VALU instruction instead of an LDS permute.  Both
+// lanes of a pair are active wherever these are used.
+__device__ __forceinline__ uint32_t pair_swap(uint32_t v) {
+  return static_cast<uint32_t>(
+      __builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0xB1, 0xF, 0xF, false));
+}
+__device__ __forceinline__ uint64_t pair_swap64(uint64_t v) {
+  return static_cast<uint64_t>(pair_swap(static_cast<uint32_t>(v))) |
+         (static_cast<uint64_t>(pair_swap(static_cast<uint32_t>(v >> 32))) << 32);
+}
+
 // one quad level for the pair: the digit at offset o of line `line` (when
 // want_digit; otherwise d is given) and the d's before position line*448+o
 struct QuadStep {
@@ -192,7 +204,7 @@ __device__ __forceinline__ QuadStep quad_step(const uint4* lines, uint64_t p, bo
     const uint32_t mine = static_cast<uint32_t>((sel >> sh) & 3u);
     const uint32_t owner = ko < 6 ? 0u : 1u;
     // the owner's digit to both lanes of the pair
-    const uint32_t other = __shfl_xor(mine, 1, 64);
+    const uint32_t other = pair_swap(mine);
     d = owner == h ? mine : other;
   }
   uint32_t cnt = 0;
@@ -205,19 +217,18 @@ __device__ __forceinline__ QuadStep quad_step(const uint4* lines, uint64_t p, bo
       cnt += static_cast<uint32_t>(__popcll(m));
     }
   }
-  cnt += __shfl_xor(cnt, 1, 64);
-  // lane 0 holds the header: counts of digits 0, 1, 2 before the line
-  uint64_t hdr = 0;
+  // lane 0 holds the header: counts of digits 0, 1, 2 before the line; each
+  // lane's part (lane 0: header + count, lane 1: count) plus the partner's
+  uint64_t part = cnt;
   if (h == 0) {
     const uint64_t h0 = (uint64_t)x0.x | ((uint64_t)x0.y << 32);
     const uint64_t h1 = (uint64_t)x0.z | ((uint64_t)x0.w << 32);
     const uint64_t m40 = (1ull << 40) - 1;
     const uint64_t c0 = h0 & m40, c1 = (h0 >> 40) | ((h1 & 0xFFFFull) << 24),
                    c2 = (h1 >> 16) & m40;
-    hdr = d == 0 ? c0 : d == 1 ? c1 : d == 2 ? c2 : line * kQuadDigits - c0 - c1 - c2;
+    part += d == 0 ? c0 : d == 1 ? c1 : d == 2 ? c2 : line * kQuadDigits - c0 - c1 - c2;
   }
-  hdr = __shfl(hdr, static_cast<int>((threadIdx.x & 63u) & ~1u), 64);
-  return QuadStep{d, hdr + cnt};
+  return QuadStep{d, part + pair_swap64(part)};
 }
 
 // access(i) over the quad lines: the symbol (minimal), for the pair
@@ -312,7 +323,6 @@ __device__ uint64_t quad_select_fallback(const TreeQueryParams& p, const uint64_
 __device__ __forceinline__ uint64_t quad_select_line(const TreeQueryParams& p, uint32_t ql,
                                                      uint32_t d, uint64_t g, bool* ok) {
   const uint32_t h = threadIdx.x & 1u;
-  const int pair0 = static_cast<int>((threadIdx.x & 63u) & ~1u);
   const uint32_t r = p.quad.sel_rate[ql][d];
   const uint64_t j = r == 1 ? g : __umul64hi(g, p.quad.sel_magic[ql][d]);
   const uint4* lp = p.quad.sel_lines[ql] + (p.quad.sel_first[ql][d] + j) * 8 + 4 * h;
@@ -327,8 +337,9 @@ __device__ __forceinline__ uint64_t quad_select_line(const TreeQueryParams& p, u
   w[5] = h ? ((uint64_t)x2.z | ((uint64_t)x2.w << 32)) : ((uint64_t)x3.z | ((uint64_t)x3.w << 32));
   w[6] = h ? ((uint64_t)x3.x | ((uint64_t)x3.y << 32)) : 0ull;
   w[7] = h ? ((uint64_t)x3.z | ((uint64_t)x3.w << 32)) : 0ull;
-  const uint64_t s = __shfl(static_cast<unsigned long long>((uint64_t)x0.x | ((uint64_t)x0.y << 32)),
-                            pair0, 64);
+  const uint64_t x0v = (uint64_t)x0.x | ((uint64_t)x0.y << 32);
+  const uint64_t x0p = pair_swap64(x0v);
+  const uint64_t s = h ? x0p : x0v;  // lane 0's header: the line's first position
   const uint64_t rem = j ? g - j * r + 1 : g;  // counted from s itself
   const uint32_t nw = h ? 8u : 6u;
   uint64_t m[8];
@@ -339,7 +350,8 @@ __device__ __forceinline__ uint64_t quad_select_line(const TreeQueryParams& p, u
     if (h == 0 && k == 0) m[k] &= ~0ull << (2 * (s & 31));  // digits before s
     cnt += static_cast<uint32_t>(__popcll(m[k]));
   }
-  const uint32_t c0 = __shfl(cnt, pair0, 64);
+  const uint32_t cp = pair_swap(cnt);
+  const uint32_t c0 = h ? cp : cnt;
   const uint32_t owner = rem <= c0 ? 0u : 1u;
   uint64_t pos = 0;
   bool found = false;
@@ -358,9 +370,12 @@ __device__ __forceinline__ uint64_t quad_select_line(const TreeQueryParams& p, u
       }
     }
   }
-  pos = __shfl(static_cast<unsigned long long>(pos), pair0 + static_cast<int>(owner), 64);
-  *ok = __shfl(found ? 1 : 0, pair0 + static_cast<int>(owner), 64) != 0;
-  return pos;
+  // the owner's answer to both lanes (found: the top bit clear)
+  const uint64_t mine = found ? pos : ~0ull;
+  const uint64_t theirs = pair_swap64(mine);
+  const uint64_t res = owner == h ? mine : theirs;
+  *ok = res != ~0ull;
+  return res;
 }
 
 // select_c(k) over the quad select lines (n when absent), for the pair
