@@ -246,6 +246,7 @@ __device__ __forceinline__ uint64_t quad_access(const TreeQueryParams& p, const 
 __device__ __forceinline__ uint64_t quad_rank(const TreeQueryParams& p, const uint64_t* base,
                                               uint64_t c, uint64_t i) {
   const uint32_t Q = p.quad.levels;
+  const uint64_t cc = detail::gld(p.view.counts, c);  // issued ahead of the descent
   uint64_t pos = i, N = 0;
   for (uint32_t l = 0; l < Q; ++l) {
     const uint32_t d = static_cast<uint32_t>(c >> (2 * (Q - 1 - l))) & 3u;
@@ -253,7 +254,7 @@ __device__ __forceinline__ uint64_t quad_rank(const TreeQueryParams& p, const ui
     pos = base[p.quad.off[l] + 4 * N + d] + st.rank;
     N = 4 * N + d;
   }
-  return pos - detail::gld(p.view.counts, c);
+  return pos - cc;
 }
 
 template <typename T>
@@ -323,9 +324,17 @@ __device__ uint64_t quad_select_fallback(const TreeQueryParams& p, const uint64_
 __device__ __forceinline__ uint64_t quad_select_line(const TreeQueryParams& p, uint32_t ql,
                                                      uint32_t d, uint64_t g, bool* ok) {
   const uint32_t h = threadIdx.x & 1u;
-  const uint32_t r = p.quad.sel_rate[ql][d];
-  const uint64_t j = r == 1 ? g : __umul64hi(g, p.quad.sel_magic[ql][d]);
-  const uint4* lp = p.quad.sel_lines[ql] + (p.quad.sel_first[ql][d] + j) * 8 + 4 * h;
+  // the level's four digits' tables (the level is the same for the whole
+  // wave), then this lane's digit picked by selects: no per-lane indexing
+  // into the kernel arguments
+  const uint32_t* rr = p.quad.sel_rate[ql];
+  const uint64_t* mg = p.quad.sel_magic[ql];
+  const uint64_t* fs = p.quad.sel_first[ql];
+  const uint32_t r = d < 2 ? (d == 0 ? rr[0] : rr[1]) : (d == 2 ? rr[2] : rr[3]);
+  const uint64_t magic = d < 2 ? (d == 0 ? mg[0] : mg[1]) : (d == 2 ? mg[2] : mg[3]);
+  const uint64_t first = d < 2 ? (d == 0 ? fs[0] : fs[1]) : (d == 2 ? fs[2] : fs[3]);
+  const uint64_t j = r == 1 ? g : __umul64hi(g, magic);
+  const uint4* lp = p.quad.sel_lines[ql] + (first + j) * 8 + 4 * h;
   const uint4 x0 = detail::gld(lp, 0), x1 = detail::gld(lp, 1), x2 = detail::gld(lp, 2),
               x3 = detail::gld(lp, 3);
   uint64_t w[8];
@@ -384,9 +393,14 @@ __device__ __forceinline__ uint64_t quad_select(const TreeQueryParams& p, const 
   const uint32_t Q = p.quad.levels;
   const uint64_t n = p.view.n, sigma = p.view.sigma;
   if (c >= sigma || k == 0) return n;
-  const uint64_t cc = detail::gld(p.view.counts, c);
-  const uint64_t cnt = (c + 1 < sigma ? detail::gld(p.view.counts, c + 1) : n) - cc;
-  if (k > cnt) return n;
+  // c's count from the node tables of the last quad level (c's node there
+  // is c >> 2, its digit c & 3): the d's before the next node minus those
+  // before this one
+  {
+    const uint64_t* CD = tab + p.quad.cd_off[Q - 1];
+    const uint64_t N = c >> 2, d = c & 3;
+    if (k > CD[4 * (N + 1) + d] - CD[4 * N + d]) return n;
+  }
   uint64_t kk = k;
   for (int32_t ql = static_cast<int32_t>(Q) - 1; ql >= 0; --ql) {
     const uint64_t N = c >> (2 * (Q - ql));
