@@ -145,14 +145,27 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void t
 // sit in LDS when they fit.
 constexpr uint64_t kQuadLdsBase = 4096;  // u64 entries
 
-__host__ __device__ __forceinline__ size_t quad_lds_bytes(const QuadView& q) {
-  return q.nbase <= kQuadLdsBase ? q.nbase * sizeof(uint64_t) : 0;
-}
-__device__ __forceinline__ const uint64_t* stage_quad_base(const QuadView& q, uint64_t* lds) {
-  if (q.nbase > kQuadLdsBase) return q.base;
-  for (uint64_t i = threadIdx.x; i < q.nbase; i += blockDim.x) lds[i] = q.base[i];
-  __syncthreads();
-  return lds;
+// A node table read from LDS (staged by the workgroup) or from global
+// memory, fixed at compile time so that the reads are ds_read or global
+// loads rather than generic (flat) loads, which wait on both counters.
+template <bool Lds>
+struct QuadTable {
+  const uint64_t* t;
+  __device__ __forceinline__ uint64_t operator[](uint64_t i) const {
+    if constexpr (Lds) return t[i];
+    else return detail::gld(t, i);
+  }
+};
+// Lds only when nbase <= kQuadLdsBase (the launch's LDS holds the table)
+template <bool Lds>
+__device__ __forceinline__ QuadTable<Lds> stage_quad_base(const QuadView& q, uint64_t* lds) {
+  if constexpr (!Lds) {
+    return QuadTable<Lds>{q.base};
+  } else {
+    for (uint64_t i = threadIdx.x; i < q.nbase; i += blockDim.x) lds[i] = q.base[i];
+    __syncthreads();
+    return QuadTable<Lds>{lds};
+  }
 }
 
 // The partner's value within a pair of lanes (2k, 2k + 1): a DPP quad_perm
@@ -232,7 +245,8 @@ __device__ __forceinline__ QuadStep quad_step(const uint4* lines, uint64_t p, bo
 }
 
 // access(i) over the quad lines: the symbol (minimal), for the pair
-__device__ __forceinline__ uint64_t quad_access(const TreeQueryParams& p, const uint64_t* base,
+template <typename Table>
+__device__ __forceinline__ uint64_t quad_access(const TreeQueryParams& p, Table base,
                                                 uint64_t i) {
   uint64_t pos = i, N = 0;
   for (uint32_t l = 0; l < p.quad.levels; ++l) {
@@ -243,7 +257,8 @@ __device__ __forceinline__ uint64_t quad_access(const TreeQueryParams& p, const 
   return N;
 }
 // rank_c(i) over the quad lines, c < sigma and i <= n, for the pair
-__device__ __forceinline__ uint64_t quad_rank(const TreeQueryParams& p, const uint64_t* base,
+template <typename Table>
+__device__ __forceinline__ uint64_t quad_rank(const TreeQueryParams& p, Table base,
                                               uint64_t c, uint64_t i) {
   const uint32_t Q = p.quad.levels;
   const uint64_t cc = detail::gld(p.view.counts, c);  // issued ahead of the descent
@@ -257,11 +272,11 @@ __device__ __forceinline__ uint64_t quad_rank(const TreeQueryParams& p, const ui
   return pos - cc;
 }
 
-template <typename T>
+template <typename T, bool Lds>
 __global__ __launch_bounds__(256) void tree_access_quad_kernel(
     TreeQueryParams p, const uint64_t* __restrict__ idx, uint64_t m, T* __restrict__ out) {
   extern __shared__ uint64_t s_qbase[];
-  const uint64_t* base = stage_quad_base(p.quad, s_qbase);
+  const QuadTable<Lds> base = stage_quad_base<Lds>(p.quad, s_qbase);
   const T* amap = static_cast<const T*>(p.alphabet_out);
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x / 2;
   for (uint64_t q = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) / 2;
@@ -273,11 +288,11 @@ __global__ __launch_bounds__(256) void tree_access_quad_kernel(
   }
 }
 
-template <typename T>
+template <typename T, bool Lds>
 __global__ __launch_bounds__(256) void tree_rank_quad_kernel(
     TreeQueryParams p, const uint8_t* __restrict__ qs, uint64_t m, uint64_t* __restrict__ out) {
   extern __shared__ uint64_t s_qbase[];
-  const uint64_t* base = stage_quad_base(p.quad, s_qbase);
+  const QuadTable<Lds> base = stage_quad_base<Lds>(p.quad, s_qbase);
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x / 2;
   for (uint64_t q = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) / 2;
        q < m; q += stride) {
@@ -299,30 +314,71 @@ __global__ __launch_bounds__(256) void tree_rank_quad_kernel(
 // levels' selects instead.
 constexpr uint64_t kQuadLdsTab = 4096;  // u64 entries
 
-__device__ uint64_t quad_select_fallback(const TreeQueryParams& p, const uint64_t* tab,
-                                         uint32_t ql, uint32_t d, uint64_t N, uint64_t kk) {
-  const uint64_t* S = tab + p.quad.s_off[ql];
-  const uint64_t* CD = tab + p.quad.cd_off[ql];
-  const uint64_t SN = S[N];
-  const uint64_t cnt0 = CD[4 * (N + 1)] - CD[4 * N], cnt1 = CD[4 * (N + 1) + 1] - CD[4 * N + 1];
-  const uint32_t hi = d >> 1, lo = d & 1;
-  // level 2q + 1: node 2N + hi starts at SM; its ones before it
-  const uint64_t SM = SN + (hi ? cnt0 + cnt1 : 0);
-  const uint64_t onesM = CD[4 * N + 1] + CD[4 * N + 3] + (hi ? cnt1 : 0);
-  const LevelView v1 = p.view.levels[2 * ql + 1];
-  const uint64_t p1 = lo ? detail::level_select<1, false, true, true>(v1, onesM + kk)
-                         : detail::level_select<0, false, true, true>(v1, (SM - onesM) + kk);
-  const uint64_t kk1 = p1 - SM + 1;
-  // level 2q: node N, bit hi
-  const uint64_t onesN = CD[4 * N + 2] + CD[4 * N + 3];
-  const LevelView v0 = p.view.levels[2 * ql];
-  return hi ? detail::level_select<1, false, true, true>(v0, onesN + kk1)
-            : detail::level_select<0, false, true, true>(v0, (SN - onesN) + kk1);
+// One 128-byte line of 448 digits for the pair (a quad line, or a quad
+// select line, which has the same layout): this lane's digit words (lane 0:
+// words 0..5, lane 1: words 6..13, zeros after) and the line's first 16
+// bytes (lane 0's header).
+struct PairLine {
+  uint64_t w[8];
+  uint64_t h0, h1;
+};
+__device__ __forceinline__ PairLine pair_line(const uint4* line) {
+  const uint32_t h = threadIdx.x & 1u;
+  const uint4* lp = line + 4 * h;
+  const uint4 x0 = detail::gld(lp, 0), x1 = detail::gld(lp, 1), x2 = detail::gld(lp, 2),
+              x3 = detail::gld(lp, 3);
+  PairLine L;
+  L.w[0] = h ? ((uint64_t)x0.x | ((uint64_t)x0.y << 32)) : ((uint64_t)x1.x | ((uint64_t)x1.y << 32));
+  L.w[1] = h ? ((uint64_t)x0.z | ((uint64_t)x0.w << 32)) : ((uint64_t)x1.z | ((uint64_t)x1.w << 32));
+  L.w[2] = h ? ((uint64_t)x1.x | ((uint64_t)x1.y << 32)) : ((uint64_t)x2.x | ((uint64_t)x2.y << 32));
+  L.w[3] = h ? ((uint64_t)x1.z | ((uint64_t)x1.w << 32)) : ((uint64_t)x2.z | ((uint64_t)x2.w << 32));
+  L.w[4] = h ? ((uint64_t)x2.x | ((uint64_t)x2.y << 32)) : ((uint64_t)x3.x | ((uint64_t)x3.y << 32));
+  L.w[5] = h ? ((uint64_t)x2.z | ((uint64_t)x2.w << 32)) : ((uint64_t)x3.z | ((uint64_t)x3.w << 32));
+  L.w[6] = h ? ((uint64_t)x3.x | ((uint64_t)x3.y << 32)) : 0ull;
+  L.w[7] = h ? ((uint64_t)x3.z | ((uint64_t)x3.w << 32)) : 0ull;
+  L.h0 = (uint64_t)x0.x | ((uint64_t)x0.y << 32);
+  L.h1 = (uint64_t)x0.z | ((uint64_t)x0.w << 32);
+  return L;
 }
 
-// position of the g-th (1-based) d of quad level ql, for the pair
+// The digit index in the line (0..447) of the need-th (1-based) match among
+// the pair's masks m, where c0 is lane 0's count of matches; ~0u to both
+// lanes when the line holds fewer than `need`.  The word is picked first
+// and selected in once: a select under each word test would run, on wave64,
+// for every word that any lane stops at.
+__device__ __forceinline__ uint32_t pair_select_in_line(const uint64_t (&m)[8], uint32_t c0,
+                                                        uint64_t need) {
+  const uint32_t h = threadIdx.x & 1u;
+  const uint32_t nw = h ? 8u : 6u;
+  const uint32_t owner = need <= c0 ? 0u : 1u;
+  uint64_t nd = h ? need - c0 : need;
+  uint64_t word = 0;
+  uint32_t kw = 0;
+  bool found = false;
+#pragma unroll
+  for (uint32_t k = 0; k < 8; ++k) {
+    const uint32_t c = static_cast<uint32_t>(__popcll(m[k]));
+    const bool hit = !found && k < nw && nd <= c;
+    word = hit ? m[k] : word;
+    kw = hit ? k : kw;
+    nd = found || hit || k >= nw ? nd : nd - c;
+    found = found || hit;
+  }
+  const uint32_t idx =
+      owner == h && found
+          ? 32u * (kw + (h ? 6u : 0u)) + (detail::select_in_word(word, static_cast<uint32_t>(nd)) >> 1)
+          : ~0u;
+  // the owner's answer to both lanes
+  const uint32_t theirs = pair_swap(idx);
+  return owner == h ? idx : theirs;
+}
+
+// position of the g-th (1-based) d of quad level ql, for the pair, from its
+// quad select line; *past is set (and n returned) when the d lies beyond
+// the line's digits, which then end before digit *end
 __device__ __forceinline__ uint64_t quad_select_line(const TreeQueryParams& p, uint32_t ql,
-                                                     uint32_t d, uint64_t g, bool* ok) {
+                                                     uint32_t d, uint64_t g, bool* past,
+                                                     uint64_t* end) {
   const uint32_t h = threadIdx.x & 1u;
   // the level's four digits' tables (the level is the same for the whole
   // wave), then this lane's digit picked by selects: no per-lane indexing
@@ -334,61 +390,66 @@ __device__ __forceinline__ uint64_t quad_select_line(const TreeQueryParams& p, u
   const uint64_t magic = d < 2 ? (d == 0 ? mg[0] : mg[1]) : (d == 2 ? mg[2] : mg[3]);
   const uint64_t first = d < 2 ? (d == 0 ? fs[0] : fs[1]) : (d == 2 ? fs[2] : fs[3]);
   const uint64_t j = r == 1 ? g : __umul64hi(g, magic);
-  const uint4* lp = p.quad.sel_lines[ql] + (first + j) * 8 + 4 * h;
-  const uint4 x0 = detail::gld(lp, 0), x1 = detail::gld(lp, 1), x2 = detail::gld(lp, 2),
-              x3 = detail::gld(lp, 3);
-  uint64_t w[8];
-  w[0] = h ? ((uint64_t)x0.x | ((uint64_t)x0.y << 32)) : ((uint64_t)x1.x | ((uint64_t)x1.y << 32));
-  w[1] = h ? ((uint64_t)x0.z | ((uint64_t)x0.w << 32)) : ((uint64_t)x1.z | ((uint64_t)x1.w << 32));
-  w[2] = h ? ((uint64_t)x1.x | ((uint64_t)x1.y << 32)) : ((uint64_t)x2.x | ((uint64_t)x2.y << 32));
-  w[3] = h ? ((uint64_t)x1.z | ((uint64_t)x1.w << 32)) : ((uint64_t)x2.z | ((uint64_t)x2.w << 32));
-  w[4] = h ? ((uint64_t)x2.x | ((uint64_t)x2.y << 32)) : ((uint64_t)x3.x | ((uint64_t)x3.y << 32));
-  w[5] = h ? ((uint64_t)x2.z | ((uint64_t)x2.w << 32)) : ((uint64_t)x3.z | ((uint64_t)x3.w << 32));
-  w[6] = h ? ((uint64_t)x3.x | ((uint64_t)x3.y << 32)) : 0ull;
-  w[7] = h ? ((uint64_t)x3.z | ((uint64_t)x3.w << 32)) : 0ull;
-  const uint64_t x0v = (uint64_t)x0.x | ((uint64_t)x0.y << 32);
-  const uint64_t x0p = pair_swap64(x0v);
-  const uint64_t s = h ? x0p : x0v;  // lane 0's header: the line's first position
+  const PairLine L = pair_line(p.quad.sel_lines[ql] + (first + j) * 8);
+  const uint64_t hp = pair_swap64(L.h0);
+  const uint64_t s = h ? hp : L.h0;  // lane 0's header: the line's first position
   const uint64_t rem = j ? g - j * r + 1 : g;  // counted from s itself
   const uint32_t nw = h ? 8u : 6u;
   uint64_t m[8];
   uint32_t cnt = 0;
 #pragma unroll
   for (uint32_t k = 0; k < 8; ++k) {
-    m[k] = k < nw ? quad_match(w[k], d) : 0ull;
+    m[k] = k < nw ? quad_match(L.w[k], d) : 0ull;
     if (h == 0 && k == 0) m[k] &= ~0ull << (2 * (s & 31));  // digits before s
     cnt += static_cast<uint32_t>(__popcll(m[k]));
   }
   const uint32_t cp = pair_swap(cnt);
   const uint32_t c0 = h ? cp : cnt;
-  const uint32_t owner = rem <= c0 ? 0u : 1u;
-  uint64_t pos = 0;
-  bool found = false;
-  if (owner == h) {
-    uint32_t need = static_cast<uint32_t>(h ? rem - c0 : rem);
+  const uint32_t idx = pair_select_in_line(m, c0, rem);
+  *past = idx == ~0u;
+  *end = (s & ~31ull) + kQuadDigits;
+  return (s & ~31ull) + idx;
+}
+
+// position of the g-th (1-based) d of quad level ql at or after digit `from`
+// (a line start; fewer than g d's lie before it), for the pair: a forward
+// walk over the quad lines, each header giving the d's before its line.  The
+// walk ends within the level's lines because g is at most the level's count
+// of d; n is returned if it does not.
+__device__ __forceinline__ uint64_t quad_select_walk(const TreeQueryParams& p, uint32_t ql,
+                                                         uint32_t d, uint64_t g, uint64_t from) {
+  const uint32_t h = threadIdx.x & 1u;
+  const uint64_t n = p.view.n;
+  const uint64_t nl = __umul64hi(n >> 6, 0x2492492492492493ull) + 1;  // n / 448 + 1 lines
+  const uint32_t nw = h ? 8u : 6u;
+  for (uint64_t l = __umul64hi(from >> 6, 0x2492492492492493ull); l < nl; ++l) {
+    const PairLine L = pair_line(p.quad.lines[ql] + l * 8);
+    uint64_t m[8];
+    uint32_t cnt = 0;
 #pragma unroll
     for (uint32_t k = 0; k < 8; ++k) {
-      const uint32_t c = static_cast<uint32_t>(__popcll(m[k]));
-      if (!found && k < nw) {
-        if (need <= c) {
-          pos = (s & ~31ull) + 32ull * (k + (h ? 6u : 0u)) + (detail::select_in_word(m[k], need) >> 1);
-          found = true;
-        } else {
-          need -= c;
-        }
-      }
+      m[k] = k < nw ? quad_match(L.w[k], d) : 0ull;
+      cnt += static_cast<uint32_t>(__popcll(m[k]));
     }
+    // lane 0: the header's count of d before the line
+    const uint64_t m40 = (1ull << 40) - 1;
+    const uint64_t c0h = L.h0 & m40, c1h = (L.h0 >> 40) | ((L.h1 & 0xFFFFull) << 24),
+                   c2h = (L.h1 >> 16) & m40;
+    const uint64_t bd =
+        d == 0 ? c0h : d == 1 ? c1h : d == 2 ? c2h : l * kQuadDigits - c0h - c1h - c2h;
+    const uint64_t bp = pair_swap64(bd);
+    const uint64_t before = h ? bp : bd;
+    const uint32_t cp = pair_swap(cnt);
+    const uint32_t c0 = h ? cp : cnt;
+    const uint64_t total = before + cnt + cp;  // both lanes: the d's up to the line's end
+    if (g <= total) return l * kQuadDigits + pair_select_in_line(m, c0, g - before);
   }
-  // the owner's answer to both lanes (found: the top bit clear)
-  const uint64_t mine = found ? pos : ~0ull;
-  const uint64_t theirs = pair_swap64(mine);
-  const uint64_t res = owner == h ? mine : theirs;
-  *ok = res != ~0ull;
-  return res;
+  return n;
 }
 
 // select_c(k) over the quad select lines (n when absent), for the pair
-__device__ __forceinline__ uint64_t quad_select(const TreeQueryParams& p, const uint64_t* tab,
+template <typename Table>
+__device__ __forceinline__ uint64_t quad_select(const TreeQueryParams& p, Table tab,
                                                 uint64_t c, uint64_t k) {
   const uint32_t Q = p.quad.levels;
   const uint64_t n = p.view.n, sigma = p.view.sigma;
@@ -397,35 +458,47 @@ __device__ __forceinline__ uint64_t quad_select(const TreeQueryParams& p, const 
   // is c >> 2, its digit c & 3): the d's before the next node minus those
   // before this one
   {
-    const uint64_t* CD = tab + p.quad.cd_off[Q - 1];
+    const uint64_t cd = p.quad.cd_off[Q - 1];
     const uint64_t N = c >> 2, d = c & 3;
-    if (k > CD[4 * (N + 1) + d] - CD[4 * N + d]) return n;
+    if (k > tab[cd + 4 * (N + 1) + d] - tab[cd + 4 * N + d]) return n;
   }
   uint64_t kk = k;
   for (int32_t ql = static_cast<int32_t>(Q) - 1; ql >= 0; --ql) {
     const uint64_t N = c >> (2 * (Q - ql));
     const uint32_t d = static_cast<uint32_t>(c >> (2 * (Q - 1 - ql))) & 3u;
     const uint64_t g = tab[p.quad.cd_off[ql] + 4 * N + d] + kk;
-    bool ok = false;
-    uint64_t pos = quad_select_line(p, static_cast<uint32_t>(ql), d, g, &ok);
-    if (!ok) pos = quad_select_fallback(p, tab, static_cast<uint32_t>(ql), d, N, kk);
+    bool past = false;
+    uint64_t end = 0;
+    uint64_t pos = quad_select_line(p, static_cast<uint32_t>(ql), d, g, &past, &end);
+    // past the line's digits (a sparse stretch of d): the quad lines from
+    // the one holding digit `end` on
+    if (past) pos = quad_select_walk(p, static_cast<uint32_t>(ql), d, g, end);
     kk = pos - tab[p.quad.s_off[ql] + N] + 1;
   }
   return kk - 1;
 }
 
-__device__ __forceinline__ const uint64_t* stage_quad_tab(const QuadView& q, uint64_t* lds) {
-  if (q.ntab > kQuadLdsTab) return q.tab;
-  for (uint64_t i = threadIdx.x; i < q.ntab; i += blockDim.x) lds[i] = q.tab[i];
-  __syncthreads();
-  return lds;
+// Lds only when ntab <= kQuadLdsTab (the launch's LDS holds the table)
+template <bool Lds>
+__device__ __forceinline__ QuadTable<Lds> stage_quad_tab(const QuadView& q, uint64_t* lds) {
+  if constexpr (!Lds) {
+    return QuadTable<Lds>{q.tab};
+  } else {
+    for (uint64_t i = threadIdx.x; i < q.ntab; i += blockDim.x) lds[i] = q.tab[i];
+    __syncthreads();
+    return QuadTable<Lds>{lds};
+  }
 }
 
-template <typename T>
-__global__ __launch_bounds__(256) void tree_select_quad_kernel(
+// Occupancy target: 6 waves per SIMD (80 VGPRs, no VGPR spills, instead of
+// the compiler's 99 and 4 waves): 1.32 vs 1.50 ms per 10^7 at n = 6e9,
+// sigma = 256; 5 waves gave 1.36 ms, 7 (4 VGPRs spilled) 1.41, 8 (10
+// spilled) 1.58.
+template <typename T, bool Lds>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void tree_select_quad_kernel(
     TreeQueryParams p, const uint8_t* __restrict__ qs, uint64_t m, uint64_t* __restrict__ out) {
   extern __shared__ uint64_t s_qtab[];
-  const uint64_t* tab = stage_quad_tab(p.quad, s_qtab);
+  const QuadTable<Lds> tab = stage_quad_tab<Lds>(p.quad, s_qtab);
   const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x / 2;
   for (uint64_t q = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) / 2;
        q < m; q += stride) {
@@ -560,14 +633,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void t
 
 // A mixed batch over the quad lines and quad select lines: the chunk's
 // queries grouped by kind as above, then the three lists walked with a pair
-// of lanes per query.  Symbols are minimal indices.
-__global__ __launch_bounds__(256) void tree_mixed_quad_kernel(
+// of lanes per query.  Symbols are minimal indices.  Occupancy target: 5
+// waves per SIMD (96 VGPRs, no spills; the compiler's choice is 121 and 4
+// waves): 3.52 vs 3.66 ms per 3e7 at n = 6e9, sigma = 256; 6 waves spill
+// 17 VGPRs and took 3.86 ms.
+template <bool Lds>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void tree_mixed_quad_kernel(
     TreeQueryParams p, const ecl_query* __restrict__ qs, uint64_t m,
     uint64_t* __restrict__ out, uint32_t chunk) {
   extern __shared__ uint64_t s_qmix[];
   __shared__ KindLists lists;
-  const uint64_t* base = stage_quad_base(p.quad, s_qmix);
-  const uint64_t* tab = stage_quad_tab(p.quad, s_qmix + (p.quad.nbase <= kQuadLdsBase ? p.quad.nbase : 0));
+  // Lds: both tables fit (nbase <= kQuadLdsBase and ntab <= kQuadLdsTab)
+  const QuadTable<Lds> base = stage_quad_base<Lds>(p.quad, s_qmix);
+  const QuadTable<Lds> tab = stage_quad_tab<Lds>(p.quad, s_qmix + (Lds ? p.quad.nbase : 0));
   const uint32_t pair = threadIdx.x >> 1, npairs = blockDim.x >> 1;
   for (uint64_t b0 = static_cast<uint64_t>(blockIdx.x) * chunk; b0 < m;
        b0 += static_cast<uint64_t>(gridDim.x) * chunk) {
@@ -905,22 +983,42 @@ void rank_g(const TreeQueryParams& p, const uint8_t* qq, uint64_t m, uint64_t* o
 
 void set_query_lanes(uint32_t lanes) { g_query_lanes.store(lanes); }
 
+template <bool Lds>
+void access_quad(const TreeQueryParams& p, const uint64_t* idx, uint64_t m, void* out,
+                 hipStream_t s) {
+  const dim3 g(query_grid(2 * m)), b(256);
+  const size_t sh = Lds ? p.quad.nbase * sizeof(uint64_t) : 0;
+  switch (p.esize) {
+    case 1: hipLaunchKernelGGL((tree_access_quad_kernel<uint8_t, Lds>), g, b, sh, s, p, idx, m, static_cast<uint8_t*>(out)); break;
+    case 2: hipLaunchKernelGGL((tree_access_quad_kernel<uint16_t, Lds>), g, b, sh, s, p, idx, m, static_cast<uint16_t*>(out)); break;
+    case 4: hipLaunchKernelGGL((tree_access_quad_kernel<uint32_t, Lds>), g, b, sh, s, p, idx, m, static_cast<uint32_t*>(out)); break;
+    default: hipLaunchKernelGGL((tree_access_quad_kernel<uint64_t, Lds>), g, b, sh, s, p, idx, m, static_cast<uint64_t*>(out)); break;
+  }
+}
+
 void launch_tree_access(const TreeQueryParams& p, const uint64_t* idx,
                         uint64_t m, void* out, hipStream_t s) {
   if (m == 0) return;
   ProfScope ps(K_WT_ACCESS, s);
   if (p.quad.levels) {
-    const dim3 g(query_grid(2 * m)), b(256);
-    const size_t sh = quad_lds_bytes(p.quad);
-    switch (p.esize) {
-      case 1: hipLaunchKernelGGL((tree_access_quad_kernel<uint8_t>), g, b, sh, s, p, idx, m, static_cast<uint8_t*>(out)); break;
-      case 2: hipLaunchKernelGGL((tree_access_quad_kernel<uint16_t>), g, b, sh, s, p, idx, m, static_cast<uint16_t*>(out)); break;
-      case 4: hipLaunchKernelGGL((tree_access_quad_kernel<uint32_t>), g, b, sh, s, p, idx, m, static_cast<uint32_t*>(out)); break;
-      default: hipLaunchKernelGGL((tree_access_quad_kernel<uint64_t>), g, b, sh, s, p, idx, m, static_cast<uint64_t*>(out)); break;
-    }
+    if (p.quad.nbase <= kQuadLdsBase) access_quad<true>(p, idx, m, out, s);
+    else access_quad<false>(p, idx, m, out, s);
   } else if (pair_lanes(p)) access_g<2>(p, idx, m, out, s);
   else access_g<1>(p, idx, m, out, s);
   ECL_HIP(hipGetLastError());
+}
+
+template <bool Lds>
+void rank_quad(const TreeQueryParams& p, const uint8_t* qq, uint64_t m, uint64_t* out,
+               hipStream_t s) {
+  const dim3 g(query_grid(2 * m)), b(256);
+  const size_t sh = Lds ? p.quad.nbase * sizeof(uint64_t) : 0;
+  switch (p.esize) {
+    case 1: hipLaunchKernelGGL((tree_rank_quad_kernel<uint8_t, Lds>), g, b, sh, s, p, qq, m, out); break;
+    case 2: hipLaunchKernelGGL((tree_rank_quad_kernel<uint16_t, Lds>), g, b, sh, s, p, qq, m, out); break;
+    case 4: hipLaunchKernelGGL((tree_rank_quad_kernel<uint32_t, Lds>), g, b, sh, s, p, qq, m, out); break;
+    default: hipLaunchKernelGGL((tree_rank_quad_kernel<uint64_t, Lds>), g, b, sh, s, p, qq, m, out); break;
+  }
 }
 
 void launch_tree_rank(const TreeQueryParams& p, const void* q, uint64_t m,
@@ -929,14 +1027,8 @@ void launch_tree_rank(const TreeQueryParams& p, const void* q, uint64_t m,
   ProfScope ps(K_WT_RANK, s);
   const uint8_t* qq = static_cast<const uint8_t*>(q);
   if (p.quad.levels) {
-    const dim3 g(query_grid(2 * m)), b(256);
-    const size_t sh = quad_lds_bytes(p.quad);
-    switch (p.esize) {
-      case 1: hipLaunchKernelGGL((tree_rank_quad_kernel<uint8_t>), g, b, sh, s, p, qq, m, out); break;
-      case 2: hipLaunchKernelGGL((tree_rank_quad_kernel<uint16_t>), g, b, sh, s, p, qq, m, out); break;
-      case 4: hipLaunchKernelGGL((tree_rank_quad_kernel<uint32_t>), g, b, sh, s, p, qq, m, out); break;
-      default: hipLaunchKernelGGL((tree_rank_quad_kernel<uint64_t>), g, b, sh, s, p, qq, m, out); break;
-    }
+    if (p.quad.nbase <= kQuadLdsBase) rank_quad<true>(p, qq, m, out, s);
+    else rank_quad<false>(p, qq, m, out, s);
   } else if (pair_lanes(p)) rank_g<2>(p, qq, m, out, s);
   else rank_g<1>(p, qq, m, out, s);
   ECL_HIP(hipGetLastError());
@@ -955,20 +1047,27 @@ void select_sl(const TreeQueryParams& p, const uint8_t* qq, uint64_t m, uint64_t
   }
 }
 
+template <bool Lds>
+void select_quad(const TreeQueryParams& p, const uint8_t* qq, uint64_t m, uint64_t* out,
+                 hipStream_t s) {
+  const dim3 g(query_grid(2 * m)), b(256);
+  const size_t sh = Lds ? p.quad.ntab * sizeof(uint64_t) : 0;
+  switch (p.esize) {
+    case 1: hipLaunchKernelGGL((tree_select_quad_kernel<uint8_t, Lds>), g, b, sh, s, p, qq, m, out); break;
+    case 2: hipLaunchKernelGGL((tree_select_quad_kernel<uint16_t, Lds>), g, b, sh, s, p, qq, m, out); break;
+    case 4: hipLaunchKernelGGL((tree_select_quad_kernel<uint32_t, Lds>), g, b, sh, s, p, qq, m, out); break;
+    default: hipLaunchKernelGGL((tree_select_quad_kernel<uint64_t, Lds>), g, b, sh, s, p, qq, m, out); break;
+  }
+}
+
 void launch_tree_select(const TreeQueryParams& p, const void* q, uint64_t m,
                         uint64_t* out, hipStream_t s) {
   if (m == 0) return;
   ProfScope ps(K_WT_SELECT, s);
   const uint8_t* qq = static_cast<const uint8_t*>(q);
   if (p.quad.has_sel) {
-    const dim3 g(query_grid(2 * m)), b(256);
-    const size_t sh = p.quad.ntab <= kQuadLdsTab ? p.quad.ntab * sizeof(uint64_t) : 0;
-    switch (p.esize) {
-      case 1: hipLaunchKernelGGL((tree_select_quad_kernel<uint8_t>), g, b, sh, s, p, qq, m, out); break;
-      case 2: hipLaunchKernelGGL((tree_select_quad_kernel<uint16_t>), g, b, sh, s, p, qq, m, out); break;
-      case 4: hipLaunchKernelGGL((tree_select_quad_kernel<uint32_t>), g, b, sh, s, p, qq, m, out); break;
-      default: hipLaunchKernelGGL((tree_select_quad_kernel<uint64_t>), g, b, sh, s, p, qq, m, out); break;
-    }
+    if (p.quad.ntab <= kQuadLdsTab) select_quad<true>(p, qq, m, out, s);
+    else select_quad<false>(p, qq, m, out, s);
   } else if (p.has_sel_lines) select_sl<true>(p, qq, m, out, s);
   else select_sl<false>(p, qq, m, out, s);
   ECL_HIP(hipGetLastError());
@@ -980,10 +1079,14 @@ void launch_tree_mixed(const TreeQueryParams& p, const ecl_query* q,
   ProfScope ps(K_WT_MIXED, s);
   if (p.quad.levels && p.quad.has_sel) {
     const uint32_t g = query_grid(2 * m);
-    const size_t sh = ((p.quad.nbase <= kQuadLdsBase ? p.quad.nbase : 0) +
-                       (p.quad.ntab <= kQuadLdsTab ? p.quad.ntab : 0)) * sizeof(uint64_t);
-    hipLaunchKernelGGL(tree_mixed_quad_kernel, dim3(g), dim3(256), sh, s, p, q, m, out,
-                       kind_chunk(m, g, 256));
+    if (p.quad.nbase <= kQuadLdsBase && p.quad.ntab <= kQuadLdsTab) {
+      const size_t sh = (p.quad.nbase + p.quad.ntab) * sizeof(uint64_t);
+      hipLaunchKernelGGL(tree_mixed_quad_kernel<true>, dim3(g), dim3(256), sh, s, p, q, m, out,
+                         kind_chunk(m, g, 256));
+    } else {
+      hipLaunchKernelGGL(tree_mixed_quad_kernel<false>, dim3(g), dim3(256), 0, s, p, q, m, out,
+                         kind_chunk(m, g, 256));
+    }
   } else {
     const uint32_t g = query_grid(m);
     hipLaunchKernelGGL(tree_mixed_kernel, dim3(g), dim3(256), meta_bytes(p.view), s, p, q, m, out,
