@@ -310,8 +310,8 @@ __global__ __launch_bounds__(256) void tree_rank_quad_kernel(
 // (CD[N][d] + kk)-th d of the level, found in one select line read by the
 // pair (lane 0: position and digit words 0..5, lane 1: words 6..13); its
 // index in node N is its position minus the node's start, plus one.  When
-// the d lies past the line's 448 digits, the step takes the two binary
-// levels' selects instead.
+// the d lies past the line's 448 digits, the step walks on over the quad
+// lines from the next one, whose headers count the d's before each line.
 constexpr uint64_t kQuadLdsTab = 4096;  // u64 entries
 
 // One 128-byte line of 448 digits for the pair (a quad line, or a quad
@@ -490,10 +490,11 @@ __device__ __forceinline__ QuadTable<Lds> stage_quad_tab(const QuadView& q, uint
   }
 }
 
-// Occupancy target: 6 waves per SIMD (80 VGPRs, no VGPR spills, instead of
-// the compiler's 99 and 4 waves): 1.32 vs 1.50 ms per 10^7 at n = 6e9,
-// sigma = 256; 5 waves gave 1.36 ms, 7 (4 VGPRs spilled) 1.41, 8 (10
-// spilled) 1.58.
+// Occupancy: the kernel takes 66 VGPRs (7 waves per SIMD) and the target
+// keeps it at 6 or more.  With the directory selects inlined for lines that
+// overflow, it had taken 99 (4 waves), and 6 waves were worth 1.32 vs 1.50 ms
+// per 10^7 at n = 6e9, sigma = 256; forcing 8 now spills 2 VGPRs and is no
+// faster (1.26 vs 1.20-1.24 ms).
 template <typename T, bool Lds>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void tree_select_quad_kernel(
     TreeQueryParams p, const uint8_t* __restrict__ qs, uint64_t m, uint64_t* __restrict__ out) {
@@ -634,9 +635,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void t
 // A mixed batch over the quad lines and quad select lines: the chunk's
 // queries grouped by kind as above, then the three lists walked with a pair
 // of lanes per query.  Symbols are minimal indices.  Occupancy target: 5
-// waves per SIMD (96 VGPRs, no spills; the compiler's choice is 121 and 4
-// waves): 3.52 vs 3.66 ms per 3e7 at n = 6e9, sigma = 256; 6 waves spill
-// 17 VGPRs and took 3.86 ms.
+// waves per SIMD (96 VGPRs, no spills), which the kernel now reaches by
+// itself; with the directory selects inlined it had taken 121 (4 waves), and
+// 5 waves were worth 3.52 vs 3.66 ms per 3e7 at n = 6e9, sigma = 256.  6
+// waves spill 10 VGPRs and are no faster (3.53 ms).
 template <bool Lds>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void tree_mixed_quad_kernel(
     TreeQueryParams p, const ecl_query* __restrict__ qs, uint64_t m,
