@@ -514,8 +514,10 @@ def bench_full_sweep(args):
     6e9 symbols uniform over sigma in {4, 16, 64}, and a uint16 text of 2e9
     symbols Zipf(1.1) over sigma = 2^16.  Per text: median of two timed
     builds after one untimed, then 10^7 device-generated queries per kind,
-    over the plain directory, again over the query and select lines, and
-    access and rank over the quad lines."""
+    over the plain directory, again over the query and select lines, access
+    and rank over the quad lines, select over the quad select lines, and
+    access and rank over the hex lines where the level count is a multiple
+    of 4."""
     import ecl_wavelet_amd as E
     timer = E.Timer()
     Q = 10_000_000
@@ -568,6 +570,23 @@ def bench_full_sweep(args):
             timer.start()
             call_()
             tq[kind] = timer.stop()
+        # select over the quad select lines
+        timer.start()
+        wt.buildQuadSelectLines()
+        qsel_ms = timer.stop()
+        timer.start()
+        calls[2][1]()
+        tq["select"] = timer.stop()
+        # access and rank over the hex lines (level counts that are multiples of 4)
+        hex_ms, th = None, {}
+        if int(wt.getNumLevels()) % 4 == 0:
+            timer.start()
+            wt.buildHexLines()
+            hex_ms = timer.stop()
+            for kind, call_ in calls[:2]:
+                timer.start()
+                call_()
+                th[kind] = timer.stop()
         build_ms = float(np.median(builds))
         r = {"dtype": np.dtype(dt).name, "n": n, "sigma": sigma,
              "distribution": "uniform" if thr is None else "zipf(1.1)",
@@ -579,14 +598,22 @@ def bench_full_sweep(args):
              "access_qps_lines": Q / (tl["access"] / 1e3), "rank_qps_lines": Q / (tl["rank"] / 1e3),
              "select_qps_lines": Q / (tl["select"] / 1e3),
              "quad_lines_build_ms": quad_ms,
-             "access_qps_quad": Q / (tq["access"] / 1e3), "rank_qps_quad": Q / (tq["rank"] / 1e3)}
+             "access_qps_quad": Q / (tq["access"] / 1e3), "rank_qps_quad": Q / (tq["rank"] / 1e3),
+             "quad_select_lines_build_ms": qsel_ms,
+             "select_qps_quad": Q / (tq["select"] / 1e3),
+             "hex_lines_build_ms": hex_ms,
+             "access_qps_hex": Q / (th["access"] / 1e3) if th else None,
+             "rank_qps_hex": Q / (th["rank"] / 1e3) if th else None}
         out.append(r)
         log(f"sweep {r['dtype']} sigma={sigma} n={n}: build {build_ms:.1f} ms, access "
             f"{r['access_qps']:.3g}, rank {r['rank_qps']:.3g}, select {r['select_qps']:.3g} q/s; "
             f"with lines ({lines_ms:.1f} ms): access {r['access_qps_lines']:.3g}, rank "
             f"{r['rank_qps_lines']:.3g}, select {r['select_qps_lines']:.3g} q/s; "
             f"quad lines ({quad_ms:.1f} ms): access {r['access_qps_quad']:.3g}, rank "
-            f"{r['rank_qps_quad']:.3g} q/s")
+            f"{r['rank_qps_quad']:.3g} q/s; quad select lines ({qsel_ms:.1f} ms): select "
+            f"{r['select_qps_quad']:.3g} q/s"
+            + (f"; hex lines ({hex_ms:.1f} ms): access {r['access_qps_hex']:.3g}, rank "
+               f"{r['rank_qps_hex']:.3g} q/s" if th else ""))
         del wt, d_text
     return out
 
@@ -698,7 +725,7 @@ def pmc_queries(args):
     tree (each batch launched once), then the query lines and the same three
     batches over them, then the select lines and the select batch again,
     then the quad lines and quad select lines and the three batches over
-    them."""
+    them, then the hex lines and the access and rank batches over them."""
     import ctypes as C
     import ecl_wavelet_amd as E
     hip = C.CDLL("libamdhip64.so")
@@ -738,9 +765,14 @@ def pmc_queries(args):
     wt.rankDevice(d_rq, Q, d_out, flags=1)
     wt.selectDevice(d_sq, Q, d_out, flags=1)
     E.synchronize()
+    # access and rank over the hex lines
+    wt.buildHexLines()
+    wt.accessDevice(d_idx, Q, d_out, flags=2)
+    wt.rankDevice(d_rq, Q, d_out, flags=1)
+    E.synchronize()
     log(f"queries: {Q} access, rank and select over n={args.n}, sigma={args.sigma}, "
         f"without and with query lines; select with select lines; the three over the quad "
-        f"lines and quad select lines")
+        f"lines and quad select lines; access and rank over the hex lines")
 
 
 def run_queries(wt, timer, Q, d_idx, d_rq, d_sq, d_mq, d_out):
@@ -801,6 +833,10 @@ def main():
     quad_build_ms, quad_sel_build_ms, quad_mixed_ms = [], [], []
     # quad lines: power-of-two alphabets with an even number of levels
     use_quad = sigma >= 4 and (sigma & (sigma - 1)) == 0 and (sigma.bit_length() - 1) % 2 == 0
+    # hex lines: ... whose level count is a multiple of 4
+    use_hex = use_quad and (sigma.bit_length() - 1) % 4 == 0
+    hex_ms, hex_build_ms = [], []
+    hex_kind_ms = {"access": [], "rank": []}
     plain_ms = {"access": [], "rank": [], "select": [], "mixed": []}
     lines_bytes = sel_bytes = qsel_bytes = 0
     wt = None
@@ -864,6 +900,18 @@ def main():
             timer.start()
             wt.queryBatchDevice(d_mq, 3 * Q, d_out)
             qmx = timer.stop()
+        ht = {}
+        if use_hex:
+            # hex lines, then access and rank over them (one line per four levels)
+            timer.start()
+            wt.buildHexLines()
+            hb = timer.stop()
+            timer.start()
+            wt.accessDevice(d_idx, Q, d_out, flags=2)
+            ht["access"] = timer.stop()
+            timer.start()
+            wt.rankDevice(d_rq, Q, d_out, flags=1)
+            ht["rank"] = timer.stop()
         log(f"step {step}{'' if timed else ' (warmup)'}: build {b:.1f} ms, "
             f"plain access {plain['access']:.2f} ms, rank {plain['rank']:.2f} ms, "
             f"select {plain['select']:.2f} ms, mixed {plain['mixed']:.2f} ms; lines {lb:.2f} ms, "
@@ -872,7 +920,9 @@ def main():
             f"select {times['select']:.2f} ms, mixed {mx:.2f} ms"
             + (f"; quad lines {qb:.2f} ms, access {qt['access']:.2f} ms, "
                f"rank {qt['rank']:.2f} ms; quad select lines {qsb:.2f} ms, "
-               f"select {qt['select']:.2f} ms, mixed {qmx:.2f} ms" if use_quad else ""))
+               f"select {qt['select']:.2f} ms, mixed {qmx:.2f} ms" if use_quad else "")
+            + (f"; hex lines {hb:.2f} ms, access {ht['access']:.2f} ms, "
+               f"rank {ht['rank']:.2f} ms" if use_hex else ""))
         if timed:
             build_ms.append(b)
             lines_ms.append(lb)
@@ -891,6 +941,11 @@ def main():
                 quad_sel_build_ms.append(qsb)
                 for k, v in qt.items():
                     quad_kind_ms[k].append(v)
+            if use_hex:
+                hex_ms.append(hb + ht["access"] + ht["rank"])
+                hex_build_ms.append(hb)
+                for k, v in ht.items():
+                    hex_kind_ms[k].append(v)
 
     # per-call latency of a plain small batch vs the same work batched
     lat = []
@@ -932,15 +987,24 @@ def main():
     line_peak = E.probe_random_lines(int(lines_bytes), Q, steps=L, lanes=2)
     line_peak_half = (E.probe_random_lines(int(lines_bytes), Q, steps=L // 2, lanes=2)
                       if use_quad else None)
+    line_peak_quarter = (E.probe_random_lines(int(lines_bytes), Q, steps=L // 4, lanes=2)
+                         if use_hex else None)
     build = float(np.median(build_ms))
     qtime = float(np.mean(q_ms)) / 1e3
     lines_value = 3 * Q / qtime
-    # value: access and rank over the quad lines where the tree has them,
-    # select over the select lines
+    # value: each kind over the fastest layout the tree has (access and rank
+    # over the hex lines, else the quad lines, else the query lines; select
+    # over the quad select lines, else the select lines)
     kind_best = {k: float(np.mean(v)) for k, v in kind_ms.items()}
     if use_quad:
         for k, v in quad_kind_ms.items():
             kind_best[k] = float(np.mean(v))
+    if use_hex:
+        for k, v in hex_kind_ms.items():
+            kind_best[k] = float(np.mean(v))
+    ar_layout, ar_lines, ar_peak = (("hex lines", L // 4, line_peak_quarter) if use_hex else
+                                    ("quad lines", L // 2, line_peak_half) if use_quad else
+                                    ("query lines", L, line_peak))
     value = 3 * Q / (sum(kind_best.values()) / 1e3)
     pb = pass_bytes(n, sigma)
     pab = pass_algorithmic_bytes(n, sigma)
@@ -963,7 +1027,8 @@ def main():
         "ms_per_step": float(np.mean(np.array(build_ms) + np.array(plain_total_ms)
                                      + np.array(lines_ms) + np.array(sel_ms)
                                      + np.array(q_ms) + np.array(mixed_ms)
-                                     + (np.array(quad_ms) if use_quad else 0.0))),
+                                     + (np.array(quad_ms) if use_quad else 0.0)
+                                     + (np.array(hex_ms) if use_hex else 0.0))),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -975,7 +1040,8 @@ def main():
                          f"{Q} each of access/rank/select + one mixed batch of {3 * Q} "
                          f"over the plain directory, the query lines and select "
                          f"lines, the same queries over them, then the quad lines and "
-                         f"quad select lines and the three kinds over them"),
+                         f"quad select lines and the three kinds over them, then the hex "
+                         f"lines and access and rank over them"),
             "n": n, "sigma": sigma, "queries_per_kind": Q, "levels": L,
         },
         "build_ms": build,
@@ -990,9 +1056,15 @@ def main():
         "rank_qps": Q / (kind_best["rank"] / 1e3),
         "select_qps": Q / (kind_best["select"] / 1e3),
         "layouts": {
-            "access_rank": "quad lines" if use_quad else "query lines",
+            "access_rank": ar_layout,
             "select": "quad select lines" if use_quad else "select lines",
         },
+        "hex_lines_build_ms": float(np.median(hex_build_ms)) if use_hex else None,
+        "hex_lines_bytes": (L // 4) * (n // 128 + 1) * 128 if use_hex else None,
+        "quad_lines_layout": ({
+            "access_qps": Q / (np.mean(quad_kind_ms["access"]) / 1e3),
+            "rank_qps": Q / (np.mean(quad_kind_ms["rank"]) / 1e3),
+        } if use_quad else None),
         "quad_lines_build_ms": float(np.median(quad_build_ms)) if use_quad else None,
         "quad_lines_bytes": (L // 2) * (n // 448 + 1) * 128 if use_quad else None,
         "quad_select_lines_build_ms": float(np.median(quad_sel_build_ms)) if use_quad else None,
@@ -1017,15 +1089,16 @@ def main():
             "peak_half_chains": line_peak_half,
             "peak_half_chains_source": (f"the same probe with {L // 2} dependent lines per "
                                         f"query (the quad lines' chains)" if use_quad else None),
+            "peak_quarter_chains": line_peak_quarter,
+            "peak_quarter_chains_source": (f"the same probe with {L // 4} dependent lines per "
+                                           f"query (the hex lines' chains)" if use_hex else None),
             **{kind: {"layout": lay, "lines_per_query": lpq,
                       "achieved": Q / (kind_best[kind] / 1e3) * lpq,
                       "peak": pk,
                       "frac": Q / (kind_best[kind] / 1e3) * lpq / pk}
                for kind, lay, lpq, pk in (
-                   ("access", "quad lines" if use_quad else "query lines",
-                    L // 2 if use_quad else L, line_peak_half if use_quad else line_peak),
-                   ("rank", "quad lines" if use_quad else "query lines",
-                    L // 2 if use_quad else L, line_peak_half if use_quad else line_peak),
+                   ("access", ar_layout, ar_lines, ar_peak),
+                   ("rank", ar_layout, ar_lines, ar_peak),
                    ("select", "quad select lines" if use_quad else "select lines",
                     L // 2 if use_quad else L, line_peak_half if use_quad else line_peak))},
         },

@@ -511,6 +511,12 @@ class WaveletTree:
         change."""
         call("ecl_wavelet_tree_build_quad_lines", self._h)
 
+    def buildHexLines(self):
+        """Hex lines (power-of-two alphabets whose level count is a multiple
+        of 4; builds the quad lines first when absent): access and rank read
+        one memory line per four levels; results do not change."""
+        call("ecl_wavelet_tree_build_hex_lines", self._h)
+
     def buildQuadSelectLines(self):
         """Quad select lines (builds the quad lines first when absent): select
         reads one memory line per two levels; results do not change."""

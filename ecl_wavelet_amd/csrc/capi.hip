@@ -566,6 +566,8 @@ struct ecl_wavelet_tree_t {
   DevBuf quad_lines, quad_base, quad_tab;  // optional quad lines, bases, node tables
   DevBuf quad_sel;                          // optional quad select lines
   QuadView quad{};
+  DevBuf hex_lines, hex_tab;                // optional hex lines, bases and superblocks
+  HexView hex{};
   std::unique_ptr<HostPipe> pipe{new HostPipe()};
   ~ecl_wavelet_tree_t() { delete rs; }
 };
@@ -1709,6 +1711,94 @@ static void build_quad_select_lines(ecl_wavelet_tree_t* wt) {
   qv.has_sel = 1;
 }
 
+// Hex lines of a tree (HexView): for hex level h, from the symbol counts,
+// per node N of level 4h (16^h nodes, N = the codes' top 4h bits) and hex
+// digit x: cnt[N][x] = the symbols with top 4h + 4 bits 16N + x; CHD[N][x] =
+// the x's before the node (exclusive prefix of cnt over N); base[16N + x] =
+// C of child 16N + x minus CHD[N][x].  The lines come from the quad lines of
+// quad levels 2h and 2h + 1 on the device, which build first; the kernel
+// also writes the superblock counts.
+static void build_hex_lines(ecl_wavelet_tree_t* wt) {
+  if (wt->hex.levels != 0) return;
+  const uint32_t L = wt->L;
+  if (!wt->pow2 || L % 4 != 0 || L < 4 || L > 4 * kMaxHexLevels)
+    throw Error(ECL_ERROR_INVALID_ARGUMENT,
+                "hex lines need a power-of-two alphabet whose level count is a multiple of 4 "
+                "(at most 16)");
+  build_quad_lines(wt);
+  const uint32_t H = L / 4;
+  const uint64_t n = wt->n, sigma = wt->sigma;
+  std::vector<uint64_t> C(sigma + 1);
+  for (uint64_t c = 0; c < sigma; ++c) C[c] = wt->counts[c];
+  C[sigma] = n;
+  const uint64_t nlines = n / kHexDigits + 1;
+  const uint64_t nsb = (nlines >> kHexSbShift) + 1;
+  // tab (kept): bases, then superblocks; btab (build only): CHD, node starts
+  std::vector<uint64_t> tab, btab;
+  uint64_t base_off[kMaxHexLevels] = {}, sb_off[kMaxHexLevels] = {};
+  uint64_t chd_off[kMaxHexLevels] = {}, s_off[kMaxHexLevels] = {}, nodes[kMaxHexLevels] = {};
+  for (uint32_t h = 0; h < H; ++h) {
+    const uint64_t nn = 1ull << (4 * h);
+    const uint32_t shift = L - 4 * h - 4;  // child 16N + x spans symbols [(16N+x) << shift, ...)
+    nodes[h] = nn;
+    base_off[h] = tab.size();
+    chd_off[h] = btab.size();
+    uint64_t acc[16] = {};
+    for (uint64_t N = 0; N <= nn; ++N) {
+      for (uint32_t x = 0; x < 16; ++x) btab.push_back(acc[x]);  // CHD[N][x]
+      if (N == nn) break;
+      for (uint32_t x = 0; x < 16; ++x) {
+        const uint64_t lo = (16 * N + x) << shift, hi = (16 * N + x + 1) << shift;
+        tab.push_back(C[lo] - acc[x]);  // base[16N + x]
+        acc[x] += C[hi] - C[lo];
+      }
+    }
+    s_off[h] = btab.size();
+    for (uint64_t N = 0; N <= nn; ++N) btab.push_back(C[std::min<uint64_t>(N << (shift + 4), sigma)]);
+  }
+  for (uint32_t h = 0; h < H; ++h) {
+    sb_off[h] = tab.size();
+    tab.resize(tab.size() + 16 * nsb, 0);  // written by the build kernel
+  }
+  const uint64_t bytes = H * nlines * 128;
+  if (bytes + (tab.size() + btab.size()) * 8 + (64u << 20) > free_device_memory())
+    throw Error(ECL_ERROR_OUT_OF_MEMORY_TREE, kOomTree);
+  DevBuf lines(bytes);
+  DevBuf dtab(tab.size() * 8);
+  DevBuf dbtab(btab.size() * 8);
+  h2d(dtab.ptr, tab.data(), tab.size());
+  h2d(dbtab.ptr, btab.data(), btab.size());
+  const QuadView& qv = wt->quad;
+  hipStream_t s = nullptr;
+  HexView hv{};
+  for (uint32_t h = 0; h < H; ++h) {
+    HexBuildParams hp{};
+    hp.qa = qv.lines[2 * h];
+    hp.qb = qv.lines[2 * h + 1];
+    hp.qbase = qv.base + qv.off[2 * h];
+    hp.qcd = qv.tab + qv.cd_off[2 * h + 1];
+    hp.chd = dbtab.as<uint64_t>() + chd_off[h];
+    hp.S = dbtab.as<uint64_t>() + s_off[h];
+    hp.nodes = nodes[h];
+    hp.n = n;
+    hp.nql = n / kQuadDigits + 1;
+    hp.nlines = nlines;
+    hp.out = lines.as<uint4>() + h * nlines * 8;
+    hp.sb = dtab.as<uint64_t>() + sb_off[h];
+    launch_hex_lines(hp, s);
+    hv.lines[h] = hp.out;
+    hv.base_off[h] = base_off[h];
+    hv.sb_off[h] = sb_off[h];
+  }
+  ECL_HIP(hipStreamSynchronize(s));
+  hv.tab = dtab.as<uint64_t>();
+  hv.ntab = tab.size();
+  hv.levels = H;
+  wt->hex_lines = std::move(lines);
+  wt->hex_tab = std::move(dtab);
+  wt->hex = hv;
+}
+
 static void build_select_lines(ecl_rank_select_t* rs) {
   if (rs->sel_lines.ptr != nullptr) return;
   const size_t na = rs->levels.size();
@@ -1954,6 +2044,7 @@ static TreeQueryParams query_params(const ecl_wavelet_tree_t* wt, uint32_t flags
   p.has_sel_lines = wt->rs->sel_lines.ptr != nullptr ? 1 : 0;
   p.lanes = wt->lanes;
   p.quad = wt->quad;
+  p.hex = wt->hex;
   return p;
 }
 
@@ -2435,6 +2526,13 @@ ecl_status ecl_wavelet_tree_build_quad_lines(ecl_wavelet_tree wt) {
     require(wt != nullptr, "NULL tree");
     set_device(wt->device);
     build_quad_lines(wt);
+  });
+}
+ecl_status ecl_wavelet_tree_build_hex_lines(ecl_wavelet_tree wt) {
+  return guard([&] {
+    require(wt != nullptr, "NULL tree");
+    set_device(wt->device);
+    build_hex_lines(wt);
   });
 }
 ecl_status ecl_wavelet_tree_build_quad_select_lines(ecl_wavelet_tree wt) {

@@ -292,9 +292,62 @@ constexpr uint32_t kQuadWgInfo = 6;
 uint64_t quad_build_workgroups(uint64_t nlines);
 void launch_quad_lines(const QuadBuildParams& qp, hipStream_t s);
 
+// Hex lines (opt-in; power-of-two alphabets whose level count is a multiple
+// of 4, at most 16): hex level h holds the 4-bit digits of the codes (bits
+// 4h .. 4h+3 from the top, the first one high) in the order of binary level
+// 4h, in 128-byte lines of 128 digits.  A line is four 32-byte parts: the
+// counts of digits 0..7 before the line (u32 each), digits 0..63, the
+// counts of digits 8..15, digits 64..127 (digit i at bits 4(i mod 16) of
+// its 64-bit word), so that each lane of a pair reads one half.  A count is
+// stored modulo 2^32: with sb = the full count at the start of the line's
+// superblock (2^24 lines, 2^31 digits), the count is sb + (u32)(stored - sb).
+// Descending from position p in node N of hex level h by digit x leads to
+// position base[16N + x] + (x's before p) of hex level h + 1 (base = the
+// child's start minus the x's before node N).  With them access and rank
+// read one line per four levels.
+constexpr uint32_t kHexDigits = 128;
+constexpr uint32_t kMaxHexLevels = 4;
+constexpr uint32_t kHexSbShift = 24;  // lines per superblock: 2^24
+// matches of digit x among the 16 digits of w, one bit per digit (bit 4i)
+__device__ __forceinline__ uint64_t hex_match(uint64_t w, uint32_t x) {
+  const uint64_t y = w ^ (0x1111111111111111ull * x);
+  return ~(y | (y >> 1) | (y >> 2) | (y >> 3)) & 0x1111111111111111ull;
+}
+struct HexView {
+  const uint4* lines[kMaxHexLevels];  // n / 128 + 1 lines per level
+  // one array of ntab u64: tab[base_off[h] + 16N + x] = base, and
+  // tab[sb_off[h] + 16 s + x] = the x's before superblock s
+  const uint64_t* tab;
+  uint64_t ntab;
+  uint64_t base_off[kMaxHexLevels], sb_off[kMaxHexLevels];
+  uint32_t levels;  // 0: no hex lines
+};
+
+// One hex level's build from the quad lines of quad levels 2h (qa) and
+// 2h + 1 (qb).  Position p of node N with quad digit d0 goes to position
+// e = qbase[4N + d0] + (d0's before p) of quad level 2h + 1, whose digit is
+// the hex digit's low half.  The x's before position P (x = 4 d0 + d1) are
+// CHD[N][x] + (d1's before e_d0(P) at level 2h + 1) - qcd[4 (4N + d0) + d1].
+struct HexBuildParams {
+  const uint4* qa;
+  const uint4* qb;
+  const uint64_t* qbase;  // quad bases of level 2h: qbase[4N + d0]
+  const uint64_t* qcd;    // quad level 2h + 1's CD table: qcd[4M + d1]
+  const uint64_t* chd;    // CHD[16N + x]: the x's before node N (nodes + 1 rows)
+  const uint64_t* S;      // starts of the nodes of hex level h (nodes + 1)
+  uint64_t nodes;         // 16^h
+  uint64_t n;
+  uint64_t nql;           // quad lines per quad level: n / 448 + 1
+  uint64_t nlines;        // hex lines: n / 128 + 1
+  uint4* out;
+  uint64_t* sb;           // superblock counts: sb[16 s + x]
+};
+void launch_hex_lines(const HexBuildParams& hp, hipStream_t s);
+
 struct TreeQueryParams {
   TreeView view;
   QuadView quad;             // quad.levels == 0: none
+  HexView hex;               // hex.levels == 0: none
   int esize;
   int symbols_minimal;       // 1: query symbols already minimal indices
   int map_kind; const uint32_t* lut; const void* alphabet;  // symbol map

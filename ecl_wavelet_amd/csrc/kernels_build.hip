@@ -1150,6 +1150,156 @@ void launch_quad_select_lines(const QuadSelBuildParams& qp, hipStream_t s) {
   ECL_HIP(hipGetLastError());
 }
 
+// ---------------------------------------------------------------------------
+// Hex lines of one hex level h from the quad lines of quad levels 2h (qa)
+// and 2h + 1 (qb).  Four lanes build a hex line, lane q the digits
+// [32 q, 32 q + 32) (one 32-digit word of qa) and the counts of hex digits
+// 4q .. 4q + 3 before the line.
+
+// digit word g of a quad level (14 per line, after the line's 16-byte header)
+__device__ __forceinline__ uint64_t quad_word(const uint4* ql, uint64_t g) {
+  const uint64_t line = __umul64hi(g >> 1, 0x2492492492492493ull);  // g / 14
+  const uint64_t k = g - line * 14;
+  return detail::gld(reinterpret_cast<const uint64_t*>(ql), line * 16 + 2 + k);
+}
+// the d's before position e of a quad level, for each d (one lane)
+__device__ __forceinline__ void quad_counts(const uint4* ql, uint64_t e, uint64_t (&c)[4]) {
+  const uint64_t line = __umul64hi(e >> 6, 0x2492492492492493ull);  // e / 448
+  const uint32_t o = static_cast<uint32_t>(e - line * kQuadDigits);
+  const uint4 hd = detail::gld(ql, line * 8);
+  const uint64_t h0 = (uint64_t)hd.x | ((uint64_t)hd.y << 32);
+  const uint64_t h1 = (uint64_t)hd.z | ((uint64_t)hd.w << 32);
+  const uint64_t m40 = (1ull << 40) - 1;
+  c[0] = h0 & m40;
+  c[1] = (h0 >> 40) | ((h1 & 0xFFFFull) << 24);
+  c[2] = (h1 >> 16) & m40;
+  c[3] = line * kQuadDigits - c[0] - c[1] - c[2];
+  const uint32_t kw = o >> 5, sh = 2 * (o & 31);
+  const uint64_t* w = reinterpret_cast<const uint64_t*>(ql) + line * 16 + 2;
+  for (uint32_t k = 0; k <= kw; ++k) {
+    const uint64_t m = k < kw ? ~0ull : (sh ? ~0ull >> (64 - sh) : 0ull);
+    if (m == 0) break;
+    const uint64_t x = detail::gld(w, k);
+    const uint64_t lo = x & m & 0x5555555555555555ull, hi = (x >> 1) & m & 0x5555555555555555ull;
+    c[1] += static_cast<uint64_t>(__popcll(lo & ~hi));
+    c[2] += static_cast<uint64_t>(__popcll(hi & ~lo));
+    c[3] += static_cast<uint64_t>(__popcll(lo & hi));
+    c[0] += static_cast<uint64_t>(__popcll(~(lo | hi) & m & 0x5555555555555555ull));
+  }
+}
+// 32 digits of a quad level from position x on (nwords: the level's words)
+__device__ __forceinline__ uint64_t quad_window(const uint4* ql, uint64_t nwords, uint64_t x) {
+  const uint64_t g = x >> 5;
+  const uint32_t sh = 2 * static_cast<uint32_t>(x & 31);
+  const uint64_t a = quad_word(ql, g);
+  if (sh == 0) return a;
+  const uint64_t b = g + 1 < nwords ? quad_word(ql, g + 1) : 0ull;
+  return (a >> sh) | (b << (64 - sh));
+}
+
+__global__ __launch_bounds__(256) void hex_lines_kernel(HexBuildParams hp) {
+  const uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  const uint64_t l = t >> 2;  // hex line
+  const uint32_t q = static_cast<uint32_t>(t & 3);
+  // the four lanes of a line are all present or all absent
+  if (l >= hp.nlines) return;
+  const uint64_t n = hp.n;
+  const uint64_t P = l * kHexDigits + 32 * q;  // this lane's first position
+  const uint64_t Pc = P < n ? P : n;
+  // node of Pc: the last node starting at or before it
+  uint64_t N = 0;
+  {
+    uint64_t hi = hp.nodes;
+    while (hi - N > 1) {
+      const uint64_t mid = (N + hi) >> 1;
+      if (detail::gld(hp.S, mid) <= Pc) N = mid;
+      else hi = mid;
+    }
+  }
+  uint64_t g[4];
+  quad_counts(hp.qa, Pc, g);
+  // ---- counts of hex digits 4q .. 4q + 3 before the line's first position
+  // (lane 0's node and quad counts)
+  const uint32_t src = (threadIdx.x & 63u) & ~3u;
+  const uint64_t N0 = __shfl(N, src, 64);
+  const uint64_t g00 = __shfl(g[0], src, 64), g01 = __shfl(g[1], src, 64),
+                 g02 = __shfl(g[2], src, 64), g03 = __shfl(g[3], src, 64);
+  {
+    const uint64_t gq = q == 0 ? g00 : q == 1 ? g01 : q == 2 ? g02 : g03;
+    const uint64_t e = detail::gld(hp.qbase, 4 * N0 + q) + gq;
+    uint64_t c2[4];
+    quad_counts(hp.qb, e, c2);
+    uint64_t cnt[4];
+#pragma unroll
+    for (uint32_t d1 = 0; d1 < 4; ++d1)
+      cnt[d1] = detail::gld(hp.chd, 16 * N0 + 4 * q + d1) + c2[d1] -
+                detail::gld(hp.qcd, 4 * (4 * N0 + q) + d1);
+    hp.out[l * 8 + (q < 2 ? q : 2 + q)] =
+        make_uint4(static_cast<uint32_t>(cnt[0]), static_cast<uint32_t>(cnt[1]),
+                   static_cast<uint32_t>(cnt[2]), static_cast<uint32_t>(cnt[3]));
+    if ((l & ((1ull << kHexSbShift) - 1)) == 0) {
+#pragma unroll
+      for (uint32_t d1 = 0; d1 < 4; ++d1) hp.sb[(l >> kHexSbShift) * 16 + 4 * q + d1] = cnt[d1];
+    }
+  }
+  // ---- the 32 digits: position P + j with quad digit d0 goes to position
+  // qbase[4N + d0] + (d0's before it) of quad level 2h + 1
+  const uint32_t valid = P < n ? (n - P < 32 ? static_cast<uint32_t>(n - P) : 32u) : 0u;
+  const uint64_t w = valid ? quad_word(hp.qa, P >> 5) : 0ull;
+  uint64_t lo = 0, hi = 0;
+  const uint64_t nend = N + 1 < hp.nodes ? detail::gld(hp.S, N + 1) : n;
+  if (valid && nend >= P + valid) {
+    // one node: the positions of each d0 take consecutive digits of one run
+    const uint64_t nwords = hp.nql * 14;
+    uint64_t w0 = quad_window(hp.qb, nwords, detail::gld(hp.qbase, 4 * N + 0) + g[0]);
+    uint64_t w1 = quad_window(hp.qb, nwords, detail::gld(hp.qbase, 4 * N + 1) + g[1]);
+    uint64_t w2 = quad_window(hp.qb, nwords, detail::gld(hp.qbase, 4 * N + 2) + g[2]);
+    uint64_t w3 = quad_window(hp.qb, nwords, detail::gld(hp.qbase, 4 * N + 3) + g[3]);
+#pragma unroll
+    for (uint32_t j = 0; j < 32; ++j) {
+      if (j < valid) {
+        const uint32_t d0 = static_cast<uint32_t>(w >> (2 * j)) & 3u;
+        const uint64_t wd = d0 == 0 ? w0 : d0 == 1 ? w1 : d0 == 2 ? w2 : w3;
+        const uint64_t x = 4 * d0 + (static_cast<uint32_t>(wd) & 3u);
+        w0 = d0 == 0 ? w0 >> 2 : w0;
+        w1 = d0 == 1 ? w1 >> 2 : w1;
+        w2 = d0 == 2 ? w2 >> 2 : w2;
+        w3 = d0 == 3 ? w3 >> 2 : w3;
+        if (j < 16) lo |= x << (4 * j);
+        else hi |= x << (4 * (j - 16));
+      }
+    }
+  } else {
+    // a node starts inside the word: position by position
+    for (uint32_t j = 0; j < valid; ++j) {
+      const uint64_t pj = P + j;
+      while (N + 1 < hp.nodes && detail::gld(hp.S, N + 1) <= pj) ++N;
+      const uint32_t d0 = static_cast<uint32_t>(w >> (2 * j)) & 3u;
+      const uint64_t gd = d0 == 0 ? g[0] : d0 == 1 ? g[1] : d0 == 2 ? g[2] : g[3];
+      const uint64_t e = detail::gld(hp.qbase, 4 * N + d0) + gd;
+      g[0] += d0 == 0;
+      g[1] += d0 == 1;
+      g[2] += d0 == 2;
+      g[3] += d0 == 3;
+      const uint64_t d1 = (quad_word(hp.qb, e >> 5) >> (2 * (e & 31))) & 3u;
+      const uint64_t x = 4 * d0 + d1;
+      if (j < 16) lo |= x << (4 * j);
+      else hi |= x << (4 * (j - 16));
+    }
+  }
+  hp.out[l * 8 + (q < 2 ? 2 + q : 4 + q)] =
+      make_uint4(static_cast<uint32_t>(lo), static_cast<uint32_t>(lo >> 32),
+                 static_cast<uint32_t>(hi), static_cast<uint32_t>(hi >> 32));
+}
+
+void launch_hex_lines(const HexBuildParams& hp, hipStream_t s) {
+  if (hp.nlines == 0) return;
+  const uint64_t blocks = (hp.nlines * 4 + 255) / 256;
+  if (blocks > 0x7FFFFFFFull) throw Error(ECL_ERROR_INVALID_ARGUMENT, "bit array too large");
+  hipLaunchKernelGGL(hex_lines_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(256), 0, s, hp);
+  ECL_HIP(hipGetLastError());
+}
+
 void launch_query_lines(const LevelView& v, uint4* lines, uint64_t nlines, hipStream_t s) {
   if (nlines == 0) return;
   const uint64_t blocks = (nlines * 8 + 255) / 256;

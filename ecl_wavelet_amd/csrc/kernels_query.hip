@@ -305,6 +305,128 @@ __global__ __launch_bounds__(256) void tree_rank_quad_kernel(
 }
 
 // ---------------------------------------------------------------------------
+// Access and rank over hex lines (see HexView): one 128-byte line per four
+// levels.  Two adjacent lanes share a query: lane h loads the line's half h
+// (the counts of digits 8h .. 8h + 7 and digits 64h .. 64h + 63); each
+// counts the digit in its digits before the offset, the lane holding the
+// digit's stored count adds it, and one pair swap adds the two parts.  The
+// bases and superblock counts sit in LDS when they fit.
+constexpr uint64_t kHexLdsTab = 4096;  // u64 entries
+
+template <bool Lds>
+__device__ __forceinline__ QuadTable<Lds> stage_hex_tab(const HexView& x, uint64_t* lds) {
+  if constexpr (!Lds) {
+    return QuadTable<Lds>{x.tab};
+  } else {
+    for (uint64_t i = threadIdx.x; i < x.ntab; i += blockDim.x) lds[i] = x.tab[i];
+    __syncthreads();
+    return QuadTable<Lds>{lds};
+  }
+}
+
+// one hex level for the pair: the digit at position p (when want_digit;
+// otherwise d is given) and the d's before p
+template <typename Table>
+__device__ __forceinline__ QuadStep hex_step(const uint4* lines, Table tab, uint64_t sb_off,
+                                             uint64_t p, bool want_digit, uint32_t d_given) {
+  const uint32_t h = threadIdx.x & 1u;
+  const uint64_t line = p >> 7;
+  const uint32_t o = static_cast<uint32_t>(p & 127);
+  const uint4* lp = lines + line * 8 + 4 * h;
+  const uint4 c0 = detail::gld(lp, 0), c1 = detail::gld(lp, 1), x0 = detail::gld(lp, 2),
+              x1 = detail::gld(lp, 3);
+  const uint64_t w[4] = {(uint64_t)x0.x | ((uint64_t)x0.y << 32),
+                         (uint64_t)x0.z | ((uint64_t)x0.w << 32),
+                         (uint64_t)x1.x | ((uint64_t)x1.y << 32),
+                         (uint64_t)x1.z | ((uint64_t)x1.w << 32)};
+  uint32_t d = d_given;
+  if (want_digit) {
+    const uint32_t k = (o >> 4) & 3u, sh = 4 * (o & 15);
+    const uint64_t sel = k == 0 ? w[0] : k == 1 ? w[1] : k == 2 ? w[2] : w[3];
+    const uint32_t mine = static_cast<uint32_t>(sel >> sh) & 15u;
+    const uint32_t other = pair_swap(mine);
+    d = (o >> 6) == h ? mine : other;  // the owner's digit to both lanes
+  }
+  uint32_t cnt = 0;
+#pragma unroll
+  for (uint32_t k = 0; k < 4; ++k) {
+    const uint32_t g0 = 64 * h + 16 * k;  // the word's first digit
+    if (g0 < o) {
+      uint64_t m = hex_match(w[k], d);
+      if (g0 + 16 > o) m &= ~0ull >> (64 - 4 * (o - g0));
+      cnt += static_cast<uint32_t>(__popcll(m));
+    }
+  }
+  // the stored count of d (mod 2^32): lane d >> 3, entry d & 7
+  const uint32_t e = d & 7u;
+  const uint4 cc = e < 4 ? c0 : c1;
+  const uint32_t cv = (e & 3u) == 0 ? cc.x : (e & 3u) == 1 ? cc.y : (e & 3u) == 2 ? cc.z : cc.w;
+  const uint32_t part = cnt + ((d >> 3) == h ? cv : 0u);
+  const uint32_t tot = part + pair_swap(part);
+  const uint64_t sbv = tab[sb_off + (line >> kHexSbShift) * 16 + d];
+  return QuadStep{d, sbv + static_cast<uint32_t>(tot - static_cast<uint32_t>(sbv))};
+}
+
+// access(i) over the hex lines: the symbol (minimal), for the pair
+template <typename Table>
+__device__ __forceinline__ uint64_t hex_access(const TreeQueryParams& p, Table tab, uint64_t i) {
+  uint64_t pos = i, N = 0;
+  for (uint32_t l = 0; l < p.hex.levels; ++l) {
+    const QuadStep st = hex_step(p.hex.lines[l], tab, p.hex.sb_off[l], pos, true, 0);
+    pos = tab[p.hex.base_off[l] + 16 * N + st.d] + st.rank;
+    N = 16 * N + st.d;
+  }
+  return N;
+}
+// rank_c(i) over the hex lines, c < sigma and i <= n, for the pair
+template <typename Table>
+__device__ __forceinline__ uint64_t hex_rank(const TreeQueryParams& p, Table tab, uint64_t c,
+                                             uint64_t i) {
+  const uint32_t H = p.hex.levels;
+  const uint64_t cc = detail::gld(p.view.counts, c);  // issued ahead of the descent
+  uint64_t pos = i, N = 0;
+  for (uint32_t l = 0; l < H; ++l) {
+    const uint32_t d = static_cast<uint32_t>(c >> (4 * (H - 1 - l))) & 15u;
+    const QuadStep st = hex_step(p.hex.lines[l], tab, p.hex.sb_off[l], pos, false, d);
+    pos = tab[p.hex.base_off[l] + 16 * N + d] + st.rank;
+    N = 16 * N + d;
+  }
+  return pos - cc;
+}
+
+template <typename T, bool Lds>
+__global__ __launch_bounds__(256) void tree_access_hex_kernel(
+    TreeQueryParams p, const uint64_t* __restrict__ idx, uint64_t m, T* __restrict__ out) {
+  extern __shared__ uint64_t s_htab[];
+  const QuadTable<Lds> tab = stage_hex_tab<Lds>(p.hex, s_htab);
+  const T* amap = static_cast<const T*>(p.alphabet_out);
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x / 2;
+  for (uint64_t q = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) / 2;
+       q < m; q += stride) {
+    const uint64_t i = idx[q];
+    if (i >= p.view.n) continue;  // accessKernel skips such indices (both lanes)
+    const uint64_t N = hex_access(p, tab, i);
+    if ((threadIdx.x & 1) == 0) out[q] = amap ? amap[N] : static_cast<T>(N);
+  }
+}
+
+template <typename T, bool Lds>
+__global__ __launch_bounds__(256) void tree_rank_hex_kernel(
+    TreeQueryParams p, const uint8_t* __restrict__ qs, uint64_t m, uint64_t* __restrict__ out) {
+  extern __shared__ uint64_t s_htab[];
+  const QuadTable<Lds> tab = stage_hex_tab<Lds>(p.hex, s_htab);
+  const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x / 2;
+  for (uint64_t q = (blockIdx.x * static_cast<uint64_t>(blockDim.x) + threadIdx.x) / 2;
+       q < m; q += stride) {
+    const uint8_t* e = qs + q * 16;
+    const uint64_t i = *reinterpret_cast<const uint64_t*>(e);
+    const uint64_t c = map_symbol<T>(p, load_sym(e, sizeof(T)));
+    const uint64_t r = i <= p.view.n && c < p.view.sigma ? hex_rank(p, tab, c, i) : 0;
+    if ((threadIdx.x & 1) == 0) out[q] = r;
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Select over quad select lines: the ascent takes two levels a step.  At
 // quad level q, the kk-th code of child 4N + d of node N is the
 // (CD[N][d] + kk)-th d of the level, found in one select line read by the
@@ -998,11 +1120,40 @@ void access_quad(const TreeQueryParams& p, const uint64_t* idx, uint64_t m, void
   }
 }
 
+template <bool Lds>
+void access_hex(const TreeQueryParams& p, const uint64_t* idx, uint64_t m, void* out,
+                hipStream_t s) {
+  const dim3 g(query_grid(2 * m)), b(256);
+  const size_t sh = Lds ? p.hex.ntab * sizeof(uint64_t) : 0;
+  switch (p.esize) {
+    case 1: hipLaunchKernelGGL((tree_access_hex_kernel<uint8_t, Lds>), g, b, sh, s, p, idx, m, static_cast<uint8_t*>(out)); break;
+    case 2: hipLaunchKernelGGL((tree_access_hex_kernel<uint16_t, Lds>), g, b, sh, s, p, idx, m, static_cast<uint16_t*>(out)); break;
+    case 4: hipLaunchKernelGGL((tree_access_hex_kernel<uint32_t, Lds>), g, b, sh, s, p, idx, m, static_cast<uint32_t*>(out)); break;
+    default: hipLaunchKernelGGL((tree_access_hex_kernel<uint64_t, Lds>), g, b, sh, s, p, idx, m, static_cast<uint64_t*>(out)); break;
+  }
+}
+
+template <bool Lds>
+void rank_hex(const TreeQueryParams& p, const uint8_t* qq, uint64_t m, uint64_t* out,
+              hipStream_t s) {
+  const dim3 g(query_grid(2 * m)), b(256);
+  const size_t sh = Lds ? p.hex.ntab * sizeof(uint64_t) : 0;
+  switch (p.esize) {
+    case 1: hipLaunchKernelGGL((tree_rank_hex_kernel<uint8_t, Lds>), g, b, sh, s, p, qq, m, out); break;
+    case 2: hipLaunchKernelGGL((tree_rank_hex_kernel<uint16_t, Lds>), g, b, sh, s, p, qq, m, out); break;
+    case 4: hipLaunchKernelGGL((tree_rank_hex_kernel<uint32_t, Lds>), g, b, sh, s, p, qq, m, out); break;
+    default: hipLaunchKernelGGL((tree_rank_hex_kernel<uint64_t, Lds>), g, b, sh, s, p, qq, m, out); break;
+  }
+}
+
 void launch_tree_access(const TreeQueryParams& p, const uint64_t* idx,
                         uint64_t m, void* out, hipStream_t s) {
   if (m == 0) return;
   ProfScope ps(K_WT_ACCESS, s);
-  if (p.quad.levels) {
+  if (p.hex.levels) {
+    if (p.hex.ntab <= kHexLdsTab) access_hex<true>(p, idx, m, out, s);
+    else access_hex<false>(p, idx, m, out, s);
+  } else if (p.quad.levels) {
     if (p.quad.nbase <= kQuadLdsBase) access_quad<true>(p, idx, m, out, s);
     else access_quad<false>(p, idx, m, out, s);
   } else if (pair_lanes(p)) access_g<2>(p, idx, m, out, s);
@@ -1028,7 +1179,10 @@ void launch_tree_rank(const TreeQueryParams& p, const void* q, uint64_t m,
   if (m == 0) return;
   ProfScope ps(K_WT_RANK, s);
   const uint8_t* qq = static_cast<const uint8_t*>(q);
-  if (p.quad.levels) {
+  if (p.hex.levels) {
+    if (p.hex.ntab <= kHexLdsTab) rank_hex<true>(p, qq, m, out, s);
+    else rank_hex<false>(p, qq, m, out, s);
+  } else if (p.quad.levels) {
     if (p.quad.nbase <= kQuadLdsBase) rank_quad<true>(p, qq, m, out, s);
     else rank_quad<false>(p, qq, m, out, s);
   } else if (pair_lanes(p)) rank_g<2>(p, qq, m, out, s);

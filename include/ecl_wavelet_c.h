@@ -346,8 +346,9 @@ ecl_status ecl_rank_select_build_select_lines(ecl_rank_select rs);
  * codes' 2-bit digits in the order of level 2q, in 128-byte lines of 448
  * digits headed by the counts of digits 0, 1 and 2 before the line.  Access
  * and rank then read one memory line per two levels (the tree's host and
- * device access/rank entry points use them; select, mixed batches and the
- * device API keep the binary levels).  Costs 128/112 of the bits (about one
+ * device access/rank entry points use them, and mixed batches once the quad
+ * select lines exist too; the device API keeps the binary levels).  Costs
+ * 128/112 of the bits (about one
  * byte per symbol at sigma = 256); results do not change.  Other trees get
  * ECL_ERROR_INVALID_ARGUMENT.  A second call keeps the existing lines; build
  * them while no query on the tree is in flight. */
@@ -357,11 +358,23 @@ ecl_status ecl_wavelet_tree_build_quad_lines(ecl_wavelet_tree wt);
  * level and digit, a 128-byte line every r occurrences (r from the digit's
  * density, so that 448 digits hold the next r), the position of the first
  * and the 448 digits from there.  The tree's select entry points then read
- * one memory line per two levels (an occurrence past its line's digits takes
- * the two binary levels' select).  Costs about 1.5 bytes per symbol and quad
+ * one memory line per two levels (an occurrence past its line's digits is
+ * found by a walk over the quad lines).  Costs about 1.5 bytes per symbol and quad
  * level (36 GB for 6e9 symbols at sigma = 256); results do not change.
  * Build them while no query on the tree is in flight. */
 ecl_status ecl_wavelet_tree_build_quad_select_lines(ecl_wavelet_tree wt);
+/* Hex lines (no counterpart in the reference; opt-in, for trees over a
+ * power-of-two alphabet whose level count is a multiple of 4, at most 16:
+ * sigma = 16, 256, 4096, 65536; built from the quad lines, which this
+ * builds first when absent): for every four levels (4h .. 4h + 3), the
+ * codes' 4-bit digits in the order of level 4h, in 128-byte lines of 128
+ * digits with the counts of the 16 digits before the line.  Access and rank
+ * then read one memory line per four levels (the tree's host and device
+ * access/rank entry points use them; select and mixed batches keep the quad
+ * layouts).  Costs about one byte per symbol and hex level; results do not
+ * change.  Other trees get ECL_ERROR_INVALID_ARGUMENT.  A second call keeps
+ * the existing lines; build them while no query on the tree is in flight. */
+ecl_status ecl_wavelet_tree_build_hex_lines(ecl_wavelet_tree wt);
 /* Persistence (no counterpart in the reference, which rebuilds every tree
  * from its text; for warm starts of servers).  The file holds a header, the
  * alphabet, the symbol counts and every level's bits; loading checks them

@@ -712,7 +712,7 @@ def test_mixed_batch_query_lines(gpu, m, text_kind):
     rank on two lanes each) answers like the oracle over the directory, then
     with the query lines and hints, then with the select lines too, then
     with the quad lines and quad select lines (every query on a pair of
-    lanes), for odd batch sizes and for balanced, one-sided and kind > 2
+    lanes), and with hex lines beside them, for odd batch sizes and for balanced, one-sided and kind > 2
     (read as select) mixes; on a skewed text the select lines overflow and
     the mixed kernel's select goes from the line to the hint and to the
     directory search, and rare digits overflow their quad select lines."""
@@ -744,13 +744,15 @@ def test_mixed_batch_query_lines(gpu, m, text_kind):
         exp[k2] = O.scan_select_batch(text, a[k2], s[k2])
         d_q, d_out = DeviceBuffer(m * 16), DeviceBuffer(m * 8)
         d_q.upload(q)
-        for stage in ("directory", "query lines", "select lines", "quad lines"):
+        for stage in ("directory", "query lines", "select lines", "quad lines", "hex lines"):
             if stage == "query lines":
                 wt.buildQueryLines()
             elif stage == "select lines":
                 wt.buildSelectLines()
             elif stage == "quad lines":
                 wt.buildQuadSelectLines()  # with the quad lines: the pair-lane mixed kernel
+            elif stage == "hex lines":
+                wt.buildHexLines()  # mixed batches keep the quad layouts
             wt.queryBatchDevice(d_q, m, d_out)
             assert np.array_equal(d_out.download(np.uint64, m), exp), (mix, stage)
 
@@ -774,6 +776,13 @@ def _quad_text(case, rng):
         return rng.integers(0, 65536, 2_000_011).astype(np.uint16), np.arange(65536, dtype=np.uint16)
     if case == "u8_256_n448k":
         return rng.integers(0, 256, 448 * 1000).astype(np.uint8), np.arange(256, dtype=np.uint8)
+    if case == "u8_256_n128k":
+        return rng.integers(0, 256, 128 * 7001).astype(np.uint8), np.arange(256, dtype=np.uint8)
+    if case == "u16_4096":
+        return rng.integers(0, 4096, 1_500_001).astype(np.uint16), np.arange(4096, dtype=np.uint16)
+    if case == "u16_65536_skewed":
+        text = np.minimum(rng.zipf(1.1, 1_000_001) - 1, 65535).astype(np.uint16)
+        return text, np.arange(65536, dtype=np.uint16)
     sigma = int(case.split("_")[1])
     return rng.integers(0, sigma, 1_234_567).astype(np.uint8), np.arange(sigma, dtype=np.uint8)
 
@@ -811,8 +820,8 @@ def test_quad_lines_same_answers(gpu, case):
 @pytest.mark.parametrize("case", QUAD_CASES)
 def test_quad_select_lines_same_answers(gpu, case):
     """Select over the quad select lines (a line every r occurrences of each
-    digit of each quad level; an occurrence past a line's 448 digits takes
-    the two binary levels' select) equals the text's occurrence positions:
+    digit of each quad level; an occurrence past a line's 448 digits is found
+    by a walk over the quad lines) equals the text's occurrence positions:
     random queries, and every occurrence of the rarest and the most frequent
     symbol and of a random one (each line boundary, both ends), plus
     count + 1 (n).  The skewed text has digits rare enough that their lines
@@ -837,6 +846,62 @@ def test_quad_select_lines_same_answers(gpu, case):
             k, exp = k[:-1], exp[:-1]
         cs = np.full(len(k), c, dtype=alpha.dtype)
         assert np.array_equal(wt.select(wt.make_queries(k, cs)), exp), int(c)
+
+
+HEX_CASES = ["u8_16", "u8_256", "u8_256_skewed", "u16_16_alpha", "u16_4096", "u16_65536",
+             "u16_65536_skewed", "u8_256_n128k"]
+
+
+@pytest.mark.parametrize("case", HEX_CASES)
+def test_hex_lines_same_answers(gpu, case):
+    """Access and rank over the hex lines (4-bit digits of four levels per
+    128-byte line of 128, built from the quad lines) equal the direct scans
+    of the text: random queries, every line boundary (128 j - 1, 128 j,
+    128 j + 1), every node boundary (the C array entries), positions 0,
+    n - 1 and n, and a spread of symbols including absent ones.  Select,
+    which keeps its own layouts, stays exact beside them.  The
+    alphabets of 2^16 symbols put many node starts inside the 32-position
+    words that the build handles together."""
+    from ecl_wavelet_amd import WaveletTree
+    rng = np.random.default_rng(sum(map(ord, case)) + 7)
+    text, alpha = _quad_text(case, rng)
+    n = len(text)
+    wt = WaveletTree(text, alpha)
+    wt.buildHexLines()  # builds the quad lines too
+    wt.buildHexLines()  # a second call keeps the lines
+    _check_queries(wt, text, None, nq=4000, seed=len(case) + 2)
+    lb = np.arange(0, n + 128, 128, dtype=np.int64)
+    pos = np.concatenate([lb - 1, lb, lb + 1, [0, n - 1], wt.getCounts().astype(np.int64)])
+    pos = np.unique(pos[(pos >= 0) & (pos < n)]).astype(np.uint64)
+    assert np.array_equal(wt.access(pos), text[pos])
+    ri = np.unique(np.concatenate([pos, [n]])).astype(np.uint64)
+    if len(ri) > 60_000:
+        ri = np.unique(np.concatenate([ri[rng.integers(0, len(ri), 60_000)], [0, n]])).astype(np.uint64)
+    syms = alpha[rng.integers(0, len(alpha), 6)]
+    for c in np.unique(np.concatenate([syms, alpha[:1], alpha[-1:]])):
+        cs = np.full(len(ri), c, dtype=alpha.dtype)
+        exp = O.scan_rank_batch(text, ri, cs.astype(np.uint64))
+        assert np.array_equal(wt.rank(wt.make_queries(ri, cs)), exp), int(c)
+    # with the quad select lines as well: select over them, access and rank
+    # still over the hex lines
+    wt.buildQuadSelectLines()
+    _check_queries(wt, text, None, nq=2000, seed=len(case) + 3)
+
+
+@pytest.mark.parametrize("sigma", [4, 8, 64, 20])
+def test_hex_lines_reject_other_trees(gpu, sigma):
+    """Alphabets that are not a power of two, or whose level count is not a
+    multiple of 4, get no hex lines, and their queries keep working."""
+    from ecl_wavelet_amd import WaveletTree
+    from ecl_wavelet_amd._lib import EclError, ECL_ERROR_INVALID_ARGUMENT
+    rng = np.random.default_rng(sigma + 100)
+    text = rng.integers(0, sigma, 100_003).astype(np.uint8)
+    wt = WaveletTree(text, np.arange(sigma, dtype=np.uint8))
+    with pytest.raises(EclError) as e:
+        wt.buildHexLines()
+    assert e.value.status == ECL_ERROR_INVALID_ARGUMENT
+    del e
+    _check_queries(wt, text, None, nq=500, seed=sigma)
 
 
 @pytest.mark.parametrize("sigma", [2, 8, 20, 128])
