@@ -1162,11 +1162,42 @@ __device__ __forceinline__ uint64_t quad_word(const uint4* ql, uint64_t g) {
   const uint64_t k = g - line * 14;
   return detail::gld(reinterpret_cast<const uint64_t*>(ql), line * 16 + 2 + k);
 }
-// the d's before position e of a quad level, for each d (one lane)
+// the d's before position e of a quad level, for each d (one lane): the
+// header and words 0..7 load together, words 8..13 only when needed
+__device__ __forceinline__ void quad_count_words(uint4 v, uint32_t k0, uint32_t kw, uint64_t pm,
+                                                 uint32_t (&n)[4]) {
+#pragma unroll
+  for (uint32_t j = 0; j < 2; ++j) {
+    const uint32_t k = k0 + j;
+    const uint64_t w = j ? ((uint64_t)v.z | ((uint64_t)v.w << 32))
+                         : ((uint64_t)v.x | ((uint64_t)v.y << 32));
+    const uint64_t m = k < kw ? 0x5555555555555555ull : (k == kw ? pm : 0ull);
+    const uint64_t lo = w & m, hi = (w >> 1) & m;
+    n[1] += static_cast<uint32_t>(__popcll(lo & ~hi));
+    n[2] += static_cast<uint32_t>(__popcll(hi & ~lo));
+    n[3] += static_cast<uint32_t>(__popcll(lo & hi));
+    n[0] += static_cast<uint32_t>(__popcll(~(lo | hi) & m));
+  }
+}
 __device__ __forceinline__ void quad_counts(const uint4* ql, uint64_t e, uint64_t (&c)[4]) {
   const uint64_t line = __umul64hi(e >> 6, 0x2492492492492493ull);  // e / 448
   const uint32_t o = static_cast<uint32_t>(e - line * kQuadDigits);
-  const uint4 hd = detail::gld(ql, line * 8);
+  const uint4* lp = ql + line * 8;
+  const uint32_t kw = o >> 5, sh = 2 * (o & 31);
+  const uint64_t pm = (sh ? ~0ull >> (64 - sh) : 0ull) & 0x5555555555555555ull;
+  const uint4 hd = detail::gld(lp, 0), v1 = detail::gld(lp, 1), v2 = detail::gld(lp, 2),
+              v3 = detail::gld(lp, 3), v4 = detail::gld(lp, 4);
+  uint32_t n[4] = {0, 0, 0, 0};
+  quad_count_words(v1, 0, kw, pm, n);
+  quad_count_words(v2, 2, kw, pm, n);
+  quad_count_words(v3, 4, kw, pm, n);
+  quad_count_words(v4, 6, kw, pm, n);
+  if (kw >= 8) {
+    const uint4 v5 = detail::gld(lp, 5), v6 = detail::gld(lp, 6), v7 = detail::gld(lp, 7);
+    quad_count_words(v5, 8, kw, pm, n);
+    quad_count_words(v6, 10, kw, pm, n);
+    quad_count_words(v7, 12, kw, pm, n);
+  }
   const uint64_t h0 = (uint64_t)hd.x | ((uint64_t)hd.y << 32);
   const uint64_t h1 = (uint64_t)hd.z | ((uint64_t)hd.w << 32);
   const uint64_t m40 = (1ull << 40) - 1;
@@ -1174,18 +1205,10 @@ __device__ __forceinline__ void quad_counts(const uint4* ql, uint64_t e, uint64_
   c[1] = (h0 >> 40) | ((h1 & 0xFFFFull) << 24);
   c[2] = (h1 >> 16) & m40;
   c[3] = line * kQuadDigits - c[0] - c[1] - c[2];
-  const uint32_t kw = o >> 5, sh = 2 * (o & 31);
-  const uint64_t* w = reinterpret_cast<const uint64_t*>(ql) + line * 16 + 2;
-  for (uint32_t k = 0; k <= kw; ++k) {
-    const uint64_t m = k < kw ? ~0ull : (sh ? ~0ull >> (64 - sh) : 0ull);
-    if (m == 0) break;
-    const uint64_t x = detail::gld(w, k);
-    const uint64_t lo = x & m & 0x5555555555555555ull, hi = (x >> 1) & m & 0x5555555555555555ull;
-    c[1] += static_cast<uint64_t>(__popcll(lo & ~hi));
-    c[2] += static_cast<uint64_t>(__popcll(hi & ~lo));
-    c[3] += static_cast<uint64_t>(__popcll(lo & hi));
-    c[0] += static_cast<uint64_t>(__popcll(~(lo | hi) & m & 0x5555555555555555ull));
-  }
+  c[0] += n[0];
+  c[1] += n[1];
+  c[2] += n[2];
+  c[3] += n[3];
 }
 // 32 digits of a quad level from position x on (nwords: the level's words)
 __device__ __forceinline__ uint64_t quad_window(const uint4* ql, uint64_t nwords, uint64_t x) {
@@ -1197,7 +1220,11 @@ __device__ __forceinline__ uint64_t quad_window(const uint4* ql, uint64_t nwords
   return (a >> sh) | (b << (64 - sh));
 }
 
-__global__ __launch_bounds__(256) void hex_lines_kernel(HexBuildParams hp) {
+// Occupancy target: 8 waves per SIMD (64 VGPRs, no spills; the compiler's
+// choice is 91 and 5 waves): 20.5 vs 21.7 ms for the two hex levels of
+// n = 6e9, sigma = 256 (23.1 ms with the quad counts loaded word by word).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void hex_lines_kernel(
+    HexBuildParams hp) {
   const uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
   const uint64_t l = t >> 2;  // hex line
   const uint32_t q = static_cast<uint32_t>(t & 3);

@@ -760,16 +760,19 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void t
 // waves per SIMD (96 VGPRs, no spills), which the kernel now reaches by
 // itself; with the directory selects inlined it had taken 121 (4 waves), and
 // 5 waves were worth 3.52 vs 3.66 ms per 3e7 at n = 6e9, sigma = 256.  6
-// waves spill 10 VGPRs and are no faster (3.53 ms).
-template <bool Lds>
+// waves spill 10 VGPRs and are no faster (3.53 ms).  Hex: access and rank
+// go over the hex lines instead of the quad lines.
+template <bool Lds, bool Hex>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void tree_mixed_quad_kernel(
     TreeQueryParams p, const ecl_query* __restrict__ qs, uint64_t m,
     uint64_t* __restrict__ out, uint32_t chunk) {
   extern __shared__ uint64_t s_qmix[];
   __shared__ KindLists lists;
-  // Lds: both tables fit (nbase <= kQuadLdsBase and ntab <= kQuadLdsTab)
-  const QuadTable<Lds> base = stage_quad_base<Lds>(p.quad, s_qmix);
-  const QuadTable<Lds> tab = stage_quad_tab<Lds>(p.quad, s_qmix + (Lds ? p.quad.nbase : 0));
+  // Lds: the tables used fit (the quad node table, and the quad bases or,
+  // with Hex, the hex table, each of at most 4096 entries)
+  const QuadTable<Lds> tab = stage_quad_tab<Lds>(p.quad, s_qmix);
+  const QuadTable<Lds> base = Hex ? stage_hex_tab<Lds>(p.hex, s_qmix + (Lds ? p.quad.ntab : 0))
+                                  : stage_quad_base<Lds>(p.quad, s_qmix + (Lds ? p.quad.ntab : 0));
   const uint32_t pair = threadIdx.x >> 1, npairs = blockDim.x >> 1;
   for (uint64_t b0 = static_cast<uint64_t>(blockIdx.x) * chunk; b0 < m;
        b0 += static_cast<uint64_t>(gridDim.x) * chunk) {
@@ -785,9 +788,11 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(5))) void t
       const ecl_query q = cq[j];
       uint64_t r;
       if (k == 0) {
-        r = q.arg < p.view.n ? quad_access(p, base, q.arg) : 0;
+        r = q.arg < p.view.n ? (Hex ? hex_access(p, base, q.arg) : quad_access(p, base, q.arg)) : 0;
       } else if (k == 1) {
-        r = q.arg <= p.view.n && q.symbol < p.view.sigma ? quad_rank(p, base, q.symbol, q.arg) : 0;
+        r = q.arg <= p.view.n && q.symbol < p.view.sigma
+                ? (Hex ? hex_rank(p, base, q.symbol, q.arg) : quad_rank(p, base, q.symbol, q.arg))
+                : 0;
       } else {
         r = quad_select(p, tab, q.symbol, q.arg);
       }
@@ -1235,13 +1240,23 @@ void launch_tree_mixed(const TreeQueryParams& p, const ecl_query* q,
   ProfScope ps(K_WT_MIXED, s);
   if (p.quad.levels && p.quad.has_sel) {
     const uint32_t g = query_grid(2 * m);
-    if (p.quad.nbase <= kQuadLdsBase && p.quad.ntab <= kQuadLdsTab) {
+    const uint32_t chunk = kind_chunk(m, g, 256);
+    if (p.hex.levels) {
+      if (p.hex.ntab <= kHexLdsTab && p.quad.ntab <= kQuadLdsTab) {
+        const size_t sh = (p.hex.ntab + p.quad.ntab) * sizeof(uint64_t);
+        hipLaunchKernelGGL((tree_mixed_quad_kernel<true, true>), dim3(g), dim3(256), sh, s, p, q, m,
+                           out, chunk);
+      } else {
+        hipLaunchKernelGGL((tree_mixed_quad_kernel<false, true>), dim3(g), dim3(256), 0, s, p, q, m,
+                           out, chunk);
+      }
+    } else if (p.quad.nbase <= kQuadLdsBase && p.quad.ntab <= kQuadLdsTab) {
       const size_t sh = (p.quad.nbase + p.quad.ntab) * sizeof(uint64_t);
-      hipLaunchKernelGGL(tree_mixed_quad_kernel<true>, dim3(g), dim3(256), sh, s, p, q, m, out,
-                         kind_chunk(m, g, 256));
+      hipLaunchKernelGGL((tree_mixed_quad_kernel<true, false>), dim3(g), dim3(256), sh, s, p, q, m,
+                         out, chunk);
     } else {
-      hipLaunchKernelGGL(tree_mixed_quad_kernel<false>, dim3(g), dim3(256), 0, s, p, q, m, out,
-                         kind_chunk(m, g, 256));
+      hipLaunchKernelGGL((tree_mixed_quad_kernel<false, false>), dim3(g), dim3(256), 0, s, p, q, m,
+                         out, chunk);
     }
   } else {
     const uint32_t g = query_grid(m);

@@ -8,8 +8,9 @@ grid-stride), over the reference-shaped directory, then with the query lines
 and select hints, then with the select lines, then with the quad lines (which
 access and rank then read: all three alphabets are powers of two with an even
 number of levels), then with the quad select lines (which select then
-reads); the mixed-kind kernel runs the same 3 * 10^7 queries in one launch in
-every layout.
+reads), then with the hex lines (which access and rank then read, where the
+level count is a multiple of 4: sigma = 256 and 2^16); the mixed-kind kernel
+runs the same 3 * 10^7 queries in one launch in every layout.
 
 Every answer is checked, over the whole launch:
   * access(i) against the text copied back from the device;
@@ -37,7 +38,7 @@ CASES = {
     "u8_sigma4": dict(n=6_000_000_000, sigma=4, dtype=np.uint8, seed=5, zipf=None),
     "u16_zipf": dict(n=2_000_000_000, sigma=1 << 16, dtype=np.uint16, seed=8, zipf=1.1),
 }
-LAYOUTS = ["plain", "query_lines", "select_lines", "quad_lines", "quad_select"]
+LAYOUTS = ["plain", "query_lines", "select_lines", "quad_lines", "quad_select", "hex_lines"]
 
 
 class Full:
@@ -108,14 +109,18 @@ def full(request, gpu):
 
 
 def _ensure(F, layout):
-    if layout in ("query_lines", "select_lines", "quad_lines", "quad_select"):
+    if layout in ("query_lines", "select_lines", "quad_lines", "quad_select", "hex_lines"):
         F.wt.buildQueryLines()
-    if layout in ("select_lines", "quad_lines", "quad_select"):
+    if layout in ("select_lines", "quad_lines", "quad_select", "hex_lines"):
         F.wt.buildSelectLines()
-    if layout in ("quad_lines", "quad_select"):
+    if layout in ("quad_lines", "quad_select", "hex_lines"):
         F.wt.buildQuadLines()
-    if layout == "quad_select":
+    if layout in ("quad_select", "hex_lines"):
         F.wt.buildQuadSelectLines()
+    if layout == "hex_lines":
+        if F.wt.getNumLevels() % 4 != 0:
+            pytest.skip("hex lines need a level count that is a multiple of 4")
+        F.wt.buildHexLines()
 
 
 def _first_bad(got, exp):
