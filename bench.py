@@ -835,7 +835,7 @@ def main():
     use_quad = sigma >= 4 and (sigma & (sigma - 1)) == 0 and (sigma.bit_length() - 1) % 2 == 0
     # hex lines: ... whose level count is a multiple of 4
     use_hex = use_quad and (sigma.bit_length() - 1) % 4 == 0
-    hex_ms, hex_build_ms = [], []
+    hex_ms, hex_build_ms, hex_mixed_ms = [], [], []
     hex_kind_ms = {"access": [], "rank": []}
     plain_ms = {"access": [], "rank": [], "select": [], "mixed": []}
     lines_bytes = sel_bytes = qsel_bytes = 0
@@ -912,6 +912,11 @@ def main():
             timer.start()
             wt.rankDevice(d_rq, Q, d_out, flags=1)
             ht["rank"] = timer.stop()
+            # the mixed batch: access and rank over the hex lines, select over
+            # the quad select lines
+            timer.start()
+            wt.queryBatchDevice(d_mq, 3 * Q, d_out)
+            hmx = timer.stop()
         log(f"step {step}{'' if timed else ' (warmup)'}: build {b:.1f} ms, "
             f"plain access {plain['access']:.2f} ms, rank {plain['rank']:.2f} ms, "
             f"select {plain['select']:.2f} ms, mixed {plain['mixed']:.2f} ms; lines {lb:.2f} ms, "
@@ -922,7 +927,7 @@ def main():
                f"rank {qt['rank']:.2f} ms; quad select lines {qsb:.2f} ms, "
                f"select {qt['select']:.2f} ms, mixed {qmx:.2f} ms" if use_quad else "")
             + (f"; hex lines {hb:.2f} ms, access {ht['access']:.2f} ms, "
-               f"rank {ht['rank']:.2f} ms" if use_hex else ""))
+               f"rank {ht['rank']:.2f} ms, mixed {hmx:.2f} ms" if use_hex else ""))
         if timed:
             build_ms.append(b)
             lines_ms.append(lb)
@@ -942,8 +947,9 @@ def main():
                 for k, v in qt.items():
                     quad_kind_ms[k].append(v)
             if use_hex:
-                hex_ms.append(hb + ht["access"] + ht["rank"])
+                hex_ms.append(hb + ht["access"] + ht["rank"] + hmx)
                 hex_build_ms.append(hb)
+                hex_mixed_ms.append(hmx)
                 for k, v in ht.items():
                     hex_kind_ms[k].append(v)
 
@@ -1040,8 +1046,9 @@ def main():
                          f"{Q} each of access/rank/select + one mixed batch of {3 * Q} "
                          f"over the plain directory, the query lines and select "
                          f"lines, the same queries over them, then the quad lines and "
-                         f"quad select lines and the three kinds over them, then the hex "
-                         f"lines and access and rank over them"),
+                         f"quad select lines and the three kinds and the mixed batch over "
+                         f"them, then the hex lines and access, rank and the mixed batch "
+                         f"over them"),
             "n": n, "sigma": sigma, "queries_per_kind": Q, "levels": L,
         },
         "build_ms": build,
@@ -1064,6 +1071,7 @@ def main():
         "quad_lines_layout": ({
             "access_qps": Q / (np.mean(quad_kind_ms["access"]) / 1e3),
             "rank_qps": Q / (np.mean(quad_kind_ms["rank"]) / 1e3),
+            "mixed_batch_qps": 3 * Q / (np.mean(quad_mixed_ms) / 1e3),
         } if use_quad else None),
         "quad_lines_build_ms": float(np.median(quad_build_ms)) if use_quad else None,
         "quad_lines_bytes": (L // 2) * (n // 448 + 1) * 128 if use_quad else None,
@@ -1076,7 +1084,8 @@ def main():
             "per_kind_qps": lines_value,
             "mixed_batch_qps": 3 * Q / (np.mean(mixed_ms) / 1e3),
         },
-        "mixed_batch_qps": 3 * Q / (np.mean(quad_mixed_ms if use_quad else mixed_ms) / 1e3),
+        "mixed_batch_qps": 3 * Q / (np.mean(hex_mixed_ms if use_hex else
+                                            quad_mixed_ms if use_quad else mixed_ms) / 1e3),
         "small_call_rank_p50_ms": float(np.percentile(lat, 50)),
         "small_call_rank_p99_ms": float(np.percentile(lat, 99)),
         "rank_kernel_ms": rank_k[1] / max(rank_k[0], 1),
