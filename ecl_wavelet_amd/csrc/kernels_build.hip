@@ -1220,12 +1220,9 @@ __device__ __forceinline__ uint64_t quad_window(const uint4* ql, uint64_t nwords
   return (a >> sh) | (b << (64 - sh));
 }
 
-// Occupancy target: 8 waves per SIMD (64 VGPRs, no spills; the compiler's
-// choice is 91 and 5 waves): 20.5 vs 21.7 ms for the two hex levels of
-// n = 6e9, sigma = 256 (23.1 ms with the quad counts loaded word by word).
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void hex_lines_kernel(
-    HexBuildParams hp) {
-  const uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+// One lane's part of a hex line, everything read from global memory: the
+// path of workgroups whose positions hold a node start.  t = 4 l + q.
+__device__ __forceinline__ void hex_lane(const HexBuildParams& hp, uint64_t t) {
   const uint64_t l = t >> 2;  // hex line
   const uint32_t q = static_cast<uint32_t>(t & 3);
   // the four lanes of a line are all present or all absent
@@ -1319,9 +1316,236 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void h
                  static_cast<uint32_t>(hi), static_cast<uint32_t>(hi >> 32));
 }
 
+// The workgroup form: 64 hex lines (8192 positions) per workgroup, one
+// 32-position word per thread.  When the positions lie in one node N, the
+// workgroup stages the quad lines of qa holding them and, per digit d0, the
+// quad lines of qb holding the run its d0's map to (from qbase[4N + d0] +
+// the d0's before the first position on); a block scan of the words' digit
+// counts gives every word its place in each run, and a block scan of the
+// words' hex-digit counts gives every line its header.  Otherwise (a node
+// starts inside) the lanes take hex_lane.
+constexpr uint32_t kHexWgLines = 64;
+constexpr uint32_t kHexWgPos = kHexWgLines * kHexDigits;         // 8192
+constexpr uint32_t kHexQaLines = kHexWgPos / kQuadDigits + 2;   // 20
+constexpr uint32_t kHexQbLines = 32;  // the four runs need at most 26
+
+// exclusive scans over the workgroup (4 waves) of u64 values made of 16-bit
+// fields whose sums stay below 2^16; s_w holds 4 * J u64
+template <int J>
+__device__ __forceinline__ void wg_excl_scan_fields(uint64_t (&v)[J], uint64_t* s_w,
+                                                    uint64_t (&total)[J]) {
+  const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  uint64_t incl[J];
+#pragma unroll
+  for (int j = 0; j < J; ++j) incl[j] = detail::wave_incl_scan_2x32(v[j]);
+  if (lane == 63) {
+#pragma unroll
+    for (int j = 0; j < J; ++j) s_w[J * wave + j] = incl[j];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int j = 0; j < J; ++j) {
+    uint64_t before = 0, tot = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < 4; ++w) {
+      const uint64_t x = s_w[J * w + j];
+      tot += x;
+      before += w < wave ? x : 0ull;
+    }
+    total[j] = tot;
+    v[j] = before + incl[j] - v[j];
+  }
+  __syncthreads();
+}
+
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void hex_lines_kernel(
+    HexBuildParams hp) {
+  __shared__ __align__(16) uint4 s_qa[kHexQaLines * 8];
+  __shared__ __align__(16) uint4 s_qb[kHexQbLines * 8];
+  __shared__ uint64_t s_w[16];
+  __shared__ uint64_t s_info[1];  // staged qb lines
+  __shared__ uint64_t s_R[4], s_rb0[4], s_hx0[16];
+  __shared__ uint32_t s_rl[4];
+  const uint32_t tid = threadIdx.x;
+  const uint64_t L0 = static_cast<uint64_t>(blockIdx.x) * kHexWgLines;
+  const uint64_t P0 = L0 * kHexDigits;
+  const uint64_t n = hp.n;
+  const uint64_t E = P0 + kHexWgPos < n ? P0 + kHexWgPos : n;  // <= P0 only when P0 >= n
+  // node of P0 and of E - 1: the node starts S[1 .. nodes - 1] at or before
+  // them, counted over the workgroup (S is nondecreasing, S[0] = 0)
+  uint32_t na0 = 0, na1 = 0;
+  for (uint64_t k0 = 1; k0 < hp.nodes; k0 += blockDim.x) {
+    const uint64_t k = k0 + tid;
+    const uint64_t sk = k < hp.nodes ? detail::gld(hp.S, k) : ~0ull;
+    na0 += __syncthreads_count(sk <= P0);
+    na1 += __syncthreads_count(E > P0 && sk <= E - 1);
+  }
+  if (P0 >= n || na0 != na1) {  // the whole workgroup
+    hex_lane(hp, static_cast<uint64_t>(blockIdx.x) * blockDim.x + tid);
+    return;
+  }
+  const uint64_t N = na0;
+  // table entries used below, issued ahead
+  const uint64_t qb_d = tid < 4 ? detail::gld(hp.qbase, 4 * N + tid) : 0ull;
+  const uint64_t chd_x = tid < 16 ? detail::gld(hp.chd, 16 * N + tid) : 0ull;
+  const uint64_t qcd_x = tid < 16 ? detail::gld(hp.qcd, 4 * (4 * N + (tid >> 2)) + (tid & 3)) : 0ull;
+  // ---- stage the quad lines of qa holding [P0, E)
+  const uint64_t a0 = __umul64hi(P0 >> 6, 0x2492492492492493ull);
+  const uint64_t a1 = __umul64hi((E - 1) >> 6, 0x2492492492492493ull);
+  const uint32_t na = static_cast<uint32_t>(a1 - a0 + 1);
+  for (uint32_t i = tid; i < na * 8; i += blockDim.x) s_qa[i] = detail::gld(hp.qa, a0 * 8 + i);
+  __syncthreads();
+  // ---- this thread's word and its digit counts (4 x 16-bit fields)
+  const uint64_t P = P0 + 32ull * tid;
+  const uint32_t valid = P < E ? (E - P < 32 ? static_cast<uint32_t>(E - P) : 32u) : 0u;
+  uint64_t w = 0;
+  if (valid) {
+    const uint64_t qi = __umul64hi(P >> 6, 0x2492492492492493ull);
+    w = qs_digit_word(s_qa, static_cast<uint32_t>(qi - a0),
+                      static_cast<uint32_t>(P - qi * kQuadDigits) >> 5);
+  }
+  const uint64_t vm = (valid == 32 ? ~0ull : (1ull << (2 * valid)) - 1) & 0x5555555555555555ull;
+  uint64_t pre[1], ctot[1];
+  {
+    const uint64_t lo = w & vm, hi = (w >> 1) & vm;
+    pre[0] = static_cast<uint64_t>(__popcll(~(lo | hi) & vm)) |
+             (static_cast<uint64_t>(__popcll(lo & ~hi)) << 16) |
+             (static_cast<uint64_t>(__popcll(hi & ~lo)) << 32) |
+             (static_cast<uint64_t>(__popcll(lo & hi)) << 48);
+  }
+  const uint64_t cword = pre[0];           // the word's own d's
+  wg_excl_scan_fields<1>(pre, s_w, ctot);  // pre: the d's in [P0, P)
+  // ---- the d's before P0 and the runs R_d of qb (threads 0..3)
+  if (tid < 4) {
+    const uint32_t d = tid;
+    const uint4 h = s_qa[0];
+    const uint64_t h0 = (uint64_t)h.x | ((uint64_t)h.y << 32);
+    const uint64_t h1 = (uint64_t)h.z | ((uint64_t)h.w << 32);
+    const uint64_t m40 = (1ull << 40) - 1;
+    const uint64_t c0 = h0 & m40, c1 = (h0 >> 40) | ((h1 & 0xFFFFull) << 24), c2 = (h1 >> 16) & m40;
+    uint64_t g = d == 0 ? c0 : d == 1 ? c1 : d == 2 ? c2 : a0 * kQuadDigits - c0 - c1 - c2;
+    const uint32_t kw = static_cast<uint32_t>(P0 - a0 * kQuadDigits) >> 5;
+    for (uint32_t k = 0; k < kw; ++k) g += static_cast<uint64_t>(__popcll(quad_match(qs_digit_word(s_qa, 0, k), d)));
+    s_R[d] = qb_d + g;
+  }
+  __syncthreads();
+  if (tid == 0) {
+    uint32_t at = 0;
+    for (uint32_t d = 0; d < 4; ++d) {
+      const uint64_t R = s_R[d], C = (ctot[0] >> (16 * d)) & 0xFFFFull;
+      const uint64_t b0 = __umul64hi(R >> 6, 0x2492492492492493ull);
+      uint64_t b1 = __umul64hi((R + C + 31) >> 6, 0x2492492492492493ull);
+      if (b1 > hp.nql - 1) b1 = hp.nql - 1;
+      s_rb0[d] = b0;
+      s_rl[d] = at;
+      at += static_cast<uint32_t>(b1 - b0 + 1);
+    }
+    s_info[0] = at;
+  }
+  __syncthreads();
+  const uint32_t nst = static_cast<uint32_t>(s_info[0]);
+  if (nst > kHexQbLines) {  // not reached (at most 26); the whole workgroup
+    hex_lane(hp, static_cast<uint64_t>(blockIdx.x) * blockDim.x + tid);
+    return;
+  }
+  for (uint32_t i = tid; i < nst * 8; i += blockDim.x) {
+    const uint32_t li = i >> 3;
+    const uint32_t d = li >= s_rl[3] ? 3u : li >= s_rl[2] ? 2u : li >= s_rl[1] ? 1u : 0u;
+    s_qb[i] = detail::gld(hp.qb, (s_rb0[d] + (li - s_rl[d])) * 8 + (i & 7));
+  }
+  __syncthreads();
+  // ---- hex digits 4 x + d1 before P0 (threads 0..15): the d1's of qb before R_d0
+  if (tid < 16) {
+    const uint32_t d0 = tid >> 2, d1 = tid & 3;
+    const uint64_t R = s_R[d0], b0 = s_rb0[d0];
+    const uint32_t li = s_rl[d0];
+    const uint4 h = s_qb[li * 8];
+    const uint64_t h0 = (uint64_t)h.x | ((uint64_t)h.y << 32);
+    const uint64_t h1 = (uint64_t)h.z | ((uint64_t)h.w << 32);
+    const uint64_t m40 = (1ull << 40) - 1;
+    const uint64_t c0 = h0 & m40, c1 = (h0 >> 40) | ((h1 & 0xFFFFull) << 24), c2 = (h1 >> 16) & m40;
+    uint64_t g = d1 == 0 ? c0 : d1 == 1 ? c1 : d1 == 2 ? c2 : b0 * kQuadDigits - c0 - c1 - c2;
+    const uint32_t o = static_cast<uint32_t>(R - b0 * kQuadDigits);
+    const uint32_t kw = o >> 5, sh = 2 * (o & 31);
+    for (uint32_t k = 0; k < kw; ++k) g += static_cast<uint64_t>(__popcll(quad_match(qs_digit_word(s_qb, li, k), d1)));
+    if (sh) g += static_cast<uint64_t>(__popcll(quad_match(qs_digit_word(s_qb, li, kw), d1) & (~0ull >> (64 - sh))));
+    s_hx0[tid] = chd_x + g - qcd_x;
+  }
+  __syncthreads();
+  // ---- the word's 32 hex digits: its d0's take the next digits of run d0
+  auto window = [&](uint32_t d) -> uint64_t {
+    const uint64_t x = s_R[d] + ((pre[0] >> (16 * d)) & 0xFFFFull);
+    const uint64_t xl = __umul64hi(x >> 6, 0x2492492492492493ull);
+    const uint32_t o = static_cast<uint32_t>(x - xl * kQuadDigits);
+    const uint32_t k = o >> 5, sh = 2 * (o & 31);
+    const uint32_t li = s_rl[d] + static_cast<uint32_t>(xl - s_rb0[d]);
+    const uint64_t a = qs_digit_word(s_qb, li, k);
+    if (sh == 0) return a;
+    // the next word (staged whenever one of its digits is used)
+    const uint64_t b = k < 13 ? qs_digit_word(s_qb, li, k + 1) : qs_digit_word(s_qb, li + 1, 0);
+    return (a >> sh) | (b << (64 - sh));
+  };
+  const uint64_t W[4] = {window(0), window(1), window(2), window(3)};
+  // position j's digit: the next unused one of window d0 (k[d0] used so far)
+  uint64_t lo = 0, hi = 0;
+  uint32_t k0 = 0, k1 = 0, k2 = 0, k3 = 0;
+#pragma unroll
+  for (uint32_t j = 0; j < 32; ++j) {
+    if (j < valid) {
+      const uint32_t d0 = static_cast<uint32_t>(w >> (2 * j)) & 3u;
+      const uint64_t wd = d0 == 0 ? W[0] : d0 == 1 ? W[1] : d0 == 2 ? W[2] : W[3];
+      const uint32_t kd = d0 == 0 ? k0 : d0 == 1 ? k1 : d0 == 2 ? k2 : k3;
+      const uint64_t x = 4 * d0 + (static_cast<uint32_t>(wd >> (2 * kd)) & 3u);
+      k0 += d0 == 0;
+      k1 += d0 == 1;
+      k2 += d0 == 2;
+      k3 += d0 == 3;
+      if (j < 16) lo |= x << (4 * j);
+      else hi |= x << (4 * (j - 16));
+    }
+  }
+  // ---- the word's hex-digit counts (field d1 of hh[d0]): the d1's among the
+  // digits its d0's used from window d0; scanned
+  uint64_t hh[4], htot[4];
+#pragma unroll
+  for (uint32_t d0 = 0; d0 < 4; ++d0) {
+    const uint32_t c = static_cast<uint32_t>(cword >> (16 * d0)) & 0xFFFFu;  // <= 32
+    const uint64_t m = c >= 32 ? ~0ull : (1ull << (2 * c)) - 1;
+    uint64_t f = 0;
+#pragma unroll
+    for (uint32_t d1 = 0; d1 < 4; ++d1)
+      f |= static_cast<uint64_t>(__popcll(quad_match(W[d0], d1) & m)) << (16 * d1);
+    hh[d0] = f;
+  }
+  wg_excl_scan_fields<4>(hh, s_w, htot);  // hh: the hex digits in [P0, P)
+  // ---- write: the digits, and the counts of hex digits 4q .. 4q + 3 before
+  // the line (its first word's prefix, from lane 4 (tid / 4))
+  const uint64_t l = L0 + (tid >> 2);
+  const uint32_t q = tid & 3;
+  const uint32_t src = (tid & 63u) & ~3u;
+  const uint64_t f0 = __shfl(hh[0], src, 64), f1 = __shfl(hh[1], src, 64),
+                 f2 = __shfl(hh[2], src, 64), f3 = __shfl(hh[3], src, 64);
+  const uint64_t mine = q == 0 ? f0 : q == 1 ? f1 : q == 2 ? f2 : f3;
+  if (l < hp.nlines) {
+    hp.out[l * 8 + (q < 2 ? 2 + q : 4 + q)] =
+        make_uint4(static_cast<uint32_t>(lo), static_cast<uint32_t>(lo >> 32),
+                   static_cast<uint32_t>(hi), static_cast<uint32_t>(hi >> 32));
+    uint64_t cnt[4];
+#pragma unroll
+    for (uint32_t j = 0; j < 4; ++j) cnt[j] = s_hx0[4 * q + j] + ((mine >> (16 * j)) & 0xFFFFull);
+    hp.out[l * 8 + (q < 2 ? q : 2 + q)] =
+        make_uint4(static_cast<uint32_t>(cnt[0]), static_cast<uint32_t>(cnt[1]),
+                   static_cast<uint32_t>(cnt[2]), static_cast<uint32_t>(cnt[3]));
+    if ((l & ((1ull << kHexSbShift) - 1)) == 0) {
+#pragma unroll
+      for (uint32_t j = 0; j < 4; ++j) hp.sb[(l >> kHexSbShift) * 16 + 4 * q + j] = cnt[j];
+    }
+  }
+}
+
 void launch_hex_lines(const HexBuildParams& hp, hipStream_t s) {
   if (hp.nlines == 0) return;
-  const uint64_t blocks = (hp.nlines * 4 + 255) / 256;
+  const uint64_t blocks = (hp.nlines + kHexWgLines - 1) / kHexWgLines;
   if (blocks > 0x7FFFFFFFull) throw Error(ECL_ERROR_INVALID_ARGUMENT, "bit array too large");
   hipLaunchKernelGGL(hex_lines_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(256), 0, s, hp);
   ECL_HIP(hipGetLastError());
