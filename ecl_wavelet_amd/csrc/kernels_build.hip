@@ -1358,7 +1358,10 @@ __device__ __forceinline__ void wg_excl_scan_fields(uint64_t (&v)[J], uint64_t* 
   __syncthreads();
 }
 
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void hex_lines_kernel(
+// Occupancy target: 7 waves per SIMD (70 VGPRs, no spills; 8 waves spill 7
+// and are no faster, 12.0 vs 12.1 ms for both levels of n = 6e9, sigma =
+// 256; the compiler's own choice is 5 waves).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void hex_lines_kernel(
     HexBuildParams hp) {
   __shared__ __align__(16) uint4 s_qa[kHexQaLines * 8];
   __shared__ __align__(16) uint4 s_qb[kHexQbLines * 8];
@@ -1366,7 +1369,26 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void h
   __shared__ uint64_t s_info[1];  // staged qb lines
   __shared__ uint64_t s_R[4], s_rb0[4], s_hx0[16];
   __shared__ uint32_t s_rl[4];
+  __shared__ uint32_t s_dlut[256];
   const uint32_t tid = threadIdx.x;
+  // deposit table for 4 positions with first digits p (2 bits each): per
+  // position i the index 4 d0_i + r_i of its second digit among the next 4
+  // of each run (r_i = the earlier positions with the same d0), and per run
+  // the digits the 4 positions take (3 bits each, from bit 16)
+  {
+    uint32_t e = 0, cnt[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (uint32_t i = 0; i < 4; ++i) {
+      const uint32_t d0 = (tid >> (2 * i)) & 3u;
+      const uint32_t r = d0 == 0 ? cnt[0] : d0 == 1 ? cnt[1] : d0 == 2 ? cnt[2] : cnt[3];
+      e |= (4 * d0 + r) << (4 * i);
+      cnt[0] += d0 == 0;
+      cnt[1] += d0 == 1;
+      cnt[2] += d0 == 2;
+      cnt[3] += d0 == 3;
+    }
+    s_dlut[tid] = e | (cnt[0] << 16) | (cnt[1] << 19) | (cnt[2] << 22) | (cnt[3] << 25);
+  }
   const uint64_t L0 = static_cast<uint64_t>(blockIdx.x) * kHexWgLines;
   const uint64_t P0 = L0 * kHexDigits;
   const uint64_t n = hp.n;
@@ -1486,23 +1508,34 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void h
     return (a >> sh) | (b << (64 - sh));
   };
   const uint64_t W[4] = {window(0), window(1), window(2), window(3)};
-  // position j's digit: the next unused one of window d0 (k[d0] used so far)
+  // 4 positions at a time: the next 4 digits of each window side by side in
+  // C, each position's digit picked by its table index, then each window
+  // advanced by the digits its positions took.  Positions past the valid
+  // ones take digits too (from the padding) and are masked off.
   uint64_t lo = 0, hi = 0;
-  uint32_t k0 = 0, k1 = 0, k2 = 0, k3 = 0;
+  {
+    uint64_t S0 = W[0], S1 = W[1], S2 = W[2], S3 = W[3];
 #pragma unroll
-  for (uint32_t j = 0; j < 32; ++j) {
-    if (j < valid) {
-      const uint32_t d0 = static_cast<uint32_t>(w >> (2 * j)) & 3u;
-      const uint64_t wd = d0 == 0 ? W[0] : d0 == 1 ? W[1] : d0 == 2 ? W[2] : W[3];
-      const uint32_t kd = d0 == 0 ? k0 : d0 == 1 ? k1 : d0 == 2 ? k2 : k3;
-      const uint64_t x = 4 * d0 + (static_cast<uint32_t>(wd >> (2 * kd)) & 3u);
-      k0 += d0 == 0;
-      k1 += d0 == 1;
-      k2 += d0 == 2;
-      k3 += d0 == 3;
-      if (j < 16) lo |= x << (4 * j);
-      else hi |= x << (4 * (j - 16));
+    for (uint32_t c = 0; c < 8; ++c) {
+      const uint32_t L = s_dlut[static_cast<uint32_t>(w >> (8 * c)) & 0xFFu];
+      const uint32_t C = (static_cast<uint32_t>(S0) & 0xFFu) | ((static_cast<uint32_t>(S1) & 0xFFu) << 8) |
+                         ((static_cast<uint32_t>(S2) & 0xFFu) << 16) |
+                         ((static_cast<uint32_t>(S3) & 0xFFu) << 24);
+      uint32_t out = 0;
+#pragma unroll
+      for (uint32_t i = 0; i < 4; ++i) {
+        const uint32_t idx = (L >> (4 * i)) & 15u;
+        out |= ((idx & 12u) | ((C >> (2 * idx)) & 3u)) << (4 * i);
+      }
+      S0 >>= 2 * ((L >> 16) & 7u);
+      S1 >>= 2 * ((L >> 19) & 7u);
+      S2 >>= 2 * ((L >> 22) & 7u);
+      S3 >>= 2 * ((L >> 25) & 7u);
+      if (c < 4) lo |= static_cast<uint64_t>(out) << (16 * c);
+      else hi |= static_cast<uint64_t>(out) << (16 * (c - 4));
     }
+    lo &= valid >= 16 ? ~0ull : (1ull << (4 * valid)) - 1;
+    hi &= valid >= 32 ? ~0ull : valid > 16 ? (1ull << (4 * (valid - 16))) - 1 : 0ull;
   }
   // ---- the word's hex-digit counts (field d1 of hh[d0]): the d1's among the
   // digits its d0's used from window d0; scanned
