@@ -658,7 +658,9 @@ class Server:
     """Query server with continuous batching (ecl_server_*): resident trees
     admitted by an HBM budget, requests coalesced into device batches.
     query_lines: 0/False none, 1/True query lines and select hints for every
-    admitted tree, 2 also select lines."""
+    admitted tree, 2 also select lines, 3 the quad layouts (quad lines and
+    quad select lines, hex lines where the level count is a multiple of 4)
+    for trees that take them and query and select lines for the others."""
 
     def __init__(self, device=0, hbm_budget=0, max_batch=0, max_inflight=0, max_trees=0,
                  coalesce=True, query_lines=0):

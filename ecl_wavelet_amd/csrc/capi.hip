@@ -2048,6 +2048,14 @@ static TreeQueryParams query_params(const ecl_wavelet_tree_t* wt, uint32_t flags
   return p;
 }
 
+namespace ecl {
+namespace lib {
+TreeQueryParams tree_query_params(ecl_wavelet_tree wt) {
+  return query_params(wt, ECL_SYMBOLS_MINIMAL | ECL_RESULTS_MINIMAL);
+}
+}  // namespace lib
+}  // namespace ecl
+
 static const uint64_t kHostChunk = 1ull << 24;
 
 static void check_query_memory(uint64_t bytes, const char* kind) {

@@ -376,6 +376,16 @@ void launch_segment_query(const TreeView* views, uint64_t num_views, const ecl_q
                           uint64_t m, const uint64_t* seg_start, const uint32_t* seg_tree,
                           uint32_t nseg, uint64_t* out, uint32_t block, uint32_t grid,
                           hipStream_t s);
+// A tree's query parameters for minimal symbols and results (the layouts it
+// carries included); valid while the tree lives.
+TreeQueryParams tree_query_params(ecl_wavelet_tree wt);
+// A served batch over trees with quad select lines (TreeQueryParams per
+// tree slot, beside the views): a pair of lanes per query, 256 threads per
+// workgroup.
+void launch_segment_query_quad(const TreeView* views, const TreeQueryParams* params,
+                               uint64_t num_views, const ecl_query* q, uint64_t m,
+                               const uint64_t* seg_start, const uint32_t* seg_tree, uint32_t nseg,
+                               uint64_t* out, uint32_t grid, hipStream_t s);
 void launch_forest_gen(const TreeView* views, uint64_t num_trees, const uint64_t* qoff,
                        uint64_t seed, uint64_t m, ecl_forest_query* out, hipStream_t s);
 void launch_rs_rank(const LevelView* lv, int value, const uint64_t* idx,

@@ -965,6 +965,60 @@ __global__ void segment_query_kernel(const TreeView* __restrict__ views, uint64_
   }
 }
 
+// A served batch over trees that carry quad lines and quad select lines
+// (and hex lines where built): as segment_query_kernel, but a pair of lanes
+// per query, each query's tree parameters read from global memory, and the
+// descents of the single-tree quad and hex kernels.  A tree without quad
+// select lines takes its binary descents (both lanes of the pair run them).
+__global__ __launch_bounds__(256) void segment_query_quad_kernel(
+    const TreeView* __restrict__ views, const TreeQueryParams* __restrict__ params,
+    uint64_t num_views, const ecl_query* __restrict__ qs, uint64_t m,
+    const uint64_t* __restrict__ seg_start, const uint32_t* __restrict__ seg_tree, uint32_t nseg,
+    uint64_t* __restrict__ out, uint32_t chunk) {
+  __shared__ KindLists lists;
+  const uint32_t pair = threadIdx.x >> 1, npairs = blockDim.x >> 1;
+  for (uint64_t b0 = static_cast<uint64_t>(blockIdx.x) * chunk; b0 < m;
+       b0 += static_cast<uint64_t>(gridDim.x) * chunk) {
+    const uint32_t len = static_cast<uint32_t>(m - b0 < chunk ? m - b0 : chunk);
+    const ecl_query* cq = qs + b0;
+    group_by_kind(lists, len, [&](uint32_t j) { return cq[j].kind; });
+    const uint32_t c0 = lists.cnt[0], c1 = lists.cnt[1], tot = c0 + c1 + lists.cnt[2];
+    for (uint32_t e = pair; e < tot; e += npairs) {
+      uint32_t j, k;
+      if (e < c0) { j = lists.idx[0][e]; k = 0; }
+      else if (e < c0 + c1) { j = lists.idx[1][e - c0]; k = 1; }
+      else { j = lists.idx[2][e - c0 - c1]; k = 2; }
+      const uint64_t q = b0 + j;
+      const ecl_query qq = cq[j];
+      const uint32_t t = seg_tree[segment_of(seg_start, nseg, q)];
+      uint64_t r = 0;
+      if (t < num_views) {
+        const TreeQueryParams& p = params[t];
+        if (p.quad.has_sel) {
+          if (k == 0) {
+            r = qq.arg < p.view.n
+                    ? (p.hex.levels ? hex_access(p, QuadTable<false>{p.hex.tab}, qq.arg)
+                                    : quad_access(p, QuadTable<false>{p.quad.base}, qq.arg))
+                    : 0;
+          } else if (k == 1) {
+            r = qq.arg <= p.view.n && qq.symbol < p.view.sigma
+                    ? (p.hex.levels
+                           ? hex_rank(p, QuadTable<false>{p.hex.tab}, qq.symbol, qq.arg)
+                           : quad_rank(p, QuadTable<false>{p.quad.base}, qq.symbol, qq.arg))
+                    : 0;
+          } else {
+            r = quad_select(p, QuadTable<false>{p.quad.tab}, qq.symbol, qq.arg);
+          }
+        } else {
+          r = forest_one(views, num_views, t, k, qq.arg, qq.symbol);
+        }
+      }
+      if ((threadIdx.x & 1) == 0) out[q] = r;
+    }
+    __syncthreads();
+  }
+}
+
 // Mixed queries for a forest: queries [qoff[i], qoff[i+1]) go to tree i;
 // kind q % 3, fields drawn like tree_gen_kernel's.
 __global__ void forest_gen_kernel(const TreeView* __restrict__ views, uint64_t num_trees,
@@ -1286,6 +1340,18 @@ void launch_segment_query(const TreeView* views, uint64_t num_views, const ecl_q
   check_kind_launch(block, grid);
   hipLaunchKernelGGL(segment_query_kernel, dim3(grid), dim3(block), 0, s, views, num_views, q, m,
                      seg_start, seg_tree, nseg, out, kind_chunk(m, grid, block));
+  ECL_HIP(hipGetLastError());
+}
+
+void launch_segment_query_quad(const TreeView* views, const TreeQueryParams* params,
+                               uint64_t num_views, const ecl_query* q, uint64_t m,
+                               const uint64_t* seg_start, const uint32_t* seg_tree, uint32_t nseg,
+                               uint64_t* out, uint32_t grid, hipStream_t s) {
+  if (m == 0) return;
+  ProfScope ps(K_FOREST_QUERY, s);
+  check_kind_launch(256, grid);
+  hipLaunchKernelGGL(segment_query_quad_kernel, dim3(grid), dim3(256), 0, s, views, params,
+                     num_views, q, m, seg_start, seg_tree, nseg, out, kind_chunk(m, grid, 256));
   ECL_HIP(hipGetLastError());
 }
 

@@ -81,6 +81,7 @@ struct ecl_server_t {
   int device = 0;
   ecl_server_config cfg{};
   DevBuf d_views;              // TreeView per tree slot
+  DevBuf d_params;             // TreeQueryParams per tree slot (query_lines 3)
   std::vector<TreeEntry> trees;
   uint64_t hbm_used = 0;
   std::vector<std::unique_ptr<Slot>> slots;
@@ -170,9 +171,15 @@ void ecl_server_t::scheduler_loop() {
                              sl->stream));
       ECL_HIP(hipMemcpyAsync(sl->d_tree.ptr, sl->pin_tree, nseg * 4ull, hipMemcpyHostToDevice,
                              sl->stream));
-      launch_segment_query(d_views.as<TreeView>(), trees.size(), sl->d_q.as<ecl_query>(), count,
-                           sl->d_seg.as<uint64_t>(), sl->d_tree.as<uint32_t>(), nseg,
-                           sl->d_out.as<uint64_t>(), q_block, q_grid, sl->stream);
+      if (cfg.query_lines >= 3)
+        launch_segment_query_quad(d_views.as<TreeView>(), d_params.as<TreeQueryParams>(),
+                                  trees.size(), sl->d_q.as<ecl_query>(), count,
+                                  sl->d_seg.as<uint64_t>(), sl->d_tree.as<uint32_t>(), nseg,
+                                  sl->d_out.as<uint64_t>(), q_grid, sl->stream);
+      else
+        launch_segment_query(d_views.as<TreeView>(), trees.size(), sl->d_q.as<ecl_query>(),
+                             count, sl->d_seg.as<uint64_t>(), sl->d_tree.as<uint32_t>(), nseg,
+                             sl->d_out.as<uint64_t>(), q_block, q_grid, sl->stream);
       ECL_HIP(hipMemcpyAsync(sl->pin_out, sl->d_out.ptr, count * 8ull, hipMemcpyDeviceToHost,
                              sl->stream));
       ECL_HIP(hipEventRecord(sl->done, sl->stream));
@@ -295,7 +302,7 @@ ecl_status ecl_server_create(int device, const ecl_server_config* config, ecl_se
     if (s->cfg.max_trees == 0) s->cfg.max_trees = 1024;
     if (s->cfg.hbm_budget == 0) s->cfg.hbm_budget = free_device_memory() / 10 * 9;
     require(s->cfg.max_inflight <= 16, "at most 16 batches in flight");
-    require(s->cfg.query_lines <= 2, "query_lines must be 0, 1 or 2");
+    require(s->cfg.query_lines <= 3, "query_lines must be 0, 1, 2 or 3");
     int cus = 0;
     if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device) == hipSuccess &&
         cus > 0)
@@ -303,6 +310,10 @@ ecl_status ecl_server_create(int device, const ecl_server_config* config, ecl_se
     s->trees.resize(s->cfg.max_trees);
     s->d_views = DevBuf(s->cfg.max_trees * sizeof(TreeView));
     ECL_HIP(hipMemset(s->d_views.ptr, 0, s->cfg.max_trees * sizeof(TreeView)));
+    if (s->cfg.query_lines >= 3) {
+      s->d_params = DevBuf(s->cfg.max_trees * sizeof(TreeQueryParams));
+      ECL_HIP(hipMemset(s->d_params.ptr, 0, s->cfg.max_trees * sizeof(TreeQueryParams)));
+    }
     ECL_HIP(hipStreamCreateWithFlags(&s->ctl, hipStreamNonBlocking));
     const uint64_t B = s->cfg.max_batch;
     for (uint32_t i = 0; i < s->cfg.max_inflight; ++i) {
@@ -365,11 +376,26 @@ ecl_status ecl_server_destroy(ecl_server s) {
 // per 4096 ones or zeros, plus one per Value) of every level; with lines == 2
 // also its select lines (per array and Value one 128 B line every r of them,
 // r = max(1, floor(800 x density)): at most 4 / 800 lines per bit).
+// quad lines and quad select lines (query_lines 3) need a power-of-two
+// alphabet with an even number of levels; hex lines a level count that is a
+// multiple of 4
+static bool quad_layout(uint64_t sigma) {
+  const uint32_t L = ecl::detail::ceil_log2(sigma);
+  return (sigma & (sigma - 1)) == 0 && L >= 2 && L % 2 == 0;
+}
+
 static uint64_t tree_estimate(uint64_t n, uint64_t sigma, uint32_t lines) {
   const uint64_t L = ecl::detail::ceil_log2(sigma);
   uint64_t b = L * ((n + 1023) / 1024 * 128 + (n + 511) / 512 * 2 + (n + 65535) / 65536 * 8 +
                     n / 4096 * 8 + 1024) +
                sigma * 64 + 4096;
+  if (lines >= 3 && quad_layout(sigma)) {
+    // quad lines, quad select lines (a line every ~85 of each digit at
+    // density 1/4: 1.5 bytes per symbol, plus 4 lines per digit), hex lines
+    b += L / 2 * ((n / kQuadDigits + 1) * 128 + n / 2 * 3 + 4 * 128 * 2) + 65536;
+    if (L % 4 == 0) b += L / 4 * ((n / kHexDigits + 1) * 128) + 65536;
+    return b;
+  }
   if (lines >= 1)
     b += L * ((n + ecl::detail::kLineBits - 1) / ecl::detail::kLineBits * 128 +
               (n / ecl::detail::kSampleRate + 2) * 64);
@@ -392,8 +418,17 @@ static size_t lib_free_memory(int device) {
 // for them; on failure the tree is destroyed and the error returned.
 static ecl_status add_query_lines(ecl_server s, ecl_wavelet_tree wt, std::string* msg) {
   if (!s->cfg.query_lines) return ECL_OK;
-  ecl_status st = ecl_wavelet_tree_build_query_lines(wt);
-  if (st == ECL_OK && s->cfg.query_lines >= 2) st = ecl_wavelet_tree_build_select_lines(wt);
+  ecl_wavelet_tree_info info{};
+  ecl_status st = ecl_wavelet_tree_info_get(wt, &info);
+  if (st == ECL_OK && s->cfg.query_lines >= 3 && quad_layout(info.alphabet_size)) {
+    // the quad layouts (quad select lines build the quad lines first), and
+    // the hex lines where the level count is a multiple of 4
+    st = ecl_wavelet_tree_build_quad_select_lines(wt);
+    if (st == ECL_OK && info.num_levels % 4 == 0) st = ecl_wavelet_tree_build_hex_lines(wt);
+  } else if (st == ECL_OK) {
+    st = ecl_wavelet_tree_build_query_lines(wt);
+    if (st == ECL_OK && s->cfg.query_lines >= 2) st = ecl_wavelet_tree_build_select_lines(wt);
+  }
   if (st != ECL_OK) {
     *msg = last_error();
     (void)ecl_wavelet_tree_destroy(wt);
@@ -430,6 +465,12 @@ static void commit_slot(ecl_server s, uint32_t slot, ecl_wavelet_tree wt, uint64
   (void)ecl_wavelet_tree_device_view(wt, &v, sizeof(v));
   ECL_HIP(hipMemcpyAsync(s->d_views.as<TreeView>() + slot, &v, sizeof(v), hipMemcpyHostToDevice,
                          s->ctl));
+  TreeQueryParams qp{};
+  if (s->cfg.query_lines >= 3) {
+    qp = tree_query_params(wt);
+    ECL_HIP(hipMemcpyAsync(s->d_params.as<TreeQueryParams>() + slot, &qp, sizeof(qp),
+                           hipMemcpyHostToDevice, s->ctl));
+  }
   ECL_HIP(hipStreamSynchronize(s->ctl));
   std::lock_guard<std::mutex> lk(s->mu);
   TreeEntry& e = s->trees[slot];

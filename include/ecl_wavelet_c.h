@@ -536,7 +536,14 @@ typedef struct ecl_server_config {
                            * select hints (ecl_wavelet_tree_build_query_lines; about
                            * 1.08x its bits more memory); 2: and its select lines
                            * (ecl_wavelet_tree_build_select_lines; about 2.7x its
-                           * bits more).  Charged to the budget. */
+                           * bits more); 3: the quad layouts for trees over a
+                           * power-of-two alphabet with an even number of levels
+                           * (ecl_wavelet_tree_build_quad_select_lines, and
+                           * ecl_wavelet_tree_build_hex_lines where the level count
+                           * is a multiple of 4; about 9 bytes per symbol at sigma
+                           * = 256) and query and select lines for the other trees;
+                           * batches then run a pair of lanes per query.  Charged
+                           * to the budget. */
 } ecl_server_config;
 typedef struct ecl_server_stats {
   uint64_t requests;      /* completed requests since the last reset */

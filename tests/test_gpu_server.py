@@ -181,6 +181,43 @@ def test_server_query_lines(gpu, tmp_path):
     srv.close()
 
 
+@pytest.mark.parametrize("max_batch", [0, 1000])
+def test_server_quad_layouts(gpu, tmp_path, max_batch):
+    """A server configured for the quad layouts (query_lines 3) gives every
+    tree over a power-of-two alphabet with an even number of levels its quad
+    lines and quad select lines, and hex lines where the level count is a
+    multiple of 4 (sigma 16 and 256 here); the other trees (sigma 20 and 8)
+    get query and select lines.  Batches then mix both kinds of tree, with a
+    pair of lanes per query; every request equals the direct scans alone and
+    in the mix, from texts and from a saved file, on a skewed text too."""
+    from ecl_wavelet_amd import Server, WaveletTree
+    rng = np.random.default_rng(77)
+    skew = np.minimum(rng.zipf(1.3, 250_003) - 1, 255).astype(np.uint8)
+    texts = [(_text(51, 300_001, 256), 256), (_text(52, 150_000, 4), 4),
+             (_text(53, 90_001, 16), 16), (_text(54, 80_000, 20), 20),
+             (_text(55, 60_000, 64), 64), (skew, 256), (_text(56, 40_000, 8), 8)]
+    WaveletTree(texts[1][0], np.arange(4, dtype=np.uint8)).save(tmp_path / "t1.ewt")
+    srv = Server(max_batch=max_batch, query_lines=3)
+    trees = [srv.addTree(t, sg) if i != 1 else srv.addTreeFile(tmp_path / "t1.ewt")
+             for i, (t, sg) in enumerate(texts)]
+    work = []
+    for j in range(36):
+        ti = int(rng.integers(0, len(trees)))
+        size = int(2 ** rng.uniform(0, 13))
+        (q, e), = _requests(texts[ti][0], texts[ti][1], [size], seed=300 + j)
+        work.append((trees[ti], q, e))
+    alone = []
+    for tree, q, e in work:
+        out, _ = srv.wait(srv.submit(tree, q))
+        assert np.array_equal(out, e)
+        alone.append(out)
+    ids = [srv.submit(tree, q) for tree, q, _ in work]
+    for rid, a, (_, _, e) in zip(ids, alone, work):
+        out, _ = srv.wait(rid)
+        assert np.array_equal(out, a)
+    srv.close()
+
+
 def test_server_tree_from_raw_text_file(gpu, tmp_path):
     """A raw uint8 text file over a sparse alphabet is admitted after its
     conversion to the minimal alphabet (convertDataToMinAlphabet,
