@@ -584,18 +584,37 @@ __device__ __forceinline__ uint64_t quad_select(const TreeQueryParams& p, Table 
     const uint64_t N = c >> 2, d = c & 3;
     if (k > tab[cd + 4 * (N + 1) + d] - tab[cd + 4 * N + d]) return n;
   }
+  // the node-table entries of a level depend only on c: the next level's
+  // are read while this level's line is in flight
   uint64_t kk = k;
-  for (int32_t ql = static_cast<int32_t>(Q) - 1; ql >= 0; --ql) {
-    const uint64_t N = c >> (2 * (Q - ql));
-    const uint32_t d = static_cast<uint32_t>(c >> (2 * (Q - 1 - ql))) & 3u;
-    const uint64_t g = tab[p.quad.cd_off[ql] + 4 * N + d] + kk;
+  int32_t ql = static_cast<int32_t>(Q) - 1;
+  uint64_t N = c >> 2;
+  uint32_t d = static_cast<uint32_t>(c) & 3u;
+  uint64_t cdv = tab[p.quad.cd_off[ql] + 4 * N + d], sv = tab[p.quad.s_off[ql] + N];
+  while (true) {
+    const uint64_t g = cdv + kk;
+    const int32_t qn = ql - 1;
+    uint64_t Nn = 0, cdn = 0, sn = 0;
+    uint32_t dn = 0;
+    if (qn >= 0) {
+      Nn = c >> (2 * (Q - qn));
+      dn = static_cast<uint32_t>(c >> (2 * (Q - 1 - qn))) & 3u;
+      cdn = tab[p.quad.cd_off[qn] + 4 * Nn + dn];
+      sn = tab[p.quad.s_off[qn] + Nn];
+    }
     bool past = false;
     uint64_t end = 0;
     uint64_t pos = quad_select_line(p, static_cast<uint32_t>(ql), d, g, &past, &end);
     // past the line's digits (a sparse stretch of d): the quad lines from
     // the one holding digit `end` on
     if (past) pos = quad_select_walk(p, static_cast<uint32_t>(ql), d, g, end);
-    kk = pos - tab[p.quad.s_off[ql] + N] + 1;
+    kk = pos - sv + 1;
+    if (qn < 0) break;
+    ql = qn;
+    N = Nn;
+    d = dn;
+    cdv = cdn;
+    sv = sn;
   }
   return kk - 1;
 }
