@@ -152,6 +152,7 @@ _SIGS = {
     "ecl_wavelet_tree_build_quad_lines": (st, [vp]),
     "ecl_wavelet_tree_build_quad_select_lines": (st, [vp]),
     "ecl_wavelet_tree_build_hex_lines": (st, [vp]),
+    "ecl_wavelet_tree_layout_views": (st, [vp, vp, vp]),
     "ecl_wavelet_tree_autotune": (st, [vp, u64, u64, C.POINTER(u32), vp]),
     "ecl_forest_create": (st, [vp, vp, vp, vp, sz, i32, u32, C.POINTER(vp)]),
     "ecl_forest_destroy": (st, [vp]),

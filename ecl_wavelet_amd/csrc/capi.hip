@@ -568,6 +568,7 @@ struct ecl_wavelet_tree_t {
   QuadView quad{};
   DevBuf hex_lines, hex_tab;                // optional hex lines, bases and superblocks
   HexView hex{};
+  DevBuf d_layouts;  // device copies of quad and hex (once asked for; kept current)
   std::unique_ptr<HostPipe> pipe{new HostPipe()};
   ~ecl_wavelet_tree_t() { delete rs; }
 };
@@ -1553,6 +1554,13 @@ static uint64_t select_line_magic(uint32_t r) { return r > 1 ? ~0ull / r + 1 : 0
 // prefix of cnt over N); base[4N + d] = C of child 4N + d minus CD[N][d].
 // The lines themselves come from the binary levels 2q and 2q + 1 on the
 // device.
+// the device copies of the tree's quad and hex views, when they exist
+static void sync_layout_views(ecl_wavelet_tree_t* wt) {
+  if (wt->d_layouts.ptr == nullptr) return;
+  h2d(wt->d_layouts.ptr, &wt->quad, 1);
+  h2d(static_cast<uint8_t*>(wt->d_layouts.ptr) + sizeof(QuadView), &wt->hex, 1);
+}
+
 static void build_quad_lines(ecl_wavelet_tree_t* wt) {
   if (wt->quad.levels != 0) return;
   const uint32_t L = wt->L;
@@ -1646,6 +1654,7 @@ static void build_quad_lines(ecl_wavelet_tree_t* wt) {
   wt->quad_base = std::move(dbase);
   wt->quad_tab = std::move(dtab);
   wt->quad = qv;
+  sync_layout_views(wt);
 }
 
 // Quad select lines (QuadView): for each quad level and digit d with t
@@ -1709,6 +1718,7 @@ static void build_quad_select_lines(ecl_wavelet_tree_t* wt) {
   ECL_HIP(hipStreamSynchronize(s));
   wt->quad_sel = std::move(lines);
   qv.has_sel = 1;
+  sync_layout_views(wt);
 }
 
 // Hex lines of a tree (HexView): for hex level h, from the symbol counts,
@@ -1797,6 +1807,7 @@ static void build_hex_lines(ecl_wavelet_tree_t* wt) {
   wt->hex_lines = std::move(lines);
   wt->hex_tab = std::move(dtab);
   wt->hex = hv;
+  sync_layout_views(wt);
 }
 
 static void build_select_lines(ecl_rank_select_t* rs) {
@@ -2534,6 +2545,19 @@ ecl_status ecl_wavelet_tree_build_quad_lines(ecl_wavelet_tree wt) {
     require(wt != nullptr, "NULL tree");
     set_device(wt->device);
     build_quad_lines(wt);
+  });
+}
+ecl_status ecl_wavelet_tree_layout_views(ecl_wavelet_tree wt, const void** quad_view,
+                                         const void** hex_view) {
+  return guard([&] {
+    require(wt != nullptr && quad_view != nullptr && hex_view != nullptr, "NULL argument");
+    set_device(wt->device);
+    if (wt->d_layouts.ptr == nullptr) {
+      wt->d_layouts = DevBuf(sizeof(QuadView) + sizeof(HexView));
+      sync_layout_views(wt);
+    }
+    *quad_view = wt->d_layouts.ptr;
+    *hex_view = static_cast<const uint8_t*>(wt->d_layouts.ptr) + sizeof(QuadView);
   });
 }
 ecl_status ecl_wavelet_tree_build_hex_lines(ecl_wavelet_tree wt) {

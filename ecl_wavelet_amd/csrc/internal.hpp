@@ -11,10 +11,22 @@
 #include <vector>
 
 #include "ecl_wavelet/detail/device_core.hpp"
+#include "ecl_wavelet/detail/quad_core.hpp"
 #include "ecl_wavelet_c.h"
 
 namespace ecl {
 namespace lib {
+
+// the quad and hex layouts (ecl_wavelet/detail/quad_core.hpp)
+using detail::HexView;
+using detail::QuadView;
+using detail::hex_match;
+using detail::kHexDigits;
+using detail::kHexSbShift;
+using detail::kMaxHexLevels;
+using detail::kMaxQuadLevels;
+using detail::kQuadDigits;
+using detail::quad_match;
 
 using detail::LevelView;
 using detail::TreeView;
@@ -218,44 +230,6 @@ void launch_presence(const void* text, int esize, uint64_t n, uint32_t* flags,
                      hipStream_t s);
 
 // ---- launchers (kernels_query.hip) -------------------------------------------
-// Quad lines (opt-in; power-of-two alphabets with an even number of levels,
-// at most 16): quad level q holds the 2-bit digits of the codes (bits 2q and
-// 2q+1 from the top, the first one high) in the order of binary level 2q,
-// in 128-byte lines: the counts of digits 0, 1 and 2 before the line as
-// three 40-bit fields in the first 16 bytes, then 448 digits, digit i at
-// bits 2i..2i+1 of the 112 bytes.  With them access and rank read one line
-// per two levels.  Descending from position p in node N of quad level q by
-// digit d leads to position base[off[q] + 4N + d] + (d's before p) of quad
-// level q + 1 (base = the child's start minus the d's before node N).
-constexpr uint32_t kQuadDigits = 448;
-// matches of digit d among the 32 digits of w, one bit per digit (bit 2i)
-__device__ __forceinline__ uint64_t quad_match(uint64_t w, uint32_t d) {
-  const uint64_t y = w ^ (0x5555555555555555ull * d);
-  return ~(y | (y >> 1)) & 0x5555555555555555ull;
-}
-constexpr uint32_t kMaxQuadLevels = 8;
-struct QuadView {
-  const uint4* lines[kMaxQuadLevels];
-  const uint64_t* base;          // every level's bases
-  uint64_t off[kMaxQuadLevels];  // level q's bases start at base + off[q]
-  uint64_t nbase;                // entries of base
-  uint32_t levels;               // 0: no quad lines
-  // Node tables of every level (for select), in one array of ntab u64:
-  // tab[s_off[q] + N] = start of node N of level 2q (N in [0, 4^q], the
-  // last one n) and tab[cd_off[q] + 4N + d] = the d's before node N.
-  const uint64_t* tab;
-  uint64_t ntab;
-  uint64_t s_off[kMaxQuadLevels], cd_off[kMaxQuadLevels];
-  // Optional quad select lines: for level q and digit d with rate
-  // r = sel_rate[q][d] > 0, line j (from sel_first[q][d] in sel_lines[q])
-  // is the position s_j of the (j r)-th d of the level (0 for j = 0), 8
-  // unused bytes, then the 448 digits from the 32-digit word holding s_j.
-  const uint4* sel_lines[kMaxQuadLevels];
-  uint64_t sel_first[kMaxQuadLevels][4];
-  uint32_t sel_rate[kMaxQuadLevels][4];
-  uint64_t sel_magic[kMaxQuadLevels][4];
-  uint32_t has_sel;
-};
 constexpr uint32_t kQuadSelSpan = 340;  // digits the r digits of a select line span on average
 
 // One quad level's select lines of every digit, from its quad lines; digit
@@ -291,37 +265,6 @@ struct QuadBuildParams {
 constexpr uint32_t kQuadWgInfo = 6;
 uint64_t quad_build_workgroups(uint64_t nlines);
 void launch_quad_lines(const QuadBuildParams& qp, hipStream_t s);
-
-// Hex lines (opt-in; power-of-two alphabets whose level count is a multiple
-// of 4, at most 16): hex level h holds the 4-bit digits of the codes (bits
-// 4h .. 4h+3 from the top, the first one high) in the order of binary level
-// 4h, in 128-byte lines of 128 digits.  A line is four 32-byte parts: the
-// counts of digits 0..7 before the line (u32 each), digits 0..63, the
-// counts of digits 8..15, digits 64..127 (digit i at bits 4(i mod 16) of
-// its 64-bit word), so that each lane of a pair reads one half.  A count is
-// stored modulo 2^32: with sb = the full count at the start of the line's
-// superblock (2^24 lines, 2^31 digits), the count is sb + (u32)(stored - sb).
-// Descending from position p in node N of hex level h by digit x leads to
-// position base[16N + x] + (x's before p) of hex level h + 1 (base = the
-// child's start minus the x's before node N).  With them access and rank
-// read one line per four levels.
-constexpr uint32_t kHexDigits = 128;
-constexpr uint32_t kMaxHexLevels = 4;
-constexpr uint32_t kHexSbShift = 24;  // lines per superblock: 2^24
-// matches of digit x among the 16 digits of w, one bit per digit (bit 4i)
-__device__ __forceinline__ uint64_t hex_match(uint64_t w, uint32_t x) {
-  const uint64_t y = w ^ (0x1111111111111111ull * x);
-  return ~(y | (y >> 1) | (y >> 2) | (y >> 3)) & 0x1111111111111111ull;
-}
-struct HexView {
-  const uint4* lines[kMaxHexLevels];  // n / 128 + 1 lines per level
-  // one array of ntab u64: tab[base_off[h] + 16N + x] = base, and
-  // tab[sb_off[h] + 16 s + x] = the x's before superblock s
-  const uint64_t* tab;
-  uint64_t ntab;
-  uint64_t base_off[kMaxHexLevels], sb_off[kMaxHexLevels];
-  uint32_t levels;  // 0: no hex lines
-};
 
 // One hex level's build from the quad lines of quad levels 2h (qa) and
 // 2h + 1 (qb).  Position p of node N with quad digit d0 goes to position

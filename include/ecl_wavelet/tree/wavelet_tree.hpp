@@ -15,7 +15,13 @@
 //    calling with the same query; its lanes split each level's rank or
 //    select and all return the answer;
 //  * the `offsets` LDS argument of deviceAccess / deviceRank is accepted and
-//    ignored (level descriptors carry pointers, not word offsets).
+//    ignored (level descriptors carry pointers, not word offsets);
+//  * when the tree carries hex or quad lines (built through the C ABI on
+//    handle(), e.g. ecl_wavelet_tree_build_hex_lines), deviceAccess and
+//    deviceRank with ThreadsPerQuery >= 2 read them, and deviceSelect reads
+//    the quad select lines: one line per four or two levels, each pair of
+//    adjacent lanes of a group running the descent.  Answers do not change;
+//    with one thread per query the binary levels are read.
 #pragma once
 
 #include <algorithm>
@@ -27,6 +33,7 @@
 #include <vector>
 
 #include "ecl_wavelet/bitarray/rank_select.hpp"
+#include "ecl_wavelet/detail/quad_core.hpp"
 
 namespace ecl {
 
@@ -103,13 +110,14 @@ class WaveletTree {
   // copy = shallow view (wavelet_tree.cuh:870-887)
   __host__ WaveletTree(WaveletTree const& o)
       : rank_select_(o.rank_select_), handle_(o.handle_), host_(o.host_), view_(o.view_),
-        info_(o.info_), is_copy_(true) {}
+        quad_(o.quad_), hex_(o.hex_), info_(o.info_), is_copy_(true) {}
   WaveletTree& operator=(const WaveletTree&) = delete;
   WaveletTree(WaveletTree&&) = delete;
   __host__ WaveletTree& operator=(WaveletTree&& o) {
     release();
     rank_select_ = std::move(o.rank_select_);
     handle_ = o.handle_; host_ = o.host_; view_ = o.view_; info_ = o.info_;
+    quad_ = o.quad_; hex_ = o.hex_;
     is_copy_ = o.is_copy_;
     o.is_copy_ = true;
     return *this;
@@ -153,6 +161,16 @@ class WaveletTree {
   __device__ T deviceAccess(size_t index, bool const /*is_pow_two*/, size_t* counts = nullptr,
                             size_t* /*offsets*/ = nullptr, size_t* ranks = nullptr) {
     check_tpq<ThreadsPerQuery>();
+    if constexpr (ThreadsPerQuery >= 2) {
+      if (hex_->levels) {
+        const detail::LayoutRef p{view_, *quad_, *hex_};
+        return static_cast<T>(detail::hex_access(p, detail::QuadTable<false>{hex_->tab}, index));
+      }
+      if (quad_->levels) {
+        const detail::LayoutRef p{view_, *quad_, *hex_};
+        return static_cast<T>(detail::quad_access(p, detail::QuadTable<false>{quad_->base}, index));
+      }
+    }
     return static_cast<T>(detail::tree_access<false, ThreadsPerQuery>(
         view_, view_.levels, counts_or(counts), ranks_or(ranks), view_.nodes_at_level, index));
   }
@@ -161,6 +179,18 @@ class WaveletTree {
                                size_t* counts = nullptr, size_t* /*offsets*/ = nullptr,
                                size_t* ranks = nullptr) {
     check_tpq<ThreadsPerQuery>();
+    if constexpr (ThreadsPerQuery >= 2) {
+      if (hex_->levels) {
+        const detail::LayoutRef p{view_, *quad_, *hex_};
+        return detail::hex_rank(p, detail::QuadTable<false>{hex_->tab}, query.symbol_,
+                                query.index_);
+      }
+      if (quad_->levels) {
+        const detail::LayoutRef p{view_, *quad_, *hex_};
+        return detail::quad_rank(p, detail::QuadTable<false>{quad_->base}, query.symbol_,
+                                 query.index_);
+      }
+    }
     return detail::tree_rank<false, ThreadsPerQuery>(view_, view_.levels, counts_or(counts),
                                                      ranks_or(ranks), view_.nodes_at_level,
                                                      query.symbol_, query.index_);
@@ -169,6 +199,13 @@ class WaveletTree {
   __device__ size_t deviceSelect(RankSelectQuery<T> const& query, bool const /*is_pow_two*/,
                                  size_t* ranks = nullptr) {
     check_tpq<ThreadsPerQuery>();
+    if constexpr (ThreadsPerQuery >= 2) {
+      if (quad_->has_sel) {
+        const detail::LayoutRef p{view_, *quad_, *hex_};
+        return detail::quad_select(p, detail::QuadTable<false>{quad_->tab}, query.symbol_,
+                                   query.index_);
+      }
+    }
     return detail::tree_select<true, true, ThreadsPerQuery>(view_, view_.levels, view_.counts,
                                                             ranks_or(ranks), view_.nodes_at_level,
                                                             query.symbol_, query.index_);
@@ -277,6 +314,10 @@ class WaveletTree {
   __host__ void bind() {
     utils::throwOnError(ecl_wavelet_tree_info_get(handle_, &info_));
     utils::throwOnError(ecl_wavelet_tree_device_view(handle_, &view_, sizeof(view_)));
+    const void *qv = nullptr, *hv = nullptr;
+    utils::throwOnError(ecl_wavelet_tree_layout_views(handle_, &qv, &hv));
+    quad_ = static_cast<const detail::QuadView*>(qv);
+    hex_ = static_cast<const detail::HexView*>(hv);
     host_->alphabet.resize(info_.alphabet_size);
     utils::throwOnError(ecl_wavelet_tree_alphabet(handle_, host_->alphabet.data()));
     host_->counts.resize(info_.alphabet_size);
@@ -306,6 +347,8 @@ class WaveletTree {
   ecl_wavelet_tree handle_ = nullptr;
   Host* host_ = nullptr;
   detail::TreeView view_{};
+  const detail::QuadView* quad_ = nullptr;  // device copies, kept current by the library
+  const detail::HexView* hex_ = nullptr;
   ecl_wavelet_tree_info info_{};
   bool is_copy_ = true;
 };

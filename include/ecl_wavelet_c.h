@@ -375,6 +375,14 @@ ecl_status ecl_wavelet_tree_build_quad_select_lines(ecl_wavelet_tree wt);
  * change.  Other trees get ECL_ERROR_INVALID_ARGUMENT.  A second call keeps
  * the existing lines; build them while no query on the tree is in flight. */
 ecl_status ecl_wavelet_tree_build_hex_lines(ecl_wavelet_tree wt);
+/* Device addresses of the tree's quad and hex views (ecl::detail::QuadView and
+ * HexView of ecl_wavelet/detail/quad_core.hpp; no counterpart in the
+ * reference).  The library keeps them current as the quad lines, quad select
+ * lines and hex lines are built, so that copies of a tree passed into
+ * kernels see layouts built later; the C++ drop-in's device API reads them.
+ * Valid until the tree is destroyed. */
+ecl_status ecl_wavelet_tree_layout_views(ecl_wavelet_tree wt, const void** quad_view,
+                                         const void** hex_view);
 /* Persistence (no counterpart in the reference, which rebuilds every tree
  * from its text; for warm starts of servers).  The file holds a header, the
  * alphabet, the symbol counts and every level's bits; loading checks them

@@ -494,6 +494,103 @@ static void test_tree_type() {
   std::printf("wavelet_tree<%zu>: ok so far (%d failures)\n", sizeof(T), g_fail);
 }
 
+// The device API over the quad and hex layouts: power-of-two alphabets (16,
+// 64 and 256 symbols; not minimal except 256 of uint8), the same user
+// kernels for every ThreadsPerQuery over the binary levels, then with the
+// quad lines (access and rank, ThreadsPerQuery >= 2), the quad select lines
+// (select) and the hex lines (access and rank, 16 and 256 symbols), built
+// through the C ABI after the tree was passed to earlier launches.
+template <typename T>
+static void test_device_api_layouts() {
+  for (size_t sigma : {size_t(16), size_t(64), size_t(256)}) {
+    std::mt19937_64 gen(sizeof(T) * 101 + sigma);
+    std::vector<T> alphabet;
+    if (sizeof(T) == 1 && sigma == 256) {
+      for (size_t c = 0; c < 256; ++c) alphabet.push_back(static_cast<T>(c));
+    } else {
+      while (alphabet.size() < sigma) {
+        T v = static_cast<T>(gen());
+        if (std::find(alphabet.begin(), alphabet.end(), v) == alphabet.end()) alphabet.push_back(v);
+      }
+      std::sort(alphabet.begin(), alphabet.end());
+    }
+    const size_t n = 150001;
+    std::vector<T> data(n);
+    for (auto& x : data) {  // skewed: symbol j with weight ~ 1 / (j + 1)
+      const size_t r = gen() % (sigma * 4);
+      x = alphabet[r < sigma ? r : gen() % (1 + r % sigma)];
+    }
+    auto a_copy = alphabet;
+    WaveletTree<T> wt(data.data(), n, std::move(a_copy), 0);
+    auto minimal = [&](T c) { return size_t(std::lower_bound(alphabet.begin(), alphabet.end(), c) - alphabet.begin()); };
+    std::vector<size_t> idx(3000), aexp(idx.size());
+    for (size_t i = 0; i < idx.size(); ++i) {
+      idx[i] = i < 2 ? (i ? n - 1 : 0) : gen() % n;
+      aexp[i] = minimal(data[idx[i]]);
+    }
+    std::vector<RankSelectQuery<T>> q, sq;
+    std::vector<size_t> exp, sexp;
+    for (int j = 0; j < 2000; ++j) {
+      const size_t i = j == 0 ? n : gen() % (n + 1);
+      const size_t c = gen() % sigma;
+      q.push_back({i, static_cast<T>(c)});
+      exp.push_back(std::count(data.begin(), data.begin() + i, alphabet[c]));
+    }
+    for (int j = 0; j < 1500; ++j) {
+      const size_t p = gen() % n;
+      const size_t k = std::count(data.begin(), data.begin() + p + 1, data[p]);
+      sq.push_back({k, static_cast<T>(minimal(data[p]))});
+      sexp.push_back(p);
+    }
+    for (size_t c : {size_t(0), sigma - 1}) {  // one past the last occurrence
+      sq.push_back({size_t(std::count(data.begin(), data.end(), alphabet[c])) + 1, static_cast<T>(c)});
+      sexp.push_back(n);
+    }
+    const size_t mq = std::max({idx.size(), q.size(), sq.size()});
+    size_t *d_idx, *d_out;
+    RankSelectQuery<T>* d_q;
+    gpuErrchk(hipMalloc(&d_idx, idx.size() * 8));
+    gpuErrchk(hipMalloc(&d_out, mq * 64 * 8));
+    gpuErrchk(hipMalloc(&d_q, mq * sizeof(RankSelectQuery<T>)));
+    gpuErrchk(hipMemcpy(d_idx, idx.data(), idx.size() * 8, hipMemcpyHostToDevice));
+    std::vector<size_t> out(mq * 64);
+    ecl_wavelet_tree_info info;
+    utils::throwOnError(ecl_wavelet_tree_info_get(wt.handle(), &info));
+    const uint32_t L = info.num_levels;
+    for (int layout = 0; layout < 4; ++layout) {
+      if (layout == 1) utils::throwOnError(ecl_wavelet_tree_build_quad_lines(wt.handle()));
+      if (layout == 2) utils::throwOnError(ecl_wavelet_tree_build_quad_select_lines(wt.handle()));
+      if (layout == 3) {
+        if (L % 4 != 0) break;
+        utils::throwOnError(ecl_wavelet_tree_build_hex_lines(wt.handle()));
+      }
+      for (int tpq : kGroupSizes) {
+        for (int op = 0; op < 3; ++op) {
+          const size_t cnt = op == 0 ? idx.size() : op == 1 ? q.size() : sq.size();
+          if (op == 1)
+            gpuErrchk(hipMemcpy(d_q, q.data(), cnt * sizeof(RankSelectQuery<T>), hipMemcpyHostToDevice));
+          if (op == 2)
+            gpuErrchk(hipMemcpy(d_q, sq.data(), cnt * sizeof(RankSelectQuery<T>), hipMemcpyHostToDevice));
+          run_user<T>(wt, tpq, op, d_idx, d_q, cnt, d_out);
+          gpuErrchk(hipMemcpy(out.data(), d_out, cnt * tpq * 8, hipMemcpyDeviceToHost));
+          const std::vector<size_t>& e = op == 0 ? aexp : op == 1 ? exp : sexp;
+          int bad = 0;
+          for (size_t j = 0; j < cnt; ++j)
+            for (int t = 0; t < tpq; ++t) bad += out[j * tpq + t] != e[j];
+          if (bad)
+            std::printf("device API layouts <%zu> sigma %zu: layout %d TPQ %d op %d: %d wrong\n",
+                        sizeof(T), sigma, layout, tpq, op, bad);
+          EXPECT_EQ(bad, 0);
+        }
+      }
+    }
+    gpuErrchk(hipFree(d_idx));
+    gpuErrchk(hipFree(d_out));
+    gpuErrchk(hipFree(d_q));
+  }
+  std::printf("device API layouts <%zu>: ok so far (%d failures)\n", sizeof(T), g_fail);
+}
+
 static void test_codes_and_nodes() {
   // createMinimalCodes KATs (test_wavelet_tree.cu:174-217)
   std::vector<uint8_t> a(72);
@@ -701,6 +798,10 @@ int main() {
     test_tree_type<uint16_t>();
     test_tree_type<uint32_t>();
     test_tree_type<uint64_t>();
+    test_device_api_layouts<uint8_t>();
+    test_device_api_layouts<uint16_t>();
+    test_device_api_layouts<uint32_t>();
+    test_device_api_layouts<uint64_t>();
     test_codes_and_nodes();
     test_oom();
     example_dna();
