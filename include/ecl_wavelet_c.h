@@ -346,8 +346,9 @@ ecl_status ecl_rank_select_build_select_lines(ecl_rank_select rs);
  * codes' 2-bit digits in the order of level 2q, in 128-byte lines of 448
  * digits headed by the counts of digits 0, 1 and 2 before the line.  Access
  * and rank then read one memory line per two levels (the tree's host and
- * device access/rank entry points use them, and mixed batches once the quad
- * select lines exist too; the device API keeps the binary levels).  Costs
+ * device access/rank entry points use them, mixed batches once the quad
+ * select lines exist too, and the C++ drop-in's device API for groups of
+ * two or more lanes).  Costs
  * 128/112 of the bits (about one
  * byte per symbol at sigma = 256); results do not change.  Other trees get
  * ECL_ERROR_INVALID_ARGUMENT.  A second call keeps the existing lines; build
