@@ -306,6 +306,20 @@ def bench_batched(args):
     batched_lines_ms = float(np.median(runs))
     log(f"batched: query lines built in {lines_ms:.1f} ms, {m} queries in "
         f"{batched_lines_ms:.2f} ms over them")
+    # and with the select lines too
+    timer.start()
+    forest.buildSelectLines()
+    sel_lines_ms = timer.stop()
+    g.launch()
+    E.synchronize()
+    runs = []
+    for _ in range(3):
+        timer.start()
+        g.launch()
+        runs.append(timer.stop())
+    batched_all_ms = float(np.median(runs))
+    log(f"batched: select lines built in {sel_lines_ms:.1f} ms, {m} queries in "
+        f"{batched_all_ms:.2f} ms over all lines")
     del g, forest
     # ---- plain: one tree per text, per-tree per-kind calls
     alph = {int(x): np.arange(int(x), dtype=np.uint8) for x in (4, 20, 64, 256)}
@@ -370,6 +384,9 @@ def bench_batched(args):
         "query_lines_build_ms": lines_ms,
         "batched_qps_query_lines": m / (batched_lines_ms / 1e3),
         "batched_launch_ms_query_lines": batched_lines_ms,
+        "select_lines_build_ms": sel_lines_ms,
+        "batched_qps_all_lines": m / (batched_all_ms / 1e3),
+        "batched_launch_ms_all_lines": batched_all_ms,
         "plain_per_tree_p50_ms": float(np.percentile(lat, 50)),
         "plain_per_tree_p99_ms": float(np.percentile(lat, 99)),
         "autotuned": {"block": block, "grid": grid, "build_tile_rows": tile_rows},

@@ -573,6 +573,13 @@ class Forest:
         queries and graphs read them, with the same answers."""
         call("ecl_forest_build_query_lines", self._h)
 
+    def buildSelectLines(self):
+        """Select lines for every tree (a select reads one 128-byte line per
+        level), in one launch over all levels; later queries and graphs read
+        them, with the same answers.  A build graph's replay drops them from
+        the trees it rebuilds."""
+        call("ecl_forest_build_select_lines", self._h)
+
     def treeInfo(self, i):
         n, s = C.c_uint64(), C.c_uint64()
         L = C.c_uint32()

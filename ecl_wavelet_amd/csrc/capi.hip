@@ -1540,13 +1540,6 @@ static void build_query_lines(ecl_rank_select_t* rs) {
   rs->lines = std::move(lines);
 }
 
-// The multiplier that turns i / r into a multiply-high (LevelView::sel_magic).
-static uint64_t select_line_magic(uint32_t r) { return r > 1 ? ~0ull / r + 1 : 0; }
-
-// Select lines for every array of a directory (opt-in): for each Value a
-// line every r Values, r = max(1, floor(800 x density)) so that the next r
-// Values lie in the line's 960 bits; the level views point to them
-// afterwards.  About 0.32 bytes per bit of every array.
 // Quad lines of a tree (QuadView): for quad level q, from the symbol
 // counts, per node N of level 2q (4^q nodes, N = the codes' top 2q bits) and
 // digit d: cnt[N][d] = the symbols with top 2q + 2 bits 4N + d; S_N = C of
@@ -1810,22 +1803,21 @@ static void build_hex_lines(ecl_wavelet_tree_t* wt) {
   sync_layout_views(wt);
 }
 
+// Select lines for every array of a directory (opt-in): for each Value a
+// line every r Values, r = max(1, floor(800 x density)) so that the next r
+// Values lie in the line's 960 bits; the level views point to them
+// afterwards.  About 0.32 bytes per bit of every array.
 static void build_select_lines(ecl_rank_select_t* rs) {
   if (rs->sel_lines.ptr != nullptr) return;
   const size_t na = rs->levels.size();
   std::vector<uint64_t> off(na), n1(na), n0(na);
   std::vector<uint32_t> r1(na), r0(na);
   uint64_t total = 0;
-  auto rate = [](uint64_t count, uint64_t size) -> uint32_t {
-    if (count == 0) return 0;
-    const uint64_t r = 800 * count / size;
-    return static_cast<uint32_t>(r < 1 ? 1 : r);
-  };
   for (size_t i = 0; i < na; ++i) {
     const LevelView& v = rs->levels[i];
     const uint64_t zeros = v.size - v.ones;
-    r1[i] = v.size ? rate(v.ones, v.size) : 0;
-    r0[i] = v.size ? rate(zeros, v.size) : 0;
+    r1[i] = v.size ? select_line_rate(v.ones, v.size) : 0;
+    r0[i] = v.size ? select_line_rate(zeros, v.size) : 0;
     n1[i] = r1[i] ? v.ones / r1[i] + 1 : 0;
     n0[i] = r0[i] ? zeros / r0[i] + 1 : 0;
     off[i] = total;

@@ -194,6 +194,7 @@ struct ecl_forest_t {
   uint32_t tile_rows = 0;              // construction tile (0: by padding)
   std::vector<LevelView*> d_levels;    // where each level's view lives in the arena
   DevBuf lines;                        // query lines and select hints (opt-in)
+  DevBuf sel_lines;                    // select lines (opt-in)
   std::shared_ptr<ForestBuild> build;  // buffers of a build graph, while one lives
 };
 
@@ -726,6 +727,81 @@ ecl_status ecl_forest_build_query_lines(ecl_forest f) {
                        static_cast<uint32_t>(na), L, tl, H, th, d_fit.as<uint32_t>(), s);
     ECL_HIP(hipStreamSynchronize(s));
     f->lines = std::move(lines);
+  });
+}
+
+// Select lines for every level of every tree (ecl_forest_build_select_lines):
+// the rates and line counts from each level's size and ones (as for one
+// tree's directory), then one launch over all levels, then the level views
+// get the lines.  Old lines are dropped from the views before they are
+// freed, so that the views never point at freed memory.
+ecl_status ecl_forest_build_select_lines(ecl_forest f) {
+  return guard([&] {
+    require(f, "NULL argument");
+    set_device(f->device);
+    const size_t na = f->d_levels.size();
+    if (na == 0) return;
+    require(na < (1ull << 32), "too many bit arrays");
+    // (the views of all trees are consecutive in the arena: one copy)
+    LevelView* d_views =
+        reinterpret_cast<LevelView*>(f->arena.as<uint8_t>() + f->trees.front().lv_at);
+    std::vector<LevelView> views(na);
+    d2h(views.data(), d_views, na);
+    bool have = f->sel_lines.ptr != nullptr;
+    for (const LevelView& v : views) have = have && (v.sel_lines != nullptr || v.size == 0);
+    if (have) return;
+    for (LevelView& v : views) {
+      v.sel_lines = nullptr;
+      v.sel_zero_line = 0;
+      v.sel_rate[0] = v.sel_rate[1] = 0;
+      v.sel_magic[0] = v.sel_magic[1] = 0;
+    }
+    h2d(d_views, views.data(), na);
+    f->sel_lines.reset();
+    std::vector<SelLinesJob> jobs(na);
+    std::vector<uint64_t> wg_start(na);
+    uint64_t total = 0, wgs = 0;
+    for (size_t a = 0; a < na; ++a) {
+      const LevelView& v = views[a];
+      SelLinesJob& J = jobs[a];
+      J.v = v;
+      const uint64_t zeros = v.size - v.ones;
+      J.r1 = v.size ? select_line_rate(v.ones, v.size) : 0;
+      J.r0 = v.size ? select_line_rate(zeros, v.size) : 0;
+      J.n1 = J.r1 ? v.ones / J.r1 + 1 : 0;
+      const uint64_t n0 = J.r0 ? zeros / J.r0 + 1 : 0;
+      J.nl2 = (v.size + detail::kL2Bits - 1) / detail::kL2Bits;
+      J.nchunks = J.n1 + n0 ? (J.nl2 + kSelLineChunk - 1) / kSelLineChunk : 0;
+      J.line_off = total;
+      wg_start[a] = wgs;
+      total += J.n1 + n0;
+      wgs += 2 * J.nchunks;
+    }
+    const uint64_t meta = na * (sizeof(SelLinesJob) + 8);
+    if (total * 128 + meta + (64u << 20) > free_device_memory())
+      throw Error(ECL_ERROR_OUT_OF_MEMORY_TREE, "Not enough memory available for the wavelet tree.");
+    DevBuf lines(std::max<uint64_t>(total, 1) * 128);
+    DevBuf d_jobs(na * sizeof(SelLinesJob));
+    DevBuf d_start(na * 8);
+    h2d(d_jobs.ptr, jobs.data(), na);
+    h2d(d_start.ptr, wg_start.data(), na);
+    hipStream_t s = nullptr;
+    launch_multi_select_lines(d_jobs.as<SelLinesJob>(), d_start.as<uint64_t>(),
+                              static_cast<uint32_t>(na), wgs, lines.as<uint4>(), s);
+    ECL_HIP(hipStreamSynchronize(s));
+    for (size_t a = 0; a < na; ++a) {
+      const SelLinesJob& J = jobs[a];
+      if (J.nchunks == 0) continue;
+      LevelView& v = views[a];
+      v.sel_lines = lines.as<uint4>() + J.line_off * 8;
+      v.sel_zero_line = J.n1;
+      v.sel_rate[1] = J.r1;
+      v.sel_rate[0] = J.r0;
+      v.sel_magic[1] = select_line_magic(J.r1);
+      v.sel_magic[0] = select_line_magic(J.r0);
+    }
+    h2d(d_views, views.data(), na);
+    f->sel_lines = std::move(lines);
   });
 }
 

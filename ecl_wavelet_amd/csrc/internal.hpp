@@ -100,6 +100,28 @@ void launch_query_lines(const LevelView& v, uint4* lines, uint64_t nlines, hipSt
 // zeros at rate r0 (layout at LevelView::sel_lines)
 void launch_select_lines(const LevelView& v, uint4* out, uint64_t n1, uint64_t n0, uint32_t r1,
                          uint32_t r0, hipStream_t s);
+// The select-line rate of a Value with `count` of `size` bits: r = max(1,
+// floor(800 x density)), so that the next r Values lie within a line's 960
+// bits (0 when the Value is absent); and the multiplier that turns i / r into
+// a multiply-high (LevelView::sel_magic).
+inline uint32_t select_line_rate(uint64_t count, uint64_t size) {
+  if (count == 0) return 0;
+  const uint64_t r = 800 * count / size;
+  return static_cast<uint32_t>(r < 1 ? 1 : r);
+}
+__host__ __device__ inline uint64_t select_line_magic(uint32_t r) { return r > 1 ? ~0ull / r + 1 : 0; }
+// Select lines of many arrays (the levels of a forest) in one launch: job a
+// takes the workgroups [wg_start[a], wg_start[a] + 2 nchunks) (its ones'
+// chunks, then its zeros'; a chunk is kSelLineChunk 512-bit blocks), and
+// writes its n1 + n0 lines from line line_off of `out`.
+constexpr uint32_t kSelLineChunk = 256;
+struct SelLinesJob {
+  LevelView v;
+  uint64_t line_off, nl2, nchunks, n1;
+  uint32_t r1, r0;
+};
+void launch_multi_select_lines(const SelLinesJob* jobs, const uint64_t* wg_start, uint32_t njobs,
+                               uint64_t total_wgs, uint4* out, hipStream_t s);
 // Query lines and select hints of many arrays in three launches: job a's
 // lines and hints start at line_start[a] / hint_start[a] of the shared
 // buffers, and its device view d_view gets their addresses and hint_values

@@ -660,7 +660,7 @@ __device__ __forceinline__ uint64_t div_rate(uint64_t x, uint32_t r, uint64_t ma
 // with its own 16-byte stores touched 64 lines per store instruction: 1.08
 // ms per level of 6e9 bits; assembling each line in LDS by one thread put
 // the lanes' stores 128 bytes apart, on the same LDS banks: 1.24 ms.)
-constexpr uint32_t kSlBlocks = 256;                 // blocks (threads) per workgroup
+constexpr uint32_t kSlBlocks = kSelLineChunk;       // blocks (threads) per workgroup
 constexpr uint32_t kSlSrcWords = (kSlBlocks + 2) * 16;
 constexpr uint32_t kSlLines = 1024;                 // line starts recorded per round
 
@@ -774,10 +774,43 @@ void launch_select_lines(const LevelView& v, uint4* out, uint64_t n1, uint64_t n
   const uint64_t nl2 = (v.size + detail::kL2Bits - 1) / detail::kL2Bits;
   const uint64_t nchunks = (nl2 + kSlBlocks - 1) / kSlBlocks;
   if (2 * nchunks > 0x7FFFFFFFull) throw Error(ECL_ERROR_INVALID_ARGUMENT, "bit array too large");
-  const uint64_t magic1 = r1 > 1 ? ~0ull / r1 + 1 : 0;
-  const uint64_t magic0 = r0 > 1 ? ~0ull / r0 + 1 : 0;
   hipLaunchKernelGGL(select_lines_kernel, dim3(static_cast<uint32_t>(2 * nchunks)), dim3(kSlBlocks),
-                     0, s, v, out, nl2, nchunks, n1, r1, r0, magic1, magic0);
+                     0, s, v, out, nl2, nchunks, n1, r1, r0, select_line_magic(r1),
+                     select_line_magic(r0));
+  ECL_HIP(hipGetLastError());
+}
+
+// The select lines of many arrays: a workgroup finds its array by a search
+// over the arrays' first workgroups (an array without lines has none, and
+// shares its start with the next one) and builds one chunk of one Value, as
+// select_lines_kernel does.
+__global__ __launch_bounds__(kSlBlocks) void multi_select_lines_kernel(
+    const SelLinesJob* __restrict__ jobs, const uint64_t* __restrict__ wg_start, uint32_t njobs,
+    uint4* __restrict__ out) {
+  __shared__ __align__(16) uint32_t s_src[kSlSrcWords];
+  __shared__ uint64_t s_pos[kSlLines];
+  const uint32_t a = owner_job(wg_start, njobs, blockIdx.x);
+  const SelLinesJob& J = jobs[a];
+  const LevelView v = J.v;
+  const uint64_t w = blockIdx.x - wg_start[a];
+  const bool one = w < J.nchunks;
+  uint4* dst = out + J.line_off * 8;
+  if (one) {
+    if (J.r1 != 0)
+      select_lines_wg<1>(v, dst, J.nl2, w, 0, J.r1, select_line_magic(J.r1), s_src, s_pos);
+  } else {
+    if (J.r0 != 0)
+      select_lines_wg<0>(v, dst, J.nl2, w - J.nchunks, J.n1, J.r0, select_line_magic(J.r0), s_src,
+                         s_pos);
+  }
+}
+
+void launch_multi_select_lines(const SelLinesJob* jobs, const uint64_t* wg_start, uint32_t njobs,
+                               uint64_t total_wgs, uint4* out, hipStream_t s) {
+  if (njobs == 0 || total_wgs == 0) return;
+  if (total_wgs > 0x7FFFFFFFull) throw Error(ECL_ERROR_INVALID_ARGUMENT, "bit arrays too large");
+  hipLaunchKernelGGL(multi_select_lines_kernel, dim3(static_cast<uint32_t>(total_wgs)),
+                     dim3(kSlBlocks), 0, s, jobs, wg_start, njobs, out);
   ECL_HIP(hipGetLastError());
 }
 

@@ -498,6 +498,13 @@ ecl_status ecl_forest_autotune_build(ecl_forest f, uint32_t* tile_rows, float* m
  * forest query or graph replay is in flight.  No counterpart in the
  * reference. */
 ecl_status ecl_forest_build_query_lines(ecl_forest f);
+/* Select lines (see ecl_wavelet_tree_build_select_lines) for every tree of
+ * the forest, in one launch over all levels; later forest queries and graphs
+ * (also graphs captured before) read them, with or without the query lines.
+ * A build graph's replay drops them from the trees it rebuilds (a second
+ * call then builds them again).  Make the call while no forest query or
+ * graph replay is in flight.  No counterpart in the reference. */
+ecl_status ecl_forest_build_select_lines(ecl_forest f);
 ecl_status ecl_forest_num_trees(ecl_forest f, size_t* out);
 /* size, alphabet size and levels of tree i */
 ecl_status ecl_forest_tree_info(ecl_forest f, size_t i, uint64_t* n, uint64_t* sigma,

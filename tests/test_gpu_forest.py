@@ -243,6 +243,54 @@ def test_forest_query_lines(gpu):
         assert np.array_equal(dout.download(np.uint64, len(q)), e)
 
 
+def test_forest_select_lines(gpu):
+    """Select lines for every tree in one launch (alone, then with the query
+    lines and hints): mixed batches answer like the oracle, including
+    texts of one symbol (a level with no ones or no zeros) and texts shorter
+    than a 512-bit block, and a graph captured before the lines existed
+    replays with the same answers.  Every select of every tree's symbols
+    at its first, last and one-past-last occurrence is checked too."""
+    from ecl_wavelet_amd import DeviceBuffer, FOREST_QUERY_DTYPE, synchronize
+    texts, sigmas = _padded_texts(41, 50, lo_log2=4, hi_log2=19)
+    texts.append(np.full(3000, 3, dtype=np.uint8))  # one symbol of 4
+    sigmas.append(4)
+    texts.append(np.array([1, 0, 1], dtype=np.uint8))
+    sigmas.append(4)
+    f, _, _ = _forest(texts, sigmas)
+    q, e = _queries(texts, sigmas, 400, seed=41)
+    # the ends of every symbol's occurrences
+    extra, eexp = [], []
+    for ti, (t, s) in enumerate(zip(texts, sigmas)):
+        cnt = np.bincount(t, minlength=s)
+        for c in range(s):
+            for k in {1, int(cnt[c]), int(cnt[c]) + 1}:
+                if k == 0:
+                    continue
+                extra.append((2, ti, k, c))
+                eexp.append(O.scan_select_batch(t, np.array([k], dtype=np.uint64),
+                                                np.array([c], dtype=np.uint64))[0])
+    xq = np.zeros(len(extra), dtype=FOREST_QUERY_DTYPE)
+    xa = np.array(extra, dtype=np.uint64)
+    xq["kind"], xq["tree"], xq["arg"], xq["symbol"] = xa[:, 0], xa[:, 1], xa[:, 2], xa[:, 3]
+    q = np.concatenate([q, xq])
+    e = np.concatenate([e, np.array(eexp, dtype=np.uint64)])
+    dq = DeviceBuffer.from_array(q)
+    dout = DeviceBuffer(len(q) * 8)
+    g = f.graph(dq, len(q), dout)
+    f.buildSelectLines()
+    f.buildSelectLines()  # a second call keeps the lines
+    assert np.array_equal(_run(f, q), e)
+    f.buildQueryLines()
+    assert np.array_equal(_run(f, q), e)
+    zeros = np.zeros(len(q), dtype=np.uint64)
+    for _ in range(2):
+        dout.upload(zeros)
+        synchronize()
+        g.launch()
+        synchronize()
+        assert np.array_equal(dout.download(np.uint64, len(q)), e)
+
+
 def _rewrite(d, starts, texts):
     """The texts written in place into the forest's device buffer."""
     for t, s0 in zip(texts, starts):
