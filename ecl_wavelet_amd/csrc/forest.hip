@@ -688,6 +688,17 @@ ecl_status ecl_forest_build_query_lines(ecl_forest f) {
     bool have = na > 0;
     for (const LevelView& v : views) have = have && (v.lines != nullptr || v.size == 0);
     if (have && f->lines.ptr != nullptr) return;
+    // the views drop the old lines and hints before they are freed (an out-
+    // of-memory error below then leaves no view pointing at freed memory)
+    if (na) {
+      for (LevelView& v : views) {
+        v.lines = nullptr;
+        v.sel_hints = nullptr;
+        v.hint_values = 0;
+      }
+      h2d(reinterpret_cast<LevelView*>(f->arena.as<uint8_t>() + f->trees.front().lv_at),
+          views.data(), na);
+    }
     f->lines.reset();
     require(na < (1ull << 32), "too many bit arrays");
     std::vector<LinesJob> jobs(na);
