@@ -320,7 +320,23 @@ def bench_batched(args):
     batched_all_ms = float(np.median(runs))
     log(f"batched: select lines built in {sel_lines_ms:.1f} ms, {m} queries in "
         f"{batched_all_ms:.2f} ms over all lines")
-    del g, forest
+    # and over the quad layouts of the trees over 4, 64 and 256 symbols (a
+    # graph captured after they exist runs the pair-lane kernel)
+    timer.start()
+    forest.buildQuadLayouts()
+    quad_ms = timer.stop()
+    gq = forest.graph(d_q, m, d_out)
+    gq.launch()
+    E.synchronize()
+    runs = []
+    for _ in range(3):
+        timer.start()
+        gq.launch()
+        runs.append(timer.stop())
+    batched_quad_ms = float(np.median(runs))
+    log(f"batched: quad layouts built in {quad_ms:.1f} ms, {m} queries in "
+        f"{batched_quad_ms:.2f} ms over them")
+    del g, gq, forest
     # ---- plain: one tree per text, per-tree per-kind calls
     alph = {int(x): np.arange(int(x), dtype=np.uint8) for x in (4, 20, 64, 256)}
     timer.start()
@@ -387,6 +403,9 @@ def bench_batched(args):
         "select_lines_build_ms": sel_lines_ms,
         "batched_qps_all_lines": m / (batched_all_ms / 1e3),
         "batched_launch_ms_all_lines": batched_all_ms,
+        "quad_layouts_build_ms": quad_ms,
+        "batched_qps_quad_layouts": m / (batched_quad_ms / 1e3),
+        "batched_launch_ms_quad_layouts": batched_quad_ms,
         "plain_per_tree_p50_ms": float(np.percentile(lat, 50)),
         "plain_per_tree_p99_ms": float(np.percentile(lat, 99)),
         "autotuned": {"block": block, "grid": grid, "build_tile_rows": tile_rows},

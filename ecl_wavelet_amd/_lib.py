@@ -160,6 +160,7 @@ _SIGS = {
     "ecl_forest_autotune_build": (st, [vp, C.POINTER(u32), C.POINTER(C.c_float)]),
     "ecl_forest_build_query_lines": (st, [vp]),
     "ecl_forest_build_select_lines": (st, [vp]),
+    "ecl_forest_build_quad_layouts": (st, [vp]),
     "ecl_forest_num_trees": (st, [vp, C.POINTER(sz)]),
     "ecl_forest_tree_info": (st, [vp, sz, C.POINTER(u64), C.POINTER(u64), C.POINTER(u32)]),
     "ecl_forest_device_view": (st, [vp, sz, vp, sz]),

@@ -580,6 +580,14 @@ class Forest:
         the trees it rebuilds."""
         call("ecl_forest_build_select_lines", self._h)
 
+    def buildQuadLayouts(self):
+        """Quad lines and quad select lines for every tree over a power-of-two
+        alphabet with an even number of levels; later batches (and graphs
+        captured afterwards) run a pair of lanes per query over them, and
+        over the binary levels for the other trees, with the same answers.
+        A build graph's replay drops them from the trees it rebuilds."""
+        call("ecl_forest_build_quad_layouts", self._h)
+
     def treeInfo(self, i):
         n, s = C.c_uint64(), C.c_uint64()
         L = C.c_uint32()

@@ -155,6 +155,17 @@ __global__ void forest_views_kernel(const RsArrayDesc* __restrict__ desc, uint32
   *lv_ptr[j] = v;
 }
 
+// One thread per tree: its quad view is cleared (a rebuilt tree's quad
+// layouts no longer match its bits; it answers over its binary levels until
+// they are built again).
+__global__ void forest_clear_quad_kernel(TreeQueryParams* __restrict__ params, uint32_t n) {
+  const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  params[t].quad.levels = 0;
+  params[t].quad.has_sel = 0;
+  params[t].hex.levels = 0;
+}
+
 }  // namespace
 
 // The device buffers and launch parameters of one forest build: everything
@@ -195,6 +206,14 @@ struct ecl_forest_t {
   std::vector<LevelView*> d_levels;    // where each level's view lives in the arena
   DevBuf lines;                        // query lines and select hints (opt-in)
   DevBuf sel_lines;                    // select lines (opt-in)
+  // Quad layouts (opt-in): every tree's query parameters (its view, and its
+  // quad view once built; a build clears the quad views on the device, so a
+  // rebuilt tree answers over its binary levels), and the layouts' arrays.
+  DevBuf d_params;
+  std::vector<TreeQueryParams> params;
+  DevBuf quad_lines, quad_sel, quad_base, quad_tab;
+  bool quad_built = false;   // batches run the pair-lane kernel
+  uint32_t q_grid_quad = 0;  // its grid (256-thread workgroups)
   std::shared_ptr<ForestBuild> build;  // buffers of a build graph, while one lives
 };
 
@@ -444,7 +463,22 @@ void enqueue_build(ecl_forest_t* f, ForestBuild& B, hipStream_t s) {
                   B.d_blk.as<uint32_t>(), s);
   hipLaunchKernelGGL(forest_views_kernel, dim3((B.ndesc + 255) / 256), dim3(256), 0, s,
                      B.d_desc.as<RsArrayDesc>(), B.ndesc, B.d_lvptr.as<LevelView*>());
+  hipLaunchKernelGGL(forest_clear_quad_kernel, dim3((B.N + 255) / 256), dim3(256), 0, s,
+                     f->d_params.as<TreeQueryParams>(), B.N);
   ECL_HIP(hipGetLastError());
+}
+
+// A batch launch: the pair-lane kernel once the quad layouts were built
+// (trees without them run their binary levels there), else the binary
+// kernel with the forest's tuned shape.
+void launch_batch(const ecl_forest_t* f, const ecl_forest_query* q, uint64_t m, uint64_t* out,
+                  hipStream_t s) {
+  if (f->quad_built)
+    launch_forest_query_quad(f->d_views.as<TreeView>(), f->d_params.as<TreeQueryParams>(),
+                             f->trees.size(), q, m, out, f->q_grid_quad, s);
+  else
+    launch_forest_query(f->d_views.as<TreeView>(), f->trees.size(), q, m, out, f->q_block,
+                        f->q_grid, s);
 }
 
 // Waits for a build on s and checks the pass's bounds flag.
@@ -588,6 +622,14 @@ ecl_status ecl_forest_create(const void* d_texts, const uint64_t* starts, const 
     h2d(A + meta_at, img.data(), img.size());
     f->d_views = DevBuf(N * sizeof(TreeView));
     h2d(f->d_views.ptr, f->views.data(), N);
+    f->params.assign(N, TreeQueryParams{});
+    for (size_t i = 0; i < N; ++i) {
+      f->params[i].view = f->views[i];
+      f->params[i].esize = 1;
+      f->params[i].symbols_minimal = 1;
+    }
+    f->d_params = DevBuf(N * sizeof(TreeQueryParams));
+    h2d(f->d_params.ptr, f->params.data(), N);
     // ---- the build: one queue of launches, one wait at the end
     {
       std::shared_ptr<ForestBuild> B = make_build(f.get(), choose_rows(f.get(), tile_rows));
@@ -595,6 +637,7 @@ ecl_status ecl_forest_create(const void* d_texts, const uint64_t* starts, const 
       finish_build(*B, s);
     }
     f->q_grid = static_cast<uint32_t>(cu_count(device)) * 8;
+    f->q_grid_quad = f->q_grid;
     *out = f.release();
   });
 }
@@ -816,6 +859,210 @@ ecl_status ecl_forest_build_select_lines(ecl_forest f) {
   });
 }
 
+// Quad lines and quad select lines for every tree over a power-of-two
+// alphabet with an even number of levels (ecl_forest_build_quad_layouts).
+// Per tree and quad level the host tables follow from the tree's symbol
+// counts (one copy of the forest's metadata block), as for a single tree:
+// node starts S and digit counts CD for the build and the select ascent,
+// bases for the descents, the select lines' rates from the level's digit
+// totals.  The lines of all levels of all trees are then built in three
+// launches, and the trees' parameters get their quad views.
+ecl_status ecl_forest_build_quad_layouts(ecl_forest f) {
+  return guard([&] {
+    require(f, "NULL argument");
+    set_device(f->device);
+    const size_t N = f->trees.size();
+    auto eligible = [&](const TreeSlot& t) {
+      return t.shape->pow2 && t.L % 2 == 0 && t.L >= 2 && t.L <= 2 * kMaxQuadLevels;
+    };
+    // the parameters as the last build left them (a build clears the quad
+    // views of the trees it rebuilt)
+    std::vector<TreeQueryParams> cur(N);
+    d2h(cur.data(), f->d_params.as<TreeQueryParams>(), N);
+    bool have = f->quad_built;
+    for (size_t i = 0; i < N; ++i) have = have && (!eligible(f->trees[i]) || cur[i].quad.has_sel);
+    if (have) return;
+    // every tree answers over its binary levels before the old arrays go
+    for (size_t i = 0; i < N; ++i) {
+      f->params[i].quad = QuadView{};
+      f->params[i].hex = HexView{};
+    }
+    h2d(f->d_params.ptr, f->params.data(), N);
+    f->quad_lines.reset();
+    f->quad_sel.reset();
+    f->quad_base.reset();
+    f->quad_tab.reset();
+    // the symbol counts of every tree (the metadata block, one copy)
+    const uint64_t meta_at = f->trees.front().ones_at;
+    const TreeSlot& last = f->trees.back();
+    const uint64_t meta_end = last.lv_at + last.L * sizeof(LevelView);
+    std::vector<uint8_t> meta(meta_end - meta_at);
+    d2h(meta.data(), f->arena.as<uint8_t>() + meta_at, meta.size());
+    auto level_view = [&](const TreeSlot& t, uint32_t l) {
+      LevelView v;
+      std::memcpy(&v, meta.data() + (t.lv_at - meta_at) + l * sizeof(LevelView), sizeof(v));
+      return v;
+    };
+    // host tables and the jobs (one per tree and quad level)
+    std::vector<uint64_t> base, tab;
+    std::vector<QuadBuildParams> jobs;
+    std::vector<QuadSelBuildParams> sel_jobs;
+    std::vector<uint64_t> wg_start, sel_start;
+    std::vector<uint32_t> job_tree;
+    std::vector<uint64_t> tree_base(N, 0), tree_tab(N, 0);
+    std::vector<QuadView> qviews(N);
+    // per job: the first of its level's lines and select lines in the
+    // forest's arrays
+    std::vector<uint64_t> job_line, job_sel;
+    uint64_t nql_total = 0, sel_total = 0, wg_total = 0, selwg_total = 0;
+    for (size_t i = 0; i < N; ++i) {
+      const TreeSlot& t = f->trees[i];
+      if (!eligible(t)) continue;
+      const uint32_t L = t.L, Q = L / 2;
+      const uint64_t n = t.n, sigma = t.sigma;
+      std::vector<uint64_t> C(sigma + 1);
+      std::memcpy(C.data(), meta.data() + (t.counts_at - meta_at), sigma * 8);
+      C[sigma] = n;
+      QuadView& qv = qviews[i];
+      tree_base[i] = base.size();
+      tree_tab[i] = tab.size();
+      const uint64_t nql = n / kQuadDigits + 1;
+      for (uint32_t q = 0; q < Q; ++q) {
+        const uint64_t nn = 1ull << (2 * q);
+        const uint32_t shift = L - 2 * q - 2;
+        qv.off[q] = base.size() - tree_base[i];
+        qv.s_off[q] = tab.size() - tree_tab[i];
+        for (uint64_t Nd = 0; Nd <= nn; ++Nd)
+          tab.push_back(C[std::min<uint64_t>(Nd << (shift + 2), sigma)]);
+        qv.cd_off[q] = tab.size() - tree_tab[i];
+        uint64_t acc[4] = {0, 0, 0, 0};
+        for (uint64_t Nd = 0; Nd <= nn; ++Nd) {
+          for (uint32_t d = 0; d < 4; ++d) tab.push_back(acc[d]);
+          if (Nd == nn) break;
+          for (uint32_t d = 0; d < 4; ++d) {
+            const uint64_t lo = (4 * Nd + d) << shift, hi = (4 * Nd + d + 1) << shift;
+            base.push_back(C[lo] - acc[d]);
+            acc[d] += C[hi] - C[lo];
+          }
+        }
+        // the level's select lines: a line every r of each digit's acc[d]
+        uint64_t run = 0;
+        for (uint32_t d = 0; d < 4; ++d) {
+          const uint64_t tot = acc[d];
+          const uint64_t r = tot ? std::max<uint64_t>(1, kQuadSelSpan * tot / n) : 0;
+          qv.sel_rate[q][d] = static_cast<uint32_t>(r);
+          qv.sel_magic[q][d] = select_line_magic(static_cast<uint32_t>(r));
+          qv.sel_first[q][d] = run;
+          run += r ? tot / r + 1 : 0;
+        }
+        QuadBuildParams qp{};
+        qp.a = level_view(t, 2 * q);
+        qp.b = level_view(t, 2 * q + 1);
+        qp.nodes = nn;
+        qp.n = n;
+        qp.nlines = nql;
+        jobs.push_back(qp);  // table, line and scratch addresses below
+        QuadSelBuildParams sp{};
+        sp.nql = nql;
+        for (uint32_t d = 0; d < 4; ++d) {
+          sp.total[d] = acc[d];
+          sp.r[d] = qv.sel_rate[q][d];
+          sp.magic[d] = qv.sel_magic[q][d];
+          sp.nlines[d] = sp.r[d] ? acc[d] / sp.r[d] + 1 : 0;
+        }
+        sel_jobs.push_back(sp);
+        job_tree.push_back(static_cast<uint32_t>(i));
+        wg_start.push_back(wg_total);
+        sel_start.push_back(selwg_total);
+        wg_total += quad_build_workgroups(nql);
+        selwg_total += quad_select_build_workgroups(nql);
+        job_line.push_back(nql_total);
+        job_sel.push_back(sel_total);
+        nql_total += nql;
+        sel_total += run;
+      }
+      qv.levels = Q;
+      qv.nbase = base.size() - tree_base[i];
+      qv.ntab = tab.size() - tree_tab[i];
+      qv.has_sel = 1;
+    }
+    const size_t nj = jobs.size();
+    if (nj == 0) {
+      f->quad_built = true;
+      return;
+    }
+    require(nj < (1ull << 32), "too many quad levels");
+    const uint64_t bytes = (nql_total + sel_total) * 128 + (base.size() + tab.size()) * 8 +
+                           wg_total * kQuadWgInfo * 8 +
+                           nj * (sizeof(QuadBuildParams) + sizeof(QuadSelBuildParams) + 16);
+    if (bytes + (64u << 20) > free_device_memory())
+      throw Error(ECL_ERROR_OUT_OF_MEMORY_TREE, "Not enough memory available for the wavelet tree.");
+    DevBuf lines(nql_total * 128), sel(std::max<uint64_t>(sel_total, 1) * 128);
+    DevBuf dbase(std::max<size_t>(base.size(), 1) * 8), dtab(tab.size() * 8);
+    DevBuf wg_info(wg_total * kQuadWgInfo * 8);
+    DevBuf dlut(4096);
+    {
+      // the 4-position deposit table (QuadBuildParams::lut)
+      std::vector<uint8_t> lut(4096);
+      for (uint32_t x = 0; x < 4096; ++x) {
+        const uint32_t a = x & 15, z = (x >> 4) & 15, o = x >> 8;
+        uint32_t iz = 0, io = 0, v = 0;
+        for (uint32_t k = 0; k < 4; ++k) {
+          const uint32_t ak = (a >> k) & 1, bk = ak ? (o >> io++) & 1 : (z >> iz++) & 1;
+          v |= ((ak << 1) | bk) << (2 * k);
+        }
+        lut[x] = static_cast<uint8_t>(v);
+      }
+      h2d(dlut.ptr, lut.data(), 4096);
+    }
+    if (!base.empty()) h2d(dbase.ptr, base.data(), base.size());
+    h2d(dtab.ptr, tab.data(), tab.size());
+    // the device addresses of every view and job (a tree's jobs are its
+    // quad levels in order)
+    for (size_t i = 0; i < N; ++i) {
+      if (!qviews[i].levels) continue;
+      qviews[i].base = dbase.as<uint64_t>() + tree_base[i];
+      qviews[i].tab = dtab.as<uint64_t>() + tree_tab[i];
+    }
+    for (size_t j = 0, q = 0; j < nj; ++j) {
+      const uint32_t i = job_tree[j];
+      q = (j > 0 && job_tree[j - 1] == i) ? q + 1 : 0;
+      QuadView& qv = qviews[i];
+      qv.lines[q] = lines.as<uint4>() + job_line[j] * 8;
+      qv.sel_lines[q] = sel.as<uint4>() + job_sel[j] * 8;
+      QuadBuildParams& qp = jobs[j];
+      qp.S = qv.tab + qv.s_off[q];
+      qp.CD = qv.tab + qv.cd_off[q];
+      qp.out = const_cast<uint4*>(qv.lines[q]);
+      qp.wg_info = wg_info.as<uint64_t>() + wg_start[j] * kQuadWgInfo;
+      qp.lut = dlut.as<uint8_t>();
+      QuadSelBuildParams& sp = sel_jobs[j];
+      sp.qlines = qv.lines[q];
+      for (uint32_t d = 0; d < 4; ++d)
+        sp.out[d] = const_cast<uint4*>(qv.sel_lines[q]) + qv.sel_first[q][d] * 8;
+    }
+    DevBuf d_jobs(nj * sizeof(QuadBuildParams)), d_sel_jobs(nj * sizeof(QuadSelBuildParams));
+    DevBuf d_wg(nj * 8), d_selwg(nj * 8);
+    h2d(d_jobs.ptr, jobs.data(), nj);
+    h2d(d_sel_jobs.ptr, sel_jobs.data(), nj);
+    h2d(d_wg.ptr, wg_start.data(), nj);
+    h2d(d_selwg.ptr, sel_start.data(), nj);
+    hipStream_t s = nullptr;
+    launch_multi_quad_layouts(d_jobs.as<QuadBuildParams>(), d_sel_jobs.as<QuadSelBuildParams>(),
+                              d_wg.as<uint64_t>(), d_selwg.as<uint64_t>(),
+                              static_cast<uint32_t>(nj), wg_total, selwg_total, s);
+    ECL_HIP(hipStreamSynchronize(s));
+    for (size_t i = 0; i < N; ++i)
+      if (qviews[i].levels) f->params[i].quad = qviews[i];
+    h2d(f->d_params.ptr, f->params.data(), N);
+    f->quad_lines = std::move(lines);
+    f->quad_sel = std::move(sel);
+    f->quad_base = std::move(dbase);
+    f->quad_tab = std::move(dtab);
+    f->quad_built = true;
+  });
+}
+
 ecl_status ecl_forest_destroy(ecl_forest f) {
   return guard([&] {
     if (!f) return;
@@ -857,8 +1104,7 @@ ecl_status ecl_forest_query_device(ecl_forest f, const ecl_forest_query* d_queri
     if (m == 0) return;
     require(d_queries && d_out, "NULL argument");
     set_device(f->device);
-    launch_forest_query(f->d_views.as<TreeView>(), f->trees.size(), d_queries, m, d_out,
-                        f->q_block, f->q_grid, static_cast<hipStream_t>(stream));
+    launch_batch(f, d_queries, m, d_out, static_cast<hipStream_t>(stream));
   });
 }
 
@@ -941,8 +1187,7 @@ ecl_status ecl_forest_graph_create(ecl_forest f, const ecl_forest_query* d_queri
     ECL_HIP(hipStreamCreate(&s));
     try {
       ECL_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-      launch_forest_query(f->d_views.as<TreeView>(), f->trees.size(), d_queries, m, d_out,
-                          f->q_block, f->q_grid, s);
+      launch_batch(f, d_queries, m, d_out, s);
       ECL_HIP(hipStreamEndCapture(s, &g->graph));
       ECL_HIP(hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0));
     } catch (...) {

@@ -266,6 +266,8 @@ struct QuadSelBuildParams {
   uint4* out[4];
 };
 void launch_quad_select_lines(const QuadSelBuildParams& qp, hipStream_t s);
+// build workgroups of the quad select lines of a level of nql quad lines
+uint64_t quad_select_build_workgroups(uint64_t nql);
 
 // One quad level's build: binary levels a = 2q and b = 2q + 1, the level-2q
 // node starts S[0..nodes] (S[nodes] = n) and per node and digit the digits
@@ -287,6 +289,15 @@ struct QuadBuildParams {
 constexpr uint32_t kQuadWgInfo = 6;
 uint64_t quad_build_workgroups(uint64_t nlines);
 void launch_quad_lines(const QuadBuildParams& qp, hipStream_t s);
+// The quad lines and quad select lines of many quad levels (a forest's) in
+// three launches: job a's build workgroups are [wg_start[a], wg_start[a] +
+// quad_build_workgroups(nlines)) of the quad-line launches (its wg_info at
+// wg_start[a] * kQuadWgInfo of one scratch array) and [sel_start[a], ... +
+// quad_select_build_workgroups(nql)) of the select-line launch.
+void launch_multi_quad_layouts(const QuadBuildParams* jobs, const QuadSelBuildParams* sel_jobs,
+                               const uint64_t* wg_start, const uint64_t* sel_start,
+                               uint32_t njobs, uint64_t total_wg, uint64_t total_sel,
+                               hipStream_t s);
 
 // One hex level's build from the quad lines of quad levels 2h (qa) and
 // 2h + 1 (qb).  Position p of node N with quad digit d0 goes to position
@@ -351,6 +362,12 @@ void launch_segment_query_quad(const TreeView* views, const TreeQueryParams* par
                                uint64_t num_views, const ecl_query* q, uint64_t m,
                                const uint64_t* seg_start, const uint32_t* seg_tree, uint32_t nseg,
                                uint64_t* out, uint32_t grid, hipStream_t s);
+// The same pair-lane batch over a forest whose trees carry TreeQueryParams
+// (query q names its tree; trees without quad select lines run the binary
+// descents).
+void launch_forest_query_quad(const TreeView* views, const TreeQueryParams* params,
+                              uint64_t num_trees, const ecl_forest_query* q, uint64_t m,
+                              uint64_t* out, uint32_t grid, hipStream_t s);
 void launch_forest_gen(const TreeView* views, uint64_t num_trees, const uint64_t* qoff,
                        uint64_t seed, uint64_t m, ecl_forest_query* out, hipStream_t s);
 void launch_rs_rank(const LevelView* lv, int value, const uint64_t* idx,

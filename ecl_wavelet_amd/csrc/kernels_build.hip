@@ -866,7 +866,7 @@ __device__ __forceinline__ uint64_t quad_node_from(const uint64_t* S, uint64_t n
 // workgroup's range; a block scan of their ones gives every word's rank from
 // one directory rank at the range's start, and the node search runs once per
 // workgroup (a thread searches only past a node start).
-__global__ __launch_bounds__(256) void quad_lines_kernel(QuadBuildParams qp) {
+__device__ __forceinline__ void quad_lines_wg(const QuadBuildParams& qp, uint64_t blk) {
   constexpr uint32_t kWords = kQuadLinesPerWg * kQuadWordsPerLine;  // 896
   __shared__ __align__(16) uint32_t s_lines[kQuadLinesPerWg * 32];
   __shared__ uint32_t s_wrank[kWords];  // ones of level 2q before each word, from the range start
@@ -879,7 +879,7 @@ __global__ __launch_bounds__(256) void quad_lines_kernel(QuadBuildParams qp) {
   reinterpret_cast<uint4*>(s_lut)[threadIdx.x] =
       detail::gld(reinterpret_cast<const uint4*>(qp.lut), threadIdx.x);
   const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
-  const uint64_t line0 = static_cast<uint64_t>(blockIdx.x) * kQuadLinesPerWg;
+  const uint64_t line0 = blk * kQuadLinesPerWg;
   const uint64_t nl = qp.nlines - line0 < kQuadLinesPerWg ? qp.nlines - line0 : kQuadLinesPerWg;
   const uint32_t nwords = static_cast<uint32_t>(nl) * kQuadWordsPerLine;
   const uint64_t n = qp.n, nw = (n + 31) / 32;
@@ -906,7 +906,7 @@ __global__ __launch_bounds__(256) void quad_lines_kernel(QuadBuildParams qp) {
     if (lane >= off) incl += y;
   }
   if (lane == 63) s_wave[wave] = incl;
-  if (tid < kQuadWgInfo) s_info[tid] = detail::gld(qp.wg_info, blockIdx.x * kQuadWgInfo + tid);
+  if (tid < kQuadWgInfo) s_info[tid] = detail::gld(qp.wg_info, blk * kQuadWgInfo + tid);
   __syncthreads();
   uint32_t run = incl - pc;
   for (uint32_t w = 0; w < wave; ++w) run += s_wave[w];
@@ -1039,15 +1039,16 @@ __global__ __launch_bounds__(256) void quad_lines_kernel(QuadBuildParams qp) {
   uint4* out = qp.out + line0 * 8;
   for (uint32_t i = tid; i < nl * 8; i += blockDim.x) out[i] = reinterpret_cast<const uint4*>(s_lines)[i];
 }
+__global__ __launch_bounds__(256) void quad_lines_kernel(QuadBuildParams qp) {
+  quad_lines_wg(qp, blockIdx.x);
+}
 
 // Per build workgroup g (positions from P = 448 * 64 g): the ones of level
 // 2q before P, the node N holding P (the last position when P >= n), N's
 // start and end, the ones of level 2q before N and the zeros in N -- the
 // chain of dependent loads (directory rank, node search, tables) that would
 // otherwise hold up every build workgroup, here one thread each, all at once.
-__global__ __launch_bounds__(256) void quad_wg_kernel(QuadBuildParams qp, uint64_t nwg) {
-  const uint64_t g = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (g >= nwg) return;
+__device__ __forceinline__ void quad_wg_one(const QuadBuildParams& qp, uint64_t g) {
   const uint64_t P = g * kQuadLinesPerWg * kQuadDigits;
   const uint64_t N = quad_node(qp.S, qp.nodes, P < qp.n ? P : qp.n - 1);
   auto cd = [&](uint64_t M, uint32_t d) { return detail::gld(qp.CD, 4 * M + d); };
@@ -1058,6 +1059,10 @@ __global__ __launch_bounds__(256) void quad_wg_kernel(QuadBuildParams qp, uint64
   o[3] = detail::gld(qp.S, N + 1);
   o[4] = cd(N, 2) + cd(N, 3);
   o[5] = (cd(N + 1, 0) - cd(N, 0)) + (cd(N + 1, 1) - cd(N, 1));
+}
+__global__ __launch_bounds__(256) void quad_wg_kernel(QuadBuildParams qp, uint64_t nwg) {
+  const uint64_t g = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (g < nwg) quad_wg_one(qp, g);
 }
 
 uint64_t quad_build_workgroups(uint64_t nlines) {
@@ -1095,11 +1100,11 @@ __device__ __forceinline__ uint64_t qs_digit_word(const uint4* s_ql, uint32_t l,
   return (wi & 1) ? ((uint64_t)q.z | ((uint64_t)q.w << 32)) : ((uint64_t)q.x | ((uint64_t)q.y << 32));
 }
 
-__global__ __launch_bounds__(256) void quad_select_lines_kernel(QuadSelBuildParams qp) {
+__device__ __forceinline__ void quad_select_lines_wg(const QuadSelBuildParams& qp, uint64_t blk) {
   __shared__ __align__(16) uint4 s_ql[(kQsQuadLines + 1) * 8];
   __shared__ uint64_t s_pos[kQsPos];
   const uint32_t tid = threadIdx.x;
-  const uint64_t L0 = static_cast<uint64_t>(blockIdx.x) * kQsQuadLines;
+  const uint64_t L0 = blk * kQsQuadLines;
   for (uint32_t i = tid; i < (kQsQuadLines + 1) * 8; i += blockDim.x) {
     const uint64_t gi = L0 * 8 + i;
     s_ql[i] = gi < qp.nql * 8 ? detail::gld(qp.qlines, gi) : make_uint4(0, 0, 0, 0);
@@ -1175,11 +1180,69 @@ __global__ __launch_bounds__(256) void quad_select_lines_kernel(QuadSelBuildPara
   }
 }
 
+__global__ __launch_bounds__(256) void quad_select_lines_kernel(QuadSelBuildParams qp) {
+  quad_select_lines_wg(qp, blockIdx.x);
+}
+
 void launch_quad_select_lines(const QuadSelBuildParams& qp, hipStream_t s) {
   if (qp.nql == 0) return;
   const uint64_t blocks = (qp.nql + kQsQuadLines - 1) / kQsQuadLines;
   if (blocks > 0x7FFFFFFFull) throw Error(ECL_ERROR_INVALID_ARGUMENT, "bit array too large");
   hipLaunchKernelGGL(quad_select_lines_kernel, dim3(static_cast<uint32_t>(blocks)), dim3(256), 0, s, qp);
+  ECL_HIP(hipGetLastError());
+}
+
+uint64_t quad_select_build_workgroups(uint64_t nql) {
+  return (nql + kQsQuadLines - 1) / kQsQuadLines;
+}
+
+// The quad lines and quad select lines of many quad levels (those of the
+// trees of a forest) in three launches.  Job a (one quad level of one tree)
+// owns the build workgroups [wg_start[a], wg_start[a + 1]) of the quad-line
+// launches and [sel_start[a], sel_start[a + 1]) of the select-line launch;
+// a workgroup (a thread, for the per-workgroup information) finds its job
+// by a search over the starts and runs the one-level code on its part.
+__global__ __launch_bounds__(256) void multi_quad_wg_kernel(const QuadBuildParams* __restrict__ jobs,
+                                                            const uint64_t* __restrict__ wg_start,
+                                                            uint32_t njobs, uint64_t total_wg) {
+  const uint64_t t = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (t >= total_wg) return;
+  const uint32_t a = owner_job(wg_start, njobs, t);
+  const QuadBuildParams qp = jobs[a];
+  quad_wg_one(qp, t - wg_start[a]);
+}
+__global__ __launch_bounds__(256) void multi_quad_lines_kernel(const QuadBuildParams* __restrict__ jobs,
+                                                               const uint64_t* __restrict__ wg_start,
+                                                               uint32_t njobs) {
+  const uint32_t a = owner_job(wg_start, njobs, blockIdx.x);
+  const QuadBuildParams qp = jobs[a];
+  quad_lines_wg(qp, blockIdx.x - wg_start[a]);
+}
+__global__ __launch_bounds__(256) void multi_quad_select_lines_kernel(
+    const QuadSelBuildParams* __restrict__ jobs, const uint64_t* __restrict__ sel_start,
+    uint32_t njobs) {
+  const uint32_t a = owner_job(sel_start, njobs, blockIdx.x);
+  // (read in place: a copy's per-digit arrays, indexed at run time, would
+  // live in scratch memory)
+  quad_select_lines_wg(jobs[a], blockIdx.x - sel_start[a]);
+}
+
+void launch_multi_quad_layouts(const QuadBuildParams* jobs, const QuadSelBuildParams* sel_jobs,
+                               const uint64_t* wg_start, const uint64_t* sel_start,
+                               uint32_t njobs, uint64_t total_wg, uint64_t total_sel,
+                               hipStream_t s) {
+  if (njobs == 0) return;
+  if (total_wg > 0x7FFFFFFFull || total_sel > 0x7FFFFFFFull)
+    throw Error(ECL_ERROR_INVALID_ARGUMENT, "bit arrays too large");
+  if (total_wg) {
+    hipLaunchKernelGGL(multi_quad_wg_kernel, dim3(static_cast<uint32_t>((total_wg + 255) / 256)),
+                       dim3(256), 0, s, jobs, wg_start, njobs, total_wg);
+    hipLaunchKernelGGL(multi_quad_lines_kernel, dim3(static_cast<uint32_t>(total_wg)), dim3(256), 0,
+                       s, jobs, wg_start, njobs);
+  }
+  if (total_sel)
+    hipLaunchKernelGGL(multi_quad_select_lines_kernel, dim3(static_cast<uint32_t>(total_sel)),
+                       dim3(256), 0, s, sel_jobs, sel_start, njobs);
   ECL_HIP(hipGetLastError());
 }
 

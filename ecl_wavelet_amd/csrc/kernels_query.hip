@@ -643,52 +643,116 @@ __global__ void segment_query_kernel(const TreeView* __restrict__ views, uint64_
 // per query, each query's tree parameters read from global memory, and the
 // descents of the single-tree quad and hex kernels.  A tree without quad
 // select lines takes its binary descents (both lanes of the pair run them).
-__global__ __launch_bounds__(256) void segment_query_quad_kernel(
+// One chunk of a batch over many trees (its queries grouped by kind in
+// lists), a pair of lanes per query: trees with quad select lines run the
+// single-tree quad and hex descents over their parameters (params[t], from
+// global memory), the others the binary ones (access and rank fetching
+// half of each level's line or block per lane, select in both lanes).
+// tree_of(q, query) names the query's tree.
+template <typename Query, typename TreeOf>
+__device__ __forceinline__ void pair_chunk(const TreeQueryParams* __restrict__ params,
+                                           uint64_t num_views, const Query* __restrict__ cq,
+                                           uint64_t b0, const KindLists& lists,
+                                           uint64_t* __restrict__ out, TreeOf tree_of) {
+  const uint32_t pair = threadIdx.x >> 1, npairs = blockDim.x >> 1;
+  const uint32_t c0 = lists.cnt[0], c1 = lists.cnt[1], tot = c0 + c1 + lists.cnt[2];
+  for (uint32_t e = pair; e < tot; e += npairs) {
+    uint32_t j, k;
+    if (e < c0) { j = lists.idx[0][e]; k = 0; }
+    else if (e < c0 + c1) { j = lists.idx[1][e - c0]; k = 1; }
+    else { j = lists.idx[2][e - c0 - c1]; k = 2; }
+    const uint64_t q = b0 + j;
+    const Query qq = cq[j];
+    const uint64_t t = tree_of(q, qq);
+    uint64_t r = 0;
+    if (t < num_views) {
+      const TreeQueryParams& p = params[t];
+      if (p.quad.has_sel) {
+        if (k == 0) {
+          r = qq.arg < p.view.n
+                  ? (p.hex.levels ? hex_access(p, QuadTable<false>{p.hex.tab}, qq.arg)
+                                  : quad_access(p, QuadTable<false>{p.quad.base}, qq.arg))
+                  : 0;
+        } else if (k == 1) {
+          r = qq.arg <= p.view.n && qq.symbol < p.view.sigma
+                  ? (p.hex.levels ? hex_rank(p, QuadTable<false>{p.hex.tab}, qq.symbol, qq.arg)
+                                  : quad_rank(p, QuadTable<false>{p.quad.base}, qq.symbol, qq.arg))
+                  : 0;
+        } else {
+          r = quad_select(p, QuadTable<false>{p.quad.tab}, qq.symbol, qq.arg);
+        }
+      } else {
+        const TreeView& v = p.view;
+        if (k == 0)
+          r = qq.arg < v.n ? detail::tree_access<true>(v, v.levels, v.counts, v.ranks,
+                                                       v.nodes_at_level, qq.arg)
+                           : 0;
+        else if (k == 1)
+          r = qq.arg > v.n ? 0
+                           : detail::tree_rank<true>(v, v.levels, v.counts, v.ranks,
+                                                     v.nodes_at_level, qq.symbol, qq.arg);
+        else
+          r = detail::tree_select(v, v.levels, v.counts, v.ranks, v.nodes_at_level, qq.symbol,
+                                  qq.arg);
+      }
+    }
+    if ((threadIdx.x & 1) == 0) out[q] = r;
+  }
+  __syncthreads();
+}
+
+// (4 waves per SIMD, as forest_query_quad_kernel below)
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void segment_query_quad_kernel(
     const TreeView* __restrict__ views, const TreeQueryParams* __restrict__ params,
     uint64_t num_views, const ecl_query* __restrict__ qs, uint64_t m,
     const uint64_t* __restrict__ seg_start, const uint32_t* __restrict__ seg_tree, uint32_t nseg,
     uint64_t* __restrict__ out, uint32_t chunk) {
   __shared__ KindLists lists;
-  const uint32_t pair = threadIdx.x >> 1, npairs = blockDim.x >> 1;
   for (uint64_t b0 = static_cast<uint64_t>(blockIdx.x) * chunk; b0 < m;
        b0 += static_cast<uint64_t>(gridDim.x) * chunk) {
     const uint32_t len = static_cast<uint32_t>(m - b0 < chunk ? m - b0 : chunk);
     const ecl_query* cq = qs + b0;
     group_by_kind(lists, len, [&](uint32_t j) { return cq[j].kind; });
-    const uint32_t c0 = lists.cnt[0], c1 = lists.cnt[1], tot = c0 + c1 + lists.cnt[2];
-    for (uint32_t e = pair; e < tot; e += npairs) {
-      uint32_t j, k;
-      if (e < c0) { j = lists.idx[0][e]; k = 0; }
-      else if (e < c0 + c1) { j = lists.idx[1][e - c0]; k = 1; }
-      else { j = lists.idx[2][e - c0 - c1]; k = 2; }
-      const uint64_t q = b0 + j;
-      const ecl_query qq = cq[j];
-      const uint32_t t = seg_tree[segment_of(seg_start, nseg, q)];
-      uint64_t r = 0;
-      if (t < num_views) {
-        const TreeQueryParams& p = params[t];
-        if (p.quad.has_sel) {
-          if (k == 0) {
-            r = qq.arg < p.view.n
-                    ? (p.hex.levels ? hex_access(p, QuadTable<false>{p.hex.tab}, qq.arg)
-                                    : quad_access(p, QuadTable<false>{p.quad.base}, qq.arg))
-                    : 0;
-          } else if (k == 1) {
-            r = qq.arg <= p.view.n && qq.symbol < p.view.sigma
-                    ? (p.hex.levels
-                           ? hex_rank(p, QuadTable<false>{p.hex.tab}, qq.symbol, qq.arg)
-                           : quad_rank(p, QuadTable<false>{p.quad.base}, qq.symbol, qq.arg))
-                    : 0;
-          } else {
-            r = quad_select(p, QuadTable<false>{p.quad.tab}, qq.symbol, qq.arg);
-          }
-        } else {
-          r = forest_one(views, num_views, t, k, qq.arg, qq.symbol);
-        }
-      }
-      if ((threadIdx.x & 1) == 0) out[q] = r;
+    pair_chunk(params, num_views, cq, b0, lists, out,
+               [&](uint64_t q, const ecl_query&) -> uint64_t {
+                 return seg_tree[segment_of(seg_start, nseg, q)];
+               });
+  }
+}
+
+// A forest batch over the trees' quad layouts: query q names its tree.  A
+// chunk whose queries all name one tree without quad layouts runs as in
+// forest_query_kernel (the tree's metadata staged in LDS, a lane per
+// query); other chunks run pair_chunk.  Occupancy target: the compiler's
+// choice is about 170 VGPRs (2 waves per SIMD); 4 waves fit in 128 without
+// spills (a forest batch of 10^8 queries: 22.1 -> 14.2 ms).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void forest_query_quad_kernel(
+    const TreeView* __restrict__ views, const TreeQueryParams* __restrict__ params,
+    uint64_t num_trees, const ecl_forest_query* __restrict__ qs, uint64_t m,
+    uint64_t* __restrict__ out, uint32_t chunk) {
+  __shared__ KindLists lists;
+  __shared__ __align__(16) uint8_t s_meta[kTreeMetaLds];
+  for (uint64_t b0 = static_cast<uint64_t>(blockIdx.x) * chunk; b0 < m;
+       b0 += static_cast<uint64_t>(gridDim.x) * chunk) {
+    const uint32_t len = static_cast<uint32_t>(m - b0 < chunk ? m - b0 : chunk);
+    const ecl_forest_query* cq = qs + b0;
+    group_by_kind(lists, len, [&](uint32_t j) { return static_cast<uint32_t>(cq[j].kind); });
+    const uint32_t t0 = cq[0].tree;
+    bool other = false;
+    for (uint32_t j = threadIdx.x; j < len; j += blockDim.x) other |= cq[j].tree != t0;
+    const bool binary_tree = !__syncthreads_or(other) && t0 < num_trees &&
+                             !params[t0].quad.has_sel && meta_bytes_of(views[t0]) <= kTreeMetaLds;
+    if (binary_tree) {
+      const TreeView t = views[t0];
+      const Meta mt = stage_meta(t, s_meta);
+      run_by_kind(lists, [&](uint32_t j, uint32_t k) {
+        const ecl_forest_query e = cq[j];
+        out[b0 + j] = tree_one(t, mt, k, e.arg, e.symbol);
+      });
+    } else {
+      pair_chunk(params, num_trees, cq, b0, lists, out,
+                 [](uint64_t, const ecl_forest_query& e) -> uint64_t { return e.tree; });
     }
-    __syncthreads();
   }
 }
 
@@ -1025,6 +1089,17 @@ void launch_segment_query_quad(const TreeView* views, const TreeQueryParams* par
   check_kind_launch(256, grid);
   hipLaunchKernelGGL(segment_query_quad_kernel, dim3(grid), dim3(256), 0, s, views, params,
                      num_views, q, m, seg_start, seg_tree, nseg, out, kind_chunk(m, grid, 256));
+  ECL_HIP(hipGetLastError());
+}
+
+void launch_forest_query_quad(const TreeView* views, const TreeQueryParams* params,
+                              uint64_t num_trees, const ecl_forest_query* q, uint64_t m,
+                              uint64_t* out, uint32_t grid, hipStream_t s) {
+  if (m == 0) return;
+  ProfScope ps(K_FOREST_QUERY, s);
+  check_kind_launch(256, grid);
+  hipLaunchKernelGGL(forest_query_quad_kernel, dim3(grid), dim3(256), 0, s, views, params,
+                     num_trees, q, m, out, kind_chunk(m, grid, 256));
   ECL_HIP(hipGetLastError());
 }
 

@@ -505,6 +505,18 @@ ecl_status ecl_forest_build_query_lines(ecl_forest f);
  * call then builds them again).  Make the call while no forest query or
  * graph replay is in flight.  No counterpart in the reference. */
 ecl_status ecl_forest_build_select_lines(ecl_forest f);
+/* Quad lines and quad select lines (see ecl_wavelet_tree_build_quad_lines
+ * and _quad_select_lines) for every tree of the forest over a power-of-two
+ * alphabet with an even number of levels (sigma = 4, 16, 64, 256), in three
+ * launches over all their levels.  Later forest batches (ecl_forest_query_device
+ * and graphs captured afterwards) then run a pair of lanes per query: over
+ * these layouts for those trees, over the binary levels (with their lines,
+ * when built) for the others.  A build or build-graph replay drops the
+ * layouts of the trees it rebuilds (they answer over their binary levels
+ * until a second call builds them again).  Graphs captured before the call
+ * keep the binary kernel.  Make the call while no forest query or graph
+ * replay is in flight.  No counterpart in the reference. */
+ecl_status ecl_forest_build_quad_layouts(ecl_forest f);
 ecl_status ecl_forest_num_trees(ecl_forest f, size_t* out);
 /* size, alphabet size and levels of tree i */
 ecl_status ecl_forest_tree_info(ecl_forest f, size_t i, uint64_t* n, uint64_t* sigma,

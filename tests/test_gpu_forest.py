@@ -291,6 +291,90 @@ def test_forest_select_lines(gpu):
         assert np.array_equal(dout.download(np.uint64, len(q)), e)
 
 
+def _select_ends(texts, sigmas):
+    """Selects of every symbol of every tree at its first, last and
+    one-past-last occurrence, with their direct-scan answers."""
+    from ecl_wavelet_amd import FOREST_QUERY_DTYPE
+    extra, exp = [], []
+    for ti, (t, s) in enumerate(zip(texts, sigmas)):
+        cnt = np.bincount(t, minlength=s)
+        for c in range(s):
+            for k in sorted({1, int(cnt[c]), int(cnt[c]) + 1} - {0}):
+                extra.append((2, ti, k, c))
+    xa = np.array(extra, dtype=np.uint64)
+    q = np.zeros(len(extra), dtype=FOREST_QUERY_DTYPE)
+    q["kind"], q["tree"], q["arg"], q["symbol"] = xa[:, 0], xa[:, 1], xa[:, 2], xa[:, 3]
+    for ti, t in enumerate(texts):
+        sel = xa[:, 1] == ti
+        exp.append((np.nonzero(sel)[0], O.scan_select_batch(t, xa[sel, 2], xa[sel, 3])))
+    e = np.zeros(len(extra), dtype=np.uint64)
+    for idx, v in exp:
+        e[idx] = v
+    return q, e
+
+
+def test_forest_quad_layouts(gpu):
+    """Quad lines and quad select lines for the trees over 4, 16, 64 and 256
+    symbols (three launches over all their levels); the trees over 20
+    symbols keep their binary levels in the same pair-lane kernel.  Mixed
+    batches and every symbol's first, last and one-past-last select answer
+    like the oracle: texts of one symbol, of 3 symbols, of exactly one and
+    one more than one quad line (448 and 449), skewed ones.  A graph captured
+    before the call keeps the binary kernel, one captured after runs the
+    pair-lane kernel; both replay with the oracle's answers.  A build-graph
+    replay over new texts drops the layouts of the rebuilt trees (answers
+    stay right), and a second call builds them again."""
+    from ecl_wavelet_amd import DeviceBuffer, synchronize
+    texts, sigmas = _padded_texts(51, 40, lo_log2=4, hi_log2=18)
+    rng = np.random.default_rng(52)
+    for n, s in ((3000, 4), (3, 4), (448, 16), (449, 64), (20000, 16), (70000, 256)):
+        if n == 3000:
+            t = np.full(n, 3, dtype=np.uint8)
+        elif n == 3:
+            t = np.array([1, 0, 1], dtype=np.uint8)
+        elif s == 256:
+            t = O.gen_cdf(53, n, O.zipf_thresholds(s, 1.5), np.uint8).astype(np.uint8)
+        else:
+            t = rng.integers(0, s, n).astype(np.uint8)
+        texts.append(t)
+        sigmas.append(s)
+    f, d, starts = _forest(texts, sigmas)
+    q, e = _queries(texts, sigmas, 300, seed=51)
+    xq, xe = _select_ends(texts, sigmas)
+    q, e = np.concatenate([q, xq]), np.concatenate([e, xe])
+    dq = DeviceBuffer.from_array(q)
+    dout = DeviceBuffer(len(q) * 8)
+    g_before = f.graph(dq, len(q), dout)
+    f.buildQuadLayouts()
+    f.buildQuadLayouts()  # a second call keeps them
+    assert np.array_equal(_run(f, q), e)
+    f.buildQueryLines()
+    f.buildSelectLines()
+    assert np.array_equal(_run(f, q), e)
+    g_after = f.graph(dq, len(q), dout)
+    zeros = np.zeros(len(q), dtype=np.uint64)
+    for g in (g_before, g_after, g_after):
+        dout.upload(zeros)
+        synchronize()
+        g.launch()
+        synchronize()
+        assert np.array_equal(dout.download(np.uint64, len(q)), e)
+    # new texts, rebuilt by a build-graph replay: the layouts are dropped
+    gb = f.buildGraph()
+    new = [rng.integers(0, s, len(t)).astype(np.uint8) for t, s in zip(texts, sigmas)]
+    _rewrite(d, starts, new)
+    synchronize()
+    gb.launch()
+    synchronize()
+    q2, e2 = _queries(new, sigmas, 300, seed=54)
+    xq2, xe2 = _select_ends(new, sigmas)
+    q2, e2 = np.concatenate([q2, xq2]), np.concatenate([e2, xe2])
+    assert np.array_equal(_run(f, q2), e2)
+    f.buildQuadLayouts()
+    assert np.array_equal(_run(f, q2), e2)
+    del g_before, g_after, gb
+
+
 def _rewrite(d, starts, texts):
     """The texts written in place into the forest's device buffer."""
     for t, s0 in zip(texts, starts):
