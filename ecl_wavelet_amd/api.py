@@ -582,7 +582,8 @@ class Forest:
 
     def buildQuadLayouts(self):
         """Quad lines and quad select lines for every tree over a power-of-two
-        alphabet with an even number of levels; later batches (and graphs
+        alphabet with an even number of levels (and hex lines for those over
+        16 or 256 symbols); later batches (and graphs
         captured afterwards) run a pair of lanes per query over them, and
         over the binary levels for the other trees, with the same answers.
         A build graph's replay drops them from the trees it rebuilds."""

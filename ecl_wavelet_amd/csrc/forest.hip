@@ -211,7 +211,7 @@ struct ecl_forest_t {
   // rebuilt tree answers over its binary levels), and the layouts' arrays.
   DevBuf d_params;
   std::vector<TreeQueryParams> params;
-  DevBuf quad_lines, quad_sel, quad_base, quad_tab;
+  DevBuf quad_lines, quad_sel, quad_base, quad_tab, hex_lines, hex_tab;
   bool quad_built = false;   // batches run the pair-lane kernel
   uint32_t q_grid_quad = 0;  // its grid (256-thread workgroups)
   std::shared_ptr<ForestBuild> build;  // buffers of a build graph, while one lives
@@ -892,6 +892,8 @@ ecl_status ecl_forest_build_quad_layouts(ecl_forest f) {
     f->quad_sel.reset();
     f->quad_base.reset();
     f->quad_tab.reset();
+    f->hex_lines.reset();
+    f->hex_tab.reset();
     // the symbol counts of every tree (the metadata block, one copy)
     const uint64_t meta_at = f->trees.front().ones_at;
     const TreeSlot& last = f->trees.back();
@@ -986,15 +988,76 @@ ecl_status ecl_forest_build_quad_layouts(ecl_forest f) {
       qv.ntab = tab.size() - tree_tab[i];
       qv.has_sel = 1;
     }
-    const size_t nj = jobs.size();
+    // hex lines for the trees whose level count is a multiple of 4, from
+    // their quad lines: per hex level the bases and (built on the device)
+    // superblock counts, kept; the x's before each node (CHD) and the node
+    // starts, for the build only
+    std::vector<uint64_t> htab, hbtab;
+    std::vector<HexBuildParams> hjobs;
+    std::vector<uint32_t> hjob_tree;
+    std::vector<uint64_t> hwg_start, hjob_line, hjob_chd, hjob_s, tree_htab(N, 0);
+    std::vector<HexView> hviews(N);
+    uint64_t nhl_total = 0, hwg_total = 0;
+    for (size_t i = 0; i < N; ++i) {
+      const TreeSlot& t = f->trees[i];
+      if (!eligible(t) || t.L % 4 != 0) continue;
+      const uint32_t L = t.L, H = L / 4;
+      const uint64_t n = t.n, sigma = t.sigma;
+      std::vector<uint64_t> C(sigma + 1);
+      std::memcpy(C.data(), meta.data() + (t.counts_at - meta_at), sigma * 8);
+      C[sigma] = n;
+      const uint64_t nlines = n / kHexDigits + 1;
+      const uint64_t nsb = (nlines >> kHexSbShift) + 1;
+      HexView& hv = hviews[i];
+      tree_htab[i] = htab.size();
+      for (uint32_t h = 0; h < H; ++h) {
+        const uint64_t nn = 1ull << (4 * h);
+        const uint32_t shift = L - 4 * h - 4;
+        hv.base_off[h] = htab.size() - tree_htab[i];
+        hjob_chd.push_back(hbtab.size());
+        uint64_t acc[16] = {};
+        for (uint64_t Nd = 0; Nd <= nn; ++Nd) {
+          for (uint32_t x = 0; x < 16; ++x) hbtab.push_back(acc[x]);
+          if (Nd == nn) break;
+          for (uint32_t x = 0; x < 16; ++x) {
+            const uint64_t lo = (16 * Nd + x) << shift, hi = (16 * Nd + x + 1) << shift;
+            htab.push_back(C[lo] - acc[x]);
+            acc[x] += C[hi] - C[lo];
+          }
+        }
+        hjob_s.push_back(hbtab.size());
+        for (uint64_t Nd = 0; Nd <= nn; ++Nd)
+          hbtab.push_back(C[std::min<uint64_t>(Nd << (shift + 4), sigma)]);
+        HexBuildParams hp{};
+        hp.nodes = nn;
+        hp.n = n;
+        hp.nql = n / kQuadDigits + 1;
+        hp.nlines = nlines;
+        hjobs.push_back(hp);  // addresses below
+        hjob_tree.push_back(static_cast<uint32_t>(i));
+        hjob_line.push_back(nhl_total);
+        hwg_start.push_back(hwg_total);
+        nhl_total += nlines;
+        hwg_total += hex_build_workgroups(nlines);
+      }
+      for (uint32_t h = 0; h < H; ++h) {
+        hv.sb_off[h] = htab.size() - tree_htab[i];
+        htab.resize(htab.size() + 16 * nsb, 0);  // written by the build kernel
+      }
+      hv.levels = H;
+      hv.ntab = htab.size() - tree_htab[i];
+    }
+    const size_t nj = jobs.size(), nh = hjobs.size();
     if (nj == 0) {
       f->quad_built = true;
       return;
     }
-    require(nj < (1ull << 32), "too many quad levels");
-    const uint64_t bytes = (nql_total + sel_total) * 128 + (base.size() + tab.size()) * 8 +
+    require(nj < (1ull << 32) && nh < (1ull << 32), "too many quad levels");
+    const uint64_t bytes = (nql_total + sel_total + nhl_total) * 128 +
+                           (base.size() + tab.size() + htab.size() + hbtab.size()) * 8 +
                            wg_total * kQuadWgInfo * 8 +
-                           nj * (sizeof(QuadBuildParams) + sizeof(QuadSelBuildParams) + 16);
+                           nj * (sizeof(QuadBuildParams) + sizeof(QuadSelBuildParams) + 16) +
+                           nh * (sizeof(HexBuildParams) + 8);
     if (bytes + (64u << 20) > free_device_memory())
       throw Error(ECL_ERROR_OUT_OF_MEMORY_TREE, "Not enough memory available for the wavelet tree.");
     DevBuf lines(nql_total * 128), sel(std::max<uint64_t>(sel_total, 1) * 128);
@@ -1051,14 +1114,46 @@ ecl_status ecl_forest_build_quad_layouts(ecl_forest f) {
     launch_multi_quad_layouts(d_jobs.as<QuadBuildParams>(), d_sel_jobs.as<QuadSelBuildParams>(),
                               d_wg.as<uint64_t>(), d_selwg.as<uint64_t>(),
                               static_cast<uint32_t>(nj), wg_total, selwg_total, s);
+    DevBuf hlines(std::max<uint64_t>(nhl_total, 1) * 128);
+    DevBuf dhtab(std::max<size_t>(htab.size(), 1) * 8), dhbtab(std::max<size_t>(hbtab.size(), 1) * 8);
+    DevBuf d_hjobs(std::max<size_t>(nh, 1) * sizeof(HexBuildParams)), d_hwg(std::max<size_t>(nh, 1) * 8);
+    if (nh) {
+      h2d(dhtab.ptr, htab.data(), htab.size());
+      h2d(dhbtab.ptr, hbtab.data(), hbtab.size());
+      for (size_t j = 0, h = 0; j < nh; ++j) {
+        const uint32_t i = hjob_tree[j];
+        h = (j > 0 && hjob_tree[j - 1] == i) ? h + 1 : 0;
+        const QuadView& qv = qviews[i];
+        HexView& hv = hviews[i];
+        HexBuildParams& hp = hjobs[j];
+        hp.qa = qv.lines[2 * h];
+        hp.qb = qv.lines[2 * h + 1];
+        hp.qbase = qv.base + qv.off[2 * h];
+        hp.qcd = qv.tab + qv.cd_off[2 * h + 1];
+        hp.chd = dhbtab.as<uint64_t>() + hjob_chd[j];
+        hp.S = dhbtab.as<uint64_t>() + hjob_s[j];
+        hp.out = hlines.as<uint4>() + hjob_line[j] * 8;
+        hp.sb = dhtab.as<uint64_t>() + tree_htab[i] + hv.sb_off[h];
+        hv.lines[h] = hp.out;
+        hv.tab = dhtab.as<uint64_t>() + tree_htab[i];
+      }
+      h2d(d_hjobs.ptr, hjobs.data(), nh);
+      h2d(d_hwg.ptr, hwg_start.data(), nh);
+      launch_multi_hex_lines(d_hjobs.as<HexBuildParams>(), d_hwg.as<uint64_t>(),
+                             static_cast<uint32_t>(nh), hwg_total, s);
+    }
     ECL_HIP(hipStreamSynchronize(s));
-    for (size_t i = 0; i < N; ++i)
+    for (size_t i = 0; i < N; ++i) {
       if (qviews[i].levels) f->params[i].quad = qviews[i];
+      if (hviews[i].levels) f->params[i].hex = hviews[i];
+    }
     h2d(f->d_params.ptr, f->params.data(), N);
     f->quad_lines = std::move(lines);
     f->quad_sel = std::move(sel);
     f->quad_base = std::move(dbase);
     f->quad_tab = std::move(dtab);
+    f->hex_lines = std::move(hlines);
+    f->hex_tab = std::move(dhtab);
     f->quad_built = true;
   });
 }

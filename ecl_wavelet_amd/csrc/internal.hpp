@@ -319,6 +319,11 @@ struct HexBuildParams {
   uint64_t* sb;           // superblock counts: sb[16 s + x]
 };
 void launch_hex_lines(const HexBuildParams& hp, hipStream_t s);
+// The hex lines of many hex levels (a forest's) in one launch: job a's build
+// workgroups are [wg_start[a], wg_start[a] + hex_build_workgroups(nlines)).
+uint64_t hex_build_workgroups(uint64_t nlines);
+void launch_multi_hex_lines(const HexBuildParams* jobs, const uint64_t* wg_start, uint32_t njobs,
+                            uint64_t total_wg, hipStream_t s);
 
 struct TreeQueryParams {
   TreeView view;

@@ -1457,8 +1457,7 @@ __device__ __forceinline__ void wg_excl_scan_fields(uint64_t (&v)[J], uint64_t* 
 // Occupancy target: 7 waves per SIMD (70 VGPRs, no spills; 8 waves spill 7
 // and are no faster, 12.0 vs 12.1 ms for both levels of n = 6e9, sigma =
 // 256; the compiler's own choice is 5 waves).
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void hex_lines_kernel(
-    HexBuildParams hp) {
+__device__ __forceinline__ void hex_lines_wg(const HexBuildParams& hp, uint64_t blk) {
   __shared__ __align__(16) uint4 s_qa[kHexQaLines * 8];
   __shared__ __align__(16) uint4 s_qb[kHexQbLines * 8];
   __shared__ uint64_t s_w[16];
@@ -1485,7 +1484,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void h
     }
     s_dlut[tid] = e | (cnt[0] << 16) | (cnt[1] << 19) | (cnt[2] << 22) | (cnt[3] << 25);
   }
-  const uint64_t L0 = static_cast<uint64_t>(blockIdx.x) * kHexWgLines;
+  const uint64_t L0 = blk * kHexWgLines;
   const uint64_t P0 = L0 * kHexDigits;
   const uint64_t n = hp.n;
   const uint64_t E = P0 + kHexWgPos < n ? P0 + kHexWgPos : n;  // <= P0 only when P0 >= n
@@ -1499,7 +1498,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void h
     na1 += __syncthreads_count(E > P0 && sk <= E - 1);
   }
   if (P0 >= n || na0 != na1) {  // the whole workgroup
-    hex_lane(hp, static_cast<uint64_t>(blockIdx.x) * blockDim.x + tid);
+    hex_lane(hp, blk * blockDim.x + tid);
     return;
   }
   const uint64_t N = na0;
@@ -1563,7 +1562,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void h
   __syncthreads();
   const uint32_t nst = static_cast<uint32_t>(s_info[0]);
   if (nst > kHexQbLines) {  // not reached (at most 26); the whole workgroup
-    hex_lane(hp, static_cast<uint64_t>(blockIdx.x) * blockDim.x + tid);
+    hex_lane(hp, blk * blockDim.x + tid);
     return;
   }
   for (uint32_t i = tid; i < nst * 8; i += blockDim.x) {
@@ -1670,6 +1669,30 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void h
       for (uint32_t j = 0; j < 4; ++j) hp.sb[(l >> kHexSbShift) * 16 + 4 * q + j] = cnt[j];
     }
   }
+}
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void hex_lines_kernel(
+    HexBuildParams hp) {
+  hex_lines_wg(hp, blockIdx.x);
+}
+// The hex lines of many hex levels (a forest's): job a owns the build
+// workgroups [wg_start[a], wg_start[a + 1]); a workgroup finds its job by a
+// search over the starts and runs the one-level code on its part.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(7))) void multi_hex_lines_kernel(
+    const HexBuildParams* __restrict__ jobs, const uint64_t* __restrict__ wg_start, uint32_t njobs) {
+  const uint32_t a = owner_job(wg_start, njobs, blockIdx.x);
+  const HexBuildParams hp = jobs[a];
+  hex_lines_wg(hp, blockIdx.x - wg_start[a]);
+}
+
+uint64_t hex_build_workgroups(uint64_t nlines) { return (nlines + kHexWgLines - 1) / kHexWgLines; }
+
+void launch_multi_hex_lines(const HexBuildParams* jobs, const uint64_t* wg_start, uint32_t njobs,
+                            uint64_t total_wg, hipStream_t s) {
+  if (njobs == 0 || total_wg == 0) return;
+  if (total_wg > 0x7FFFFFFFull) throw Error(ECL_ERROR_INVALID_ARGUMENT, "bit arrays too large");
+  hipLaunchKernelGGL(multi_hex_lines_kernel, dim3(static_cast<uint32_t>(total_wg)), dim3(256), 0, s,
+                     jobs, wg_start, njobs);
+  ECL_HIP(hipGetLastError());
 }
 
 void launch_hex_lines(const HexBuildParams& hp, hipStream_t s) {

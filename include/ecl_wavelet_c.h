@@ -508,7 +508,8 @@ ecl_status ecl_forest_build_select_lines(ecl_forest f);
 /* Quad lines and quad select lines (see ecl_wavelet_tree_build_quad_lines
  * and _quad_select_lines) for every tree of the forest over a power-of-two
  * alphabet with an even number of levels (sigma = 4, 16, 64, 256), in three
- * launches over all their levels.  Later forest batches (ecl_forest_query_device
+ * launches over all their levels, and hex lines (one more launch) for those
+ * whose level count is a multiple of 4 (sigma = 16, 256).  Later forest batches (ecl_forest_query_device
  * and graphs captured afterwards) then run a pair of lanes per query: over
  * these layouts for those trees, over the binary levels (with their lines,
  * when built) for the others.  A build or build-graph replay drops the

@@ -315,7 +315,8 @@ def _select_ends(texts, sigmas):
 
 def test_forest_quad_layouts(gpu):
     """Quad lines and quad select lines for the trees over 4, 16, 64 and 256
-    symbols (three launches over all their levels); the trees over 20
+    symbols (three launches over all their levels), hex lines for those over
+    16 and 256 (one more launch); the trees over 20
     symbols keep their binary levels in the same pair-lane kernel.  Mixed
     batches and every symbol's first, last and one-past-last select answer
     like the oracle: texts of one symbol, of 3 symbols, of exactly one and
