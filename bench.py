@@ -231,7 +231,9 @@ def bench_batched(args):
     HIP graph.  Plain form: one build per text through the reference-shaped
     API and three device calls (access, rank, select) per tree.  Both over the
     plain directory; the batched graph is then replayed over the forest's
-    query lines and select hints (reported beside)."""
+    query lines and select hints, then with its select lines too, and a graph
+    captured after the quad layouts and hex lines exist is replayed over them
+    (reported beside)."""
     import ecl_wavelet_amd as E
     rng = np.random.default_rng(9)
     N = args.bt_texts
