@@ -327,6 +327,7 @@ def bench_batched(args):
     timer.start()
     forest.buildQuadLayouts()
     quad_ms = timer.stop()
+    qblock, qgrid = forest.autotune(d_q, m, d_out)  # now the pair-lane kernel's grid
     gq = forest.graph(d_q, m, d_out)
     gq.launch()
     E.synchronize()
@@ -337,7 +338,7 @@ def bench_batched(args):
         runs.append(timer.stop())
     batched_quad_ms = float(np.median(runs))
     log(f"batched: quad layouts built in {quad_ms:.1f} ms, {m} queries in "
-        f"{batched_quad_ms:.2f} ms over them")
+        f"{batched_quad_ms:.2f} ms over them (grid {qgrid})")
     del g, gq, forest
     # ---- plain: one tree per text, per-tree per-kind calls
     alph = {int(x): np.arange(int(x), dtype=np.uint8) for x in (4, 20, 64, 256)}

@@ -1230,21 +1230,30 @@ ecl_status ecl_forest_autotune(ecl_forest f, const ecl_forest_query* d_queries, 
     set_device(f->device);
     const uint64_t sample = std::min<uint64_t>(m, 1ull << 22);
     const uint32_t cus = static_cast<uint32_t>(cu_count(f->device));
+    const bool quad = f->quad_built;  // the kernel the batches launch
     hipEvent_t e0, e1;
     ECL_HIP(hipEventCreate(&e0));
     ECL_HIP(hipEventCreate(&e1));
     float best = 1e30f;
-    uint32_t bb = f->q_block, bg = f->q_grid;
+    uint32_t bb = quad ? 256u : f->q_block, bg = quad ? f->q_grid_quad : f->q_grid;
+    auto launch = [&](uint32_t b, uint32_t g) {
+      if (quad)
+        launch_forest_query_quad(f->d_views.as<TreeView>(), f->d_params.as<TreeQueryParams>(),
+                                 f->trees.size(), d_queries, sample, d_out, g, nullptr);
+      else
+        launch_forest_query(f->d_views.as<TreeView>(), f->trees.size(), d_queries, sample, d_out,
+                            b, g, nullptr);
+    };
     try {
-      for (uint32_t b : {64u, 128u, 256u, 512u}) {
+      // the pair-lane kernel runs 256-thread workgroups
+      const std::vector<uint32_t> blocks =
+          quad ? std::vector<uint32_t>{256u} : std::vector<uint32_t>{64u, 128u, 256u, 512u};
+      for (uint32_t b : blocks) {
         for (uint32_t per_cu : {1u, 2u, 4u, 8u, 16u, 32u}) {
           const uint32_t g = cus * per_cu * 256 / b;  // same waves per CU across block sizes
-          launch_forest_query(f->d_views.as<TreeView>(), f->trees.size(), d_queries, sample, d_out,
-                              b, g, nullptr);
+          launch(b, g);
           ECL_HIP(hipEventRecord(e0, nullptr));
-          for (int r = 0; r < 3; ++r)
-            launch_forest_query(f->d_views.as<TreeView>(), f->trees.size(), d_queries, sample,
-                                d_out, b, g, nullptr);
+          for (int r = 0; r < 3; ++r) launch(b, g);
           ECL_HIP(hipEventRecord(e1, nullptr));
           ECL_HIP(hipEventSynchronize(e1));
           float ms = 0.f;
@@ -1263,8 +1272,12 @@ ecl_status ecl_forest_autotune(ecl_forest f, const ecl_forest_query* d_queries, 
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
-    f->q_block = bb;
-    f->q_grid = bg;
+    if (quad) {
+      f->q_grid_quad = bg;
+    } else {
+      f->q_block = bb;
+      f->q_grid = bg;
+    }
     if (block) *block = bb;
     if (grid) *grid = bg;
   });

@@ -534,7 +534,9 @@ ecl_status ecl_forest_query_device(ecl_forest f, const ecl_forest_query* d_queri
 ecl_status ecl_forest_generate_queries(ecl_forest f, const uint64_t* qoff, uint64_t seed,
                                        ecl_forest_query* d_out, void* stream);
 /* Times the batch query launch over workgroup sizes and grid sizes on (a
- * prefix of) the given batch and keeps the fastest for this forest. */
+ * prefix of) the given batch and keeps the fastest for this forest.  Once
+ * the quad layouts exist it tunes the pair-lane kernel the batches then run
+ * (256-thread workgroups: grid sizes only). */
 ecl_status ecl_forest_autotune(ecl_forest f, const ecl_forest_query* d_queries, size_t m,
                                uint64_t* d_out, uint32_t* block, uint32_t* grid);
 /* The batch launch captured once into a HIP graph and replayed: the

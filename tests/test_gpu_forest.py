@@ -349,6 +349,9 @@ def test_forest_quad_layouts(gpu):
     f.buildQuadLayouts()
     f.buildQuadLayouts()  # a second call keeps them
     assert np.array_equal(_run(f, q), e)
+    block, grid = f.autotune(dq, len(q), dout)  # tunes the pair-lane kernel's grid
+    assert block == 256 and grid > 0
+    assert np.array_equal(_run(f, q), e)
     f.buildQueryLines()
     f.buildSelectLines()
     assert np.array_equal(_run(f, q), e)
