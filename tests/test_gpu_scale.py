@@ -110,6 +110,22 @@ def test_forest_full_batched_configuration(gpu):
     g.launch()
     E.synchronize()
     assert np.array_equal(d_out.download(np.uint64, m), out)
+    # with the select lines, then over the quad layouts and hex lines (a
+    # graph captured afterwards runs the pair-lane kernel): all 1e8 answers
+    # equal the directory's, which the sample above checked
+    _say("forest: select lines")
+    forest.buildSelectLines()
+    g.launch()
+    E.synchronize()
+    assert np.array_equal(d_out.download(np.uint64, m), out)
+    _say("forest: quad layouts")
+    forest.buildQuadLayouts()
+    forest.autotune(d_q, m, d_out)
+    gq = forest.graph(d_q, m, d_out)
+    gq.launch()
+    E.synchronize()
+    assert np.array_equal(d_out.download(np.uint64, m), out)
+    del gq
 
 
 def _requests_for(text, sigma, sizes, rng):
