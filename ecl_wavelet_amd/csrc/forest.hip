@@ -1048,10 +1048,7 @@ ecl_status ecl_forest_build_quad_layouts(ecl_forest f) {
       hv.ntab = htab.size() - tree_htab[i];
     }
     const size_t nj = jobs.size(), nh = hjobs.size();
-    if (nj == 0) {
-      f->quad_built = true;
-      return;
-    }
+    if (nj == 0) return;  // no tree takes them: batches keep the binary kernel
     require(nj < (1ull << 32) && nh < (1ull << 32), "too many quad levels");
     const uint64_t bytes = (nql_total + sel_total + nhl_total) * 128 +
                            (base.size() + tab.size() + htab.size() + hbtab.size()) * 8 +
